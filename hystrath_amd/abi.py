@@ -1,0 +1,130 @@
+"""ctypes mirror of include/hystrath_dsmc.h (the C ABI of the engine).
+
+The structures here must match the header field for field; the
+`test_abi_layout` test compares their sizes and offsets with what the
+compiled library reports.
+"""
+import ctypes as C
+
+HS_MAX_SPECIES = 8
+HS_MAX_VIB_MODES = 3
+HS_KB = 1.38064852e-23
+
+HS_OK = 0
+HS_E_INVALID = -1
+HS_E_CAPACITY = -2
+HS_E_HIP = -3
+HS_E_OVERFLOW = -4
+HS_E_UNSUPPORTED = -5
+
+HS_COLL_NONE, HS_COLL_VHS, HS_COLL_VSS, HS_COLL_LB_VHS, HS_COLL_LB_VSS = range(5)
+HS_PATCH_CYCLIC, HS_PATCH_SPECULAR, HS_PATCH_DIFFUSE, HS_PATCH_DELETION, HS_PATCH_EMPTY = range(5)
+
+HS_FLAG_SAMPLE = 0x1
+HS_FLAG_HEATFLUX = 0x2
+HS_FLAG_COLL_SEP = 0x4
+HS_FLAG_TIMING = 0x8
+
+HS_XMIN, HS_XMAX, HS_YMIN, HS_YMAX, HS_ZMIN, HS_ZMAX = range(6)
+
+# sampling slots (enum hs_sample_slot)
+(HS_S_N, HS_S_M, HS_S_MUX, HS_S_MUY, HS_S_MUZ, HS_S_MCC, HS_S_EROT, HS_S_ZROT,
+ HS_S_NCOLL, HS_S_COLLSEP, HS_S_BASE_COUNT) = range(11)
+
+# names used by the reference dictionaries -> enums
+COLLISION_MODELS = {
+    "NoBinaryCollision": HS_COLL_NONE,
+    "VariableHardSphere": HS_COLL_VHS,
+    "VariableSoftSphere": HS_COLL_VSS,
+    "LarsenBorgnakkeVariableHardSphere": HS_COLL_LB_VHS,
+    "LarsenBorgnakkeVariableSoftSphere": HS_COLL_LB_VSS,
+}
+PATCH_MODELS = {
+    "cyclic": HS_PATCH_CYCLIC,
+    "dsmcSpecularWallPatch": HS_PATCH_SPECULAR,
+    "dsmcDiffuseWallPatch": HS_PATCH_DIFFUSE,
+    "dsmcDeletionPatch": HS_PATCH_DELETION,
+    "empty": HS_PATCH_EMPTY,
+}
+
+
+class hs_species(C.Structure):
+    _fields_ = [
+        ("mass", C.c_double),
+        ("diameter", C.c_double),
+        ("omega", C.c_double),
+        ("alpha", C.c_double),
+        ("rot_dof", C.c_double),
+        ("n_vib_modes", C.c_int32),
+        ("charge", C.c_int32),
+        ("theta_v", C.c_double * HS_MAX_VIB_MODES),
+        ("zref", C.c_double * HS_MAX_VIB_MODES),
+        ("tref_zv", C.c_double * HS_MAX_VIB_MODES),
+        ("theta_d", C.c_double),
+    ]
+
+
+class hs_patch(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("inflow", C.c_int32),
+        ("temperature", C.c_double),
+        ("velocity", C.c_double * 3),
+    ]
+
+
+class hs_config(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("origin", C.c_double * 3),
+        ("delta", C.c_double * 3),
+        ("cell_volume", C.POINTER(C.c_double)),
+        ("patch", hs_patch * 6),
+        ("cyl_enabled", C.c_int32),
+        ("cyl_center", C.c_double * 2),
+        ("cyl_radius", C.c_double),
+        ("cyl_patch", hs_patch),
+        ("n_species", C.c_int32),
+        ("species", hs_species * HS_MAX_SPECIES),
+        ("collision_model", C.c_int32),
+        ("tref", C.c_double),
+        ("z_rot", C.c_double),
+        ("z_vib_fixed", C.c_double),
+        ("z_el", C.c_double),
+        ("inv_zv_form", C.c_int32),
+        ("inflow_n", C.c_double * HS_MAX_SPECIES),
+        ("inflow_t", C.c_double),
+        ("inflow_trot", C.c_double),
+        ("inflow_tvib", C.c_double),
+        ("inflow_u", C.c_double * 3),
+        ("fn", C.c_double),
+        ("dt", C.c_double),
+        ("seed", C.c_uint64),
+        ("capacity", C.c_int64),
+        ("flags", C.c_int32),
+    ]
+
+
+class hs_particles(C.Structure):
+    _fields_ = [
+        ("px", C.POINTER(C.c_double)), ("py", C.POINTER(C.c_double)), ("pz", C.POINTER(C.c_double)),
+        ("ux", C.POINTER(C.c_float)), ("uy", C.POINTER(C.c_float)), ("uz", C.POINTER(C.c_float)),
+        ("cell", C.POINTER(C.c_int32)),
+        ("erot", C.POINTER(C.c_float)),
+        ("vib", C.POINTER(C.c_uint16)),
+        ("type", C.POINTER(C.c_uint8)),
+    ]
+
+
+def ptr(arr, ctype):
+    """ctypes pointer to a numpy array's data (None for None)."""
+    if arr is None:
+        return None
+    return arr.ctypes.data_as(C.POINTER(ctype))
+
+
+def sample_slots(cfg):
+    n = HS_S_BASE_COUNT + cfg.n_species + cfg.n_species * HS_MAX_VIB_MODES
+    if cfg.flags & HS_FLAG_HEATFLUX:
+        n += 13
+    return n

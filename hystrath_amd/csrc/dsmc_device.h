@@ -1,0 +1,496 @@
+// dsmc_device.h -- device-side building blocks of the MI355X DSMC engine:
+// the parameter block, the counter-based random streams, FP64 math built from
+// IEEE +,-,*,/ and sqrt only (so every compiler rounds it identically), and
+// the collision physics of dsmcFoam+:
+//   VariableHardSphere.C:80-228, VariableSoftSphere.C:78-247,
+//   LarsenBorgnakkeVariableHardSphere.C:110-279, dsmcCloud.C:1015-1040,
+//   dsmcCloud.C:1181-1244,1327-1531.
+// Compile with -ffp-contract=off: each a*b+c below is two roundings.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/hystrath_dsmc.h"
+
+#define HS_VSMALL 1.0e-300
+#define HS_SMALL 1.0e-15
+#define HS_ROOTVSMALL 1.0e-150
+#define HS_PI 3.14159265358979323846
+#define HS_TWO_PI 6.28318530717958647692
+#define HS_MAX_TRACK 1000
+#define HS_MAX_REDRAW 100000
+#define HS_MAX_CAND (1 << 22)
+#define HS_MAX_FILL_TRIES 100000
+
+// random-stream purposes (top byte of counter word 2)
+enum { RNG_MOVE = 1, RNG_COLLIDE = 2, RNG_INJECT = 3, RNG_FILL_COUNT = 4,
+       RNG_FILL_GEN = 5, RNG_REMAINDER = 6 };
+
+// counters[] slots
+enum { CNT_CAND = 0, CNT_COLL = 1, CNT_DEL = 2, CNT_INJ = 3, CNT_WALL = 4,
+       CNT_TRACK_OVF = 5, CNT_TILE_OVF = 6, CNT_CAP_OVF = 7, CNT_N = 8 };
+
+// Everything a kernel needs about the case, resident in device memory and
+// read through uniform (scalar) loads.
+struct DevParams {
+    int nx, ny, nz, nc;
+    double origin[3], delta[3];
+    int empty[3];
+    double centre[3];
+    int patch_model[6];
+    double patch_T[6];
+    double patch_U[6][3];
+    int cyl;
+    double cyl_c[2], cyl_r;
+    int cyl_model;
+    double cyl_T;
+    double cyl_U[3];
+    int ns, max_modes;
+    double mass[HS_MAX_SPECIES], rotdof[HS_MAX_SPECIES];
+    int nvib[HS_MAX_SPECIES], charge[HS_MAX_SPECIES];
+    double theta_v[HS_MAX_SPECIES][HS_MAX_VIB_MODES];
+    double sig_coef[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double ref_term[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double omega_pq[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double alpha_pq[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double theta_d[HS_MAX_SPECIES];
+    double zv_b[HS_MAX_SPECIES][HS_MAX_SPECIES][HS_MAX_VIB_MODES];
+    double zv_pow2[HS_MAX_SPECIES][HS_MAX_VIB_MODES];
+    int coll_model;
+    double z_rot, z_el, z_vib_fixed;
+    int inv_zv_form;
+    double fn, dt;
+    uint32_t seed_lo, seed_hi;
+    int flags, nslots;
+    int has_erot, has_vib, has_type;
+    int64_t capacity;
+    // freestream inflow
+    double inflow_t, inflow_trot, inflow_tvib;
+    double inflow_u[3];
+    double inflow_n[HS_MAX_SPECIES];
+    int n_inflow_faces;
+};
+
+// Device view of one particle buffer (structure of arrays).
+struct DevParticles {
+    double* px; double* py; double* pz;
+    float* ux; float* uy; float* uz;
+    int32_t* cell;
+    float* erot;     // nullptr when no species rotates
+    uint16_t* vib;   // [mode][capacity] or nullptr
+    uint8_t* type;   // nullptr for one species
+};
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox4x32_10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                              uint32_t k0, uint32_t k1, uint32_t out[4]) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0;
+        const uint32_t hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2;
+        const uint32_t hi1 = __umulhi(0xCD9E8D57u, c2);
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+__device__ __forceinline__ double uniform53(uint32_t hi, uint32_t lo) {
+    const uint64_t x = (((uint64_t)hi << 32) | (uint64_t)lo) >> 11;
+    return (double)x * (1.0 / 9007199254740992.0);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 math from IEEE +,-,*,/ only (same operation order as any host build)
+// ---------------------------------------------------------------------------
+#define HS_LN2_HI 6.93147180369123816490e-01
+#define HS_LN2_LO 1.90821492927058770002e-10
+#define HS_INV_LN2 1.44269504088896338700e+00
+
+__device__ __forceinline__ double dm_log(double x) {
+    if (x != x) return x;
+    if (x < 0.0) return __longlong_as_double(0x7FF8000000000000ll);
+    if (x == 0.0) return -__longlong_as_double(0x7FF0000000000000ll);
+    if (x == __longlong_as_double(0x7FF0000000000000ll)) return x;
+    int k = 0;
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    if (((b >> 52) & 0x7ff) == 0) { x = x * 18014398509481984.0; b = (uint64_t)__double_as_longlong(x); k = -54; }
+    k += (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+    if (m > 1.4142135623730951) { m = m * 0.5; k += 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double t = 1.0 / 23.0;
+    t = 1.0 / 21.0 + s2 * t;
+    t = 1.0 / 19.0 + s2 * t;
+    t = 1.0 / 17.0 + s2 * t;
+    t = 1.0 / 15.0 + s2 * t;
+    t = 1.0 / 13.0 + s2 * t;
+    t = 1.0 / 11.0 + s2 * t;
+    t = 1.0 / 9.0 + s2 * t;
+    t = 1.0 / 7.0 + s2 * t;
+    t = 1.0 / 5.0 + s2 * t;
+    t = 1.0 / 3.0 + s2 * t;
+    const double lm = (2.0 * s) + (2.0 * s) * (s2 * t);
+    const double dk = (double)k;
+    return dk * HS_LN2_HI + (dk * HS_LN2_LO + lm);
+}
+
+__device__ __forceinline__ double dm_exp(double x) {
+    if (x != x) return x;
+    if (x > 709.782712893384) return __longlong_as_double(0x7FF0000000000000ll);
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = floor(x * HS_INV_LN2 + 0.5);
+    const double r = (x - kd * HS_LN2_HI) - kd * HS_LN2_LO;
+    double p = 1.0 / 87178291200.0;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    int k = (int)kd;
+    if (k < -1000) { p = p * 0x1p-511; k += 511; }
+    if (k > 1000) { p = p * 0x1p511; k -= 511; }
+    return p * __longlong_as_double((long long)((uint64_t)(k + 1023) << 52));
+}
+
+__device__ __forceinline__ double dm_pow(double x, double y) {
+    if (y == 0.0) return 1.0;
+    return dm_exp(y * dm_log(x));
+}
+
+__device__ __forceinline__ void dm_sincos2pi(double u, double* s_out, double* c_out) {
+    const double q = floor(u * 4.0 + 0.5);
+    const double d = u - q * 0.25;
+    const double r = d * HS_TWO_PI;
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;
+    ps = -1.0 / 1307674368000.0 + r2 * ps;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    const double sr = r + (r * r2) * ps;
+    double pc = -1.0 / 6402373705728000.0;
+    pc = 1.0 / 20922789888000.0 + r2 * pc;
+    pc = -1.0 / 87178291200.0 + r2 * pc;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    pc = -0.5 + r2 * pc;
+    const double cr = 1.0 + r2 * pc;
+    switch (((int)q) & 3) {
+    case 0: *s_out = sr; *c_out = cr; break;
+    case 1: *s_out = cr; *c_out = -sr; break;
+    case 2: *s_out = -sr; *c_out = -cr; break;
+    default: *s_out = -cr; *c_out = sr; break;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Random streams.  PhiloxStream: key = seed, counter = (id, step,
+// purpose<<24 | sub, block); each block yields two doubles.  InjStream
+// serves caller-provided uniforms (test hook).
+// ---------------------------------------------------------------------------
+struct PhiloxStream {
+    uint32_t k0, k1, c0, c1, c2, blk;
+    double spare_u, spare_g;
+    bool have_u, have_g;
+    __device__ __forceinline__ void init(const DevParams& P, int purpose, uint32_t id, uint32_t step, uint32_t sub) {
+        k0 = P.seed_lo; k1 = P.seed_hi;
+        c0 = id; c1 = step; c2 = ((uint32_t)purpose << 24) | (sub & 0xFFFFFFu);
+        blk = 0; have_u = false; have_g = false; spare_u = 0.0; spare_g = 0.0;
+    }
+    __device__ __forceinline__ double u() {
+        if (have_u) { have_u = false; return spare_u; }
+        uint32_t o[4];
+        philox4x32_10(c0, c1, c2, blk, k0, k1, o);
+        blk += 1;
+        spare_u = uniform53(o[2], o[3]);
+        have_u = true;
+        return uniform53(o[0], o[1]);
+    }
+    __device__ __forceinline__ double gauss() {
+        if (have_g) { have_g = false; return spare_g; }
+        const double u1 = u();
+        const double u2 = u();
+        const double rad = sqrt(-2.0 * dm_log(1.0 - u1));
+        double sn, cs;
+        dm_sincos2pi(u2, &sn, &cs);
+        spare_g = rad * sn;
+        have_g = true;
+        return rad * cs;
+    }
+};
+
+struct InjStream {
+    const double* src;
+    int n, used;
+    __device__ __forceinline__ double u() {
+        const int k = used++;
+        return k < n ? src[k] : 0.5;
+    }
+};
+
+// dsmcCloud::randomLabel (dsmcCloud.C:1015-1040)
+template <class R>
+__device__ __forceinline__ int random_label(R& r, int a, int b) {
+    if (a == b) return a;
+    const int start = a < b ? a : b;
+    const int end = a < b ? b : a;
+    for (int guard = 0; guard < HS_MAX_REDRAW; ++guard) {
+        const int v = start + (int)(r.u() * (double)(end - start + 1));
+        if (v != end + 1) return v;
+    }
+    return end;
+}
+
+// ---------------------------------------------------------------------------
+// equipartition samplers (dsmcCloud.C:1181-1244)
+// ---------------------------------------------------------------------------
+template <class R>
+__device__ __forceinline__ double equip_erot(R& r, double T, double dof) {
+    if (dof < HS_SMALL) return 0.0;
+    if (dof < 2.0 + HS_SMALL && dof > 2.0 - HS_SMALL) return -dm_log(1.0 - r.u()) * HS_KB * T;
+    const double a = 0.5 * dof - 1.0;
+    double ratio, P;
+    int guard = 0;
+    do {
+        ratio = 10.0 * r.u();
+        P = dm_pow(ratio / a, a) * dm_exp(a - ratio);
+    } while (P < r.u() && ++guard < HS_MAX_REDRAW);
+    return ratio * HS_KB * T;
+}
+
+template <class R>
+__device__ __forceinline__ int equip_vib_level(R& r, double T, double theta_v) {
+    const int l = (int)(-dm_log(1.0 - r.u()) * T / theta_v);
+    return l > 65535 ? 65535 : l;
+}
+
+// ---------------------------------------------------------------------------
+// sigmaTcR (VariableHardSphere.C:80-121); pair constants precomputed on host
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double sigma_tcr(const DevParams& P, int tp, int tq, const double up[3],
+                                            const double uq[3]) {
+    const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+    const double cr2 = dx * dx + dy * dy + dz * dz;
+    const double cr = sqrt(cr2);
+    if (cr < HS_VSMALL) return 0.0;
+    const double om = P.omega_pq[tp][tq];
+    return P.sig_coef[tp][tq] * dm_pow(P.ref_term[tp][tq] / cr2, om - 0.5) * cr;
+}
+
+// postCollisionVelocities: VHS (VariableHardSphere.C:191-228) or VSS
+// (VariableSoftSphere.C:189-247); cr < 0 means |UP-UQ|; rescale scales the
+// pre-collision relative velocity to |cr| (LB-VSS energy conservation).
+template <class R>
+__device__ __forceinline__ void post_coll_vel(const DevParams& P, bool vss, bool rescale, int tp, int tq,
+                                              double up[3], double uq[3], double cr, R& r) {
+    const double mp = P.mass[tp], mq = P.mass[tq];
+    if (cr < 0.0) {
+        const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+        cr = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double msum = mp + mq;
+    double ucm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ucm[c] = (mp * up[c] + mq * uq[c]) / msum;
+    double rel[3];
+    if (!vss) {
+        const double cos_t = 2.0 * r.u() - 1.0;
+        const double sin_t = sqrt(1.0 - cos_t * cos_t);
+        double sp, cp;
+        dm_sincos2pi(r.u(), &sp, &cp);
+        rel[0] = cr * cos_t;
+        rel[1] = cr * (sin_t * cp);
+        rel[2] = cr * (sin_t * sp);
+    } else {
+        double c0 = up[0] - uq[0], c1 = up[1] - uq[1], c2 = up[2] - uq[2];
+        if (rescale) {
+            const double cm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            if (cm > HS_VSMALL) { const double f = cr / cm; c0 *= f; c1 *= f; c2 *= f; }
+        }
+        const double cos_t = 2.0 * dm_pow(r.u(), 1.0 / P.alpha_pq[tp][tq]) - 1.0;
+        const double sin_t = sqrt(1.0 - cos_t * cos_t);
+        double sp, cp;
+        dm_sincos2pi(r.u(), &sp, &cp);
+        const double D = sqrt(c1 * c1 + c2 * c2);
+        if (D > HS_VSMALL) {
+            rel[0] = cos_t * c0 + sin_t * sp * D;
+            rel[1] = cos_t * c1 + sin_t * (cr * c2 * cp - c0 * c1 * sp) / D;
+            rel[2] = cos_t * c2 - sin_t * (cr * c1 * cp + c0 * c2 * sp) / D;
+        } else {
+            rel[0] = cos_t * c0;
+            rel[1] = cr * (sin_t * cp);
+            rel[2] = cr * (sin_t * sp);
+        }
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        up[c] = ucm[c] + rel[c] * mq / msum;
+        uq[c] = ucm[c] - rel[c] * mp / msum;
+    }
+}
+
+// dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531)
+template <class R>
+__device__ int vib_level(const DevParams& P, int t, int partner, int mode, int level, int imax,
+                         double ec, double om, R& r) {
+    const double theta_v = P.theta_v[t][mode];
+    double inv_zv = 1.0;
+    if (P.z_vib_fixed == 0.0) {
+        const double T = (double)imax * theta_v / (3.5 - om);
+        const double pow1 = dm_pow(P.theta_d[t] / T, 1.0 / 3.0) - 1.0;
+        const double zvp1 = dm_pow(P.theta_d[t] / T, om);
+        const double zvp2 = dm_pow(P.zv_b[t][partner][mode], pow1 / P.zv_pow2[t][mode]);
+        const double zv = zvp1 * zvp2;
+        inv_zv = P.inv_zv_form == 2 ? 1.0 / (5.0 * zv) : 1.0 / zv;
+    } else {
+        inv_zv = 1.0 / P.z_vib_fixed;
+    }
+    int idash = level;
+    if (inv_zv > r.u()) {
+        double func;
+        int guard = 0;
+        do {
+            idash = random_label(r, 0, imax);
+            const double evib = (double)idash * HS_KB * theta_v;
+            func = dm_pow(1.0 - evib / ec, 1.5 - om);
+        } while (func < r.u() && ++guard < HS_MAX_REDRAW);
+    }
+    return idash;
+}
+
+// dsmcCloud::postCollisionRotationalEnergy (dsmcCloud.C:1327-1387)
+template <class R>
+__device__ double rot_ratio(double dof, double chib, R& r) {
+    if (dof == 2.0) return 1.0 - dm_pow(r.u(), 1.0 / chib);
+    const double a1 = 0.5 * dof - 1.0, b1 = chib - 1.0;
+    if (a1 < HS_SMALL && b1 < HS_SMALL) return r.u();
+    double ratio, Pp;
+    int guard = 0;
+    do {
+        ratio = r.u();
+        if (a1 < HS_SMALL) Pp = dm_pow(1.0 - ratio, b1);
+        else if (b1 < HS_SMALL) Pp = dm_pow(1.0 - ratio, a1);
+        else Pp = dm_pow((a1 + b1) * ratio / a1, a1) * dm_pow((a1 + b1) * (1.0 - ratio) / b1, b1);
+    } while (Pp < r.u() && ++guard < HS_MAX_REDRAW);
+    return ratio;
+}
+
+// LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
+// species with one electronic level
+template <class R>
+__device__ void lb_redistribute(const DevParams& P, int t, int partner, double* et, double* erot,
+                                int* vib, double om, R& r) {
+    if (P.charge[t] == -1) return;
+    if (1.0 / P.z_el > r.u()) (void)r.u();
+    const int nm = P.nvib[t];
+    if (nm > 0) {
+        double pre[HS_MAX_VIB_MODES];
+        for (int m = 0; m < nm; ++m) pre[m] = (double)vib[m] * HS_KB * P.theta_v[t][m];
+        for (int m = 0; m < nm; ++m) {
+            const double ec = *et + pre[m];
+            const int imax = (int)(ec / (HS_KB * P.theta_v[t][m]));
+            if (imax > 0) {
+                vib[m] = vib_level(P, t, partner, m, vib[m], imax, ec, om, r);
+                *et = ec - (double)vib[m] * HS_KB * P.theta_v[t][m];
+            }
+        }
+    }
+    if (P.rotdof[t] > 0) {
+        const double pre = *erot;
+        if (1.0 / P.z_rot > r.u()) {
+            const double ec = *et + pre;
+            const double ratio = rot_ratio(P.rotdof[t], 2.5 - om, r);
+            *erot = ratio * ec;
+            *et = ec - *erot;
+        }
+    }
+}
+
+// BinaryCollisionModel::collide for the configured model
+template <class R>
+__device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3], double uq[3],
+                               double* erp, double* erq, int* vp, int* vq, R& r) {
+    const int model = P.coll_model;
+    if (model == HS_COLL_VHS || model == HS_COLL_VSS) {
+        post_coll_vel(P, model == HS_COLL_VSS, false, tp, tq, up, uq, -1.0, r);
+        return;
+    }
+    const double mp = P.mass[tp], mq = P.mass[tq];
+    const double mr = mp * mq / (mp + mq);
+    const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+    const double cr2 = dx * dx + dy * dy + dz * dz;
+    double et = 0.5 * mr * cr2;
+    const double om = P.omega_pq[tp][tq];
+    lb_redistribute(P, tp, tq, &et, erp, vp, om, r);
+    lb_redistribute(P, tq, tp, &et, erq, vq, om, r);
+    const double cr = sqrt(2.0 * et / mr);
+    post_coll_vel(P, model == HS_COLL_LB_VSS, true, tp, tq, up, uq, cr, r);
+}
+
+// ---------------------------------------------------------------------------
+// walls
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double face_coord(const DevParams& P, int a, int idx) {
+    return P.origin[a] + (double)idx * P.delta[a];
+}
+
+// dsmcDiffuseWallPatch::performDiffuseReflection (dsmcDiffuseWallPatch.C:67-139)
+// + dsmcPatchBoundary::calculateWallUnitVectors (dsmcPatchBoundary.C:216-261);
+// nw points out of the gas.  Internal energies re-sampled at the wall T.
+__device__ void diffuse_reflect(const DevParams& P, double T, const double uw[3], const double nw[3], int t,
+                                double U[3], float* erot, uint16_t* vib, int64_t vstride,
+                                PhiloxStream& r) {
+    double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+    double ut[3] = { U[0] - und * nw[0], U[1] - und * nw[1], U[2] - und * nw[2] };
+    int guard = 0;
+    while (sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]) < HS_SMALL && guard++ < 100) {
+        U[0] = U[0] * (0.8 + 0.2 * r.u());
+        U[1] = U[1] * (0.8 + 0.2 * r.u());
+        U[2] = U[2] * (0.8 + 0.2 * r.u());
+        und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+        ut[0] = U[0] - und * nw[0]; ut[1] = U[1] - und * nw[1]; ut[2] = U[2] - und * nw[2];
+    }
+    const double mut = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+    double tw1[3] = { ut[0] / mut, ut[1] / mut, ut[2] / mut };
+    if (!(mut > 0.0)) { tw1[0] = nw[1]; tw1[1] = nw[2]; tw1[2] = nw[0]; }
+    const double tw2[3] = { nw[1] * tw1[2] - nw[2] * tw1[1],
+                            nw[2] * tw1[0] - nw[0] * tw1[2],
+                            nw[0] * tw1[1] - nw[1] * tw1[0] };
+    const double g1 = r.gauss();
+    const double g2 = r.gauss();
+    const double u = r.u();
+    const double sc = sqrt(HS_KB * T / P.mass[t]);
+    double arg = 1.0 - u;
+    if (arg < HS_VSMALL) arg = HS_VSMALL;
+    const double rn = sqrt(-2.0 * dm_log(arg));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) U[c] = sc * ((g1 * tw1[c] + g2 * tw2[c]) - rn * nw[c]) + uw[c];
+    if (erot) *erot = (float)equip_erot(r, T, P.rotdof[t]);
+    if (vib)
+        for (int m = 0; m < P.nvib[t]; ++m) vib[m * vstride] = (uint16_t)equip_vib_level(r, T, P.theta_v[t][m]);
+}

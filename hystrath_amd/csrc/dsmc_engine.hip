@@ -1,0 +1,895 @@
+// dsmc_engine.hip -- the C ABI of include/hystrath_dsmc.h: engine lifetime,
+// device memory, state transfer and the per-step launch sequence.
+//
+// One engine = one HIP stream on the device that was current at create time.
+// Particle state is two structure-of-arrays buffers; every step moves the
+// current buffer in place and the collide kernel writes the cell-sorted,
+// collided population into the other one.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "dsmc_kernels.h"
+
+namespace {
+
+thread_local std::string g_create_error;
+
+enum { T_INJECT = 0, T_MOVE, T_SCAN, T_SCATTER, T_COLLIDE, T_NSTAGE };
+constexpr int kMaxTimedPairs = 65536;
+
+struct TimedPair {
+    int stage;
+    hipEvent_t a, b;
+};
+
+}  // namespace
+
+struct hs_dsmc_engine {
+    hs_config cfg;
+    std::vector<double> vol_h;
+    int device = 0;
+    int nc = 0, ns = 0, general = 0, nslots = 0;
+    int has_erot = 0, has_vib = 0, has_type = 0, max_modes = 0;
+    int64_t cap = 0;
+    hipStream_t stream = nullptr;
+    DevParams hp;
+    DevParams* dp = nullptr;
+    DevParticles buf[2];
+    int cur = 0;
+    std::vector<void*> allocs;
+    double* d_vol = nullptr;
+    double* d_sigma = nullptr;
+    double* d_rem = nullptr;
+    double* d_samples = nullptr;
+    int32_t* d_count = nullptr;
+    int32_t* d_start = nullptr;
+    int32_t* d_slot = nullptr;
+    int32_t* d_perm = nullptr;
+    uint16_t* d_big_list = nullptr;
+    uint8_t* d_big_sub = nullptr;
+    DevFaces faces;
+    int32_t* d_face_off = nullptr;
+    DevScalars* d_S = nullptr;
+    ScanWs ws;
+    int64_t n_sampled = 0;
+    int move_grid = 1024;
+    std::string err;
+    // timing
+    std::vector<TimedPair> timed;
+    std::vector<hipEvent_t> event_pool;
+    int64_t launches[T_NSTAGE] = { 0, 0, 0, 0, 0 };
+    double closed_ms[T_NSTAGE] = { 0, 0, 0, 0, 0 };
+};
+
+// ---------------------------------------------------------------------------
+// helpers
+// ---------------------------------------------------------------------------
+#define HS_TRY(expr)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (expr);                                                           \
+        if (e_ != hipSuccess) return fail(eng, HS_E_HIP, std::string(#expr) + ": " +       \
+                                                             hipGetErrorString(e_));     \
+    } while (0)
+
+static int fail(hs_dsmc_engine* eng, int code, const std::string& msg) {
+    if (eng) eng->err = msg;
+    else g_create_error = msg;
+    return code;
+}
+
+template <class T>
+static int dev_alloc(hs_dsmc_engine* eng, T** p, int64_t count) {
+    if (count <= 0) { *p = nullptr; return 0; }
+    void* q = nullptr;
+    hipError_t e = hipMalloc(&q, (size_t)count * sizeof(T));
+    if (e != hipSuccess) return fail(eng, HS_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    eng->allocs.push_back(q);
+    *p = (T*)q;
+    return 0;
+}
+
+static int check_launch(hs_dsmc_engine* eng, const char* what) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(eng, HS_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+    return 0;
+}
+
+static hipEvent_t take_event(hs_dsmc_engine* eng) {
+    if (!eng->event_pool.empty()) {
+        hipEvent_t e = eng->event_pool.back();
+        eng->event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+struct StageTimer {
+    hs_dsmc_engine* eng;
+    int stage;
+    hipEvent_t a = nullptr;
+    StageTimer(hs_dsmc_engine* e, int s) : eng(e), stage(s) {
+        eng->launches[s] += 1;
+        if ((eng->cfg.flags & HS_FLAG_TIMING) && (int)eng->timed.size() < kMaxTimedPairs) {
+            a = take_event(eng);
+            if (a) (void)hipEventRecord(a, eng->stream);
+        }
+    }
+    ~StageTimer() {
+        if (!a) return;
+        hipEvent_t b = take_event(eng);
+        if (!b) return;
+        (void)hipEventRecord(b, eng->stream);
+        eng->timed.push_back({ stage, a, b });
+    }
+};
+
+static DevParticles& cur_buf(hs_dsmc_engine* eng) { return eng->buf[eng->cur]; }
+static DevParticles& nxt_buf(hs_dsmc_engine* eng) { return eng->buf[1 - eng->cur]; }
+
+// inflow faces in the order: sides (xmin,xmax,ymin,ymax,zmin,zmax) with an
+// inflow model, then the two tangential indices (second fastest)
+struct HostFace {
+    int32_t cell;
+    int8_t axis, dir;
+    int32_t a1i, a2i;
+};
+
+static std::vector<HostFace> list_faces(const hs_config& c, const std::vector<double>& vol) {
+    std::vector<HostFace> out;
+    const int nn[3] = { c.nx, c.ny, c.nz };
+    for (int side = 0; side < 6; ++side) {
+        if (!c.patch[side].inflow) continue;
+        const int axis = side / 2, dir = (side & 1) ? 1 : -1;
+        const int a1 = axis == 0 ? 1 : 0;
+        const int a2 = axis == 2 ? 1 : 2;
+        for (int j2 = 0; j2 < nn[a2]; ++j2)
+            for (int j1 = 0; j1 < nn[a1]; ++j1) {
+                int idx[3];
+                idx[axis] = dir > 0 ? nn[axis] - 1 : 0;
+                idx[a1] = j1;
+                idx[a2] = j2;
+                const int32_t cell = idx[0] + c.nx * (idx[1] + c.ny * idx[2]);
+                if (!(vol[cell] > 0.0)) continue;
+                out.push_back({ cell, (int8_t)axis, (int8_t)dir, j1, j2 });
+            }
+    }
+    return out;
+}
+
+static int validate(const hs_config* c) {
+    if (!c) return fail(nullptr, HS_E_INVALID, "config is NULL");
+    if (c->nx <= 0 || c->ny <= 0 || c->nz <= 0) return fail(nullptr, HS_E_INVALID, "mesh dimensions must be positive");
+    const int64_t nc = (int64_t)c->nx * c->ny * c->nz;
+    if (nc > (int64_t)1 << 30) return fail(nullptr, HS_E_INVALID, "too many cells");
+    for (int a = 0; a < 3; ++a)
+        if (!(c->delta[a] > 0.0)) return fail(nullptr, HS_E_INVALID, "cell size must be positive");
+    if (!c->cell_volume) return fail(nullptr, HS_E_INVALID, "cell_volume is NULL");
+    if (c->n_species < 1 || c->n_species > HS_MAX_SPECIES) return fail(nullptr, HS_E_INVALID, "n_species out of range");
+    for (int s = 0; s < c->n_species; ++s) {
+        const hs_species& sp = c->species[s];
+        if (!(sp.mass > 0.0) || !(sp.diameter > 0.0)) return fail(nullptr, HS_E_INVALID, "species mass/diameter must be positive");
+        if (sp.n_vib_modes < 0 || sp.n_vib_modes > HS_MAX_VIB_MODES)
+            return fail(nullptr, HS_E_INVALID, "nVibrationalModes out of range");
+        if (sp.charge < -1 || sp.charge > 1)
+            return fail(nullptr, HS_E_INVALID, "Charge value should be 0 for neutrals, 1 for ions, or -1 for electrons");
+        for (int m = 0; m < sp.n_vib_modes; ++m)
+            if (!(sp.theta_v[m] > 0.0)) return fail(nullptr, HS_E_INVALID, "characteristicVibrationalTemperature must be positive");
+    }
+    if (c->collision_model < HS_COLL_NONE || c->collision_model > HS_COLL_LB_VSS)
+        return fail(nullptr, HS_E_INVALID, "unknown BinaryCollisionModel");
+    if (c->collision_model >= HS_COLL_LB_VHS) {
+        if (!(c->z_rot > 0.0) || !(c->z_el > 0.0))
+            return fail(nullptr, HS_E_INVALID, "relaxation collision numbers must be positive");
+        if (c->inv_zv_form == 1)
+            return fail(nullptr, HS_E_UNSUPPORTED, "inverseZvFormulation 2008 (macroscopic temperature) is not supported");
+        if (c->inv_zv_form != 0 && c->inv_zv_form != 2) return fail(nullptr, HS_E_INVALID, "inverseZvFormulation out of range");
+    }
+    const int nn[3] = { c->nx, c->ny, c->nz };
+    for (int a = 0; a < 3; ++a) {
+        const int m0 = c->patch[2 * a].model, m1 = c->patch[2 * a + 1].model;
+        if (m0 < HS_PATCH_CYCLIC || m0 > HS_PATCH_EMPTY || m1 < HS_PATCH_CYCLIC || m1 > HS_PATCH_EMPTY)
+            return fail(nullptr, HS_E_INVALID, "unknown patch model");
+        if ((m0 == HS_PATCH_CYCLIC) != (m1 == HS_PATCH_CYCLIC))
+            return fail(nullptr, HS_E_INVALID, "cyclic patches must come in pairs");
+        if ((m0 == HS_PATCH_EMPTY) != (m1 == HS_PATCH_EMPTY))
+            return fail(nullptr, HS_E_INVALID, "empty patches must come in pairs");
+        if (m0 == HS_PATCH_EMPTY && nn[a] != 1) return fail(nullptr, HS_E_INVALID, "an empty direction must be one cell thick");
+        if ((c->patch[2 * a].inflow || c->patch[2 * a + 1].inflow) && m0 == HS_PATCH_EMPTY)
+            return fail(nullptr, HS_E_INVALID, "inflow on an empty patch");
+    }
+    if (c->cyl_enabled) {
+        if (!(c->cyl_radius > 0.0)) return fail(nullptr, HS_E_INVALID, "cylinder radius must be positive");
+        if (c->cyl_patch.model != HS_PATCH_DIFFUSE && c->cyl_patch.model != HS_PATCH_SPECULAR)
+            return fail(nullptr, HS_E_INVALID, "cylinder wall must be diffuse or specular");
+    }
+    if (c->capacity <= 0 || c->capacity >= ((int64_t)1 << 31) - 1) return fail(nullptr, HS_E_INVALID, "capacity out of range");
+    if (!(c->fn > 0.0) || !(c->dt > 0.0)) return fail(nullptr, HS_E_INVALID, "nEquivalentParticles and deltaT must be positive");
+    return 0;
+}
+
+static void derive(hs_dsmc_engine* eng) {
+    const hs_config& c = eng->cfg;
+    DevParams& P = eng->hp;
+    memset(&P, 0, sizeof(P));
+    P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
+    P.nc = eng->nc;
+    const int nn[3] = { c.nx, c.ny, c.nz };
+    for (int a = 0; a < 3; ++a) {
+        P.origin[a] = c.origin[a];
+        P.delta[a] = c.delta[a];
+        P.empty[a] = c.patch[2 * a].model == HS_PATCH_EMPTY;
+        P.centre[a] = c.origin[a] + 0.5 * ((double)nn[a] * c.delta[a]);
+    }
+    for (int s = 0; s < 6; ++s) {
+        P.patch_model[s] = c.patch[s].model;
+        P.patch_T[s] = c.patch[s].temperature;
+        for (int a = 0; a < 3; ++a) P.patch_U[s][a] = c.patch[s].velocity[a];
+    }
+    P.cyl = c.cyl_enabled;
+    P.cyl_c[0] = c.cyl_center[0];
+    P.cyl_c[1] = c.cyl_center[1];
+    P.cyl_r = c.cyl_radius;
+    P.cyl_model = c.cyl_patch.model;
+    P.cyl_T = c.cyl_patch.temperature;
+    for (int a = 0; a < 3; ++a) P.cyl_U[a] = c.cyl_patch.velocity[a];
+    P.ns = c.n_species;
+    for (int p = 0; p < P.ns; ++p) {
+        const hs_species& sp = c.species[p];
+        P.mass[p] = sp.mass;
+        P.rotdof[p] = sp.rot_dof;
+        P.nvib[p] = sp.n_vib_modes;
+        P.charge[p] = sp.charge;
+        P.theta_d[p] = sp.theta_d;
+        if (sp.n_vib_modes > P.max_modes) P.max_modes = sp.n_vib_modes;
+        for (int m = 0; m < sp.n_vib_modes; ++m) {
+            P.theta_v[p][m] = sp.theta_v[m];
+            P.zv_pow2[p][m] = pow(sp.theta_d / sp.tref_zv[m], 1.0 / 3.0) - 1.0;
+        }
+        for (int q = 0; q < P.ns; ++q) {
+            const hs_species& sq = c.species[q];
+            const double dpq = 0.5 * (sp.diameter + sq.diameter);
+            const double om = 0.5 * (sp.omega + sq.omega);
+            const double mr = sp.mass * sq.mass / (sp.mass + sq.mass);
+            P.sig_coef[p][q] = 3.14159265358979323846 * dpq * dpq / exp(lgamma(2.5 - om));
+            P.ref_term[p][q] = 2.0 * HS_KB * c.tref / mr;
+            P.omega_pq[p][q] = om;
+            P.alpha_pq[p][q] = 0.5 * (sp.alpha + sq.alpha);
+            for (int m = 0; m < sp.n_vib_modes; ++m) P.zv_b[p][q][m] = sp.zref[m] * pow(sp.theta_d / sp.tref_zv[m], -om);
+        }
+    }
+    P.coll_model = c.collision_model;
+    P.z_rot = c.z_rot;
+    P.z_el = c.z_el;
+    P.z_vib_fixed = c.z_vib_fixed;
+    P.inv_zv_form = c.inv_zv_form;
+    P.fn = c.fn;
+    P.dt = c.dt;
+    P.seed_lo = (uint32_t)c.seed;
+    P.seed_hi = (uint32_t)(c.seed >> 32);
+    P.flags = c.flags;
+    P.nslots = eng->nslots;
+    P.has_erot = eng->has_erot;
+    P.has_vib = eng->has_vib;
+    P.has_type = eng->has_type;
+    P.capacity = eng->cap;
+    P.inflow_t = c.inflow_t;
+    P.inflow_trot = c.inflow_trot;
+    P.inflow_tvib = c.inflow_tvib;
+    for (int a = 0; a < 3; ++a) P.inflow_u[a] = c.inflow_u[a];
+    for (int s = 0; s < HS_MAX_SPECIES; ++s) P.inflow_n[s] = c.inflow_n[s];
+}
+
+static int alloc_particles(hs_dsmc_engine* eng, DevParticles& b) {
+    const int64_t cap = eng->cap;
+    int rc;
+    if ((rc = dev_alloc(eng, &b.px, cap)) || (rc = dev_alloc(eng, &b.py, cap)) || (rc = dev_alloc(eng, &b.pz, cap)) ||
+        (rc = dev_alloc(eng, &b.ux, cap)) || (rc = dev_alloc(eng, &b.uy, cap)) || (rc = dev_alloc(eng, &b.uz, cap)) ||
+        (rc = dev_alloc(eng, &b.cell, cap)))
+        return rc;
+    b.erot = nullptr;
+    b.vib = nullptr;
+    b.type = nullptr;
+    if (eng->has_erot && (rc = dev_alloc(eng, &b.erot, cap))) return rc;
+    if (eng->has_vib && (rc = dev_alloc(eng, &b.vib, cap * eng->max_modes))) return rc;
+    if (eng->has_type && (rc = dev_alloc(eng, &b.type, cap))) return rc;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// lifetime
+// ---------------------------------------------------------------------------
+extern "C" const char* hs_dsmc_create_error(void) { return g_create_error.c_str(); }
+
+extern "C" const char* hs_dsmc_last_error(const hs_dsmc_engine* eng) {
+    return eng ? eng->err.c_str() : g_create_error.c_str();
+}
+
+extern "C" int hs_dsmc_device_count(int* n) {
+    int k = 0;
+    if (hipGetDeviceCount(&k) != hipSuccess) k = 0;
+    if (n) *n = k;
+    return 0;
+}
+
+extern "C" int32_t hs_dsmc_sample_slots(const hs_config* cfg) {
+    int32_t n = HS_S_BASE_COUNT + cfg->n_species + cfg->n_species * HS_MAX_VIB_MODES;
+    if (cfg->flags & HS_FLAG_HEATFLUX) n += 13;
+    return n;
+}
+
+extern "C" void hs_dsmc_destroy(hs_dsmc_engine* eng) {
+    if (!eng) return;
+    (void)hipSetDevice(eng->device);
+    if (eng->stream) (void)hipStreamSynchronize(eng->stream);
+    for (auto& t : eng->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+    for (auto e : eng->event_pool) (void)hipEventDestroy(e);
+    for (void* p : eng->allocs) (void)hipFree(p);
+    if (eng->stream) (void)hipStreamDestroy(eng->stream);
+    delete eng;
+}
+
+extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
+    g_create_error.clear();
+    if (!out) return fail(nullptr, HS_E_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate(cfg);
+    if (rc) return rc;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return fail(nullptr, HS_E_HIP, "no HIP device available (the engine has no CPU path)");
+    hs_dsmc_engine* eng = new hs_dsmc_engine();
+    eng->cfg = *cfg;
+    (void)hipGetDevice(&eng->device);
+    eng->nc = cfg->nx * cfg->ny * cfg->nz;
+    eng->ns = cfg->n_species;
+    eng->cap = cfg->capacity;
+    eng->vol_h.assign(cfg->cell_volume, cfg->cell_volume + eng->nc);
+    eng->cfg.cell_volume = eng->vol_h.data();
+    for (int s = 0; s < eng->ns; ++s) {
+        if (cfg->species[s].rot_dof > 0) eng->has_erot = 1;
+        if (cfg->species[s].n_vib_modes > 0) eng->has_vib = 1;
+        if (cfg->species[s].n_vib_modes > eng->max_modes) eng->max_modes = cfg->species[s].n_vib_modes;
+    }
+    eng->has_type = eng->ns > 1;
+    eng->general = eng->has_erot || eng->has_vib || eng->has_type;
+    eng->nslots = hs_dsmc_sample_slots(cfg);
+    derive(eng);
+    auto bail = [&](int code) {
+        g_create_error = eng->err;
+        hs_dsmc_destroy(eng);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
+        return bail(fail(eng, HS_E_HIP, "hipStreamCreate failed"));
+    const int nc = eng->nc;
+    if ((rc = dev_alloc(eng, &eng->dp, 1)) || (rc = alloc_particles(eng, eng->buf[0])) ||
+        (rc = alloc_particles(eng, eng->buf[1])) || (rc = dev_alloc(eng, &eng->d_vol, nc)) ||
+        (rc = dev_alloc(eng, &eng->d_sigma, nc)) || (rc = dev_alloc(eng, &eng->d_rem, nc)) ||
+        (rc = dev_alloc(eng, &eng->d_samples, (int64_t)eng->nslots * nc)) ||
+        (rc = dev_alloc(eng, &eng->d_count, nc)) || (rc = dev_alloc(eng, &eng->d_start, (int64_t)nc + 1)) ||
+        (rc = dev_alloc(eng, &eng->d_slot, eng->cap)) || (rc = dev_alloc(eng, &eng->d_perm, eng->cap)) ||
+        (rc = dev_alloc(eng, &eng->d_big_list, eng->cap)) || (rc = dev_alloc(eng, &eng->d_big_sub, eng->cap)) ||
+        (rc = dev_alloc(eng, &eng->d_S, 1)))
+        return bail(rc);
+    // inflow faces
+    std::vector<HostFace> hf = list_faces(eng->cfg, eng->vol_h);
+    eng->faces.n = (int32_t)hf.size();
+    const int nf = eng->faces.n, ns = eng->ns;
+    // scan workspace sized for the largest scanned array (cells, faces*ns, cells*ns)
+    int64_t scan_n = (int64_t)nc * ns;
+    if ((int64_t)nf * ns > scan_n) scan_n = (int64_t)nf * ns;
+    eng->ws.nb_max = (int32_t)((scan_n + 2047) / 2048 + 1);
+    if ((rc = dev_alloc(eng, &eng->ws.bsum, eng->ws.nb_max))) return bail(rc);
+    if (nf > 0) {
+        std::vector<int32_t> fc(nf), a1(nf), a2(nf);
+        std::vector<int8_t> ax(nf), dr(nf);
+        std::vector<double> inc((size_t)ns * nf, 0.0);
+        const double sqrt_pi = sqrt(3.14159265358979323846);
+        for (int f = 0; f < nf; ++f) {
+            fc[f] = hf[f].cell; ax[f] = hf[f].axis; dr[f] = hf[f].dir; a1[f] = hf[f].a1i; a2[f] = hf[f].a2i;
+        }
+        for (int s = 0; s < ns; ++s) {
+            if (!(cfg->inflow_n[s] > 0.0)) continue;
+            const double mass = cfg->species[s].mass;
+            const double mps = sqrt(2.0 * HS_KB * cfg->inflow_t / mass);
+            for (int f = 0; f < nf; ++f) {
+                const int a = hf[f].axis;
+                const int a1i = a == 0 ? 1 : 0, a2i = a == 2 ? 1 : 2;
+                const double fa = cfg->delta[a1i] * cfg->delta[a2i];
+                const double s_cos = (cfg->inflow_u[a] * (double)(-hf[f].dir)) / mps;
+                inc[(size_t)s * nf + f] = (fa * cfg->inflow_n[s] * cfg->dt * mps
+                                           * (exp(-(s_cos * s_cos)) + sqrt_pi * s_cos * (1.0 + erf(s_cos))))
+                                          / (2.0 * sqrt_pi * cfg->fn);
+            }
+        }
+        DevFaces& F = eng->faces;
+        if ((rc = dev_alloc(eng, &F.cell, nf)) || (rc = dev_alloc(eng, &F.axis, nf)) ||
+            (rc = dev_alloc(eng, &F.dir, nf)) || (rc = dev_alloc(eng, &F.a1i, nf)) || (rc = dev_alloc(eng, &F.a2i, nf)) ||
+            (rc = dev_alloc(eng, &F.inc, (int64_t)ns * nf)) || (rc = dev_alloc(eng, &F.acc, (int64_t)ns * nf)) ||
+            (rc = dev_alloc(eng, &F.n_new, (int64_t)ns * nf)) || (rc = dev_alloc(eng, &F.off, (int64_t)ns * nf + 1)))
+            return bail(rc);
+        if (hipMemcpy(F.cell, fc.data(), nf * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(F.axis, ax.data(), nf, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(F.dir, dr.data(), nf, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(F.a1i, a1.data(), nf * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(F.a2i, a2.data(), nf * 4, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(F.inc, inc.data(), inc.size() * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(eng, HS_E_HIP, "hipMemcpy of inflow faces failed"));
+        launch_fill_f64(F.acc, 0.0, (int64_t)ns * nf, eng->stream);
+    }
+    if (hipMemcpy(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(eng->d_vol, eng->vol_h.data(), (size_t)nc * 8, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(eng, HS_E_HIP, "hipMemcpy of parameters failed"));
+    DevScalars s0;
+    memset(&s0, 0, sizeof(s0));
+    if (hipMemcpy(eng->d_S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(fail(eng, HS_E_HIP, "hipMemcpy of scalars failed"));
+    launch_fill_i32(eng->d_count, 0, nc, eng->stream);
+    launch_fill_f64(eng->d_sigma, 0.0, nc, eng->stream);
+    launch_fill_f64(eng->d_rem, 0.0, nc, eng->stream);
+    launch_fill_f64(eng->d_samples, 0.0, (int64_t)eng->nslots * nc, eng->stream);
+    int ncu = 256;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, eng->device) == hipSuccess) ncu = prop.multiProcessorCount;
+    eng->move_grid = ncu * 8;
+    if (check_launch(eng, "init fills") || hipStreamSynchronize(eng->stream) != hipSuccess)
+        return bail(fail(eng, HS_E_HIP, "initialisation failed"));
+    *out = eng;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// state transfer
+// ---------------------------------------------------------------------------
+static int read_scalars(hs_dsmc_engine* eng, DevScalars* s) {
+    HS_TRY(hipMemcpyAsync(s, eng->d_S, sizeof(DevScalars), hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_upload_particles(hs_dsmc_engine* eng, const hs_particles* p, int64_t n) {
+    if (!eng || !p) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (n < 0 || n > eng->cap) return fail(eng, HS_E_CAPACITY, "particle count exceeds capacity");
+    if (!p->px || !p->py || !p->pz || !p->ux || !p->uy || !p->uz || !p->cell)
+        return fail(eng, HS_E_INVALID, "position, velocity and cell arrays are required");
+    if ((eng->has_erot && !p->erot) || (eng->has_vib && !p->vib) || (eng->has_type && !p->type))
+        return fail(eng, HS_E_INVALID, "erot/vib/type arrays required by the species table are missing");
+    for (int64_t i = 0; i < n; ++i) {
+        const int32_t c = p->cell[i];
+        if (c < -1 || c >= eng->nc) return fail(eng, HS_E_INVALID, "cell index out of range");
+        if (eng->has_type && p->type[i] >= eng->ns) return fail(eng, HS_E_INVALID, "typeId out of range");
+    }
+    HS_TRY(hipSetDevice(eng->device));
+    DevParticles& b = cur_buf(eng);
+    hipStream_t st = eng->stream;
+    if (n > 0) {
+        HS_TRY(hipMemcpyAsync(b.px, p->px, n * 8, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.py, p->py, n * 8, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.pz, p->pz, n * 8, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.ux, p->ux, n * 4, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.uy, p->uy, n * 4, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.uz, p->uz, n * 4, hipMemcpyHostToDevice, st));
+        HS_TRY(hipMemcpyAsync(b.cell, p->cell, n * 4, hipMemcpyHostToDevice, st));
+        if (eng->has_erot) HS_TRY(hipMemcpyAsync(b.erot, p->erot, n * 4, hipMemcpyHostToDevice, st));
+        if (eng->has_type) HS_TRY(hipMemcpyAsync(b.type, p->type, n, hipMemcpyHostToDevice, st));
+        if (eng->has_vib)
+            for (int m = 0; m < eng->max_modes; ++m)
+                HS_TRY(hipMemcpyAsync(b.vib + m * eng->cap, p->vib + m * n, n * 2, hipMemcpyHostToDevice, st));
+    }
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    s.n = (int32_t)n;
+    s.n_old = (int32_t)n;
+    HS_TRY(hipMemcpyAsync(eng->d_S, &s, sizeof(s), hipMemcpyHostToDevice, st));
+    HS_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int hs_dsmc_particle_count(hs_dsmc_engine* eng, int64_t* n_out) {
+    if (!eng || !n_out) return fail(eng, HS_E_INVALID, "NULL argument");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    *n_out = s.n;
+    return 0;
+}
+
+extern "C" int hs_dsmc_download_particles(hs_dsmc_engine* eng, hs_particles* p, int64_t cap, int64_t* n_out) {
+    if (!eng || !p) return fail(eng, HS_E_INVALID, "NULL argument");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    const int64_t n = s.n;
+    if (n_out) *n_out = n;
+    if (n > cap) return fail(eng, HS_E_CAPACITY, "host buffer too small for the population");
+    DevParticles& b = cur_buf(eng);
+    hipStream_t st = eng->stream;
+    if (n > 0) {
+        if (p->px) HS_TRY(hipMemcpyAsync(p->px, b.px, n * 8, hipMemcpyDeviceToHost, st));
+        if (p->py) HS_TRY(hipMemcpyAsync(p->py, b.py, n * 8, hipMemcpyDeviceToHost, st));
+        if (p->pz) HS_TRY(hipMemcpyAsync(p->pz, b.pz, n * 8, hipMemcpyDeviceToHost, st));
+        if (p->ux) HS_TRY(hipMemcpyAsync(p->ux, b.ux, n * 4, hipMemcpyDeviceToHost, st));
+        if (p->uy) HS_TRY(hipMemcpyAsync(p->uy, b.uy, n * 4, hipMemcpyDeviceToHost, st));
+        if (p->uz) HS_TRY(hipMemcpyAsync(p->uz, b.uz, n * 4, hipMemcpyDeviceToHost, st));
+        if (p->cell) HS_TRY(hipMemcpyAsync(p->cell, b.cell, n * 4, hipMemcpyDeviceToHost, st));
+        if (p->erot && eng->has_erot) HS_TRY(hipMemcpyAsync(p->erot, b.erot, n * 4, hipMemcpyDeviceToHost, st));
+        if (p->type && eng->has_type) HS_TRY(hipMemcpyAsync(p->type, b.type, n, hipMemcpyDeviceToHost, st));
+        if (p->vib && eng->has_vib)
+            for (int m = 0; m < eng->max_modes; ++m)
+                HS_TRY(hipMemcpyAsync(p->vib + m * n, b.vib + m * eng->cap, n * 2, hipMemcpyDeviceToHost, st));
+    }
+    HS_TRY(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int hs_dsmc_set_cell_state(hs_dsmc_engine* eng, const double* sigma, const double* rem) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    if (sigma) HS_TRY(hipMemcpyAsync(eng->d_sigma, sigma, (size_t)eng->nc * 8, hipMemcpyHostToDevice, eng->stream));
+    if (rem) HS_TRY(hipMemcpyAsync(eng->d_rem, rem, (size_t)eng->nc * 8, hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_get_cell_state(hs_dsmc_engine* eng, double* sigma, double* rem) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    if (sigma) HS_TRY(hipMemcpyAsync(sigma, eng->d_sigma, (size_t)eng->nc * 8, hipMemcpyDeviceToHost, eng->stream));
+    if (rem) HS_TRY(hipMemcpyAsync(rem, eng->d_rem, (size_t)eng->nc * 8, hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_inflow_face_count(hs_dsmc_engine* eng, int64_t* n) {
+    if (!eng || !n) return fail(eng, HS_E_INVALID, "NULL argument");
+    *n = eng->faces.n;
+    return 0;
+}
+
+extern "C" int hs_dsmc_set_inflow_accumulators(hs_dsmc_engine* eng, const double* acc) {
+    if (!eng || !acc) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (eng->faces.n == 0) return 0;
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipMemcpyAsync(eng->faces.acc, acc, (size_t)eng->ns * eng->faces.n * 8, hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_get_inflow_accumulators(hs_dsmc_engine* eng, double* acc) {
+    if (!eng || !acc) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (eng->faces.n == 0) return 0;
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipMemcpyAsync(acc, eng->faces.acc, (size_t)eng->ns * eng->faces.n * 8, hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_set_step(hs_dsmc_engine* eng, uint32_t step) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    s.step = step;
+    HS_TRY(hipMemcpyAsync(eng->d_S, &s, sizeof(s), hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_get_step(hs_dsmc_engine* eng, uint32_t* step) {
+    if (!eng || !step) return fail(eng, HS_E_INVALID, "NULL argument");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    *step = s.step;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// initial fill
+// ---------------------------------------------------------------------------
+extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib,
+                            const double u[3], int32_t exact, uint32_t fill_id) {
+    if (!eng || !n || !u) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (!(t >= 0.0)) return fail(eng, HS_E_INVALID, "temperature must be non-negative");
+    HS_TRY(hipSetDevice(eng->device));
+    const int nc = eng->nc, ns = eng->ns;
+    // species-independent expected count guards the capacity
+    double expect = 0.0;
+    for (int s = 0; s < ns; ++s) {
+        if (n[s] < 0.0) return fail(eng, HS_E_INVALID, "number density must be non-negative");
+        for (int c = 0; c < nc; ++c) expect += n[s] * eng->vol_h[c] / eng->cfg.fn;
+    }
+    if (expect > (double)eng->cap) return fail(eng, HS_E_CAPACITY, "fill would exceed the particle capacity");
+    int32_t* cnt = nullptr;
+    int32_t* off = nullptr;
+    double* dn = nullptr;
+    double* du = nullptr;
+    hipStream_t st = eng->stream;
+    HS_TRY(hipMalloc(&cnt, (size_t)nc * ns * 4));
+    HS_TRY(hipMalloc(&off, ((size_t)nc * ns + 1) * 4));
+    HS_TRY(hipMalloc(&dn, HS_MAX_SPECIES * 8));
+    HS_TRY(hipMalloc(&du, 3 * 8));
+    double nh[HS_MAX_SPECIES] = { 0 };
+    for (int s = 0; s < ns; ++s) nh[s] = n[s];
+    int rc = 0;
+    if (hipMemcpyAsync(dn, nh, sizeof(nh), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(du, u, 24, hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = fail(eng, HS_E_HIP, "hipMemcpy in fill failed");
+    } else {
+        launch_fill_count(eng->dp, dn, exact, fill_id, eng->d_vol, cnt, nc, ns, st);
+        launch_scan(cnt, off, nc * ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap, st);
+        launch_fill_gen(eng->dp, dn, t, trot, tvib, du, fill_id, cnt, off, nc, ns, cur_buf(eng), eng->d_S, st);
+        launch_set_nold(eng->d_S, st);
+        int most = 0;
+        for (int s = 1; s < ns; ++s)
+            if (n[s] > n[most]) most = s;
+        const hs_species& sm = eng->cfg.species[most];
+        const double smax = (3.14159265358979323846 * sm.diameter * sm.diameter) * sqrt(2.0 * HS_KB * t / sm.mass);
+        launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, st);
+        rc = check_launch(eng, "fill");
+        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(eng, HS_E_HIP, "fill failed");
+    }
+    (void)hipFree(cnt);
+    (void)hipFree(off);
+    (void)hipFree(dn);
+    (void)hipFree(du);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// the time step
+// ---------------------------------------------------------------------------
+static void enqueue_inject(hs_dsmc_engine* eng) {
+    if (eng->faces.n == 0) return;
+    StageTimer tm(eng, T_INJECT);
+    launch_inject_count(eng->dp, eng->faces, eng->ns, eng->d_S, eng->stream);
+    launch_scan(eng->faces.n_new, eng->faces.off, eng->faces.n * eng->ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap,
+                eng->stream);
+    launch_inject_gen(eng->dp, eng->faces, eng->ns, cur_buf(eng), eng->d_S, eng->stream);
+}
+
+static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
+    StageTimer tm(eng, T_MOVE);
+    launch_move(eng->dp, cur_buf(eng), eng->d_S, eng->d_count, eng->d_slot, do_move, do_hist, eng->move_grid,
+                eng->stream);
+}
+
+static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance) {
+    {
+        StageTimer tm(eng, T_SCAN);
+        launch_scan(eng->d_count, eng->d_start, eng->nc, SCAN_CELLS, eng->ws, eng->d_S, eng->cap, eng->stream);
+    }
+    {
+        StageTimer tm(eng, T_SCATTER);
+        launch_scatter(cur_buf(eng), eng->d_S, eng->d_start, eng->d_slot, eng->d_perm, eng->move_grid, eng->stream);
+    }
+    {
+        StageTimer tm(eng, T_COLLIDE);
+        launch_collide(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_sigma, eng->d_rem,
+                       eng->d_samples, eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc,
+                       eng->d_big_list, eng->d_big_sub, eng->stream);
+    }
+    launch_commit(eng->d_S, advance, eng->stream);
+    eng->cur = 1 - eng->cur;
+    if (do_sample) eng->n_sampled += 1;
+}
+
+extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    if (nsteps < 0) return fail(eng, HS_E_INVALID, "nsteps must be non-negative");
+    HS_TRY(hipSetDevice(eng->device));
+    const int sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
+    const int collide = eng->cfg.collision_model != HS_COLL_NONE;
+    for (int k = 0; k < nsteps; ++k) {
+        enqueue_inject(eng);
+        enqueue_move(eng, 1, 1);
+        enqueue_sort_collide(eng, collide, sample, 1);
+    }
+    return check_launch(eng, "evolve");
+}
+
+extern "C" int hs_dsmc_inject(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    enqueue_inject(eng);
+    return check_launch(eng, "inject");
+}
+
+extern "C" int hs_dsmc_move(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    enqueue_move(eng, 1, 0);
+    return check_launch(eng, "move");
+}
+
+extern "C" int hs_dsmc_sort(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    enqueue_move(eng, 0, 1);
+    enqueue_sort_collide(eng, 0, 0, 0);
+    return check_launch(eng, "sort");
+}
+
+extern "C" int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    enqueue_move(eng, 0, 1);
+    enqueue_sort_collide(eng, eng->cfg.collision_model != HS_COLL_NONE, sample ? 1 : 0, 0);
+    return check_launch(eng, "collide");
+}
+
+extern "C" int hs_dsmc_advance_step(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    uint32_t s = 0;
+    if (int rc = hs_dsmc_get_step(eng, &s)) return rc;
+    return hs_dsmc_set_step(eng, s + 1);
+}
+
+extern "C" int hs_dsmc_synchronize(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// sampling, counters, timing
+// ---------------------------------------------------------------------------
+extern "C" int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    const int64_t need = (int64_t)eng->nslots * eng->nc;
+    if (out && n_doubles < need) return fail(eng, HS_E_INVALID, "sample buffer too small");
+    HS_TRY(hipSetDevice(eng->device));
+    if (out) HS_TRY(hipMemcpyAsync(out, eng->d_samples, need * 8, hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    if (nsteps_out) *nsteps_out = eng->n_sampled;
+    return 0;
+}
+
+extern "C" int hs_dsmc_reset_samples(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    launch_fill_f64(eng->d_samples, 0.0, (int64_t)eng->nslots * eng->nc, eng->stream);
+    eng->n_sampled = 0;
+    return check_launch(eng, "reset samples");
+}
+
+extern "C" int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n) {
+    if (!eng || !out) return fail(eng, HS_E_INVALID, "NULL argument");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    for (int k = 0; k < n && k < CNT_N; ++k) out[k] = (int64_t)s.counters[k];
+    return 0;
+}
+
+extern "C" int hs_dsmc_reset_counters(hs_dsmc_engine* eng) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    DevScalars s;
+    if (int rc = read_scalars(eng, &s)) return rc;
+    memset(s.counters, 0, sizeof(s.counters));
+    HS_TRY(hipMemcpyAsync(eng->d_S, &s, sizeof(s), hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_kernel_times(hs_dsmc_engine* eng, double* ms, int64_t* launches, int32_t n) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    for (auto& t : eng->timed) {
+        float x = 0.0f;
+        if (hipEventElapsedTime(&x, t.a, t.b) == hipSuccess) eng->closed_ms[t.stage] += x;
+        eng->event_pool.push_back(t.a);
+        eng->event_pool.push_back(t.b);
+    }
+    eng->timed.clear();
+    for (int k = 0; k < n && k < T_NSTAGE; ++k) {
+        if (ms) ms[k] = eng->closed_ms[k];
+        if (launches) launches[k] = eng->launches[k];
+    }
+    if (n < 0) {  // negative n: reset
+        for (int k = 0; k < T_NSTAGE; ++k) { eng->closed_ms[k] = 0.0; eng->launches[k] = 0; }
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// test hooks
+// ---------------------------------------------------------------------------
+extern "C" int hs_dsmc_binary_collide(hs_dsmc_engine* eng, int64_t n, const int32_t* type_p, const int32_t* type_q,
+                                      double* up, double* uq, double* erot_p, double* erot_q, int32_t* vib_p,
+                                      int32_t* vib_q, const double* uniforms, int32_t n_uniform, int32_t* used_out) {
+    if (!eng || !type_p || !type_q || !up || !uq || !uniforms || n_uniform <= 0)
+        return fail(eng, HS_E_INVALID, "NULL argument");
+    if (n <= 0) return 0;
+    for (int64_t k = 0; k < n; ++k)
+        if (type_p[k] < 0 || type_p[k] >= eng->ns || type_q[k] < 0 || type_q[k] >= eng->ns)
+            return fail(eng, HS_E_INVALID, "typeId out of range");
+    HS_TRY(hipSetDevice(eng->device));
+    hipStream_t st = eng->stream;
+    hipError_t ce = hipSuccess;
+    auto ck = [&](hipError_t x) { if (ce == hipSuccess) ce = x; };
+    int32_t *dtp, *dtq, *dvp = nullptr, *dvq = nullptr, *dused;
+    double *dup, *duq, *dep = nullptr, *deq = nullptr, *duni;
+    HS_TRY(hipMalloc(&dtp, n * 4));
+    HS_TRY(hipMalloc(&dtq, n * 4));
+    HS_TRY(hipMalloc(&dup, n * 24));
+    HS_TRY(hipMalloc(&duq, n * 24));
+    HS_TRY(hipMalloc(&duni, (size_t)n * n_uniform * 8));
+    HS_TRY(hipMalloc(&dused, n * 4));
+    if (erot_p) HS_TRY(hipMalloc(&dep, n * 8));
+    if (erot_q) HS_TRY(hipMalloc(&deq, n * 8));
+    if (vib_p) HS_TRY(hipMalloc(&dvp, n * 4 * HS_MAX_VIB_MODES));
+    if (vib_q) HS_TRY(hipMalloc(&dvq, n * 4 * HS_MAX_VIB_MODES));
+    ck(hipMemcpyAsync(dtp, type_p, n * 4, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(dtq, type_q, n * 4, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(dup, up, n * 24, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(duq, uq, n * 24, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(duni, uniforms, (size_t)n * n_uniform * 8, hipMemcpyHostToDevice, st));
+    if (dep) ck(hipMemcpyAsync(dep, erot_p, n * 8, hipMemcpyHostToDevice, st));
+    if (deq) ck(hipMemcpyAsync(deq, erot_q, n * 8, hipMemcpyHostToDevice, st));
+    if (dvp) ck(hipMemcpyAsync(dvp, vib_p, n * 4 * HS_MAX_VIB_MODES, hipMemcpyHostToDevice, st));
+    if (dvq) ck(hipMemcpyAsync(dvq, vib_q, n * 4 * HS_MAX_VIB_MODES, hipMemcpyHostToDevice, st));
+    launch_binary_collide(eng->dp, n, dtp, dtq, dup, duq, dep, deq, dvp, dvq, duni, n_uniform, dused, st);
+    int rc = check_launch(eng, "binary collide");
+    ck(hipMemcpyAsync(up, dup, n * 24, hipMemcpyDeviceToHost, st));
+    ck(hipMemcpyAsync(uq, duq, n * 24, hipMemcpyDeviceToHost, st));
+    if (dep) ck(hipMemcpyAsync(erot_p, dep, n * 8, hipMemcpyDeviceToHost, st));
+    if (deq) ck(hipMemcpyAsync(erot_q, deq, n * 8, hipMemcpyDeviceToHost, st));
+    if (dvp) ck(hipMemcpyAsync(vib_p, dvp, n * 4 * HS_MAX_VIB_MODES, hipMemcpyDeviceToHost, st));
+    if (dvq) ck(hipMemcpyAsync(vib_q, dvq, n * 4 * HS_MAX_VIB_MODES, hipMemcpyDeviceToHost, st));
+    std::vector<int32_t> used(n);
+    ck(hipMemcpyAsync(used.data(), dused, n * 4, hipMemcpyDeviceToHost, st));
+    ck(hipStreamSynchronize(st));
+    if (ce != hipSuccess && !rc) rc = fail(eng, HS_E_HIP, std::string("binary collide: ") + hipGetErrorString(ce));
+    if (used_out) memcpy(used_out, used.data(), n * 4);
+    for (int64_t k = 0; k < n && !rc; ++k)
+        if (used[k] > n_uniform) rc = fail(eng, HS_E_OVERFLOW, "a pair consumed more uniforms than provided");
+    (void)hipFree(dtp); (void)hipFree(dtq); (void)hipFree(dup); (void)hipFree(duq); (void)hipFree(duni); (void)hipFree(dused);
+    if (dep) (void)hipFree(dep);
+    if (deq) (void)hipFree(deq);
+    if (dvp) (void)hipFree(dvp);
+    if (dvq) (void)hipFree(dvq);
+    return rc;
+}
+
+extern "C" int hs_dsmc_sigma_tcr(hs_dsmc_engine* eng, int64_t n, const int32_t* type_p, const int32_t* type_q,
+                                 const double* up, const double* uq, double* out) {
+    if (!eng || !type_p || !type_q || !up || !uq || !out) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (n <= 0) return 0;
+    for (int64_t k = 0; k < n; ++k)
+        if (type_p[k] < 0 || type_p[k] >= eng->ns || type_q[k] < 0 || type_q[k] >= eng->ns)
+            return fail(eng, HS_E_INVALID, "typeId out of range");
+    HS_TRY(hipSetDevice(eng->device));
+    hipStream_t st = eng->stream;
+    hipError_t ce = hipSuccess;
+    auto ck = [&](hipError_t x) { if (ce == hipSuccess) ce = x; };
+    int32_t *dtp, *dtq;
+    double *dup, *duq, *dout;
+    HS_TRY(hipMalloc(&dtp, n * 4));
+    HS_TRY(hipMalloc(&dtq, n * 4));
+    HS_TRY(hipMalloc(&dup, n * 24));
+    HS_TRY(hipMalloc(&duq, n * 24));
+    HS_TRY(hipMalloc(&dout, n * 8));
+    ck(hipMemcpyAsync(dtp, type_p, n * 4, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(dtq, type_q, n * 4, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(dup, up, n * 24, hipMemcpyHostToDevice, st));
+    ck(hipMemcpyAsync(duq, uq, n * 24, hipMemcpyHostToDevice, st));
+    launch_sigma_tcr(eng->dp, n, dtp, dtq, dup, duq, dout, st);
+    int rc = check_launch(eng, "sigma_tcr");
+    ck(hipMemcpyAsync(out, dout, n * 8, hipMemcpyDeviceToHost, st));
+    ck(hipStreamSynchronize(st));
+    if (ce != hipSuccess && !rc) rc = fail(eng, HS_E_HIP, std::string("sigma_tcr: ") + hipGetErrorString(ce));
+    (void)hipFree(dtp); (void)hipFree(dtq); (void)hipFree(dup); (void)hipFree(duq); (void)hipFree(dout);
+    return rc;
+}

@@ -1,0 +1,74 @@
+// dsmc_kernels.h -- host-side launchers of the engine's HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dsmc_device.h"
+
+// Inflow face table (device arrays, one entry per inflow face).
+struct DevFaces {
+    int32_t* cell;
+    int8_t* axis;
+    int8_t* dir;
+    int32_t* a1i;
+    int32_t* a2i;
+    double* inc;       // [species][face] per-step accumulator increment
+    double* acc;       // [species][face] accumulatedParcelsToInsert_
+    int32_t* n_new;    // [face*ns + s] parcels to insert this step
+    int32_t* off;      // exclusive scan of n_new
+    int32_t n;
+};
+
+// Device scalars of one engine (one allocation).
+struct DevScalars {
+    int32_t n;        // current particle count
+    int32_t n_old;    // count before this step's injection
+    uint32_t step;    // time-step index (random-stream counter)
+    int32_t total;    // scratch: last scan total
+    int32_t n_sorted; // population after the cell scan (published by commit)
+    unsigned long long counters[CNT_N];
+};
+
+// scan workspace for up to n elements
+struct ScanWs {
+    int32_t* bsum;
+    int32_t nb_max;
+};
+
+// what the scan writes besides the exclusive prefix
+enum ScanMode {
+    SCAN_PLAIN = 0,     // out[n] = total
+    SCAN_CELLS = 1,     // out[n] = total, zero the input, S.n_sorted = total
+    SCAN_APPEND = 2     // S.n_old = S.n, S.n = min(S.n + total, capacity)
+};
+
+void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, ScanWs& ws, DevScalars* S,
+                 int64_t capacity, hipStream_t st);
+void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st);
+void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
+                       hipStream_t st);
+void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
+                 int do_move, int do_hist, int grid, hipStream_t st);
+void launch_set_nold(DevScalars* S, hipStream_t st);
+void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
+                    int32_t* perm, int grid, hipStream_t st);
+void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
+                    const int32_t* perm, double* sigma, double* rem, double* samples, const double* vol,
+                    DevScalars* S, int do_collide, int do_sample, int general, int nc,
+                    uint16_t* big_list, uint8_t* big_sub, hipStream_t st);
+void launch_commit(DevScalars* S, int advance, hipStream_t st);
+void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
+                       int32_t* cnt, int nc, int ns, hipStream_t st);
+void launch_fill_gen(const DevParams* P, const double* nd, double T, double Trot, double Tvib, const double* u3,
+                     uint32_t fill_id, const int32_t* cnt, const int32_t* off, int nc, int ns, DevParticles b,
+                     DevScalars* S, hipStream_t st);
+void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double* sigma, double* rem, int nc,
+                       hipStream_t st);
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
+void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
+void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, double* up,
+                           double* uq, double* erp, double* erq, int32_t* vp, int32_t* vq, const double* uni,
+                           int32_t nu, int32_t* used, hipStream_t st);
+void launch_sigma_tcr(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, const double* up,
+                      const double* uq, double* out, hipStream_t st);

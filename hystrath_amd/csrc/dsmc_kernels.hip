@@ -1,0 +1,940 @@
+// dsmc_kernels.hip -- the per-timestep kernels of the MI355X DSMC engine.
+//
+// One time step (dsmcCloud::evolve, dsmcCloud.C:819-926) is
+//   inject_count -> scan(append) -> inject_gen        freestream inflow
+//   move (+ per-cell histogram via atomics)           Cloud<dsmcParcel>::move
+//   scan(cells)                                       cell start offsets
+//   scatter                                           slot -> old index
+//   collide                                           stable gather into cell
+//                                                     order, NTC + binary
+//                                                     collisions, sampling
+// All counts live on the device so a step can be replayed without host
+// round trips.  Every kernel checks its indices against the device-side
+// counts; loops that draw random numbers are bounded.
+#include "dsmc_kernels.h"
+
+#define BLOCK 256
+#define WAVE 64
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+__device__ __forceinline__ int wave_incl_scan(int x, int lane) {
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+        const int y = __shfl_up(x, o, WAVE);
+        if (lane >= o) x += y;
+    }
+    return x;
+}
+
+// exclusive scan over a 256-thread block; returns the exclusive prefix of v
+// and the block total.
+__device__ __forceinline__ int block_excl_scan(int v, int* total) {
+    __shared__ int ws[BLOCK / WAVE];
+    const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+    const int incl = wave_incl_scan(v, lane);
+    if (lane == WAVE - 1) ws[wid] = incl;
+    __syncthreads();
+    int woff = 0, tot = 0;
+#pragma unroll
+    for (int w = 0; w < BLOCK / WAVE; ++w) {
+        const int x = ws[w];
+        if (w < wid) woff += x;
+        tot += x;
+    }
+    __syncthreads();
+    *total = tot;
+    return incl - v + woff;
+}
+
+__device__ __forceinline__ void add_counter(DevScalars* S, int k, unsigned long long v) {
+    v = wave_sum_u64(v);
+    if ((threadIdx.x & (WAVE - 1)) == 0 && v) atomicAdd(&S->counters[k], v);
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan (reduce -> scan block sums -> apply), 2048 items per block
+// ---------------------------------------------------------------------------
+#define SCAN_ITEMS 8
+#define SCAN_TILE (BLOCK * SCAN_ITEMS)
+
+__global__ __launch_bounds__(BLOCK) void k_scan_reduce(const int32_t* __restrict__ in, int32_t n,
+                                                       int32_t* __restrict__ bsum) {
+    const int base = blockIdx.x * SCAN_TILE;
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int i = base + k * BLOCK + threadIdx.x;
+        if (i < n) s += in[i];
+    }
+    int tot;
+    (void)block_excl_scan(s, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_blocks(int32_t* __restrict__ bsum, int32_t nb,
+                                                       DevScalars* __restrict__ S) {
+    int carry = 0;
+    for (int base = 0; base < nb; base += BLOCK) {
+        const int i = base + threadIdx.x;
+        const int v = i < nb ? bsum[i] : 0;
+        int tot;
+        const int ex = block_excl_scan(v, &tot);
+        if (i < nb) bsum[i] = ex + carry;
+        carry += tot;
+    }
+    if (threadIdx.x == 0) S->total = carry;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_apply(int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                      int32_t n, const int32_t* __restrict__ bsum, int mode,
+                                                      DevScalars* __restrict__ S, int64_t capacity) {
+    const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_ITEMS;
+    int v[SCAN_ITEMS];
+    int s = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int i = base + k;
+        v[k] = i < n ? in[i] : 0;
+        s += v[k];
+    }
+    int tot;
+    int ex = block_excl_scan(s, &tot) + bsum[blockIdx.x];
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int i = base + k;
+        if (i < n) {
+            out[i] = ex;
+            if (mode == SCAN_CELLS) in[i] = 0;
+        }
+        ex += v[k];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
+        const int total = S->total;
+        out[n] = total;
+        if (mode == SCAN_CELLS) {
+            S->n_sorted = total;
+        } else if (mode == SCAN_APPEND) {
+            const int64_t nn = (int64_t)S->n + total;
+            S->n_old = S->n;
+            S->n = (int32_t)(nn > capacity ? capacity : nn);
+            if (nn > capacity) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - capacity));
+        }
+    }
+}
+
+void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, ScanWs& ws, DevScalars* S,
+                 int64_t capacity, hipStream_t st) {
+    const int nb = n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+    k_scan_reduce<<<nb, BLOCK, 0, st>>>(in, n, ws.bsum);
+    k_scan_blocks<<<1, BLOCK, 0, st>>>(ws.bsum, nb, S);
+    k_scan_apply<<<nb, BLOCK, 0, st>>>(const_cast<int32_t*>(in), out, n, ws.bsum, (int)mode, S, capacity);
+}
+
+// ---------------------------------------------------------------------------
+// generic fills (graph-safe replacements for hipMemsetAsync)
+// ---------------------------------------------------------------------------
+__global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+__global__ void k_fill_f64(double* p, double v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+static int grid_for(int64_t n, int cap = 4096) {
+    int64_t g = (n + BLOCK - 1) / BLOCK;
+    if (g < 1) g = 1;
+    if (g > cap) g = cap;
+    return (int)g;
+}
+void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
+    k_fill_i32<<<grid_for(n), BLOCK, 0, st>>>(p, v, n);
+}
+void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
+    k_fill_f64<<<grid_for(n), BLOCK, 0, st>>>(p, v, n);
+}
+
+// ---------------------------------------------------------------------------
+// new-particle internal state (equipartition at the given temperatures)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void new_internal(const DevParams& P, DevParticles& b, int64_t j, int t, double Trot,
+                                             double Tvib, PhiloxStream& r) {
+    if (b.erot) b.erot[j] = (float)equip_erot(r, Trot, P.rotdof[t]);
+    if (b.vib)
+        for (int m = 0; m < P.nvib[t]; ++m)
+            b.vib[m * P.capacity + j] = (uint16_t)equip_vib_level(r, Tvib, P.theta_v[t][m]);
+    if (b.type) b.type[j] = (uint8_t)t;
+}
+
+// ---------------------------------------------------------------------------
+// freestream inflow (dsmcFreeStreamInflowPatch.C:91-364)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restrict__ Pp, DevFaces F, int ns,
+                                                        DevScalars* __restrict__ S) {
+    const DevParams& P = *Pp;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= F.n * ns) return;
+    const int f = e / ns, s = e - f * ns;
+    if (!(P.inflow_n[s] > 0.0)) { F.n_new[e] = 0; return; }
+    double* acc = &F.acc[s * F.n + f];
+    double a = *acc + F.inc[s * F.n + f];
+    int ni = (int)a;
+    if (ni < 0) ni = 0;
+    PhiloxStream r;
+    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 0);
+    if ((a - (double)ni) > r.u()) ni += 1;
+    a -= (double)ni;
+    *acc = a;
+    F.n_new[e] = ni;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
+                                                      DevParticles b, DevScalars* __restrict__ S) {
+    const DevParams& P = *Pp;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= F.n * ns) return;
+    const int ni = F.n_new[e];
+    if (ni == 0) return;
+    const int f = e / ns, s = e - f * ns;
+    const int64_t base = (int64_t)S->n_old + F.off[e];
+    PhiloxStream r;
+    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 0);
+    (void)r.u();  // the Poisson-rounding draw of k_inject_count
+    const int a = F.axis[f], dir = F.dir[f];
+    const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+    const int nn[3] = { P.nx, P.ny, P.nz };
+    const double mass = P.mass[s];
+    const double mps = sqrt(2.0 * HS_KB * P.inflow_t / mass);
+    const double un_face = P.inflow_u[a] * (double)(-dir);
+    const double s_cos = un_face / mps;
+    const double coef_a = s_cos + sqrt(s_cos * s_cos + 2.0);
+    const double coef_b = 0.5 * (1.0 + s_cos * (s_cos - sqrt(s_cos * s_cos + 2.0)));
+    double scaling = 3.0;
+    if (s_cos < -3.0) scaling = fabs(s_cos) + 1.0;
+    const double sc = sqrt(HS_KB * P.inflow_t / mass);
+    unsigned long long injected = 0;
+    for (int k = 0; k < ni; ++k) {
+        const int64_t j = base + k;
+        if (j >= P.capacity) break;
+        double pos[3];
+        const double ua = r.u(), ub = r.u();
+        pos[a] = face_coord(P, a, dir > 0 ? nn[a] : 0);
+        pos[a1] = P.empty[a1] ? P.centre[a1] : face_coord(P, a1, F.a1i[f]) + ua * P.delta[a1];
+        pos[a2] = P.empty[a2] ? P.centre[a2] : face_coord(P, a2, F.a2i[f]) + ub * P.delta[a2];
+        double un;
+        if (fabs(un_face) > HS_VSMALL) {
+            double Pa;
+            int guard = 0;
+            do {
+                const double unt = scaling * (2.0 * r.u() - 1.0);
+                un = unt + s_cos;
+                if (un < 0.0) Pa = -1.0;
+                else Pa = 2.0 * un / coef_a * dm_exp(coef_b - unt * unt);
+            } while (Pa < r.u() && ++guard < HS_MAX_REDRAW);
+        } else {
+            un = sqrt(-dm_log(1.0 - r.u()));
+        }
+        const double g1 = r.gauss(), g2 = r.gauss();
+        double U[3];
+        U[a] = mps * un * (double)(-dir);
+        U[a1] = sc * g1 + P.inflow_u[a1];
+        U[a2] = sc * g2 + P.inflow_u[a2];
+        b.px[j] = pos[0]; b.py[j] = pos[1]; b.pz[j] = pos[2];
+        b.ux[j] = (float)U[0]; b.uy[j] = (float)U[1]; b.uz[j] = (float)U[2];
+        b.cell[j] = F.cell[f];
+        if (b.vib)
+            for (int m = 0; m < P.max_modes; ++m) b.vib[m * P.capacity + j] = 0;
+        new_internal(P, b, j, s, P.inflow_trot, P.inflow_tvib, r);
+        ++injected;
+    }
+    if (injected) atomicAdd(&S->counters[CNT_INJ], injected);
+}
+
+void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st) {
+    const int n = F.n * ns;
+    if (n == 0) return;
+    k_inject_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, S);
+}
+
+void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
+                       hipStream_t st) {
+    const int n = F.n * ns;
+    if (n == 0) return;
+    k_inject_gen<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, b, S);
+}
+
+// ---------------------------------------------------------------------------
+// move (dsmcParcel::move, dsmcParcel.C:38-148, on the Cartesian block) with
+// the cell histogram of buildCellOccupancy fused in
+// ---------------------------------------------------------------------------
+struct MoveCounts {
+    unsigned long long del, wall, ovf;
+};
+
+__device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is_new, uint32_t step, MoveCounts& mc) {
+    PhiloxStream r;
+    r.init(P, RNG_MOVE, (uint32_t)i, step, 0);
+    double t = P.dt;
+    if (is_new) {
+        const double sf = r.u();
+        t = (1.0 - sf) * P.dt;
+    }
+    const int32_t c = b.cell[i];
+    if (c < 0) return;
+    double p[3] = { b.px[i], b.py[i], b.pz[i] };
+    double U[3] = { (double)b.ux[i], (double)b.uy[i], (double)b.uz[i] };
+    int idx[3] = { c % P.nx, (c / P.nx) % P.ny, c / (P.nx * P.ny) };
+    const int nn[3] = { P.nx, P.ny, P.nz };
+    const int t_id = b.type ? (int)b.type[i] : 0;
+    float* erot = b.erot ? &b.erot[i] : nullptr;
+    uint16_t* vib = (b.vib && P.nvib[t_id] > 0) ? &b.vib[i] : nullptr;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+        if (P.empty[a]) p[a] = P.centre[a];
+    bool keep = true;
+    int iter = 0;
+    while (keep && t > HS_ROOTVSMALL) {
+        if (++iter > HS_MAX_TRACK) { mc.ovf += 1; break; }
+        double ut[3], e[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            ut[a] = P.empty[a] ? 0.0 : U[a];
+            e[a] = p[a] + t * ut[a];
+        }
+        double lam = 1.0;
+        int axis = -1, dir = 0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const double den = e[a] - p[a];
+            if (ut[a] > 0.0 && den > 0.0) {
+                double l = (face_coord(P, a, idx[a] + 1) - p[a]) / den;
+                if (l < 0.0) l = 0.0;
+                if (l < lam) { lam = l; axis = a; dir = 1; }
+            } else if (ut[a] < 0.0 && den < 0.0) {
+                double l = (face_coord(P, a, idx[a]) - p[a]) / den;
+                if (l < 0.0) l = 0.0;
+                if (l < lam) { lam = l; axis = a; dir = -1; }
+            }
+        }
+        bool wall = false;
+        if (P.cyl) {
+            const double dx = e[0] - p[0], dy = e[1] - p[1];
+            const double rx = p[0] - P.cyl_c[0], ry = p[1] - P.cyl_c[1];
+            const double bb = rx * dx + ry * dy;
+            if (bb < 0.0) {
+                const double aa = dx * dx + dy * dy;
+                const double cc = rx * rx + ry * ry - P.cyl_r * P.cyl_r;
+                const double disc = bb * bb - aa * cc;
+                if (disc >= 0.0) {
+                    const double q = sqrt(disc) - bb;
+                    double sh = cc / q;
+                    if (sh < 0.0) sh = 0.0;
+                    if (sh < lam) { lam = sh; wall = true; }
+                }
+            }
+        }
+        if (!wall && axis < 0) {
+            p[0] = e[0]; p[1] = e[1]; p[2] = e[2];
+            t = 0.0;
+            break;
+        }
+#pragma unroll
+        for (int q = 0; q < 3; ++q) p[q] = p[q] + lam * (e[q] - p[q]);
+        t = t - t * lam;
+        if (wall) {
+            mc.wall += 1;
+            const double nx_ = P.cyl_c[0] - p[0], ny_ = P.cyl_c[1] - p[1];
+            const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
+            const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
+            if (P.cyl_model == HS_PATCH_DIFFUSE) {
+                diffuse_reflect(P, P.cyl_T, P.cyl_U, nw, t_id, U, erot, vib, P.capacity, r);
+            } else {
+                const double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+                if (und > 0.0)
+                    for (int q = 0; q < 3; ++q) U[q] = U[q] - 2.0 * und * nw[q];
+            }
+            continue;
+        }
+        p[axis] = face_coord(P, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
+        const int ni = idx[axis] + dir;
+        if (ni >= 0 && ni < nn[axis]) { idx[axis] = ni; continue; }
+        const int side = 2 * axis + (dir > 0 ? 1 : 0);
+        const int model = P.patch_model[side];
+        if (model == HS_PATCH_CYCLIC) {
+            p[axis] = face_coord(P, axis, dir > 0 ? 0 : nn[axis]);
+            idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
+        } else if (model == HS_PATCH_DIFFUSE) {
+            mc.wall += 1;
+            double nw[3] = { 0.0, 0.0, 0.0 };
+            nw[axis] = (double)dir;
+            diffuse_reflect(P, P.patch_T[side], P.patch_U[side], nw, t_id, U, erot, vib, P.capacity, r);
+        } else if (model == HS_PATCH_DELETION) {
+            keep = false;
+        } else {
+            mc.wall += 1;
+            if (U[axis] * (double)dir > 0.0) U[axis] = -U[axis];
+        }
+    }
+    b.px[i] = p[0]; b.py[i] = p[1]; b.pz[i] = p[2];
+    b.ux[i] = (float)U[0]; b.uy[i] = (float)U[1]; b.uz[i] = (float)U[2];
+    if (!keep) { b.cell[i] = -1; mc.del += 1; }
+    else b.cell[i] = idx[0] + P.nx * (idx[1] + P.ny * idx[2]);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
+                                                DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                                int32_t* __restrict__ slot, int do_move, int do_hist) {
+    const DevParams& P = *Pp;
+    const int64_t n = S->n, n_old = S->n_old;
+    const uint32_t step = S->step;
+    MoveCounts mc = { 0, 0, 0 };
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        if (do_move) move_one(P, b, i, i >= n_old, step, mc);
+        if (do_hist) {
+            const int32_t c = b.cell[i];
+            if (c >= 0 && c < P.nc) slot[i] = atomicAdd(&count[c], 1);
+            else b.cell[i] = -1;
+        }
+    }
+    add_counter(S, CNT_DEL, mc.del);
+    add_counter(S, CNT_WALL, mc.wall);
+    add_counter(S, CNT_TRACK_OVF, mc.ovf);
+}
+
+void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot, int do_move,
+                 int do_hist, int grid, hipStream_t st) {
+    k_move<<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, do_move, do_hist);
+}
+
+__global__ void k_set_nold(DevScalars* S) { S->n_old = S->n; }
+void launch_set_nold(DevScalars* S, hipStream_t st) { k_set_nold<<<1, 1, 0, st>>>(S); }
+
+// ---------------------------------------------------------------------------
+// scatter: perm[start[cell] + slot] = old index
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ cell, const DevScalars* __restrict__ S,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ slot, int32_t* __restrict__ perm) {
+    const int64_t n = S->n;  // pre-sort population (the cell scan writes n_sorted)
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+        const int32_t c = cell[i];
+        if (c >= 0) perm[start[c] + slot[i]] = (int32_t)i;
+    }
+}
+
+void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
+                    int32_t* perm, int grid, hipStream_t st) {
+    k_scatter<<<grid, BLOCK, 0, st>>>(b.cell, S, start, slot, perm);
+}
+
+// ---------------------------------------------------------------------------
+// collide: stable gather into cell order, NTC, binary collisions, sampling
+// (noTimeCounter.C:81-339, dsmcVolFields.C:1115-1239)
+// ---------------------------------------------------------------------------
+#define TILE_CELLS 64
+
+// 8 sub-cell counters packed as 16-bit fields in two 64-bit words
+struct Pack8 {
+    unsigned long long lo, hi;
+    __device__ __forceinline__ void zero() { lo = 0; hi = 0; }
+    __device__ __forceinline__ int get(int s) const {
+        return s < 4 ? (int)((lo >> (16 * s)) & 0xFFFF) : (int)((hi >> (16 * (s - 4))) & 0xFFFF);
+    }
+    __device__ __forceinline__ void inc(int s) {
+        if (s < 4) lo += 1ull << (16 * s);
+        else hi += 1ull << (16 * (s - 4));
+    }
+};
+
+struct CellCtx {
+    unsigned long long cand, colls, ovf;
+};
+
+// NTC over one cell whose particles sit at ss..ss+nC-1 of the given arrays
+// (LDS tile or, for an oversize cell, global memory).
+template <bool GEN>
+__device__ __forceinline__ void ntc_cell(const DevParams& P, int c, int ss, int nC, float* vx, float* vy, float* vz,
+                                         const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
+                                         const uint8_t* ty, double* sigma, double* rem, const double* vol,
+                                         uint32_t step, const DevParticles& pos, int64_t gs, CellCtx& cx,
+                                         double& ncoll, double& sep) {
+    if (nC <= 1) return;
+    Pack8 cnt;
+    cnt.zero();
+    for (int k = 0; k < nC; ++k) cnt.inc(sub[ss + k]);
+    Pack8 off, fill;
+    off.zero();
+    {
+        int acc = 0;
+        for (int q = 0; q < 8; ++q) {
+            if (q < 4) off.lo |= (unsigned long long)acc << (16 * q);
+            else off.hi |= (unsigned long long)acc << (16 * (q - 4));
+            acc += cnt.get(q);
+        }
+    }
+    fill = off;
+    for (int k = 0; k < nC; ++k) {
+        const int q = sub[ss + k];
+        slist[ss + fill.get(q)] = (uint16_t)k;
+        fill.inc(q);
+    }
+    PhiloxStream r;
+    r.init(P, RNG_COLLIDE, (uint32_t)c, step, 0);
+    const double smax = sigma[c];
+    double smax_cur = smax;
+    const double sel = rem[c] + 0.5 * nC * (nC - 1) * P.fn * smax * P.dt / vol[c];
+    long long ncand = (long long)sel;
+    rem[c] = sel - (double)ncand;
+    if (ncand > HS_MAX_CAND) { ncand = HS_MAX_CAND; cx.ovf += 1; }
+    cx.cand += (unsigned long long)ncand;
+    for (long long k = 0; k < ncand; ++k) {
+        const int pp = random_label(r, 0, nC - 1);
+        const int q = sub[ss + pp];
+        const int nsc = cnt.get(q);
+        int qq = -1, guard = 0;
+        if (nsc > 1) {
+            const int o = off.get(q);
+            do { qq = slist[ss + o + random_label(r, 0, nsc - 1)]; } while (pp == qq && ++guard < HS_MAX_REDRAW);
+        } else {
+            do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
+        }
+        if (pp == qq) continue;
+        const int ip = ss + pp, iq = ss + qq;
+        const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
+        if (P.charge[tp] == -1 && P.charge[tq] == -1) continue;
+        double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
+        double uq[3] = { (double)vx[iq], (double)vy[iq], (double)vz[iq] };
+        const double stcr = sigma_tcr(P, tp, tq, up, uq);
+        if (stcr > smax_cur) smax_cur = stcr;
+        if (stcr / smax > r.u()) {
+            double erp = 0.0, erq = 0.0;
+            int vp[HS_MAX_VIB_MODES] = { 0, 0, 0 }, vq[HS_MAX_VIB_MODES] = { 0, 0, 0 };
+            if (GEN) {
+                if (P.has_erot) { erp = (double)er[ip]; erq = (double)er[iq]; }
+                for (int m = 0; m < P.nvib[tp]; ++m) vp[m] = vb[m * vstride + ip];
+                for (int m = 0; m < P.nvib[tq]; ++m) vq[m] = vb[m * vstride + iq];
+            }
+            binary_collide(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
+            vx[ip] = (float)up[0]; vy[ip] = (float)up[1]; vz[ip] = (float)up[2];
+            vx[iq] = (float)uq[0]; vy[iq] = (float)uq[1]; vz[iq] = (float)uq[2];
+            if (GEN) {
+                if (P.has_erot) { er[ip] = (float)erp; er[iq] = (float)erq; }
+                for (int m = 0; m < P.nvib[tp]; ++m) vb[m * vstride + ip] = (uint16_t)vp[m];
+                for (int m = 0; m < P.nvib[tq]; ++m) vb[m * vstride + iq] = (uint16_t)vq[m];
+            }
+            ncoll += 1.0;
+            if (P.flags & HS_FLAG_COLL_SEP) {
+                const int64_t gp = gs + pp, gq = gs + qq;
+                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq],
+                             ddz = pos.pz[gp] - pos.pz[gq];
+                sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            }
+            cx.colls += 1;
+        }
+    }
+    sigma[c] = smax_cur;
+}
+
+// dsmcVolFields::calculateField per-cell sums for one cell
+template <bool GEN>
+__device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, int nC, const float* vx,
+                                            const float* vy, const float* vz, const float* er, const uint16_t* vb,
+                                            int vstride, const uint8_t* ty, double* samples, int nc, double ncoll,
+                                            double sep) {
+    double aN = 0.0, aM = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0, aZr = 0.0;
+    for (int k = 0; k < nC; ++k) {
+        const int i = ss + k;
+        const int t = GEN ? (int)ty[i] : 0;
+        const double m = P.mass[t];
+        const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+        const double lke = m * (ux * ux + uy * uy + uz * uz);
+        aN += 1.0;
+        aM += m;
+        aX += m * ux;
+        aY += m * uy;
+        aZ += m * uz;
+        aC += lke;
+        if (GEN && P.has_erot) aR += (double)er[i];
+        aZr += P.rotdof[t];
+    }
+    samples[(int64_t)HS_S_N * nc + c] += aN;
+    samples[(int64_t)HS_S_M * nc + c] += aM;
+    samples[(int64_t)HS_S_MUX * nc + c] += aX;
+    samples[(int64_t)HS_S_MUY * nc + c] += aY;
+    samples[(int64_t)HS_S_MUZ * nc + c] += aZ;
+    samples[(int64_t)HS_S_MCC * nc + c] += aC;
+    samples[(int64_t)HS_S_EROT * nc + c] += aR;
+    samples[(int64_t)HS_S_ZROT * nc + c] += aZr;
+    samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
+    samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
+    const int ns = P.ns;
+    if (!GEN) {
+        samples[(int64_t)HS_S_BASE_COUNT * nc + c] += aN;
+    } else {
+        for (int s = 0; s < ns; ++s) {
+            double cntS = 0.0;
+            double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+            for (int k = 0; k < nC; ++k) {
+                const int i = ss + k;
+                if ((int)ty[i] != s) continue;
+                cntS += 1.0;
+                for (int m = 0; m < P.nvib[s]; ++m) ev[m] += (double)vb[m * vstride + i] * HS_KB * P.theta_v[s][m];
+            }
+            samples[(int64_t)(HS_S_BASE_COUNT + s) * nc + c] += cntS;
+            for (int m = 0; m < P.nvib[s]; ++m)
+                samples[(int64_t)(HS_S_BASE_COUNT + ns + s * HS_MAX_VIB_MODES + m) * nc + c] += ev[m];
+        }
+    }
+    if (P.flags & HS_FLAG_HEATFLUX) {
+        double h[13];
+        for (int q = 0; q < 13; ++q) h[q] = 0.0;
+        for (int k = 0; k < nC; ++k) {
+            const int i = ss + k;
+            const int t = GEN ? (int)ty[i] : 0;
+            const double m = P.mass[t];
+            const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+            const double lke = m * (ux * ux + uy * uy + uz * uz);
+            double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
+            double ev = 0.0;
+            if (GEN)
+                for (int mo = 0; mo < P.nvib[t]; ++mo) ev += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[t][mo];
+            const double eint = erv + ev;
+            h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+            h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+            h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+            h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+        }
+        const int hb = HS_S_BASE_COUNT + ns + ns * HS_MAX_VIB_MODES;
+        for (int q = 0; q < 13; ++q) samples[(int64_t)(hb + q) * nc + c] += h[q];
+    }
+}
+
+__device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, double y, double z) {
+    const int ci = c % P.nx, cj = (c / P.nx) % P.ny, ck = c / (P.nx * P.ny);
+    const double cc0 = P.origin[0] + ((double)ci + 0.5) * P.delta[0];
+    const double cc1 = P.origin[1] + ((double)cj + 0.5) * P.delta[1];
+    const double cc2 = P.origin[2] + ((double)ck + 0.5) * P.delta[2];
+    return (x - cc0 >= 0.0 ? 1 : 0) + 2 * (y - cc1 >= 0.0 ? 1 : 0) + 4 * (z - cc2 >= 0.0 ? 1 : 0);
+}
+
+template <bool GEN, int PMAX>
+__global__ __launch_bounds__(WAVE) void k_collide(const DevParams* __restrict__ Pp, DevParticles src,
+                                                  DevParticles dst, const int32_t* __restrict__ start,
+                                                  const int32_t* __restrict__ perm, double* __restrict__ sigma,
+                                                  double* __restrict__ rem, double* __restrict__ samples,
+                                                  const double* __restrict__ vol, DevScalars* __restrict__ S,
+                                                  int do_collide, int do_sample, int nc, uint16_t* big_list,
+                                                  uint8_t* big_sub) {
+    __shared__ int32_t key[PMAX];
+    __shared__ uint8_t loc[PMAX];
+    __shared__ float vx[PMAX], vy[PMAX], vz[PMAX];
+    __shared__ uint8_t sub[PMAX];
+    __shared__ uint16_t slist[PMAX];
+    __shared__ float er[GEN ? PMAX : 1];
+    __shared__ uint16_t vb[GEN ? HS_MAX_VIB_MODES * PMAX : 1];
+    __shared__ uint8_t ty[GEN ? PMAX : 1];
+    __shared__ int32_t segS[TILE_CELLS], segE[TILE_CELLS];
+
+    const DevParams& P = *Pp;
+    const int lane = threadIdx.x;
+    const int c0 = blockIdx.x * TILE_CELLS;
+    if (c0 >= nc) return;
+    const int ncell = min(TILE_CELLS, nc - c0);
+    const uint32_t step = S->step;
+    const int64_t cap = P.capacity;
+    int s_l = 0, e_l = 0;
+    if (lane < ncell) { s_l = start[c0 + lane]; e_l = start[c0 + lane + 1]; }
+    CellCtx cx = { 0, 0, 0 };
+    int a = 0;
+    while (a < ncell) {
+        const int S0 = __shfl(s_l, a, WAVE);
+        const bool ok = (lane >= a) && (lane < ncell) && (e_l - S0 <= PMAX);
+        const unsigned long long bal = __ballot(ok);
+        const int b = a + __popcll(bal);
+        if (b == a) {
+            // one cell larger than the tile: stable order and collisions in
+            // global memory
+            const int nC = e_l - s_l;  // valid on lane a only
+            const int nCa = __shfl(nC, a, WAVE);
+            const int c = c0 + a;
+            for (int j = lane; j < nCa; j += WAVE) {
+                const int k = perm[S0 + j];
+                int rank = 0;
+                for (int q = 0; q < nCa; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+                const int64_t d = (int64_t)S0 + rank;
+                const double x = src.px[k], y = src.py[k], z = src.pz[k];
+                dst.px[d] = x; dst.py[d] = y; dst.pz[d] = z;
+                dst.ux[d] = src.ux[k]; dst.uy[d] = src.uy[k]; dst.uz[d] = src.uz[k];
+                dst.cell[d] = c;
+                if (dst.erot) dst.erot[d] = src.erot[k];
+                if (dst.vib)
+                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + d] = src.vib[m * cap + k];
+                if (dst.type) dst.type[d] = src.type[k];
+                big_sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
+            }
+            __syncthreads();
+            if (lane == a) {
+                double ncoll = 0.0, sep = 0.0;
+                if (nCa > 65535) {
+                    cx.ovf += 1;
+                } else {
+                    if (do_collide)
+                        ntc_cell<GEN>(P, c, 0, nCa, dst.ux + S0, dst.uy + S0, dst.uz + S0, big_sub + S0,
+                                      big_list + S0, GEN && dst.erot ? dst.erot + S0 : nullptr,
+                                      GEN && dst.vib ? dst.vib + S0 : nullptr, (int)cap,
+                                      GEN ? dst.type + S0 : nullptr, sigma, rem, vol, step, dst, S0, cx, ncoll,
+                                      sep);
+                    if (do_sample)
+                        sample_cell<GEN>(P, c, 0, nCa, dst.ux + S0, dst.uy + S0, dst.uz + S0,
+                                         GEN && dst.erot ? dst.erot + S0 : nullptr,
+                                         GEN && dst.vib ? dst.vib + S0 : nullptr, (int)cap,
+                                         GEN ? dst.type + S0 : nullptr, samples, nc, ncoll, sep);
+                }
+            }
+            __syncthreads();
+            a += 1;
+            continue;
+        }
+        const int E0 = __shfl(e_l, b - 1, WAVE);
+        const int cnt = E0 - S0;
+        if (lane >= a && lane < b) {
+            segS[lane] = s_l - S0;
+            segE[lane] = e_l - S0;
+            for (int j = s_l - S0; j < e_l - S0; ++j) loc[j] = (uint8_t)lane;
+        }
+        for (int j = lane; j < cnt; j += WAVE) key[j] = perm[S0 + j];
+        __syncthreads();
+        for (int j = lane; j < cnt; j += WAVE) {
+            const int l = loc[j];
+            const int ss = segS[l], ee = segE[l];
+            const int k = key[j];
+            int rank = 0;
+            for (int q = ss; q < ee; ++q) rank += (key[q] < k) ? 1 : 0;
+            const int d = ss + rank;
+            const int c = c0 + l;
+            const int64_t gd = (int64_t)S0 + d;
+            const double x = src.px[k], y = src.py[k], z = src.pz[k];
+            dst.px[gd] = x; dst.py[gd] = y; dst.pz[gd] = z;
+            dst.cell[gd] = c;
+            sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
+            vx[d] = src.ux[k]; vy[d] = src.uy[k]; vz[d] = src.uz[k];
+            if (GEN) {
+                er[d] = src.erot ? src.erot[k] : 0.0f;
+                for (int m = 0; m < P.max_modes; ++m) vb[m * PMAX + d] = src.vib[m * cap + k];
+                ty[d] = src.type ? src.type[k] : (uint8_t)0;
+            }
+        }
+        __syncthreads();
+        if (lane >= a && lane < b) {
+            const int c = c0 + lane;
+            const int ss = s_l - S0, nC = e_l - s_l;
+            double ncoll = 0.0, sep = 0.0;
+            if (do_collide)
+                ntc_cell<GEN>(P, c, ss, nC, vx, vy, vz, sub, slist, er, vb, PMAX, ty, sigma, rem, vol, step, dst,
+                              (int64_t)S0 + ss, cx, ncoll, sep);
+            if (do_sample)
+                sample_cell<GEN>(P, c, ss, nC, vx, vy, vz, er, vb, PMAX, ty, samples, nc, ncoll, sep);
+        }
+        __syncthreads();
+        for (int j = lane; j < cnt; j += WAVE) {
+            const int64_t gd = (int64_t)S0 + j;
+            dst.ux[gd] = vx[j]; dst.uy[gd] = vy[j]; dst.uz[gd] = vz[j];
+            if (GEN) {
+                if (dst.erot) dst.erot[gd] = er[j];
+                if (dst.vib)
+                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = vb[m * PMAX + j];
+                if (dst.type) dst.type[gd] = ty[j];
+            }
+        }
+        __syncthreads();
+        a = b;
+    }
+    add_counter(S, CNT_CAND, cx.cand);
+    add_counter(S, CNT_COLL, cx.colls);
+    add_counter(S, CNT_TILE_OVF, cx.ovf);
+}
+
+void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
+                    const int32_t* perm, double* sigma, double* rem, double* samples, const double* vol,
+                    DevScalars* S, int do_collide, int do_sample, int general, int nc, uint16_t* big_list,
+                    uint8_t* big_sub, hipStream_t st) {
+    const int nt = (nc + TILE_CELLS - 1) / TILE_CELLS;
+    if (general)
+        k_collide<true, 1024><<<nt, WAVE, 0, st>>>(P, src, dst, start, perm, sigma, rem, samples, vol, S,
+                                                    do_collide, do_sample, nc, big_list, big_sub);
+    else
+        k_collide<false, 1024><<<nt, WAVE, 0, st>>>(P, src, dst, start, perm, sigma, rem, samples, vol, S,
+                                                     do_collide, do_sample, nc, big_list, big_sub);
+}
+
+// publish the sorted population; optionally advance the step counter
+__global__ void k_commit(DevScalars* S, int advance) {
+    S->n = S->n_sorted;
+    S->n_old = S->n_sorted;
+    if (advance) S->step += 1;
+}
+void launch_commit(DevScalars* S, int advance, hipStream_t st) { k_commit<<<1, 1, 0, st>>>(S, advance); }
+
+// ---------------------------------------------------------------------------
+// initial fill (dsmcMeshFill.C:68-~230)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_fill_count(const DevParams* __restrict__ Pp, const double* __restrict__ nd,
+                                                      int exact, uint32_t fill_id, const double* __restrict__ vol,
+                                                      int32_t* __restrict__ cnt, int nc, int ns) {
+    const DevParams& P = *Pp;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= nc * ns) return;
+    const int c = e / ns, s = e - c * ns;
+    const double V = vol[c];
+    if (!(V > 0.0) || !(nd[s] > 0.0)) { cnt[e] = 0; return; }
+    const double req = nd[s] * V / P.fn;
+    int ni;
+    if (exact) {
+        ni = (int)floor(req + 0.5);
+    } else {
+        PhiloxStream r;
+        r.init(P, RNG_FILL_COUNT, (uint32_t)c, fill_id, (uint32_t)s);
+        ni = (int)req;
+        if ((req - (double)ni) > r.u()) ni += 1;
+    }
+    cnt[e] = ni;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_gen(const DevParams* __restrict__ Pp, const double* __restrict__ nd,
+                                                    double T, double Trot, double Tvib, const double* __restrict__ u3,
+                                                    uint32_t fill_id, const int32_t* __restrict__ cnt,
+                                                    const int32_t* __restrict__ off, int nc, int ns, DevParticles b,
+                                                    DevScalars* __restrict__ S) {
+    const DevParams& P = *Pp;
+    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    if (e >= nc * ns) return;
+    const int ni = cnt[e];
+    if (ni == 0) return;
+    const int c = e / ns, s = e - c * ns;
+    const int idx[3] = { c % P.nx, (c / P.nx) % P.ny, c / (P.nx * P.ny) };
+    PhiloxStream r;
+    r.init(P, RNG_FILL_GEN, (uint32_t)c, fill_id, (uint32_t)s);
+    const double sc = sqrt(HS_KB * T / P.mass[s]);
+    const int64_t base = (int64_t)S->n_old + off[e];
+    for (int k = 0; k < ni; ++k) {
+        const int64_t j = base + k;
+        if (j >= P.capacity) break;
+        double pos[3] = { 0.0, 0.0, 0.0 };
+        bool ok = false;
+        for (int tries = 0; tries < HS_MAX_FILL_TRIES && !ok; ++tries) {
+            for (int a = 0; a < 3; ++a) {
+                const double ua = r.u();
+                pos[a] = P.empty[a] ? P.centre[a] : face_coord(P, a, idx[a]) + ua * P.delta[a];
+            }
+            ok = true;
+            if (P.cyl) {
+                const double rx = pos[0] - P.cyl_c[0], ry = pos[1] - P.cyl_c[1];
+                if (rx * rx + ry * ry < P.cyl_r * P.cyl_r) ok = false;
+            }
+        }
+        const double g0 = r.gauss(), g1 = r.gauss(), g2 = r.gauss();
+        b.px[j] = pos[0]; b.py[j] = pos[1]; b.pz[j] = pos[2];
+        b.ux[j] = (float)(sc * g0 + u3[0]);
+        b.uy[j] = (float)(sc * g1 + u3[1]);
+        b.uz[j] = (float)(sc * g2 + u3[2]);
+        b.cell[j] = ok ? c : -1;
+        if (b.vib)
+            for (int m = 0; m < P.max_modes; ++m) b.vib[m * P.capacity + j] = 0;
+        new_internal(P, b, j, s, Trot, Tvib, r);
+    }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_fill_cells(const DevParams* __restrict__ Pp, double smax, uint32_t fill_id,
+                                                      double* __restrict__ sigma, double* __restrict__ rem, int nc) {
+    const DevParams& P = *Pp;
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    sigma[c] = smax;
+    PhiloxStream r;
+    r.init(P, RNG_REMAINDER, (uint32_t)c, fill_id, 0);
+    rem[c] = r.u();
+}
+
+void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
+                       int32_t* cnt, int nc, int ns, hipStream_t st) {
+    const int n = nc * ns;
+    k_fill_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, exact, fill_id, vol, cnt, nc, ns);
+}
+
+void launch_fill_gen(const DevParams* P, const double* nd, double T, double Trot, double Tvib, const double* u3,
+                     uint32_t fill_id, const int32_t* cnt, const int32_t* off, int nc, int ns, DevParticles b,
+                     DevScalars* S, hipStream_t st) {
+    const int n = nc * ns;
+    k_fill_gen<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, T, Trot, Tvib, u3, fill_id, cnt, off, nc, ns, b, S);
+}
+
+void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double* sigma, double* rem, int nc,
+                       hipStream_t st) {
+    k_fill_cells<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, smax, fill_id, sigma, rem, nc);
+}
+
+// ---------------------------------------------------------------------------
+// test hooks
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_binary_collide(const DevParams* __restrict__ Pp, int64_t n,
+                                                          const int32_t* tp, const int32_t* tq, double* up, double* uq,
+                                                          double* erp, double* erq, int32_t* vp, int32_t* vq,
+                                                          const double* uni, int32_t nu, int32_t* used) {
+    const DevParams& P = *Pp;
+    const int64_t k = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (k >= n) return;
+    InjStream r;
+    r.src = uni + k * nu;
+    r.n = nu;
+    r.used = 0;
+    double a[3] = { up[3 * k], up[3 * k + 1], up[3 * k + 2] };
+    double b[3] = { uq[3 * k], uq[3 * k + 1], uq[3 * k + 2] };
+    double ep = erp ? erp[k] : 0.0, eq = erq ? erq[k] : 0.0;
+    int va[HS_MAX_VIB_MODES] = { 0, 0, 0 }, vbq[HS_MAX_VIB_MODES] = { 0, 0, 0 };
+    if (vp)
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) va[m] = vp[k * HS_MAX_VIB_MODES + m];
+    if (vq)
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) vbq[m] = vq[k * HS_MAX_VIB_MODES + m];
+    binary_collide(P, tp[k], tq[k], a, b, &ep, &eq, va, vbq, r);
+    for (int c = 0; c < 3; ++c) { up[3 * k + c] = a[c]; uq[3 * k + c] = b[c]; }
+    if (erp) erp[k] = ep;
+    if (erq) erq[k] = eq;
+    if (vp)
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) vp[k * HS_MAX_VIB_MODES + m] = va[m];
+    if (vq)
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) vq[k * HS_MAX_VIB_MODES + m] = vbq[m];
+    used[k] = r.used;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_sigma_tcr(const DevParams* __restrict__ Pp, int64_t n, const int32_t* tp,
+                                                     const int32_t* tq, const double* up, const double* uq,
+                                                     double* out) {
+    const DevParams& P = *Pp;
+    const int64_t k = blockIdx.x * (int64_t)BLOCK + threadIdx.x;
+    if (k >= n) return;
+    const double a[3] = { up[3 * k], up[3 * k + 1], up[3 * k + 2] };
+    const double b[3] = { uq[3 * k], uq[3 * k + 1], uq[3 * k + 2] };
+    out[k] = sigma_tcr(P, tp[k], tq[k], a, b);
+}
+
+void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, double* up,
+                           double* uq, double* erp, double* erq, int32_t* vp, int32_t* vq, const double* uni,
+                           int32_t nu, int32_t* used, hipStream_t st) {
+    if (n <= 0) return;
+    k_binary_collide<<<(int)((n + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(P, n, tp, tq, up, uq, erp, erq, vp, vq, uni,
+                                                                      nu, used);
+}
+
+void launch_sigma_tcr(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, const double* up,
+                      const double* uq, double* out, hipStream_t st) {
+    if (n <= 0) return;
+    k_sigma_tcr<<<(int)((n + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(P, n, tp, tq, up, uq, out);
+}

@@ -1,0 +1,261 @@
+/*
+ * hystrath_dsmc.h -- C ABI of the MI355X particle engine for dsmcFoam+.
+ *
+ * This is the drop-in boundary for dsmcCloud's per-timestep particle path
+ * (src/lagrangian/dsmc/clouds/dsmcCloud.C:819-926).  Every entry point takes
+ * plain pointers and sizes; no C++ or torch types cross it.  One engine owns
+ * one HIP stream and all of its device memory; calls on one engine must not
+ * overlap, independent engines are thread-safe.
+ *
+ * Every function returns 0 on success and a negative HS_E* code on failure;
+ * hs_dsmc_last_error() then gives a message.  Host buffers passed in are only
+ * borrowed for the duration of the call.
+ *
+ * Mesh: a uniform Cartesian hex block (what a single-block blockMeshDict
+ * "hex (...) (nx ny nz) simpleGrading (1 1 1)" produces), cells numbered
+ * i + nx*(j + ny*k) as blockMesh numbers them, six outer patches and an
+ * optional embedded cylinder wall (axis z).  Cell volumes are an input so
+ * cut cells around the cylinder carry their fluid volume.
+ */
+#ifndef HYSTRATH_DSMC_H
+#define HYSTRATH_DSMC_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HS_MAX_SPECIES   8
+#define HS_MAX_VIB_MODES 3
+
+/* Boltzmann constant used everywhere (CODATA 2014, the value of the
+ * OpenFOAM-v1706 era physicoChemical::k). */
+#define HS_KB 1.38064852e-23
+
+/* ---- error codes -------------------------------------------------------- */
+#define HS_OK             0
+#define HS_E_INVALID    (-1)   /* invalid argument / configuration          */
+#define HS_E_CAPACITY   (-2)   /* particle capacity exceeded                */
+#define HS_E_HIP        (-3)   /* HIP runtime error                         */
+#define HS_E_OVERFLOW   (-4)   /* a device-side bound tripped (see flags)    */
+#define HS_E_UNSUPPORTED (-5)  /* model/option not implemented              */
+
+/* ---- runtime-selection enums (dictionary names in comments) ------------- */
+/* BinaryCollisionModel (BinaryCollisionModel.C:59-90) */
+enum hs_collision_model {
+    HS_COLL_NONE   = 0,  /* NoBinaryCollision                   */
+    HS_COLL_VHS    = 1,  /* VariableHardSphere                  */
+    HS_COLL_VSS    = 2,  /* VariableSoftSphere                  */
+    HS_COLL_LB_VHS = 3,  /* LarsenBorgnakkeVariableHardSphere   */
+    HS_COLL_LB_VSS = 4   /* LarsenBorgnakkeVariableSoftSphere   */
+};
+
+/* dsmcPatchBoundaries / dsmcCyclicBoundaries / empty patches */
+enum hs_patch_model {
+    HS_PATCH_CYCLIC   = 0,  /* cyclic patch pair (particleTemplates.C:1525)  */
+    HS_PATCH_SPECULAR = 1,  /* dsmcSpecularWallPatch                          */
+    HS_PATCH_DIFFUSE  = 2,  /* dsmcDiffuseWallPatch                           */
+    HS_PATCH_DELETION = 3,  /* dsmcDeletionPatch                              */
+    HS_PATCH_EMPTY    = 4   /* empty (reduced-D direction, no tracking)       */
+};
+
+/* config.flags */
+#define HS_FLAG_SAMPLE        0x1  /* dsmcVolFields sampling every step      */
+#define HS_FLAG_HEATFLUX      0x2  /* measureHeatFluxShearStress sums         */
+#define HS_FLAG_COLL_SEP      0x4  /* collision separation (meanCollisionSeparation) */
+#define HS_FLAG_TIMING        0x8  /* per-kernel HIP event timing             */
+
+/* sides of the block, patch[] order */
+enum hs_side { HS_XMIN = 0, HS_XMAX = 1, HS_YMIN = 2, HS_YMAX = 3, HS_ZMIN = 4, HS_ZMAX = 5 };
+
+/* dsmcParcel::constantProperties (dsmcParcelI.H:37-200) */
+typedef struct hs_species {
+    double mass;                       /* mass                                 */
+    double diameter;                   /* diameter                             */
+    double omega;                      /* omega (viscosity index)              */
+    double alpha;                      /* alpha (VSS), default 1               */
+    double rot_dof;                    /* rotationalDegreesOfFreedom           */
+    int32_t n_vib_modes;               /* nVibrationalModes (<= HS_MAX_VIB_MODES) */
+    int32_t charge;                    /* charge (-1 electron, 0, 1)           */
+    double theta_v[HS_MAX_VIB_MODES];  /* characteristicVibrationalTemperature */
+    double zref[HS_MAX_VIB_MODES];     /* Zref                                 */
+    double tref_zv[HS_MAX_VIB_MODES];  /* referenceTempForZref                 */
+    double theta_d;                    /* dissociationTemperature              */
+} hs_species;
+
+/* one patch: model + its Properties sub-dictionary + optional
+ * dsmcFreeStreamInflowPatch general boundary on the same patch */
+typedef struct hs_patch {
+    int32_t model;                     /* hs_patch_model                       */
+    int32_t inflow;                    /* 1: dsmcFreeStreamInflowPatch here     */
+    double temperature;                /* dsmcDiffuseWallPatchProperties/temperature */
+    double velocity[3];                /* dsmcDiffuseWallPatchProperties/velocity    */
+} hs_patch;
+
+typedef struct hs_config {
+    /* ---- mesh ---- */
+    int32_t nx, ny, nz;
+    double origin[3];                  /* lower corner                          */
+    double delta[3];                   /* cell size per axis                    */
+    const double* cell_volume;         /* nx*ny*nz fluid volumes (host, borrowed at create) */
+    hs_patch patch[6];                 /* hs_side order                         */
+    int32_t cyl_enabled;               /* embedded cylinder wall, axis z        */
+    double cyl_center[2];
+    double cyl_radius;
+    hs_patch cyl_patch;                /* SPECULAR or DIFFUSE                   */
+
+    /* ---- species table ---- */
+    int32_t n_species;
+    hs_species species[HS_MAX_SPECIES];
+
+    /* ---- BinaryCollisionModel + Coeffs ---- */
+    int32_t collision_model;           /* hs_collision_model                    */
+    double tref;                       /* <Model>Coeffs/Tref (default 273)      */
+    double z_rot;                      /* rotationalRelaxationCollisionNumber (5) */
+    double z_vib_fixed;                /* vibrationalRelaxationCollisionNumber (0 = Bird 2010 Zv) */
+    double z_el;                       /* electronicRelaxationCollisionNumber (500) */
+    int32_t inv_zv_form;               /* inverseZvFormulation: 0 pre-2008, 2 default (1 unsupported) */
+
+    /* ---- dsmcFreeStreamInflowPatch properties (shared by inflow patches) ---- */
+    double inflow_n[HS_MAX_SPECIES];   /* numberDensities per species           */
+    double inflow_t;                   /* translationalTemperature              */
+    double inflow_trot;                /* rotationalTemperature                 */
+    double inflow_tvib;                /* vibrationalTemperature                */
+    double inflow_u[3];                /* velocity                              */
+
+    /* ---- numerics ---- */
+    double fn;                         /* nEquivalentParticles                  */
+    double dt;                         /* deltaT                                */
+    uint64_t seed;                     /* seedNumber                            */
+    int64_t capacity;                  /* max simulator particles               */
+    int32_t flags;                     /* HS_FLAG_*                             */
+} hs_config;
+
+/* Per-particle state, structure of arrays.  erot is used when any species has
+ * rotational DoF, vib (n_vib_modes planes of length n) when any species has
+ * vibrational modes, type when n_species > 1; pass NULL otherwise. */
+typedef struct hs_particles {
+    double* px; double* py; double* pz;    /* FP64 positions                    */
+    float*  ux; float*  uy; float*  uz;    /* FP32 velocities                   */
+    int32_t* cell;                         /* owning cell                       */
+    float*  erot;                          /* rotational energy (J)             */
+    uint16_t* vib;                         /* vibLevel, plane-major [mode][n]   */
+    uint8_t* type;                         /* typeId                            */
+} hs_particles;
+
+/* Per-cell sampling accumulators (dsmcVolFields.C:1115-1239): plane-major
+ * FP64 arrays [slot][cell].  The base slots are: */
+enum hs_sample_slot {
+    HS_S_N = 0,       /* dsmcNCum: simulator particles                      */
+    HS_S_M,           /* dsmcMCum: sum m                                    */
+    HS_S_MUX, HS_S_MUY, HS_S_MUZ,  /* sum m U                               */
+    HS_S_MCC,         /* sum m |U|^2 (linearKE without 1/2)                 */
+    HS_S_EROT,        /* sum ERot                                           */
+    HS_S_ZROT,        /* sum rotational DoF                                 */
+    HS_S_NCOLL,       /* nColls (cellMeasurements)                          */
+    HS_S_COLLSEP,     /* collision separation                               */
+    HS_S_BASE_COUNT
+};
+/* then n_species slots of per-species N, then n_species*HS_MAX_VIB_MODES
+ * slots of per-species per-mode vibrational energy, then (HS_FLAG_HEATFLUX)
+ * 13 slots: Muu Muv Muw Mvv Mvw Mww Mccu Mccv Mccw Eu Ev Ew E. */
+int32_t hs_dsmc_sample_slots(const hs_config* cfg);
+
+/* ---- engine lifetime ---------------------------------------------------- */
+typedef struct hs_dsmc_engine hs_dsmc_engine;
+
+/* Replaces dsmcCloud::dsmcCloud(Time&, word, dynamicFvMesh&, bool)
+ * (dsmcCloud.C:586-691): species table, models, sigmaTcRMax, remainder. */
+int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out);
+void hs_dsmc_destroy(hs_dsmc_engine* eng);
+const char* hs_dsmc_last_error(const hs_dsmc_engine* eng);
+/* message of the last failed hs_dsmc_create (thread-local) */
+const char* hs_dsmc_create_error(void);
+int hs_dsmc_device_count(int* n);
+
+/* ---- state transfer ----------------------------------------------------- */
+/* Replaces dsmcParcel::readFields (dsmcParcelIO.C:133-337) for the SoA. */
+int hs_dsmc_upload_particles(hs_dsmc_engine* eng, const hs_particles* p, int64_t n);
+/* Replaces dsmcParcel::writeFields (dsmcParcelIO.C:338-519).  n_out gets the
+ * particle count; fails with HS_E_CAPACITY if it exceeds cap. */
+int hs_dsmc_download_particles(hs_dsmc_engine* eng, hs_particles* p, int64_t cap, int64_t* n_out);
+int hs_dsmc_particle_count(hs_dsmc_engine* eng, int64_t* n_out);
+/* sigmaTcRMax_ and collisionSelectionRemainder_ (dsmcCloud.H:144-147);
+ * either pointer may be NULL. */
+int hs_dsmc_set_cell_state(hs_dsmc_engine* eng, const double* sigma_tcr_max, const double* remainder);
+int hs_dsmc_get_cell_state(hs_dsmc_engine* eng, double* sigma_tcr_max, double* remainder);
+/* freestream inflow accumulators accumulatedParcelsToInsert_
+ * (dsmcFreeStreamInflowPatch.C:132-140), [species][inflow face] */
+int hs_dsmc_inflow_face_count(hs_dsmc_engine* eng, int64_t* n_faces);
+int hs_dsmc_set_inflow_accumulators(hs_dsmc_engine* eng, const double* acc);
+int hs_dsmc_get_inflow_accumulators(hs_dsmc_engine* eng, double* acc);
+/* time-step index used as the random-stream counter */
+int hs_dsmc_set_step(hs_dsmc_engine* eng, uint32_t step);
+int hs_dsmc_get_step(hs_dsmc_engine* eng, uint32_t* step);
+
+/* ---- initial state ------------------------------------------------------ */
+/* Replaces dsmcMeshFill::setInitialConfiguration (dsmcMeshFill.C:68-~230):
+ * Poisson-rounded uniform fill of every cell with a Maxwellian at
+ * (t, trot, tvib, u) and number densities n[species]; when exact_per_cell
+ * is non-zero each cell gets exactly round(n*V/Fn) particles per species.
+ * Appends to the current population.  Also resets sigmaTcRMax to
+ * sigmaT*sqrt(2kT/m) of the most abundant species and draws the remainder
+ * (dsmcCloud.C:105-116). */
+int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot,
+                 double tvib, const double u[3], int32_t exact_per_cell,
+                 uint32_t fill_id);
+
+/* ---- the timestep ------------------------------------------------------- */
+/* Replaces dsmcCloud::evolve (dsmcCloud.C:819-926) nsteps times: freestream
+ * injection (dsmcFreeStreamInflowPatch::controlParcelsBeforeMove), move
+ * (Cloud<dsmcParcel>::move, Cloud.C:202-466), buildCellOccupancy
+ * (dsmcCloud.C:63-74), noTimeCounter::collide (noTimeCounter.C:81-339) and
+ * dsmcVolFields::calculateField sums (dsmcVolFields.C:1071-1290). */
+int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps);
+/* Stage entry points (same kernels, for tests and custom loops). */
+int hs_dsmc_inject(hs_dsmc_engine* eng);                 /* controlParcelsBeforeMove */
+int hs_dsmc_move(hs_dsmc_engine* eng);                   /* Cloud<dsmcParcel>::move  */
+int hs_dsmc_sort(hs_dsmc_engine* eng);                   /* buildCellOccupancy       */
+int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample); /* noTimeCounter::collide (+ sampling) */
+int hs_dsmc_advance_step(hs_dsmc_engine* eng);           /* step counter += 1 */
+int hs_dsmc_synchronize(hs_dsmc_engine* eng);
+
+/* ---- sampling ----------------------------------------------------------- */
+/* dsmcVolFields cumulative sums, plane-major [slot][cell]; nsteps_out gets
+ * the number of sampled steps (nTimeSteps_). */
+int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
+/* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290) */
+int hs_dsmc_reset_samples(hs_dsmc_engine* eng);
+/* counters of the last evolve: [0] candidates, [1] collisions, [2] deleted,
+ * [3] injected, [4] wall hits, [5] tracking-iteration overflows,
+ * [6] cell-tile overflows */
+int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n);
+int hs_dsmc_reset_counters(hs_dsmc_engine* eng);
+/* per-kernel device time (ms, summed over launches since last reset) and
+ * launch counts, in the order inject, move, scan, scatter, collide;
+ * requires HS_FLAG_TIMING */
+int hs_dsmc_kernel_times(hs_dsmc_engine* eng, double* ms, int64_t* launches, int32_t n);
+
+/* ---- binary collision test hook ----------------------------------------- */
+/* BinaryCollisionModel::collide (VariableHardSphere.C:125-228,
+ * VariableSoftSphere.C:189-247, LarsenBorgnakkeVariableHardSphere.C:110-279)
+ * for n independent pairs with an injected uniform stream: pair k reads
+ * uniforms[k*n_uniform .. k*n_uniform + n_uniform) in the reference's draw
+ * order and computes in FP64.  Velocities are [3*k+c], FP64 in/out.
+ * erot/vib may be NULL for monatomic species.  used_out[k] = draws consumed
+ * (an error if a pair needs more than n_uniform). */
+int hs_dsmc_binary_collide(hs_dsmc_engine* eng, int64_t n, const int32_t* type_p,
+                           const int32_t* type_q, double* up, double* uq,
+                           double* erot_p, double* erot_q, int32_t* vib_p,
+                           int32_t* vib_q, const double* uniforms,
+                           int32_t n_uniform, int32_t* used_out);
+/* sigmaTcR (VariableHardSphere.C:80-121) for n pairs, FP64. */
+int hs_dsmc_sigma_tcr(hs_dsmc_engine* eng, int64_t n, const int32_t* type_p,
+                      const int32_t* type_q, const double* up, const double* uq,
+                      double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HYSTRATH_DSMC_H */

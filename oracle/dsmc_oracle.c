@@ -1,0 +1,1138 @@
+/*
+ * dsmc_oracle.c -- CPU restatement of dsmcFoam+'s per-timestep particle
+ * path.  TEST INFRASTRUCTURE ONLY: the parity checker for the HIP engine
+ * and the CPU baseline of the benchmark.  The product never links this.
+ *
+ * Reference files followed (paths relative to the hyStrath tree):
+ *   NTC selection      src/lagrangian/dsmc/collisionPartnerSelection/derived/noTimeCounter/noTimeCounter.C:81-339
+ *   randomLabel        src/lagrangian/dsmc/clouds/dsmcCloud.C:1015-1040
+ *   VHS                src/lagrangian/dsmc/collisions/derived/VariableHardSphere/VariableHardSphere.C:80-228
+ *   VSS                .../VariableSoftSphere/VariableSoftSphere.C:78-247
+ *   Larsen-Borgnakke   .../LarsenBorgnakkeVariableHardSphere/LarsenBorgnakkeVariableHardSphere.C:110-279
+ *   LB samplers        src/lagrangian/dsmc/clouds/dsmcCloud.C:1181-1244,1327-1531,1534-1656
+ *   move               src/lagrangian/dsmc/parcels/dsmcParcel.C:38-148 and the tracker
+ *                      src/lagrangian/basic/particle/particleTemplates.C:727-1241,1525-1571
+ *   walls              .../dsmcDiffuseWallPatch/dsmcDiffuseWallPatch.C:67-139,
+ *                      .../basic/dsmcPatchBoundary/dsmcPatchBoundary.C:216-261,
+ *                      .../dsmcSpecularWallPatch/dsmcSpecularWallPatch.C:47-61,
+ *                      .../dsmcDeletionPatch/dsmcDeletionPatch.C:140-150
+ *   inflow             .../dsmcFreeStreamInflowPatch/dsmcFreeStreamInflowPatch.C:91-364
+ *   cell occupancy     src/lagrangian/dsmc/clouds/dsmcCloud.C:63-74
+ *   sampling sums      src/lagrangian/dsmc/macroscopicProperties/derived/combined/dsmcVolFields/dsmcVolFields.C:1115-1239
+ *   mesh fill          src/lagrangian/dsmc/initialiseDsmcParcels/derived/dsmcMeshFill/dsmcMeshFill.C:68-~230
+ *
+ * Deliberate, documented differences from the reference (all of them change
+ * which random numbers are used, never the distributions):
+ *   - Random numbers come from Philox4x32-10 streams keyed by (seed, purpose,
+ *     entity, step) instead of OpenFOAM's single sequential Random.
+ *   - Gaussians are Box-Muller pairs (second value cached) instead of the
+ *     polar method; arguments of log are 1-u so they are never 0.
+ *   - A freestream particle's face position is uniform on the rectangular
+ *     face from two draws instead of triangle selection + randomPoint.
+ * The mesh is a uniform Cartesian block; the tet-walk of the reference
+ * reduces to axis-aligned face crossings on it, plus an analytic cylinder.
+ */
+#include "dsmc_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define OR_VSMALL 1.0e-300
+#define OR_SMALL 1.0e-15
+#define OR_ROOTVSMALL 1.0e-150
+#define OR_PI 3.14159265358979323846
+#define OR_TWO_PI 6.28318530717958647692
+#define OR_MAX_TRACK 1000
+#define OR_MAX_REDRAW 100000
+#define OR_MAX_CAND (1 << 22)
+#define OR_MAX_FILL_TRIES 100000
+
+/* ------------------------------------------------------------------------ */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11)                         */
+/* ------------------------------------------------------------------------ */
+void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4])
+{
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        const uint64_t p0 = (uint64_t)0xD2511F53u * (uint64_t)c0;
+        const uint64_t p1 = (uint64_t)0xCD9E8D57u * (uint64_t)c2;
+        const uint32_t hi0 = (uint32_t)(p0 >> 32), lo0 = (uint32_t)p0;
+        const uint32_t hi1 = (uint32_t)(p1 >> 32), lo1 = (uint32_t)p1;
+        c0 = hi1 ^ c1 ^ k0;
+        c1 = lo1;
+        c2 = hi0 ^ c3 ^ k1;
+        c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* 53 random bits -> [0,1) exactly */
+double or_uniform_from(uint32_t hi, uint32_t lo)
+{
+    const uint64_t x = (((uint64_t)hi << 32) | (uint64_t)lo) >> 11;
+    return (double)x * (1.0 / 9007199254740992.0);
+}
+
+typedef struct {
+    uint32_t key[2];
+    uint32_t ctr[3];
+    uint32_t blk;
+    int have_u, have_g;
+    double spare_u, spare_g;
+    const double* inj;       /* injected uniforms (test hook) or NULL */
+    int inj_n, inj_used;
+} or_rng;
+
+static void rng_init(or_rng* r, uint64_t seed, int purpose, uint32_t id, uint32_t step, uint32_t sub)
+{
+    r->key[0] = (uint32_t)seed;
+    r->key[1] = (uint32_t)(seed >> 32);
+    r->ctr[0] = id;
+    r->ctr[1] = step;
+    r->ctr[2] = ((uint32_t)purpose << 24) | (sub & 0xFFFFFFu);
+    r->blk = 0;
+    r->have_u = 0;
+    r->have_g = 0;
+    r->spare_u = 0.0;
+    r->spare_g = 0.0;
+    r->inj = NULL;
+    r->inj_n = 0;
+    r->inj_used = 0;
+}
+
+static double rng_u(or_rng* r)
+{
+    if (r->inj) {
+        const int k = r->inj_used++;
+        return k < r->inj_n ? r->inj[k] : 0.5;
+    }
+    if (r->have_u) { r->have_u = 0; return r->spare_u; }
+    uint32_t c[4] = { r->ctr[0], r->ctr[1], r->ctr[2], r->blk };
+    uint32_t o[4];
+    or_philox4x32(c, r->key, o);
+    r->blk += 1;
+    r->spare_u = or_uniform_from(o[2], o[3]);
+    r->have_u = 1;
+    return or_uniform_from(o[0], o[1]);
+}
+
+/* ------------------------------------------------------------------------ */
+/* Deterministic FP64 math from IEEE +,-,*,/ only                           */
+/* ------------------------------------------------------------------------ */
+#define OR_LN2_HI 6.93147180369123816490e-01 /* 0x3fe62e42fee00000 */
+#define OR_LN2_LO 1.90821492927058770002e-10 /* 0x3dea39ef35793c76 */
+#define OR_INV_LN2 1.44269504088896338700e+00
+
+static double bits_to_d(uint64_t b) { double d; memcpy(&d, &b, 8); return d; }
+static uint64_t d_to_bits(double d) { uint64_t b; memcpy(&b, &d, 8); return b; }
+
+/* log(x): x = m*2^k, m in [sqrt(1/2), sqrt(2)); log m = 2 atanh(s),
+ * s = (m-1)/(m+1), series to s^23 (|s| <= 0.1716, truncation < 1e-19). */
+double or_log(double x)
+{
+    if (x != x) return x;
+    if (x < 0.0) return NAN;
+    if (x == 0.0) return -INFINITY;
+    if (x == INFINITY) return x;
+    int k = 0;
+    uint64_t b = d_to_bits(x);
+    if (((b >> 52) & 0x7ff) == 0) { x = x * 18014398509481984.0; b = d_to_bits(x); k = -54; }
+    k += (int)((b >> 52) & 0x7ff) - 1023;
+    double m = bits_to_d((b & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull);
+    if (m > 1.4142135623730951) { m = m * 0.5; k += 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double t = 1.0 / 23.0;
+    t = 1.0 / 21.0 + s2 * t;
+    t = 1.0 / 19.0 + s2 * t;
+    t = 1.0 / 17.0 + s2 * t;
+    t = 1.0 / 15.0 + s2 * t;
+    t = 1.0 / 13.0 + s2 * t;
+    t = 1.0 / 11.0 + s2 * t;
+    t = 1.0 / 9.0 + s2 * t;
+    t = 1.0 / 7.0 + s2 * t;
+    t = 1.0 / 5.0 + s2 * t;
+    t = 1.0 / 3.0 + s2 * t;
+    const double lm = (2.0 * s) + (2.0 * s) * (s2 * t);
+    const double dk = (double)k;
+    return dk * OR_LN2_HI + (dk * OR_LN2_LO + lm);
+}
+
+/* exp(x) = 2^k e^r, |r| <= ln2/2, Taylor to r^14 (truncation < 1e-18). */
+double or_exp(double x)
+{
+    if (x != x) return x;
+    if (x > 709.782712893384) return INFINITY;
+    if (x < -745.1332191019412) return 0.0;
+    const double kd = floor(x * OR_INV_LN2 + 0.5);
+    const double r = (x - kd * OR_LN2_HI) - kd * OR_LN2_LO;
+    double p = 1.0 / 87178291200.0;
+    p = 1.0 / 6227020800.0 + r * p;
+    p = 1.0 / 479001600.0 + r * p;
+    p = 1.0 / 39916800.0 + r * p;
+    p = 1.0 / 3628800.0 + r * p;
+    p = 1.0 / 362880.0 + r * p;
+    p = 1.0 / 40320.0 + r * p;
+    p = 1.0 / 5040.0 + r * p;
+    p = 1.0 / 720.0 + r * p;
+    p = 1.0 / 120.0 + r * p;
+    p = 1.0 / 24.0 + r * p;
+    p = 1.0 / 6.0 + r * p;
+    p = 0.5 + r * p;
+    p = 1.0 + r * p;
+    p = 1.0 + r * p;
+    int k = (int)kd;
+    if (k < -1000) { p = p * 0x1p-511; k += 511; }
+    if (k > 1000) { p = p * 0x1p511; k -= 511; }
+    return p * bits_to_d((uint64_t)(k + 1023) << 52);
+}
+
+double or_pow(double x, double y)
+{
+    if (y == 0.0) return 1.0;
+    return or_exp(y * or_log(x));
+}
+
+/* sin and cos of 2*pi*u for u in [0,1): quadrant from round(4u), remainder
+ * d = u - q/4 is exact, r = 2*pi*d in [-pi/4, pi/4]; Taylor to r^17 / r^18. */
+void or_sincos2pi(double u, double* s_out, double* c_out)
+{
+    const double q = floor(u * 4.0 + 0.5);
+    const double d = u - q * 0.25;
+    const double r = d * OR_TWO_PI;
+    const double r2 = r * r;
+    double ps = 1.0 / 355687428096000.0;
+    ps = -1.0 / 1307674368000.0 + r2 * ps;
+    ps = 1.0 / 6227020800.0 + r2 * ps;
+    ps = -1.0 / 39916800.0 + r2 * ps;
+    ps = 1.0 / 362880.0 + r2 * ps;
+    ps = -1.0 / 5040.0 + r2 * ps;
+    ps = 1.0 / 120.0 + r2 * ps;
+    ps = -1.0 / 6.0 + r2 * ps;
+    const double sr = r + (r * r2) * ps;
+    double pc = -1.0 / 6402373705728000.0;
+    pc = 1.0 / 20922789888000.0 + r2 * pc;
+    pc = -1.0 / 87178291200.0 + r2 * pc;
+    pc = 1.0 / 479001600.0 + r2 * pc;
+    pc = -1.0 / 3628800.0 + r2 * pc;
+    pc = 1.0 / 40320.0 + r2 * pc;
+    pc = -1.0 / 720.0 + r2 * pc;
+    pc = 1.0 / 24.0 + r2 * pc;
+    pc = -0.5 + r2 * pc;
+    const double cr = 1.0 + r2 * pc;
+    switch (((int)q) & 3) {
+    case 0: *s_out = sr; *c_out = cr; break;
+    case 1: *s_out = cr; *c_out = -sr; break;
+    case 2: *s_out = -sr; *c_out = -cr; break;
+    default: *s_out = -cr; *c_out = sr; break;
+    }
+}
+
+/* one standard normal; Box-Muller pair, the second value cached */
+static double rng_gauss(or_rng* r)
+{
+    if (r->have_g) { r->have_g = 0; return r->spare_g; }
+    const double u1 = rng_u(r);
+    const double u2 = rng_u(r);
+    const double rad = sqrt(-2.0 * or_log(1.0 - u1));
+    double sn, cs;
+    or_sincos2pi(u2, &sn, &cs);
+    r->spare_g = rad * sn;
+    r->have_g = 1;
+    return rad * cs;
+}
+
+void or_stream_draws(uint64_t seed, int purpose, uint32_t id, uint32_t step,
+                     uint32_t sub, int n, double* out)
+{
+    or_rng r;
+    rng_init(&r, seed, purpose, id, step, sub);
+    for (int i = 0; i < n; ++i) out[i] = rng_u(&r);
+}
+
+void or_stream_gauss(uint64_t seed, int purpose, uint32_t id, uint32_t step,
+                     uint32_t sub, int n, double* out)
+{
+    or_rng r;
+    rng_init(&r, seed, purpose, id, step, sub);
+    for (int i = 0; i < n; ++i) out[i] = rng_gauss(&r);
+}
+
+/* dsmcCloud::randomLabel (dsmcCloud.C:1015-1040) */
+static int rng_label(or_rng* r, int a, int b)
+{
+    if (a == b) return a;
+    const int start = a < b ? a : b;
+    const int end = a < b ? b : a;
+    for (;;) {
+        const int v = start + (int)(rng_u(r) * (double)(end - start + 1));
+        if (v != end + 1) return v;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* derived constants                                                        */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    int ns;
+    double sig_coef[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double ref_term[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double omega_pq[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    double alpha_pq[HS_MAX_SPECIES][HS_MAX_SPECIES];
+    /* Bird 2010 Zv: [species][partner][mode] Zref*(thetaD/TrefZv)^-omegaPQ */
+    double zv_b[HS_MAX_SPECIES][HS_MAX_SPECIES][HS_MAX_VIB_MODES];
+    double zv_pow2[HS_MAX_SPECIES][HS_MAX_VIB_MODES];
+    int empty[3];
+    double centre[3];
+    int any_rot, any_vib, multi;
+    int max_modes;
+} or_derived;
+
+static void derive(const hs_config* cfg, or_derived* d)
+{
+    memset(d, 0, sizeof(*d));
+    d->ns = cfg->n_species;
+    for (int p = 0; p < d->ns; ++p) {
+        const hs_species* sp = &cfg->species[p];
+        if (sp->rot_dof > 0) d->any_rot = 1;
+        if (sp->n_vib_modes > 0) d->any_vib = 1;
+        if (sp->n_vib_modes > d->max_modes) d->max_modes = sp->n_vib_modes;
+        for (int m = 0; m < sp->n_vib_modes; ++m)
+            d->zv_pow2[p][m] = pow(sp->theta_d / sp->tref_zv[m], 1.0 / 3.0) - 1.0;
+        for (int q = 0; q < d->ns; ++q) {
+            const hs_species* sq = &cfg->species[q];
+            const double dpq = 0.5 * (sp->diameter + sq->diameter);
+            const double om = 0.5 * (sp->omega + sq->omega);
+            const double mr = sp->mass * sq->mass / (sp->mass + sq->mass);
+            d->sig_coef[p][q] = OR_PI * dpq * dpq / exp(lgamma(2.5 - om));
+            d->ref_term[p][q] = 2.0 * HS_KB * cfg->tref / mr;
+            d->omega_pq[p][q] = om;
+            d->alpha_pq[p][q] = 0.5 * (sp->alpha + sq->alpha);
+            for (int m = 0; m < sp->n_vib_modes; ++m)
+                d->zv_b[p][q][m] = sp->zref[m] * pow(sp->theta_d / sp->tref_zv[m], -om);
+        }
+    }
+    d->multi = d->ns > 1;
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    for (int a = 0; a < 3; ++a) {
+        d->empty[a] = cfg->patch[2 * a].model == HS_PATCH_EMPTY;
+        d->centre[a] = cfg->origin[a] + 0.5 * ((double)nn[a] * cfg->delta[a]);
+    }
+}
+
+int32_t or_sample_slots(const hs_config* cfg)
+{
+    int32_t n = HS_S_BASE_COUNT + cfg->n_species + cfg->n_species * HS_MAX_VIB_MODES;
+    if (cfg->flags & HS_FLAG_HEATFLUX) n += 13;
+    return n;
+}
+
+/* ------------------------------------------------------------------------ */
+/* inflow faces: sides in hs_side order, faces in blockMesh order on each   */
+/* ------------------------------------------------------------------------ */
+typedef struct { int32_t cell; int8_t axis, dir; int32_t a1i, a2i; } or_face;
+
+static int64_t list_inflow_faces(const hs_config* cfg, or_face* out)
+{
+    int64_t nf = 0;
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    for (int side = 0; side < 6; ++side) {
+        if (!cfg->patch[side].inflow) continue;
+        const int axis = side / 2, dir = (side & 1) ? 1 : -1;
+        const int a1 = axis == 0 ? 1 : 0;
+        const int a2 = axis == 2 ? 1 : 2;
+        for (int j2 = 0; j2 < nn[a2]; ++j2)
+            for (int j1 = 0; j1 < nn[a1]; ++j1) {
+                int idx[3];
+                idx[axis] = dir > 0 ? nn[axis] - 1 : 0;
+                idx[a1] = j1;
+                idx[a2] = j2;
+                const int32_t c = idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2]);
+                if (!(cfg->cell_volume[c] > 0.0)) continue;
+                if (out) {
+                    out[nf].cell = c; out[nf].axis = (int8_t)axis; out[nf].dir = (int8_t)dir;
+                    out[nf].a1i = j1; out[nf].a2i = j2;
+                }
+                ++nf;
+            }
+    }
+    return nf;
+}
+
+int64_t or_inflow_face_count(const hs_config* cfg) { return list_inflow_faces(cfg, NULL); }
+
+/* ------------------------------------------------------------------------ */
+/* equipartition samplers (dsmcCloud.C:1181-1244)                           */
+/* ------------------------------------------------------------------------ */
+static double equip_erot(or_rng* r, double T, double dof)
+{
+    if (dof < OR_SMALL) return 0.0;
+    if (dof < 2.0 + OR_SMALL && dof > 2.0 - OR_SMALL)
+        return -or_log(1.0 - rng_u(r)) * HS_KB * T;
+    const double a = 0.5 * dof - 1.0;
+    double ratio, P;
+    int guard = 0;
+    do {
+        ratio = 10.0 * rng_u(r);
+        P = or_pow(ratio / a, a) * or_exp(a - ratio);
+    } while (P < rng_u(r) && ++guard < OR_MAX_REDRAW);
+    return ratio * HS_KB * T;
+}
+
+static void equip_vib(or_rng* r, double T, const hs_species* sp, uint16_t* lv, int64_t stride)
+{
+    for (int m = 0; m < sp->n_vib_modes; ++m) {
+        const int l = (int)(-or_log(1.0 - rng_u(r)) * T / sp->theta_v[m]);
+        lv[m * stride] = (uint16_t)(l > 65535 ? 65535 : l);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* sigmaTcR and binary collisions                                           */
+/* ------------------------------------------------------------------------ */
+static double sigma_tcr(const or_derived* d, int tp, int tq, const double up[3], const double uq[3])
+{
+    const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+    const double cr2 = dx * dx + dy * dy + dz * dz;
+    const double cr = sqrt(cr2);
+    if (cr < OR_VSMALL) return 0.0;
+    const double om = d->omega_pq[tp][tq];
+    return d->sig_coef[tp][tq] * or_pow(d->ref_term[tp][tq] / cr2, om - 0.5) * cr;
+}
+
+/* postCollisionVelocities, VHS (VariableHardSphere.C:191-228) or VSS
+ * (VariableSoftSphere.C:189-247).  cr < 0 means |UP-UQ|.  In the LB-VSS path
+ * (rescale=1) the pre-collision relative-velocity components are scaled to
+ * magnitude cr so Bird eq. 2.22 conserves the redistributed energy. */
+static void post_coll_vel(const hs_config* cfg, const or_derived* d, int vss, int rescale,
+                          int tp, int tq, double up[3], double uq[3], double cr, or_rng* r)
+{
+    const double mp = cfg->species[tp].mass, mq = cfg->species[tq].mass;
+    if (cr < 0.0) {
+        const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+        cr = sqrt(dx * dx + dy * dy + dz * dz);
+    }
+    const double msum = mp + mq;
+    double ucm[3];
+    for (int c = 0; c < 3; ++c) ucm[c] = (mp * up[c] + mq * uq[c]) / msum;
+    double rel[3];
+    if (!vss) {
+        const double cos_t = 2.0 * rng_u(r) - 1.0;
+        const double sin_t = sqrt(1.0 - cos_t * cos_t);
+        double sp, cp;
+        or_sincos2pi(rng_u(r), &sp, &cp);
+        rel[0] = cr * cos_t;
+        rel[1] = cr * (sin_t * cp);
+        rel[2] = cr * (sin_t * sp);
+    } else {
+        double c0 = up[0] - uq[0], c1 = up[1] - uq[1], c2 = up[2] - uq[2];
+        if (rescale) {
+            const double cm = sqrt(c0 * c0 + c1 * c1 + c2 * c2);
+            if (cm > OR_VSMALL) { const double f = cr / cm; c0 *= f; c1 *= f; c2 *= f; }
+        }
+        const double cos_t = 2.0 * or_pow(rng_u(r), 1.0 / d->alpha_pq[tp][tq]) - 1.0;
+        const double sin_t = sqrt(1.0 - cos_t * cos_t);
+        double sp, cp;
+        or_sincos2pi(rng_u(r), &sp, &cp);
+        const double D = sqrt(c1 * c1 + c2 * c2);
+        if (D > OR_VSMALL) {
+            rel[0] = cos_t * c0 + sin_t * sp * D;
+            rel[1] = cos_t * c1 + sin_t * (cr * c2 * cp - c0 * c1 * sp) / D;
+            rel[2] = cos_t * c2 - sin_t * (cr * c1 * cp + c0 * c2 * sp) / D;
+        } else {
+            /* relative velocity along x: Bird 2.22 is singular, the
+             * deflection is isotropic about x */
+            rel[0] = cos_t * c0;
+            rel[1] = cr * (sin_t * cp);
+            rel[2] = cr * (sin_t * sp);
+        }
+    }
+    for (int c = 0; c < 3; ++c) {
+        up[c] = ucm[c] + rel[c] * mq / msum;
+        uq[c] = ucm[c] - rel[c] * mp / msum;
+    }
+}
+
+/* dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531),
+ * postReaction = false */
+static int vib_level(const hs_config* cfg, const or_derived* d, int t, int partner, int mode,
+                     int level, int imax, double ec, double om, or_rng* r)
+{
+    const hs_species* sp = &cfg->species[t];
+    const double theta_v = sp->theta_v[mode];
+    double inv_zv = 1.0;
+    if (cfg->z_vib_fixed == 0.0) {
+        const double T = (double)imax * theta_v / (3.5 - om);
+        const double pow1 = or_pow(sp->theta_d / T, 1.0 / 3.0) - 1.0;
+        const double zvp1 = or_pow(sp->theta_d / T, om);
+        const double zvp2 = or_pow(d->zv_b[t][partner][mode], pow1 / d->zv_pow2[t][mode]);
+        const double zv = zvp1 * zvp2;
+        inv_zv = cfg->inv_zv_form == 2 ? 1.0 / (5.0 * zv) : 1.0 / zv;
+    } else {
+        inv_zv = 1.0 / cfg->z_vib_fixed;
+    }
+    int idash = level;
+    if (inv_zv > rng_u(r)) {
+        double func;
+        int guard = 0;
+        do {
+            idash = rng_label(r, 0, imax);
+            const double evib = (double)idash * HS_KB * theta_v;
+            func = or_pow(1.0 - evib / ec, 1.5 - om);
+        } while (func < rng_u(r) && ++guard < OR_MAX_REDRAW);
+    }
+    return idash;
+}
+
+/* dsmcCloud::postCollisionRotationalEnergy (dsmcCloud.C:1327-1387) */
+static double rot_ratio(double dof, double chib, or_rng* r)
+{
+    if (dof == 2.0) return 1.0 - or_pow(rng_u(r), 1.0 / chib);
+    const double a1 = 0.5 * dof - 1.0, b1 = chib - 1.0;
+    if (a1 < OR_SMALL && b1 < OR_SMALL) return rng_u(r);
+    double ratio, P;
+    int guard = 0;
+    do {
+        ratio = rng_u(r);
+        if (a1 < OR_SMALL) P = or_pow(1.0 - ratio, b1);
+        else if (b1 < OR_SMALL) P = or_pow(1.0 - ratio, a1);
+        else P = or_pow((a1 + b1) * ratio / a1, a1) * or_pow((a1 + b1) * (1.0 - ratio) / b1, b1);
+    } while (P < rng_u(r) && ++guard < OR_MAX_REDRAW);
+    return ratio;
+}
+
+/* LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
+ * species with a single electronic level */
+static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, int partner,
+                            double* et, double* erot, int32_t* vib, double om, or_rng* r)
+{
+    const hs_species* sp = &cfg->species[t];
+    if (sp->charge == -1) return;
+    /* electronic mode: one level, the Liechty loop takes exactly one draw */
+    if (1.0 / cfg->z_el > rng_u(r)) (void)rng_u(r);
+    if (sp->n_vib_modes > 0) {
+        double pre[HS_MAX_VIB_MODES];
+        for (int m = 0; m < sp->n_vib_modes; ++m) pre[m] = (double)vib[m] * HS_KB * sp->theta_v[m];
+        for (int m = 0; m < sp->n_vib_modes; ++m) {
+            const double ec = *et + pre[m];
+            const int imax = (int)(ec / (HS_KB * sp->theta_v[m]));
+            if (imax > 0) {
+                vib[m] = vib_level(cfg, d, t, partner, m, vib[m], imax, ec, om, r);
+                *et = ec - (double)vib[m] * HS_KB * sp->theta_v[m];
+            }
+        }
+    }
+    if (sp->rot_dof > 0) {
+        const double pre = *erot;
+        if (1.0 / cfg->z_rot > rng_u(r)) {
+            const double ec = *et + pre;
+            const double ratio = rot_ratio(sp->rot_dof, 2.5 - om, r);
+            *erot = ratio * ec;
+            *et = ec - *erot;
+        }
+    }
+}
+
+/* BinaryCollisionModel::collide for the configured model */
+static void binary_collide(const hs_config* cfg, const or_derived* d, int tp, int tq,
+                           double up[3], double uq[3], double* erp, double* erq,
+                           int32_t* vp, int32_t* vq, or_rng* r)
+{
+    const int model = cfg->collision_model;
+    if (model == HS_COLL_VHS || model == HS_COLL_VSS) {
+        post_coll_vel(cfg, d, model == HS_COLL_VSS, 0, tp, tq, up, uq, -1.0, r);
+        return;
+    }
+    /* Larsen-Borgnakke (LB-VHS.C:110-147) */
+    const double mp = cfg->species[tp].mass, mq = cfg->species[tq].mass;
+    const double mr = mp * mq / (mp + mq);
+    const double dx = up[0] - uq[0], dy = up[1] - uq[1], dz = up[2] - uq[2];
+    const double cr2 = dx * dx + dy * dy + dz * dz;
+    double et = 0.5 * mr * cr2;
+    const double om = d->omega_pq[tp][tq];
+    lb_redistribute(cfg, d, tp, tq, &et, erp, vp, om, r);
+    lb_redistribute(cfg, d, tq, tp, &et, erq, vq, om, r);
+    const double cr = sqrt(2.0 * et / mr);
+    post_coll_vel(cfg, d, model == HS_COLL_LB_VSS, 1, tp, tq, up, uq, cr, r);
+}
+
+int or_sigma_tcr(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
+                 const double* up, const double* uq, double* out)
+{
+    or_derived d;
+    derive(cfg, &d);
+    for (int64_t k = 0; k < n; ++k) out[k] = sigma_tcr(&d, tp[k], tq[k], up + 3 * k, uq + 3 * k);
+    return 0;
+}
+
+int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
+                      double* up, double* uq, double* erot_p, double* erot_q,
+                      int32_t* vib_p, int32_t* vib_q, const double* uniforms,
+                      int32_t n_uniform, int32_t* used_out)
+{
+    or_derived d;
+    derive(cfg, &d);
+    int rc = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        or_rng r;
+        rng_init(&r, 0, 0, 0, 0, 0);
+        r.inj = uniforms + k * n_uniform;
+        r.inj_n = n_uniform;
+        double erp = erot_p ? erot_p[k] : 0.0, erq = erot_q ? erot_q[k] : 0.0;
+        int32_t* vp = vib_p ? vib_p + k * HS_MAX_VIB_MODES : NULL;
+        int32_t* vq = vib_q ? vib_q + k * HS_MAX_VIB_MODES : NULL;
+        int32_t zp[HS_MAX_VIB_MODES] = { 0 }, zq[HS_MAX_VIB_MODES] = { 0 };
+        binary_collide(cfg, &d, tp[k], tq[k], up + 3 * k, uq + 3 * k, &erp, &erq,
+                       vp ? vp : zp, vq ? vq : zq, &r);
+        if (erot_p) erot_p[k] = erp;
+        if (erot_q) erot_q[k] = erq;
+        if (used_out) used_out[k] = r.inj_used;
+        if (r.inj_used > n_uniform) rc = HS_E_OVERFLOW;
+    }
+    return rc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* move (dsmcParcel::move + tracker on the Cartesian block)                 */
+/* ------------------------------------------------------------------------ */
+static double face_coord(const hs_config* cfg, int a, int idx)
+{
+    return cfg->origin[a] + (double)idx * cfg->delta[a];
+}
+
+/* dsmcDiffuseWallPatch::performDiffuseReflection (dsmcDiffuseWallPatch.C:67-139)
+ * with dsmcPatchBoundary::calculateWallUnitVectors (dsmcPatchBoundary.C:216-261).
+ * nw is the wall unit normal pointing out of the gas. */
+static void diffuse_reflect(const hs_config* cfg, const hs_patch* pt, const double nw[3], int t,
+                            double U[3], float* erot, uint16_t* vib, int64_t vstride, or_rng* r)
+{
+    const hs_species* sp = &cfg->species[t];
+    double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+    double ut[3] = { U[0] - und * nw[0], U[1] - und * nw[1], U[2] - und * nw[2] };
+    int guard = 0;
+    while (sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]) < OR_SMALL && guard++ < 100) {
+        U[0] = U[0] * (0.8 + 0.2 * rng_u(r));
+        U[1] = U[1] * (0.8 + 0.2 * rng_u(r));
+        U[2] = U[2] * (0.8 + 0.2 * rng_u(r));
+        und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+        ut[0] = U[0] - und * nw[0]; ut[1] = U[1] - und * nw[1]; ut[2] = U[2] - und * nw[2];
+    }
+    const double mut = sqrt(ut[0] * ut[0] + ut[1] * ut[1] + ut[2] * ut[2]);
+    double tw1[3] = { ut[0] / mut, ut[1] / mut, ut[2] / mut };
+    if (!(mut > 0.0)) { tw1[0] = nw[1]; tw1[1] = nw[2]; tw1[2] = nw[0]; }
+    const double tw2[3] = { nw[1] * tw1[2] - nw[2] * tw1[1],
+                            nw[2] * tw1[0] - nw[0] * tw1[2],
+                            nw[0] * tw1[1] - nw[1] * tw1[0] };
+    const double g1 = rng_gauss(r);
+    const double g2 = rng_gauss(r);
+    const double u = rng_u(r);
+    const double T = pt->temperature;
+    const double sc = sqrt(HS_KB * T / sp->mass);
+    double arg = 1.0 - u;
+    if (arg < OR_VSMALL) arg = OR_VSMALL;
+    const double rn = sqrt(-2.0 * or_log(arg));
+    for (int c = 0; c < 3; ++c)
+        U[c] = sc * ((g1 * tw1[c] + g2 * tw2[c]) - rn * nw[c]) + pt->velocity[c];
+    if (erot) *erot = (float)equip_erot(r, T, sp->rot_dof);
+    if (vib) equip_vib(r, T, sp, vib, vstride);
+}
+
+static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, int64_t i, int is_new)
+{
+    or_rng r;
+    rng_init(&r, cfg->seed, OR_RNG_MOVE, (uint32_t)i, s->step, 0);
+    hs_particles* P = &s->p;
+    double t = cfg->dt;
+    if (is_new) {
+        const double sf = rng_u(&r);
+        t = (1.0 - sf) * cfg->dt;
+    }
+    double p[3] = { P->px[i], P->py[i], P->pz[i] };
+    double U[3] = { (double)P->ux[i], (double)P->uy[i], (double)P->uz[i] };
+    int32_t c = P->cell[i];
+    if (c < 0) return;
+    int idx[3] = { c % cfg->nx, (c / cfg->nx) % cfg->ny, c / (cfg->nx * cfg->ny) };
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    const int t_id = P->type ? P->type[i] : 0;
+    float* erot = P->erot ? &P->erot[i] : NULL;
+    uint16_t* vib = (P->vib && cfg->species[t_id].n_vib_modes > 0) ? &P->vib[i] : NULL;
+    for (int a = 0; a < 3; ++a)
+        if (d->empty[a]) p[a] = d->centre[a];
+    int keep = 1, iter = 0;
+    while (keep && t > OR_ROOTVSMALL) {
+        if (++iter > OR_MAX_TRACK) { s->counters[5] += 1; break; }
+        double ut[3], e[3];
+        for (int a = 0; a < 3; ++a) {
+            ut[a] = d->empty[a] ? 0.0 : U[a];
+            e[a] = p[a] + t * ut[a];
+        }
+        double lam = 1.0;
+        int axis = -1, dir = 0;
+        for (int a = 0; a < 3; ++a) {
+            const double den = e[a] - p[a];
+            if (ut[a] > 0.0 && den > 0.0) {
+                double l = (face_coord(cfg, a, idx[a] + 1) - p[a]) / den;
+                if (l < 0.0) l = 0.0;
+                if (l < lam) { lam = l; axis = a; dir = 1; }
+            } else if (ut[a] < 0.0 && den < 0.0) {
+                double l = (face_coord(cfg, a, idx[a]) - p[a]) / den;
+                if (l < 0.0) l = 0.0;
+                if (l < lam) { lam = l; axis = a; dir = -1; }
+            }
+        }
+        int wall = 0;
+        if (cfg->cyl_enabled) {
+            const double dx = e[0] - p[0], dy = e[1] - p[1];
+            const double rx = p[0] - cfg->cyl_center[0], ry = p[1] - cfg->cyl_center[1];
+            const double b = rx * dx + ry * dy;
+            if (b < 0.0) {
+                const double aa = dx * dx + dy * dy;
+                const double cc = rx * rx + ry * ry - cfg->cyl_radius * cfg->cyl_radius;
+                const double disc = b * b - aa * cc;
+                if (disc >= 0.0) {
+                    const double q = sqrt(disc) - b;
+                    double sh = cc / q;
+                    if (sh < 0.0) sh = 0.0;
+                    if (sh < lam) { lam = sh; wall = 1; }
+                }
+            }
+        }
+        if (!wall && axis < 0) {
+            p[0] = e[0]; p[1] = e[1]; p[2] = e[2];
+            t = 0.0;
+            break;
+        }
+        for (int b = 0; b < 3; ++b) p[b] = p[b] + lam * (e[b] - p[b]);
+        t = t - t * lam;
+        if (wall) {
+            s->counters[4] += 1;
+            double nx_ = cfg->cyl_center[0] - p[0], ny_ = cfg->cyl_center[1] - p[1];
+            const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
+            const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
+            if (cfg->cyl_patch.model == HS_PATCH_DIFFUSE) {
+                diffuse_reflect(cfg, &cfg->cyl_patch, nw, t_id, U, erot, vib, s->capacity, &r);
+            } else {
+                const double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+                if (und > 0.0)
+                    for (int b = 0; b < 3; ++b) U[b] = U[b] - 2.0 * und * nw[b];
+            }
+            continue;
+        }
+        p[axis] = face_coord(cfg, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
+        const int ni = idx[axis] + dir;
+        if (ni >= 0 && ni < nn[axis]) { idx[axis] = ni; continue; }
+        const hs_patch* pt = &cfg->patch[2 * axis + (dir > 0 ? 1 : 0)];
+        switch (pt->model) {
+        case HS_PATCH_CYCLIC:
+            p[axis] = face_coord(cfg, axis, dir > 0 ? 0 : nn[axis]);
+            idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
+            break;
+        case HS_PATCH_DIFFUSE: {
+            s->counters[4] += 1;
+            double nw[3] = { 0.0, 0.0, 0.0 };
+            nw[axis] = (double)dir;
+            diffuse_reflect(cfg, pt, nw, t_id, U, erot, vib, s->capacity, &r);
+            break;
+        }
+        case HS_PATCH_DELETION:
+            keep = 0;
+            break;
+        default: /* specular wall, or an empty/symmetry plane */
+            s->counters[4] += 1;
+            if (U[axis] * (double)dir > 0.0) U[axis] = -U[axis];
+            break;
+        }
+    }
+    P->px[i] = p[0]; P->py[i] = p[1]; P->pz[i] = p[2];
+    P->ux[i] = (float)U[0]; P->uy[i] = (float)U[1]; P->uz[i] = (float)U[2];
+    if (!keep) { P->cell[i] = -1; s->counters[2] += 1; }
+    else P->cell[i] = idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2]);
+}
+
+int or_move(const hs_config* cfg, or_state* s, int64_t n_old)
+{
+    or_derived d;
+    derive(cfg, &d);
+    for (int64_t i = 0; i < s->n; ++i) move_one(cfg, &d, s, i, i >= n_old);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* cell occupancy: stable counting sort (dsmcCloud.C:63-74)                 */
+/* ------------------------------------------------------------------------ */
+static void copy_particle(const hs_config* cfg, const or_derived* d, const hs_particles* src,
+                          int64_t i, int64_t src_stride, hs_particles* dst, int64_t j,
+                          int64_t dst_stride)
+{
+    dst->px[j] = src->px[i]; dst->py[j] = src->py[i]; dst->pz[j] = src->pz[i];
+    dst->ux[j] = src->ux[i]; dst->uy[j] = src->uy[i]; dst->uz[j] = src->uz[i];
+    dst->cell[j] = src->cell[i];
+    if (src->erot && dst->erot) dst->erot[j] = src->erot[i];
+    if (src->type && dst->type) dst->type[j] = src->type[i];
+    if (src->vib && dst->vib)
+        for (int m = 0; m < d->max_modes; ++m) dst->vib[m * dst_stride + j] = src->vib[m * src_stride + i];
+    (void)cfg;
+}
+
+int or_sort(const hs_config* cfg, or_state* s)
+{
+    or_derived d;
+    derive(cfg, &d);
+    const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    memset(s->count, 0, sizeof(int32_t) * nc);
+    for (int64_t i = 0; i < s->n; ++i)
+        if (s->p.cell[i] >= 0) s->count[s->p.cell[i]] += 1;
+    s->start[0] = 0;
+    for (int64_t c = 0; c < nc; ++c) s->start[c + 1] = s->start[c] + s->count[c];
+    int32_t* cur = (int32_t*)malloc(sizeof(int32_t) * nc);
+    memcpy(cur, s->start, sizeof(int32_t) * nc);
+    for (int64_t i = 0; i < s->n; ++i) {
+        const int32_t c = s->p.cell[i];
+        if (c < 0) continue;
+        copy_particle(cfg, &d, &s->p, i, s->capacity, &s->tmp, cur[c]++, s->capacity);
+    }
+    free(cur);
+    s->n = s->start[nc];
+    for (int64_t j = 0; j < s->n; ++j) copy_particle(cfg, &d, &s->tmp, j, s->capacity, &s->p, j, s->capacity);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* NTC collisions (noTimeCounter.C:81-339) + sampling (dsmcVolFields.C)     */
+/* ------------------------------------------------------------------------ */
+static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c,
+                        int32_t b, int32_t nC, int64_t nc)
+{
+    const int ns = cfg->n_species;
+    const int nslot = or_sample_slots(cfg);
+    double acc[HS_S_BASE_COUNT + HS_MAX_SPECIES + HS_MAX_SPECIES * HS_MAX_VIB_MODES + 13];
+    memset(acc, 0, sizeof(acc));
+    const hs_particles* P = &s->p;
+    for (int32_t k = 0; k < nC; ++k) {
+        const int64_t i = b + k;
+        const int t = P->type ? P->type[i] : 0;
+        const hs_species* sp = &cfg->species[t];
+        const double m = sp->mass;
+        const double ux = P->ux[i], uy = P->uy[i], uz = P->uz[i];
+        const double lke = m * (ux * ux + uy * uy + uz * uz);
+        const double er = P->erot ? (double)P->erot[i] : 0.0;
+        double ev = 0.0;
+        acc[HS_S_N] += 1.0;
+        acc[HS_S_M] += m;
+        acc[HS_S_MUX] += m * ux;
+        acc[HS_S_MUY] += m * uy;
+        acc[HS_S_MUZ] += m * uz;
+        acc[HS_S_MCC] += lke;
+        acc[HS_S_EROT] += er;
+        acc[HS_S_ZROT] += sp->rot_dof;
+        acc[HS_S_BASE_COUNT + t] += 1.0;
+        for (int mo = 0; mo < sp->n_vib_modes; ++mo) {
+            const double e = (double)P->vib[mo * s->capacity + i] * HS_KB * sp->theta_v[mo];
+            acc[HS_S_BASE_COUNT + ns + t * HS_MAX_VIB_MODES + mo] += e;
+            ev += e;
+        }
+        if (cfg->flags & HS_FLAG_HEATFLUX) {
+            double* h = &acc[HS_S_BASE_COUNT + ns + ns * HS_MAX_VIB_MODES];
+            const double eint = er + ev;
+            h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+            h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+            h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+            h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+        }
+    }
+    for (int sl = 0; sl < nslot; ++sl) {
+        if (sl == HS_S_NCOLL || sl == HS_S_COLLSEP) continue;
+        s->samples[(int64_t)sl * nc + c] += acc[sl];
+    }
+    (void)d;
+}
+
+int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
+{
+    or_derived d;
+    derive(cfg, &d);
+    const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    hs_particles* P = &s->p;
+    int32_t* sub_list = (int32_t*)malloc(sizeof(int32_t) * (s->n > 0 ? s->n : 1));
+    int32_t* which = (int32_t*)malloc(sizeof(int32_t) * (s->n > 0 ? s->n : 1));
+    const int active = cfg->collision_model != HS_COLL_NONE;
+    for (int64_t c = 0; c < nc; ++c) {
+        const int32_t b = s->start[c];
+        const int32_t nC = s->start[c + 1] - b;
+        double ncoll = 0.0, sep = 0.0;
+        if (active && nC > 1) {
+            const int ci = (int)(c % cfg->nx), cj = (int)((c / cfg->nx) % cfg->ny), ck = (int)(c / ((int64_t)cfg->nx * cfg->ny));
+            const double cc[3] = { cfg->origin[0] + ((double)ci + 0.5) * cfg->delta[0],
+                                   cfg->origin[1] + ((double)cj + 0.5) * cfg->delta[1],
+                                   cfg->origin[2] + ((double)ck + 0.5) * cfg->delta[2] };
+            int32_t scn[8] = { 0 }, sco[9];
+            for (int32_t k = 0; k < nC; ++k) {
+                const int64_t i = b + k;
+                const int sc = (P->px[i] - cc[0] >= 0.0 ? 1 : 0) + 2 * (P->py[i] - cc[1] >= 0.0 ? 1 : 0)
+                             + 4 * (P->pz[i] - cc[2] >= 0.0 ? 1 : 0);
+                which[k] = sc;
+                scn[sc] += 1;
+            }
+            sco[0] = 0;
+            for (int q = 0; q < 8; ++q) sco[q + 1] = sco[q] + scn[q];
+            {
+                int32_t fill[8];
+                memcpy(fill, sco, sizeof(fill));
+                for (int32_t k = 0; k < nC; ++k) sub_list[fill[which[k]]++] = k;
+            }
+            or_rng r;
+            rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, 0);
+            const double smax = s->sigma_tcr_max[c];
+            const double sel = s->remainder[c]
+                + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
+            int64_t ncand = (int64_t)sel;
+            s->remainder[c] = sel - (double)ncand;
+            if (ncand > OR_MAX_CAND) { ncand = OR_MAX_CAND; s->counters[6] += 1; }
+            s->counters[0] += ncand;
+            for (int64_t k = 0; k < ncand; ++k) {
+                const int pp = rng_label(&r, 0, nC - 1);
+                const int sc = which[pp];
+                const int nsc = scn[sc];
+                int qq = -1, guard = 0;
+                if (nsc > 1) {
+                    do { qq = sub_list[sco[sc] + rng_label(&r, 0, nsc - 1)]; }
+                    while (pp == qq && ++guard < OR_MAX_REDRAW);
+                } else {
+                    do { qq = rng_label(&r, 0, nC - 1); } while (pp == qq && ++guard < OR_MAX_REDRAW);
+                }
+                if (pp == qq) continue;
+                const int64_t ip = b + pp, iq = b + qq;
+                const int tp = P->type ? P->type[ip] : 0, tq = P->type ? P->type[iq] : 0;
+                if (cfg->species[tp].charge == -1 && cfg->species[tq].charge == -1) continue;
+                double up[3] = { P->ux[ip], P->uy[ip], P->uz[ip] };
+                double uq[3] = { P->ux[iq], P->uy[iq], P->uz[iq] };
+                const double stcr = sigma_tcr(&d, tp, tq, up, uq);
+                if (stcr > s->sigma_tcr_max[c]) s->sigma_tcr_max[c] = stcr;
+                if (stcr / smax > rng_u(&r)) {
+                    double erp = P->erot ? P->erot[ip] : 0.0, erq = P->erot ? P->erot[iq] : 0.0;
+                    int32_t vp[HS_MAX_VIB_MODES] = { 0 }, vq[HS_MAX_VIB_MODES] = { 0 };
+                    for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) vp[m] = P->vib[m * s->capacity + ip];
+                    for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) vq[m] = P->vib[m * s->capacity + iq];
+                    binary_collide(cfg, &d, tp, tq, up, uq, &erp, &erq, vp, vq, &r);
+                    P->ux[ip] = (float)up[0]; P->uy[ip] = (float)up[1]; P->uz[ip] = (float)up[2];
+                    P->ux[iq] = (float)uq[0]; P->uy[iq] = (float)uq[1]; P->uz[iq] = (float)uq[2];
+                    if (P->erot) { P->erot[ip] = (float)erp; P->erot[iq] = (float)erq; }
+                    for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) P->vib[m * s->capacity + ip] = (uint16_t)vp[m];
+                    for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) P->vib[m * s->capacity + iq] = (uint16_t)vq[m];
+                    ncoll += 1.0;
+                    if (cfg->flags & HS_FLAG_COLL_SEP) {
+                        const double ddx = P->px[ip] - P->px[iq], ddy = P->py[ip] - P->py[iq], ddz = P->pz[ip] - P->pz[iq];
+                        sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    }
+                    s->counters[1] += 1;
+                }
+            }
+        }
+        if (sample && s->samples) {
+            sample_cell(cfg, &d, s, c, b, nC, nc);
+            s->samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
+            s->samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
+        }
+    }
+    if (sample && s->samples) s->n_sampled += 1;
+    free(sub_list);
+    free(which);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* freestream inflow (dsmcFreeStreamInflowPatch.C:91-364)                   */
+/* ------------------------------------------------------------------------ */
+static void new_particle_internal(const hs_config* cfg, or_state* s, int64_t j, int t, double T_rot,
+                                  double T_vib, or_rng* r)
+{
+    const hs_species* sp = &cfg->species[t];
+    if (s->p.erot) s->p.erot[j] = (float)equip_erot(r, T_rot, sp->rot_dof);
+    if (s->p.vib && sp->n_vib_modes > 0) equip_vib(r, T_vib, sp, &s->p.vib[j], s->capacity);
+    if (s->p.type) s->p.type[j] = (uint8_t)t;
+}
+
+int or_inject(const hs_config* cfg, or_state* s)
+{
+    or_derived d;
+    derive(cfg, &d);
+    const int64_t nf = list_inflow_faces(cfg, NULL);
+    if (nf == 0) return 0;
+    or_face* faces = (or_face*)malloc(sizeof(or_face) * nf);
+    list_inflow_faces(cfg, faces);
+    const int ns = cfg->n_species;
+    const double sqrt_pi = sqrt(OR_PI);
+    /* accumulation (Bird eq. 4.22) */
+    for (int sp = 0; sp < ns; ++sp) {
+        if (!(cfg->inflow_n[sp] > 0.0)) continue;
+        const double mass = cfg->species[sp].mass;
+        const double mps = sqrt(2.0 * HS_KB * cfg->inflow_t / mass);
+        for (int64_t f = 0; f < nf; ++f) {
+            const int a = faces[f].axis;
+            const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+            const double fa = cfg->delta[a1] * cfg->delta[a2];
+            /* U . (-Sf/|Sf|): inward normal component */
+            const double s_cos = (cfg->inflow_u[a] * (double)(-faces[f].dir)) / mps;
+            s->inflow_acc[sp * nf + f] +=
+                (fa * cfg->inflow_n[sp] * cfg->dt * mps
+                 * (exp(-(s_cos * s_cos)) + sqrt_pi * s_cos * (1.0 + erf(s_cos))))
+                / (2.0 * sqrt_pi * cfg->fn);
+        }
+    }
+    /* insertion */
+    int64_t n = s->n;
+    for (int64_t f = 0; f < nf; ++f) {
+        const int a = faces[f].axis, dir = faces[f].dir;
+        const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+        const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+        for (int sp = 0; sp < ns; ++sp) {
+            if (!(cfg->inflow_n[sp] > 0.0)) continue;
+            or_rng r;
+            rng_init(&r, cfg->seed, OR_RNG_INJECT, (uint32_t)(f * ns + sp), s->step, 0);
+            double* acc = &s->inflow_acc[sp * nf + f];
+            int ni = (int)(*acc);
+            if (ni < 0) ni = 0;
+            if ((*acc - (double)ni) > rng_u(&r)) ni += 1;
+            *acc -= (double)ni;
+            const double mass = cfg->species[sp].mass;
+            const double mps = sqrt(2.0 * HS_KB * cfg->inflow_t / mass);
+            const double un_face = cfg->inflow_u[a] * (double)(-dir);
+            const double s_cos = un_face / mps;
+            const double coef_a = s_cos + sqrt(s_cos * s_cos + 2.0);
+            const double coef_b = 0.5 * (1.0 + s_cos * (s_cos - sqrt(s_cos * s_cos + 2.0)));
+            double scaling = 3.0;
+            if (s_cos < -3.0) scaling = fabs(s_cos) + 1.0;
+            const double sc = sqrt(HS_KB * cfg->inflow_t / mass);
+            for (int k = 0; k < ni; ++k) {
+                if (n >= s->capacity) { s->counters[6] += 1; break; }
+                double pos[3];
+                const double ua = rng_u(&r), ub = rng_u(&r);
+                pos[a] = face_coord(cfg, a, dir > 0 ? nn[a] : 0);
+                pos[a1] = d.empty[a1] ? d.centre[a1]
+                                      : face_coord(cfg, a1, faces[f].a1i) + ua * cfg->delta[a1];
+                pos[a2] = d.empty[a2] ? d.centre[a2]
+                                      : face_coord(cfg, a2, faces[f].a2i) + ub * cfg->delta[a2];
+                double un;
+                if (fabs(un_face) > OR_VSMALL) {
+                    double P;
+                    int guard = 0;
+                    do {
+                        const double unt = scaling * (2.0 * rng_u(&r) - 1.0);
+                        un = unt + s_cos;
+                        if (un < 0.0) P = -1.0;
+                        else P = 2.0 * un / coef_a * or_exp(coef_b - unt * unt);
+                    } while (P < rng_u(&r) && ++guard < OR_MAX_REDRAW);
+                } else {
+                    un = sqrt(-or_log(1.0 - rng_u(&r)));
+                }
+                const double g1 = rng_gauss(&r), g2 = rng_gauss(&r);
+                double U[3];
+                U[a] = mps * un * (double)(-dir);
+                U[a1] = sc * g1 + cfg->inflow_u[a1];
+                U[a2] = sc * g2 + cfg->inflow_u[a2];
+                s->p.px[n] = pos[0]; s->p.py[n] = pos[1]; s->p.pz[n] = pos[2];
+                s->p.ux[n] = (float)U[0]; s->p.uy[n] = (float)U[1]; s->p.uz[n] = (float)U[2];
+                s->p.cell[n] = faces[f].cell;
+                if (s->p.vib)
+                    for (int m = 0; m < d.max_modes; ++m) s->p.vib[m * s->capacity + n] = 0;
+                new_particle_internal(cfg, s, n, sp, cfg->inflow_trot, cfg->inflow_tvib, &r);
+                s->counters[3] += 1;
+                ++n;
+            }
+        }
+    }
+    s->n = n;
+    free(faces);
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* initial fill (dsmcMeshFill.C:68-~230)                                    */
+/* ------------------------------------------------------------------------ */
+int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, double Trot,
+            double Tvib, const double u[3], int32_t exact, uint32_t fill_id)
+{
+    or_derived d;
+    derive(cfg, &d);
+    const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+    const int ns = cfg->n_species;
+    int64_t n = s->n;
+    for (int64_t c = 0; c < nc; ++c) {
+        const double V = cfg->cell_volume[c];
+        if (!(V > 0.0)) continue;
+        const int idx[3] = { (int)(c % cfg->nx), (int)((c / cfg->nx) % cfg->ny), (int)(c / ((int64_t)cfg->nx * cfg->ny)) };
+        for (int sp = 0; sp < ns; ++sp) {
+            if (!(nd[sp] > 0.0)) continue;
+            or_rng rc, rg;
+            rng_init(&rc, cfg->seed, OR_RNG_FILL_COUNT, (uint32_t)c, fill_id, (uint32_t)sp);
+            rng_init(&rg, cfg->seed, OR_RNG_FILL_GEN, (uint32_t)c, fill_id, (uint32_t)sp);
+            const double req = nd[sp] * V / cfg->fn;
+            int ni;
+            if (exact) ni = (int)floor(req + 0.5);
+            else { ni = (int)req; if ((req - (double)ni) > rng_u(&rc)) ni += 1; }
+            const hs_species* spp = &cfg->species[sp];
+            const double sc = sqrt(HS_KB * T / spp->mass);
+            for (int k = 0; k < ni; ++k) {
+                if (n >= s->capacity) { s->counters[6] += 1; break; }
+                double pos[3] = { 0.0, 0.0, 0.0 };
+                int ok = 0;
+                for (int tries = 0; tries < OR_MAX_FILL_TRIES && !ok; ++tries) {
+                    for (int a = 0; a < 3; ++a) {
+                        const double ua = rng_u(&rg);
+                        pos[a] = d.empty[a] ? d.centre[a] : face_coord(cfg, a, idx[a]) + ua * cfg->delta[a];
+                    }
+                    ok = 1;
+                    if (cfg->cyl_enabled) {
+                        const double rx = pos[0] - cfg->cyl_center[0], ry = pos[1] - cfg->cyl_center[1];
+                        if (rx * rx + ry * ry < cfg->cyl_radius * cfg->cyl_radius) ok = 0;
+                    }
+                }
+                /* a failed rejection keeps its slot as a deleted parcel
+                 * (cell -1) so the population layout does not depend on it */
+                const double g0 = rng_gauss(&rg), g1 = rng_gauss(&rg), g2 = rng_gauss(&rg);
+                s->p.px[n] = pos[0]; s->p.py[n] = pos[1]; s->p.pz[n] = pos[2];
+                s->p.ux[n] = (float)(sc * g0 + u[0]);
+                s->p.uy[n] = (float)(sc * g1 + u[1]);
+                s->p.uz[n] = (float)(sc * g2 + u[2]);
+                s->p.cell[n] = ok ? (int32_t)c : -1;
+                if (s->p.vib)
+                    for (int m = 0; m < d.max_modes; ++m) s->p.vib[m * s->capacity + n] = 0;
+                new_particle_internal(cfg, s, n, sp, Trot, Tvib, &rg);
+                ++n;
+            }
+        }
+    }
+    s->n = n;
+    /* sigmaTcRMax = sigmaT * most probable speed of the most abundant species */
+    int most = 0;
+    for (int sp = 1; sp < ns; ++sp) if (nd[sp] > nd[most]) most = sp;
+    const hs_species* sm = &cfg->species[most];
+    const double smax = (OR_PI * sm->diameter * sm->diameter) * sqrt(2.0 * HS_KB * T / sm->mass);
+    for (int64_t c = 0; c < nc; ++c) {
+        s->sigma_tcr_max[c] = smax;
+        or_rng r;
+        rng_init(&r, cfg->seed, OR_RNG_REMAINDER, (uint32_t)c, fill_id, 0);
+        s->remainder[c] = rng_u(&r);
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* dsmcCloud::evolve                                                        */
+/* ------------------------------------------------------------------------ */
+int or_evolve(const hs_config* cfg, or_state* s, int32_t nsteps)
+{
+    for (int32_t k = 0; k < nsteps; ++k) {
+        const int64_t n_old = s->n;
+        if (s->inflow_acc) or_inject(cfg, s);
+        or_move(cfg, s, n_old);
+        or_sort(cfg, s);
+        or_collide(cfg, s, (cfg->flags & HS_FLAG_SAMPLE) ? 1 : 0);
+        s->step += 1;
+    }
+    return 0;
+}

@@ -1,0 +1,81 @@
+/*
+ * dsmc_oracle.h -- CPU restatement of hyStrath dsmcFoam+'s per-timestep
+ * particle path, used ONLY as test infrastructure (parity checker and the
+ * CPU baseline of the benchmark).  Nothing in the product links it.
+ *
+ * Each function cites the reference file:line it follows.  The random
+ * stream is a counter-based Philox4x32-10 (Salmon et al., SC'11) keyed by
+ * (seed, purpose, entity, step) so that the GPU engine can replay it exactly;
+ * transcendental functions are this file's own FP64 routines built from
+ * IEEE +,-,*,/ and sqrt only, so results are bit-reproducible across
+ * compilers.  Build with -ffp-contract=off.
+ */
+#ifndef DSMC_ORACLE_H
+#define DSMC_ORACLE_H
+
+#include <stdint.h>
+#include "../include/hystrath_dsmc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* random-stream purposes (Philox counter word 2, top byte) */
+enum { OR_RNG_MOVE = 1, OR_RNG_COLLIDE = 2, OR_RNG_INJECT = 3,
+       OR_RNG_FILL_COUNT = 4, OR_RNG_FILL_GEN = 5, OR_RNG_REMAINDER = 6 };
+
+typedef struct or_state {
+    int64_t n, capacity;
+    hs_particles p;          /* current population (capacity-sized arrays) */
+    hs_particles tmp;        /* sort scratch (capacity-sized)               */
+    double* sigma_tcr_max;   /* [nc]                                        */
+    double* remainder;       /* [nc]                                        */
+    double* samples;         /* [slots][nc] or NULL                         */
+    int64_t n_sampled;       /* nTimeSteps_                                  */
+    double* inflow_acc;      /* [n_species][n_inflow_faces] or NULL          */
+    int32_t* count;          /* [nc]                                        */
+    int32_t* start;          /* [nc+1]                                      */
+    uint32_t step;
+    int64_t counters[8];     /* candidates, collisions, deleted, injected,
+                                wall hits, track overflow, -, - */
+} or_state;
+
+/* primitives (exported for pinning tests) */
+void or_philox4x32(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+double or_uniform_from(uint32_t hi, uint32_t lo);
+double or_log(double x);
+double or_exp(double x);
+double or_pow(double x, double y);
+void or_sincos2pi(double u, double* s, double* c);
+/* n draws of stream (seed, purpose, id, step, sub) into out */
+void or_stream_draws(uint64_t seed, int purpose, uint32_t id, uint32_t step,
+                     uint32_t sub, int n, double* out);
+void or_stream_gauss(uint64_t seed, int purpose, uint32_t id, uint32_t step,
+                     uint32_t sub, int n, double* out);
+
+int32_t or_sample_slots(const hs_config* cfg);
+int64_t or_inflow_face_count(const hs_config* cfg);
+
+/* sigmaTcR (VariableHardSphere.C:80-121) */
+int or_sigma_tcr(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
+                 const double* up, const double* uq, double* out);
+/* BinaryCollisionModel::collide with injected uniforms (see header of the
+ * C ABI, hs_dsmc_binary_collide) */
+int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
+                      double* up, double* uq, double* erot_p, double* erot_q,
+                      int32_t* vib_p, int32_t* vib_q, const double* uniforms,
+                      int32_t n_uniform, int32_t* used_out);
+
+/* stages on an or_state */
+int or_fill(const hs_config* cfg, or_state* s, const double* n, double t, double trot,
+            double tvib, const double u[3], int32_t exact_per_cell, uint32_t fill_id);
+int or_inject(const hs_config* cfg, or_state* s);
+int or_move(const hs_config* cfg, or_state* s, int64_t n_old);
+int or_sort(const hs_config* cfg, or_state* s);
+int or_collide(const hs_config* cfg, or_state* s, int32_t sample);
+int or_evolve(const hs_config* cfg, or_state* s, int32_t nsteps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

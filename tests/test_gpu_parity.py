@@ -1,0 +1,185 @@
+"""GPU parity: every stage of the HIP engine against the CPU oracle on the same
+seeded inputs.  Integer/index state (cells, counts, order) and FP64 positions
+must be bit-exact; the engine and the oracle evaluate the same IEEE operation
+sequence, so velocities are required bit-exact as well (stricter than the
+1e-6 relative bar).  Sizes are kept small enough for the oracle to finish in
+seconds."""
+import numpy as np
+import pytest
+
+from hystrath_amd import abi, cases
+from hystrath_amd.engine import Engine
+import helpers as H
+import oracle_bind as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(case, filler):
+    eng = Engine(case)
+    st = ob.OracleState(case)
+    filler(eng, case)
+    filler(st, case)
+    return eng, st
+
+
+def _check_state(eng, st, label):
+    H.assert_same_particles(eng.download(), st.particles(), label)
+    sig, rem = eng.cell_state()
+    np.testing.assert_array_equal(sig, st.sigma, err_msg=f"{label}: sigmaTcRMax")
+    np.testing.assert_array_equal(rem, st.rem, err_msg=f"{label}: collisionSelectionRemainder")
+
+
+@pytest.mark.parametrize("which", ["box", "cylinder", "couette"])
+def test_fill_parity(which):
+    case, filler = {
+        "box": (H.small_box(), H.fill_box),
+        "cylinder": (H.small_cylinder(), H.fill_cylinder),
+        "couette": (cases.couette_n2o2(), H.fill_couette),
+    }[which]
+    eng, st = _pair(case, filler)
+    assert st.n > 0
+    _check_state(eng, st, f"fill/{which}")
+    eng.close()
+
+
+@pytest.mark.parametrize("model", [abi.HS_COLL_VHS, abi.HS_COLL_VSS])
+def test_sigma_and_binary_collide_argon(model):
+    case = H.small_box(collision_model=model)
+    case.species[0].alpha = 1.4
+    case.cfg.species[0].alpha = 1.4
+    eng = Engine(case)
+    rng = np.random.default_rng(3)
+    n = 4096
+    tp = np.zeros(n, np.int32)
+    up = rng.normal(0, 400, (n, 3))
+    uq = rng.normal(0, 400, (n, 3))
+    uq[:8] = up[:8]  # zero relative speed
+    s_gpu = eng.sigma_tcr(tp, tp, up, uq)
+    s_cpu = ob.sigma_tcr(case, tp, tp, up, uq)
+    np.testing.assert_array_equal(s_gpu, s_cpu)
+    uni = rng.uniform(0, 1, (n, 8))
+    g = eng.binary_collide(tp, tp, up, uq, uni)
+    c = ob.binary_collide(case, tp, tp, up, uq, uni)
+    assert g[7] == 0 and c[7] == 0
+    np.testing.assert_array_equal(g[0], c[0])
+    np.testing.assert_array_equal(g[1], c[1])
+    np.testing.assert_array_equal(g[6], c[6])
+    m = case.species[0].mass
+    p0 = m * (up + uq)
+    p1 = m * (g[0] + g[1])
+    scale = m * (np.abs(up) + np.abs(uq)).sum(axis=1, keepdims=True)
+    assert np.max(np.abs(p1 - p0) / scale) < 1e-14
+    e0 = 0.5 * m * ((up ** 2).sum(1) + (uq ** 2).sum(1))
+    e1 = 0.5 * m * ((g[0] ** 2).sum(1) + (g[1] ** 2).sum(1))
+    assert np.max(np.abs(e1 - e0) / e0) < 1e-14
+    eng.close()
+
+
+@pytest.mark.parametrize("model", [abi.HS_COLL_LB_VHS, abi.HS_COLL_LB_VSS])
+def test_binary_collide_lb(model):
+    case = cases.couette_n2o2()
+    case.cfg.collision_model = model
+    eng = Engine(case)
+    rng = np.random.default_rng(4)
+    n = 4096
+    tp = rng.integers(0, 2, n).astype(np.int32)
+    tq = rng.integers(0, 2, n).astype(np.int32)
+    up = rng.normal(0, 2500, (n, 3))
+    uq = rng.normal(0, 2500, (n, 3))
+    ep = rng.exponential(5e-20, n)
+    eq = rng.exponential(5e-20, n)
+    vp = np.zeros((n, 3), np.int32)
+    vq = np.zeros((n, 3), np.int32)
+    vp[:, 0] = rng.integers(0, 4, n)
+    vq[:, 0] = rng.integers(0, 4, n)
+    uni = rng.uniform(0, 1, (n, 64))
+    g = eng.binary_collide(tp, tq, up, uq, uni, ep, eq, vp, vq)
+    c = ob.binary_collide(case, tp, tq, up, uq, uni, ep, eq, vp, vq)
+    assert g[7] == 0 and c[7] == 0
+    for k in range(7):
+        np.testing.assert_array_equal(g[k], c[k], err_msg=f"output {k}")
+    # total energy (translational + rotational + vibrational) conserved
+    mass = np.array([case.species[0].mass, case.species[1].mass])
+    th = np.array([case.species[0].theta_v[0], case.species[1].theta_v[0]])
+    kb = abi.HS_KB
+
+    def etot(u1, u2, e1, e2, v1, v2):
+        return (0.5 * mass[tp] * (u1 ** 2).sum(1) + 0.5 * mass[tq] * (u2 ** 2).sum(1) + e1 + e2
+                + v1[:, 0] * kb * th[tp] + v2[:, 0] * kb * th[tq])
+
+    E0 = etot(up, uq, ep, eq, vp, vq)
+    E1 = etot(g[0], g[1], g[2], g[3], g[4], g[5])
+    assert np.max(np.abs(E1 - E0) / E0) < 1e-13
+    P0 = mass[tp][:, None] * up + mass[tq][:, None] * uq
+    P1 = mass[tp][:, None] * g[0] + mass[tq][:, None] * g[1]
+    scale = (mass[tp][:, None] * np.abs(up) + mass[tq][:, None] * np.abs(uq)).sum(1)
+    assert np.max(np.abs(P1 - P0).sum(1) / scale) < 1e-14
+    eng.close()
+
+
+@pytest.mark.parametrize("which", ["box", "cylinder", "couette"])
+def test_move_sort_collide_stages(which):
+    case, filler = {
+        "box": (H.small_box(), H.fill_box),
+        "cylinder": (H.small_cylinder(), H.fill_cylinder),
+        "couette": (cases.couette_n2o2(), H.fill_couette),
+    }[which]
+    eng, st = _pair(case, filler)
+    eng.move()
+    st.move()
+    _check_state(eng, st, f"{which}/move")
+    eng.sort()
+    st.sort()
+    _check_state(eng, st, f"{which}/sort")
+    counts_eng = np.bincount(eng.download()["cell"], minlength=case.ncells)
+    np.testing.assert_array_equal(counts_eng, st.count)
+    eng.collide(sample=True)
+    st.collide(sample=True)
+    _check_state(eng, st, f"{which}/collide")
+    s_eng, n_eng = eng.samples()
+    np.testing.assert_array_equal(s_eng, st.sample_array())
+    assert n_eng == st.s.n_sampled == 1
+    c = eng.counters()
+    assert c["candidates"] == st.counters()[0]
+    assert c["collisions"] == st.counters()[1]
+    eng.close()
+
+
+@pytest.mark.parametrize("which,steps", [("box", 6), ("cylinder", 8), ("couette", 5)])
+def test_evolve_parity(which, steps):
+    case, filler = {
+        "box": (H.small_box(), H.fill_box),
+        "cylinder": (H.small_cylinder(), H.fill_cylinder),
+        "couette": (cases.couette_n2o2(), H.fill_couette),
+    }[which]
+    eng, st = _pair(case, filler)
+    for k in range(steps):
+        eng.evolve(1)
+        st.evolve(1)
+        _check_state(eng, st, f"{which}/step{k}")
+    s_eng, n_eng = eng.samples()
+    np.testing.assert_array_equal(s_eng, st.sample_array())
+    assert n_eng == steps
+    if st.inflow is not None:
+        np.testing.assert_array_equal(eng.inflow()[: st.inflow.size], st.inflow)
+    c = eng.counters()
+    oc = st.counters()
+    assert (c["candidates"], c["collisions"], c["deleted"], c["injected"], c["wall_hits"]) == tuple(oc[:5])
+    assert c["track_overflow"] == 0 and c["tile_overflow"] == 0 and c["capacity_overflow"] == 0
+    assert eng.step == st.step == steps
+    eng.close()
+
+
+def test_oversize_cell_parity():
+    """A cell holding more parcels than an LDS tile goes through the global
+    memory path; the result must still equal the oracle."""
+    case = cases.periodic_box(n_side=1, L=0.01, ppc=3000, seed=17)
+    eng, st = _pair(case, H.fill_box)
+    assert st.n == 3000
+    for k in range(3):
+        eng.evolve(1)
+        st.evolve(1)
+        _check_state(eng, st, f"big/step{k}")
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    eng.close()
