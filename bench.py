@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Throughput of the dsmcFoam+ per-timestep particle engine on one MI355X.
+
+A step is one dsmcCloud::evolve of the workload: freestream injection, move
+with wall/deletion boundaries, cell sort, NTC + VHS collisions and
+dsmcVolFields sampling.  Default workload: 2D Mach-10 argon flow over a
+cylinder of radius 0.1 m, 400x400 cells, 25 simulators per freestream cell
+(about 4M particles).  Inputs are synthetic (a uniform freestream fill) and
+resident in HBM before the timed region; the warm-up steps develop the
+shock layer first.
+
+Prints one JSON line: particle-steps/s, the HBM roofline of the dominant
+kernel (algorithmic bytes / HIP-event duration) and a bounded CPU baseline
+(the C restatement of the reference algorithm, single thread, on the same
+evolved state).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) HBM bytes of each stage for the argon
+# structure-of-arrays layout: FP64 position (24 B), FP32 velocity (12 B),
+# int32 cell (4 B).  Per particle and per cell of one launch.
+STAGE_BYTES = {
+    # read pos+vel+cell 40, write pos+cell 28 and the histogram slot 4
+    "move": dict(per_particle=72.0, per_cell=0.0),
+    # read counts, write start offsets, clear counts
+    "scan": dict(per_particle=0.0, per_cell=12.0),
+    # read cell + slot, write the permutation entry
+    "scatter": dict(per_particle=12.0, per_cell=0.0),
+    # read perm + pos + vel, write sorted pos + vel + cell; per cell:
+    # start, sigmaTcRMax and remainder (read+write), volume, 11 sample sums (read+write)
+    "collide": dict(per_particle=80.0, per_cell=4 + 16 + 16 + 8 + 11 * 16),
+    # per inserted particle: 40 B written (counted per particle moved: small)
+    "inject": dict(per_particle=0.0, per_cell=0.0),
+}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_case(workload):
+    from hystrath_amd import abi, cases
+
+    if workload == "cylinder":
+        case = cases.cylinder_flow(nx=400, ny=400, ppc=25, dt=2.5e-7, seed=20251202, flags=abi.HS_FLAG_SAMPLE)
+        desc = {"workload": "cylinder_mach10_argon_400x400", "cells": case.ncells, "ppc_freestream": 25,
+                "n_inf": case.n_inf, "T_inf": case.T_inf, "U_inf": round(case.U_inf, 3), "T_wall": 500.0,
+                "radius": 0.1, "dt": case.cfg.dt, "collision_model": "VariableHardSphere",
+                "partner_selection": "noTimeCounter", "sampling": "every step"}
+
+        def fill(e):
+            e.fill([case.n_inf], case.T_inf, u=(case.U_inf, 0.0, 0.0), fill_id=0)
+    else:
+        case = cases.periodic_box(n_side=20, ppc=20, seed=20251202, flags=abi.HS_FLAG_SAMPLE)
+        desc = {"workload": "periodic_box_argon_20x20x20", "cells": case.ncells, "ppc": 20, "dt": case.cfg.dt,
+                "collision_model": "VariableHardSphere", "partner_selection": "noTimeCounter",
+                "sampling": "every step"}
+
+        def fill(e):
+            e.fill([case.n_density / 2], 300.0, exact=True, fill_id=0)
+            e.fill([case.n_density / 2], case.T_hot, exact=True, fill_id=1)
+    return case, desc, fill
+
+
+def run_evolve(eng, nsteps, chunk=250, what="steps"):
+    done = 0
+    t0 = time.time()
+    while done < nsteps:
+        k = min(chunk, nsteps - done)
+        eng.evolve(k)
+        done += k
+        if time.time() - t0 > 20:
+            eng.synchronize()
+            log(f"  {what}: {done}/{nsteps}")
+            t0 = time.time()
+
+
+def cpu_baseline(case, eng, budget_s):
+    """Time the CPU oracle on the engine's current state (bounded sample)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import oracle_bind as ob
+
+    st = ob.OracleState(case)
+    st.load(eng.download(), *eng.cell_state())
+    if st.inflow is not None:
+        st.inflow[:] = eng.inflow()[: st.inflow.size]
+    st.step = eng.step
+    n0 = st.n
+    t = time.perf_counter()
+    st.evolve(1)
+    t1 = time.perf_counter() - t
+    psteps = n0
+    steps = 1
+    more = max(0, int(budget_s / max(t1, 1e-6)) - 1)
+    if more:
+        n1 = st.n
+        t = time.perf_counter()
+        st.evolve(more)
+        t1 += time.perf_counter() - t
+        psteps += int(0.5 * (n1 + st.n) * more)
+        steps += more
+    return {"value": psteps / t1, "unit": "particle-steps/s", "cores": 1, "kind": "port",
+            "sample": f"{steps} full steps of the same workload state ({n0} particles), C restatement "
+                      f"of the reference loops (oracle/dsmc_oracle.c, gcc -O2), single thread",
+            "seconds": round(t1, 3)}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=1500)
+    ap.add_argument("--workload", choices=["cylinder", "box"], default="cylinder")
+    ap.add_argument("--profile-steps", type=int, default=200, help="instrumented steps for kernel durations")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
+              f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
+        return 2
+
+    from hystrath_amd import abi
+    from hystrath_amd.engine import Engine, device_copy_ms
+
+    case, desc, fill = build_case(args.workload)
+    log(f"[bench] {desc['workload']}: {case.ncells} cells, capacity {case.cfg.capacity}")
+    eng = Engine(case)
+    fill(eng)
+    n_fill = eng.count()
+    log(f"[bench] filled {n_fill} particles; warm-up {args.warmup} steps")
+    run_evolve(eng, args.warmup, what="warm-up")
+    eng.synchronize()
+
+    # ---- timed region ---------------------------------------------------------
+    eng.reset_counters()
+    eng.synchronize()
+    t0 = time.perf_counter()
+    run_evolve(eng, args.steps, what="timed")
+    eng.synchronize()
+    elapsed = time.perf_counter() - t0
+    cnt = eng.counters()
+    psteps = cnt["particle_steps"]
+    value = psteps / elapsed
+    ms_per_step = 1e3 * elapsed / max(args.steps, 1)
+    n_now = eng.count()
+    bad = {k: cnt[k] for k in ("track_overflow", "tile_overflow", "capacity_overflow") if cnt[k]}
+    log(f"[bench] {args.steps} steps in {elapsed:.3f}s: {value:.4e} particle-steps/s, N={n_now} {bad or ''}")
+
+    # ---- instrumented pass: per-kernel durations on the engine's stream ------------
+    eng.set_flags(case.cfg.flags | abi.HS_FLAG_TIMING)
+    eng.kernel_times(reset=True)
+    eng.reset_counters()
+    run_evolve(eng, args.profile_steps, what="profile")
+    kt = eng.kernel_times()
+    pc = eng.counters()
+    eng.set_flags(case.cfg.flags & ~abi.HS_FLAG_TIMING)
+    n_step = pc["particle_steps"] / max(args.profile_steps, 1)
+    stages = {}
+    for name, (ms, launches) in kt.items():
+        if launches == 0:
+            continue
+        avg_ms = ms / launches
+        b = STAGE_BYTES[name]
+        nbytes = b["per_particle"] * n_step + b["per_cell"] * case.ncells
+        stages[name] = {"avg_ms": round(avg_ms, 5), "launches": launches,
+                        "algorithmic_bytes": int(nbytes),
+                        "gbs": round(nbytes / (avg_ms * 1e-3) / 1e9, 1) if avg_ms > 0 else None}
+    dominant = max(stages, key=lambda k: stages[k]["avg_ms"])
+    dom = stages[dominant]
+    step_bytes = sum(s["algorithmic_bytes"] for s in stages.values())
+    log(f"[bench] per-stage: {json.dumps(stages)}")
+
+    # achievable bandwidth of a plain copy
+    copy_bytes = 1 << 30
+    copy_ms = device_copy_ms(copy_bytes, 10)
+    copy_gbs = 2 * copy_bytes / (copy_ms * 1e-3) / 1e9
+
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"traffic_{dominant}_{args.workload}.txt")
+    if os.path.exists(tfile):
+        for line in open(tfile):
+            if line.startswith("bytes_per_launch_per_particle"):
+                traffic = int(float(line.split("=")[1]) * n_step)
+
+    roofline = {"bound": "hbm", "achieved": dom["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(dom["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                "algorithmic_bytes_per_launch": dom["algorithmic_bytes"], "avg_launch_ms": dom["avg_ms"],
+                "step_algorithmic_bytes": step_bytes,
+                "step_gbs": round(step_bytes / (ms_per_step * 1e-3) / 1e9, 1),
+                "achievable_copy_gbs": round(copy_gbs, 1), "stages": stages}
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log(f"[bench] CPU baseline ({args.cpu_seconds}s budget)")
+        cpu = cpu_baseline(case, eng, args.cpu_seconds)
+        log(f"[bench] CPU: {cpu['value']:.4e} particle-steps/s")
+    eng.close()
+
+    desc.update({"particles": n_now, "warmup_steps": args.warmup, "parallelism": "single GPU"})
+    out = {"metric": "particle-steps/s on 1 GPU", "value": round(value, 1), "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (uniform freestream fill, flow developed by the warm-up steps)",
+           "config": desc, "roofline": roofline, "cpu_baseline": cpu,
+           "counters": cnt}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -64,9 +64,10 @@ def _disk_rect_area(x0, x1, y0, y1, R):
                     pts.add(x)
     pts = sorted(pts)
 
-    def G(x):  # antiderivative of sqrt(R^2 - x^2)
+    def G(x):  # antiderivative of sqrt(R^2 - x^2), well conditioned near |x| = R
         x = min(max(x, -R), R)
-        return 0.5 * (x * math.sqrt(max(R * R - x * x, 0.0)) + R * R * math.asin(x / R))
+        s = math.sqrt(max((R - x) * (R + x), 0.0))
+        return 0.5 * (x * s + R * R * math.atan2(x, s))
 
     area = 0.0
     for lo, hi in zip(pts[:-1], pts[1:]):
@@ -82,7 +83,7 @@ def _disk_rect_area(x0, x1, y0, y1, R):
         c = (0.0 if top_is_h else y1) - (0.0 if bot_is_h else y0)
         k = (1.0 if top_is_h else 0.0) + (1.0 if bot_is_h else 0.0)
         area += c * (hi - lo) + k * (G(hi) - G(lo))
-    return area
+    return min(max(area, 0.0), (x1 - x0) * (y1 - y0))
 
 
 def cell_volumes(nx, ny, nz, origin, delta, cyl=None):

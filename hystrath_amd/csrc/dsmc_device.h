@@ -29,7 +29,7 @@ enum { RNG_MOVE = 1, RNG_COLLIDE = 2, RNG_INJECT = 3, RNG_FILL_COUNT = 4,
 
 // counters[] slots
 enum { CNT_CAND = 0, CNT_COLL = 1, CNT_DEL = 2, CNT_INJ = 3, CNT_WALL = 4,
-       CNT_TRACK_OVF = 5, CNT_TILE_OVF = 6, CNT_CAP_OVF = 7, CNT_N = 8 };
+       CNT_TRACK_OVF = 5, CNT_TILE_OVF = 6, CNT_CAP_OVF = 7, CNT_PSTEPS = 8, CNT_N = 9 };
 
 // Everything a kernel needs about the case, resident in device memory and
 // read through uniform (scalar) loads.

@@ -733,6 +733,18 @@ extern "C" int hs_dsmc_advance_step(hs_dsmc_engine* eng) {
     return hs_dsmc_set_step(eng, s + 1);
 }
 
+extern "C" int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    if ((flags & HS_FLAG_HEATFLUX) && !(eng->cfg.flags & HS_FLAG_HEATFLUX))
+        return fail(eng, HS_E_INVALID, "heat-flux sums can only be enabled at create time");
+    HS_TRY(hipSetDevice(eng->device));
+    eng->cfg.flags = flags;
+    eng->hp.flags = flags;
+    HS_TRY(hipMemcpyAsync(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
 extern "C" int hs_dsmc_synchronize(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
@@ -891,5 +903,41 @@ extern "C" int hs_dsmc_sigma_tcr(hs_dsmc_engine* eng, int64_t n, const int32_t* 
     ck(hipStreamSynchronize(st));
     if (ce != hipSuccess && !rc) rc = fail(eng, HS_E_HIP, std::string("sigma_tcr: ") + hipGetErrorString(ce));
     (void)hipFree(dtp); (void)hipFree(dtq); (void)hipFree(dup); (void)hipFree(duq); (void)hipFree(dout);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// device copy: achievable HBM bandwidth of a plain read+write stream
+// ---------------------------------------------------------------------------
+extern "C" int hs_dsmc_device_copy(int64_t bytes, int32_t iters, double* ms_out) {
+    if (bytes <= 0 || bytes % 32 != 0 || iters <= 0 || !ms_out) return fail(nullptr, HS_E_INVALID, "bad copy size");
+    void *a = nullptr, *b = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc(&a, bytes) != hipSuccess || hipMalloc(&b, bytes) != hipSuccess ||
+        hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess || hipEventCreate(&e0) != hipSuccess ||
+        hipEventCreate(&e1) != hipSuccess) {
+        rc = fail(nullptr, HS_E_HIP, "device copy: allocation failed");
+    } else {
+        int dev = 0, ncu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+            ncu = prop.multiProcessorCount;
+        launch_fill_f64((double*)a, 1.0, bytes / 8, st);
+        launch_copy(a, b, bytes, ncu * 16, st);  // warm
+        (void)hipEventRecord(e0, st);
+        for (int k = 0; k < iters; ++k) launch_copy(a, b, bytes, ncu * 16, st);
+        (void)hipEventRecord(e1, st);
+        float ms = 0.0f;
+        if (hipStreamSynchronize(st) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+            rc = fail(nullptr, HS_E_HIP, "device copy failed");
+        *ms_out = (double)ms / iters;
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
     return rc;
 }

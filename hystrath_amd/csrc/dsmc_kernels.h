@@ -72,3 +72,4 @@ void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, con
                            int32_t nu, int32_t* used, hipStream_t st);
 void launch_sigma_tcr(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, const double* up,
                       const double* uq, double* out, hipStream_t st);
+void launch_copy(const void* a, void* b, int64_t bytes, int grid, hipStream_t st);

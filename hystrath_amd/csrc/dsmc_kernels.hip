@@ -404,6 +404,7 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
             else b.cell[i] = -1;
         }
     }
+    if (do_move && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S->counters[CNT_PSTEPS], (unsigned long long)n);
     add_counter(S, CNT_DEL, mc.del);
     add_counter(S, CNT_WALL, mc.wall);
     add_counter(S, CNT_TRACK_OVF, mc.ovf);
@@ -937,4 +938,14 @@ void launch_sigma_tcr(const DevParams* P, int64_t n, const int32_t* tp, const in
                       const double* uq, double* out, hipStream_t st) {
     if (n <= 0) return;
     k_sigma_tcr<<<(int)((n + BLOCK - 1) / BLOCK), BLOCK, 0, st>>>(P, n, tp, tq, up, uq, out);
+}
+
+// ---------------------------------------------------------------------------
+// plain device copy (HBM bandwidth and counter calibration)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(BLOCK) void k_copy(const double4* __restrict__ a, double4* __restrict__ b, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) b[i] = a[i];
+}
+void launch_copy(const void* a, void* b, int64_t bytes, int grid, hipStream_t st) {
+    k_copy<<<grid, BLOCK, 0, st>>>((const double4*)a, (double4*)b, bytes / 32);
 }

@@ -24,12 +24,12 @@ EXPORTS = [
     "hs_dsmc_fill", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
     "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples",
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
-    "hs_dsmc_sigma_tcr",
+    "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide"]
 COUNTERS = ["candidates", "collisions", "deleted", "injected", "wall_hits", "track_overflow", "tile_overflow",
-            "capacity_overflow"]
+            "capacity_overflow", "particle_steps"]
 
 _lib = None
 
@@ -85,6 +85,8 @@ def load_library(path=LIB_PATH):
                                     C.c_int32, P(C.c_int32)], C.c_int),
         "hs_dsmc_sigma_tcr": ([eng, C.c_int64, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double),
                                P(C.c_double)], C.c_int),
+        "hs_dsmc_device_copy": ([C.c_int64, C.c_int32, P(C.c_double)], C.c_int),
+        "hs_dsmc_set_flags": ([eng, C.c_int32], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -98,6 +100,16 @@ def device_count():
     n = C.c_int(0)
     load_library().hs_dsmc_device_count(C.byref(n))
     return n.value
+
+
+def device_copy_ms(nbytes, iters=20):
+    """Mean ms of a plain device copy of nbytes (read nbytes + write nbytes)."""
+    L = load_library()
+    ms = C.c_double()
+    rc = L.hs_dsmc_device_copy(nbytes, iters, C.byref(ms))
+    if rc != 0:
+        raise EngineError(f"device copy failed ({rc}): {L.hs_dsmc_create_error().decode()}")
+    return ms.value
 
 
 def _d(a):
@@ -235,6 +247,10 @@ class Engine:
     def collide(self, sample=True):
         self._ck(self.L.hs_dsmc_collide(self.h, 1 if sample else 0), "collide")
 
+    def set_flags(self, flags):
+        self._ck(self.L.hs_dsmc_set_flags(self.h, flags), "set_flags")
+        self.case.cfg.flags = flags
+
     def synchronize(self):
         self._ck(self.L.hs_dsmc_synchronize(self.h), "synchronize")
 
@@ -249,8 +265,8 @@ class Engine:
         self._ck(self.L.hs_dsmc_reset_samples(self.h), "reset_samples")
 
     def counters(self):
-        out = np.zeros(8, np.int64)
-        self._ck(self.L.hs_dsmc_read_counters(self.h, abi.ptr(out, C.c_int64), 8), "read_counters")
+        out = np.zeros(len(COUNTERS), np.int64)
+        self._ck(self.L.hs_dsmc_read_counters(self.h, abi.ptr(out, C.c_int64), len(COUNTERS)), "read_counters")
         return dict(zip(COUNTERS, out.tolist()))
 
     def reset_counters(self):

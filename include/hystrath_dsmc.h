@@ -220,6 +220,10 @@ int hs_dsmc_sort(hs_dsmc_engine* eng);                   /* buildCellOccupancy  
 int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample); /* noTimeCounter::collide (+ sampling) */
 int hs_dsmc_advance_step(hs_dsmc_engine* eng);           /* step counter += 1 */
 int hs_dsmc_synchronize(hs_dsmc_engine* eng);
+/* Switch HS_FLAG_SAMPLE / HS_FLAG_COLL_SEP / HS_FLAG_TIMING between steps
+ * (the sampling schedule of timeDataMeas, timeDataMeas.C:71-104);
+ * HS_FLAG_HEATFLUX can only be kept as it was at create time. */
+int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
 
 /* ---- sampling ----------------------------------------------------------- */
 /* dsmcVolFields cumulative sums, plane-major [slot][cell]; nsteps_out gets
@@ -227,15 +231,23 @@ int hs_dsmc_synchronize(hs_dsmc_engine* eng);
 int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
 /* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290) */
 int hs_dsmc_reset_samples(hs_dsmc_engine* eng);
-/* counters of the last evolve: [0] candidates, [1] collisions, [2] deleted,
- * [3] injected, [4] wall hits, [5] tracking-iteration overflows,
- * [6] cell-tile overflows */
+/* counters since the last reset: [0] candidates, [1] collisions,
+ * [2] deleted, [3] injected, [4] wall hits, [5] tracking-iteration
+ * overflows, [6] oversize cells / candidate clamps, [7] particles dropped at
+ * capacity, [8] particle-steps moved */
 int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n);
 int hs_dsmc_reset_counters(hs_dsmc_engine* eng);
 /* per-kernel device time (ms, summed over launches since last reset) and
  * launch counts, in the order inject, move, scan, scatter, collide;
  * requires HS_FLAG_TIMING */
 int hs_dsmc_kernel_times(hs_dsmc_engine* eng, double* ms, int64_t* launches, int32_t n);
+
+/* ---- calibration -------------------------------------------------------- */
+/* No counterpart in the reference: times a plain device-to-device copy of
+ * `bytes` (multiple of 32) on the current device, iters times, and returns
+ * the mean milliseconds per copy.  Used to state the achievable HBM
+ * bandwidth beside the nominal one and to scale profiler byte counters. */
+int hs_dsmc_device_copy(int64_t bytes, int32_t iters, double* ms_out);
 
 /* ---- binary collision test hook ----------------------------------------- */
 /* BinaryCollisionModel::collide (VariableHardSphere.C:125-228,
