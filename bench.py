@@ -71,14 +71,16 @@ def build_case(workload):
     return case, desc, fill
 
 
-def run_evolve(eng, nsteps, chunk=250, what="steps"):
+def run_evolve(eng, nsteps, chunk=50, what="steps"):
+    """Enqueue nsteps; synchronise after each chunk of steps that is in flight
+    for more than ~10 s so progress is reported while the GPU works."""
     done = 0
     t0 = time.time()
     while done < nsteps:
         k = min(chunk, nsteps - done)
         eng.evolve(k)
         done += k
-        if time.time() - t0 > 20:
+        if time.time() - t0 > 10:
             eng.synchronize()
             log(f"  {what}: {done}/{nsteps}")
             t0 = time.time()

@@ -52,6 +52,8 @@ struct hs_dsmc_engine {
     int32_t* d_perm = nullptr;
     uint16_t* d_big_list = nullptr;
     uint8_t* d_big_sub = nullptr;
+    int32_t* d_tile_begin = nullptr;
+    int tile_parcels = 1536;  // parcels per collide workgroup (particle-balanced tiles)
     DevFaces faces;
     int32_t* d_face_off = nullptr;
     DevScalars* d_S = nullptr;
@@ -376,6 +378,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         (rc = dev_alloc(eng, &eng->d_count, nc)) || (rc = dev_alloc(eng, &eng->d_start, (int64_t)nc + 1)) ||
         (rc = dev_alloc(eng, &eng->d_slot, eng->cap)) || (rc = dev_alloc(eng, &eng->d_perm, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_big_list, eng->cap)) || (rc = dev_alloc(eng, &eng->d_big_sub, eng->cap)) ||
+        (rc = dev_alloc(eng, &eng->d_tile_begin, eng->cap / 64 + 4)) ||
         (rc = dev_alloc(eng, &eng->d_S, 1)))
         return bail(rc);
     // inflow faces
@@ -672,9 +675,10 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     {
         StageTimer tm(eng, T_COLLIDE);
-        launch_collide(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_sigma, eng->d_rem,
-                       eng->d_samples, eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc,
-                       eng->d_big_list, eng->d_big_sub, eng->stream);
+        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->stream);
+        launch_collide(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
+                       eng->tile_parcels, eng->cap, eng->d_sigma, eng->d_rem, eng->d_samples, eng->d_vol, eng->d_S,
+                       do_collide, do_sample, eng->general, eng->nc, eng->d_big_list, eng->d_big_sub, eng->stream);
     }
     launch_commit(eng->d_S, advance, eng->stream);
     eng->cur = 1 - eng->cur;

@@ -53,10 +53,11 @@ void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* cou
 void launch_set_nold(DevScalars* S, hipStream_t st);
 void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
                     int32_t* perm, int grid, hipStream_t st);
+void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, hipStream_t st);
 void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
-                    const int32_t* perm, double* sigma, double* rem, double* samples, const double* vol,
-                    DevScalars* S, int do_collide, int do_sample, int general, int nc,
-                    uint16_t* big_list, uint8_t* big_sub, hipStream_t st);
+                    const int32_t* perm, const int32_t* tile_begin, int pt, int64_t capacity, double* sigma,
+                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
+                    int general, int nc, uint16_t* big_list, uint8_t* big_sub, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, hipStream_t st);
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
                        int32_t* cnt, int nc, int ns, hipStream_t st);

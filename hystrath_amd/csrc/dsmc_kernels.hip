@@ -389,19 +389,45 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
     else b.cell[i] = idx[0] + P.nx * (idx[1] + P.ny * idx[2]);
 }
 
+// Histogram slot of each particle.  Particles are nearly cell-sorted, so the
+// lanes of a wave form runs of equal cells: one atomic per run returns the
+// run's base slot and each lane adds its rank in the run.
+__device__ __forceinline__ void hist_slots(int32_t c, bool valid, int64_t i, int lane, int32_t* __restrict__ count,
+                                           int32_t* __restrict__ slot) {
+    const int32_t prev = __shfl_up(c, 1, WAVE);
+    const bool head = (lane == 0) || (prev != c);
+    const unsigned long long heads = __ballot(head);
+    const unsigned long long upto = lane == WAVE - 1 ? ~0ull : ((2ull << lane) - 1ull);
+    const int leader = 63 - __clzll((long long)(heads & upto));
+    const unsigned long long lead_mask = leader == WAVE - 1 ? ~0ull : ((2ull << leader) - 1ull);
+    const unsigned long long above = heads & ~lead_mask;
+    int base = 0;
+    if (head && c >= 0) {
+        const int next = above ? __ffsll((long long)above) - 1 : WAVE;
+        base = atomicAdd(&count[c], next - lane);
+    }
+    base = __shfl(base, leader, WAVE);
+    if (valid && c >= 0) slot[i] = base + (lane - leader);
+}
+
 __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
                                                 DevScalars* __restrict__ S, int32_t* __restrict__ count,
                                                 int32_t* __restrict__ slot, int do_move, int do_hist) {
     const DevParams& P = *Pp;
     const int64_t n = S->n, n_old = S->n_old;
     const uint32_t step = S->step;
+    const int lane = threadIdx.x & (WAVE - 1);
     MoveCounts mc = { 0, 0, 0 };
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
-        if (do_move) move_one(P, b, i, i >= n_old, step, mc);
+    // wave-uniform trip count: every lane of a wave sees the same bases
+    for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < n;
+         base += (int64_t)gridDim.x * BLOCK) {
+        const int64_t i = base + lane;
+        const bool valid = i < n;
+        if (valid && do_move) move_one(P, b, i, i >= n_old, step, mc);
         if (do_hist) {
-            const int32_t c = b.cell[i];
-            if (c >= 0 && c < P.nc) slot[i] = atomicAdd(&count[c], 1);
-            else b.cell[i] = -1;
+            int32_t c = valid ? b.cell[i] : -1;
+            if (valid && c >= P.nc) { b.cell[i] = -1; c = -1; }
+            hist_slots(c, valid, i, lane, count, slot);
         }
     }
     if (do_move && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S->counters[CNT_PSTEPS], (unsigned long long)n);
@@ -440,7 +466,9 @@ void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, c
 // collide: stable gather into cell order, NTC, binary collisions, sampling
 // (noTimeCounter.C:81-339, dsmcVolFields.C:1115-1239)
 // ---------------------------------------------------------------------------
-#define TILE_CELLS 64
+#define CT 256           // collide workgroup: 4 waves
+#define CT_CELLS 256     // cells per sub-tile: one thread per cell
+#define DENSE_CAND 32    // cells with more candidates use the wave-cooperative NTC
 
 // 8 sub-cell counters packed as 16-bit fields in two 64-bit words
 struct Pack8 {
@@ -453,95 +481,252 @@ struct Pack8 {
         if (s < 4) lo += 1ull << (16 * s);
         else hi += 1ull << (16 * (s - 4));
     }
+    __device__ __forceinline__ void set(int s, int v) {
+        if (s < 4) lo = (lo & ~(0xFFFFull << (16 * s))) | ((unsigned long long)v << (16 * s));
+        else hi = (hi & ~(0xFFFFull << (16 * (s - 4)))) | ((unsigned long long)v << (16 * (s - 4)));
+    }
 };
 
 struct CellCtx {
     unsigned long long cand, colls, ovf;
 };
 
-// NTC over one cell whose particles sit at ss..ss+nC-1 of the given arrays
-// (LDS tile or, for an oversize cell, global memory).
+// Candidate count of noTimeCounter.C:142-153; updates the remainder.
+__device__ __forceinline__ long long ntc_candidates(const DevParams& P, int c, int nC, const double* sigma,
+                                                   double* rem, const double* vol, double* smax_out, CellCtx& cx) {
+    const double smax = sigma[c];
+    const double sel = rem[c] + 0.5 * nC * (nC - 1) * P.fn * smax * P.dt / vol[c];
+    long long ncand = (long long)sel;
+    rem[c] = sel - (double)ncand;
+    if (ncand > HS_MAX_CAND) { ncand = HS_MAX_CAND; cx.ovf += 1; }
+    *smax_out = smax;
+    return ncand;
+}
+
+// Draws the pair of candidate k (its own stream) and reports whether it may
+// collide: noTimeCounter.C:157-228.
 template <bool GEN>
-__device__ __forceinline__ void ntc_cell(const DevParams& P, int c, int ss, int nC, float* vx, float* vy, float* vz,
-                                         const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
-                                         const uint8_t* ty, double* sigma, double* rem, const double* vol,
-                                         uint32_t step, const DevParticles& pos, int64_t gs, CellCtx& cx,
-                                         double& ncoll, double& sep) {
-    if (nC <= 1) return;
-    Pack8 cnt;
+__device__ __forceinline__ bool draw_pair(const DevParams& P, PhiloxStream& r, int nC, const uint8_t* sub,
+                                          const uint16_t* slist, int ss, const Pack8& cnt, const Pack8& off,
+                                          const uint8_t* ty, int* pp_out, int* qq_out) {
+    const int pp = random_label(r, 0, nC - 1);
+    const int q = sub[ss + pp];
+    const int nsc = cnt.get(q);
+    int qq = -1, guard = 0;
+    if (nsc > 1) {
+        const int o = off.get(q);
+        do { qq = slist[ss + o + random_label(r, 0, nsc - 1)]; } while (pp == qq && ++guard < HS_MAX_REDRAW);
+    } else {
+        do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
+    }
+    *pp_out = pp;
+    *qq_out = qq;
+    if (pp == qq) return false;
+    const int tp = GEN ? (int)ty[ss + pp] : 0, tq = GEN ? (int)ty[ss + qq] : 0;
+    return !(P.charge[tp] == -1 && P.charge[tq] == -1);
+}
+
+// The accepted collision of candidate (pp, qq): BinaryCollisionModel::collide.
+template <bool GEN>
+__device__ __forceinline__ void do_collision(const DevParams& P, PhiloxStream& r, int ip, int iq, double up[3],
+                                             double uq[3], float* vx, float* vy, float* vz, float* er, uint16_t* vb,
+                                             int vstride, const uint8_t* ty) {
+    const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
+    double erp = 0.0, erq = 0.0;
+    int vp[HS_MAX_VIB_MODES] = { 0, 0, 0 }, vq[HS_MAX_VIB_MODES] = { 0, 0, 0 };
+    if (GEN) {
+        if (P.has_erot) { erp = (double)er[ip]; erq = (double)er[iq]; }
+        for (int m = 0; m < P.nvib[tp]; ++m) vp[m] = vb[m * vstride + ip];
+        for (int m = 0; m < P.nvib[tq]; ++m) vq[m] = vb[m * vstride + iq];
+    }
+    binary_collide(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
+    vx[ip] = (float)up[0]; vy[ip] = (float)up[1]; vz[ip] = (float)up[2];
+    vx[iq] = (float)uq[0]; vy[iq] = (float)uq[1]; vz[iq] = (float)uq[2];
+    if (GEN) {
+        if (P.has_erot) { er[ip] = (float)erp; er[iq] = (float)erq; }
+        for (int m = 0; m < P.nvib[tp]; ++m) vb[m * vstride + ip] = (uint16_t)vp[m];
+        for (int m = 0; m < P.nvib[tq]; ++m) vb[m * vstride + iq] = (uint16_t)vq[m];
+    }
+}
+
+// Sub-cell lists of one cell built by one lane (ascending index per sub-cell).
+__device__ __forceinline__ void subcell_lists_lane(const uint8_t* sub, uint16_t* slist, int ss, int nC, Pack8& cnt,
+                                                   Pack8& off) {
     cnt.zero();
     for (int k = 0; k < nC; ++k) cnt.inc(sub[ss + k]);
-    Pack8 off, fill;
     off.zero();
-    {
-        int acc = 0;
-        for (int q = 0; q < 8; ++q) {
-            if (q < 4) off.lo |= (unsigned long long)acc << (16 * q);
-            else off.hi |= (unsigned long long)acc << (16 * (q - 4));
-            acc += cnt.get(q);
-        }
-    }
-    fill = off;
+    int acc = 0;
+    for (int q = 0; q < 8; ++q) { off.set(q, acc); acc += cnt.get(q); }
+    Pack8 fill = off;
     for (int k = 0; k < nC; ++k) {
         const int q = sub[ss + k];
         slist[ss + fill.get(q)] = (uint16_t)k;
         fill.inc(q);
     }
-    PhiloxStream r;
-    r.init(P, RNG_COLLIDE, (uint32_t)c, step, 0);
-    const double smax = sigma[c];
+}
+
+// NTC of one cell by one lane, candidate after candidate.
+template <bool GEN>
+__device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
+                         float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
+                         const uint8_t* ty, double* sigma, uint32_t step, const DevParticles& pos, int64_t gs,
+                         CellCtx& cx, double& ncoll, double& sep) {
+    Pack8 cnt, off;
+    subcell_lists_lane(sub, slist, ss, nC, cnt, off);
     double smax_cur = smax;
-    const double sel = rem[c] + 0.5 * nC * (nC - 1) * P.fn * smax * P.dt / vol[c];
-    long long ncand = (long long)sel;
-    rem[c] = sel - (double)ncand;
-    if (ncand > HS_MAX_CAND) { ncand = HS_MAX_CAND; cx.ovf += 1; }
     cx.cand += (unsigned long long)ncand;
     for (long long k = 0; k < ncand; ++k) {
-        const int pp = random_label(r, 0, nC - 1);
-        const int q = sub[ss + pp];
-        const int nsc = cnt.get(q);
-        int qq = -1, guard = 0;
-        if (nsc > 1) {
-            const int o = off.get(q);
-            do { qq = slist[ss + o + random_label(r, 0, nsc - 1)]; } while (pp == qq && ++guard < HS_MAX_REDRAW);
-        } else {
-            do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
-        }
-        if (pp == qq) continue;
+        PhiloxStream r;
+        r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
+        int pp, qq;
+        if (!draw_pair<GEN>(P, r, nC, sub, slist, ss, cnt, off, ty, &pp, &qq)) continue;
         const int ip = ss + pp, iq = ss + qq;
         const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
-        if (P.charge[tp] == -1 && P.charge[tq] == -1) continue;
         double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
         double uq[3] = { (double)vx[iq], (double)vy[iq], (double)vz[iq] };
         const double stcr = sigma_tcr(P, tp, tq, up, uq);
         if (stcr > smax_cur) smax_cur = stcr;
         if (stcr / smax > r.u()) {
-            double erp = 0.0, erq = 0.0;
-            int vp[HS_MAX_VIB_MODES] = { 0, 0, 0 }, vq[HS_MAX_VIB_MODES] = { 0, 0, 0 };
-            if (GEN) {
-                if (P.has_erot) { erp = (double)er[ip]; erq = (double)er[iq]; }
-                for (int m = 0; m < P.nvib[tp]; ++m) vp[m] = vb[m * vstride + ip];
-                for (int m = 0; m < P.nvib[tq]; ++m) vq[m] = vb[m * vstride + iq];
-            }
-            binary_collide(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
-            vx[ip] = (float)up[0]; vy[ip] = (float)up[1]; vz[ip] = (float)up[2];
-            vx[iq] = (float)uq[0]; vy[iq] = (float)uq[1]; vz[iq] = (float)uq[2];
-            if (GEN) {
-                if (P.has_erot) { er[ip] = (float)erp; er[iq] = (float)erq; }
-                for (int m = 0; m < P.nvib[tp]; ++m) vb[m * vstride + ip] = (uint16_t)vp[m];
-                for (int m = 0; m < P.nvib[tq]; ++m) vb[m * vstride + iq] = (uint16_t)vq[m];
-            }
+            do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
             ncoll += 1.0;
             if (P.flags & HS_FLAG_COLL_SEP) {
                 const int64_t gp = gs + pp, gq = gs + qq;
-                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq],
-                             ddz = pos.pz[gp] - pos.pz[gq];
+                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
                 sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
             }
             cx.colls += 1;
         }
     }
     sigma[c] = smax_cur;
+}
+
+// NTC of one dense cell by a whole wave.  Each batch evaluates the next 64
+// candidates against the current velocities; an accepted candidate stamps its
+// two parcels (LDS atomicMin of the lane id) and the first candidate touching
+// a parcel stamped by an earlier lane ends the batch.  Everything before it
+// is committed, which is exactly what the candidate-by-candidate order
+// produces: their parcels were not changed by any earlier accepted candidate.
+template <bool GEN>
+__device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
+                         float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
+                         const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step, const DevParticles& pos,
+                         int64_t gs, CellCtx& cx, double& ncoll_out, double& sep_out) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+    // sub-cell counts (lane q holds the count of sub-cell q)
+    int myc = 0;
+    for (int base = 0; base < nC; base += WAVE) {
+        const int j = base + lane;
+        const int q = j < nC ? (int)sub[ss + j] : 8;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const int pc = __popcll(__ballot(q == qq));
+            if (lane == qq) myc += pc;
+        }
+    }
+    Pack8 cnt, off;
+    cnt.zero();
+    off.zero();
+    {
+        int acc = 0;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const int cq = __shfl(myc, qq, WAVE);
+            cnt.set(qq, cq);
+            off.set(qq, acc);
+            acc += cq;
+        }
+    }
+    int run[8];
+#pragma unroll
+    for (int qq = 0; qq < 8; ++qq) run[qq] = off.get(qq);
+    for (int base = 0; base < nC; base += WAVE) {
+        const int j = base + lane;
+        const int q = j < nC ? (int)sub[ss + j] : 8;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const unsigned long long m = __ballot(q == qq);
+            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (uint16_t)j;
+            run[qq] += __popcll(m);
+        }
+    }
+    for (int j = lane; j < nC; j += WAVE) stamp[ss + j] = 0x7FFFFFFF;
+    double smax_cur = smax;
+    double ncoll = 0.0, sep = 0.0;
+    unsigned long long ncoll_i = 0;
+    for (long long k0 = 0; k0 < ncand;) {
+        const long long k = k0 + lane;
+        const bool active = k < ncand;
+        int pp = -1, qq = -1;
+        bool valid = false, acc = false;
+        double stcr = 0.0;
+        double up[3] = { 0.0, 0.0, 0.0 }, uq[3] = { 0.0, 0.0, 0.0 };
+        PhiloxStream r;
+        if (active) {
+            r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
+            valid = draw_pair<GEN>(P, r, nC, sub, slist, ss, cnt, off, ty, &pp, &qq);
+            if (valid) {
+                const int ip = ss + pp, iq = ss + qq;
+                const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
+                up[0] = vx[ip]; up[1] = vy[ip]; up[2] = vz[ip];
+                uq[0] = vx[iq]; uq[1] = vy[iq]; uq[2] = vz[iq];
+                stcr = sigma_tcr(P, tp, tq, up, uq);
+                acc = stcr / smax > r.u();
+            }
+        }
+        if (acc) {
+            atomicMin(&stamp[ss + pp], lane);
+            atomicMin(&stamp[ss + qq], lane);
+        }
+        __builtin_amdgcn_wave_barrier();
+        bool conf = false;
+        if (valid) conf = stamp[ss + pp] < lane || stamp[ss + qq] < lane;
+        const unsigned long long cm = __ballot(conf);
+        const long long left = ncand - k0;
+        int fc = cm ? __ffsll((long long)cm) - 1 : WAVE;
+        if ((long long)fc > left) fc = (int)left;
+        __builtin_amdgcn_wave_barrier();
+        if (acc) {
+            stamp[ss + pp] = 0x7FFFFFFF;
+            stamp[ss + qq] = 0x7FFFFFFF;
+        }
+        const bool commit = active && lane < fc;
+        // running maximum of sigmaTcR over the committed candidates
+        double sm = (commit && valid) ? stcr : 0.0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) sm = fmax(sm, __shfl_xor(sm, o, WAVE));
+        if (sm > smax_cur) smax_cur = sm;
+        const bool doit = commit && acc;
+        double sep_l = 0.0;
+        if (doit) {
+            const int ip = ss + pp, iq = ss + qq;
+            do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+            if (P.flags & HS_FLAG_COLL_SEP) {
+                const int64_t gp = gs + pp, gq = gs + qq;
+                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
+                sep_l = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            }
+        }
+        const unsigned long long dm = __ballot(doit);
+        ncoll_i += (unsigned long long)__popcll(dm);
+        if (P.flags & HS_FLAG_COLL_SEP) {
+            // add in candidate order, as the sequential loop does
+            for (int j = 0; j < fc; ++j) {
+                const double v = __shfl(sep_l, j, WAVE);
+                if ((dm >> j) & 1ull) sep += v;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        k0 += fc;
+    }
+    ncoll = (double)ncoll_i;
+    if (lane == 0) {
+        sigma[c] = smax_cur;
+        cx.cand += (unsigned long long)ncand;
+        cx.colls += ncoll_i;
+    }
+    ncoll_out = ncoll;
+    sep_out = sep;
 }
 
 // dsmcVolFields::calculateField per-cell sums for one cell
@@ -627,49 +812,50 @@ __device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, d
 }
 
 template <bool GEN, int PMAX>
-__global__ __launch_bounds__(WAVE) void k_collide(const DevParams* __restrict__ Pp, DevParticles src,
-                                                  DevParticles dst, const int32_t* __restrict__ start,
-                                                  const int32_t* __restrict__ perm, double* __restrict__ sigma,
-                                                  double* __restrict__ rem, double* __restrict__ samples,
-                                                  const double* __restrict__ vol, DevScalars* __restrict__ S,
-                                                  int do_collide, int do_sample, int nc, uint16_t* big_list,
-                                                  uint8_t* big_sub) {
-    __shared__ int32_t key[PMAX];
-    __shared__ uint8_t loc[PMAX];
-    __shared__ float vx[PMAX], vy[PMAX], vz[PMAX];
-    __shared__ uint8_t sub[PMAX];
-    __shared__ uint16_t slist[PMAX];
-    __shared__ float er[GEN ? PMAX : 1];
-    __shared__ uint16_t vb[GEN ? HS_MAX_VIB_MODES * PMAX : 1];
-    __shared__ uint8_t ty[GEN ? PMAX : 1];
-    __shared__ int32_t segS[TILE_CELLS], segE[TILE_CELLS];
+struct TileLds {
+    int32_t key[PMAX];       // old index of each slot; conflict stamps of the dense NTC
+    uint8_t loc[PMAX];       // sub-tile cell of each slot
+    uint8_t sub[PMAX];
+    uint16_t slist[PMAX];
+    float vx[PMAX], vy[PMAX], vz[PMAX];
+    float er[GEN ? PMAX : 1];
+    uint16_t vb[GEN ? HS_MAX_VIB_MODES * PMAX : 1];
+    uint8_t ty[GEN ? PMAX : 1];
+    int32_t segS[CT_CELLS + 1];
+    double ncoll[CT_CELLS], sep[CT_CELLS];
+    long long ncand[CT_CELLS];
+    double smax[CT_CELLS];
+    int32_t dense[CT_CELLS];
+    int32_t ndense;
+};
 
-    const DevParams& P = *Pp;
-    const int lane = threadIdx.x;
-    const int c0 = blockIdx.x * TILE_CELLS;
-    if (c0 >= nc) return;
-    const int ncell = min(TILE_CELLS, nc - c0);
-    const uint32_t step = S->step;
+// Gather, collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS)
+// in LDS chunks of at most PMAX parcels.
+template <bool GEN, int PMAX>
+__device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int ncell, DevParticles& src,
+                              DevParticles& dst, const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                              double* __restrict__ sigma, double* __restrict__ rem, double* __restrict__ samples,
+                              const double* __restrict__ vol, uint32_t step, int do_collide, int do_sample, int nc,
+                              uint16_t* big_list, uint8_t* big_sub, CellCtx& cx) {
+    const int t = threadIdx.x;
+    const int wid = t / WAVE, lane = t & (WAVE - 1);
     const int64_t cap = P.capacity;
-    int s_l = 0, e_l = 0;
-    if (lane < ncell) { s_l = start[c0 + lane]; e_l = start[c0 + lane + 1]; }
-    CellCtx cx = { 0, 0, 0 };
+    if (t < ncell) L.segS[t] = start[c0 + t];
+    if (t == 0) L.segS[ncell] = start[c0 + ncell];
+    __syncthreads();
     int a = 0;
     while (a < ncell) {
-        const int S0 = __shfl(s_l, a, WAVE);
-        const bool ok = (lane >= a) && (lane < ncell) && (e_l - S0 <= PMAX);
-        const unsigned long long bal = __ballot(ok);
-        const int b = a + __popcll(bal);
+        const int S0 = L.segS[a];
+        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX;
+        const int b = a + __syncthreads_count(ok);
         if (b == a) {
-            // one cell larger than the tile: stable order and collisions in
-            // global memory
-            const int nC = e_l - s_l;  // valid on lane a only
-            const int nCa = __shfl(nC, a, WAVE);
+            // one cell larger than the tile: stable order and NTC in global memory
+            const int nC = L.segS[a + 1] - S0;
             const int c = c0 + a;
-            for (int j = lane; j < nCa; j += WAVE) {
+            for (int j = t; j < nC; j += CT) {
                 const int k = perm[S0 + j];
                 int rank = 0;
-                for (int q = 0; q < nCa; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+                for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
                 const int64_t d = (int64_t)S0 + rank;
                 const double x = src.px[k], y = src.py[k], z = src.pz[k];
                 dst.px[d] = x; dst.py[d] = y; dst.pz[d] = z;
@@ -682,98 +868,168 @@ __global__ __launch_bounds__(WAVE) void k_collide(const DevParams* __restrict__ 
                 big_sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
             }
             __syncthreads();
-            if (lane == a) {
+            if (t == 0) {
                 double ncoll = 0.0, sep = 0.0;
-                if (nCa > 65535) {
+                if (nC > 65535) {
                     cx.ovf += 1;
                 } else {
-                    if (do_collide)
-                        ntc_cell<GEN>(P, c, 0, nCa, dst.ux + S0, dst.uy + S0, dst.uz + S0, big_sub + S0,
-                                      big_list + S0, GEN && dst.erot ? dst.erot + S0 : nullptr,
-                                      GEN && dst.vib ? dst.vib + S0 : nullptr, (int)cap,
-                                      GEN ? dst.type + S0 : nullptr, sigma, rem, vol, step, dst, S0, cx, ncoll,
-                                      sep);
+                    float* gvx = dst.ux + S0;
+                    float* gvy = dst.uy + S0;
+                    float* gvz = dst.uz + S0;
+                    float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
+                    uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
+                    const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
+                    if (do_collide && nC > 1) {
+                        double smax;
+                        const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+                        ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, big_sub + S0, big_list + S0, ger, gvb,
+                                      (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
+                    }
                     if (do_sample)
-                        sample_cell<GEN>(P, c, 0, nCa, dst.ux + S0, dst.uy + S0, dst.uz + S0,
-                                         GEN && dst.erot ? dst.erot + S0 : nullptr,
-                                         GEN && dst.vib ? dst.vib + S0 : nullptr, (int)cap,
-                                         GEN ? dst.type + S0 : nullptr, samples, nc, ncoll, sep);
+                        sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
                 }
             }
             __syncthreads();
             a += 1;
             continue;
         }
-        const int E0 = __shfl(e_l, b - 1, WAVE);
+        const int E0 = L.segS[b];
         const int cnt = E0 - S0;
-        if (lane >= a && lane < b) {
-            segS[lane] = s_l - S0;
-            segE[lane] = e_l - S0;
-            for (int j = s_l - S0; j < e_l - S0; ++j) loc[j] = (uint8_t)lane;
-        }
-        for (int j = lane; j < cnt; j += WAVE) key[j] = perm[S0 + j];
+        // slot -> cell map and the old indices of the chunk
+        if (t >= a && t < b)
+            for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
+        for (int j = t; j < cnt; j += CT) L.key[j] = perm[S0 + j];
         __syncthreads();
-        for (int j = lane; j < cnt; j += WAVE) {
-            const int l = loc[j];
-            const int ss = segS[l], ee = segE[l];
-            const int k = key[j];
+        // stable order inside each cell (rank by old index) and gather
+        for (int j = t; j < cnt; j += CT) {
+            const int l = L.loc[j];
+            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
+            const int k = L.key[j];
             int rank = 0;
-            for (int q = ss; q < ee; ++q) rank += (key[q] < k) ? 1 : 0;
+            for (int q = ss; q < ee; ++q) rank += (L.key[q] < k) ? 1 : 0;
             const int d = ss + rank;
             const int c = c0 + l;
             const int64_t gd = (int64_t)S0 + d;
             const double x = src.px[k], y = src.py[k], z = src.pz[k];
             dst.px[gd] = x; dst.py[gd] = y; dst.pz[gd] = z;
             dst.cell[gd] = c;
-            sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
-            vx[d] = src.ux[k]; vy[d] = src.uy[k]; vz[d] = src.uz[k];
+            L.sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
+            L.vx[d] = src.ux[k]; L.vy[d] = src.uy[k]; L.vz[d] = src.uz[k];
             if (GEN) {
-                er[d] = src.erot ? src.erot[k] : 0.0f;
-                for (int m = 0; m < P.max_modes; ++m) vb[m * PMAX + d] = src.vib[m * cap + k];
-                ty[d] = src.type ? src.type[k] : (uint8_t)0;
+                L.er[d] = src.erot ? src.erot[k] : 0.0f;
+                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + d] = src.vib[m * cap + k];
+                L.ty[d] = src.type ? src.type[k] : (uint8_t)0;
             }
         }
+        if (t == 0) L.ndense = 0;
         __syncthreads();
-        if (lane >= a && lane < b) {
-            const int c = c0 + lane;
-            const int ss = s_l - S0, nC = e_l - s_l;
+        // candidates per cell; sparse cells run their NTC on one lane now
+        if (t >= a && t < b) {
+            const int c = c0 + t;
+            const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
             double ncoll = 0.0, sep = 0.0;
-            if (do_collide)
-                ntc_cell<GEN>(P, c, ss, nC, vx, vy, vz, sub, slist, er, vb, PMAX, ty, sigma, rem, vol, step, dst,
-                              (int64_t)S0 + ss, cx, ncoll, sep);
-            if (do_sample)
-                sample_cell<GEN>(P, c, ss, nC, vx, vy, vz, er, vb, PMAX, ty, samples, nc, ncoll, sep);
+            if (do_collide && nC > 1) {
+                double smax;
+                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+                if (ncand > DENSE_CAND) {
+                    L.ncand[t] = ncand;
+                    L.smax[t] = smax;
+                    L.dense[atomicAdd(&L.ndense, 1)] = t;
+                } else {
+                    ntc_lane<GEN>(P, c, ss, nC, ncand, smax, L.vx, L.vy, L.vz, L.sub, L.slist, L.er, L.vb, PMAX,
+                                  L.ty, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
+                }
+            }
+            L.ncoll[t] = ncoll;
+            L.sep[t] = sep;
         }
         __syncthreads();
-        for (int j = lane; j < cnt; j += WAVE) {
+        // dense cells: one wave each
+        const int nd = L.ndense;
+        for (int di = wid; di < nd; di += CT / WAVE) {
+            const int l = L.dense[di];
+            const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
+            double ncoll, sep;
+            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, L.slist, L.er, L.vb,
+                          PMAX, L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
+            if (lane == 0) { L.ncoll[l] = ncoll; L.sep[l] = sep; }
+        }
+        __syncthreads();
+        if (do_sample && t >= a && t < b) {
+            const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
+            sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc, L.ncoll[t],
+                             L.sep[t]);
+        }
+        for (int j = t; j < cnt; j += CT) {
             const int64_t gd = (int64_t)S0 + j;
-            dst.ux[gd] = vx[j]; dst.uy[gd] = vy[j]; dst.uz[gd] = vz[j];
+            dst.ux[gd] = L.vx[j]; dst.uy[gd] = L.vy[j]; dst.uz[gd] = L.vz[j];
             if (GEN) {
-                if (dst.erot) dst.erot[gd] = er[j];
+                if (dst.erot) dst.erot[gd] = L.er[j];
                 if (dst.vib)
-                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = vb[m * PMAX + j];
-                if (dst.type) dst.type[gd] = ty[j];
+                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
+                if (dst.type) dst.type[gd] = L.ty[j];
             }
         }
         __syncthreads();
         a = b;
     }
+}
+
+// Tile q of the particle-balanced decomposition owns the cells whose start
+// offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.
+__global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt,
+                                                 int32_t* __restrict__ tile_begin) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i > nc) return;
+    if (i == 0) { tile_begin[0] = 0; return; }
+    const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
+    for (int q = q0 + 1; q <= q1; ++q) tile_begin[q] = i;
+    if (i == nc) tile_begin[q1 + 1] = nc;
+}
+
+template <bool GEN, int PMAX>
+__global__ __launch_bounds__(CT) void k_collide(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
+                                                const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                                                const int32_t* __restrict__ tile_begin, int pt,
+                                                double* __restrict__ sigma, double* __restrict__ rem,
+                                                double* __restrict__ samples, const double* __restrict__ vol,
+                                                DevScalars* __restrict__ S, int do_collide, int do_sample, int nc,
+                                                uint16_t* big_list, uint8_t* big_sub) {
+    __shared__ TileLds<GEN, PMAX> L;
+    const DevParams& P = *Pp;
+    const int q = blockIdx.x;
+    const int64_t nsort = S->n_sorted;
+    if ((int64_t)q * pt > nsort) return;
+    const int cb = tile_begin[q], ce = tile_begin[q + 1];
+    if (cb >= ce) return;
+    const uint32_t step = S->step;
+    CellCtx cx = { 0, 0, 0 };
+    for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
+        collide_cells<GEN, PMAX>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sigma, rem, samples, vol,
+                                 step, do_collide, do_sample, nc, big_list, big_sub, cx);
     add_counter(S, CNT_CAND, cx.cand);
     add_counter(S, CNT_COLL, cx.colls);
     add_counter(S, CNT_TILE_OVF, cx.ovf);
 }
 
+void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, hipStream_t st) {
+    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin);
+}
+
+#define PMAX_MONO 3072
+#define PMAX_GEN 1792
+
 void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
-                    const int32_t* perm, double* sigma, double* rem, double* samples, const double* vol,
-                    DevScalars* S, int do_collide, int do_sample, int general, int nc, uint16_t* big_list,
-                    uint8_t* big_sub, hipStream_t st) {
-    const int nt = (nc + TILE_CELLS - 1) / TILE_CELLS;
+                    const int32_t* perm, const int32_t* tile_begin, int pt, int64_t capacity, double* sigma,
+                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
+                    int general, int nc, uint16_t* big_list, uint8_t* big_sub, hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
     if (general)
-        k_collide<true, 1024><<<nt, WAVE, 0, st>>>(P, src, dst, start, perm, sigma, rem, samples, vol, S,
-                                                    do_collide, do_sample, nc, big_list, big_sub);
+        k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, sigma, rem, samples,
+                                                      vol, S, do_collide, do_sample, nc, big_list, big_sub);
     else
-        k_collide<false, 1024><<<nt, WAVE, 0, st>>>(P, src, dst, start, perm, sigma, rem, samples, vol, S,
-                                                     do_collide, do_sample, nc, big_list, big_sub);
+        k_collide<false, PMAX_MONO><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, sigma, rem, samples,
+                                                        vol, S, do_collide, do_sample, nc, big_list, big_sub);
 }
 
 // publish the sorted population; optionally advance the step counter

@@ -24,7 +24,8 @@
  * Deliberate, documented differences from the reference (all of them change
  * which random numbers are used, never the distributions):
  *   - Random numbers come from Philox4x32-10 streams keyed by (seed, purpose,
- *     entity, step) instead of OpenFOAM's single sequential Random.
+ *     entity, step) instead of OpenFOAM's single sequential Random; each
+ *     NTC collision candidate has its own stream (cell, step, candidate).
  *   - Gaussians are Box-Muller pairs (second value cached) instead of the
  *     polar method; arguments of log are 1-u so they are never 0.
  *   - A freestream particle's face position is uniform on the rectangular
@@ -884,8 +885,6 @@ int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
                 memcpy(fill, sco, sizeof(fill));
                 for (int32_t k = 0; k < nC; ++k) sub_list[fill[which[k]]++] = k;
             }
-            or_rng r;
-            rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, 0);
             const double smax = s->sigma_tcr_max[c];
             const double sel = s->remainder[c]
                 + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
@@ -894,6 +893,9 @@ int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
             if (ncand > OR_MAX_CAND) { ncand = OR_MAX_CAND; s->counters[6] += 1; }
             s->counters[0] += ncand;
             for (int64_t k = 0; k < ncand; ++k) {
+                /* every candidate draws from its own stream (cell, step, k) */
+                or_rng r;
+                rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, (uint32_t)k);
                 const int pp = rng_label(&r, 0, nC - 1);
                 const int sc = which[pp];
                 const int nsc = scn[sc];

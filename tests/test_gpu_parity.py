@@ -171,15 +171,31 @@ def test_evolve_parity(which, steps):
     eng.close()
 
 
-def test_oversize_cell_parity():
-    """A cell holding more parcels than an LDS tile goes through the global
-    memory path; the result must still equal the oracle."""
-    case = cases.periodic_box(n_side=1, L=0.01, ppc=3000, seed=17)
+@pytest.mark.parametrize("ppc,steps", [(600, 4), (5000, 2)])
+def test_dense_and_oversize_cell_parity(ppc, steps):
+    """A single dense cell: 600 parcels run the wave-cooperative NTC (many
+    candidates per step), 5000 exceed an LDS tile and take the global-memory
+    path.  Both must equal the candidate-by-candidate oracle."""
+    case = cases.periodic_box(n_side=1, L=0.01, ppc=ppc, seed=17)
     eng, st = _pair(case, H.fill_box)
-    assert st.n == 3000
-    for k in range(3):
+    assert st.n == ppc
+    for k in range(steps):
         eng.evolve(1)
         st.evolve(1)
-        _check_state(eng, st, f"big/step{k}")
+        _check_state(eng, st, f"dense{ppc}/step{k}")
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    assert eng.counters()["collisions"] > 50 * steps
+    eng.close()
+
+
+def test_sparse_wide_tiles_parity():
+    """Two parcels per cell: a particle-balanced tile spans more cells than one
+    sub-tile of 256, so the sub-tile loop and its boundaries are exercised."""
+    case = cases.periodic_box(n_side=16, ppc=2, seed=23)
+    eng, st = _pair(case, H.fill_box)
+    for k in range(4):
+        eng.evolve(1)
+        st.evolve(1)
+        _check_state(eng, st, f"sparse/step{k}")
     np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
     eng.close()
