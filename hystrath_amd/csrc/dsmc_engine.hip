@@ -412,6 +412,11 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
                                           / (2.0 * sqrt_pi * cfg->fn);
             }
         }
+        // ni <= floor(acc + inc) + 1 with acc < 1 after every step
+        double max_new = 0.0;
+        for (double x : inc) max_new += std::floor(x) + 2.0;
+        if (max_new > (double)eng->cap) max_new = (double)eng->cap;
+        eng->faces.max_new = (int32_t)max_new;
         DevFaces& F = eng->faces;
         if ((rc = dev_alloc(eng, &F.cell, nf)) || (rc = dev_alloc(eng, &F.axis, nf)) ||
             (rc = dev_alloc(eng, &F.dir, nf)) || (rc = dev_alloc(eng, &F.a1i, nf)) || (rc = dev_alloc(eng, &F.a2i, nf)) ||
@@ -559,6 +564,9 @@ extern "C" int hs_dsmc_inflow_face_count(hs_dsmc_engine* eng, int64_t* n) {
 extern "C" int hs_dsmc_set_inflow_accumulators(hs_dsmc_engine* eng, const double* acc) {
     if (!eng || !acc) return fail(eng, HS_E_INVALID, "NULL argument");
     if (eng->faces.n == 0) return 0;
+    for (int64_t k = 0; k < (int64_t)eng->ns * eng->faces.n; ++k)
+        if (!(acc[k] >= -1.0 && acc[k] < 1.0))
+            return fail(eng, HS_E_INVALID, "inflow accumulators must lie in [-1, 1)");
     HS_TRY(hipSetDevice(eng->device));
     HS_TRY(hipMemcpyAsync(eng->faces.acc, acc, (size_t)eng->ns * eng->faces.n * 8, hipMemcpyHostToDevice, eng->stream));
     HS_TRY(hipStreamSynchronize(eng->stream));

@@ -18,6 +18,7 @@ struct DevFaces {
     int32_t* n_new;    // [face*ns + s] parcels to insert this step
     int32_t* off;      // exclusive scan of n_new
     int32_t n;
+    int32_t max_new;   // bound on parcels inserted in one step (launch size)
 };
 
 // Device scalars of one engine (one allocation).

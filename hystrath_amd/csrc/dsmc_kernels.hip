@@ -196,18 +196,22 @@ __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restr
     F.n_new[e] = ni;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
-                                                      DevParticles b, DevScalars* __restrict__ S) {
-    const DevParams& P = *Pp;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
-    if (e >= F.n * ns) return;
-    const int ni = F.n_new[e];
-    if (ni == 0) return;
+// Parcel j of this step belongs to the entry e (face*ns + species) with
+// off[e] <= j < off[e+1] and is its (j-off[e])-th; it is written at idx.
+__device__ void inject_one(const DevParams& P, const DevFaces& F, int ns, DevParticles& b, const DevScalars* S, int j,
+                           int64_t idx) {
+    const int ne = F.n * ns;
+    int lo = 0, hi = ne;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (F.off[mid] <= j) lo = mid;
+        else hi = mid;
+    }
+    const int e = lo;
+    const int k = j - F.off[e];
     const int f = e / ns, s = e - f * ns;
-    const int64_t base = (int64_t)S->n_old + F.off[e];
     PhiloxStream r;
-    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 0);
-    (void)r.u();  // the Poisson-rounding draw of k_inject_count
+    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 1u + (uint32_t)k);
     const int a = F.axis[f], dir = F.dir[f];
     const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
     const int nn[3] = { P.nx, P.ny, P.nz };
@@ -220,42 +224,48 @@ __global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restric
     double scaling = 3.0;
     if (s_cos < -3.0) scaling = fabs(s_cos) + 1.0;
     const double sc = sqrt(HS_KB * P.inflow_t / mass);
-    unsigned long long injected = 0;
-    for (int k = 0; k < ni; ++k) {
-        const int64_t j = base + k;
-        if (j >= P.capacity) break;
-        double pos[3];
-        const double ua = r.u(), ub = r.u();
-        pos[a] = face_coord(P, a, dir > 0 ? nn[a] : 0);
-        pos[a1] = P.empty[a1] ? P.centre[a1] : face_coord(P, a1, F.a1i[f]) + ua * P.delta[a1];
-        pos[a2] = P.empty[a2] ? P.centre[a2] : face_coord(P, a2, F.a2i[f]) + ub * P.delta[a2];
-        double un;
-        if (fabs(un_face) > HS_VSMALL) {
-            double Pa;
-            int guard = 0;
-            do {
-                const double unt = scaling * (2.0 * r.u() - 1.0);
-                un = unt + s_cos;
-                if (un < 0.0) Pa = -1.0;
-                else Pa = 2.0 * un / coef_a * dm_exp(coef_b - unt * unt);
-            } while (Pa < r.u() && ++guard < HS_MAX_REDRAW);
-        } else {
-            un = sqrt(-dm_log(1.0 - r.u()));
-        }
-        const double g1 = r.gauss(), g2 = r.gauss();
-        double U[3];
-        U[a] = mps * un * (double)(-dir);
-        U[a1] = sc * g1 + P.inflow_u[a1];
-        U[a2] = sc * g2 + P.inflow_u[a2];
-        b.px[j] = pos[0]; b.py[j] = pos[1]; b.pz[j] = pos[2];
-        b.ux[j] = (float)U[0]; b.uy[j] = (float)U[1]; b.uz[j] = (float)U[2];
-        b.cell[j] = F.cell[f];
-        if (b.vib)
-            for (int m = 0; m < P.max_modes; ++m) b.vib[m * P.capacity + j] = 0;
-        new_internal(P, b, j, s, P.inflow_trot, P.inflow_tvib, r);
-        ++injected;
+    double pos[3];
+    const double ua = r.u(), ub = r.u();
+    pos[a] = face_coord(P, a, dir > 0 ? nn[a] : 0);
+    pos[a1] = P.empty[a1] ? P.centre[a1] : face_coord(P, a1, F.a1i[f]) + ua * P.delta[a1];
+    pos[a2] = P.empty[a2] ? P.centre[a2] : face_coord(P, a2, F.a2i[f]) + ub * P.delta[a2];
+    double un;
+    if (fabs(un_face) > HS_VSMALL) {
+        double Pa;
+        int guard = 0;
+        do {
+            const double unt = scaling * (2.0 * r.u() - 1.0);
+            un = unt + s_cos;
+            if (un < 0.0) Pa = -1.0;
+            else Pa = 2.0 * un / coef_a * dm_exp(coef_b - unt * unt);
+        } while (Pa < r.u() && ++guard < HS_MAX_REDRAW);
+    } else {
+        un = sqrt(-dm_log(1.0 - r.u()));
     }
-    if (injected) atomicAdd(&S->counters[CNT_INJ], injected);
+    const double g1 = r.gauss(), g2 = r.gauss();
+    double U[3];
+    U[a] = mps * un * (double)(-dir);
+    U[a1] = sc * g1 + P.inflow_u[a1];
+    U[a2] = sc * g2 + P.inflow_u[a2];
+    b.px[idx] = pos[0]; b.py[idx] = pos[1]; b.pz[idx] = pos[2];
+    b.ux[idx] = (float)U[0]; b.uy[idx] = (float)U[1]; b.uz[idx] = (float)U[2];
+    b.cell[idx] = F.cell[f];
+    if (b.vib)
+        for (int m = 0; m < P.max_modes; ++m) b.vib[m * P.capacity + idx] = 0;
+    new_internal(P, b, idx, s, P.inflow_trot, P.inflow_tvib, r);
+}
+
+// One thread per new parcel.
+__global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
+                                                      DevParticles b, DevScalars* __restrict__ S) {
+    const DevParams& P = *Pp;
+    const int ne = F.n * ns;
+    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int64_t idx = (int64_t)S->n_old + j;
+    const bool mine = j < F.off[ne] && idx < P.capacity;
+    if (mine) inject_one(P, F, ns, b, S, j, idx);
+    // every lane reaches the wave reduction of the counter
+    add_counter(S, CNT_INJ, mine ? 1ull : 0ull);
 }
 
 void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st) {
@@ -266,9 +276,8 @@ void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScala
 
 void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
                        hipStream_t st) {
-    const int n = F.n * ns;
-    if (n == 0) return;
-    k_inject_gen<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, b, S);
+    if (F.n * ns == 0 || F.max_new <= 0) return;
+    k_inject_gen<<<(F.max_new + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, b, S);
 }
 
 // ---------------------------------------------------------------------------

@@ -186,7 +186,8 @@ int hs_dsmc_particle_count(hs_dsmc_engine* eng, int64_t* n_out);
 int hs_dsmc_set_cell_state(hs_dsmc_engine* eng, const double* sigma_tcr_max, const double* remainder);
 int hs_dsmc_get_cell_state(hs_dsmc_engine* eng, double* sigma_tcr_max, double* remainder);
 /* freestream inflow accumulators accumulatedParcelsToInsert_
- * (dsmcFreeStreamInflowPatch.C:132-140), [species][inflow face] */
+ * (dsmcFreeStreamInflowPatch.C:132-140), [species][inflow face]; values
+ * lie in [-1, 1) between steps and the setter rejects others */
 int hs_dsmc_inflow_face_count(hs_dsmc_engine* eng, int64_t* n_faces);
 int hs_dsmc_set_inflow_accumulators(hs_dsmc_engine* eng, const double* acc);
 int hs_dsmc_get_inflow_accumulators(hs_dsmc_engine* eng, double* acc);

@@ -1011,8 +1011,11 @@ int or_inject(const hs_config* cfg, or_state* s)
             const double sc = sqrt(HS_KB * cfg->inflow_t / mass);
             for (int k = 0; k < ni; ++k) {
                 if (n >= s->capacity) { s->counters[6] += 1; break; }
+                /* parcel k of this face and species draws from its own stream */
+                or_rng rp;
+                rng_init(&rp, cfg->seed, OR_RNG_INJECT, (uint32_t)(f * ns + sp), s->step, 1u + (uint32_t)k);
                 double pos[3];
-                const double ua = rng_u(&r), ub = rng_u(&r);
+                const double ua = rng_u(&rp), ub = rng_u(&rp);
                 pos[a] = face_coord(cfg, a, dir > 0 ? nn[a] : 0);
                 pos[a1] = d.empty[a1] ? d.centre[a1]
                                       : face_coord(cfg, a1, faces[f].a1i) + ua * cfg->delta[a1];
@@ -1023,15 +1026,15 @@ int or_inject(const hs_config* cfg, or_state* s)
                     double P;
                     int guard = 0;
                     do {
-                        const double unt = scaling * (2.0 * rng_u(&r) - 1.0);
+                        const double unt = scaling * (2.0 * rng_u(&rp) - 1.0);
                         un = unt + s_cos;
                         if (un < 0.0) P = -1.0;
                         else P = 2.0 * un / coef_a * or_exp(coef_b - unt * unt);
-                    } while (P < rng_u(&r) && ++guard < OR_MAX_REDRAW);
+                    } while (P < rng_u(&rp) && ++guard < OR_MAX_REDRAW);
                 } else {
-                    un = sqrt(-or_log(1.0 - rng_u(&r)));
+                    un = sqrt(-or_log(1.0 - rng_u(&rp)));
                 }
-                const double g1 = rng_gauss(&r), g2 = rng_gauss(&r);
+                const double g1 = rng_gauss(&rp), g2 = rng_gauss(&rp);
                 double U[3];
                 U[a] = mps * un * (double)(-dir);
                 U[a1] = sc * g1 + cfg->inflow_u[a1];
@@ -1041,7 +1044,7 @@ int or_inject(const hs_config* cfg, or_state* s)
                 s->p.cell[n] = faces[f].cell;
                 if (s->p.vib)
                     for (int m = 0; m < d.max_modes; ++m) s->p.vib[m * s->capacity + n] = 0;
-                new_particle_internal(cfg, s, n, sp, cfg->inflow_trot, cfg->inflow_tvib, &r);
+                new_particle_internal(cfg, s, n, sp, cfg->inflow_trot, cfg->inflow_tvib, &rp);
                 s->counters[3] += 1;
                 ++n;
             }
