@@ -27,9 +27,16 @@ HS_FLAG_TIMING = 0x8
 
 HS_XMIN, HS_XMAX, HS_YMIN, HS_YMAX, HS_ZMIN, HS_ZMAX = range(6)
 
-# sampling slots (enum hs_sample_slot)
-(HS_S_N, HS_S_M, HS_S_MUX, HS_S_MUY, HS_S_MUZ, HS_S_MCC, HS_S_EROT, HS_S_ZROT,
- HS_S_NCOLL, HS_S_COLLSEP, HS_S_BASE_COUNT) = range(11)
+# sampling slots (enum hs_sample_slot / hs_species_slot)
+HS_S_NCOLL, HS_S_COLLSEP, HS_S_GLOBAL_COUNT = 0, 1, 2
+(HS_SS_N, HS_SS_MUX, HS_SS_MUY, HS_SS_MUZ, HS_SS_MCC, HS_SS_EROT, HS_SS_EVIB0) = range(7)
+HS_SS_COUNT = HS_SS_EVIB0 + HS_MAX_VIB_MODES
+HS_HF_COUNT = 13
+
+
+def species_slot(s, k):
+    """Row of the sample array holding quantity k of species s."""
+    return HS_S_GLOBAL_COUNT + s * HS_SS_COUNT + k
 
 # names used by the reference dictionaries -> enums
 COLLISION_MODELS = {
@@ -124,7 +131,7 @@ def ptr(arr, ctype):
 
 
 def sample_slots(cfg):
-    n = HS_S_BASE_COUNT + cfg.n_species + cfg.n_species * HS_MAX_VIB_MODES
+    n = HS_S_GLOBAL_COUNT + cfg.n_species * HS_SS_COUNT
     if cfg.flags & HS_FLAG_HEATFLUX:
-        n += 13
+        n += cfg.n_species * HS_HF_COUNT
     return n

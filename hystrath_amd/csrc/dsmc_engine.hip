@@ -321,8 +321,8 @@ extern "C" int hs_dsmc_device_count(int* n) {
 }
 
 extern "C" int32_t hs_dsmc_sample_slots(const hs_config* cfg) {
-    int32_t n = HS_S_BASE_COUNT + cfg->n_species + cfg->n_species * HS_MAX_VIB_MODES;
-    if (cfg->flags & HS_FLAG_HEATFLUX) n += 13;
+    int32_t n = HS_S_GLOBAL_COUNT + cfg->n_species * HS_SS_COUNT;
+    if (cfg->flags & HS_FLAG_HEATFLUX) n += cfg->n_species * HS_HF_COUNT;
     return n;
 }
 

@@ -738,77 +738,62 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
     sep_out = sep;
 }
 
-// dsmcVolFields::calculateField per-cell sums for one cell
+// dsmcVolFields::calculateField per-cell sums for one cell: per species, the
+// parcels of the cell in order (dsmcVolFields.C:1115-1239)
 template <bool GEN>
 __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, int nC, const float* vx,
                                             const float* vy, const float* vz, const float* er, const uint16_t* vb,
                                             int vstride, const uint8_t* ty, double* samples, int nc, double ncoll,
                                             double sep) {
-    double aN = 0.0, aM = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0, aZr = 0.0;
-    for (int k = 0; k < nC; ++k) {
-        const int i = ss + k;
-        const int t = GEN ? (int)ty[i] : 0;
-        const double m = P.mass[t];
-        const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
-        const double lke = m * (ux * ux + uy * uy + uz * uz);
-        aN += 1.0;
-        aM += m;
-        aX += m * ux;
-        aY += m * uy;
-        aZ += m * uz;
-        aC += lke;
-        if (GEN && P.has_erot) aR += (double)er[i];
-        aZr += P.rotdof[t];
-    }
-    samples[(int64_t)HS_S_N * nc + c] += aN;
-    samples[(int64_t)HS_S_M * nc + c] += aM;
-    samples[(int64_t)HS_S_MUX * nc + c] += aX;
-    samples[(int64_t)HS_S_MUY * nc + c] += aY;
-    samples[(int64_t)HS_S_MUZ * nc + c] += aZ;
-    samples[(int64_t)HS_S_MCC * nc + c] += aC;
-    samples[(int64_t)HS_S_EROT * nc + c] += aR;
-    samples[(int64_t)HS_S_ZROT * nc + c] += aZr;
     samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
     samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
-    const int ns = P.ns;
-    if (!GEN) {
-        samples[(int64_t)HS_S_BASE_COUNT * nc + c] += aN;
-    } else {
-        for (int s = 0; s < ns; ++s) {
-            double cntS = 0.0;
-            double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
-            for (int k = 0; k < nC; ++k) {
-                const int i = ss + k;
-                if ((int)ty[i] != s) continue;
-                cntS += 1.0;
-                for (int m = 0; m < P.nvib[s]; ++m) ev[m] += (double)vb[m * vstride + i] * HS_KB * P.theta_v[s][m];
-            }
-            samples[(int64_t)(HS_S_BASE_COUNT + s) * nc + c] += cntS;
-            for (int m = 0; m < P.nvib[s]; ++m)
-                samples[(int64_t)(HS_S_BASE_COUNT + ns + s * HS_MAX_VIB_MODES + m) * nc + c] += ev[m];
-        }
-    }
-    if (P.flags & HS_FLAG_HEATFLUX) {
-        double h[13];
-        for (int q = 0; q < 13; ++q) h[q] = 0.0;
+    const int ns = GEN ? P.ns : 1;
+    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    for (int s = 0; s < ns; ++s) {
+        const double m = P.mass[s];
+        double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
+        double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+        const int nvib = GEN ? P.nvib[s] : 0;
         for (int k = 0; k < nC; ++k) {
             const int i = ss + k;
-            const int t = GEN ? (int)ty[i] : 0;
-            const double m = P.mass[t];
+            if (GEN && (int)ty[i] != s) continue;
             const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
-            const double lke = m * (ux * ux + uy * uy + uz * uz);
-            double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
-            double ev = 0.0;
-            if (GEN)
-                for (int mo = 0; mo < P.nvib[t]; ++mo) ev += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[t][mo];
-            const double eint = erv + ev;
-            h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
-            h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
-            h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
-            h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            aN += 1.0;
+            aX += m * ux;
+            aY += m * uy;
+            aZ += m * uz;
+            aC += m * (ux * ux + uy * uy + uz * uz);
+            if (GEN && P.has_erot) aR += (double)er[i];
+            for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
         }
-        const int hb = HS_S_BASE_COUNT + ns + ns * HS_MAX_VIB_MODES;
-        for (int q = 0; q < 13; ++q) samples[(int64_t)(hb + q) * nc + c] += h[q];
+        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+        samples[(base + HS_SS_N) * nc + c] += aN;
+        samples[(base + HS_SS_MUX) * nc + c] += aX;
+        samples[(base + HS_SS_MUY) * nc + c] += aY;
+        samples[(base + HS_SS_MUZ) * nc + c] += aZ;
+        samples[(base + HS_SS_MCC) * nc + c] += aC;
+        if (GEN && P.has_erot) samples[(base + HS_SS_EROT) * nc + c] += aR;
+        for (int mo = 0; mo < nvib; ++mo) samples[(base + HS_SS_EVIB0 + mo) * nc + c] += ev[mo];
+        if (hf) {
+            double h[HS_HF_COUNT];
+            for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
+            for (int k = 0; k < nC; ++k) {
+                const int i = ss + k;
+                if (GEN && (int)ty[i] != s) continue;
+                const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+                const double lke = m * (ux * ux + uy * uy + uz * uz);
+                const double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
+                double evs = 0.0;
+                for (int mo = 0; mo < nvib; ++mo) evs += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+                const double eint = erv + evs;
+                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            }
+            const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
+            for (int q = 0; q < HS_HF_COUNT; ++q) samples[(hb + q) * nc + c] += h[q];
+        }
     }
 }
 

@@ -144,22 +144,29 @@ typedef struct hs_particles {
     uint8_t* type;                         /* typeId                            */
 } hs_particles;
 
-/* Per-cell sampling accumulators (dsmcVolFields.C:1115-1239): plane-major
- * FP64 arrays [slot][cell].  The base slots are: */
+/* Per-cell sampling accumulators (dsmcVolFields.C:1115-1239), cumulative
+ * over sampled steps, plane-major FP64 arrays [slot][cell].  Sums are kept
+ * per species so every dsmcVolFields model (its typeIds subset, e.g. "N2" or
+ * "mixture") is a sum over its species.  Slots 0..1 are per cell: */
 enum hs_sample_slot {
-    HS_S_N = 0,       /* dsmcNCum: simulator particles                      */
-    HS_S_M,           /* dsmcMCum: sum m                                    */
-    HS_S_MUX, HS_S_MUY, HS_S_MUZ,  /* sum m U                               */
-    HS_S_MCC,         /* sum m |U|^2 (linearKE without 1/2)                 */
-    HS_S_EROT,        /* sum ERot                                           */
-    HS_S_ZROT,        /* sum rotational DoF                                 */
-    HS_S_NCOLL,       /* nColls (cellMeasurements)                          */
-    HS_S_COLLSEP,     /* collision separation                               */
-    HS_S_BASE_COUNT
+    HS_S_NCOLL = 0,    /* nColls (cellMeasurements), all species           */
+    HS_S_COLLSEP = 1,  /* collision separation sum                          */
+    HS_S_GLOBAL_COUNT = 2
 };
-/* then n_species slots of per-species N, then n_species*HS_MAX_VIB_MODES
- * slots of per-species per-mode vibrational energy, then (HS_FLAG_HEATFLUX)
- * 13 slots: Muu Muv Muw Mvv Mvw Mww Mccu Mccv Mccw Eu Ev Ew E. */
+/* then species s occupies slots HS_S_GLOBAL_COUNT + s*HS_SS_COUNT + k: */
+enum hs_species_slot {
+    HS_SS_N = 0,       /* dsmcNSpeciesCum: simulator parcels                */
+    HS_SS_MUX, HS_SS_MUY, HS_SS_MUZ, /* sum m U                              */
+    HS_SS_MCC,         /* sum m |U|^2 (linear KE without the 1/2)           */
+    HS_SS_EROT,        /* sum ERot                                          */
+    HS_SS_EVIB0,       /* sum of vibrational energy per mode (3 slots)      */
+    HS_SS_COUNT = HS_SS_EVIB0 + HS_MAX_VIB_MODES
+};
+/* and with HS_FLAG_HEATFLUX, after all species blocks, species s has 13
+ * slots: m uu, m uv, m uw, m vv, m vw, m ww, m|U|^2 u, m|U|^2 v, m|U|^2 w,
+ * Eint u, Eint v, Eint w, Eint (measureHeatFluxShearStress,
+ * dsmcVolFields.C:1190-1210). */
+#define HS_HF_COUNT 13
 int32_t hs_dsmc_sample_slots(const hs_config* cfg);
 
 /* ---- engine lifetime ---------------------------------------------------- */

@@ -327,8 +327,8 @@ static void derive(const hs_config* cfg, or_derived* d)
 
 int32_t or_sample_slots(const hs_config* cfg)
 {
-    int32_t n = HS_S_BASE_COUNT + cfg->n_species + cfg->n_species * HS_MAX_VIB_MODES;
-    if (cfg->flags & HS_FLAG_HEATFLUX) n += 13;
+    int32_t n = HS_S_GLOBAL_COUNT + cfg->n_species * HS_SS_COUNT;
+    if (cfg->flags & HS_FLAG_HEATFLUX) n += cfg->n_species * HS_HF_COUNT;
     return n;
 }
 
@@ -805,49 +805,53 @@ int or_sort(const hs_config* cfg, or_state* s)
 /* ------------------------------------------------------------------------ */
 /* NTC collisions (noTimeCounter.C:81-339) + sampling (dsmcVolFields.C)     */
 /* ------------------------------------------------------------------------ */
+/* dsmcVolFields::calculateField (dsmcVolFields.C:1115-1239): per species,
+ * the parcels of the cell in order */
 static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c,
                         int32_t b, int32_t nC, int64_t nc)
 {
     const int ns = cfg->n_species;
-    const int nslot = or_sample_slots(cfg);
-    double acc[HS_S_BASE_COUNT + HS_MAX_SPECIES + HS_MAX_SPECIES * HS_MAX_VIB_MODES + 13];
-    memset(acc, 0, sizeof(acc));
     const hs_particles* P = &s->p;
-    for (int32_t k = 0; k < nC; ++k) {
-        const int64_t i = b + k;
-        const int t = P->type ? P->type[i] : 0;
-        const hs_species* sp = &cfg->species[t];
+    for (int sp_id = 0; sp_id < ns; ++sp_id) {
+        const hs_species* sp = &cfg->species[sp_id];
         const double m = sp->mass;
-        const double ux = P->ux[i], uy = P->uy[i], uz = P->uz[i];
-        const double lke = m * (ux * ux + uy * uy + uz * uz);
-        const double er = P->erot ? (double)P->erot[i] : 0.0;
-        double ev = 0.0;
-        acc[HS_S_N] += 1.0;
-        acc[HS_S_M] += m;
-        acc[HS_S_MUX] += m * ux;
-        acc[HS_S_MUY] += m * uy;
-        acc[HS_S_MUZ] += m * uz;
-        acc[HS_S_MCC] += lke;
-        acc[HS_S_EROT] += er;
-        acc[HS_S_ZROT] += sp->rot_dof;
-        acc[HS_S_BASE_COUNT + t] += 1.0;
-        for (int mo = 0; mo < sp->n_vib_modes; ++mo) {
-            const double e = (double)P->vib[mo * s->capacity + i] * HS_KB * sp->theta_v[mo];
-            acc[HS_S_BASE_COUNT + ns + t * HS_MAX_VIB_MODES + mo] += e;
-            ev += e;
+        double acc[HS_SS_COUNT];
+        double h[HS_HF_COUNT];
+        memset(acc, 0, sizeof(acc));
+        memset(h, 0, sizeof(h));
+        for (int32_t k = 0; k < nC; ++k) {
+            const int64_t i = b + k;
+            const int t = P->type ? P->type[i] : 0;
+            if (t != sp_id) continue;
+            const double ux = P->ux[i], uy = P->uy[i], uz = P->uz[i];
+            const double lke = m * (ux * ux + uy * uy + uz * uz);
+            const double er = P->erot ? (double)P->erot[i] : 0.0;
+            double ev = 0.0;
+            acc[HS_SS_N] += 1.0;
+            acc[HS_SS_MUX] += m * ux;
+            acc[HS_SS_MUY] += m * uy;
+            acc[HS_SS_MUZ] += m * uz;
+            acc[HS_SS_MCC] += lke;
+            acc[HS_SS_EROT] += er;
+            for (int mo = 0; mo < sp->n_vib_modes; ++mo) {
+                const double e = (double)P->vib[mo * s->capacity + i] * HS_KB * sp->theta_v[mo];
+                acc[HS_SS_EVIB0 + mo] += e;
+                ev += e;
+            }
+            if (cfg->flags & HS_FLAG_HEATFLUX) {
+                const double eint = er + ev;
+                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            }
         }
+        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)sp_id * HS_SS_COUNT;
+        for (int k = 0; k < HS_SS_COUNT; ++k) s->samples[(base + k) * nc + c] += acc[k];
         if (cfg->flags & HS_FLAG_HEATFLUX) {
-            double* h = &acc[HS_S_BASE_COUNT + ns + ns * HS_MAX_VIB_MODES];
-            const double eint = er + ev;
-            h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
-            h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
-            h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
-            h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)ns * HS_SS_COUNT + (int64_t)sp_id * HS_HF_COUNT;
+            for (int k = 0; k < HS_HF_COUNT; ++k) s->samples[(hb + k) * nc + c] += h[k];
         }
-    }
-    for (int sl = 0; sl < nslot; ++sl) {
-        if (sl == HS_S_NCOLL || sl == HS_S_COLLSEP) continue;
-        s->samples[(int64_t)sl * nc + c] += acc[sl];
     }
     (void)d;
 }

@@ -146,12 +146,14 @@ def test_move_sort_collide_stages(which):
     eng.close()
 
 
-@pytest.mark.parametrize("which,steps", [("box", 6), ("cylinder", 8), ("couette", 5)])
+@pytest.mark.parametrize("which,steps", [("box", 6), ("cylinder", 8), ("couette", 5), ("couette_hf", 4)])
 def test_evolve_parity(which, steps):
+    hf = abi.HS_FLAG_SAMPLE | abi.HS_FLAG_HEATFLUX | abi.HS_FLAG_COLL_SEP
     case, filler = {
         "box": (H.small_box(), H.fill_box),
         "cylinder": (H.small_cylinder(), H.fill_cylinder),
         "couette": (cases.couette_n2o2(), H.fill_couette),
+        "couette_hf": (cases.couette_n2o2(flags=hf), H.fill_couette),
     }[which]
     eng, st = _pair(case, filler)
     for k in range(steps):

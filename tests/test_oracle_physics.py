@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 from scipy import integrate, special
 
-from hystrath_amd import abi, cases
+from hystrath_amd import abi, cases, fields
 import helpers as H
 import oracle_bind as ob
 
@@ -135,10 +135,7 @@ def test_lb_rotational_relaxation_to_equipartition():
     st.samples[:] = 0
     st.s.n_sampled = 0
     st.evolve(200)
-    s = st.sample_array().sum(axis=1)
-    T_tr = (s[abi.HS_S_MCC] - (s[abi.HS_S_MUX] ** 2 + s[abi.HS_S_MUY] ** 2 + s[abi.HS_S_MUZ] ** 2) / s[abi.HS_S_M]) \
-        / (3 * KB * s[abi.HS_S_N])
-    T_rot = 2 * s[abi.HS_S_EROT] / (KB * s[abi.HS_S_ZROT])
+    T_tr, T_rot, _ = fields.global_temperature(st.sample_array(), case)
     assert abs(T_tr - 180.0) < 0.02 * 180
     assert abs(T_rot - 180.0) < 0.02 * 180
 
@@ -190,7 +187,7 @@ def test_equilibrium_collision_rate_and_temperature():
     T1 = temp(st.particles())
     assert abs(T1 / T0 - 1) < 1e-5  # energy conserved up to FP32 velocity storage
     s = st.sample_array().sum(axis=1)
-    n_sim = s[abi.HS_S_N]
+    n_sim = s[abi.species_slot(0, abi.HS_SS_N)]
     rate = 2 * s[abi.HS_S_NCOLL] / (n_sim * case.cfg.dt)
     n = case.n_density
     d, w = AR["diameter"], AR["omega"]
@@ -374,10 +371,14 @@ def test_sampling_sums_on_hand_input():
     st.sort()
     st.collide(sample=True)
     s = st.sample_array()[:, 0]
-    assert s[abi.HS_S_N] == 3 and s[abi.HS_S_M] == 6
-    assert (s[abi.HS_S_MUX], s[abi.HS_S_MUY], s[abi.HS_S_MUZ]) == (4.0, 8.0, 18.0)
-    assert s[abi.HS_S_MCC] == m * (14 + 17 + 12)
-    assert s[abi.HS_S_BASE_COUNT] == 3 and st.s.n_sampled == 1
+    slot = lambda k: s[abi.species_slot(0, k)]  # noqa: E731
+    assert slot(abi.HS_SS_N) == 3
+    assert (slot(abi.HS_SS_MUX), slot(abi.HS_SS_MUY), slot(abi.HS_SS_MUZ)) == (4.0, 8.0, 18.0)
+    assert slot(abi.HS_SS_MCC) == m * (14 + 17 + 12)
+    assert slot(abi.HS_SS_EROT) == 0.0 and st.s.n_sampled == 1
+    f = fields.vol_fields(st.sample_array(), 1, case)
+    assert f["rhoN"][0] == 3 * case.cfg.fn
+    np.testing.assert_allclose(f["U"][:, 0], [2 / 3, 4 / 3, 3.0], rtol=1e-15)
 
 
 def test_fill_counts_positions_and_sigma():
