@@ -28,6 +28,37 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
+HOST = os.path.join(HERE, "host")
+BIN = os.path.join(HERE, "bin")
+HOST_LIB_SOURCES = ["dictionary.C", "dsmcCloud.C"]
+APPS = ["dsmcFoamPlus", "dsmcInitialisePlus"]
+
+
+def build_host(verbose=False):
+    """The C++ host mirror of dsmcCloud and its applications (g++, linked
+    against the engine library)."""
+    os.makedirs(BIN, exist_ok=True)
+    common = ["g++", "-O2", "-std=c++17", "-Wall", "-Wextra", "-I", os.path.join(HERE, "..", "include")]
+    srcs = [os.path.join(HOST, s) for s in HOST_LIB_SOURCES]
+    outs = []
+    for app in APPS:
+        out = os.path.join(BIN, app)
+        cmd = common + srcs + [os.path.join(HOST, app + ".C"), "-L", HERE, "-lhystrath_dsmc",
+                               "-Wl,-rpath,$ORIGIN/..", "-o", out]
+        outs.append((cmd, out))
+    outs.append((common + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "dictCheck.C"), "-o",
+                           os.path.join(BIN, "dictCheck")], os.path.join(BIN, "dictCheck")))
+    for cmd, out in outs:
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {out}:\n{r.stdout}{r.stderr}")
+        if verbose and (r.stdout or r.stderr):
+            print(r.stdout + r.stderr)
+    return [o for _, o in outs]
+
+
 def build(verbose=False):
     cmd = [hipcc()] + HIPCC_FLAGS + [os.path.join(CSRC, s) for s in SOURCES] + ["-o", LIB + ".tmp"]
     if verbose:
@@ -39,6 +70,7 @@ def build(verbose=False):
     if verbose and msgs:
         print("\n".join(msgs))
     os.replace(LIB + ".tmp", LIB)
+    build_host(verbose)
     return LIB
 
 

@@ -1,0 +1,125 @@
+"""The C++ host mirror of dsmcCloud (hystrath_amd/host): dictionary reading,
+runtime selection by the reference's model names, FatalError behaviour, and
+(on a GPU) the dsmcInitialisePlus -> dsmcFoamPlus pipeline with restart."""
+import glob
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from hystrath_amd import casegen
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+BIN = os.path.join(ROOT, "hystrath_amd", "bin")
+REF = "/root/reference/run/hyStrath/dsmcFoam+"
+
+
+def run(app, case, timeout=300):
+    return subprocess.run([os.path.join(BIN, app), "-case", case], capture_output=True, text=True, timeout=timeout)
+
+
+def no_gpu():
+    from hystrath_amd import engine
+    return engine.device_count() == 0
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
+def test_reads_every_reference_tutorial_dictionary():
+    files = []
+    for c in sorted(os.listdir(REF)):
+        for rel in ("constant/dsmcProperties", "system/boundariesDict", "system/fieldPropertiesDict",
+                    "system/controlDict", "system/dsmcInitialiseDict", "system/blockMeshDict", "system/chemReactDict"):
+            p = os.path.join(REF, c, rel)
+            if os.path.exists(p):
+                files.append(p)
+    assert len(files) > 25
+    r = subprocess.run([os.path.join(BIN, "dictCheck")] + files, capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout[-2000:]
+    assert "BinaryCollisionModel" in r.stdout and "dsmcPatchBoundaries" in r.stdout
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
+def test_reference_case_outcomes(tmp_path):
+    """couette_N2-O2 is accepted up to the device; the others stop with the
+    reason they are outside this engine (multi-block mesh, QK chemistry)."""
+    outcomes = {}
+    for c in ("couette_N2-O2", "heatBath-5species", "hypersonicCorner", "supersonicFlatPlate"):
+        d = tmp_path / c
+        shutil.copytree(os.path.join(REF, c, "system"), d / "system")
+        shutil.copytree(os.path.join(REF, c, "constant"), d / "constant", ignore=shutil.ignore_patterns("polyMesh"))
+        r = run("dsmcInitialisePlus", str(d))
+        outcomes[c] = r.stdout + r.stderr
+    assert "FOAM FATAL ERROR" in outcomes["heatBath-5species"] and "QK chemistry" in outcomes["heatBath-5species"]
+    assert "one block" in outcomes["hypersonicCorner"] and "one block" in outcomes["supersonicFlatPlate"]
+    if no_gpu():
+        assert "no HIP device" in outcomes["couette_N2-O2"]
+
+
+def test_runtime_selection_errors(tmp_path):
+    case = casegen.couette_case(str(tmp_path / "c"))
+    p = os.path.join(case, "constant/dsmcProperties")
+    txt = open(p).read()
+    open(p, "w").write(txt.replace("LarsenBorgnakkeVariableHardSphere;", "MaxwellMolecule;"))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1
+    assert "Unknown BinaryCollisionModel type MaxwellMolecule" in r.stderr
+    assert "VariableSoftSphere" in r.stderr  # lists the valid types, as the reference does
+    open(p, "w").write(txt)
+    b = os.path.join(case, "system/boundariesDict")
+    btxt = open(b).read()
+    open(b, "w").write(btxt.replace("dsmcDiffuseWallPatch;", "dsmcCLLWallPatch;", 1))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1 and "unknown dsmcPatchBoundary type dsmcCLLWallPatch" in r.stderr
+
+
+def test_generated_cases_parse(tmp_path):
+    for maker in (casegen.couette_case, casegen.cylinder_case):
+        case = maker(str(tmp_path / maker.__name__))
+        files = glob.glob(os.path.join(case, "*/*Dict")) + glob.glob(os.path.join(case, "constant/*"))
+        r = subprocess.run([os.path.join(BIN, "dictCheck")] + files, capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout
+        if no_gpu():
+            r = run("dsmcInitialisePlus", case)
+            assert "no HIP device" in r.stderr
+
+
+def _field(path):
+    txt = open(path).read()
+    m = re.search(r"internalField\s+nonuniform List<(\w+)>\s*\n(\d+)\n\(\n(.*?)\n\)\n;", txt, re.S)
+    rows = m.group(3).split("\n")
+    if m.group(1) == "vector":
+        return np.array([[float(x) for x in r.strip("()").split()] for r in rows])
+    return np.array([float(x) for x in rows])
+
+
+@pytest.mark.gpu
+def test_couette_pipeline_and_restart(tmp_path):
+    case = casegen.couette_case(str(tmp_path / "couette"), end_time=4e-4, write_interval=2e-4)
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 0, r.stderr
+    assert os.path.exists(os.path.join(case, "0/lagrangian/dsmc/hystrath.state"))
+    r = run("dsmcFoamPlus", case)
+    assert r.returncode == 0, r.stderr
+    assert "Number of DSMC particles" in r.stdout
+    last = os.path.join(case, "0.0004")
+    for f in ("rhoN_mixture", "Ttra_mixture", "Trot_N2", "U_mixture", "Tov_O2", "measuredCollisionRate_mixture"):
+        assert os.path.exists(os.path.join(last, f)), f
+    rho = _field(os.path.join(last, "rhoN_mixture"))
+    assert rho.shape == (500,)
+    assert abs(rho.mean() / 4.09e19 - 1) < 0.05  # 3.23e19 N2 + 0.86e19 O2, walls conserve parcels
+    T = _field(os.path.join(last, "Ttra_mixture"))
+    assert 1500 < T.mean() < 3500
+    U = _field(os.path.join(last, "U_mixture"))
+    assert U.shape == (500, 3)
+    # restart from the latest time and continue to a later end time
+    cd = os.path.join(case, "system/controlDict")
+    ctxt = open(cd).read()
+    assert "endTime 0.0004;" in ctxt
+    open(cd, "w").write(ctxt.replace("endTime 0.0004;", "endTime 0.0006;"))
+    r = run("dsmcFoamPlus", case)
+    assert r.returncode == 0, r.stderr
+    assert "from" in r.stdout and "0.0004" in r.stdout
+    assert os.path.exists(os.path.join(case, "0.0006", "Ttra_mixture"))
