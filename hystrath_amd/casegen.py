@@ -130,7 +130,7 @@ O2 = dict(mass=53.12e-27, diameter=4.07e-10, omega=0.77, alpha=1.4, rotationalDe
 AR = dict(mass=6.63e-26, diameter=4.17e-10, omega=0.81, alpha=1.0)
 
 
-def couette_case(root, end_time=2e-4, write_interval=1e-4, seed=11):
+def couette_case(root, end_time=2e-4, write_interval=1e-4, seed=11, n_terminal=50):
     """The couette_N2-O2 tutorial set-up (5x100x1 cells, cyclic x, diffuse
     walls at 2000 K and 3000 K moving at 300 m/s, LB-VHS N2/O2)."""
     wall = "wall"
@@ -144,7 +144,7 @@ def couette_case(root, end_time=2e-4, write_interval=1e-4, seed=11):
         model="LarsenBorgnakkeVariableHardSphere",
         coeffs={"rotationalRelaxationCollisionNumber": 5.0, "inverseZvFormulation": '"pre-2008"'}, seed=seed,
         fields={"N2": ["N2"], "O2": ["O2"], "mixture": ["N2", "O2"]},
-        init=[{"n": {"N2": 3.2320494711701479e19, "O2": 8.591523910705456e18}, "T": 2000.0}])
+        init=[{"n": {"N2": 3.2320494711701479e19, "O2": 8.591523910705456e18}, "T": 2000.0}], n_terminal=n_terminal)
 
 
 def cylinder_case(root, nx=400, ny=400, end_time=5e-4, write_interval=2.5e-4, seed=7, ppc=25):

@@ -640,14 +640,18 @@ void dsmcCloud::info() {
     const double fn = cfg_.fn;
     std::cout << "    Number of DSMC particles        = " << nParticles() << "\n";
     if (N > 0) {
-        std::cout << "    Number of free particles        = " << N * fn / fn << "\n"
-                  << "    Average linear kinetic energy   = " << lke / N << "\n"
+        std::cout << "    Average linear kinetic energy   = " << lke / N << "\n"
                   << "    Average rotational energy       = " << er / N << "\n"
                   << "    Average vibrational energy      = " << ev / N << "\n"
                   << "    Average electronic energy       = " << 0.0 << "\n"
                   << "    Total energy                    = " << fn * (lke + er + ev) << "\n";
     }
     std::cout << "    Collisions                      = " << cnt[1] << std::endl;
+    if (cnt[5] || cnt[6] || cnt[7])
+        throw FatalError("device-side bound tripped since the last report: " + std::to_string(cnt[5]) +
+                         " parcels hit the tracking limit, " + std::to_string(cnt[6]) +
+                         " cells exceeded the NTC candidate or parcel limit, " + std::to_string(cnt[7]) +
+                         " inflow parcels dropped at capacity " + std::to_string(cfg_.capacity));
     check(hs_dsmc_reset_counters(eng_), "reset counters");
 }
 

@@ -97,13 +97,13 @@ def _field(path):
 
 @pytest.mark.gpu
 def test_couette_pipeline_and_restart(tmp_path):
-    case = casegen.couette_case(str(tmp_path / "couette"), end_time=4e-4, write_interval=2e-4)
+    case = casegen.couette_case(str(tmp_path / "couette"), end_time=4e-4, write_interval=2e-4, n_terminal=10)
     r = run("dsmcInitialisePlus", case)
     assert r.returncode == 0, r.stderr
     assert os.path.exists(os.path.join(case, "0/lagrangian/dsmc/hystrath.state"))
     r = run("dsmcFoamPlus", case)
     assert r.returncode == 0, r.stderr
-    assert "Number of DSMC particles" in r.stdout
+    assert r.stdout.count("Number of DSMC particles") == 4, r.stdout
     last = os.path.join(case, "0.0004")
     for f in ("rhoN_mixture", "Ttra_mixture", "Trot_N2", "U_mixture", "Tov_O2", "measuredCollisionRate_mixture"):
         assert os.path.exists(os.path.join(last, f)), f
