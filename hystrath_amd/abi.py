@@ -24,6 +24,7 @@ HS_FLAG_SAMPLE = 0x1
 HS_FLAG_HEATFLUX = 0x2
 HS_FLAG_COLL_SEP = 0x4
 HS_FLAG_TIMING = 0x8
+HS_FLAG_WALL = 0x10
 
 HS_XMIN, HS_XMAX, HS_YMIN, HS_YMAX, HS_ZMIN, HS_ZMAX = range(6)
 
@@ -32,6 +33,12 @@ HS_S_NCOLL, HS_S_COLLSEP, HS_S_GLOBAL_COUNT = 0, 1, 2
 (HS_SS_N, HS_SS_MUX, HS_SS_MUY, HS_SS_MUZ, HS_SS_MCC, HS_SS_EROT, HS_SS_EVIB0) = range(7)
 HS_SS_COUNT = HS_SS_EVIB0 + HS_MAX_VIB_MODES
 HS_HF_COUNT = 13
+# wall-face sums, [species*HS_WS_COUNT + slot][wall face]
+(HS_WS_RHON, HS_WS_KE, HS_WS_MOMX, HS_WS_MOMY, HS_WS_MOMZ, HS_WS_EROT, HS_WS_EVIB0) = range(7)
+HS_WS_Q = HS_WS_EVIB0 + HS_MAX_VIB_MODES
+HS_WS_FDX, HS_WS_FDY, HS_WS_FDZ = HS_WS_Q + 1, HS_WS_Q + 2, HS_WS_Q + 3
+HS_WS_COUNT = HS_WS_FDZ + 1
+HS_WALL_CYLINDER = 6  # side value of the cylinder's wall faces
 
 
 def species_slot(s, k):

@@ -324,3 +324,24 @@ def couette_n2o2(seed=11, capacity=200000, flags=abi.HS_FLAG_SAMPLE):
     case.init_n = [3.2320494711701479e19, 8.591523910705456e18]
     case.init_T = 2000.0
     return case
+
+
+def wall_channel(nx=4, ny=20, L=0.02, n_density=1e21, T_gas=600.0, T_lo=300.0, T_hi=1000.0, u_hi=(0.0, 200.0, 0.0),
+                 lo_model=abi.HS_PATCH_DIFFUSE, hi_model=abi.HS_PATCH_DIFFUSE, ppc=50, dt=2e-7, seed=31,
+                 collision_model=abi.HS_COLL_VHS, species=None, flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL):
+    """Argon between two walls normal to x (xmin at T_lo at rest, xmax at
+    T_hi moving with u_hi), cyclic in y, empty in z: the heat-transfer and
+    shear set-up that exercises boundaryMeasurements."""
+    d = L / nx
+    sp = species or [make_species("Ar")]
+    cyc = patch(abi.HS_PATCH_CYCLIC)
+    empty = patch(abi.HS_PATCH_EMPTY)
+    lo = patch(lo_model, temperature=T_lo)
+    hi = patch(hi_model, temperature=T_hi, velocity=u_hi)
+    fn = n_density * d ** 3 / ppc
+    cap = int(nx * ny * ppc * 1.5) + 1024
+    case = Case(nx, ny, 1, (0.0, 0.0, 0.0), (d, d, d), sp, [lo, hi, cyc, cyc, empty, empty],
+                collision_model=collision_model, fn=fn, dt=dt, seed=seed, capacity=cap, flags=flags,
+                name="wall_channel")
+    case.n_density, case.T_gas = n_density, T_gas
+    return case

@@ -70,6 +70,12 @@ struct DevParams {
     double inflow_u[3];
     double inflow_n[HS_MAX_SPECIES];
     int n_inflow_faces;
+    // wall-face sampling (boundaryMeasurements), HS_FLAG_WALL
+    int n_wall;                 // wall faces
+    int wall_off[6];            // first face of each measured side, -1 otherwise
+    const double* wall_area;    // [n_wall]
+    const int32_t* cyl_face;    // [nc] wall face of the cylinder in each cell, -1 if none
+    double* wall_sum;           // [ns*HS_WS_COUNT][n_wall]
 };
 
 // Device view of one particle buffer (structure of arrays).
@@ -493,4 +499,55 @@ __device__ void diffuse_reflect(const DevParams& P, double T, const double uw[3]
     if (erot) *erot = (float)equip_erot(r, T, P.rotdof[t]);
     if (vib)
         for (int m = 0; m < P.nvib[t]; ++m) vib[m * vstride] = (uint16_t)equip_vib_level(r, T, P.theta_v[t][m]);
+}
+
+// ---------------------------------------------------------------------------
+// wall-face sampling: dsmcPatchBoundary::measurePropertiesBeforeControl /
+// AfterControl (dsmcPatchBoundary.C:263-483) for one hit on wall face f; nw
+// is the unit normal pointing out of the gas.  wall_side_terms adds the
+// flux-weighted terms of one side of the interaction and returns the
+// parcel's total energy; the momentum is m*U.
+// ---------------------------------------------------------------------------
+__device__ double wall_side_terms(const DevParams& P, int64_t f, int t, const double nw[3], const double U[3],
+                                  const float* erot, const uint16_t* vib, int64_t vstride) {
+    const double m = P.mass[t], adt = P.wall_area[f] * P.dt;
+    const double un = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+    double dn = fabs(un) * adt;
+    if (dn < HS_SMALL) dn = HS_SMALL;
+    const double w = 1.0 / dn;
+    const double ke = 0.5 * m * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    const int64_t st = P.n_wall;
+    double* base = P.wall_sum + (int64_t)t * HS_WS_COUNT * st + f;
+    unsafeAtomicAdd(base + HS_WS_RHON * st, w);
+    unsafeAtomicAdd(base + HS_WS_KE * st, ke * w);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(base + (HS_WS_MOMX + c) * st, m * (U[c] - un * nw[c]) * w);
+    double e = ke;
+    if (P.rotdof[t] > 0) {
+        const double er = (double)*erot;
+        unsafeAtomicAdd(base + HS_WS_EROT * st, er * w);
+        e += er;
+    }
+    const int nv = P.nvib[t];
+#pragma unroll
+    for (int k = 0; k < HS_MAX_VIB_MODES; ++k) {
+        if (k < nv) {
+            const double ev = (double)vib[k * vstride] * HS_KB * P.theta_v[t][k];
+            unsafeAtomicAdd(base + (HS_WS_EVIB0 + k) * st, ev * w);
+            e += ev;
+        }
+    }
+    return e;
+}
+
+// q and fD of one hit from the energies and velocities before (a) and after (b)
+__device__ __forceinline__ void wall_exchange_terms(const DevParams& P, int64_t f, int t, double ea, const double Ua[3],
+                                                    double eb, const double Ub[3]) {
+    const double m = P.mass[t];
+    const double cq = P.fn / (P.wall_area[f] * P.dt);
+    const int64_t st = P.n_wall;
+    double* base = P.wall_sum + (int64_t)t * HS_WS_COUNT * st + f;
+    unsafeAtomicAdd(base + HS_WS_Q * st, cq * (ea - eb));
+#pragma unroll
+    for (int c = 0; c < 3; ++c) unsafeAtomicAdd(base + (HS_WS_FDX + c) * st, cq * (m * Ua[c] - m * Ub[c]));
 }

@@ -7,6 +7,7 @@
 // collided population into the other one.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -56,6 +57,12 @@ struct hs_dsmc_engine {
     int tile_parcels = 1536;  // parcels per collide workgroup (particle-balanced tiles)
     DevFaces faces;
     int32_t* d_face_off = nullptr;
+    // wall faces (boundaryMeasurements)
+    std::vector<int32_t> wall_side, wall_cell;
+    std::vector<double> wall_area, wall_normal;
+    double* d_wall_area = nullptr;
+    int32_t* d_cyl_face = nullptr;
+    double* d_wall_sum = nullptr;
     DevScalars* d_S = nullptr;
     ScanWs ws;
     int64_t n_sampled = 0;
@@ -163,6 +170,91 @@ static std::vector<HostFace> list_faces(const hs_config& c, const std::vector<do
             }
     }
     return out;
+}
+
+// Length of the circle (cx, cy, r) inside [x0,x1]x[y0,y1] and its
+// length-weighted unit normal pointing to the centre: the circle's crossings
+// of the four lines split [0, 2pi) into arcs wholly inside or outside.
+static double arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2]) {
+    const double pi = 3.14159265358979323846, two_pi = 6.28318530717958647692;
+    std::vector<double> th = { 0.0, two_pi };
+    const double xs[2] = { x0, x1 }, ys[2] = { y0, y1 };
+    for (int j = 0; j < 2; ++j) {
+        const double u = (xs[j] - cx) / r;
+        if (u > -1.0 && u < 1.0) {
+            const double a = acos(u);
+            th.push_back(a);
+            th.push_back(two_pi - a);
+        }
+        const double v = (ys[j] - cy) / r;
+        if (v > -1.0 && v < 1.0) {
+            double a = asin(v);
+            th.push_back(pi - a);
+            if (a < 0.0) a += two_pi;
+            th.push_back(a);
+        }
+    }
+    std::sort(th.begin(), th.end());
+    double len = 0.0, nx = 0.0, ny = 0.0;
+    for (size_t j = 0; j + 1 < th.size(); ++j) {
+        const double dth = th[j + 1] - th[j];
+        if (!(dth > 0.0)) continue;
+        const double m = 0.5 * (th[j] + th[j + 1]);
+        const double px = cx + r * cos(m), py = cy + r * sin(m);
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) {
+            len += r * dth;
+            nx -= r * dth * cos(m);
+            ny -= r * dth * sin(m);
+        }
+    }
+    const double nm = sqrt(nx * nx + ny * ny);
+    n_out[0] = nm > 0.0 ? nx / nm : 0.0;
+    n_out[1] = nm > 0.0 ? ny / nm : 0.0;
+    return len;
+}
+
+// Wall faces: every face of each diffuse or specular side (first tangential
+// index fastest), then one face per cell that the cylinder wall crosses.
+static void list_wall_faces(hs_dsmc_engine* eng, int wall_off[6]) {
+    const hs_config& c = eng->cfg;
+    const int nn[3] = { c.nx, c.ny, c.nz };
+    auto coord = [&](int a, int i) { return c.origin[a] + (double)i * c.delta[a]; };
+    for (int side = 0; side < 6; ++side) {
+        wall_off[side] = -1;
+        const int m = c.patch[side].model;
+        if (m != HS_PATCH_DIFFUSE && m != HS_PATCH_SPECULAR) continue;
+        wall_off[side] = (int)eng->wall_side.size();
+        const int axis = side / 2, dir = (side & 1) ? 1 : -1;
+        const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+        for (int j2 = 0; j2 < nn[a2]; ++j2)
+            for (int j1 = 0; j1 < nn[a1]; ++j1) {
+                int idx[3];
+                idx[axis] = dir > 0 ? nn[axis] - 1 : 0;
+                idx[a1] = j1;
+                idx[a2] = j2;
+                eng->wall_side.push_back(side);
+                eng->wall_cell.push_back(idx[0] + c.nx * (idx[1] + c.ny * idx[2]));
+                eng->wall_area.push_back(c.delta[a1] * c.delta[a2]);
+                double n[3] = { 0.0, 0.0, 0.0 };
+                n[axis] = (double)dir;
+                eng->wall_normal.insert(eng->wall_normal.end(), n, n + 3);
+            }
+    }
+    if (!c.cyl_enabled) return;
+    for (int k = 0; k < c.nz; ++k)
+        for (int j = 0; j < c.ny; ++j)
+            for (int i = 0; i < c.nx; ++i) {
+                double nw[2];
+                const double len = arc_in_rect(c.cyl_center[0], c.cyl_center[1], c.cyl_radius, coord(0, i),
+                                               coord(0, i + 1), coord(1, j), coord(1, j + 1), nw);
+                // arcs shorter than 1e-6 of the cell (slivers where the circle
+                // grazes a grid line) carry no face
+                if (!(len > 1e-6 * std::min(c.delta[0], c.delta[1]))) continue;
+                eng->wall_side.push_back(6);
+                eng->wall_cell.push_back(i + c.nx * (j + c.ny * k));
+                eng->wall_area.push_back(len * c.delta[2]);
+                eng->wall_normal.insert(eng->wall_normal.end(), { nw[0], nw[1], 0.0 });
+            }
 }
 
 static int validate(const hs_config* c) {
@@ -432,6 +524,29 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
             return bail(fail(eng, HS_E_HIP, "hipMemcpy of inflow faces failed"));
         launch_fill_f64(F.acc, 0.0, (int64_t)ns * nf, eng->stream);
     }
+    // wall faces
+    list_wall_faces(eng, eng->hp.wall_off);
+    const int64_t nw = (int64_t)eng->wall_side.size();
+    eng->hp.n_wall = (int)nw;
+    if (nw > 0) {
+        if ((rc = dev_alloc(eng, &eng->d_wall_area, nw)) ||
+            (rc = dev_alloc(eng, &eng->d_wall_sum, (int64_t)ns * HS_WS_COUNT * nw)))
+            return bail(rc);
+        if (hipMemcpy(eng->d_wall_area, eng->wall_area.data(), nw * 8, hipMemcpyHostToDevice) != hipSuccess)
+            return bail(fail(eng, HS_E_HIP, "hipMemcpy of wall faces failed"));
+        launch_fill_f64(eng->d_wall_sum, 0.0, (int64_t)ns * HS_WS_COUNT * nw, eng->stream);
+        if (cfg->cyl_enabled) {
+            std::vector<int32_t> cf(nc, -1);
+            for (int64_t f = 0; f < nw; ++f)
+                if (eng->wall_side[f] == 6) cf[eng->wall_cell[f]] = (int32_t)f;
+            if ((rc = dev_alloc(eng, &eng->d_cyl_face, nc))) return bail(rc);
+            if (hipMemcpy(eng->d_cyl_face, cf.data(), (size_t)nc * 4, hipMemcpyHostToDevice) != hipSuccess)
+                return bail(fail(eng, HS_E_HIP, "hipMemcpy of cylinder faces failed"));
+        }
+    }
+    eng->hp.wall_area = eng->d_wall_area;
+    eng->hp.cyl_face = eng->d_cyl_face;
+    eng->hp.wall_sum = eng->d_wall_sum;
     if (hipMemcpy(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(eng->d_vol, eng->vol_h.data(), (size_t)nc * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "hipMemcpy of parameters failed"));
@@ -782,8 +897,38 @@ extern "C" int hs_dsmc_reset_samples(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
     launch_fill_f64(eng->d_samples, 0.0, (int64_t)eng->nslots * eng->nc, eng->stream);
+    if (eng->hp.n_wall > 0)
+        launch_fill_f64(eng->d_wall_sum, 0.0, (int64_t)eng->ns * HS_WS_COUNT * eng->hp.n_wall, eng->stream);
     eng->n_sampled = 0;
     return check_launch(eng, "reset samples");
+}
+
+extern "C" int hs_dsmc_wall_face_count(hs_dsmc_engine* eng, int64_t* n_faces) {
+    if (!eng || !n_faces) return fail(eng, HS_E_INVALID, "NULL argument");
+    *n_faces = eng->hp.n_wall;
+    return 0;
+}
+
+extern "C" int hs_dsmc_wall_faces(hs_dsmc_engine* eng, int32_t* side, int32_t* cell, double* area, double* normal) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    const size_t n = eng->wall_side.size();
+    if (side) std::copy(eng->wall_side.begin(), eng->wall_side.end(), side);
+    if (cell) std::copy(eng->wall_cell.begin(), eng->wall_cell.end(), cell);
+    if (area) std::copy(eng->wall_area.begin(), eng->wall_area.end(), area);
+    if (normal) std::copy(eng->wall_normal.begin(), eng->wall_normal.begin() + 3 * n, normal);
+    return 0;
+}
+
+extern "C" int hs_dsmc_read_wall_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    const int64_t need = (int64_t)eng->ns * HS_WS_COUNT * eng->hp.n_wall;
+    if (out && n_doubles < need) return fail(eng, HS_E_INVALID, "wall sample buffer too small");
+    HS_TRY(hipSetDevice(eng->device));
+    if (out && need > 0)
+        HS_TRY(hipMemcpyAsync(out, eng->d_wall_sum, need * 8, hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    if (nsteps_out) *nsteps_out = eng->n_sampled;
+    return 0;
 }
 
 extern "C" int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n) {

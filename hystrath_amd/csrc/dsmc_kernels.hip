@@ -308,6 +308,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
 #pragma unroll
     for (int a = 0; a < 3; ++a)
         if (P.empty[a]) p[a] = P.centre[a];
+    const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
     bool keep = true;
     int iter = 0;
     while (keep && t > HS_ROOTVSMALL) {
@@ -363,12 +364,19 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
             const double nx_ = P.cyl_c[0] - p[0], ny_ = P.cyl_c[1] - p[1];
             const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
             const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
+            const int32_t f = measure ? P.cyl_face[idx[0] + P.nx * (idx[1] + P.ny * idx[2])] : -1;
+            double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
+            if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
             if (P.cyl_model == HS_PATCH_DIFFUSE) {
                 diffuse_reflect(P, P.cyl_T, P.cyl_U, nw, t_id, U, erot, vib, P.capacity, r);
             } else {
                 const double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
                 if (und > 0.0)
                     for (int q = 0; q < 3; ++q) U[q] = U[q] - 2.0 * und * nw[q];
+            }
+            if (f >= 0) {
+                const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+                wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
             }
             continue;
         }
@@ -380,16 +388,29 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
         if (model == HS_PATCH_CYCLIC) {
             p[axis] = face_coord(P, axis, dir > 0 ? 0 : nn[axis]);
             idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
-        } else if (model == HS_PATCH_DIFFUSE) {
-            mc.wall += 1;
-            double nw[3] = { 0.0, 0.0, 0.0 };
-            nw[axis] = (double)dir;
-            diffuse_reflect(P, P.patch_T[side], P.patch_U[side], nw, t_id, U, erot, vib, P.capacity, r);
         } else if (model == HS_PATCH_DELETION) {
             keep = false;
         } else {
+            // diffuse or specular wall (an empty plane is never crossed)
             mc.wall += 1;
-            if (U[axis] * (double)dir > 0.0) U[axis] = -U[axis];
+            double nw[3] = { 0.0, 0.0, 0.0 };
+            nw[axis] = (double)dir;
+            int64_t f = -1;
+            if (measure && P.wall_off[side] >= 0) {
+                const int j1 = axis == 0 ? idx[1] : idx[0], j2 = axis == 2 ? idx[1] : idx[2];
+                const int n1 = axis == 0 ? P.ny : P.nx;
+                f = P.wall_off[side] + j1 + (int64_t)n1 * j2;
+            }
+            double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
+            if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+            if (model == HS_PATCH_DIFFUSE)
+                diffuse_reflect(P, P.patch_T[side], P.patch_U[side], nw, t_id, U, erot, vib, P.capacity, r);
+            else if (U[axis] * (double)dir > 0.0)
+                U[axis] = -U[axis];
+            if (f >= 0) {
+                const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+                wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
+            }
         }
     }
     b.px[i] = p[0]; b.py[i] = p[1]; b.pz[i] = p[2];

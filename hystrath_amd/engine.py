@@ -24,7 +24,8 @@ EXPORTS = [
     "hs_dsmc_fill", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
     "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples",
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
-    "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags",
+    "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
+    "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide"]
@@ -87,6 +88,9 @@ def load_library(path=LIB_PATH):
                                P(C.c_double)], C.c_int),
         "hs_dsmc_device_copy": ([C.c_int64, C.c_int32, P(C.c_double)], C.c_int),
         "hs_dsmc_set_flags": ([eng, C.c_int32], C.c_int),
+        "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
+        "hs_dsmc_wall_faces": ([eng, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)], C.c_int),
+        "hs_dsmc_read_wall_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_int64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -263,6 +267,28 @@ class Engine:
 
     def reset_samples(self):
         self._ck(self.L.hs_dsmc_reset_samples(self.h), "reset_samples")
+
+    def wall_faces(self):
+        """dict of side, cell, area, normal[n,3] of the wall faces"""
+        n = C.c_int64()
+        self._ck(self.L.hs_dsmc_wall_face_count(self.h, C.byref(n)), "wall_face_count")
+        n = n.value
+        side = np.zeros(n, np.int32)
+        cell = np.zeros(n, np.int32)
+        area = np.zeros(n)
+        normal = np.zeros((n, 3))
+        self._ck(self.L.hs_dsmc_wall_faces(self.h, _i(side), _i(cell), _d(area), _d(normal)), "wall_faces")
+        return dict(side=side, cell=cell, area=area, normal=normal)
+
+    def wall_samples(self):
+        """([species*HS_WS_COUNT + slot, face] sums, sampled steps)"""
+        n = C.c_int64()
+        self._ck(self.L.hs_dsmc_wall_face_count(self.h, C.byref(n)), "wall_face_count")
+        rows = self.case.cfg.n_species * abi.HS_WS_COUNT
+        out = np.zeros(rows * max(n.value, 1))
+        ns = C.c_int64()
+        self._ck(self.L.hs_dsmc_read_wall_samples(self.h, _d(out), out.size, C.byref(ns)), "read_wall_samples")
+        return out[: rows * n.value].reshape(rows, n.value), ns.value
 
     def counters(self):
         out = np.zeros(len(COUNTERS), np.int64)

@@ -181,3 +181,69 @@ def global_temperature(samples, case, type_ids=None):
     er = sum(S[s]["EROT"].sum() for s in ids)
     Trot = 2.0 * er / (KB * z) if z > 0 else 0.0
     return Ttr, Trot, N
+
+
+def wall_fields(wall_sums, nsteps, case, faces, type_ids=None):
+    """Boundary values of one dsmcVolFields model on the wall faces
+    (dsmcVolFields.C:1890-2126) from the engine's wall-face sums
+    [species*HS_WS_COUNT + slot][face].  `faces` is the engine's wall-face
+    list (side, cell, area, normal).  Returns per-face arrays: rhoN, rhoM,
+    U[3], Ttra, Trot, Tvib, Tov, fD[3], p (= fD.n), tau (tangential |fD|)
+    and q (wall heat flux, positive into the wall)."""
+    cfg = case.cfg
+    sp = case.species
+    ids = list(range(cfg.n_species)) if type_ids is None else list(type_ids)
+    nf = wall_sums.shape[1]
+    W = [wall_sums[s * abi.HS_WS_COUNT:(s + 1) * abi.HS_WS_COUNT] for s in range(cfg.n_species)]
+    T = max(nsteps, 1)
+    fn = cfg.fn
+    rhoNBF = sum(W[s][abi.HS_WS_RHON] for s in ids)
+    rhoMBF = sum(sp[s].mass * W[s][abi.HS_WS_RHON] for s in ids)
+    keBF = sum(W[s][abi.HS_WS_KE] for s in ids)
+    momBF = sum(W[s][abi.HS_WS_MOMX:abi.HS_WS_MOMZ + 1] for s in ids)
+    erotBF = sum(W[s][abi.HS_WS_EROT] for s in ids)
+    zrotBF = sum(sp[s].rot_dof * W[s][abi.HS_WS_RHON] for s in ids)
+    qBF = sum(W[s][abi.HS_WS_Q] for s in ids)
+    fDBF = sum(W[s][abi.HS_WS_FDX:abi.HS_WS_FDZ + 1] for s in ids)
+    f = {}
+    with np.errstate(divide="ignore", invalid="ignore"):
+        rhoN = rhoNBF * fn / T
+        rhoM = rhoMBF * fn / T
+        lke = keBF * fn / T
+        ok = rhoM > VSMALL
+        U = np.where(ok, momBF / np.where(ok, rhoMBF, 1.0), 0.0)
+        Ttra = np.where(ok, 2.0 / (3.0 * KB * np.where(ok, rhoN, 1.0)) * (lke - 0.5 * rhoM * (U * U).sum(axis=0)),
+                        0.0)
+        zeta_rot_tot = np.where(rhoNBF > SMALL, zrotBF / np.where(rhoNBF > SMALL, rhoNBF, 1.0), 0.0)
+        Trot = np.where(zrotBF > SMALL, 2.0 * erotBF / (KB * np.where(zrotBF > SMALL, zrotBF, 1.0)), 0.0)
+        Tvib = np.zeros(nf)
+        zvib = np.zeros(nf)
+        mol = np.zeros(nf)
+        for s in ids:
+            ns_ = W[s][abi.HS_WS_RHON]
+            zeta_s = np.zeros(nf)
+            zt = np.zeros(nf)
+            for mo in range(sp[s].n_vib_modes):
+                th = sp[s].theta_v[mo]
+                good = ns_ > SMALL
+                imean = np.where(good, W[s][abi.HS_WS_EVIB0 + mo] / (KB * th * np.where(good, ns_, 1.0)), 0.0)
+                good &= imean > SMALL
+                lf = np.where(good, np.log(1.0 + 1.0 / np.where(good, imean, 1.0)), 0.0)
+                tv = np.where(good, th / np.where(good, lf, 1.0), 0.0)
+                zm = np.where(good, 2.0 * imean * lf, 0.0)
+                zeta_s += zm
+                zt += zm * tv
+            has = zeta_s > SMALL
+            mol += np.where(has, ns_, 0.0)
+            Tvib += np.where(has, ns_ * zt / np.where(has, zeta_s, 1.0), 0.0)
+            zvib += np.where(has, ns_ * zeta_s, 0.0)
+        m = mol > SMALL
+        Tvib = np.where(m, Tvib / np.where(m, mol, 1.0), 0.0)
+        zvib = np.where(m, zvib / np.where(m, mol, 1.0), 0.0)
+        Tov = (3.0 * Ttra + zeta_rot_tot * Trot + zvib * Tvib) / (3.0 + zeta_rot_tot + zvib)
+    fD = fDBF / T
+    n = faces["normal"].T
+    p = (fD * n).sum(axis=0)
+    tau = np.sqrt(np.maximum((fD * fD).sum(axis=0) - p * p, 0.0))
+    f.update(rhoN=rhoN, rhoM=rhoM, U=U, Ttra=Ttra, Trot=Trot, Tvib=Tvib, Tov=Tov, fD=fD, p=p, tau=tau, q=qBF / T)
+    return f

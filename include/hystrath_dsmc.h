@@ -65,6 +65,7 @@ enum hs_patch_model {
 #define HS_FLAG_HEATFLUX      0x2  /* measureHeatFluxShearStress sums         */
 #define HS_FLAG_COLL_SEP      0x4  /* collision separation (meanCollisionSeparation) */
 #define HS_FLAG_TIMING        0x8  /* per-kernel HIP event timing             */
+#define HS_FLAG_WALL         0x10  /* boundaryMeasurements at diffuse/specular walls */
 
 /* sides of the block, patch[] order */
 enum hs_side { HS_XMIN = 0, HS_XMAX = 1, HS_YMIN = 2, HS_YMAX = 3, HS_ZMIN = 4, HS_ZMAX = 5 };
@@ -169,6 +170,27 @@ enum hs_species_slot {
 #define HS_HF_COUNT 13
 int32_t hs_dsmc_sample_slots(const hs_config* cfg);
 
+/* Wall-face sampling with HS_FLAG_WALL: the per-hit sums of
+ * dsmcPatchBoundary::measurePropertiesBeforeControl / AfterControl
+ * (dsmcPatchBoundary.C:263-483) that dsmcVolFields accumulates into its
+ * boundary fields (dsmcVolFields.C:1292-1356), cumulative over sampled steps,
+ * plane-major FP64 [species*HS_WS_COUNT + slot][wall face].  Every hit adds
+ * a "before" and an "after" term weighted by w = 1/max(|U.n| A dt, 1e-15)
+ * (A = face area, n = unit normal pointing out of the gas); q and fD add
+ * Fn*(pre - post)/(dt A).  Wall faces are measured on diffuse and specular
+ * walls (dsmcDiffuseWallPatch.C:167, dsmcSpecularWallPatch.C:81), not on
+ * deletion patches (dsmcDeletionPatch.C:115). */
+enum hs_wall_slot {
+    HS_WS_RHON = 0,    /* sum w                      (speciesRhoNBF)          */
+    HS_WS_KE,          /* sum 1/2 m|U|^2 w           (speciesLinearKEBF; speciesMccBF = 2x) */
+    HS_WS_MOMX, HS_WS_MOMY, HS_WS_MOMZ, /* sum m U_t w, U_t = U - (U.n)n (speciesMomentumBF) */
+    HS_WS_EROT,        /* sum ERot w                 (speciesErotBF)          */
+    HS_WS_EVIB0,       /* sum eVib_m w per mode, 3 slots (speciesEvibModBF)   */
+    HS_WS_Q = HS_WS_EVIB0 + HS_MAX_VIB_MODES, /* sum Fn (E_pre - E_post)/(dt A) (speciesqBF) */
+    HS_WS_FDX, HS_WS_FDY, HS_WS_FDZ,          /* sum Fn m (U_pre - U_post)/(dt A) (speciesfDBF) */
+    HS_WS_COUNT
+};
+
 /* ---- engine lifetime ---------------------------------------------------- */
 typedef struct hs_dsmc_engine hs_dsmc_engine;
 
@@ -228,7 +250,7 @@ int hs_dsmc_sort(hs_dsmc_engine* eng);                   /* buildCellOccupancy  
 int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample); /* noTimeCounter::collide (+ sampling) */
 int hs_dsmc_advance_step(hs_dsmc_engine* eng);           /* step counter += 1 */
 int hs_dsmc_synchronize(hs_dsmc_engine* eng);
-/* Switch HS_FLAG_SAMPLE / HS_FLAG_COLL_SEP / HS_FLAG_TIMING between steps
+/* Switch HS_FLAG_SAMPLE / HS_FLAG_COLL_SEP / HS_FLAG_TIMING / HS_FLAG_WALL between steps
  * (the sampling schedule of timeDataMeas, timeDataMeas.C:71-104);
  * HS_FLAG_HEATFLUX can only be kept as it was at create time. */
 int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
@@ -237,8 +259,22 @@ int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
 /* dsmcVolFields cumulative sums, plane-major [slot][cell]; nsteps_out gets
  * the number of sampled steps (nTimeSteps_). */
 int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
-/* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290) */
+/* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290); clears the cell
+ * and the wall-face sums */
 int hs_dsmc_reset_samples(hs_dsmc_engine* eng);
+/* Wall faces (the boundary faces of boundaryMeasurements, boundaryMeasurements.H:74-90):
+ * first every face of each diffuse or specular side, sides in hs_side order,
+ * faces ordered by the two tangential cell indices (x before y before z, the
+ * first one fastest); then one face per cell that the embedded cylinder wall
+ * crosses (arc longer than 1e-6 of the cell size), in cell order, with
+ * area = arc length inside the cell times the cell's z size and the
+ * area-weighted normal.  Any output may be NULL:
+ * side[f] = hs_side or 6 for the cylinder, cell[f] = adjacent cell,
+ * area[f], normal[3*f+a] = unit normal pointing out of the gas. */
+int hs_dsmc_wall_face_count(hs_dsmc_engine* eng, int64_t* n_faces);
+int hs_dsmc_wall_faces(hs_dsmc_engine* eng, int32_t* side, int32_t* cell, double* area, double* normal);
+/* [species*HS_WS_COUNT + slot][wall face] sums; nsteps_out as above */
+int hs_dsmc_read_wall_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
 /* counters since the last reset: [0] candidates, [1] collisions,
  * [2] deleted, [3] injected, [4] wall hits, [5] tracking-iteration
  * overflows, [6] oversize cells / candidate clamps, [7] particles dropped at

@@ -642,7 +642,200 @@ static void diffuse_reflect(const hs_config* cfg, const hs_patch* pt, const doub
     if (vib) equip_vib(r, T, sp, vib, vstride);
 }
 
-static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, int64_t i, int is_new)
+/* ------------------------------------------------------------------------ */
+/* wall-face sampling (dsmcPatchBoundary::measurePropertiesBeforeControl /  */
+/* AfterControl, dsmcPatchBoundary.C:263-483)                               */
+/* ------------------------------------------------------------------------ */
+static int cmp_double(const void* a, const void* b)
+{
+    const double x = *(const double*)a, y = *(const double*)b;
+    return (x > y) - (x < y);
+}
+
+double or_arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2])
+{
+    /* the circle's crossings of the four lines split [0, 2pi) into arcs that
+     * lie wholly inside or wholly outside the rectangle */
+    double th[10];
+    int k = 0;
+    th[k++] = 0.0;
+    th[k++] = OR_TWO_PI;
+    const double xs[2] = { x0, x1 }, ys[2] = { y0, y1 };
+    for (int j = 0; j < 2; ++j) {
+        const double u = (xs[j] - cx) / r;
+        if (u > -1.0 && u < 1.0) {
+            const double a = acos(u);
+            th[k++] = a;
+            th[k++] = OR_TWO_PI - a;
+        }
+        const double v = (ys[j] - cy) / r;
+        if (v > -1.0 && v < 1.0) {
+            double a = asin(v);
+            th[k++] = OR_PI - a;
+            if (a < 0.0) a += OR_TWO_PI;
+            th[k++] = a;
+        }
+    }
+    qsort(th, (size_t)k, sizeof(double), cmp_double);
+    double len = 0.0, nx = 0.0, ny = 0.0;
+    for (int j = 0; j + 1 < k; ++j) {
+        const double dth = th[j + 1] - th[j];
+        if (!(dth > 0.0)) continue;
+        const double m = 0.5 * (th[j] + th[j + 1]);
+        const double px = cx + r * cos(m), py = cy + r * sin(m);
+        if (px >= x0 && px <= x1 && py >= y0 && py <= y1) {
+            len += r * dth;
+            nx -= r * dth * cos(m);
+            ny -= r * dth * sin(m);
+        }
+    }
+    if (n_out) {
+        const double nm = sqrt(nx * nx + ny * ny);
+        n_out[0] = nm > 0.0 ? nx / nm : 0.0;
+        n_out[1] = nm > 0.0 ? ny / nm : 0.0;
+    }
+    return len;
+}
+
+static int side_measured(const hs_config* cfg, int side)
+{
+    const int m = cfg->patch[side].model;
+    return m == HS_PATCH_DIFFUSE || m == HS_PATCH_SPECULAR;
+}
+
+/* wall-face layout: side faces first (first tangential index fastest), then
+ * one face per cell crossed by the cylinder wall */
+typedef struct {
+    int64_t n;
+    int64_t off[6];    /* first face of each measured side, -1 otherwise */
+    int32_t* cyl_face; /* [nc] or NULL */
+    double* area;      /* [n] */
+} or_wall;
+
+int64_t or_wall_faces(const hs_config* cfg, int32_t* side, int32_t* cell, double* area, double* normal)
+{
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    int64_t f = 0;
+    for (int sd = 0; sd < 6; ++sd) {
+        if (!side_measured(cfg, sd)) continue;
+        const int axis = sd / 2, dir = (sd & 1) ? 1 : -1;
+        const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+        for (int j2 = 0; j2 < nn[a2]; ++j2)
+            for (int j1 = 0; j1 < nn[a1]; ++j1, ++f) {
+                int idx[3];
+                idx[axis] = dir > 0 ? nn[axis] - 1 : 0;
+                idx[a1] = j1;
+                idx[a2] = j2;
+                if (side) side[f] = sd;
+                if (cell) cell[f] = idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2]);
+                if (area) area[f] = cfg->delta[a1] * cfg->delta[a2];
+                if (normal) {
+                    normal[3 * f] = normal[3 * f + 1] = normal[3 * f + 2] = 0.0;
+                    normal[3 * f + axis] = (double)dir;
+                }
+            }
+    }
+    if (cfg->cyl_enabled) {
+        for (int k = 0; k < cfg->nz; ++k)
+            for (int j = 0; j < cfg->ny; ++j)
+                for (int i = 0; i < cfg->nx; ++i) {
+                    const double x0 = face_coord(cfg, 0, i), x1 = face_coord(cfg, 0, i + 1);
+                    const double y0 = face_coord(cfg, 1, j), y1 = face_coord(cfg, 1, j + 1);
+                    double nw[2];
+                    const double len = or_arc_in_rect(cfg->cyl_center[0], cfg->cyl_center[1], cfg->cyl_radius,
+                                                      x0, x1, y0, y1, nw);
+                    /* arcs shorter than 1e-6 of the cell (slivers where the circle
+                     * grazes a grid line) carry no face */
+                    if (!(len > 1e-6 * (cfg->delta[0] < cfg->delta[1] ? cfg->delta[0] : cfg->delta[1]))) continue;
+                    if (side) side[f] = 6;
+                    if (cell) cell[f] = i + cfg->nx * (j + cfg->ny * k);
+                    if (area) area[f] = len * cfg->delta[2];
+                    if (normal) { normal[3 * f] = nw[0]; normal[3 * f + 1] = nw[1]; normal[3 * f + 2] = 0.0; }
+                    ++f;
+                }
+    }
+    return f;
+}
+
+static void wall_layout(const hs_config* cfg, or_wall* w)
+{
+    memset(w, 0, sizeof(*w));
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    int64_t f = 0;
+    for (int sd = 0; sd < 6; ++sd) {
+        w->off[sd] = -1;
+        if (!side_measured(cfg, sd)) continue;
+        const int axis = sd / 2;
+        const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+        w->off[sd] = f;
+        f += (int64_t)nn[a1] * nn[a2];
+    }
+    w->n = or_wall_faces(cfg, NULL, NULL, NULL, NULL);
+    w->area = (double*)malloc((size_t)(w->n > 0 ? w->n : 1) * sizeof(double));
+    int32_t* cell = (int32_t*)malloc((size_t)(w->n > 0 ? w->n : 1) * sizeof(int32_t));
+    or_wall_faces(cfg, NULL, cell, w->area, NULL);
+    if (cfg->cyl_enabled) {
+        const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+        w->cyl_face = (int32_t*)malloc((size_t)nc * sizeof(int32_t));
+        for (int64_t c = 0; c < nc; ++c) w->cyl_face[c] = -1;
+        for (int64_t g = f; g < w->n; ++g) w->cyl_face[cell[g]] = (int32_t)g;
+    }
+    free(cell);
+}
+
+static void wall_free(or_wall* w)
+{
+    free(w->area);
+    free(w->cyl_face);
+}
+
+/* the flux-weighted terms of one side of a hit (before or after the
+ * interaction, dsmcPatchBoundary.C:263-357 and :359-426); returns the
+ * parcel's total energy */
+static double wall_side_terms(const hs_config* cfg, const or_wall* w, double* sums, int64_t f, int t,
+                              const double nw[3], const double U[3], const float* erot, const uint16_t* vib,
+                              int64_t vstride)
+{
+    const hs_species* sp = &cfg->species[t];
+    const double m = sp->mass, adt = w->area[f] * cfg->dt;
+    const double un = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+    double dn = fabs(un) * adt;
+    if (dn < OR_SMALL) dn = OR_SMALL;
+    const double wt = 1.0 / dn;
+    const double ke = 0.5 * m * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
+    const int64_t st = w->n;
+    double* base = sums + (int64_t)t * HS_WS_COUNT * st + f;
+    base[HS_WS_RHON * st] += wt;
+    base[HS_WS_KE * st] += ke * wt;
+    for (int c = 0; c < 3; ++c) base[(HS_WS_MOMX + c) * st] += m * (U[c] - un * nw[c]) * wt;
+    double e = ke;
+    if (sp->rot_dof > 0) {
+        const double er = (double)*erot;
+        base[HS_WS_EROT * st] += er * wt;
+        e += er;
+    }
+    for (int k = 0; k < sp->n_vib_modes; ++k) {
+        const double ev = (double)vib[k * vstride] * HS_KB * sp->theta_v[k];
+        base[(HS_WS_EVIB0 + k) * st] += ev * wt;
+        e += ev;
+    }
+    return e;
+}
+
+/* heat flux and force density of one hit (dsmcPatchBoundary.C:428-482) */
+static void wall_exchange_terms(const hs_config* cfg, const or_wall* w, double* sums, int64_t f, int t, double ea,
+                                const double Ua[3], double eb, const double Ub[3])
+{
+    const double m = cfg->species[t].mass;
+    const double cq = cfg->fn / (w->area[f] * cfg->dt);
+    const int64_t st = w->n;
+    double* base = sums + (int64_t)t * HS_WS_COUNT * st + f;
+    base[HS_WS_Q * st] += cq * (ea - eb);
+    for (int c = 0; c < 3; ++c) base[(HS_WS_FDX + c) * st] += cq * (m * Ua[c] - m * Ub[c]);
+}
+
+static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, const or_wall* w, int64_t i,
+                     int is_new)
 {
     or_rng r;
     rng_init(&r, cfg->seed, OR_RNG_MOVE, (uint32_t)i, s->step, 0);
@@ -714,6 +907,10 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, int
             double nx_ = cfg->cyl_center[0] - p[0], ny_ = cfg->cyl_center[1] - p[1];
             const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
             const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
+            const int32_t f = w ? w->cyl_face[idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2])] : -1;
+            double e_pre = 0.0;
+            const double u_pre[3] = { U[0], U[1], U[2] };
+            if (f >= 0) e_pre = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
             if (cfg->cyl_patch.model == HS_PATCH_DIFFUSE) {
                 diffuse_reflect(cfg, &cfg->cyl_patch, nw, t_id, U, erot, vib, s->capacity, &r);
             } else {
@@ -721,31 +918,45 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, int
                 if (und > 0.0)
                     for (int b = 0; b < 3; ++b) U[b] = U[b] - 2.0 * und * nw[b];
             }
+            if (f >= 0) {
+                const double e_post = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
+                wall_exchange_terms(cfg, w, s->wall, f, t_id, e_pre, u_pre, e_post, U);
+            }
             continue;
         }
         p[axis] = face_coord(cfg, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
         const int ni = idx[axis] + dir;
         if (ni >= 0 && ni < nn[axis]) { idx[axis] = ni; continue; }
-        const hs_patch* pt = &cfg->patch[2 * axis + (dir > 0 ? 1 : 0)];
-        switch (pt->model) {
-        case HS_PATCH_CYCLIC:
+        const int sd = 2 * axis + (dir > 0 ? 1 : 0);
+        const hs_patch* pt = &cfg->patch[sd];
+        if (pt->model == HS_PATCH_CYCLIC) {
             p[axis] = face_coord(cfg, axis, dir > 0 ? 0 : nn[axis]);
             idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
-            break;
-        case HS_PATCH_DIFFUSE: {
-            s->counters[4] += 1;
-            double nw[3] = { 0.0, 0.0, 0.0 };
-            nw[axis] = (double)dir;
-            diffuse_reflect(cfg, pt, nw, t_id, U, erot, vib, s->capacity, &r);
-            break;
+            continue;
         }
-        case HS_PATCH_DELETION:
+        if (pt->model == HS_PATCH_DELETION) {
             keep = 0;
-            break;
-        default: /* specular wall, or an empty/symmetry plane */
-            s->counters[4] += 1;
-            if (U[axis] * (double)dir > 0.0) U[axis] = -U[axis];
-            break;
+            continue;
+        }
+        /* diffuse or specular wall (an empty plane is never crossed) */
+        s->counters[4] += 1;
+        double nw[3] = { 0.0, 0.0, 0.0 };
+        nw[axis] = (double)dir;
+        int64_t f = -1;
+        if (w && w->off[sd] >= 0) {
+            const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
+            f = w->off[sd] + idx[a1] + (int64_t)nn[a1] * idx[a2];
+        }
+        double e_pre = 0.0;
+        const double u_pre[3] = { U[0], U[1], U[2] };
+        if (f >= 0) e_pre = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
+        if (pt->model == HS_PATCH_DIFFUSE)
+            diffuse_reflect(cfg, pt, nw, t_id, U, erot, vib, s->capacity, &r);
+        else if (U[axis] * (double)dir > 0.0)
+            U[axis] = -U[axis];
+        if (f >= 0) {
+            const double e_post = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
+            wall_exchange_terms(cfg, w, s->wall, f, t_id, e_pre, u_pre, e_post, U);
         }
     }
     P->px[i] = p[0]; P->py[i] = p[1]; P->pz[i] = p[2];
@@ -758,7 +969,11 @@ int or_move(const hs_config* cfg, or_state* s, int64_t n_old)
 {
     or_derived d;
     derive(cfg, &d);
-    for (int64_t i = 0; i < s->n; ++i) move_one(cfg, &d, s, i, i >= n_old);
+    or_wall w;
+    const int measure = s->wall && (cfg->flags & HS_FLAG_WALL) && (cfg->flags & HS_FLAG_SAMPLE);
+    if (measure) wall_layout(cfg, &w);
+    for (int64_t i = 0; i < s->n; ++i) move_one(cfg, &d, s, measure ? &w : NULL, i, i >= n_old);
+    if (measure) wall_free(&w);
     return 0;
 }
 

@@ -38,6 +38,9 @@ typedef struct or_state {
     uint32_t step;
     int64_t counters[8];     /* candidates, collisions, deleted, injected,
                                 wall hits, track overflow, -, - */
+    double* wall;            /* [n_species*HS_WS_COUNT][wall faces] or NULL;
+                                summed by or_move when HS_FLAG_WALL and
+                                HS_FLAG_SAMPLE are set                    */
 } or_state;
 
 /* primitives (exported for pinning tests) */
@@ -71,6 +74,12 @@ int or_fill(const hs_config* cfg, or_state* s, const double* n, double t, double
             double tvib, const double u[3], int32_t exact_per_cell, uint32_t fill_id);
 int or_inject(const hs_config* cfg, or_state* s);
 int or_move(const hs_config* cfg, or_state* s, int64_t n_old);
+/* wall faces of boundaryMeasurements in the engine's order; any output may be
+ * NULL; returns the face count */
+int64_t or_wall_faces(const hs_config* cfg, int32_t* side, int32_t* cell, double* area, double* normal);
+/* length of the circle (cx, cy, r) inside [x0,x1]x[y0,y1]; n_out (may be
+ * NULL) gets the length-weighted unit normal pointing to the centre */
+double or_arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2]);
 int or_sort(const hs_config* cfg, or_state* s);
 int or_collide(const hs_config* cfg, or_state* s, int32_t sample);
 int or_evolve(const hs_config* cfg, or_state* s, int32_t nsteps);

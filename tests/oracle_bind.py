@@ -30,6 +30,7 @@ class or_state(C.Structure):
         ("start", C.POINTER(C.c_int32)),
         ("step", C.c_uint32),
         ("counters", C.c_int64 * 8),
+        ("wall", C.POINTER(C.c_double)),
     ]
 
 
@@ -70,6 +71,10 @@ def lib():
         L.or_sort.argtypes = [cfgp, stp]
         L.or_collide.argtypes = [cfgp, stp, C.c_int32]
         L.or_evolve.argtypes = [cfgp, stp, C.c_int32]
+        L.or_wall_faces.argtypes = [cfgp, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)]
+        L.or_wall_faces.restype = C.c_int64
+        L.or_arc_in_rect.argtypes = [C.c_double] * 7 + [P(C.c_double)]
+        L.or_arc_in_rect.restype = C.c_double
         _lib = L
     return _lib
 
@@ -156,6 +161,8 @@ class OracleState:
         self.inflow = np.zeros(max(case.cfg.n_species * nf, 1)) if nf else None
         self.count = np.zeros(nc, np.int32)
         self.start = np.zeros(nc + 1, np.int32)
+        self.n_wall = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
+        self.wall = np.zeros(max(case.cfg.n_species * abi.HS_WS_COUNT * self.n_wall, 1))
         s = or_state()
         s.n = 0
         s.capacity = cap
@@ -167,6 +174,7 @@ class OracleState:
         s.inflow_acc = dptr(self.inflow) if nf else None
         s.count = iptr(self.count)
         s.start = iptr(self.start)
+        s.wall = dptr(self.wall)
         self.s = s
 
     @property
@@ -218,3 +226,23 @@ class OracleState:
 
     def sample_array(self):
         return self.samples.reshape(self.nslots, self.case.ncells)
+
+    def wall_array(self):
+        rows = self.case.cfg.n_species * abi.HS_WS_COUNT
+        return self.wall[: rows * self.n_wall].reshape(rows, self.n_wall)
+
+
+def wall_faces(case):
+    n = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
+    side = np.zeros(n, np.int32)
+    cell = np.zeros(n, np.int32)
+    area = np.zeros(n)
+    normal = np.zeros((n, 3))
+    lib().or_wall_faces(C.byref(case.cfg), iptr(side), iptr(cell), dptr(area), dptr(normal))
+    return dict(side=side, cell=cell, area=area, normal=normal)
+
+
+def arc_in_rect(cx, cy, r, x0, x1, y0, y1):
+    nrm = np.zeros(2)
+    length = lib().or_arc_in_rect(cx, cy, r, x0, x1, y0, y1, dptr(nrm))
+    return length, nrm
