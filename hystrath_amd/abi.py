@@ -25,6 +25,7 @@ HS_FLAG_HEATFLUX = 0x2
 HS_FLAG_COLL_SEP = 0x4
 HS_FLAG_TIMING = 0x8
 HS_FLAG_WALL = 0x10
+HS_FLAG_FACE_FLUX = 0x20
 
 HS_XMIN, HS_XMAX, HS_YMIN, HS_YMAX, HS_ZMIN, HS_ZMAX = range(6)
 

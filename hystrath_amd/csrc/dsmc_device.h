@@ -76,7 +76,24 @@ struct DevParams {
     const double* wall_area;    // [n_wall]
     const int32_t* cyl_face;    // [nc] wall face of the cylinder in each cell, -1 if none
     double* wall_sum;           // [ns*HS_WS_COUNT][n_wall]
+    int cyl_first;              // first cylinder face in the wall-face list
+    // face tracker (dsmcFaceTracker), HS_FLAG_FACE_FLUX
+    unsigned long long* face_flux;  // [ns][n_faces] two's-complement int64 counts
+    int64_t n_faces;
+    int64_t face_off[4];        // first face of the x, y, z planes; cylinder faces
 };
+
+// face-tracker face of plane k on axis a at the tangential indices of idx
+__device__ __forceinline__ int64_t mesh_face(const DevParams& P, int a, int k, const int idx[3]) {
+    const int j1 = a == 0 ? idx[1] : idx[0], j2 = a == 2 ? idx[1] : idx[2];
+    const int n1 = a == 0 ? P.ny : P.nx;
+    const int na = a == 0 ? P.nx : (a == 1 ? P.ny : P.nz);
+    return P.face_off[a] + k + (int64_t)(na + 1) * (j1 + (int64_t)n1 * j2);
+}
+
+__device__ __forceinline__ void track_face(const DevParams& P, int t, int64_t face, long long v) {
+    atomicAdd(&P.face_flux[(int64_t)t * P.n_faces + face], (unsigned long long)v);
+}
 
 // Device view of one particle buffer (structure of arrays).
 struct DevParticles {

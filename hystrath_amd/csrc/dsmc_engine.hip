@@ -63,6 +63,7 @@ struct hs_dsmc_engine {
     double* d_wall_area = nullptr;
     int32_t* d_cyl_face = nullptr;
     double* d_wall_sum = nullptr;
+    unsigned long long* d_face_flux = nullptr;
     DevScalars* d_S = nullptr;
     ScanWs ws;
     int64_t n_sampled = 0;
@@ -547,6 +548,21 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     eng->hp.wall_area = eng->d_wall_area;
     eng->hp.cyl_face = eng->d_cyl_face;
     eng->hp.wall_sum = eng->d_wall_sum;
+    eng->hp.cyl_first = (int)nw;
+    for (int64_t f = 0; f < nw; ++f)
+        if (eng->wall_side[f] == 6) { eng->hp.cyl_first = (int)f; break; }
+    // face tracker
+    {
+        const int64_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+        eng->hp.face_off[0] = 0;
+        eng->hp.face_off[1] = (nx + 1) * ny * nz;
+        eng->hp.face_off[2] = eng->hp.face_off[1] + nx * (ny + 1) * nz;
+        eng->hp.face_off[3] = eng->hp.face_off[2] + nx * ny * (nz + 1);
+        eng->hp.n_faces = eng->hp.face_off[3] + (nw - eng->hp.cyl_first);
+        if ((rc = dev_alloc(eng, &eng->d_face_flux, (int64_t)ns * eng->hp.n_faces))) return bail(rc);
+        launch_fill_i64((int64_t*)eng->d_face_flux, 0, (int64_t)ns * eng->hp.n_faces, eng->stream);
+        eng->hp.face_flux = eng->d_face_flux;
+    }
     if (hipMemcpy(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(eng->d_vol, eng->vol_h.data(), (size_t)nc * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "hipMemcpy of parameters failed"));
@@ -899,8 +915,26 @@ extern "C" int hs_dsmc_reset_samples(hs_dsmc_engine* eng) {
     launch_fill_f64(eng->d_samples, 0.0, (int64_t)eng->nslots * eng->nc, eng->stream);
     if (eng->hp.n_wall > 0)
         launch_fill_f64(eng->d_wall_sum, 0.0, (int64_t)eng->ns * HS_WS_COUNT * eng->hp.n_wall, eng->stream);
+    launch_fill_i64((int64_t*)eng->d_face_flux, 0, (int64_t)eng->ns * eng->hp.n_faces, eng->stream);
     eng->n_sampled = 0;
     return check_launch(eng, "reset samples");
+}
+
+extern "C" int hs_dsmc_face_count(hs_dsmc_engine* eng, int64_t* n_faces) {
+    if (!eng || !n_faces) return fail(eng, HS_E_INVALID, "NULL argument");
+    *n_faces = eng->hp.n_faces;
+    return 0;
+}
+
+extern "C" int hs_dsmc_read_face_flux(hs_dsmc_engine* eng, int64_t* out, int64_t n, int64_t* nsteps_out) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    const int64_t need = (int64_t)eng->ns * eng->hp.n_faces;
+    if (out && n < need) return fail(eng, HS_E_INVALID, "face flux buffer too small");
+    HS_TRY(hipSetDevice(eng->device));
+    if (out) HS_TRY(hipMemcpyAsync(out, eng->d_face_flux, need * 8, hipMemcpyDeviceToHost, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    if (nsteps_out) *nsteps_out = eng->n_sampled;
+    return 0;
 }
 
 extern "C" int hs_dsmc_wall_face_count(hs_dsmc_engine* eng, int64_t* n_faces) {

@@ -69,6 +69,7 @@ void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double
                        hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
+void launch_fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t st);
 void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, double* up,
                            double* uq, double* erp, double* erq, int32_t* vp, int32_t* vq, const double* uni,
                            int32_t nu, int32_t* used, hipStream_t st);

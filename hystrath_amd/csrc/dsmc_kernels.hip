@@ -145,6 +145,10 @@ __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
+__global__ void k_fill_i64(int64_t* p, int64_t v, int64_t n) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
 __global__ void k_fill_f64(double* p, double v, int64_t n) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = v;
@@ -160,6 +164,9 @@ void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st) {
 }
 void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st) {
     k_fill_f64<<<grid_for(n), BLOCK, 0, st>>>(p, v, n);
+}
+void launch_fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t st) {
+    k_fill_i64<<<grid_for(n), BLOCK, 0, st>>>(p, v, n);
 }
 
 // ---------------------------------------------------------------------------
@@ -309,6 +316,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
     for (int a = 0; a < 3; ++a)
         if (P.empty[a]) p[a] = P.centre[a];
     const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
+    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
     bool keep = true;
     int iter = 0;
     while (keep && t > HS_ROOTVSMALL) {
@@ -364,7 +372,8 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
             const double nx_ = P.cyl_c[0] - p[0], ny_ = P.cyl_c[1] - p[1];
             const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
             const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
-            const int32_t f = measure ? P.cyl_face[idx[0] + P.nx * (idx[1] + P.ny * idx[2])] : -1;
+            const int32_t wf = (measure || track) ? P.cyl_face[idx[0] + P.nx * (idx[1] + P.ny * idx[2])] : -1;
+            const int32_t f = measure ? wf : -1;
             double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
             if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
             if (P.cyl_model == HS_PATCH_DIFFUSE) {
@@ -378,17 +387,30 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
                 const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
                 wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
             }
+            if (track && wf >= 0)
+                track_face(P, t_id, P.face_off[3] + (wf - P.cyl_first),
+                           (U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2]) >= 0.0 ? 1 : -1);
             continue;
         }
         p[axis] = face_coord(P, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
         const int ni = idx[axis] + dir;
-        if (ni >= 0 && ni < nn[axis]) { idx[axis] = ni; continue; }
+        if (ni >= 0 && ni < nn[axis]) {
+            // internal face: Sf points +axis, U crosses it along dir
+            if (track) track_face(P, t_id, mesh_face(P, axis, dir > 0 ? idx[axis] + 1 : idx[axis], idx), dir);
+            idx[axis] = ni;
+            continue;
+        }
         const int side = 2 * axis + (dir > 0 ? 1 : 0);
         const int model = P.patch_model[side];
         if (model == HS_PATCH_CYCLIC) {
+            // counted, unsigned, on the coupled face of the other side
+            if (track) track_face(P, t_id, mesh_face(P, axis, dir > 0 ? 0 : nn[axis], idx), 1);
             p[axis] = face_coord(P, axis, dir > 0 ? 0 : nn[axis]);
             idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
         } else if (model == HS_PATCH_DELETION) {
+            if (track)
+                track_face(P, t_id, mesh_face(P, axis, dir > 0 ? nn[axis] : 0, idx),
+                           U[axis] * (double)dir >= 0.0 ? 1 : -1);
             keep = false;
         } else {
             // diffuse or specular wall (an empty plane is never crossed)
@@ -411,6 +433,9 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
                 const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
                 wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
             }
+            if (track)
+                track_face(P, t_id, mesh_face(P, axis, dir > 0 ? nn[axis] : 0, idx),
+                           U[axis] * (double)dir >= 0.0 ? 1 : -1);
         }
     }
     b.px[i] = p[0]; b.py[i] = p[1]; b.pz[i] = p[2];

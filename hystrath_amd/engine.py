@@ -25,7 +25,7 @@ EXPORTS = [
     "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples",
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
-    "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples",
+    "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide"]
@@ -91,6 +91,8 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_wall_faces": ([eng, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)], C.c_int),
         "hs_dsmc_read_wall_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_int64)], C.c_int),
+        "hs_dsmc_face_count": ([eng, P(C.c_int64)], C.c_int),
+        "hs_dsmc_read_face_flux": ([eng, P(C.c_int64), C.c_int64, P(C.c_int64)], C.c_int),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -279,6 +281,16 @@ class Engine:
         normal = np.zeros((n, 3))
         self._ck(self.L.hs_dsmc_wall_faces(self.h, _i(side), _i(cell), _d(area), _d(normal)), "wall_faces")
         return dict(side=side, cell=cell, area=area, normal=normal)
+
+    def face_flux(self):
+        """([species, face] int64 parcel fluxes of the face tracker, sampled steps)"""
+        n = C.c_int64()
+        self._ck(self.L.hs_dsmc_face_count(self.h, C.byref(n)), "face_count")
+        out = np.zeros((self.case.cfg.n_species, n.value), np.int64)
+        ns = C.c_int64()
+        self._ck(self.L.hs_dsmc_read_face_flux(self.h, out.ctypes.data_as(C.POINTER(C.c_int64)), out.size,
+                                               C.byref(ns)), "read_face_flux")
+        return out, ns.value
 
     def wall_samples(self):
         """([species*HS_WS_COUNT + slot, face] sums, sampled steps)"""

@@ -127,8 +127,18 @@ void writeHeader(std::ostream& o, const std::string& cls, const std::string& loc
       << cls << ";\n    location    \"" << loc << "\";\n    object      " << obj << ";\n}\n\n";
 }
 
+// Values of a field on wall patches, by patch name (faces in engine order).
+using PatchValues = std::map<std::string, std::vector<double>>;
+
+std::string patchType(const BlockMesh& m, const std::string& p) {
+    for (int s = 0; s < 6; ++s)
+        if (m.sidePatch[s] == p && (m.sideType[s] == "empty" || m.sideType[s] == "cyclic")) return m.sideType[s];
+    return "calculated";
+}
+
 void writeScalarField(const std::string& path, const std::string& time, const std::string& name,
-                      const std::string& dims, const std::vector<double>& v, const BlockMesh& m, int prec) {
+                      const std::string& dims, const std::vector<double>& v, const BlockMesh& m, int prec,
+                      const PatchValues* bv = nullptr) {
     std::ofstream o(path);
     if (!o) throw FatalError("cannot write " + path);
     writeHeader(o, "volScalarField", time, name);
@@ -137,19 +147,27 @@ void writeScalarField(const std::string& path, const std::string& time, const st
     for (double x : v) o << x << "\n";
     o << ")\n;\n\nboundaryField\n{\n";
     for (const auto& p : m.patchNames) {
-        std::string type = "calculated";
-        for (int s = 0; s < 6; ++s)
-            if (m.sidePatch[s] == p && (m.sideType[s] == "empty" || m.sideType[s] == "cyclic")) type = m.sideType[s];
+        const std::string type = patchType(m, p);
         o << "    " << p << "\n    {\n        type            " << type << ";\n";
-        if (type == "calculated") o << "        value           uniform 0;\n";
+        if (type == "calculated") {
+            auto it = bv ? bv->find(p) : PatchValues::const_iterator();
+            if (bv && it != bv->end()) {
+                o << "        value           nonuniform List<scalar> \n" << it->second.size() << "\n(\n";
+                for (double x : it->second) o << x << "\n";
+                o << ")\n;\n";
+            } else {
+                o << "        value           uniform 0;\n";
+            }
+        }
         o << "    }\n";
     }
     o << "}\n";
 }
 
+// bv values are (x y z) triples per face
 void writeVectorField(const std::string& path, const std::string& time, const std::string& name,
                       const std::string& dims, const std::vector<double>& x, const std::vector<double>& y,
-                      const std::vector<double>& z, const BlockMesh& m, int prec) {
+                      const std::vector<double>& z, const BlockMesh& m, int prec, const PatchValues* bv = nullptr) {
     std::ofstream o(path);
     if (!o) throw FatalError("cannot write " + path);
     writeHeader(o, "volVectorField", time, name);
@@ -158,11 +176,20 @@ void writeVectorField(const std::string& path, const std::string& time, const st
     for (size_t i = 0; i < x.size(); ++i) o << "(" << x[i] << " " << y[i] << " " << z[i] << ")\n";
     o << ")\n;\n\nboundaryField\n{\n";
     for (const auto& p : m.patchNames) {
-        std::string type = "calculated";
-        for (int s = 0; s < 6; ++s)
-            if (m.sidePatch[s] == p && (m.sideType[s] == "empty" || m.sideType[s] == "cyclic")) type = m.sideType[s];
+        const std::string type = patchType(m, p);
         o << "    " << p << "\n    {\n        type            " << type << ";\n";
-        if (type == "calculated") o << "        value           uniform (0 0 0);\n";
+        if (type == "calculated") {
+            auto it = bv ? bv->find(p) : PatchValues::const_iterator();
+            if (bv && it != bv->end()) {
+                const auto& w = it->second;
+                o << "        value           nonuniform List<vector> \n" << w.size() / 3 << "\n(\n";
+                for (size_t i = 0; i + 2 < w.size(); i += 3)
+                    o << "(" << w[i] << " " << w[i + 1] << " " << w[i + 2] << ")\n";
+                o << ")\n;\n";
+            } else {
+                o << "        value           uniform (0 0 0);\n";
+            }
+        }
         o << "    }\n";
     }
     o << "}\n";
@@ -311,6 +338,8 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
         cy = c[1];
         r = cd.scalar("radius");
         const std::string model = cd.word("boundaryModel");
+        cylPatch_ = cd.lookupOrDefault("patchName", std::string("cylinder"));
+        mesh_.patchNames.push_back(cylPatch_);
         cfg_.cyl_enabled = 1;
         cfg_.cyl_center[0] = cx;
         cfg_.cyl_center[1] = cy;
@@ -329,6 +358,12 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
     readBoundaries(readDictionary(case_ + "/system/boundariesDict"));
     if (isFile(case_ + "/system/fieldPropertiesDict")) readFields(readDictionary(case_ + "/system/fieldPropertiesDict"));
     cfg_.flags |= HS_FLAG_SAMPLE;
+    // boundaryMeasurements at every diffuse or specular wall
+    // (dsmcDiffuseWallPatch.C:167, dsmcSpecularWallPatch.C:81)
+    bool walls = cfg_.cyl_enabled != 0;
+    for (int sd = 0; sd < 6; ++sd)
+        walls = walls || cfg_.patch[sd].model == HS_PATCH_DIFFUSE || cfg_.patch[sd].model == HS_PATCH_SPECULAR;
+    if (walls) cfg_.flags |= HS_FLAG_WALL;
     if (isFile(case_ + "/system/chemReactDict")) {
         const dictionary cr = readDictionary(case_ + "/system/chemReactDict");
         if (cr.found("reactions") && !cr.list("reactions").empty())
@@ -763,6 +798,82 @@ VolFields computeVolFields(const dsmcCloud& cloud, const std::vector<double>& s,
     return f;
 }
 
+WallFields computeWallFields(const dsmcCloud& cloud, const std::vector<double>& W, int64_t nf,
+                             const std::vector<double>& normal, long long nSteps, const FieldModel& fm) {
+    const hs_config& cfg = cloud.config();
+    const double kB = HS_KB, fn = cfg.fn, T = (double)std::max(nSteps, 1LL);
+    const double SMALL = 1e-15, VSMALL = 1e-300;
+    auto at = [&](int sp, int k, int64_t f) { return W[((size_t)sp * HS_WS_COUNT + k) * nf + f]; };
+    WallFields w;
+    for (auto* v : { &w.rhoN, &w.rhoM, &w.Ttra, &w.Trot, &w.Tvib, &w.Tov, &w.p, &w.tau, &w.q }) v->assign(nf, 0.0);
+    w.U.assign(3 * nf, 0.0);
+    w.fD.assign(3 * nf, 0.0);
+    for (int64_t f = 0; f < nf; ++f) {
+        double rhoNBF = 0, rhoMBF = 0, ke = 0, mom[3] = { 0, 0, 0 }, erot = 0, zrot = 0, q = 0, fd[3] = { 0, 0, 0 };
+        for (int sp : fm.typeIds) {
+            const hs_species& c = cfg.species[sp];
+            const double n = at(sp, HS_WS_RHON, f);
+            rhoNBF += n;
+            rhoMBF += c.mass * n;
+            ke += at(sp, HS_WS_KE, f);
+            for (int a = 0; a < 3; ++a) {
+                mom[a] += at(sp, HS_WS_MOMX + a, f);
+                fd[a] += at(sp, HS_WS_FDX + a, f);
+            }
+            erot += at(sp, HS_WS_EROT, f);
+            zrot += c.rot_dof * n;
+            q += at(sp, HS_WS_Q, f);
+        }
+        const double rhoN = rhoNBF * fn / T, rhoM = rhoMBF * fn / T, lke = ke * fn / T;
+        w.rhoN[f] = rhoN;
+        w.rhoM[f] = rhoM;
+        double U[3] = { 0, 0, 0 };
+        if (rhoM > VSMALL) {
+            for (int a = 0; a < 3; ++a) U[a] = mom[a] / rhoMBF;
+            w.Ttra[f] = 2.0 / (3.0 * kB * rhoN) * (lke - 0.5 * rhoM * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]));
+        }
+        const double zetaRotTot = rhoNBF > SMALL ? zrot / rhoNBF : 0.0;
+        w.Trot[f] = zrot > SMALL ? 2.0 * erot / (kB * zrot) : 0.0;
+        double Tvib = 0.0, zvib = 0.0, mol = 0.0;
+        for (int sp : fm.typeIds) {
+            const hs_species& c = cfg.species[sp];
+            const double ns = at(sp, HS_WS_RHON, f);
+            if (!(ns > SMALL)) continue;
+            double zs = 0.0, zt = 0.0;
+            for (int m = 0; m < c.n_vib_modes; ++m) {
+                const double iMean = at(sp, HS_WS_EVIB0 + m, f) / (kB * c.theta_v[m] * ns);
+                if (!(iMean > SMALL)) continue;
+                const double lf = std::log(1.0 + 1.0 / iMean);
+                const double zm = 2.0 * iMean * lf;
+                zs += zm;
+                zt += zm * c.theta_v[m] / lf;
+            }
+            if (zs > SMALL) {
+                mol += ns;
+                Tvib += ns * zt / zs;
+                zvib += ns * zs;
+            }
+        }
+        if (mol > SMALL) {
+            Tvib /= mol;
+            zvib /= mol;
+        }
+        w.Tvib[f] = Tvib;
+        w.Tov[f] = (3.0 * w.Ttra[f] + zetaRotTot * w.Trot[f] + zvib * Tvib) / (3.0 + zetaRotTot + zvib);
+        double pn = 0.0, f2 = 0.0;
+        for (int a = 0; a < 3; ++a) {
+            w.U[3 * f + a] = U[a];
+            w.fD[3 * f + a] = fd[a] / T;
+            pn += w.fD[3 * f + a] * normal[3 * f + a];
+            f2 += w.fD[3 * f + a] * w.fD[3 * f + a];
+        }
+        w.p[f] = pn;
+        w.tau[f] = std::sqrt(std::max(f2 - pn * pn, 0.0));
+        w.q[f] = q / T;
+    }
+    return w;
+}
+
 void dsmcCloud::write() {
     const std::string tn = timeName();
     const std::string dir = case_ + "/" + tn;
@@ -771,20 +882,56 @@ void dsmcCloud::write() {
     std::vector<double> s((size_t)nslots * vol_.size());
     int64_t nsampled = 0;
     check(hs_dsmc_read_samples(eng_, s.data(), (int64_t)s.size(), &nsampled), "read samples");
+    // wall faces (boundaryMeasurements)
+    int64_t nw = 0;
+    check(hs_dsmc_wall_face_count(eng_, &nw), "wall face count");
+    std::vector<int32_t> wside(nw);
+    std::vector<double> wnormal(3 * nw), wsums((size_t)cfg_.n_species * HS_WS_COUNT * nw);
+    if (nw) {
+        check(hs_dsmc_wall_faces(eng_, wside.data(), nullptr, nullptr, wnormal.data()), "wall faces");
+        check(hs_dsmc_read_wall_samples(eng_, wsums.data(), (int64_t)wsums.size(), nullptr), "wall samples");
+    }
+    auto patchOf = [&](int64_t f) { return wside[f] < 6 ? mesh_.sidePatch[wside[f]] : cylPatch_; };
+    auto byPatch = [&](const std::vector<double>& v, int width) {
+        PatchValues out;
+        for (int64_t f = 0; f < nw; ++f) {
+            auto& dst = out[patchOf(f)];
+            for (int a = 0; a < width; ++a) dst.push_back(v[(size_t)f * width + a]);
+        }
+        return out;
+    };
     bool reset = false;
     for (const auto& fm : fields_) {
         const VolFields f = computeVolFields(*this, s, nsampled, fm);
+        const WallFields w = computeWallFields(*this, wsums, nw, wnormal, nsampled, fm);
         const std::string n = "_" + fm.fieldName;
-        writeScalarField(dir + "/rhoN" + n, tn, "rhoN" + n, "[0 -3 0 0 0 0 0]", f.rhoN, mesh_, precision_);
-        writeScalarField(dir + "/rhoM" + n, tn, "rhoM" + n, "[1 -3 0 0 0 0 0]", f.rhoM, mesh_, precision_);
-        writeScalarField(dir + "/p" + n, tn, "p" + n, "[1 -1 -2 0 0 0 0]", f.p, mesh_, precision_);
-        writeScalarField(dir + "/Ttra" + n, tn, "Ttra" + n, "[0 0 0 1 0 0 0]", f.Ttra, mesh_, precision_);
-        writeScalarField(dir + "/Trot" + n, tn, "Trot" + n, "[0 0 0 1 0 0 0]", f.Trot, mesh_, precision_);
-        writeScalarField(dir + "/Tvib" + n, tn, "Tvib" + n, "[0 0 0 1 0 0 0]", f.Tvib, mesh_, precision_);
-        writeScalarField(dir + "/Tov" + n, tn, "Tov" + n, "[0 0 0 1 0 0 0]", f.Tov, mesh_, precision_);
+        const std::vector<double> zero(vol_.size(), 0.0);
+        auto wr1 = [&](const char* name, const char* dims, const std::vector<double>& cellv,
+                       const std::vector<double>& wallv) {
+            const PatchValues bv = byPatch(wallv, 1);
+            writeScalarField(dir + "/" + name + n, tn, name + n, dims, cellv, mesh_, precision_, nw ? &bv : nullptr);
+        };
+        wr1("rhoN", "[0 -3 0 0 0 0 0]", f.rhoN, w.rhoN);
+        wr1("rhoM", "[1 -3 0 0 0 0 0]", f.rhoM, w.rhoM);
+        wr1("p", "[1 -1 -2 0 0 0 0]", f.p, w.p);
+        wr1("Ttra", "[0 0 0 1 0 0 0]", f.Ttra, w.Ttra);
+        wr1("Trot", "[0 0 0 1 0 0 0]", f.Trot, w.Trot);
+        wr1("Tvib", "[0 0 0 1 0 0 0]", f.Tvib, w.Tvib);
+        wr1("Tov", "[0 0 0 1 0 0 0]", f.Tov, w.Tov);
         writeScalarField(dir + "/Ma" + n, tn, "Ma" + n, "[0 0 0 0 0 0 0]", f.Ma, mesh_, precision_);
         writeScalarField(dir + "/dsmcNMean" + n, tn, "dsmcNMean" + n, "[0 0 0 0 0 0 0]", f.dsmcNMean, mesh_, precision_);
-        writeVectorField(dir + "/U" + n, tn, "U" + n, "[0 1 -1 0 0 0 0]", f.Ux, f.Uy, f.Uz, mesh_, precision_);
+        {
+            const PatchValues bu = byPatch(w.U, 3);
+            writeVectorField(dir + "/U" + n, tn, "U" + n, "[0 1 -1 0 0 0 0]", f.Ux, f.Uy, f.Uz, mesh_, precision_,
+                             nw ? &bu : nullptr);
+        }
+        if (nw) {
+            wr1("wallHeatFlux", "[1 0 -3 0 0 0 0]", zero, w.q);
+            wr1("wallShearStress", "[1 -1 -2 0 0 0 0]", zero, w.tau);
+            const PatchValues bf = byPatch(w.fD, 3);
+            writeVectorField(dir + "/fD" + n, tn, "fD" + n, "[1 -1 -2 0 0 0 0]", zero, zero, zero, mesh_, precision_,
+                             &bf);
+        }
         if (fm.measureMeanFreePath) {
             writeScalarField(dir + "/mfp" + n, tn, "mfp" + n, "[0 1 0 0 0 0 0]", f.mfp, mesh_, precision_);
             writeScalarField(dir + "/meanCollisionRate" + n, tn, "meanCollisionRate" + n, "[0 0 -1 0 0 0 0]", f.mcr,

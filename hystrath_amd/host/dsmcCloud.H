@@ -103,6 +103,7 @@ private:
     int precision_ = 10;
     std::vector<double> prevSums_;
     long long prevSampled_ = 0;
+    std::string cylPatch_ = "cylinder";   // patch name of the embedded cylinder's wall faces
 };
 
 // Derived dsmcVolFields quantities of one field model from the cumulative
@@ -113,5 +114,15 @@ struct VolFields {
 };
 VolFields computeVolFields(const dsmcCloud& cloud, const std::vector<double>& sums, long long nSteps,
                            const FieldModel& fm);
+
+// Wall-face values of one field model (dsmcVolFields.C:1890-2126) from the
+// engine's wall-face sums [species*HS_WS_COUNT + slot][face]; normal is
+// [3*face + axis], pointing out of the gas.
+struct WallFields {
+    std::vector<double> rhoN, rhoM, Ttra, Trot, Tvib, Tov, p, tau, q;
+    std::vector<double> U, fD;   // 3 per face
+};
+WallFields computeWallFields(const dsmcCloud& cloud, const std::vector<double>& wallSums, int64_t nFaces,
+                             const std::vector<double>& normal, long long nSteps, const FieldModel& fm);
 
 }  // namespace hs

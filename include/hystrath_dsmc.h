@@ -66,6 +66,7 @@ enum hs_patch_model {
 #define HS_FLAG_COLL_SEP      0x4  /* collision separation (meanCollisionSeparation) */
 #define HS_FLAG_TIMING        0x8  /* per-kernel HIP event timing             */
 #define HS_FLAG_WALL         0x10  /* boundaryMeasurements at diffuse/specular walls */
+#define HS_FLAG_FACE_FLUX    0x20  /* dsmcFaceTracker parcel fluxes through every face */
 
 /* sides of the block, patch[] order */
 enum hs_side { HS_XMIN = 0, HS_XMAX = 1, HS_YMIN = 2, HS_YMAX = 3, HS_ZMIN = 4, HS_ZMAX = 5 };
@@ -250,7 +251,8 @@ int hs_dsmc_sort(hs_dsmc_engine* eng);                   /* buildCellOccupancy  
 int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample); /* noTimeCounter::collide (+ sampling) */
 int hs_dsmc_advance_step(hs_dsmc_engine* eng);           /* step counter += 1 */
 int hs_dsmc_synchronize(hs_dsmc_engine* eng);
-/* Switch HS_FLAG_SAMPLE / HS_FLAG_COLL_SEP / HS_FLAG_TIMING / HS_FLAG_WALL between steps
+/* Switch HS_FLAG_SAMPLE / HS_FLAG_COLL_SEP / HS_FLAG_TIMING / HS_FLAG_WALL /
+ * HS_FLAG_FACE_FLUX between steps
  * (the sampling schedule of timeDataMeas, timeDataMeas.C:71-104);
  * HS_FLAG_HEATFLUX can only be kept as it was at create time. */
 int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
@@ -260,7 +262,7 @@ int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
  * the number of sampled steps (nTimeSteps_). */
 int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
 /* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290); clears the cell
- * and the wall-face sums */
+ * and wall-face sums and the face fluxes */
 int hs_dsmc_reset_samples(hs_dsmc_engine* eng);
 /* Wall faces (the boundary faces of boundaryMeasurements, boundaryMeasurements.H:74-90):
  * first every face of each diffuse or specular side, sides in hs_side order,
@@ -275,6 +277,21 @@ int hs_dsmc_wall_face_count(hs_dsmc_engine* eng, int64_t* n_faces);
 int hs_dsmc_wall_faces(hs_dsmc_engine* eng, int32_t* side, int32_t* cell, double* area, double* normal);
 /* [species*HS_WS_COUNT + slot][wall face] sums; nsteps_out as above */
 int hs_dsmc_read_wall_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, int64_t* nsteps_out);
+/* dsmcFaceTracker::trackFaceTransition (dsmcFaceTracker.C:124-193), with
+ * HS_FLAG_FACE_FLUX (on sampled steps): parcelIdFlux per species and face,
+ * cumulative since the last hs_dsmc_reset_samples (the reference clears it
+ * every step, dsmcFaceTracker.C:86-97; a consumer sums it over steps).
+ * Every face crossing adds sign(U.Sf): Sf points +axis on internal faces
+ * and out of the domain on boundary faces; U is the velocity after a wall
+ * or deletion interaction; a cyclic crossing adds +1 on the coupled face of
+ * the other side.  massIdFlux is mass[species] times the count.  Faces:
+ * for axis a (x, y, z) the planes k = 0..n_a, face id
+ * off_a + k + (n_a+1)*(j1 + n_a1*j2) with (j1, j2) the tangential cell
+ * indices as for wall faces, off_x = 0, off_y = (nx+1)*ny*nz,
+ * off_z = off_y + nx*(ny+1)*nz; then the cylinder's wall faces in wall-face
+ * order.  out is int64 [species][face]. */
+int hs_dsmc_face_count(hs_dsmc_engine* eng, int64_t* n_faces);
+int hs_dsmc_read_face_flux(hs_dsmc_engine* eng, int64_t* out, int64_t n, int64_t* nsteps_out);
 /* counters since the last reset: [0] candidates, [1] collisions,
  * [2] deleted, [3] injected, [4] wall hits, [5] tracking-iteration
  * overflows, [6] oversize cells / candidate clamps, [7] particles dropped at

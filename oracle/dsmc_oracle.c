@@ -710,7 +710,46 @@ typedef struct {
     int64_t off[6];    /* first face of each measured side, -1 otherwise */
     int32_t* cyl_face; /* [nc] or NULL */
     double* area;      /* [n] */
+    int64_t cyl_first; /* first cylinder face in the wall-face list */
+    int measure;       /* wall sums on */
+    /* face tracker (dsmcFaceTracker) */
+    int64_t* flux;     /* [ns][n_faces] or NULL */
+    int64_t n_faces, face_off[4];
 } or_wall;
+
+/* sign() of OpenFOAM: +1 for x >= 0 */
+static int of_sign(double x) { return x >= 0.0 ? 1 : -1; }
+
+/* face-tracker face of plane k on axis a at tangential cell indices of idx */
+static int64_t mesh_face(const hs_config* cfg, const or_wall* w, int a, int k, const int idx[3])
+{
+    const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
+    const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
+    return w->face_off[a] + k + (int64_t)(nn[a] + 1) * (idx[a1] + (int64_t)nn[a1] * idx[a2]);
+}
+
+static void track_face(const or_wall* w, int t, int64_t face, int64_t v)
+{
+    w->flux[(int64_t)t * w->n_faces + face] += v;
+}
+
+int64_t or_face_count(const hs_config* cfg)
+{
+    const int64_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    int64_t n = (nx + 1) * ny * nz + nx * (ny + 1) * nz + nx * ny * (nz + 1);
+    if (cfg->cyl_enabled) {
+        const int64_t nw = or_wall_faces(cfg, NULL, NULL, NULL, NULL);
+        int64_t ns = 0;
+        for (int sd = 0; sd < 6; ++sd)
+            if (cfg->patch[sd].model == HS_PATCH_DIFFUSE || cfg->patch[sd].model == HS_PATCH_SPECULAR) {
+                const int axis = sd / 2;
+                const int64_t nn[3] = { nx, ny, nz };
+                ns += nn[axis == 0 ? 1 : 0] * nn[axis == 2 ? 1 : 2];
+            }
+        n += nw - ns;
+    }
+    return n;
+}
 
 int64_t or_wall_faces(const hs_config* cfg, int32_t* side, int32_t* cell, double* area, double* normal)
 {
@@ -770,6 +809,7 @@ static void wall_layout(const hs_config* cfg, or_wall* w)
         w->off[sd] = f;
         f += (int64_t)nn[a1] * nn[a2];
     }
+    w->cyl_first = f;
     w->n = or_wall_faces(cfg, NULL, NULL, NULL, NULL);
     w->area = (double*)malloc((size_t)(w->n > 0 ? w->n : 1) * sizeof(double));
     int32_t* cell = (int32_t*)malloc((size_t)(w->n > 0 ? w->n : 1) * sizeof(int32_t));
@@ -907,7 +947,8 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
             double nx_ = cfg->cyl_center[0] - p[0], ny_ = cfg->cyl_center[1] - p[1];
             const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
             const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
-            const int32_t f = w ? w->cyl_face[idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2])] : -1;
+            const int32_t wf = w->cyl_face ? w->cyl_face[idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2])] : -1;
+            const int32_t f = w->measure ? wf : -1;
             double e_pre = 0.0;
             const double u_pre[3] = { U[0], U[1], U[2] };
             if (f >= 0) e_pre = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
@@ -922,19 +963,31 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
                 const double e_post = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
                 wall_exchange_terms(cfg, w, s->wall, f, t_id, e_pre, u_pre, e_post, U);
             }
+            if (w->flux && wf >= 0)
+                track_face(w, t_id, w->face_off[3] + (wf - w->cyl_first),
+                           of_sign(U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2]));
             continue;
         }
         p[axis] = face_coord(cfg, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
         const int ni = idx[axis] + dir;
-        if (ni >= 0 && ni < nn[axis]) { idx[axis] = ni; continue; }
+        if (ni >= 0 && ni < nn[axis]) {
+            /* internal face: Sf points +axis, U crosses it along dir */
+            if (w->flux) track_face(w, t_id, mesh_face(cfg, w, axis, dir > 0 ? idx[axis] + 1 : idx[axis], idx), dir);
+            idx[axis] = ni;
+            continue;
+        }
         const int sd = 2 * axis + (dir > 0 ? 1 : 0);
         const hs_patch* pt = &cfg->patch[sd];
         if (pt->model == HS_PATCH_CYCLIC) {
+            /* counted, unsigned, on the coupled face of the other side */
+            if (w->flux) track_face(w, t_id, mesh_face(cfg, w, axis, dir > 0 ? 0 : nn[axis], idx), 1);
             p[axis] = face_coord(cfg, axis, dir > 0 ? 0 : nn[axis]);
             idx[axis] = dir > 0 ? 0 : nn[axis] - 1;
             continue;
         }
         if (pt->model == HS_PATCH_DELETION) {
+            if (w->flux)
+                track_face(w, t_id, mesh_face(cfg, w, axis, dir > 0 ? nn[axis] : 0, idx), of_sign(U[axis] * (double)dir));
             keep = 0;
             continue;
         }
@@ -943,7 +996,7 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
         double nw[3] = { 0.0, 0.0, 0.0 };
         nw[axis] = (double)dir;
         int64_t f = -1;
-        if (w && w->off[sd] >= 0) {
+        if (w->measure && w->off[sd] >= 0) {
             const int a1 = axis == 0 ? 1 : 0, a2 = axis == 2 ? 1 : 2;
             f = w->off[sd] + idx[a1] + (int64_t)nn[a1] * idx[a2];
         }
@@ -958,6 +1011,8 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
             const double e_post = wall_side_terms(cfg, w, s->wall, f, t_id, nw, U, erot, vib, s->capacity);
             wall_exchange_terms(cfg, w, s->wall, f, t_id, e_pre, u_pre, e_post, U);
         }
+        if (w->flux)
+            track_face(w, t_id, mesh_face(cfg, w, axis, dir > 0 ? nn[axis] : 0, idx), of_sign(U[axis] * (double)dir));
     }
     P->px[i] = p[0]; P->py[i] = p[1]; P->pz[i] = p[2];
     P->ux[i] = (float)U[0]; P->uy[i] = (float)U[1]; P->uz[i] = (float)U[2];
@@ -965,15 +1020,37 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
     else P->cell[i] = idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2]);
 }
 
+/* the wall-sum and face-tracker context of a run of move steps */
+static void move_context(const hs_config* cfg, or_state* s, or_wall* w)
+{
+    const int sample = (cfg->flags & HS_FLAG_SAMPLE) != 0;
+    const int measure = s->wall && (cfg->flags & HS_FLAG_WALL) && sample;
+    const int track = s->face_flux && (cfg->flags & HS_FLAG_FACE_FLUX) && sample;
+    if (measure || track) wall_layout(cfg, w);
+    else memset(w, 0, sizeof(*w));
+    w->measure = measure;
+    w->flux = track ? s->face_flux : NULL;
+    const int64_t nx = cfg->nx, ny = cfg->ny, nz = cfg->nz;
+    w->face_off[0] = 0;
+    w->face_off[1] = (nx + 1) * ny * nz;
+    w->face_off[2] = w->face_off[1] + nx * (ny + 1) * nz;
+    w->face_off[3] = w->face_off[2] + nx * ny * (nz + 1);
+    w->n_faces = or_face_count(cfg);
+}
+
+static void move_all(const hs_config* cfg, const or_derived* d, or_state* s, const or_wall* w, int64_t n_old)
+{
+    for (int64_t i = 0; i < s->n; ++i) move_one(cfg, d, s, w, i, i >= n_old);
+}
+
 int or_move(const hs_config* cfg, or_state* s, int64_t n_old)
 {
     or_derived d;
     derive(cfg, &d);
     or_wall w;
-    const int measure = s->wall && (cfg->flags & HS_FLAG_WALL) && (cfg->flags & HS_FLAG_SAMPLE);
-    if (measure) wall_layout(cfg, &w);
-    for (int64_t i = 0; i < s->n; ++i) move_one(cfg, &d, s, measure ? &w : NULL, i, i >= n_old);
-    if (measure) wall_free(&w);
+    move_context(cfg, s, &w);
+    move_all(cfg, &d, s, &w, n_old);
+    wall_free(&w);
     return 0;
 }
 
@@ -1350,13 +1427,18 @@ int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, doubl
 /* ------------------------------------------------------------------------ */
 int or_evolve(const hs_config* cfg, or_state* s, int32_t nsteps)
 {
+    or_derived d;
+    derive(cfg, &d);
+    or_wall w;
+    move_context(cfg, s, &w);
     for (int32_t k = 0; k < nsteps; ++k) {
         const int64_t n_old = s->n;
         if (s->inflow_acc) or_inject(cfg, s);
-        or_move(cfg, s, n_old);
+        move_all(cfg, &d, s, &w, n_old);
         or_sort(cfg, s);
         or_collide(cfg, s, (cfg->flags & HS_FLAG_SAMPLE) ? 1 : 0);
         s->step += 1;
     }
+    wall_free(&w);
     return 0;
 }

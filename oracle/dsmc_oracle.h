@@ -41,6 +41,9 @@ typedef struct or_state {
     double* wall;            /* [n_species*HS_WS_COUNT][wall faces] or NULL;
                                 summed by or_move when HS_FLAG_WALL and
                                 HS_FLAG_SAMPLE are set                    */
+    int64_t* face_flux;      /* [n_species][or_face_count] or NULL; counted
+                                by or_move with HS_FLAG_FACE_FLUX and
+                                HS_FLAG_SAMPLE                            */
 } or_state;
 
 /* primitives (exported for pinning tests) */
@@ -77,6 +80,8 @@ int or_move(const hs_config* cfg, or_state* s, int64_t n_old);
 /* wall faces of boundaryMeasurements in the engine's order; any output may be
  * NULL; returns the face count */
 int64_t or_wall_faces(const hs_config* cfg, int32_t* side, int32_t* cell, double* area, double* normal);
+/* faces of the face tracker: mesh faces by axis, then the cylinder's wall faces */
+int64_t or_face_count(const hs_config* cfg);
 /* length of the circle (cx, cy, r) inside [x0,x1]x[y0,y1]; n_out (may be
  * NULL) gets the length-weighted unit normal pointing to the centre */
 double or_arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2]);

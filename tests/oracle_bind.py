@@ -31,6 +31,7 @@ class or_state(C.Structure):
         ("step", C.c_uint32),
         ("counters", C.c_int64 * 8),
         ("wall", C.POINTER(C.c_double)),
+        ("face_flux", C.POINTER(C.c_int64)),
     ]
 
 
@@ -75,6 +76,8 @@ def lib():
         L.or_wall_faces.restype = C.c_int64
         L.or_arc_in_rect.argtypes = [C.c_double] * 7 + [P(C.c_double)]
         L.or_arc_in_rect.restype = C.c_double
+        L.or_face_count.argtypes = [cfgp]
+        L.or_face_count.restype = C.c_int64
         _lib = L
     return _lib
 
@@ -163,6 +166,8 @@ class OracleState:
         self.start = np.zeros(nc + 1, np.int32)
         self.n_wall = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
         self.wall = np.zeros(max(case.cfg.n_species * abi.HS_WS_COUNT * self.n_wall, 1))
+        self.n_faces = lib().or_face_count(C.byref(case.cfg))
+        self.flux = np.zeros((case.cfg.n_species, self.n_faces), np.int64)
         s = or_state()
         s.n = 0
         s.capacity = cap
@@ -175,6 +180,7 @@ class OracleState:
         s.count = iptr(self.count)
         s.start = iptr(self.start)
         s.wall = dptr(self.wall)
+        s.face_flux = self.flux.ctypes.data_as(C.POINTER(C.c_int64))
         self.s = s
 
     @property

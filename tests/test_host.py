@@ -86,6 +86,16 @@ def test_generated_cases_parse(tmp_path):
             assert "no HIP device" in r.stderr
 
 
+def _patch_values(path, patch):
+    txt = open(path).read()
+    m = re.search(patch + r"\s*\{\s*type\s+calculated;\s*value\s+nonuniform List<(\w+)>\s*\n(\d+)\n\(\n(.*?)\n\)\n;",
+                  txt, re.S)
+    rows = m.group(3).split("\n")
+    if m.group(1) == "vector":
+        return np.array([[float(x) for x in r.strip("()").split()] for r in rows])
+    return np.array([float(x) for x in rows])
+
+
 def _field(path):
     txt = open(path).read()
     m = re.search(r"internalField\s+nonuniform List<(\w+)>\s*\n(\d+)\n\(\n(.*?)\n\)\n;", txt, re.S)
@@ -114,6 +124,16 @@ def test_couette_pipeline_and_restart(tmp_path):
     assert 1500 < T.mean() < 3500
     U = _field(os.path.join(last, "U_mixture"))
     assert U.shape == (500, 3)
+    # wall values (boundaryMeasurements): 5 faces per wall
+    q_up = _patch_values(os.path.join(last, "wallHeatFlux_mixture"), "upperWall")
+    q_lo = _patch_values(os.path.join(last, "wallHeatFlux_mixture"), "lowerWall")
+    assert q_up.shape == q_lo.shape == (5,)
+    assert q_up.mean() < 0          # the 3000 K wall heats the 2000 K gas
+    p_lo = _patch_values(os.path.join(last, "p_mixture"), "lowerWall")
+    kT = 1.38064852e-23 * 2000.0
+    assert 0.5 * 4.09e19 * kT < p_lo.mean() < 2.0 * 4.09e19 * kT
+    fD = _patch_values(os.path.join(last, "fD_mixture"), "upperWall")
+    assert fD.shape == (5, 3) and (fD[:, 1] > 0).all()  # the gas pushes the upper wall outward (+y)
     # restart from the latest time and continue to a later end time
     cd = os.path.join(case, "system/controlDict")
     ctxt = open(cd).read()

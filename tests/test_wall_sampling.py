@@ -199,3 +199,108 @@ def test_wall_sampling_parity_cylinder():
 def test_wall_sampling_parity_couette_lb():
     case = cases.couette_n2o2(flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
     _parity(case, lambda x: x.fill(case.init_n, case.init_T), 4)
+
+
+# ---- face tracker (dsmcFaceTracker.C:124-193) -----------------------------------------
+def _face_planes(cfg):
+    nx, ny, nz = cfg.nx, cfg.ny, cfg.nz
+    off = [0, (nx + 1) * ny * nz]
+    off.append(off[1] + nx * (ny + 1) * nz)
+    off.append(off[2] + nx * ny * (nz + 1))
+    return off
+
+
+def _face(cfg, off, a, k, i, j, kk):
+    nn = (cfg.nx, cfg.ny, cfg.nz)
+    idx = (i, j, kk)
+    a1, a2 = (1, 2) if a == 0 else ((0, 2) if a == 1 else (0, 1))
+    return off[a] + k + (nn[a] + 1) * (idx[a1] + nn[a1] * idx[a2])
+
+
+def _cell_counts(case, d):
+    c = d["cell"]
+    return np.bincount(c[c >= 0], minlength=case.ncells)
+
+
+def test_face_tracker_cell_balance_and_wall_signs():
+    """Every cell away from the cyclic sides gains exactly its signed
+    internal-face flux; each wall hit counts -1 (the reflected parcel moves
+    against the outward normal)."""
+    case = cases.wall_channel(nx=6, ny=12, flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_FACE_FLUX)
+    cfg = case.cfg
+    o = ob.OracleState(case)
+    o.fill([case.n_density], case.T_gas, exact=True)
+    N0 = _cell_counts(case, o.particles())
+    o.evolve(30)
+    N1 = _cell_counts(case, o.particles())
+    F = o.flux.sum(axis=0)
+    off = _face_planes(cfg)
+    assert o.n_faces == off[3]
+    for j in range(1, cfg.ny - 1):
+        for i in range(cfg.nx):
+            c = i + cfg.nx * j
+            inflow = 0
+            if i > 0:
+                inflow += F[_face(cfg, off, 0, i, i, j, 0)]
+            if i < cfg.nx - 1:
+                inflow -= F[_face(cfg, off, 0, i + 1, i, j, 0)]
+            inflow += F[_face(cfg, off, 1, j, i, j, 0)] - F[_face(cfg, off, 1, j + 1, i, j, 0)]
+            assert N1[c] - N0[c] == inflow, (i, j)
+    walls = [_face(cfg, off, 0, 0, 0, j, 0) for j in range(cfg.ny)] + \
+            [_face(cfg, off, 0, cfg.nx, cfg.nx - 1, j, 0) for j in range(cfg.ny)]
+    assert (F[walls] <= 0).all()
+    assert -F[walls].sum() == o.counters()[4] > 0
+    # z is empty: never crossed; the cyclic y planes count unsigned
+    assert (F[off[2]:off[3]] == 0).all()
+    cyc = [_face(cfg, off, 1, k, i, k if k < cfg.ny else cfg.ny - 1, 0) for k in (0, cfg.ny) for i in range(cfg.nx)]
+    assert (F[cyc] >= 0).all() and F[cyc].sum() > 0
+
+
+def test_face_tracker_deletion_and_cylinder():
+    """Outer deletion faces count +1 per deleted parcel; the cylinder's wall
+    faces -1 per hit (the only wall)."""
+    case = cases.cylinder_flow(nx=40, ny=40, ppc=6, dt=2.5e-7 * 10, seed=3,
+                               flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_FACE_FLUX)
+    o = ob.OracleState(case)
+    o.fill([case.n_inf], case.T_inf, u=(case.U_inf, 0.0, 0.0))
+    o.evolve(12)
+    F = o.flux.sum(axis=0)
+    off = _face_planes(case.cfg)
+    nw = len(ob.wall_faces(case)["side"])
+    assert o.n_faces == off[3] + nw
+    cnt = o.counters()
+    assert -F[off[3]:].sum() == cnt[4] > 0
+    assert (F[off[3]:] <= 0).all()
+    nx, ny = case.cfg.nx, case.cfg.ny
+    outer = [_face(case.cfg, off, 0, k, 0, j, 0) for k in (0, nx) for j in range(ny)] + \
+            [_face(case.cfg, off, 1, k, i, 0, 0) for k in (0, ny) for i in range(nx)]
+    assert F[outer].sum() == cnt[2] > 0
+    # per species: mass flux is mass times the count (one species here)
+    assert o.flux.shape == (1, o.n_faces)
+
+
+@pytest.mark.gpu
+def test_face_tracker_parity():
+    from hystrath_amd import engine
+    for case, fill, steps in (
+        (cases.wall_channel(nx=6, ny=12, flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_FACE_FLUX | abi.HS_FLAG_WALL),
+         lambda x, c: x.fill([c.n_density], c.T_gas, exact=True), 10),
+        (cases.cylinder_flow(nx=60, ny=60, ppc=10, dt=2.5e-7 * 400 / 60, seed=5,
+                             flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_FACE_FLUX),
+         lambda x, c: x.fill([c.n_inf], c.T_inf, u=(c.U_inf, 0.0, 0.0)), 10),
+        (cases.couette_n2o2(flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_FACE_FLUX), lambda x, c: x.fill(c.init_n, c.init_T),
+         4),
+    ):
+        o = ob.OracleState(case)
+        fill(o, case)
+        with engine.Engine(case) as e:
+            fill(e, case)
+            e.evolve(steps)
+            o.evolve(steps)
+            assert_same_particles(e.download(), o.particles(), case.name)
+            Fe, ns = e.face_flux()
+            assert ns == steps
+            np.testing.assert_array_equal(Fe, o.flux)
+            assert np.abs(Fe).sum() > 0
+            e.reset_samples()
+            assert not e.face_flux()[0].any()
