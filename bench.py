@@ -51,11 +51,14 @@ def build_case(workload):
     from hystrath_amd import abi, cases
 
     if workload == "cylinder":
-        case = cases.cylinder_flow(nx=400, ny=400, ppc=25, dt=2.5e-7, seed=20251202, flags=abi.HS_FLAG_SAMPLE)
+        # dsmcVolFields sums every step and the wall's boundaryMeasurements, which
+        # the reference's diffuse wall always takes (dsmcDiffuseWallPatch.C:167)
+        case = cases.cylinder_flow(nx=400, ny=400, ppc=25, dt=2.5e-7, seed=20251202,
+                                   flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
         desc = {"workload": "cylinder_mach10_argon_400x400", "cells": case.ncells, "ppc_freestream": 25,
                 "n_inf": case.n_inf, "T_inf": case.T_inf, "U_inf": round(case.U_inf, 3), "T_wall": 500.0,
                 "radius": 0.1, "dt": case.cfg.dt, "collision_model": "VariableHardSphere",
-                "partner_selection": "noTimeCounter", "sampling": "every step"}
+                "partner_selection": "noTimeCounter", "sampling": "every step, cells and wall faces"}
 
         def fill(e):
             e.fill([case.n_inf], case.T_inf, u=(case.U_inf, 0.0, 0.0), fill_id=0)
