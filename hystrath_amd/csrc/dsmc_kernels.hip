@@ -546,30 +546,50 @@ struct CellCtx {
     unsigned long long cand, colls, ovf;
 };
 
-// Candidate count of noTimeCounter.C:142-153; updates the remainder.
-__device__ __forceinline__ long long ntc_candidates(const DevParams& P, int c, int nC, const double* sigma,
-                                                   double* rem, const double* vol, double* smax_out, CellCtx& cx) {
-    const double smax = sigma[c];
-    const double sel = rem[c] + 0.5 * nC * (nC - 1) * P.fn * smax * P.dt / vol[c];
+// Sub-cell sizes and list offsets of one cell: packed in registers (one lane
+// per cell) or a per-cell LDS table.
+struct PackTab {
+    Pack8 c, o;
+    __device__ __forceinline__ int cnt(int q) const { return c.get(q); }
+    __device__ __forceinline__ int off(int q) const { return o.get(q); }
+};
+struct LdsTab {
+    const uint16_t* c;
+    const uint16_t* o;
+    __device__ __forceinline__ int cnt(int q) const { return c[q]; }
+    __device__ __forceinline__ int off(int q) const { return o[q]; }
+};
+
+// Candidate count of noTimeCounter.C:142-153 from the cell's sigmaTcRMax,
+// remainder and volume; stores the new remainder.
+__device__ __forceinline__ long long ntc_candidates_of(const DevParams& P, int c, int nC, double smax, double rem0,
+                                                      double v, double* rem, CellCtx& cx) {
+    const double sel = rem0 + 0.5 * nC * (nC - 1) * P.fn * smax * P.dt / v;
     long long ncand = (long long)sel;
     rem[c] = sel - (double)ncand;
     if (ncand > HS_MAX_CAND) { ncand = HS_MAX_CAND; cx.ovf += 1; }
-    *smax_out = smax;
     return ncand;
+}
+
+__device__ __forceinline__ long long ntc_candidates(const DevParams& P, int c, int nC, const double* sigma,
+                                                   double* rem, const double* vol, double* smax_out, CellCtx& cx) {
+    const double smax = sigma[c];
+    *smax_out = smax;
+    return ntc_candidates_of(P, c, nC, smax, rem[c], vol[c], rem, cx);
 }
 
 // Draws the pair of candidate k (its own stream) and reports whether it may
 // collide: noTimeCounter.C:157-228.
-template <bool GEN>
+template <bool GEN, class Tab>
 __device__ __forceinline__ bool draw_pair(const DevParams& P, PhiloxStream& r, int nC, const uint8_t* sub,
-                                          const uint16_t* slist, int ss, const Pack8& cnt, const Pack8& off,
-                                          const uint8_t* ty, int* pp_out, int* qq_out) {
+                                          const uint16_t* slist, int ss, const Tab& tab, const uint8_t* ty,
+                                          int* pp_out, int* qq_out) {
     const int pp = random_label(r, 0, nC - 1);
     const int q = sub[ss + pp];
-    const int nsc = cnt.get(q);
+    const int nsc = tab.cnt(q);
     int qq = -1, guard = 0;
     if (nsc > 1) {
-        const int o = off.get(q);
+        const int o = tab.off(q);
         do { qq = slist[ss + o + random_label(r, 0, nsc - 1)]; } while (pp == qq && ++guard < HS_MAX_REDRAW);
     } else {
         do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
@@ -626,15 +646,15 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
                          float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
                          const uint8_t* ty, double* sigma, uint32_t step, const DevParticles& pos, int64_t gs,
                          CellCtx& cx, double& ncoll, double& sep) {
-    Pack8 cnt, off;
-    subcell_lists_lane(sub, slist, ss, nC, cnt, off);
+    PackTab tab;
+    subcell_lists_lane(sub, slist, ss, nC, tab.c, tab.o);
     double smax_cur = smax;
     cx.cand += (unsigned long long)ncand;
     for (long long k = 0; k < ncand; ++k) {
         PhiloxStream r;
         r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
         int pp, qq;
-        if (!draw_pair<GEN>(P, r, nC, sub, slist, ss, cnt, off, ty, &pp, &qq)) continue;
+        if (!draw_pair<GEN>(P, r, nC, sub, slist, ss, tab, ty, &pp, &qq)) continue;
         const int ip = ss + pp, iq = ss + qq;
         const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
         double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
@@ -661,53 +681,15 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
 // a parcel stamped by an earlier lane ends the batch.  Everything before it
 // is committed, which is exactly what the candidate-by-candidate order
 // produces: their parcels were not changed by any earlier accepted candidate.
+// The sub-cell lists and tables are built and the stamps cleared beforehand.
 template <bool GEN>
 __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
-                         float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
-                         const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step, const DevParticles& pos,
-                         int64_t gs, CellCtx& cx, double& ncoll_out, double& sep_out) {
+                         float* vz, const uint8_t* sub, const uint16_t* slist, const LdsTab& tab, float* er,
+                         uint16_t* vb, int vstride, const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step,
+                         const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out) {
     const int lane = threadIdx.x & (WAVE - 1);
-    const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
-    // sub-cell counts (lane q holds the count of sub-cell q)
-    int myc = 0;
-    for (int base = 0; base < nC; base += WAVE) {
-        const int j = base + lane;
-        const int q = j < nC ? (int)sub[ss + j] : 8;
-#pragma unroll
-        for (int qq = 0; qq < 8; ++qq) {
-            const int pc = __popcll(__ballot(q == qq));
-            if (lane == qq) myc += pc;
-        }
-    }
-    Pack8 cnt, off;
-    cnt.zero();
-    off.zero();
-    {
-        int acc = 0;
-#pragma unroll
-        for (int qq = 0; qq < 8; ++qq) {
-            const int cq = __shfl(myc, qq, WAVE);
-            cnt.set(qq, cq);
-            off.set(qq, acc);
-            acc += cq;
-        }
-    }
-    int run[8];
-#pragma unroll
-    for (int qq = 0; qq < 8; ++qq) run[qq] = off.get(qq);
-    for (int base = 0; base < nC; base += WAVE) {
-        const int j = base + lane;
-        const int q = j < nC ? (int)sub[ss + j] : 8;
-#pragma unroll
-        for (int qq = 0; qq < 8; ++qq) {
-            const unsigned long long m = __ballot(q == qq);
-            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (uint16_t)j;
-            run[qq] += __popcll(m);
-        }
-    }
-    for (int j = lane; j < nC; j += WAVE) stamp[ss + j] = 0x7FFFFFFF;
     double smax_cur = smax;
-    double ncoll = 0.0, sep = 0.0;
+    double sep = 0.0;
     unsigned long long ncoll_i = 0;
     for (long long k0 = 0; k0 < ncand;) {
         const long long k = k0 + lane;
@@ -719,7 +701,7 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
         PhiloxStream r;
         if (active) {
             r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-            valid = draw_pair<GEN>(P, r, nC, sub, slist, ss, cnt, off, ty, &pp, &qq);
+            valid = draw_pair<GEN>(P, r, nC, sub, slist, ss, tab, ty, &pp, &qq);
             if (valid) {
                 const int ip = ss + pp, iq = ss + qq;
                 const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
@@ -774,13 +756,12 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
         __builtin_amdgcn_wave_barrier();
         k0 += fc;
     }
-    ncoll = (double)ncoll_i;
     if (lane == 0) {
         sigma[c] = smax_cur;
         cx.cand += (unsigned long long)ncand;
         cx.colls += ncoll_i;
     }
-    ncoll_out = ncoll;
+    ncoll_out = (uint32_t)ncoll_i;
     sep_out = sep;
 }
 
@@ -800,6 +781,7 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
         double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
         double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
         const int nvib = GEN ? P.nvib[s] : 0;
+#pragma unroll 4
         for (int k = 0; k < nC; ++k) {
             const int i = ss + k;
             if (GEN && (int)ty[i] != s) continue;
@@ -851,23 +833,235 @@ __device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, d
     return (x - cc0 >= 0.0 ? 1 : 0) + 2 * (y - cc1 >= 0.0 ? 1 : 0) + 4 * (z - cc2 >= 0.0 ? 1 : 0);
 }
 
+// Rank of key k among key[ss..ee) (LDS, 16-byte aligned array, reads may run
+// up to 3 entries past ee inside the array): 4 keys per 128-bit read.
+__device__ __forceinline__ int rank_in(const int32_t* key, int ss, int ee, int k) {
+    int r = 0;
+#pragma unroll 4
+    for (int q = ss & ~3; q < ee; q += 4) {
+        const int4 v = *reinterpret_cast<const int4*>(key + q);
+        r += (q >= ss && v.x < k) ? 1 : 0;
+        r += (q + 1 >= ss && q + 1 < ee && v.y < k) ? 1 : 0;
+        r += (q + 2 >= ss && q + 2 < ee && v.z < k) ? 1 : 0;
+        r += (q + 3 >= ss && q + 3 < ee && v.w < k) ? 1 : 0;
+    }
+    return r;
+}
+
+// 0x80 in byte i when lo <= p + i < hi
+__device__ __forceinline__ uint32_t byte_window(int p, int lo, int hi) {
+    uint32_t m = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) m |= (p + i >= lo && p + i < hi) ? (0x80u << (8 * i)) : 0u;
+    return m;
+}
+
+// Sub-cell statistics of slot j (sub-cell q) in cell [ss, ee): parcels of
+// lower sub-cells, of sub-cell q, and of sub-cell q before j.  16 sub-cell
+// bytes per 128-bit read; bytes are 0..7, so per-byte adds never carry:
+// (b + 0x7F) sets bit 7 iff b != 0, (b + 0x80 - q) sets it iff b >= q.
+__device__ __forceinline__ void subcell_rank(const uint8_t* sub, int ss, int ee, int j, int q, int& before, int& tot,
+                                             int& rnk) {
+    const uint32_t qq = (uint32_t)q * 0x01010101u;
+    const uint32_t ge_add = (0x80u - (uint32_t)q) * 0x01010101u;
+    before = 0; tot = 0; rnk = 0;
+    for (int p0 = ss & ~15; p0 < ee; p0 += 16) {
+        const uint4 v = *reinterpret_cast<const uint4*>(sub + p0);
+        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int p = p0 + 4 * i;
+            const uint32_t w = w4[i];
+            const uint32_t in_cell = byte_window(p, ss, ee);
+            const uint32_t in_pre = byte_window(p, ss, j);
+            const uint32_t ne = ((w ^ qq) + 0x7F7F7F7Fu) & 0x80808080u;  // byte != q
+            const uint32_t ge = (w + ge_add) & 0x80808080u;               // byte >= q
+            const uint32_t eq = ~ne & 0x80808080u;
+            before += __popc(in_cell & ~ge);
+            tot += __popc(in_cell & eq);
+            rnk += __popc(in_pre & eq);
+        }
+    }
+}
+
 template <bool GEN, int PMAX>
 struct TileLds {
-    int32_t key[PMAX];       // old index of each slot; conflict stamps of the dense NTC
+    alignas(16) int32_t key[PMAX];  // old index of each slot; then the NTC conflict stamps
     uint8_t loc[PMAX];       // sub-tile cell of each slot
-    uint8_t sub[PMAX];
-    uint16_t slist[PMAX];
+    alignas(16) uint8_t sub[PMAX];
+    uint16_t slist[PMAX];    // per cell, its parcels by sub-cell, ascending
     float vx[PMAX], vy[PMAX], vz[PMAX];
     float er[GEN ? PMAX : 1];
     uint16_t vb[GEN ? HS_MAX_VIB_MODES * PMAX : 1];
     uint8_t ty[GEN ? PMAX : 1];
     int32_t segS[CT_CELLS + 1];
-    double ncoll[CT_CELLS], sep[CT_CELLS];
-    long long ncand[CT_CELLS];
-    double smax[CT_CELLS];
+    uint16_t scnt[CT_CELLS][8], soff[CT_CELLS][8];  // sub-cell sizes and list offsets
+    double sep[CT_CELLS];
+    double smax[CT_CELLS];   // sigmaTcRMax at the start of the step
+    int32_t ncand[CT_CELLS];
     int32_t dense[CT_CELLS];
     int32_t ndense;
+    // candidate rounds of the sparse cells
+    int32_t rk[CT_CELLS];    // next candidate of each cell
+    int32_t rn[CT_CELLS];    // candidates handled by the rounds (0 for dense cells)
+    int32_t rpfx[CT_CELLS];  // first round slot of each cell
+    int32_t rfirst[CT_CELLS];  // first slot of the cell that saw a changed parcel
+    unsigned long long rsmax[CT_CELLS];  // running max of sigmaTcR (bits of a non-negative double)
+    uint32_t rcoll[CT_CELLS];
+    double gsep[CT];         // collision separation of each round slot
+    uint8_t gacc[CT];        // round slot committed a collision
 };
+
+// NTC of the sparse cells of a chunk (at most DENSE_CAND candidates each),
+// candidate-parallel over the workgroup.  A round gives every pending
+// candidate of every cell a slot (lane), in cell then candidate order; each
+// slot draws its pair from the candidate's own stream and evaluates it
+// against the velocities at the start of the round.  An accepted slot stamps
+// its two parcels (LDS atomicMin of the slot); in each cell the slots before
+// the first one that touches a parcel stamped by an earlier slot are
+// committed -- they saw exactly the velocities the sequential loop of
+// noTimeCounter.C:157-302 sees -- and the rest of that cell's candidates wait
+// for the next round.
+template <bool GEN, int PMAX>
+__device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int a, int b, int S0, uint32_t step,
+                           const DevParticles& pos, CellCtx& cx) {
+    const int t = threadIdx.x;
+    const bool own = t >= a && t < b;
+    const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
+    for (;;) {
+        const int pend = own ? L.rn[t] - L.rk[t] : 0;
+        int total;
+        const int first = block_excl_scan(pend, &total);
+        if (total == 0) break;
+        if (own) {
+            L.rpfx[t] = first;
+            L.rfirst[t] = 0x7FFFFFFF;
+        }
+        __syncthreads();
+        const int g = t;
+        const bool active = g < total;
+        int l = a, ss = 0, pp = -1, qq = -1;
+        bool valid = false, acc = false;
+        double stcr = 0.0;
+        double up[3] = { 0.0, 0.0, 0.0 }, uq[3] = { 0.0, 0.0, 0.0 };
+        PhiloxStream r;
+        if (active) {
+            // the last cell whose first slot is <= g
+            int lo = a, hi = b - 1;
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (L.rpfx[mid] <= g) lo = mid;
+                else hi = mid - 1;
+            }
+            l = lo;
+            const int k = L.rk[l] + (g - L.rpfx[l]);
+            ss = L.segS[l] - S0;
+            const int nC = L.segS[l + 1] - L.segS[l];
+            r.init(P, RNG_COLLIDE, (uint32_t)(c0 + l), step, (uint32_t)k);
+            const LdsTab tab = { L.scnt[l], L.soff[l] };
+            valid = draw_pair<GEN>(P, r, nC, L.sub, L.slist, ss, tab, L.ty, &pp, &qq);
+            if (valid) {
+                const int ip = ss + pp, iq = ss + qq;
+                const int tp = GEN ? (int)L.ty[ip] : 0, tq = GEN ? (int)L.ty[iq] : 0;
+                up[0] = L.vx[ip]; up[1] = L.vy[ip]; up[2] = L.vz[ip];
+                uq[0] = L.vx[iq]; uq[1] = L.vy[iq]; uq[2] = L.vz[iq];
+                stcr = sigma_tcr(P, tp, tq, up, uq);
+                acc = stcr / L.smax[l] > r.u();
+            }
+            if (acc) {
+                atomicMin(&L.key[ss + pp], g);
+                atomicMin(&L.key[ss + qq], g);
+            }
+        }
+        __syncthreads();
+        if (valid && (L.key[ss + pp] < g || L.key[ss + qq] < g)) atomicMin(&L.rfirst[l], g);
+        __syncthreads();
+        const bool commit = active && g < L.rfirst[l];
+        if (commit && valid) atomicMax(&L.rsmax[l], (unsigned long long)__double_as_longlong(stcr));
+        const bool doit = commit && acc;
+        if (doit) {
+            do_collision<GEN>(P, r, ss + pp, ss + qq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
+            atomicAdd(&L.rcoll[l], 1u);
+            if (sep_on) {
+                const int64_t gp = (int64_t)S0 + ss + pp, gq = (int64_t)S0 + ss + qq;
+                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
+                L.gsep[g] = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            }
+        }
+        L.gacc[g] = doit ? 1 : 0;
+        __syncthreads();
+        if (acc) {
+            L.key[ss + pp] = 0x7FFFFFFF;
+            L.key[ss + qq] = 0x7FFFFFFF;
+        }
+        if (own && pend > 0) {
+            // slots [beg, beg+pend) of this cell that existed this round, up to the first dirty one;
+            // the first slot of a cell is never dirty, so every cell with beg < CT advances
+            const int beg = L.rpfx[t];
+            int stop = beg + pend < CT ? beg + pend : CT;
+            if (stop < beg) stop = beg;
+            if (L.rfirst[t] < stop) stop = L.rfirst[t];
+            if (sep_on)
+                for (int q = beg; q < stop; ++q)
+                    if (L.gacc[q]) L.sep[t] += L.gsep[q];
+            L.rk[t] += stop - beg;
+        }
+        __syncthreads();
+    }
+    (void)cx;
+}
+
+// One cell with more parcels than an LDS chunk holds: the stable gather,
+// NTC (one lane) and sampling in global memory.  Kept out of line: it is
+// rare, and inlining it would cost the common path registers.
+template <bool GEN>
+__device__ __noinline__ void collide_oversize(const DevParams& P, int c, int S0, int nC, DevParticles& src,
+                                              DevParticles& dst, const int32_t* __restrict__ perm,
+                                              double* __restrict__ sigma, double* __restrict__ rem,
+                                              double* __restrict__ samples, const double* __restrict__ vol,
+                                              uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
+                                              uint8_t* big_sub, CellCtx& cx) {
+    const int t = threadIdx.x;
+    const int64_t cap = P.capacity;
+    for (int j = t; j < nC; j += CT) {
+        const int k = perm[S0 + j];
+        int rank = 0;
+        for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+        const int64_t d = (int64_t)S0 + rank;
+        const double x = src.px[k], y = src.py[k], z = src.pz[k];
+        dst.px[d] = x; dst.py[d] = y; dst.pz[d] = z;
+        dst.ux[d] = src.ux[k]; dst.uy[d] = src.uy[k]; dst.uz[d] = src.uz[k];
+        dst.cell[d] = c;
+        if (dst.erot) dst.erot[d] = src.erot[k];
+        if (dst.vib)
+            for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + d] = src.vib[m * cap + k];
+        if (dst.type) dst.type[d] = src.type[k];
+        big_sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
+    }
+    __syncthreads();
+    if (t == 0) {
+        double ncoll = 0.0, sep = 0.0;
+        if (nC > 65535) {
+            cx.ovf += 1;
+        } else {
+            float* gvx = dst.ux + S0;
+            float* gvy = dst.uy + S0;
+            float* gvz = dst.uz + S0;
+            float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
+            uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
+            const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
+            if (do_collide && nC > 1) {
+                double smax;
+                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+                ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, big_sub + S0, big_list + S0, ger, gvb,
+                              (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
+            }
+            if (do_sample)
+                sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
+        }
+    }
+    __syncthreads();
+}
 
 // Gather, collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS)
 // in LDS chunks of at most PMAX parcels.
@@ -890,98 +1084,108 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         const int b = a + __syncthreads_count(ok);
         if (b == a) {
             // one cell larger than the tile: stable order and NTC in global memory
-            const int nC = L.segS[a + 1] - S0;
-            const int c = c0 + a;
-            for (int j = t; j < nC; j += CT) {
-                const int k = perm[S0 + j];
-                int rank = 0;
-                for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
-                const int64_t d = (int64_t)S0 + rank;
-                const double x = src.px[k], y = src.py[k], z = src.pz[k];
-                dst.px[d] = x; dst.py[d] = y; dst.pz[d] = z;
-                dst.ux[d] = src.ux[k]; dst.uy[d] = src.uy[k]; dst.uz[d] = src.uz[k];
-                dst.cell[d] = c;
-                if (dst.erot) dst.erot[d] = src.erot[k];
-                if (dst.vib)
-                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + d] = src.vib[m * cap + k];
-                if (dst.type) dst.type[d] = src.type[k];
-                big_sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
-            }
-            __syncthreads();
-            if (t == 0) {
-                double ncoll = 0.0, sep = 0.0;
-                if (nC > 65535) {
-                    cx.ovf += 1;
-                } else {
-                    float* gvx = dst.ux + S0;
-                    float* gvy = dst.uy + S0;
-                    float* gvz = dst.uz + S0;
-                    float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
-                    uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
-                    const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
-                    if (do_collide && nC > 1) {
-                        double smax;
-                        const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-                        ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, big_sub + S0, big_list + S0, ger, gvb,
-                                      (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
-                    }
-                    if (do_sample)
-                        sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
-                }
-            }
-            __syncthreads();
+            collide_oversize<GEN>(P, c0 + a, S0, L.segS[a + 1] - S0, src, dst, perm, sigma, rem, samples, vol, step,
+                                  do_collide, do_sample, nc, big_list, big_sub, cx);
             a += 1;
             continue;
         }
         const int E0 = L.segS[b];
         const int cnt = E0 - S0;
-        // slot -> cell map and the old indices of the chunk
-        if (t >= a && t < b)
+        // slot -> cell map and the old indices of the chunk; each cell's
+        // sigmaTcRMax, remainder and volume are fetched now, used after the gather
+        double pf_sig = 0.0, pf_rem = 0.0, pf_vol = 1.0;
+        if (t >= a && t < b) {
+            if (do_collide) {
+                pf_sig = sigma[c0 + t];
+                pf_rem = rem[c0 + t];
+                pf_vol = vol[c0 + t];
+            }
             for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
+        }
+#pragma unroll 4
         for (int j = t; j < cnt; j += CT) L.key[j] = perm[S0 + j];
         __syncthreads();
-        // stable order inside each cell (rank by old index) and gather
-        for (int j = t; j < cnt; j += CT) {
-            const int l = L.loc[j];
-            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
-            const int k = L.key[j];
-            int rank = 0;
-            for (int q = ss; q < ee; ++q) rank += (L.key[q] < k) ? 1 : 0;
-            const int d = ss + rank;
-            const int c = c0 + l;
-            const int64_t gd = (int64_t)S0 + d;
-            const double x = src.px[k], y = src.py[k], z = src.pz[k];
-            dst.px[gd] = x; dst.py[gd] = y; dst.pz[gd] = z;
-            dst.cell[gd] = c;
-            L.sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
-            L.vx[d] = src.ux[k]; L.vy[d] = src.uy[k]; L.vz[d] = src.uz[k];
-            if (GEN) {
-                L.er[d] = src.erot ? src.erot[k] : 0.0f;
-                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + d] = src.vib[m * cap + k];
-                L.ty[d] = src.type ? src.type[k] : (uint8_t)0;
+        // stable order inside each cell (rank by old index) and gather, four
+        // parcels per thread with their loads in flight together
+        for (int j0 = t; j0 < cnt; j0 += 4 * CT) {
+            int kk[4], dd[4], cc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int j = j0 + u * CT;
+                kk[u] = -1;
+                dd[u] = 0;
+                cc[u] = 0;
+                if (j < cnt) {
+                    const int l = L.loc[j];
+                    const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
+                    kk[u] = L.key[j];
+                    dd[u] = ss + rank_in(L.key, ss, ee, kk[u]);
+                    cc[u] = c0 + l;
+                }
+            }
+            double x[4], y[4], z[4];
+            float ux[4], uy[4], uz[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = kk[u] >= 0 ? kk[u] : kk[0];
+                x[u] = src.px[k]; y[u] = src.py[k]; z[u] = src.pz[k];
+                ux[u] = src.ux[k]; uy[u] = src.uy[k]; uz[u] = src.uz[k];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                if (kk[u] < 0) continue;
+                const int d = dd[u], c = cc[u], k = kk[u];
+                const int64_t gd = (int64_t)S0 + d;
+                dst.px[gd] = x[u]; dst.py[gd] = y[u]; dst.pz[gd] = z[u];
+                dst.cell[gd] = c;
+                L.sub[d] = (uint8_t)subcell_of(P, c, x[u], y[u], z[u]);
+                L.vx[d] = ux[u]; L.vy[d] = uy[u]; L.vz[d] = uz[u];
+                if (GEN) {
+                    L.er[d] = src.erot ? src.erot[k] : 0.0f;
+                    for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + d] = src.vib[m * cap + k];
+                    L.ty[d] = src.type ? src.type[k] : (uint8_t)0;
+                }
             }
         }
         if (t == 0) L.ndense = 0;
         __syncthreads();
-        // candidates per cell; sparse cells run their NTC on one lane now
+        // sub-cell lists in parallel: each parcel finds its place among the
+        // parcels of its sub-cell (ascending index) and clears its stamp
+        for (int j = t; j < cnt; j += CT) {
+            const int l = L.loc[j];
+            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
+            const int q = L.sub[j];
+            int before, tot, rnk;
+            subcell_rank(L.sub, ss, ee, j, q, before, tot, rnk);
+            L.slist[ss + before + rnk] = (uint16_t)(j - ss);
+            if (rnk == 0) {
+                L.scnt[l][q] = (uint16_t)tot;
+                L.soff[l][q] = (uint16_t)before;
+            }
+            L.key[j] = 0x7FFFFFFF;
+        }
+        // candidates per cell (noTimeCounter.C:142-153)
         if (t >= a && t < b) {
             const int c = c0 + t;
-            const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
-            double ncoll = 0.0, sep = 0.0;
+            const int nC = L.segS[t + 1] - L.segS[t];
+            int nsparse = 0;
+            L.rcoll[t] = 0u;
+            L.sep[t] = 0.0;
+            L.rk[t] = 0;
             if (do_collide && nC > 1) {
-                double smax;
-                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+                const double smax = pf_sig;
+                const long long ncand = ntc_candidates_of(P, c, nC, smax, pf_rem, pf_vol, rem, cx);
+                L.smax[t] = smax;
+                L.rsmax[t] = (unsigned long long)__double_as_longlong(smax);
+                L.ncand[t] = (int32_t)ncand;
                 if (ncand > DENSE_CAND) {
-                    L.ncand[t] = ncand;
-                    L.smax[t] = smax;
                     L.dense[atomicAdd(&L.ndense, 1)] = t;
                 } else {
-                    ntc_lane<GEN>(P, c, ss, nC, ncand, smax, L.vx, L.vy, L.vz, L.sub, L.slist, L.er, L.vb, PMAX,
-                                  L.ty, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
+                    nsparse = (int)ncand;
+                    cx.cand += (unsigned long long)ncand;
                 }
             }
-            L.ncoll[t] = ncoll;
-            L.sep[t] = sep;
+            L.rn[t] = nsparse;
         }
         __syncthreads();
         // dense cells: one wave each
@@ -989,16 +1193,24 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         for (int di = wid; di < nd; di += CT / WAVE) {
             const int l = L.dense[di];
             const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
-            double ncoll, sep;
-            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, L.slist, L.er, L.vb,
+            const LdsTab tab = { L.scnt[l], L.soff[l] };
+            uint32_t ncoll;
+            double sep;
+            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, L.slist, tab, L.er, L.vb,
                           PMAX, L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
-            if (lane == 0) { L.ncoll[l] = ncoll; L.sep[l] = sep; }
+            if (lane == 0) { L.rcoll[l] = ncoll; L.sep[l] = sep; }
+        }
+        // sparse cells: candidate-parallel rounds over the workgroup
+        ntc_rounds<GEN, PMAX>(L, P, c0, a, b, S0, step, dst, cx);
+        if (t >= a && t < b && L.rn[t] > 0) {
+            sigma[c0 + t] = __longlong_as_double((long long)L.rsmax[t]);
+            cx.colls += L.rcoll[t];
         }
         __syncthreads();
         if (do_sample && t >= a && t < b) {
             const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
-            sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc, L.ncoll[t],
-                             L.sep[t]);
+            sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
+                             (double)L.rcoll[t], L.sep[t]);
         }
         for (int j = t; j < cnt; j += CT) {
             const int64_t gd = (int64_t)S0 + j;
@@ -1028,7 +1240,7 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ sta
 }
 
 template <bool GEN, int PMAX>
-__global__ __launch_bounds__(CT) void k_collide(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
+__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
                                                 const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
                                                 const int32_t* __restrict__ tile_begin, int pt,
                                                 double* __restrict__ sigma, double* __restrict__ rem,
@@ -1056,7 +1268,7 @@ void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, hip
     k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin);
 }
 
-#define PMAX_MONO 3072
+#define PMAX_MONO 2816
 #define PMAX_GEN 1792
 
 void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
