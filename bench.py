@@ -35,9 +35,11 @@ STAGE_BYTES = {
     "scan": dict(per_particle=0.0, per_cell=12.0),
     # read cell + slot, write the permutation entry
     "scatter": dict(per_particle=12.0, per_cell=0.0),
-    # read perm + pos + vel, write sorted pos + vel + cell; per cell:
-    # start, sigmaTcRMax and remainder (read+write), volume, 11 sample sums (read+write)
-    "collide": dict(per_particle=80.0, per_cell=4 + 16 + 16 + 8 + 11 * 16),
+    # read perm + pos + vel, write sorted pos + vel + cell + sub-cell byte; per cell: start
+    "gather": dict(per_particle=81.0, per_cell=4.0),
+    # read sorted vel + sub-cell byte, write vel; per cell: start, sigmaTcRMax
+    # and remainder (read+write), volume, 11 sample sums (read+write)
+    "collide": dict(per_particle=25.0, per_cell=4 + 16 + 16 + 8 + 11 * 16),
     # per inserted particle: 40 B written (counted per particle moved: small)
     "inject": dict(per_particle=0.0, per_cell=0.0),
 }

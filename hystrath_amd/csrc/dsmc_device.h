@@ -425,11 +425,14 @@ __device__ double rot_ratio(double dof, double chib, R& r) {
 
 // LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
 // species with one electronic level
-template <class R>
+// INTERNAL false: every species has neither rotational nor vibrational
+// modes, and only the electronic test draw remains.
+template <bool INTERNAL, class R>
 __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* et, double* erot,
                                 int* vib, double om, R& r) {
     if (P.charge[t] == -1) return;
     if (1.0 / P.z_el > r.u()) (void)r.u();
+    if (!INTERNAL) return;
     const int nm = P.nvib[t];
     if (nm > 0) {
         double pre[HS_MAX_VIB_MODES];
@@ -454,8 +457,9 @@ __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* 
     }
 }
 
-// BinaryCollisionModel::collide for the configured model
-template <class R>
+// BinaryCollisionModel::collide for the configured model (INTERNAL as for
+// lb_redistribute)
+template <bool INTERNAL = true, class R>
 __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3], double uq[3],
                                double* erp, double* erq, int* vp, int* vq, R& r) {
     const int model = P.coll_model;
@@ -469,8 +473,8 @@ __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3],
     const double cr2 = dx * dx + dy * dy + dz * dz;
     double et = 0.5 * mr * cr2;
     const double om = P.omega_pq[tp][tq];
-    lb_redistribute(P, tp, tq, &et, erp, vp, om, r);
-    lb_redistribute(P, tq, tp, &et, erq, vq, om, r);
+    lb_redistribute<INTERNAL>(P, tp, tq, &et, erp, vp, om, r);
+    lb_redistribute<INTERNAL>(P, tq, tp, &et, erq, vq, om, r);
     const double cr = sqrt(2.0 * et / mr);
     post_coll_vel(P, model == HS_COLL_LB_VSS, true, tp, tq, up, uq, cr, r);
 }

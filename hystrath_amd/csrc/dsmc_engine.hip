@@ -20,7 +20,7 @@ namespace {
 
 thread_local std::string g_create_error;
 
-enum { T_INJECT = 0, T_MOVE, T_SCAN, T_SCATTER, T_COLLIDE, T_NSTAGE };
+enum { T_INJECT = 0, T_MOVE, T_SCAN, T_SCATTER, T_COLLIDE, T_GATHER, T_NSTAGE };
 constexpr int kMaxTimedPairs = 65536;
 
 struct TimedPair {
@@ -54,6 +54,9 @@ struct hs_dsmc_engine {
     uint16_t* d_big_list = nullptr;
     uint8_t* d_big_sub = nullptr;
     int32_t* d_tile_begin = nullptr;
+    int32_t* d_tile_order = nullptr;   // collide workgroup -> tile, heavy tiles first
+    int32_t* d_tile_heavy = nullptr;
+    int32_t* d_order_ctr = nullptr;
     int tile_parcels = 1536;  // parcels per collide workgroup (particle-balanced tiles)
     DevFaces faces;
     int32_t* d_face_off = nullptr;
@@ -72,8 +75,8 @@ struct hs_dsmc_engine {
     // timing
     std::vector<TimedPair> timed;
     std::vector<hipEvent_t> event_pool;
-    int64_t launches[T_NSTAGE] = { 0, 0, 0, 0, 0 };
-    double closed_ms[T_NSTAGE] = { 0, 0, 0, 0, 0 };
+    int64_t launches[T_NSTAGE] = { 0, 0, 0, 0, 0, 0 };
+    double closed_ms[T_NSTAGE] = { 0, 0, 0, 0, 0, 0 };
 };
 
 // ---------------------------------------------------------------------------
@@ -472,6 +475,8 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         (rc = dev_alloc(eng, &eng->d_slot, eng->cap)) || (rc = dev_alloc(eng, &eng->d_perm, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_big_list, eng->cap)) || (rc = dev_alloc(eng, &eng->d_big_sub, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_tile_begin, eng->cap / 64 + 4)) ||
+        (rc = dev_alloc(eng, &eng->d_tile_order, eng->cap / 64 + 4)) || (rc = dev_alloc(eng, &eng->d_order_ctr, 2)) ||
+        (rc = dev_alloc(eng, &eng->d_tile_heavy, eng->cap / 64 + 4)) ||
         (rc = dev_alloc(eng, &eng->d_S, 1)))
         return bail(rc);
     // inflow faces
@@ -813,11 +818,19 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
         launch_scatter(cur_buf(eng), eng->d_S, eng->d_start, eng->d_slot, eng->d_perm, eng->move_grid, eng->stream);
     }
     {
+        StageTimer tm(eng, T_GATHER);
+        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->cap, eng->d_tile_begin, eng->d_tile_heavy,
+                     eng->stream);
+        launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
+                      eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
+    }
+    if (do_collide || do_sample) {
         StageTimer tm(eng, T_COLLIDE);
-        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->stream);
-        launch_collide(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
-                       eng->tile_parcels, eng->cap, eng->d_sigma, eng->d_rem, eng->d_samples, eng->d_vol, eng->d_S,
-                       do_collide, do_sample, eng->general, eng->nc, eng->d_big_list, eng->d_big_sub, eng->stream);
+        launch_collide(eng->dp, nxt_buf(eng), eng->d_start, eng->d_tile_begin, eng->d_tile_heavy, eng->d_tile_order,
+                       eng->d_order_ctr,
+                       eng->tile_parcels, eng->cap, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
+                       eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc, eng->d_big_list,
+                       eng->stream);
     }
     launch_commit(eng->d_S, advance, eng->stream);
     eng->cur = 1 - eng->cur;

@@ -54,11 +54,19 @@ void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* cou
 void launch_set_nold(DevScalars* S, hipStream_t st);
 void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
                     int32_t* perm, int grid, hipStream_t st);
-void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, hipStream_t st);
-void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
-                    const int32_t* perm, const int32_t* tile_begin, int pt, int64_t capacity, double* sigma,
+// particle-balanced tiles of pt parcels (tile_begin) and their heavy flags
+void launch_tiles(const int32_t* start, int nc, int pt, int64_t capacity, int32_t* tile_begin, int32_t* heavy,
+                  hipStream_t st);
+// stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel
+void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
+                   const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
+                   int general, hipStream_t st);
+// NTC, collisions and sampling on the sorted population in dst
+void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
+                    const int32_t* heavy, int32_t* order, int32_t* order_ctr, int pt, int64_t capacity, uint8_t* sub,
+                    double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
-                    int general, int nc, uint16_t* big_list, uint8_t* big_sub, hipStream_t st);
+                    int general, int nc, uint16_t* big_list, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, hipStream_t st);
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
                        int32_t* cnt, int nc, int ns, hipStream_t st);

@@ -546,18 +546,48 @@ struct CellCtx {
     unsigned long long cand, colls, ovf;
 };
 
-// Sub-cell sizes and list offsets of one cell: packed in registers (one lane
-// per cell) or a per-cell LDS table.
+// Position of the k-th (from 0) set bit of m.
+__device__ __forceinline__ int select_bit(unsigned long long m, int k) {
+    int pos = 0;
+    const int c0 = __popc((uint32_t)m);
+    if (k >= c0) { k -= c0; m >>= 32; pos += 32; }
+    uint32_t w = (uint32_t)m;
+    int c = __popc(w & 0xFFFFu);
+    if (k >= c) { k -= c; w >>= 16; pos += 16; }
+    c = __popc(w & 0xFFu);
+    if (k >= c) { k -= c; w >>= 8; pos += 8; }
+    c = __popc(w & 0xFu);
+    if (k >= c) { k -= c; w >>= 4; pos += 4; }
+    c = __popc(w & 0x3u);
+    if (k >= c) { k -= c; w >>= 2; pos += 2; }
+    c = (int)(w & 1u);
+    if (k >= c) pos += 1;
+    return pos;
+}
+
+// The members of each sub-cell of one cell, in ascending index order: cnt(q)
+// and member(q, k), the k-th member's index in the cell.
+// PackTab: one lane's packed sizes/offsets into a member list.
 struct PackTab {
     Pack8 c, o;
+    const uint16_t* slist;
+    int ss;
     __device__ __forceinline__ int cnt(int q) const { return c.get(q); }
-    __device__ __forceinline__ int off(int q) const { return o.get(q); }
+    __device__ __forceinline__ int member(int q, int k) const { return slist[ss + o.get(q) + k]; }
 };
-struct LdsTab {
-    const uint16_t* c;
-    const uint16_t* o;
-    __device__ __forceinline__ int cnt(int q) const { return c[q]; }
-    __device__ __forceinline__ int off(int q) const { return o[q]; }
+// CellTab: the LDS table of a cell -- for at most 64 parcels, a membership
+// bit mask per sub-cell; for more, (offset | count << 16) into the member list.
+struct CellTab {
+    const unsigned long long* m;
+    const uint16_t* slist;
+    int ss;
+    bool small;
+    __device__ __forceinline__ int cnt(int q) const {
+        return small ? __popcll(m[q]) : (int)((m[q] >> 16) & 0xFFFFull);
+    }
+    __device__ __forceinline__ int member(int q, int k) const {
+        return small ? select_bit(m[q], k) : (int)slist[ss + (int)(m[q] & 0xFFFFull) + k];
+    }
 };
 
 // Candidate count of noTimeCounter.C:142-153 from the cell's sigmaTcRMax,
@@ -581,16 +611,14 @@ __device__ __forceinline__ long long ntc_candidates(const DevParams& P, int c, i
 // Draws the pair of candidate k (its own stream) and reports whether it may
 // collide: noTimeCounter.C:157-228.
 template <bool GEN, class Tab>
-__device__ __forceinline__ bool draw_pair(const DevParams& P, PhiloxStream& r, int nC, const uint8_t* sub,
-                                          const uint16_t* slist, int ss, const Tab& tab, const uint8_t* ty,
-                                          int* pp_out, int* qq_out) {
+__device__ __forceinline__ bool draw_pair(const DevParams& P, PhiloxStream& r, int nC, const uint8_t* sub, int ss,
+                                          const Tab& tab, const uint8_t* ty, int* pp_out, int* qq_out) {
     const int pp = random_label(r, 0, nC - 1);
     const int q = sub[ss + pp];
     const int nsc = tab.cnt(q);
     int qq = -1, guard = 0;
     if (nsc > 1) {
-        const int o = tab.off(q);
-        do { qq = slist[ss + o + random_label(r, 0, nsc - 1)]; } while (pp == qq && ++guard < HS_MAX_REDRAW);
+        do { qq = tab.member(q, random_label(r, 0, nsc - 1)); } while (pp == qq && ++guard < HS_MAX_REDRAW);
     } else {
         do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
     }
@@ -614,7 +642,7 @@ __device__ __forceinline__ void do_collision(const DevParams& P, PhiloxStream& r
         for (int m = 0; m < P.nvib[tp]; ++m) vp[m] = vb[m * vstride + ip];
         for (int m = 0; m < P.nvib[tq]; ++m) vq[m] = vb[m * vstride + iq];
     }
-    binary_collide(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
+    binary_collide<GEN>(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
     vx[ip] = (float)up[0]; vy[ip] = (float)up[1]; vz[ip] = (float)up[2];
     vx[iq] = (float)uq[0]; vy[iq] = (float)uq[1]; vz[iq] = (float)uq[2];
     if (GEN) {
@@ -648,13 +676,15 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
                          CellCtx& cx, double& ncoll, double& sep) {
     PackTab tab;
     subcell_lists_lane(sub, slist, ss, nC, tab.c, tab.o);
+    tab.slist = slist;
+    tab.ss = ss;
     double smax_cur = smax;
     cx.cand += (unsigned long long)ncand;
     for (long long k = 0; k < ncand; ++k) {
         PhiloxStream r;
         r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
         int pp, qq;
-        if (!draw_pair<GEN>(P, r, nC, sub, slist, ss, tab, ty, &pp, &qq)) continue;
+        if (!draw_pair<GEN>(P, r, nC, sub, ss, tab, ty, &pp, &qq)) continue;
         const int ip = ss + pp, iq = ss + qq;
         const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
         double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
@@ -684,7 +714,7 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
 // The sub-cell lists and tables are built and the stamps cleared beforehand.
 template <bool GEN>
 __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
-                         float* vz, const uint8_t* sub, const uint16_t* slist, const LdsTab& tab, float* er,
+                         float* vz, const uint8_t* sub, const CellTab& tab, float* er,
                          uint16_t* vb, int vstride, const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step,
                          const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out) {
     const int lane = threadIdx.x & (WAVE - 1);
@@ -701,7 +731,7 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
         PhiloxStream r;
         if (active) {
             r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-            valid = draw_pair<GEN>(P, r, nC, sub, slist, ss, tab, ty, &pp, &qq);
+            valid = draw_pair<GEN>(P, r, nC, sub, ss, tab, ty, &pp, &qq);
             if (valid) {
                 const int ip = ss + pp, iq = ss + qq;
                 const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
@@ -889,13 +919,13 @@ struct TileLds {
     alignas(16) int32_t key[PMAX];  // old index of each slot; then the NTC conflict stamps
     uint8_t loc[PMAX];       // sub-tile cell of each slot
     alignas(16) uint8_t sub[PMAX];
-    uint16_t slist[PMAX];    // per cell, its parcels by sub-cell, ascending
+    uint16_t slist[PMAX];    // cells of more than 64 parcels: their parcels by sub-cell, ascending
     float vx[PMAX], vy[PMAX], vz[PMAX];
     float er[GEN ? PMAX : 1];
     uint16_t vb[GEN ? HS_MAX_VIB_MODES * PMAX : 1];
     uint8_t ty[GEN ? PMAX : 1];
     int32_t segS[CT_CELLS + 1];
-    uint16_t scnt[CT_CELLS][8], soff[CT_CELLS][8];  // sub-cell sizes and list offsets
+    unsigned long long smask[CT_CELLS][8];  // sub-cell tables (CellTab)
     double sep[CT_CELLS];
     double smax[CT_CELLS];   // sigmaTcRMax at the start of the step
     int32_t ncand[CT_CELLS];
@@ -958,8 +988,8 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
             ss = L.segS[l] - S0;
             const int nC = L.segS[l + 1] - L.segS[l];
             r.init(P, RNG_COLLIDE, (uint32_t)(c0 + l), step, (uint32_t)k);
-            const LdsTab tab = { L.scnt[l], L.soff[l] };
-            valid = draw_pair<GEN>(P, r, nC, L.sub, L.slist, ss, tab, L.ty, &pp, &qq);
+            const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
+            valid = draw_pair<GEN>(P, r, nC, L.sub, ss, tab, L.ty, &pp, &qq);
             if (valid) {
                 const int ip = ss + pp, iq = ss + qq;
                 const int tp = GEN ? (int)L.ty[ip] : 0, tq = GEN ? (int)L.ty[iq] : 0;
@@ -1011,102 +1041,70 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
     (void)cx;
 }
 
-// One cell with more parcels than an LDS chunk holds: the stable gather,
-// NTC (one lane) and sampling in global memory.  Kept out of line: it is
-// rare, and inlining it would cost the common path registers.
-template <bool GEN>
-__device__ __noinline__ void collide_oversize(const DevParams& P, int c, int S0, int nC, DevParticles& src,
-                                              DevParticles& dst, const int32_t* __restrict__ perm,
-                                              double* __restrict__ sigma, double* __restrict__ rem,
-                                              double* __restrict__ samples, const double* __restrict__ vol,
-                                              uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
-                                              uint8_t* big_sub, CellCtx& cx) {
-    const int t = threadIdx.x;
+// ---------------------------------------------------------------------------
+// k_gather: the stable cell order of buildCellOccupancy (dsmcCloud.C:63-74).
+// Each parcel's rank among the parcels of its cell by old index gives its
+// slot; the population is copied to the other buffer in that order together
+// with each parcel's sub-cell byte (noTimeCounter.C:126-138).  Small LDS and
+// few registers: many workgroups per CU hide the scattered reads.
+// ---------------------------------------------------------------------------
+#define PMAX_SORT 4096    // parcels per LDS chunk of the gather
+
+struct GatherLds {
+    alignas(16) int32_t key[PMAX_SORT];  // old index of each slot of the chunk
+    uint8_t loc[PMAX_SORT];              // sub-tile cell of each slot
+    int32_t segS[CT_CELLS + 1];
+};
+
+__device__ __forceinline__ void copy_internal(const DevParams& P, const DevParticles& src, DevParticles& dst, int k,
+                                              int64_t gd) {
     const int64_t cap = P.capacity;
-    for (int j = t; j < nC; j += CT) {
-        const int k = perm[S0 + j];
-        int rank = 0;
-        for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
-        const int64_t d = (int64_t)S0 + rank;
-        const double x = src.px[k], y = src.py[k], z = src.pz[k];
-        dst.px[d] = x; dst.py[d] = y; dst.pz[d] = z;
-        dst.ux[d] = src.ux[k]; dst.uy[d] = src.uy[k]; dst.uz[d] = src.uz[k];
-        dst.cell[d] = c;
-        if (dst.erot) dst.erot[d] = src.erot[k];
-        if (dst.vib)
-            for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + d] = src.vib[m * cap + k];
-        if (dst.type) dst.type[d] = src.type[k];
-        big_sub[d] = (uint8_t)subcell_of(P, c, x, y, z);
-    }
-    __syncthreads();
-    if (t == 0) {
-        double ncoll = 0.0, sep = 0.0;
-        if (nC > 65535) {
-            cx.ovf += 1;
-        } else {
-            float* gvx = dst.ux + S0;
-            float* gvy = dst.uy + S0;
-            float* gvz = dst.uz + S0;
-            float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
-            uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
-            const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
-            if (do_collide && nC > 1) {
-                double smax;
-                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-                ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, big_sub + S0, big_list + S0, ger, gvb,
-                              (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
-            }
-            if (do_sample)
-                sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
-        }
-    }
-    __syncthreads();
+    if (dst.erot) dst.erot[gd] = src.erot[k];
+    if (dst.vib)
+        for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = src.vib[m * cap + k];
+    if (dst.type) dst.type[gd] = src.type[k];
 }
 
-// Gather, collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS)
-// in LDS chunks of at most PMAX parcels.
-template <bool GEN, int PMAX>
-__device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int ncell, DevParticles& src,
-                              DevParticles& dst, const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
-                              double* __restrict__ sigma, double* __restrict__ rem, double* __restrict__ samples,
-                              const double* __restrict__ vol, uint32_t step, int do_collide, int do_sample, int nc,
-                              uint16_t* big_list, uint8_t* big_sub, CellCtx& cx) {
+template <bool GEN>
+__device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell, const DevParticles& src,
+                             DevParticles& dst, const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                             uint8_t* __restrict__ sub_out) {
     const int t = threadIdx.x;
-    const int wid = t / WAVE, lane = t & (WAVE - 1);
-    const int64_t cap = P.capacity;
     if (t < ncell) L.segS[t] = start[c0 + t];
     if (t == 0) L.segS[ncell] = start[c0 + ncell];
     __syncthreads();
     int a = 0;
     while (a < ncell) {
         const int S0 = L.segS[a];
-        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX;
+        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX_SORT;
         const int b = a + __syncthreads_count(ok);
         if (b == a) {
-            // one cell larger than the tile: stable order and NTC in global memory
-            collide_oversize<GEN>(P, c0 + a, S0, L.segS[a + 1] - S0, src, dst, perm, sigma, rem, samples, vol, step,
-                                  do_collide, do_sample, nc, big_list, big_sub, cx);
+            // one cell larger than a chunk: ranks from global memory
+            const int nC = L.segS[a + 1] - S0;
+            const int c = c0 + a;
+            for (int j = t; j < nC; j += CT) {
+                const int k = perm[S0 + j];
+                int rank = 0;
+                for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+                const int64_t gd = (int64_t)S0 + rank;
+                const double x = src.px[k], y = src.py[k], z = src.pz[k];
+                dst.px[gd] = x; dst.py[gd] = y; dst.pz[gd] = z;
+                dst.ux[gd] = src.ux[k]; dst.uy[gd] = src.uy[k]; dst.uz[gd] = src.uz[k];
+                dst.cell[gd] = c;
+                if (GEN) copy_internal(P, src, dst, k, gd);
+                sub_out[gd] = (uint8_t)subcell_of(P, c, x, y, z);
+            }
+            __syncthreads();
             a += 1;
             continue;
         }
-        const int E0 = L.segS[b];
-        const int cnt = E0 - S0;
-        // slot -> cell map and the old indices of the chunk; each cell's
-        // sigmaTcRMax, remainder and volume are fetched now, used after the gather
-        double pf_sig = 0.0, pf_rem = 0.0, pf_vol = 1.0;
-        if (t >= a && t < b) {
-            if (do_collide) {
-                pf_sig = sigma[c0 + t];
-                pf_rem = rem[c0 + t];
-                pf_vol = vol[c0 + t];
-            }
+        const int cnt = L.segS[b] - S0;
+        if (t >= a && t < b)
             for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
-        }
 #pragma unroll 4
         for (int j = t; j < cnt; j += CT) L.key[j] = perm[S0 + j];
         __syncthreads();
-        // stable order inside each cell (rank by old index) and gather, four
-        // parcels per thread with their loads in flight together
+        // four parcels per thread with their loads in flight together
         for (int j0 = t; j0 < cnt; j0 += 4 * CT) {
             int kk[4], dd[4], cc[4];
 #pragma unroll
@@ -1134,33 +1132,136 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
 #pragma unroll
             for (int u = 0; u < 4; ++u) {
                 if (kk[u] < 0) continue;
-                const int d = dd[u], c = cc[u], k = kk[u];
-                const int64_t gd = (int64_t)S0 + d;
+                const int64_t gd = (int64_t)S0 + dd[u];
                 dst.px[gd] = x[u]; dst.py[gd] = y[u]; dst.pz[gd] = z[u];
-                dst.cell[gd] = c;
-                L.sub[d] = (uint8_t)subcell_of(P, c, x[u], y[u], z[u]);
-                L.vx[d] = ux[u]; L.vy[d] = uy[u]; L.vz[d] = uz[u];
-                if (GEN) {
-                    L.er[d] = src.erot ? src.erot[k] : 0.0f;
-                    for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + d] = src.vib[m * cap + k];
-                    L.ty[d] = src.type ? src.type[k] : (uint8_t)0;
-                }
+                dst.ux[gd] = ux[u]; dst.uy[gd] = uy[u]; dst.uz[gd] = uz[u];
+                dst.cell[gd] = cc[u];
+                if (GEN) copy_internal(P, src, dst, kk[u], gd);
+                sub_out[gd] = (uint8_t)subcell_of(P, cc[u], x[u], y[u], z[u]);
+            }
+        }
+        __syncthreads();
+        a = b;
+    }
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(CT) void k_gather(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
+                                               const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                                               const int32_t* __restrict__ tile_begin, int pt,
+                                               const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out) {
+    __shared__ GatherLds L;
+    const DevParams& P = *Pp;
+    const int q = blockIdx.x;
+    if ((int64_t)q * pt > (int64_t)S->n_sorted) return;
+    const int cb = tile_begin[q], ce = tile_begin[q + 1];
+    for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
+        gather_cells<GEN>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sub_out);
+}
+
+// ---------------------------------------------------------------------------
+// k_collide: NTC, collisions and sampling on the sorted population
+// ---------------------------------------------------------------------------
+// One cell with more parcels than an LDS chunk holds: NTC on one lane and the
+// sampling in global memory.  Inlined: a call would cost the kernel more
+// registers (the caller keeps its live state across it) than the path does.
+template <bool GEN>
+__device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int S0, int nC, DevParticles& dst,
+                                              double* __restrict__ sigma, double* __restrict__ rem,
+                                              double* __restrict__ samples, const double* __restrict__ vol,
+                                              uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
+                                              uint8_t* sub, CellCtx& cx) {
+    if (threadIdx.x == 0) {
+        const int64_t cap = P.capacity;
+        double ncoll = 0.0, sep = 0.0;
+        if (nC > 65535) {
+            cx.ovf += 1;
+        } else {
+            float* gvx = dst.ux + S0;
+            float* gvy = dst.uy + S0;
+            float* gvz = dst.uz + S0;
+            float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
+            uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
+            const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
+            if (do_collide && nC > 1) {
+                double smax;
+                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+                ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb,
+                              (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
+            }
+            if (do_sample)
+                sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
+        }
+    }
+    __syncthreads();
+}
+
+// Collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS) in LDS
+// chunks of at most PMAX parcels.
+template <bool GEN, int PMAX>
+__device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int ncell, DevParticles& dst,
+                              const int32_t* __restrict__ start, uint8_t* __restrict__ sub,
+                              double* __restrict__ sigma, double* __restrict__ rem, double* __restrict__ samples,
+                              const double* __restrict__ vol, uint32_t step, int do_collide, int do_sample, int nc,
+                              uint16_t* big_list, CellCtx& cx) {
+    const int t = threadIdx.x;
+    const int wid = t / WAVE, lane = t & (WAVE - 1);
+    const int64_t cap = P.capacity;
+    if (t < ncell) L.segS[t] = start[c0 + t];
+    if (t == 0) L.segS[ncell] = start[c0 + ncell];
+    __syncthreads();
+    int a = 0;
+    while (a < ncell) {
+        const int S0 = L.segS[a];
+        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX;
+        const int b = a + __syncthreads_count(ok);
+        if (b == a) {
+            collide_oversize<GEN>(P, c0 + a, S0, L.segS[a + 1] - S0, dst, sigma, rem, samples, vol, step, do_collide,
+                                  do_sample, nc, big_list, sub, cx);
+            a += 1;
+            continue;
+        }
+        const int cnt = L.segS[b] - S0;
+        // slot -> cell map; each cell's sigmaTcRMax, remainder and volume; the
+        // sorted velocities and sub-cells of the chunk (contiguous)
+        double pf_sig = 0.0, pf_rem = 0.0, pf_vol = 1.0;
+        if (t >= a && t < b) {
+            if (do_collide) {
+                pf_sig = sigma[c0 + t];
+                pf_rem = rem[c0 + t];
+                pf_vol = vol[c0 + t];
+            }
+            for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
+        }
+#pragma unroll 4
+        for (int j = t; j < cnt; j += CT) {
+            const int64_t gj = (int64_t)S0 + j;
+            L.vx[j] = dst.ux[gj]; L.vy[j] = dst.uy[gj]; L.vz[j] = dst.uz[gj];
+            L.sub[j] = sub[gj];
+            if (GEN) {
+                L.er[j] = dst.erot ? dst.erot[gj] : 0.0f;
+                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + j] = dst.vib[m * cap + gj];
+                L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
             }
         }
         if (t == 0) L.ndense = 0;
         __syncthreads();
-        // sub-cell lists in parallel: each parcel finds its place among the
-        // parcels of its sub-cell (ascending index) and clears its stamp
+        // sub-cell tables in parallel: a parcel of a cell of at most 64 sets
+        // its bit in its sub-cell's mask; in a larger cell it finds its place
+        // among the parcels of its sub-cell (ascending index).  Stamps cleared.
         for (int j = t; j < cnt; j += CT) {
             const int l = L.loc[j];
             const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
             const int q = L.sub[j];
-            int before, tot, rnk;
-            subcell_rank(L.sub, ss, ee, j, q, before, tot, rnk);
-            L.slist[ss + before + rnk] = (uint16_t)(j - ss);
-            if (rnk == 0) {
-                L.scnt[l][q] = (uint16_t)tot;
-                L.soff[l][q] = (uint16_t)before;
+            if (ee - ss <= 64) {
+                atomicOr(&L.smask[l][q], 1ull << (j - ss));
+            } else {
+                int before, tot, rnk;
+                subcell_rank(L.sub, ss, ee, j, q, before, tot, rnk);
+                L.slist[ss + before + rnk] = (uint16_t)(j - ss);
+                if (rnk == 0) L.smask[l][q] = (unsigned long long)before | ((unsigned long long)tot << 16);
             }
             L.key[j] = 0x7FFFFFFF;
         }
@@ -1193,11 +1294,11 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         for (int di = wid; di < nd; di += CT / WAVE) {
             const int l = L.dense[di];
             const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
-            const LdsTab tab = { L.scnt[l], L.soff[l] };
+            const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
             uint32_t ncoll;
             double sep;
-            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, L.slist, tab, L.er, L.vb,
-                          PMAX, L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
+            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, tab, L.er, L.vb, PMAX,
+                          L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
             if (lane == 0) { L.rcoll[l] = ncoll; L.sep[l] = sep; }
         }
         // sparse cells: candidate-parallel rounds over the workgroup
@@ -1219,7 +1320,6 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
                 if (dst.erot) dst.erot[gd] = L.er[j];
                 if (dst.vib)
                     for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
-                if (dst.type) dst.type[gd] = L.ty[j];
             }
         }
         __syncthreads();
@@ -1229,59 +1329,93 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
 
 // Tile q of the particle-balanced decomposition owns the cells whose start
 // offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.
+// Also flags the tiles holding a cell of more than HEAVY_CELL parcels
+// (heavy[] cleared beforehand).
+#define HEAVY_CELL 128
 __global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt,
-                                                 int32_t* __restrict__ tile_begin) {
+                                                 int32_t* __restrict__ tile_begin, int32_t* __restrict__ heavy) {
     const int i = blockIdx.x * BLOCK + threadIdx.x;
     if (i > nc) return;
     if (i == 0) { tile_begin[0] = 0; return; }
     const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
     for (int q = q0 + 1; q <= q1; ++q) tile_begin[q] = i;
     if (i == nc) tile_begin[q1 + 1] = nc;
+    if (start[i] - start[i - 1] > HEAVY_CELL) heavy[q0] = 1;
+}
+
+// Order in which the collide workgroups take the tiles: heavy tiles first
+// (they run longest), the others after.  Tiles are independent, so the order
+// does not change any result.
+__global__ __launch_bounds__(BLOCK) void k_tile_order(const int32_t* __restrict__ heavy, int pt,
+                                                      const DevScalars* __restrict__ S, int32_t* __restrict__ ctr,
+                                                      int32_t* __restrict__ order) {
+    const int q = blockIdx.x * BLOCK + threadIdx.x;
+    const int nt = (int)((int64_t)S->n_sorted / pt) + 1;
+    if (q >= nt) return;
+    if (heavy[q]) order[atomicAdd(&ctr[0], 1)] = q;
+    else order[nt - 1 - atomicAdd(&ctr[1], 1)] = q;
 }
 
 template <bool GEN, int PMAX>
-__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
-                                                const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
-                                                const int32_t* __restrict__ tile_begin, int pt,
-                                                double* __restrict__ sigma, double* __restrict__ rem,
-                                                double* __restrict__ samples, const double* __restrict__ vol,
-                                                DevScalars* __restrict__ S, int do_collide, int do_sample, int nc,
-                                                uint16_t* big_list, uint8_t* big_sub) {
+__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ tile_begin,
+                                                   const int32_t* __restrict__ order, int pt,
+                                                   uint8_t* __restrict__ sub, double* __restrict__ sigma,
+                                                   double* __restrict__ rem, double* __restrict__ samples,
+                                                   const double* __restrict__ vol, DevScalars* __restrict__ S,
+                                                   int do_collide, int do_sample, int nc, uint16_t* big_list) {
     __shared__ TileLds<GEN, PMAX> L;
     const DevParams& P = *Pp;
-    const int q = blockIdx.x;
     const int64_t nsort = S->n_sorted;
-    if ((int64_t)q * pt > nsort) return;
+    if ((int64_t)blockIdx.x * pt > nsort) return;
+    const int q = order[blockIdx.x];
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
     if (cb >= ce) return;
     const uint32_t step = S->step;
     CellCtx cx = { 0, 0, 0 };
     for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
-        collide_cells<GEN, PMAX>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sigma, rem, samples, vol,
-                                 step, do_collide, do_sample, nc, big_list, big_sub, cx);
+        collide_cells<GEN, PMAX>(L, P, c0, min(CT_CELLS, ce - c0), dst, start, sub, sigma, rem, samples, vol, step,
+                                 do_collide, do_sample, nc, big_list, cx);
     add_counter(S, CNT_CAND, cx.cand);
     add_counter(S, CNT_COLL, cx.colls);
     add_counter(S, CNT_TILE_OVF, cx.ovf);
 }
 
-void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, hipStream_t st) {
-    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin);
+void launch_tiles(const int32_t* start, int nc, int pt, int64_t capacity, int32_t* tile_begin, int32_t* heavy,
+                  hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
+    k_fill_i32<<<grid_for(nt), BLOCK, 0, st>>>(heavy, 0, nt);
+    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin, heavy);
 }
 
-#define PMAX_MONO 2816
-#define PMAX_GEN 1792
+#define PMAX_MONO 2304
+#define PMAX_GEN 1536
 
-void launch_collide(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start,
-                    const int32_t* perm, const int32_t* tile_begin, int pt, int64_t capacity, double* sigma,
-                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
-                    int general, int nc, uint16_t* big_list, uint8_t* big_sub, hipStream_t st) {
+void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
+                   const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
+                   int general, hipStream_t st) {
     const int nt = (int)(capacity / pt) + 2;
     if (general)
-        k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, sigma, rem, samples,
-                                                      vol, S, do_collide, do_sample, nc, big_list, big_sub);
+        k_gather<true><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
     else
-        k_collide<false, PMAX_MONO><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, sigma, rem, samples,
-                                                        vol, S, do_collide, do_sample, nc, big_list, big_sub);
+        k_gather<false><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
+}
+
+void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
+                    const int32_t* heavy, int32_t* order, int32_t* order_ctr, int pt, int64_t capacity, uint8_t* sub,
+                    double* sigma,
+                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
+                    int general, int nc, uint16_t* big_list, hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
+    k_fill_i32<<<1, 64, 0, st>>>(order_ctr, 0, 2);
+    k_tile_order<<<(nt + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(heavy, pt, S, order_ctr, order);
+    if (general)
+        k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples,
+                                                      vol, S, do_collide, do_sample, nc, big_list);
+    else
+        k_collide<false, PMAX_MONO><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem,
+                                                        samples, vol, S, do_collide, do_sample, nc, big_list);
 }
 
 // publish the sorted population; optionally advance the step counter

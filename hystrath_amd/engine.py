@@ -28,7 +28,7 @@ EXPORTS = [
     "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
 ]
 
-STAGES = ["inject", "move", "scan", "scatter", "collide"]
+STAGES = ["inject", "move", "scan", "scatter", "collide", "gather"]
 COUNTERS = ["candidates", "collisions", "deleted", "injected", "wall_hits", "track_overflow", "tile_overflow",
             "capacity_overflow", "particle_steps"]
 
@@ -311,9 +311,9 @@ class Engine:
         self._ck(self.L.hs_dsmc_reset_counters(self.h), "reset_counters")
 
     def kernel_times(self, reset=False):
-        ms = np.zeros(5)
-        nl = np.zeros(5, np.int64)
-        self._ck(self.L.hs_dsmc_kernel_times(self.h, _d(ms), abi.ptr(nl, C.c_int64), 5), "kernel_times")
+        ms = np.zeros(len(STAGES))
+        nl = np.zeros(len(STAGES), np.int64)
+        self._ck(self.L.hs_dsmc_kernel_times(self.h, _d(ms), abi.ptr(nl, C.c_int64), len(STAGES)), "kernel_times")
         if reset:
             self._ck(self.L.hs_dsmc_kernel_times(self.h, None, None, -1), "kernel_times reset")
         return {k: (float(ms[i]), int(nl[i])) for i, k in enumerate(STAGES)}

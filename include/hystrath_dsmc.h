@@ -299,8 +299,8 @@ int hs_dsmc_read_face_flux(hs_dsmc_engine* eng, int64_t* out, int64_t n, int64_t
 int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n);
 int hs_dsmc_reset_counters(hs_dsmc_engine* eng);
 /* per-kernel device time (ms, summed over launches since last reset) and
- * launch counts, in the order inject, move, scan, scatter, collide;
- * requires HS_FLAG_TIMING */
+ * launch counts, in the order inject, move, scan, scatter, collide, gather
+ * (the stable cell-order gather, before collide); requires HS_FLAG_TIMING */
 int hs_dsmc_kernel_times(hs_dsmc_engine* eng, double* ms, int64_t* launches, int32_t n);
 
 /* ---- calibration -------------------------------------------------------- */
