@@ -295,7 +295,9 @@ struct MoveCounts {
     unsigned long long del, wall, ovf;
 };
 
-__device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is_new, uint32_t step, MoveCounts& mc) {
+// Returns the parcel's new cell (-1: deleted or not in the domain).
+__device__ int32_t move_one(const DevParams& P, DevParticles& b, int64_t i, bool is_new, uint32_t step,
+                            MoveCounts& mc) {
     PhiloxStream r;
     r.init(P, RNG_MOVE, (uint32_t)i, step, 0);
     double t = P.dt;
@@ -304,7 +306,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
         t = (1.0 - sf) * P.dt;
     }
     const int32_t c = b.cell[i];
-    if (c < 0) return;
+    if (c < 0) return -1;
     double p[3] = { b.px[i], b.py[i], b.pz[i] };
     double U[3] = { (double)b.ux[i], (double)b.uy[i], (double)b.uz[i] };
     int idx[3] = { c % P.nx, (c / P.nx) % P.ny, c / (P.nx * P.ny) };
@@ -318,6 +320,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
     const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
     const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
     bool keep = true;
+    bool vch = false;  // the velocity changed (a wall interaction)
     int iter = 0;
     while (keep && t > HS_ROOTVSMALL) {
         if (++iter > HS_MAX_TRACK) { mc.ovf += 1; break; }
@@ -376,6 +379,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
             const int32_t f = measure ? wf : -1;
             double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
             if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+            vch = true;
             if (P.cyl_model == HS_PATCH_DIFFUSE) {
                 diffuse_reflect(P, P.cyl_T, P.cyl_U, nw, t_id, U, erot, vib, P.capacity, r);
             } else {
@@ -425,6 +429,7 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
             }
             double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
             if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+            vch = true;
             if (model == HS_PATCH_DIFFUSE)
                 diffuse_reflect(P, P.patch_T[side], P.patch_U[side], nw, t_id, U, erot, vib, P.capacity, r);
             else if (U[axis] * (double)dir > 0.0)
@@ -439,9 +444,11 @@ __device__ void move_one(const DevParams& P, DevParticles& b, int64_t i, bool is
         }
     }
     b.px[i] = p[0]; b.py[i] = p[1]; b.pz[i] = p[2];
-    b.ux[i] = (float)U[0]; b.uy[i] = (float)U[1]; b.uz[i] = (float)U[2];
-    if (!keep) { b.cell[i] = -1; mc.del += 1; }
-    else b.cell[i] = idx[0] + P.nx * (idx[1] + P.ny * idx[2]);
+    if (vch) { b.ux[i] = (float)U[0]; b.uy[i] = (float)U[1]; b.uz[i] = (float)U[2]; }
+    const int32_t cn = keep ? idx[0] + P.nx * (idx[1] + P.ny * idx[2]) : -1;
+    if (!keep) mc.del += 1;
+    b.cell[i] = cn;
+    return cn;
 }
 
 // Histogram slot of each particle.  Particles are nearly cell-sorted, so the
@@ -478,9 +485,9 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
          base += (int64_t)gridDim.x * BLOCK) {
         const int64_t i = base + lane;
         const bool valid = i < n;
-        if (valid && do_move) move_one(P, b, i, i >= n_old, step, mc);
+        int32_t c = -1;
+        if (valid) c = do_move ? move_one(P, b, i, i >= n_old, step, mc) : b.cell[i];
         if (do_hist) {
-            int32_t c = valid ? b.cell[i] : -1;
             if (valid && c >= P.nc) { b.cell[i] = -1; c = -1; }
             hist_slots(c, valid, i, lane, count, slot);
         }
@@ -796,14 +803,16 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
 }
 
 // dsmcVolFields::calculateField per-cell sums for one cell: per species, the
-// parcels of the cell in order (dsmcVolFields.C:1115-1239)
+// parcels of the cell in order (dsmcVolFields.C:1115-1239).  Each cell has
+// one writer; the cumulative sums take non-returning atomic adds (the same
+// IEEE addition as a read-add-write, without waiting for the read).
 template <bool GEN>
 __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, int nC, const float* vx,
                                             const float* vy, const float* vz, const float* er, const uint16_t* vb,
                                             int vstride, const uint8_t* ty, double* samples, int nc, double ncoll,
                                             double sep) {
-    samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
-    samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
+    unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
+    unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
     const int ns = GEN ? P.ns : 1;
     const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
     for (int s = 0; s < ns; ++s) {
@@ -825,13 +834,13 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
             for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
         }
         const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
-        samples[(base + HS_SS_N) * nc + c] += aN;
-        samples[(base + HS_SS_MUX) * nc + c] += aX;
-        samples[(base + HS_SS_MUY) * nc + c] += aY;
-        samples[(base + HS_SS_MUZ) * nc + c] += aZ;
-        samples[(base + HS_SS_MCC) * nc + c] += aC;
-        if (GEN && P.has_erot) samples[(base + HS_SS_EROT) * nc + c] += aR;
-        for (int mo = 0; mo < nvib; ++mo) samples[(base + HS_SS_EVIB0 + mo) * nc + c] += ev[mo];
+        unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
+        if (GEN && P.has_erot) unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
+        for (int mo = 0; mo < nvib; ++mo) unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
         if (hf) {
             double h[HS_HF_COUNT];
             for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
@@ -850,7 +859,7 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
                 h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
             }
             const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
-            for (int q = 0; q < HS_HF_COUNT; ++q) samples[(hb + q) * nc + c] += h[q];
+            for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
         }
     }
 }
@@ -914,6 +923,46 @@ __device__ __forceinline__ void subcell_rank(const uint8_t* sub, int ss, int ee,
     }
 }
 
+// Sub-cell member lists of one cell (ss, nC) by a whole wave: counts and
+// ascending fills from ballots over 64-parcel windows; tab[q] gets
+// (offset | count << 16) of sub-cell q.
+__device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t* slist, int ss, int nC,
+                                                   unsigned long long* tab) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
+    int myc = 0;
+    for (int base = 0; base < nC; base += WAVE) {
+        const int j = base + lane;
+        const int q = j < nC ? (int)sub[ss + j] : 8;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const int pc = __popcll(__ballot(q == qq));
+            if (lane == qq) myc += pc;
+        }
+    }
+    int run[8];
+    {
+        int acc = 0;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const int cq = __shfl(myc, qq, WAVE);
+            run[qq] = acc;
+            if (lane == qq) tab[qq] = (unsigned long long)acc | ((unsigned long long)cq << 16);
+            acc += cq;
+        }
+    }
+    for (int base = 0; base < nC; base += WAVE) {
+        const int j = base + lane;
+        const int q = j < nC ? (int)sub[ss + j] : 8;
+#pragma unroll
+        for (int qq = 0; qq < 8; ++qq) {
+            const unsigned long long m = __ballot(q == qq);
+            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (uint16_t)j;
+            run[qq] += __popcll(m);
+        }
+    }
+}
+
 template <bool GEN, int PMAX>
 struct TileLds {
     alignas(16) int32_t key[PMAX];  // old index of each slot; then the NTC conflict stamps
@@ -931,6 +980,8 @@ struct TileLds {
     int32_t ncand[CT_CELLS];
     int32_t dense[CT_CELLS];
     int32_t ndense;
+    int32_t big[CT_CELLS];   // cells of more than 64 parcels
+    int32_t nbig;
     // candidate rounds of the sparse cells
     int32_t rk[CT_CELLS];    // next candidate of each cell
     int32_t rn[CT_CELLS];    // candidates handled by the rounds (0 for dense cells)
@@ -1208,7 +1259,10 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
     const int wid = t / WAVE, lane = t & (WAVE - 1);
     const int64_t cap = P.capacity;
     if (t < ncell) L.segS[t] = start[c0 + t];
-    if (t == 0) L.segS[ncell] = start[c0 + ncell];
+    if (t == 0) {
+        L.segS[ncell] = start[c0 + ncell];
+        L.nbig = 0;
+    }
     __syncthreads();
     int a = 0;
     while (a < ncell) {
@@ -1234,6 +1288,7 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
             for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
 #pragma unroll
             for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
+            if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
         }
 #pragma unroll 4
         for (int j = t; j < cnt; j += CT) {
@@ -1254,17 +1309,14 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         for (int j = t; j < cnt; j += CT) {
             const int l = L.loc[j];
             const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
-            const int q = L.sub[j];
-            if (ee - ss <= 64) {
-                atomicOr(&L.smask[l][q], 1ull << (j - ss));
-            } else {
-                int before, tot, rnk;
-                subcell_rank(L.sub, ss, ee, j, q, before, tot, rnk);
-                L.slist[ss + before + rnk] = (uint16_t)(j - ss);
-                if (rnk == 0) L.smask[l][q] = (unsigned long long)before | ((unsigned long long)tot << 16);
-            }
+            if (ee - ss <= 64) atomicOr(&L.smask[l][L.sub[j]], 1ull << (j - ss));
             L.key[j] = 0x7FFFFFFF;
         }
+        // cells of more than 64 parcels: one wave each builds the member
+        // lists with ballots over 64-parcel windows
+        for (int bi = wid; bi < L.nbig; bi += CT / WAVE)
+            subcell_lists_wave(L.sub, L.slist, L.segS[L.big[bi]] - S0,
+                               L.segS[L.big[bi] + 1] - L.segS[L.big[bi]], L.smask[L.big[bi]]);
         // candidates per cell (noTimeCounter.C:142-153)
         if (t >= a && t < b) {
             const int c = c0 + t;
@@ -1322,6 +1374,7 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
                     for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
             }
         }
+        if (t == 0) L.nbig = 0;
         __syncthreads();
         a = b;
     }
@@ -1390,7 +1443,7 @@ void launch_tiles(const int32_t* start, int nc, int pt, int64_t capacity, int32_
 }
 
 #define PMAX_MONO 2304
-#define PMAX_GEN 1536
+#define PMAX_GEN 1520
 
 void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
                    const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
