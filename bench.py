@@ -38,8 +38,9 @@ STAGE_BYTES = {
     # read perm + pos + vel, write sorted pos + vel + cell + sub-cell byte; per cell: start
     "gather": dict(per_particle=81.0, per_cell=4.0),
     # read sorted vel + sub-cell byte, write vel; per cell: start, sigmaTcRMax
-    # and remainder (read+write), volume, 11 sample sums (read+write)
-    "collide": dict(per_particle=25.0, per_cell=4 + 16 + 16 + 8 + 11 * 16),
+    # and remainder (read+write), volume and the 7 sample sums of one species
+    # without internal modes (nColls, collision separation, N, m U, m|U|^2), read+write
+    "collide": dict(per_particle=25.0, per_cell=4 + 16 + 16 + 8 + 7 * 16),
     # per inserted particle: 40 B written (counted per particle moved: small)
     "inject": dict(per_particle=0.0, per_cell=0.0),
 }
