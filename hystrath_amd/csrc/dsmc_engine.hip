@@ -56,7 +56,7 @@ struct hs_dsmc_engine {
     int32_t* d_tile_begin = nullptr;
     int32_t* d_tile_order = nullptr;   // collide workgroup -> tile, heavy tiles first
     int32_t* d_tile_heavy = nullptr;
-    int32_t* d_order_ctr = nullptr;
+    WallQueue wq = { nullptr, nullptr, nullptr };  // flights stopped at walls
     int tile_parcels = 1536;  // parcels per collide workgroup (particle-balanced tiles)
     DevFaces faces;
     int32_t* d_face_off = nullptr;
@@ -475,8 +475,10 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         (rc = dev_alloc(eng, &eng->d_slot, eng->cap)) || (rc = dev_alloc(eng, &eng->d_perm, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_big_list, eng->cap)) || (rc = dev_alloc(eng, &eng->d_big_sub, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_tile_begin, eng->cap / 64 + 4)) ||
-        (rc = dev_alloc(eng, &eng->d_tile_order, eng->cap / 64 + 4)) || (rc = dev_alloc(eng, &eng->d_order_ctr, 2)) ||
+        (rc = dev_alloc(eng, &eng->d_tile_order, eng->cap / 64 + 4)) || 
         (rc = dev_alloc(eng, &eng->d_tile_heavy, eng->cap / 64 + 4)) ||
+        (rc = dev_alloc(eng, &eng->wq.i, eng->cap)) || (rc = dev_alloc(eng, &eng->wq.t, eng->cap)) ||
+        (rc = dev_alloc(eng, &eng->wq.code_iter, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_S, 1)))
         return bail(rc);
     // inflow faces
@@ -576,6 +578,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     if (hipMemcpy(eng->d_S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "hipMemcpy of scalars failed"));
     launch_fill_i32(eng->d_count, 0, nc, eng->stream);
+    launch_fill_i32(eng->d_tile_heavy, 0, eng->cap / 64 + 4, eng->stream);
     launch_fill_f64(eng->d_sigma, 0.0, nc, eng->stream);
     launch_fill_f64(eng->d_rem, 0.0, nc, eng->stream);
     launch_fill_f64(eng->d_samples, 0.0, (int64_t)eng->nslots * nc, eng->stream);
@@ -804,7 +807,7 @@ static void enqueue_inject(hs_dsmc_engine* eng) {
 
 static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
     StageTimer tm(eng, T_MOVE);
-    launch_move(eng->dp, cur_buf(eng), eng->d_S, eng->d_count, eng->d_slot, do_move, do_hist, eng->move_grid,
+    launch_move(eng->dp, cur_buf(eng), eng->d_S, eng->d_count, eng->d_slot, eng->wq, do_move, do_hist, eng->move_grid,
                 eng->stream);
 }
 
@@ -819,15 +822,13 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     {
         StageTimer tm(eng, T_GATHER);
-        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->cap, eng->d_tile_begin, eng->d_tile_heavy,
-                     eng->stream);
+        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->d_tile_heavy, eng->stream);
         launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
                       eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
     }
     if (do_collide || do_sample) {
         StageTimer tm(eng, T_COLLIDE);
         launch_collide(eng->dp, nxt_buf(eng), eng->d_start, eng->d_tile_begin, eng->d_tile_heavy, eng->d_tile_order,
-                       eng->d_order_ctr,
                        eng->tile_parcels, eng->cap, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
                        eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc, eng->d_big_list,
                        eng->stream);

@@ -28,6 +28,9 @@ struct DevScalars {
     uint32_t step;    // time-step index (random-stream counter)
     int32_t total;    // scratch: last scan total
     int32_t n_sorted; // population after the cell scan (published by commit)
+    int32_t ord[2];   // heavy / light tile counters of the collide order (cleared by commit)
+    int32_t qn;       // flights queued for the wall pass of the move (cleared by that pass)
+    int32_t qdone;    // workgroups of the wall pass that finished
     unsigned long long counters[CNT_N];
 };
 
@@ -49,22 +52,29 @@ void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, Scan
 void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st);
 void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
                        hipStream_t st);
+// Flights of the move stopped at a wall, for its wall pass: parcel index,
+// flight time left, wall code | tracking iterations << 8 (capacity entries).
+struct WallQueue {
+    int32_t* i;
+    double* t;
+    int32_t* code_iter;
+};
+// move (+ the flights queued at walls) and/or the cell histogram
 void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
-                 int do_move, int do_hist, int grid, hipStream_t st);
+                 const WallQueue& Q, int do_move, int do_hist, int grid, hipStream_t st);
 void launch_set_nold(DevScalars* S, hipStream_t st);
 void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
                     int32_t* perm, int grid, hipStream_t st);
 // particle-balanced tiles of pt parcels (tile_begin) and their heavy flags
-void launch_tiles(const int32_t* start, int nc, int pt, int64_t capacity, int32_t* tile_begin, int32_t* heavy,
-                  hipStream_t st);
+// (clear before the first use; the collide order clears the flags it reads)
+void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, int32_t* heavy, hipStream_t st);
 // stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel
 void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
                    const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
                    int general, hipStream_t st);
 // NTC, collisions and sampling on the sorted population in dst
 void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
-                    const int32_t* heavy, int32_t* order, int32_t* order_ctr, int pt, int64_t capacity, uint8_t* sub,
-                    double* sigma,
+                    int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, hipStream_t st);

@@ -295,35 +295,91 @@ struct MoveCounts {
     unsigned long long del, wall, ovf;
 };
 
-// Returns the parcel's new cell (-1: deleted or not in the domain).
-__device__ int32_t move_one(const DevParams& P, DevParticles& b, int64_t i, bool is_new, uint32_t step,
-                            MoveCounts& mc) {
-    PhiloxStream r;
-    r.init(P, RNG_MOVE, (uint32_t)i, step, 0);
-    double t = P.dt;
-    if (is_new) {
-        const double sf = r.u();
-        t = (1.0 - sf) * P.dt;
+// State of one parcel's flight through the block.
+struct Flight {
+    double p[3], U[3];
+    int idx[3];
+    double t;      // flight time left
+    int iter;      // tracking iterations so far
+    bool keep;     // false: deleted
+    bool vch;      // the velocity changed (a wall interaction)
+};
+
+// Wall interaction at p (code: side 0-5 of the block, 6 = the cylinder):
+// dsmcDiffuseWallPatch / dsmcSpecularWallPatch with the boundaryMeasurements
+// and face-tracker terms.
+__device__ void wall_hit(const DevParams& P, Flight& s, int code, int t_id, float* erot, uint16_t* vib,
+                         PhiloxStream& r, bool measure, bool track, MoveCounts& mc) {
+    mc.wall += 1;
+    s.vch = true;
+    double* U = s.U;
+    const int* idx = s.idx;
+    if (code == 6) {
+        const double nx_ = P.cyl_c[0] - s.p[0], ny_ = P.cyl_c[1] - s.p[1];
+        const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
+        const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
+        const int32_t wf = (measure || track) ? P.cyl_face[idx[0] + P.nx * (idx[1] + P.ny * idx[2])] : -1;
+        const int32_t f = measure ? wf : -1;
+        double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
+        if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+        if (P.cyl_model == HS_PATCH_DIFFUSE) {
+            diffuse_reflect(P, P.cyl_T, P.cyl_U, nw, t_id, U, erot, vib, P.capacity, r);
+        } else {
+            const double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
+            if (und > 0.0)
+                for (int q = 0; q < 3; ++q) U[q] = U[q] - 2.0 * und * nw[q];
+        }
+        if (f >= 0) {
+            const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+            wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
+        }
+        if (track && wf >= 0)
+            track_face(P, t_id, P.face_off[3] + (wf - P.cyl_first),
+                       (U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2]) >= 0.0 ? 1 : -1);
+        return;
     }
-    const int32_t c = b.cell[i];
-    if (c < 0) return -1;
-    double p[3] = { b.px[i], b.py[i], b.pz[i] };
-    double U[3] = { (double)b.ux[i], (double)b.uy[i], (double)b.uz[i] };
-    int idx[3] = { c % P.nx, (c / P.nx) % P.ny, c / (P.nx * P.ny) };
+    const int axis = code / 2, dir = (code & 1) ? 1 : -1;
+    const int model = P.patch_model[code];
+    double nw[3] = { 0.0, 0.0, 0.0 };
+    nw[axis] = (double)dir;
+    int64_t f = -1;
+    if (measure && P.wall_off[code] >= 0) {
+        const int j1 = axis == 0 ? idx[1] : idx[0], j2 = axis == 2 ? idx[1] : idx[2];
+        const int n1 = axis == 0 ? P.ny : P.nx;
+        f = P.wall_off[code] + j1 + (int64_t)n1 * j2;
+    }
+    double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
+    if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+    if (model == HS_PATCH_DIFFUSE)
+        diffuse_reflect(P, P.patch_T[code], P.patch_U[code], nw, t_id, U, erot, vib, P.capacity, r);
+    else if (U[axis] * (double)dir > 0.0)
+        U[axis] = -U[axis];
+    if (f >= 0) {
+        const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
+        wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
+    }
+    if (track) {
+        const int nn_a = axis == 0 ? P.nx : (axis == 1 ? P.ny : P.nz);
+        track_face(P, t_id, mesh_face(P, axis, dir > 0 ? nn_a : 0, idx), U[axis] * (double)dir >= 0.0 ? 1 : -1);
+    }
+}
+
+// Flight of one parcel (dsmcParcel::move, dsmcParcel.C:38-148, on the
+// Cartesian block): face crossings, cyclic and deletion sides, walls.
+// DEFER: stops at the first wall whose interaction draws random numbers or
+// takes wall sums (a diffuse side, the cylinder, a specular side while
+// sampling) with p on the wall and returns its code (side 0-5, 6 cylinder);
+// otherwise every wall is handled here.  Returns -1 when the flight is over.
+template <bool DEFER>
+__device__ int fly(const DevParams& P, Flight& s, int t_id, float* erot, uint16_t* vib, PhiloxStream& r,
+                   bool measure, bool track, MoveCounts& mc) {
     const int nn[3] = { P.nx, P.ny, P.nz };
-    const int t_id = b.type ? (int)b.type[i] : 0;
-    float* erot = b.erot ? &b.erot[i] : nullptr;
-    uint16_t* vib = (b.vib && P.nvib[t_id] > 0) ? &b.vib[i] : nullptr;
-#pragma unroll
-    for (int a = 0; a < 3; ++a)
-        if (P.empty[a]) p[a] = P.centre[a];
-    const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
-    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
-    bool keep = true;
-    bool vch = false;  // the velocity changed (a wall interaction)
-    int iter = 0;
-    while (keep && t > HS_ROOTVSMALL) {
-        if (++iter > HS_MAX_TRACK) { mc.ovf += 1; break; }
+    double* p = s.p;
+    double* U = s.U;
+    int* idx = s.idx;
+    while (s.keep && s.t > HS_ROOTVSMALL) {
+        if (++s.iter > HS_MAX_TRACK) { mc.ovf += 1; break; }
+        const double t = s.t;
         double ut[3], e[3];
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -364,36 +420,15 @@ __device__ int32_t move_one(const DevParams& P, DevParticles& b, int64_t i, bool
         }
         if (!wall && axis < 0) {
             p[0] = e[0]; p[1] = e[1]; p[2] = e[2];
-            t = 0.0;
+            s.t = 0.0;
             break;
         }
 #pragma unroll
         for (int q = 0; q < 3; ++q) p[q] = p[q] + lam * (e[q] - p[q]);
-        t = t - t * lam;
+        s.t = t - t * lam;
         if (wall) {
-            mc.wall += 1;
-            const double nx_ = P.cyl_c[0] - p[0], ny_ = P.cyl_c[1] - p[1];
-            const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
-            const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
-            const int32_t wf = (measure || track) ? P.cyl_face[idx[0] + P.nx * (idx[1] + P.ny * idx[2])] : -1;
-            const int32_t f = measure ? wf : -1;
-            double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
-            if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
-            vch = true;
-            if (P.cyl_model == HS_PATCH_DIFFUSE) {
-                diffuse_reflect(P, P.cyl_T, P.cyl_U, nw, t_id, U, erot, vib, P.capacity, r);
-            } else {
-                const double und = U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2];
-                if (und > 0.0)
-                    for (int q = 0; q < 3; ++q) U[q] = U[q] - 2.0 * und * nw[q];
-            }
-            if (f >= 0) {
-                const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
-                wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
-            }
-            if (track && wf >= 0)
-                track_face(P, t_id, P.face_off[3] + (wf - P.cyl_first),
-                           (U[0] * nw[0] + U[1] * nw[1] + U[2] * nw[2]) >= 0.0 ? 1 : -1);
+            if (DEFER) return 6;
+            wall_hit(P, s, 6, t_id, erot, vib, r, measure, track, mc);
             continue;
         }
         p[axis] = face_coord(P, axis, dir > 0 ? idx[axis] + 1 : idx[axis]);
@@ -415,40 +450,57 @@ __device__ int32_t move_one(const DevParams& P, DevParticles& b, int64_t i, bool
             if (track)
                 track_face(P, t_id, mesh_face(P, axis, dir > 0 ? nn[axis] : 0, idx),
                            U[axis] * (double)dir >= 0.0 ? 1 : -1);
-            keep = false;
-        } else {
-            // diffuse or specular wall (an empty plane is never crossed)
+            s.keep = false;
+        } else if (DEFER && (model == HS_PATCH_DIFFUSE || measure)) {
+            return side;
+        } else if (DEFER) {
+            // specular side without wall sums (an empty plane is never crossed)
             mc.wall += 1;
-            double nw[3] = { 0.0, 0.0, 0.0 };
-            nw[axis] = (double)dir;
-            int64_t f = -1;
-            if (measure && P.wall_off[side] >= 0) {
-                const int j1 = axis == 0 ? idx[1] : idx[0], j2 = axis == 2 ? idx[1] : idx[2];
-                const int n1 = axis == 0 ? P.ny : P.nx;
-                f = P.wall_off[side] + j1 + (int64_t)n1 * j2;
-            }
-            double e_pre = 0.0, u_pre[3] = { U[0], U[1], U[2] };
-            if (f >= 0) e_pre = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
-            vch = true;
-            if (model == HS_PATCH_DIFFUSE)
-                diffuse_reflect(P, P.patch_T[side], P.patch_U[side], nw, t_id, U, erot, vib, P.capacity, r);
-            else if (U[axis] * (double)dir > 0.0)
-                U[axis] = -U[axis];
-            if (f >= 0) {
-                const double e_post = wall_side_terms(P, f, t_id, nw, U, erot, vib, P.capacity);
-                wall_exchange_terms(P, f, t_id, e_pre, u_pre, e_post, U);
-            }
+            s.vch = true;
+            if (U[axis] * (double)dir > 0.0) U[axis] = -U[axis];
             if (track)
                 track_face(P, t_id, mesh_face(P, axis, dir > 0 ? nn[axis] : 0, idx),
                            U[axis] * (double)dir >= 0.0 ? 1 : -1);
+        } else {
+            wall_hit(P, s, side, t_id, erot, vib, r, measure, track, mc);
         }
     }
-    b.px[i] = p[0]; b.py[i] = p[1]; b.pz[i] = p[2];
-    if (vch) { b.ux[i] = (float)U[0]; b.uy[i] = (float)U[1]; b.uz[i] = (float)U[2]; }
-    const int32_t cn = keep ? idx[0] + P.nx * (idx[1] + P.ny * idx[2]) : -1;
-    if (!keep) mc.del += 1;
+    return -1;
+}
+
+// End of a flight: the new state; returns the new cell (-1: deleted).
+__device__ __forceinline__ int32_t land(const DevParams& P, DevParticles& b, int64_t i, const Flight& s,
+                                        MoveCounts& mc) {
+    b.px[i] = s.p[0]; b.py[i] = s.p[1]; b.pz[i] = s.p[2];
+    if (s.vch) { b.ux[i] = (float)s.U[0]; b.uy[i] = (float)s.U[1]; b.uz[i] = (float)s.U[2]; }
+    const int32_t cn = s.keep ? s.idx[0] + P.nx * (s.idx[1] + P.ny * s.idx[2]) : -1;
+    if (!s.keep) mc.del += 1;
     b.cell[i] = cn;
     return cn;
+}
+
+__device__ __forceinline__ void load_flight(const DevParams& P, const DevParticles& b, int64_t i, int32_t c,
+                                            Flight& s) {
+    s.p[0] = b.px[i]; s.p[1] = b.py[i]; s.p[2] = b.pz[i];
+    s.U[0] = (double)b.ux[i]; s.U[1] = (double)b.uy[i]; s.U[2] = (double)b.uz[i];
+    s.idx[0] = c % P.nx;
+    s.idx[1] = (c / P.nx) % P.ny;
+    s.idx[2] = c / (P.nx * P.ny);
+    s.keep = true;
+    s.vch = false;
+}
+
+// The move stream of parcel i: its draws in order are the step fraction of a
+// new parcel (dsmcParcel.C:52-59), then the wall interactions.
+__device__ __forceinline__ void move_stream(const DevParams& P, PhiloxStream& r, int64_t i, uint32_t step,
+                                            bool is_new, double* t0) {
+    r.init(P, RNG_MOVE, (uint32_t)i, step, 0);
+    double t = P.dt;
+    if (is_new) {
+        const double sf = r.u();
+        t = (1.0 - sf) * P.dt;
+    }
+    *t0 = t;
 }
 
 // Histogram slot of each particle.  Particles are nearly cell-sorted, so the
@@ -472,13 +524,20 @@ __device__ __forceinline__ void hist_slots(int32_t c, bool valid, int64_t i, int
     if (valid && c >= 0) slot[i] = base + (lane - leader);
 }
 
+// move (Cloud<dsmcParcel>::move) with the cell histogram of
+// buildCellOccupancy fused in.  Flights that reach a wall needing random
+// draws or wall sums are queued for k_move_wall, which keeps this kernel's
+// registers (and so its occupancy) those of the free flight.
 __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
                                                 DevScalars* __restrict__ S, int32_t* __restrict__ count,
-                                                int32_t* __restrict__ slot, int do_move, int do_hist) {
+                                                int32_t* __restrict__ slot, WallQueue Q, int do_move, int do_hist) {
     const DevParams& P = *Pp;
     const int64_t n = S->n, n_old = S->n_old;
     const uint32_t step = S->step;
     const int lane = threadIdx.x & (WAVE - 1);
+    const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
+    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
+    const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
     MoveCounts mc = { 0, 0, 0 };
     // wave-uniform trip count: every lane of a wave sees the same bases
     for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < n;
@@ -486,7 +545,45 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
         const int64_t i = base + lane;
         const bool valid = i < n;
         int32_t c = -1;
-        if (valid) c = do_move ? move_one(P, b, i, i >= n_old, step, mc) : b.cell[i];
+        int code = -1;
+        Flight s;
+        if (valid) {
+            c = b.cell[i];
+            if (do_move && c >= 0) {
+                PhiloxStream r;
+                move_stream(P, r, i, step, i >= n_old, &s.t);
+                load_flight(P, b, i, c, s);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+                    if (P.empty[a]) s.p[a] = P.centre[a];
+                s.iter = 0;
+                const int t_id = b.type ? (int)b.type[i] : 0;
+                code = fly<true>(P, s, t_id, nullptr, nullptr, r, measure, track, mc);
+                if (code >= 0) {
+                    // stopped at a wall: position on it, cell unchanged until the wall pass
+                    b.px[i] = s.p[0]; b.py[i] = s.p[1]; b.pz[i] = s.p[2];
+                    const int32_t cw = s.idx[0] + P.nx * (s.idx[1] + P.ny * s.idx[2]);
+                    b.cell[i] = cw;
+                    c = -1;
+                } else {
+                    c = land(P, b, i, s, mc);
+                }
+            }
+        }
+        if (do_move) {
+            const unsigned long long dm = __ballot(code >= 0);
+            if (dm) {
+                int qb = 0;
+                if (lane == __ffsll((long long)dm) - 1) qb = atomicAdd(&S->qn, __popcll(dm));
+                qb = __shfl(qb, __ffsll((long long)dm) - 1, WAVE);
+                if (code >= 0) {
+                    const int k = qb + __popcll(dm & lt);
+                    Q.i[k] = (int32_t)i;
+                    Q.t[k] = s.t;
+                    Q.code_iter[k] = code | (s.iter << 8);
+                }
+            }
+        }
         if (do_hist) {
             if (valid && c >= P.nc) { b.cell[i] = -1; c = -1; }
             hist_slots(c, valid, i, lane, count, slot);
@@ -498,9 +595,62 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
     add_counter(S, CNT_TRACK_OVF, mc.ovf);
 }
 
-void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot, int do_move,
-                 int do_hist, int grid, hipStream_t st) {
-    k_move<<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, do_move, do_hist);
+// The queued flights: the wall interaction they stopped at, the rest of the
+// flight (every further wall handled here), the cell histogram.  The last
+// workgroup to finish clears the queue for the next step.
+__global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict__ Pp, DevParticles b,
+                                                     DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                                     int32_t* __restrict__ slot, WallQueue Q, int do_hist) {
+    const DevParams& P = *Pp;
+    const int64_t nq = S->qn, n_old = S->n_old;
+    const uint32_t step = S->step;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
+    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
+    MoveCounts mc = { 0, 0, 0 };
+    for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < nq;
+         base += (int64_t)gridDim.x * BLOCK) {
+        const int64_t k = base + lane;
+        const bool valid = k < nq;
+        int32_t c = -1;
+        int64_t i = 0;
+        if (valid) {
+            i = Q.i[k];
+            const int ci = Q.code_iter[k];
+            Flight s;
+            PhiloxStream r;
+            double t0;
+            move_stream(P, r, i, step, i >= n_old, &t0);
+            load_flight(P, b, i, b.cell[i], s);
+            s.t = Q.t[k];
+            s.iter = ci >> 8;
+            const int t_id = b.type ? (int)b.type[i] : 0;
+            float* erot = b.erot ? &b.erot[i] : nullptr;
+            uint16_t* vib = (b.vib && P.nvib[t_id] > 0) ? &b.vib[i] : nullptr;
+            wall_hit(P, s, ci & 0xFF, t_id, erot, vib, r, measure, track, mc);
+            fly<false>(P, s, t_id, erot, vib, r, measure, track, mc);
+            c = land(P, b, i, s, mc);
+        }
+        if (do_hist) hist_slots(c, valid, i, lane, count, slot);
+    }
+    add_counter(S, CNT_DEL, mc.del);
+    add_counter(S, CNT_WALL, mc.wall);
+    add_counter(S, CNT_TRACK_OVF, mc.ovf);
+    // the last workgroup clears the queue
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&S->qdone, 1) == (int)gridDim.x - 1) {
+            S->qn = 0;
+            S->qdone = 0;
+        }
+    }
+}
+
+void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
+                 const WallQueue& Q, int do_move, int do_hist, int grid, hipStream_t st) {
+    k_move<<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_move, do_hist);
+    if (do_move) k_move_wall<<<grid / 8 > 0 ? grid / 8 : 1, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_hist);
 }
 
 __global__ void k_set_nold(DevScalars* S) { S->n_old = S->n; }
@@ -1399,14 +1549,17 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ sta
 // Order in which the collide workgroups take the tiles: heavy tiles first
 // (they run longest), the others after.  Tiles are independent, so the order
 // does not change any result.
-__global__ __launch_bounds__(BLOCK) void k_tile_order(const int32_t* __restrict__ heavy, int pt,
-                                                      const DevScalars* __restrict__ S, int32_t* __restrict__ ctr,
+__global__ __launch_bounds__(BLOCK) void k_tile_order(int32_t* __restrict__ heavy, int pt, DevScalars* __restrict__ S,
                                                       int32_t* __restrict__ order) {
     const int q = blockIdx.x * BLOCK + threadIdx.x;
     const int nt = (int)((int64_t)S->n_sorted / pt) + 1;
     if (q >= nt) return;
-    if (heavy[q]) order[atomicAdd(&ctr[0], 1)] = q;
-    else order[nt - 1 - atomicAdd(&ctr[1], 1)] = q;
+    if (heavy[q]) {
+        heavy[q] = 0;
+        order[atomicAdd(&S->ord[0], 1)] = q;
+    } else {
+        order[nt - 1 - atomicAdd(&S->ord[1], 1)] = q;
+    }
 }
 
 template <bool GEN, int PMAX>
@@ -1435,10 +1588,7 @@ __global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__
     add_counter(S, CNT_TILE_OVF, cx.ovf);
 }
 
-void launch_tiles(const int32_t* start, int nc, int pt, int64_t capacity, int32_t* tile_begin, int32_t* heavy,
-                  hipStream_t st) {
-    const int nt = (int)(capacity / pt) + 2;
-    k_fill_i32<<<grid_for(nt), BLOCK, 0, st>>>(heavy, 0, nt);
+void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, int32_t* heavy, hipStream_t st) {
     k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin, heavy);
 }
 
@@ -1456,13 +1606,11 @@ void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const
 }
 
 void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
-                    const int32_t* heavy, int32_t* order, int32_t* order_ctr, int pt, int64_t capacity, uint8_t* sub,
-                    double* sigma,
+                    int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st) {
     const int nt = (int)(capacity / pt) + 2;
-    k_fill_i32<<<1, 64, 0, st>>>(order_ctr, 0, 2);
-    k_tile_order<<<(nt + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(heavy, pt, S, order_ctr, order);
+    k_tile_order<<<(nt + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(heavy, pt, S, order);
     if (general)
         k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples,
                                                       vol, S, do_collide, do_sample, nc, big_list);
@@ -1475,6 +1623,8 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
 __global__ void k_commit(DevScalars* S, int advance) {
     S->n = S->n_sorted;
     S->n_old = S->n_sorted;
+    S->ord[0] = 0;
+    S->ord[1] = 0;
     if (advance) S->step += 1;
 }
 void launch_commit(DevScalars* S, int advance, hipStream_t st) { k_commit<<<1, 1, 0, st>>>(S, advance); }
