@@ -65,6 +65,21 @@ __device__ __forceinline__ void add_counter(DevScalars* S, int k, unsigned long 
 #define SCAN_ITEMS 8
 #define SCAN_TILE (BLOCK * SCAN_ITEMS)
 
+// publishes the scan total as k_scan_apply does
+__device__ __forceinline__ void scan_publish(int32_t* out, int32_t n, int total, int mode, DevScalars* S,
+                                             int64_t capacity) {
+    out[n] = total;
+    S->total = total;
+    if (mode == SCAN_CELLS) {
+        S->n_sorted = total;
+    } else if (mode == SCAN_APPEND) {
+        const int64_t nn = (int64_t)S->n + total;
+        S->n_old = S->n;
+        S->n = (int32_t)(nn > capacity ? capacity : nn);
+        if (nn > capacity) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - capacity));
+    }
+}
+
 __global__ __launch_bounds__(BLOCK) void k_scan_reduce(const int32_t* __restrict__ in, int32_t n,
                                                        int32_t* __restrict__ bsum) {
     const int base = blockIdx.x * SCAN_TILE;
@@ -116,23 +131,47 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(int32_t* __restrict__ in, 
         }
         ex += v[k];
     }
-    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) {
-        const int total = S->total;
-        out[n] = total;
-        if (mode == SCAN_CELLS) {
-            S->n_sorted = total;
-        } else if (mode == SCAN_APPEND) {
-            const int64_t nn = (int64_t)S->n + total;
-            S->n_old = S->n;
-            S->n = (int32_t)(nn > capacity ? capacity : nn);
-            if (nn > capacity) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - capacity));
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) scan_publish(out, n, S->total, mode, S, capacity);
+}
+
+// The whole scan in one workgroup (short arrays): tiles in order with a carry.
+__global__ __launch_bounds__(BLOCK) void k_scan_single(int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                       int32_t n, int mode, DevScalars* __restrict__ S,
+                                                       int64_t capacity) {
+    int carry = 0;
+    for (int tile = 0; tile < n; tile += SCAN_TILE) {
+        const int base = tile + threadIdx.x * SCAN_ITEMS;
+        int v[SCAN_ITEMS];
+        int s = 0;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int i = base + k;
+            v[k] = i < n ? in[i] : 0;
+            s += v[k];
         }
+        int tot;
+        int ex = block_excl_scan(s, &tot) + carry;
+#pragma unroll
+        for (int k = 0; k < SCAN_ITEMS; ++k) {
+            const int i = base + k;
+            if (i < n) {
+                out[i] = ex;
+                if (mode == SCAN_CELLS) in[i] = 0;
+            }
+            ex += v[k];
+        }
+        carry += tot;
     }
+    if (threadIdx.x == 0) scan_publish(out, n, carry, mode, S, capacity);
 }
 
 void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, ScanWs& ws, DevScalars* S,
                  int64_t capacity, hipStream_t st) {
     const int nb = n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1;
+    if (nb <= 8) {
+        k_scan_single<<<1, BLOCK, 0, st>>>(const_cast<int32_t*>(in), out, n, (int)mode, S, capacity);
+        return;
+    }
     k_scan_reduce<<<nb, BLOCK, 0, st>>>(in, n, ws.bsum);
     k_scan_blocks<<<1, BLOCK, 0, st>>>(ws.bsum, nb, S);
     k_scan_apply<<<nb, BLOCK, 0, st>>>(const_cast<int32_t*>(in), out, n, ws.bsum, (int)mode, S, capacity);
@@ -205,17 +244,18 @@ __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restr
 
 // Parcel j of this step belongs to the entry e (face*ns + species) with
 // off[e] <= j < off[e+1] and is its (j-off[e])-th; it is written at idx.
-__device__ void inject_one(const DevParams& P, const DevFaces& F, int ns, DevParticles& b, const DevScalars* S, int j,
-                           int64_t idx) {
+// off: the entries' offsets (LDS copy or global).
+__device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t* off, int ns, DevParticles& b,
+                           const DevScalars* S, int j, int64_t idx) {
     const int ne = F.n * ns;
     int lo = 0, hi = ne;
     while (hi - lo > 1) {
         const int mid = (lo + hi) >> 1;
-        if (F.off[mid] <= j) lo = mid;
+        if (off[mid] <= j) lo = mid;
         else hi = mid;
     }
     const int e = lo;
-    const int k = j - F.off[e];
+    const int k = j - off[e];
     const int f = e / ns, s = e - f * ns;
     PhiloxStream r;
     r.init(P, RNG_INJECT, (uint32_t)e, S->step, 1u + (uint32_t)k);
@@ -262,15 +302,24 @@ __device__ void inject_one(const DevParams& P, const DevFaces& F, int ns, DevPar
     new_internal(P, b, idx, s, P.inflow_trot, P.inflow_tvib, r);
 }
 
-// One thread per new parcel.
+// One thread per new parcel; the entry offsets are searched in an LDS copy
+// when they fit.
+#define INJ_LDS_OFF 4096
 __global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
                                                       DevParticles b, DevScalars* __restrict__ S) {
+    __shared__ int32_t soff[INJ_LDS_OFF];
     const DevParams& P = *Pp;
     const int ne = F.n * ns;
+    const bool lds = ne + 1 <= INJ_LDS_OFF;
+    if (lds) {
+        for (int q = threadIdx.x; q <= ne; q += BLOCK) soff[q] = F.off[q];
+        __syncthreads();
+    }
+    const int32_t* off = lds ? soff : F.off;
     const int j = blockIdx.x * BLOCK + threadIdx.x;
     const int64_t idx = (int64_t)S->n_old + j;
-    const bool mine = j < F.off[ne] && idx < P.capacity;
-    if (mine) inject_one(P, F, ns, b, S, j, idx);
+    const bool mine = j < off[ne] && idx < P.capacity;
+    if (mine) inject_one(P, F, off, ns, b, S, j, idx);
     // every lane reaches the wave reduction of the counter
     add_counter(S, CNT_INJ, mine ? 1ull : 0ull);
 }
