@@ -70,7 +70,9 @@ struct hs_dsmc_engine {
     DevScalars* d_S = nullptr;
     ScanWs ws;
     int64_t n_sampled = 0;
-    int move_grid = 1024;
+    int move_grid = 1024;     // grid-stride launches over the parcels (scatter, hist-only move)
+    int flight_grid[2] = { 1024, 1024 };  // the moving launch, without / with the face tracker:
+                                          // as many blocks as fit on the device at once
     std::string err;
     // timing
     std::vector<TimedPair> timed;
@@ -586,6 +588,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, eng->device) == hipSuccess) ncu = prop.multiProcessorCount;
     eng->move_grid = ncu * 8;
+    for (int tr = 0; tr < 2; ++tr) eng->flight_grid[tr] = ncu * move_blocks_per_cu(tr);
     if (check_launch(eng, "init fills") || hipStreamSynchronize(eng->stream) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "initialisation failed"));
     *out = eng;
@@ -807,8 +810,9 @@ static void enqueue_inject(hs_dsmc_engine* eng) {
 
 static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
     StageTimer tm(eng, T_MOVE);
-    launch_move(eng->dp, cur_buf(eng), eng->d_S, eng->d_count, eng->d_slot, eng->wq, do_move, do_hist, eng->move_grid,
-                eng->stream);
+    const int track = (eng->cfg.flags & HS_FLAG_FACE_FLUX) && (eng->cfg.flags & HS_FLAG_SAMPLE);
+    launch_move(eng->dp, cur_buf(eng), eng->d_S, eng->d_count, eng->d_slot, eng->wq, do_move, do_hist, track,
+                do_move ? eng->flight_grid[track] : eng->move_grid, eng->stream);
 }
 
 static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance) {

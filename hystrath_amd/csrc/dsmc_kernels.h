@@ -61,8 +61,9 @@ struct WallQueue {
 };
 // move (+ the flights queued at walls) and/or the cell histogram
 void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
-                 const WallQueue& Q, int do_move, int do_hist, int grid, hipStream_t st);
+                 const WallQueue& Q, int do_move, int do_hist, int track, int grid, hipStream_t st);
 void launch_set_nold(DevScalars* S, hipStream_t st);
+int move_blocks_per_cu(int track);
 void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
                     int32_t* perm, int grid, hipStream_t st);
 // particle-balanced tiles of pt parcels (tile_begin) and their heavy flags

@@ -429,45 +429,66 @@ __device__ int fly(const DevParams& P, Flight& s, int t_id, float* erot, uint16_
     while (s.keep && s.t > HS_ROOTVSMALL) {
         if (++s.iter > HS_MAX_TRACK) { mc.ovf += 1; break; }
         const double t = s.t;
-        double ut[3], e[3];
+        // Which faces the straight flight to e crosses.  The fraction to the
+        // face ahead on axis a is l = num/den (clamped at 0) and the face is
+        // crossed when l < 1; for den > 0 that is exactly num < den (for den < 0,
+        // num > den), since a correctly rounded quotient of two positive
+        // doubles is below 1 exactly when the dividend is below the divisor.
+        // The divisions are taken only for the faces actually crossed.
+        double ut[3], e[3], den[3], num[3];
+        bool cr[3];
+        bool any = false;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             ut[a] = P.empty[a] ? 0.0 : U[a];
             e[a] = p[a] + t * ut[a];
+            den[a] = e[a] - p[a];
+            const bool up = ut[a] > 0.0 && den[a] > 0.0;
+            const bool dn = ut[a] < 0.0 && den[a] < 0.0;
+            num[a] = face_coord(P, a, up ? idx[a] + 1 : idx[a]) - p[a];
+            cr[a] = (up && num[a] < den[a]) || (dn && num[a] > den[a]);
+            any = any || cr[a];
+        }
+        // the cylinder: its hit fraction cc/qc (clamped at 0, qc > 0) is below 1
+        // exactly when cc < qc
+        bool hit = false;
+        double cc = 0.0, qc = 1.0;
+        if (P.cyl) {
+            const double dx = den[0], dy = den[1];
+            const double rx = p[0] - P.cyl_c[0], ry = p[1] - P.cyl_c[1];
+            const double bb = rx * dx + ry * dy;
+            if (bb < 0.0) {
+                const double aa = dx * dx + dy * dy;
+                cc = rx * rx + ry * ry - P.cyl_r * P.cyl_r;
+                const double disc = bb * bb - aa * cc;
+                if (disc >= 0.0) {
+                    qc = sqrt(disc) - bb;
+                    hit = cc < qc;
+                }
+            }
+        }
+        if (!any && !hit) {
+            p[0] = e[0]; p[1] = e[1]; p[2] = e[2];
+            s.t = 0.0;
+            break;
         }
         double lam = 1.0;
         int axis = -1, dir = 0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-            const double den = e[a] - p[a];
-            if (ut[a] > 0.0 && den > 0.0) {
-                double l = (face_coord(P, a, idx[a] + 1) - p[a]) / den;
+            if (cr[a]) {
+                double l = num[a] / den[a];
                 if (l < 0.0) l = 0.0;
-                if (l < lam) { lam = l; axis = a; dir = 1; }
-            } else if (ut[a] < 0.0 && den < 0.0) {
-                double l = (face_coord(P, a, idx[a]) - p[a]) / den;
-                if (l < 0.0) l = 0.0;
-                if (l < lam) { lam = l; axis = a; dir = -1; }
+                if (l < lam) { lam = l; axis = a; dir = ut[a] > 0.0 ? 1 : -1; }
             }
         }
         bool wall = false;
-        if (P.cyl) {
-            const double dx = e[0] - p[0], dy = e[1] - p[1];
-            const double rx = p[0] - P.cyl_c[0], ry = p[1] - P.cyl_c[1];
-            const double bb = rx * dx + ry * dy;
-            if (bb < 0.0) {
-                const double aa = dx * dx + dy * dy;
-                const double cc = rx * rx + ry * ry - P.cyl_r * P.cyl_r;
-                const double disc = bb * bb - aa * cc;
-                if (disc >= 0.0) {
-                    const double q = sqrt(disc) - bb;
-                    double sh = cc / q;
-                    if (sh < 0.0) sh = 0.0;
-                    if (sh < lam) { lam = sh; wall = true; }
-                }
-            }
+        if (hit) {
+            double sh = cc / qc;
+            if (sh < 0.0) sh = 0.0;
+            if (sh < lam) { lam = sh; wall = true; }
         }
-        if (!wall && axis < 0) {
+        if (!wall && axis < 0) {  // not reached: a flagged face or hit always sets one
             p[0] = e[0]; p[1] = e[1]; p[2] = e[2];
             s.t = 0.0;
             break;
@@ -528,10 +549,12 @@ __device__ __forceinline__ int32_t land(const DevParams& P, DevParticles& b, int
     return cn;
 }
 
-__device__ __forceinline__ void load_flight(const DevParams& P, const DevParticles& b, int64_t i, int32_t c,
-                                            Flight& s) {
+__device__ __forceinline__ void load_pu(const DevParticles& b, int64_t i, Flight& s) {
     s.p[0] = b.px[i]; s.p[1] = b.py[i]; s.p[2] = b.pz[i];
     s.U[0] = (double)b.ux[i]; s.U[1] = (double)b.uy[i]; s.U[2] = (double)b.uz[i];
+}
+
+__device__ __forceinline__ void set_cell(const DevParams& P, int32_t c, Flight& s) {
     s.idx[0] = c % P.nx;
     s.idx[1] = (c / P.nx) % P.ny;
     s.idx[2] = c / (P.nx * P.ny);
@@ -576,7 +599,10 @@ __device__ __forceinline__ void hist_slots(int32_t c, bool valid, int64_t i, int
 // move (Cloud<dsmcParcel>::move) with the cell histogram of
 // buildCellOccupancy fused in.  Flights that reach a wall needing random
 // draws or wall sums are queued for k_move_wall, which keeps this kernel's
-// registers (and so its occupancy) those of the free flight.
+// registers (and so its occupancy) those of the free flight.  TRACK: the face
+// tracker is on (HS_FLAG_FACE_FLUX with sampling); without it the tracker's
+// code is left out of the kernel.
+template <bool TRACK>
 __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
                                                 DevScalars* __restrict__ S, int32_t* __restrict__ count,
                                                 int32_t* __restrict__ slot, WallQueue Q, int do_move, int do_hist) {
@@ -585,7 +611,7 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
     const uint32_t step = S->step;
     const int lane = threadIdx.x & (WAVE - 1);
     const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
-    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
+    const bool track = TRACK;
     const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
     MoveCounts mc = { 0, 0, 0 };
     // wave-uniform trip count: every lane of a wave sees the same bases
@@ -598,10 +624,11 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
         Flight s;
         if (valid) {
             c = b.cell[i];
+            if (do_move) load_pu(b, i, s);  // in flight together with the cell
             if (do_move && c >= 0) {
                 PhiloxStream r;
                 move_stream(P, r, i, step, i >= n_old, &s.t);
-                load_flight(P, b, i, c, s);
+                set_cell(P, c, s);
 #pragma unroll
                 for (int a = 0; a < 3; ++a)
                     if (P.empty[a]) s.p[a] = P.centre[a];
@@ -647,6 +674,7 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
 // The queued flights: the wall interaction they stopped at, the rest of the
 // flight (every further wall handled here), the cell histogram.  The last
 // workgroup to finish clears the queue for the next step.
+template <bool TRACK>
 __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict__ Pp, DevParticles b,
                                                      DevScalars* __restrict__ S, int32_t* __restrict__ count,
                                                      int32_t* __restrict__ slot, WallQueue Q, int do_hist) {
@@ -655,7 +683,7 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
     const uint32_t step = S->step;
     const int lane = threadIdx.x & (WAVE - 1);
     const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
-    const bool track = (P.flags & HS_FLAG_FACE_FLUX) && (P.flags & HS_FLAG_SAMPLE);
+    const bool track = TRACK;
     MoveCounts mc = { 0, 0, 0 };
     for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < nq;
          base += (int64_t)gridDim.x * BLOCK) {
@@ -670,7 +698,8 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
             PhiloxStream r;
             double t0;
             move_stream(P, r, i, step, i >= n_old, &t0);
-            load_flight(P, b, i, b.cell[i], s);
+            load_pu(b, i, s);
+            set_cell(P, b.cell[i], s);
             s.t = Q.t[k];
             s.iter = ci >> 8;
             const int t_id = b.type ? (int)b.type[i] : 0;
@@ -697,9 +726,24 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
 }
 
 void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
-                 const WallQueue& Q, int do_move, int do_hist, int grid, hipStream_t st) {
-    k_move<<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_move, do_hist);
-    if (do_move) k_move_wall<<<grid / 8 > 0 ? grid / 8 : 1, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_hist);
+                 const WallQueue& Q, int do_move, int do_hist, int track, int grid, hipStream_t st) {
+    const int gw = grid / 8 > 0 ? grid / 8 : 1;
+    if (track) {
+        k_move<true><<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_move, do_hist);
+        if (do_move) k_move_wall<true><<<gw, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_hist);
+    } else {
+        k_move<false><<<grid, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_move, do_hist);
+        if (do_move) k_move_wall<false><<<gw, BLOCK, 0, st>>>(P, b, S, count, slot, Q, do_hist);
+    }
+}
+
+// Workgroups of k_move resident per CU (its register use sets the occupancy).
+int move_blocks_per_cu(int track) {
+    int nb = 0;
+    const hipError_t e = track ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_move<true>, BLOCK, 0)
+                               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_move<false>, BLOCK, 0);
+    if (e != hipSuccess || nb < 1) nb = 4;
+    return nb;
 }
 
 __global__ void k_set_nold(DevScalars* S) { S->n_old = S->n; }
@@ -911,86 +955,85 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
     sigma[c] = smax_cur;
 }
 
-// NTC of one dense cell by a whole wave.  Each batch evaluates the next 64
-// candidates against the current velocities; an accepted candidate stamps its
-// two parcels (LDS atomicMin of the lane id) and the first candidate touching
-// a parcel stamped by an earlier lane ends the batch.  Everything before it
-// is committed, which is exactly what the candidate-by-candidate order
-// produces: their parcels were not changed by any earlier accepted candidate.
-// The sub-cell lists and tables are built and the stamps cleared beforehand.
+// NTC of one dense cell by a whole wave, 64 candidates per batch.  A
+// candidate's pair comes from its own stream alone, so each lane draws its
+// pair once; only the acceptance test and the collision need the velocities.
+// A candidate is evaluated as soon as every earlier candidate of the batch
+// that shares one of its parcels has been: each round, every pending lane
+// stamps its two parcels (LDS atomicMin of the lane id), and a lane that is
+// the earliest pending claimant of both is ready.  Ready lanes share no
+// parcel, so they run together, and each sees exactly the velocities the
+// candidate-by-candidate loop (noTimeCounter.C:157-302) gives it.  The rounds
+// per batch are the depth of its parcel-sharing chains.  The sub-cell lists
+// and tables are built and the stamps cleared beforehand; they are clear
+// again on return.
 template <bool GEN>
 __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
                          float* vz, const uint8_t* sub, const CellTab& tab, float* er,
                          uint16_t* vb, int vstride, const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step,
                          const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out) {
     const int lane = threadIdx.x & (WAVE - 1);
+    const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
     double smax_cur = smax;
     double sep = 0.0;
     unsigned long long ncoll_i = 0;
-    for (long long k0 = 0; k0 < ncand;) {
+    for (long long k0 = 0; k0 < ncand; k0 += WAVE) {
         const long long k = k0 + lane;
-        const bool active = k < ncand;
         int pp = -1, qq = -1;
-        bool valid = false, acc = false;
-        double stcr = 0.0;
-        double up[3] = { 0.0, 0.0, 0.0 }, uq[3] = { 0.0, 0.0, 0.0 };
+        bool pend = false;
         PhiloxStream r;
-        if (active) {
+        if (k < ncand) {
             r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-            valid = draw_pair<GEN>(P, r, nC, sub, ss, tab, ty, &pp, &qq);
-            if (valid) {
+            pend = draw_pair<GEN>(P, r, nC, sub, ss, tab, ty, &pp, &qq);
+        }
+        bool doit = false;
+        double sm = 0.0, sep_l = 0.0;
+        while (__ballot(pend)) {
+            if (pend) {
+                atomicMin(&stamp[ss + pp], lane);
+                atomicMin(&stamp[ss + qq], lane);
+            }
+            __builtin_amdgcn_wave_barrier();
+            const bool ready = pend && stamp[ss + pp] == lane && stamp[ss + qq] == lane;
+            __builtin_amdgcn_wave_barrier();
+            if (pend) {
+                stamp[ss + pp] = 0x7FFFFFFF;
+                stamp[ss + qq] = 0x7FFFFFFF;
+            }
+            if (ready) {
                 const int ip = ss + pp, iq = ss + qq;
                 const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
-                up[0] = vx[ip]; up[1] = vy[ip]; up[2] = vz[ip];
-                uq[0] = vx[iq]; uq[1] = vy[iq]; uq[2] = vz[iq];
-                stcr = sigma_tcr(P, tp, tq, up, uq);
-                acc = stcr / smax > r.u();
+                double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
+                double uq[3] = { (double)vx[iq], (double)vy[iq], (double)vz[iq] };
+                const double stcr = sigma_tcr(P, tp, tq, up, uq);
+                sm = stcr;
+                if (stcr / smax > r.u()) {
+                    do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+                    doit = true;
+                    if (sep_on) {
+                        const int64_t gp = gs + pp, gq = gs + qq;
+                        const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq],
+                                     ddz = pos.pz[gp] - pos.pz[gq];
+                        sep_l = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    }
+                }
+                pend = false;
             }
+            __builtin_amdgcn_wave_barrier();
         }
-        if (acc) {
-            atomicMin(&stamp[ss + pp], lane);
-            atomicMin(&stamp[ss + qq], lane);
-        }
-        __builtin_amdgcn_wave_barrier();
-        bool conf = false;
-        if (valid) conf = stamp[ss + pp] < lane || stamp[ss + qq] < lane;
-        const unsigned long long cm = __ballot(conf);
-        const long long left = ncand - k0;
-        int fc = cm ? __ffsll((long long)cm) - 1 : WAVE;
-        if ((long long)fc > left) fc = (int)left;
-        __builtin_amdgcn_wave_barrier();
-        if (acc) {
-            stamp[ss + pp] = 0x7FFFFFFF;
-            stamp[ss + qq] = 0x7FFFFFFF;
-        }
-        const bool commit = active && lane < fc;
-        // running maximum of sigmaTcR over the committed candidates
-        double sm = (commit && valid) ? stcr : 0.0;
+        // running maximum of sigmaTcR over the batch
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) sm = fmax(sm, __shfl_xor(sm, o, WAVE));
         if (sm > smax_cur) smax_cur = sm;
-        const bool doit = commit && acc;
-        double sep_l = 0.0;
-        if (doit) {
-            const int ip = ss + pp, iq = ss + qq;
-            do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
-            if (P.flags & HS_FLAG_COLL_SEP) {
-                const int64_t gp = gs + pp, gq = gs + qq;
-                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
-                sep_l = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            }
-        }
         const unsigned long long dm = __ballot(doit);
         ncoll_i += (unsigned long long)__popcll(dm);
-        if (P.flags & HS_FLAG_COLL_SEP) {
+        if (sep_on) {
             // add in candidate order, as the sequential loop does
-            for (int j = 0; j < fc; ++j) {
+            for (int j = 0; j < WAVE; ++j) {
                 const double v = __shfl(sep_l, j, WAVE);
                 if ((dm >> j) & 1ull) sep += v;
             }
         }
-        __builtin_amdgcn_wave_barrier();
-        k0 += fc;
     }
     if (lane == 0) {
         sigma[c] = smax_cur;
@@ -1182,50 +1225,41 @@ struct TileLds {
     int32_t big[CT_CELLS];   // cells of more than 64 parcels
     int32_t nbig;
     // candidate rounds of the sparse cells
-    int32_t rk[CT_CELLS];    // next candidate of each cell
     int32_t rn[CT_CELLS];    // candidates handled by the rounds (0 for dense cells)
-    int32_t rpfx[CT_CELLS];  // first round slot of each cell
-    int32_t rfirst[CT_CELLS];  // first slot of the cell that saw a changed parcel
+    int32_t rpfx[CT_CELLS];  // first slot of each cell
     unsigned long long rsmax[CT_CELLS];  // running max of sigmaTcR (bits of a non-negative double)
     uint32_t rcoll[CT_CELLS];
-    double gsep[CT];         // collision separation of each round slot
-    uint8_t gacc[CT];        // round slot committed a collision
+    double gsep[CT];         // collision separation of each slot of a pass
+    uint8_t gacc[CT];        // the slot's candidate collided
 };
 
 // NTC of the sparse cells of a chunk (at most DENSE_CAND candidates each),
-// candidate-parallel over the workgroup.  A round gives every pending
-// candidate of every cell a slot (lane), in cell then candidate order; each
-// slot draws its pair from the candidate's own stream and evaluates it
-// against the velocities at the start of the round.  An accepted slot stamps
-// its two parcels (LDS atomicMin of the slot); in each cell the slots before
-// the first one that touches a parcel stamped by an earlier slot are
-// committed -- they saw exactly the velocities the sequential loop of
-// noTimeCounter.C:157-302 sees -- and the rest of that cell's candidates wait
-// for the next round.
+// candidate-parallel over the workgroup.  The candidates of all cells, in cell
+// then candidate order, take the workgroup's slots CT at a time (a pass); each
+// slot draws its pair once from the candidate's own stream.  Rounds then run
+// as in ntc_wave: every pending slot stamps its two parcels (LDS atomicMin of
+// the slot), a slot that is the earliest pending claimant of both is ready,
+// and the ready slots -- which share no parcel -- evaluate and collide
+// against exactly the velocities the sequential loop of noTimeCounter.C:157-302
+// gives them.  A pass ends when all its slots are done; the next pass (the
+// later candidates) starts after it.
 template <bool GEN, int PMAX>
 __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int a, int b, int S0, uint32_t step,
                            const DevParticles& pos, CellCtx& cx) {
     const int t = threadIdx.x;
     const bool own = t >= a && t < b;
     const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
-    for (;;) {
-        const int pend = own ? L.rn[t] - L.rk[t] : 0;
-        int total;
-        const int first = block_excl_scan(pend, &total);
-        if (total == 0) break;
-        if (own) {
-            L.rpfx[t] = first;
-            L.rfirst[t] = 0x7FFFFFFF;
-        }
-        __syncthreads();
-        const int g = t;
-        const bool active = g < total;
+    int total;
+    const int first = block_excl_scan(own ? L.rn[t] : 0, &total);
+    if (own) L.rpfx[t] = first;
+    __syncthreads();
+    for (int g0 = 0; g0 < total; g0 += CT) {
+        const int g = g0 + t;
         int l = a, ss = 0, pp = -1, qq = -1;
-        bool valid = false, acc = false;
-        double stcr = 0.0;
-        double up[3] = { 0.0, 0.0, 0.0 }, uq[3] = { 0.0, 0.0, 0.0 };
+        bool pend = false, doit = false;
+        double sep_l = 0.0;
         PhiloxStream r;
-        if (active) {
+        if (g < total) {
             // the last cell whose first slot is <= g
             int lo = a, hi = b - 1;
             while (lo < hi) {
@@ -1234,59 +1268,59 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
                 else hi = mid - 1;
             }
             l = lo;
-            const int k = L.rk[l] + (g - L.rpfx[l]);
+            const int k = g - L.rpfx[l];
             ss = L.segS[l] - S0;
             const int nC = L.segS[l + 1] - L.segS[l];
             r.init(P, RNG_COLLIDE, (uint32_t)(c0 + l), step, (uint32_t)k);
             const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
-            valid = draw_pair<GEN>(P, r, nC, L.sub, ss, tab, L.ty, &pp, &qq);
-            if (valid) {
-                const int ip = ss + pp, iq = ss + qq;
-                const int tp = GEN ? (int)L.ty[ip] : 0, tq = GEN ? (int)L.ty[iq] : 0;
-                up[0] = L.vx[ip]; up[1] = L.vy[ip]; up[2] = L.vz[ip];
-                uq[0] = L.vx[iq]; uq[1] = L.vy[iq]; uq[2] = L.vz[iq];
-                stcr = sigma_tcr(P, tp, tq, up, uq);
-                acc = stcr / L.smax[l] > r.u();
-            }
-            if (acc) {
+            pend = draw_pair<GEN>(P, r, nC, L.sub, ss, tab, L.ty, &pp, &qq);
+        }
+        for (;;) {
+            if (pend) {
                 atomicMin(&L.key[ss + pp], g);
                 atomicMin(&L.key[ss + qq], g);
             }
-        }
-        __syncthreads();
-        if (valid && (L.key[ss + pp] < g || L.key[ss + qq] < g)) atomicMin(&L.rfirst[l], g);
-        __syncthreads();
-        const bool commit = active && g < L.rfirst[l];
-        if (commit && valid) atomicMax(&L.rsmax[l], (unsigned long long)__double_as_longlong(stcr));
-        const bool doit = commit && acc;
-        if (doit) {
-            do_collision<GEN>(P, r, ss + pp, ss + qq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
-            atomicAdd(&L.rcoll[l], 1u);
-            if (sep_on) {
-                const int64_t gp = (int64_t)S0 + ss + pp, gq = (int64_t)S0 + ss + qq;
-                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
-                L.gsep[g] = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            __syncthreads();
+            const bool ready = pend && L.key[ss + pp] == g && L.key[ss + qq] == g;
+            __syncthreads();
+            if (pend) {
+                L.key[ss + pp] = 0x7FFFFFFF;
+                L.key[ss + qq] = 0x7FFFFFFF;
             }
+            if (ready) {
+                const int ip = ss + pp, iq = ss + qq;
+                const int tp = GEN ? (int)L.ty[ip] : 0, tq = GEN ? (int)L.ty[iq] : 0;
+                double up[3] = { (double)L.vx[ip], (double)L.vy[ip], (double)L.vz[ip] };
+                double uq[3] = { (double)L.vx[iq], (double)L.vy[iq], (double)L.vz[iq] };
+                const double stcr = sigma_tcr(P, tp, tq, up, uq);
+                atomicMax(&L.rsmax[l], (unsigned long long)__double_as_longlong(stcr));
+                if (stcr / L.smax[l] > r.u()) {
+                    do_collision<GEN>(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
+                    atomicAdd(&L.rcoll[l], 1u);
+                    doit = true;
+                    if (sep_on) {
+                        const int64_t gp = (int64_t)S0 + ip, gq = (int64_t)S0 + iq;
+                        const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq],
+                                     ddz = pos.pz[gp] - pos.pz[gq];
+                        sep_l = sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    }
+                }
+                pend = false;
+            }
+            if (!__syncthreads_or(pend)) break;
         }
-        L.gacc[g] = doit ? 1 : 0;
-        __syncthreads();
-        if (acc) {
-            L.key[ss + pp] = 0x7FFFFFFF;
-            L.key[ss + qq] = 0x7FFFFFFF;
+        if (sep_on) {
+            // each cell adds its slots' separations in candidate order
+            L.gsep[t] = sep_l;
+            L.gacc[t] = doit ? 1 : 0;
+            __syncthreads();
+            if (own) {
+                const int beg = max(L.rpfx[t], g0), end = min(L.rpfx[t] + L.rn[t], g0 + CT);
+                for (int q = beg; q < end; ++q)
+                    if (L.gacc[q - g0]) L.sep[t] += L.gsep[q - g0];
+            }
+            __syncthreads();
         }
-        if (own && pend > 0) {
-            // slots [beg, beg+pend) of this cell that existed this round, up to the first dirty one;
-            // the first slot of a cell is never dirty, so every cell with beg < CT advances
-            const int beg = L.rpfx[t];
-            int stop = beg + pend < CT ? beg + pend : CT;
-            if (stop < beg) stop = beg;
-            if (L.rfirst[t] < stop) stop = L.rfirst[t];
-            if (sep_on)
-                for (int q = beg; q < stop; ++q)
-                    if (L.gacc[q]) L.sep[t] += L.gsep[q];
-            L.rk[t] += stop - beg;
-        }
-        __syncthreads();
     }
     (void)cx;
 }
@@ -1523,7 +1557,6 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
             int nsparse = 0;
             L.rcoll[t] = 0u;
             L.sep[t] = 0.0;
-            L.rk[t] = 0;
             if (do_collide && nC > 1) {
                 const double smax = pf_sig;
                 const long long ncand = ntc_candidates_of(P, c, nC, smax, pf_rem, pf_vol, rem, cx);

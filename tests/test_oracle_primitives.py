@@ -68,3 +68,22 @@ def test_sincos2pi_against_libm():
         assert abs(s * s + c * c - 1.0) < 4e-16
     assert worst < 1e-15
     assert ob.sincos2pi(0.0) == (0.0, 1.0)
+
+
+def test_quotient_below_one_iff_dividend_below_divisor():
+    """The flight kernel decides 'face crossed this step' (l = num/den < 1,
+    l clamped at 0) without dividing: num < den for den > 0, num > den for
+    den < 0.  For IEEE round-to-nearest doubles this is exact; checked here on
+    random pairs, on every adjacent pair (pred(b), b) and (b, succ(b)), and
+    across binade edges, where a rounded quotient next to 1 could misround."""
+    rng = np.random.default_rng(5)
+    b = np.concatenate([rng.uniform(1e-9, 1e-2, 200000), 2.0 ** rng.integers(-40, 10, 20000),
+                        np.nextafter(2.0 ** rng.integers(-40, 10, 20000), 0.0)])
+    near = [np.nextafter(b, 0.0), b, np.nextafter(b, np.inf), np.nextafter(np.nextafter(b, 0.0), 0.0)]
+    cands = near + [b * rng.uniform(0.0, 2.0, b.size), -b * rng.uniform(0.0, 1e-3, b.size)]
+    for num in cands:
+        for sgn in (1.0, -1.0):
+            n_, d_ = sgn * num, sgn * b
+            l = np.maximum(n_ / d_, 0.0)
+            fast = (n_ < d_) if sgn > 0 else (n_ > d_)
+            assert np.array_equal(l < 1.0, fast)
