@@ -53,11 +53,13 @@ struct hs_dsmc_engine {
     int32_t* d_perm = nullptr;
     uint16_t* d_big_list = nullptr;
     uint8_t* d_big_sub = nullptr;
-    int32_t* d_tile_begin = nullptr;
+    int32_t* d_tile_begin = nullptr;    // gather tiles
+    int32_t* d_ctile_begin = nullptr;   // collide tiles
     int32_t* d_tile_order = nullptr;   // collide workgroup -> tile, heavy tiles first
     int32_t* d_tile_heavy = nullptr;
     WallQueue wq = { nullptr, nullptr, nullptr };  // flights stopped at walls
-    int tile_parcels = 1536;  // parcels per collide workgroup (particle-balanced tiles)
+    int tile_parcels = 1536;     // parcels per gather workgroup (particle-balanced tiles)
+    int collide_parcels = 2048;  // parcels per collide workgroup
     DevFaces faces;
     int32_t* d_face_off = nullptr;
     // wall faces (boundaryMeasurements)
@@ -459,6 +461,8 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     }
     eng->has_type = eng->ns > 1;
     eng->general = eng->has_erot || eng->has_vib || eng->has_type;
+    // collide tiles sized to the LDS chunk of the kernel variant
+    eng->collide_parcels = eng->general ? 1536 : 2048;
     eng->nslots = hs_dsmc_sample_slots(cfg);
     derive(eng);
     auto bail = [&](int code) {
@@ -477,6 +481,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         (rc = dev_alloc(eng, &eng->d_slot, eng->cap)) || (rc = dev_alloc(eng, &eng->d_perm, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_big_list, eng->cap)) || (rc = dev_alloc(eng, &eng->d_big_sub, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_tile_begin, eng->cap / 64 + 4)) ||
+        (rc = dev_alloc(eng, &eng->d_ctile_begin, eng->cap / 64 + 4)) ||
         (rc = dev_alloc(eng, &eng->d_tile_order, eng->cap / 64 + 4)) || 
         (rc = dev_alloc(eng, &eng->d_tile_heavy, eng->cap / 64 + 4)) ||
         (rc = dev_alloc(eng, &eng->wq.i, eng->cap)) || (rc = dev_alloc(eng, &eng->wq.t, eng->cap)) ||
@@ -826,14 +831,15 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     {
         StageTimer tm(eng, T_GATHER);
-        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->d_tile_heavy, eng->stream);
+        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->collide_parcels,
+                     eng->d_ctile_begin, eng->d_tile_heavy, eng->stream);
         launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
                       eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
     }
     if (do_collide || do_sample) {
         StageTimer tm(eng, T_COLLIDE);
-        launch_collide(eng->dp, nxt_buf(eng), eng->d_start, eng->d_tile_begin, eng->d_tile_heavy, eng->d_tile_order,
-                       eng->tile_parcels, eng->cap, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
+        launch_collide(eng->dp, nxt_buf(eng), eng->d_start, eng->d_ctile_begin, eng->d_tile_heavy, eng->d_tile_order,
+                       eng->collide_parcels, eng->cap, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
                        eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc, eng->d_big_list,
                        eng->stream);
     }

@@ -68,7 +68,8 @@ void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, c
                     int32_t* perm, int grid, hipStream_t st);
 // particle-balanced tiles of pt parcels (tile_begin) and their heavy flags
 // (clear before the first use; the collide order clears the flags it reads)
-void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, int32_t* heavy, hipStream_t st);
+void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
+                  int32_t* heavy, hipStream_t st);
 // stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel
 void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
                    const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,

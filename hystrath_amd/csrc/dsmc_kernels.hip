@@ -1613,19 +1613,29 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
 }
 
 // Tile q of the particle-balanced decomposition owns the cells whose start
-// offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.
-// Also flags the tiles holding a cell of more than HEAVY_CELL parcels
+// offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.  Two
+// tilings from one pass: the gather's (pt_g parcels) and the collide's (pt_c).
+// Also flags the collide tiles holding a cell of more than HEAVY_CELL parcels
 // (heavy[] cleared beforehand).
 #define HEAVY_CELL 128
-__global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt,
-                                                 int32_t* __restrict__ tile_begin, int32_t* __restrict__ heavy) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i > nc) return;
-    if (i == 0) { tile_begin[0] = 0; return; }
+__device__ __forceinline__ void tile_marks(const int32_t* __restrict__ start, int i, int nc, int pt,
+                                           int32_t* __restrict__ tile_begin, int* q0_out) {
     const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
     for (int q = q0 + 1; q <= q1; ++q) tile_begin[q] = i;
     if (i == nc) tile_begin[q1 + 1] = nc;
-    if (start[i] - start[i - 1] > HEAVY_CELL) heavy[q0] = 1;
+    *q0_out = q0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt_g,
+                                                 int32_t* __restrict__ tile_begin_g, int pt_c,
+                                                 int32_t* __restrict__ tile_begin_c, int32_t* __restrict__ heavy) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i > nc) return;
+    if (i == 0) { tile_begin_g[0] = 0; tile_begin_c[0] = 0; return; }
+    int qg, qc;
+    tile_marks(start, i, nc, pt_g, tile_begin_g, &qg);
+    tile_marks(start, i, nc, pt_c, tile_begin_c, &qc);
+    if (start[i] - start[i - 1] > HEAVY_CELL) heavy[qc] = 1;
 }
 
 // Order in which the collide workgroups take the tiles: heavy tiles first
@@ -1670,8 +1680,9 @@ __global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__
     add_counter(S, CNT_TILE_OVF, cx.ovf);
 }
 
-void launch_tiles(const int32_t* start, int nc, int pt, int32_t* tile_begin, int32_t* heavy, hipStream_t st) {
-    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt, tile_begin, heavy);
+void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
+                  int32_t* heavy, hipStream_t st) {
+    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt_g, tile_begin_g, pt_c, tile_begin_c, heavy);
 }
 
 #define PMAX_MONO 2304
