@@ -38,6 +38,9 @@ struct hs_dsmc_engine {
     int has_erot = 0, has_vib = 0, has_type = 0, max_modes = 0;
     int64_t cap = 0;
     hipStream_t stream = nullptr;
+    hipStream_t side = nullptr;          // the next step's inflow, overlapped with this step's sort and collide
+    hipEvent_t ev_scanned = nullptr;     // this step's cell scan is done (n_sorted known)
+    hipEvent_t ev_injected = nullptr;    // the next step's inflow is written
     DevParams hp;
     DevParams* dp = nullptr;
     DevParticles buf[2];
@@ -130,19 +133,20 @@ static hipEvent_t take_event(hs_dsmc_engine* eng) {
 struct StageTimer {
     hs_dsmc_engine* eng;
     int stage;
+    hipStream_t st;
     hipEvent_t a = nullptr;
-    StageTimer(hs_dsmc_engine* e, int s) : eng(e), stage(s) {
+    StageTimer(hs_dsmc_engine* e, int s, hipStream_t on = nullptr) : eng(e), stage(s), st(on ? on : e->stream) {
         eng->launches[s] += 1;
         if ((eng->cfg.flags & HS_FLAG_TIMING) && (int)eng->timed.size() < kMaxTimedPairs) {
             a = take_event(eng);
-            if (a) (void)hipEventRecord(a, eng->stream);
+            if (a) (void)hipEventRecord(a, st);
         }
     }
     ~StageTimer() {
         if (!a) return;
         hipEvent_t b = take_event(eng);
         if (!b) return;
-        (void)hipEventRecord(b, eng->stream);
+        (void)hipEventRecord(b, st);
         eng->timed.push_back({ stage, a, b });
     }
 };
@@ -430,9 +434,13 @@ extern "C" void hs_dsmc_destroy(hs_dsmc_engine* eng) {
     if (!eng) return;
     (void)hipSetDevice(eng->device);
     if (eng->stream) (void)hipStreamSynchronize(eng->stream);
+    if (eng->side) (void)hipStreamSynchronize(eng->side);
+    if (eng->ev_scanned) (void)hipEventDestroy(eng->ev_scanned);
+    if (eng->ev_injected) (void)hipEventDestroy(eng->ev_injected);
     for (auto& t : eng->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
     for (auto e : eng->event_pool) (void)hipEventDestroy(e);
     for (void* p : eng->allocs) (void)hipFree(p);
+    if (eng->side) (void)hipStreamDestroy(eng->side);
     if (eng->stream) (void)hipStreamDestroy(eng->stream);
     delete eng;
 }
@@ -470,8 +478,11 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         hs_dsmc_destroy(eng);
         return code;
     };
-    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(fail(eng, HS_E_HIP, "hipStreamCreate failed"));
+    if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&eng->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&eng->ev_scanned, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&eng->ev_injected, hipEventDisableTiming) != hipSuccess)
+        return bail(fail(eng, HS_E_HIP, "hipStreamCreate / hipEventCreate failed"));
     const int nc = eng->nc;
     if ((rc = dev_alloc(eng, &eng->dp, 1)) || (rc = alloc_particles(eng, eng->buf[0])) ||
         (rc = alloc_particles(eng, eng->buf[1])) || (rc = dev_alloc(eng, &eng->d_vol, nc)) ||
@@ -804,13 +815,18 @@ extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, doub
 // ---------------------------------------------------------------------------
 // the time step
 // ---------------------------------------------------------------------------
-static void enqueue_inject(hs_dsmc_engine* eng) {
+// ahead = 0: this step's inflow, appended to the current population.
+// ahead = 1: the next step's inflow on the side stream, appended after this
+// step's sorted population in the buffer that becomes current at the swap
+// (the sort and collide write only below n_sorted); commit installs it.
+static void enqueue_inject(hs_dsmc_engine* eng, int ahead) {
     if (eng->faces.n == 0) return;
-    StageTimer tm(eng, T_INJECT);
-    launch_inject_count(eng->dp, eng->faces, eng->ns, eng->d_S, eng->stream);
-    launch_scan(eng->faces.n_new, eng->faces.off, eng->faces.n * eng->ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap,
-                eng->stream);
-    launch_inject_gen(eng->dp, eng->faces, eng->ns, cur_buf(eng), eng->d_S, eng->stream);
+    hipStream_t st = ahead ? eng->side : eng->stream;
+    StageTimer tm(eng, T_INJECT, st);
+    launch_inject_count(eng->dp, eng->faces, eng->ns, eng->d_S, ahead, st);
+    launch_scan(eng->faces.n_new, eng->faces.off, eng->faces.n * eng->ns, ahead ? SCAN_APPEND_NEXT : SCAN_APPEND,
+                eng->ws, eng->d_S, eng->cap, st);
+    launch_inject_gen(eng->dp, eng->faces, eng->ns, ahead ? nxt_buf(eng) : cur_buf(eng), eng->d_S, ahead, st);
 }
 
 static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
@@ -820,10 +836,19 @@ static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
                 do_move ? eng->flight_grid[track] : eng->move_grid, eng->stream);
 }
 
-static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance) {
+// inject_next: the next step's inflow runs on the side stream, concurrently
+// with this step's scatter, gather and collide (it needs only n_sorted).
+static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance, int inject_next) {
     {
         StageTimer tm(eng, T_SCAN);
         launch_scan(eng->d_count, eng->d_start, eng->nc, SCAN_CELLS, eng->ws, eng->d_S, eng->cap, eng->stream);
+    }
+    inject_next = inject_next && advance && eng->faces.n > 0;
+    if (inject_next) {
+        (void)hipEventRecord(eng->ev_scanned, eng->stream);
+        (void)hipStreamWaitEvent(eng->side, eng->ev_scanned, 0);
+        enqueue_inject(eng, 1);
+        (void)hipEventRecord(eng->ev_injected, eng->side);
     }
     {
         StageTimer tm(eng, T_SCATTER);
@@ -843,7 +868,8 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
                        eng->d_vol, eng->d_S, do_collide, do_sample, eng->general, eng->nc, eng->d_big_list,
                        eng->stream);
     }
-    launch_commit(eng->d_S, advance, eng->stream);
+    if (inject_next) (void)hipStreamWaitEvent(eng->stream, eng->ev_injected, 0);
+    launch_commit(eng->d_S, advance, inject_next, eng->stream);
     eng->cur = 1 - eng->cur;
     if (do_sample) eng->n_sampled += 1;
 }
@@ -855,9 +881,9 @@ extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
     const int sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
     const int collide = eng->cfg.collision_model != HS_COLL_NONE;
     for (int k = 0; k < nsteps; ++k) {
-        enqueue_inject(eng);
+        if (k == 0) enqueue_inject(eng, 0);  // later steps' inflow is injected ahead, in the step before
         enqueue_move(eng, 1, 1);
-        enqueue_sort_collide(eng, collide, sample, 1);
+        enqueue_sort_collide(eng, collide, sample, 1, k + 1 < nsteps);
     }
     return check_launch(eng, "evolve");
 }
@@ -865,7 +891,7 @@ extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
 extern "C" int hs_dsmc_inject(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
-    enqueue_inject(eng);
+    enqueue_inject(eng, 0);
     return check_launch(eng, "inject");
 }
 
@@ -880,7 +906,7 @@ extern "C" int hs_dsmc_sort(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
     enqueue_move(eng, 0, 1);
-    enqueue_sort_collide(eng, 0, 0, 0);
+    enqueue_sort_collide(eng, 0, 0, 0, 0);
     return check_launch(eng, "sort");
 }
 
@@ -888,7 +914,7 @@ extern "C" int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
     enqueue_move(eng, 0, 1);
-    enqueue_sort_collide(eng, eng->cfg.collision_model != HS_COLL_NONE, sample ? 1 : 0, 0);
+    enqueue_sort_collide(eng, eng->cfg.collision_model != HS_COLL_NONE, sample ? 1 : 0, 0, 0);
     return check_launch(eng, "collide");
 }
 

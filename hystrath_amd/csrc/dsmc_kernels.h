@@ -28,6 +28,7 @@ struct DevScalars {
     uint32_t step;    // time-step index (random-stream counter)
     int32_t total;    // scratch: last scan total
     int32_t n_sorted; // population after the cell scan (published by commit)
+    int32_t n_next;   // n_sorted plus the next step's inflow when injected ahead (installed by commit)
     int32_t ord[2];   // heavy / light tile counters of the collide order (cleared by commit)
     int32_t qn;       // flights queued for the wall pass of the move (cleared by that pass)
     int32_t qdone;    // workgroups of the wall pass that finished
@@ -44,13 +45,14 @@ struct ScanWs {
 enum ScanMode {
     SCAN_PLAIN = 0,     // out[n] = total
     SCAN_CELLS = 1,     // out[n] = total, zero the input, S.n_sorted = total
-    SCAN_APPEND = 2     // S.n_old = S.n, S.n = min(S.n + total, capacity)
+    SCAN_APPEND = 2,    // S.n_old = S.n, S.n = min(S.n + total, capacity)
+    SCAN_APPEND_NEXT = 3  // S.n_next = min(S.n_sorted + total, capacity): the next step's inflow
 };
 
 void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, ScanWs& ws, DevScalars* S,
                  int64_t capacity, hipStream_t st);
-void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st);
-void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
+void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, int ahead, hipStream_t st);
+void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S, int ahead,
                        hipStream_t st);
 // Flights of the move stopped at a wall, for its wall pass: parcel index,
 // flight time left, wall code | tracking iterations << 8 (capacity entries).
@@ -79,7 +81,7 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                     int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st);
-void launch_commit(DevScalars* S, int advance, hipStream_t st);
+void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
                        int32_t* cnt, int nc, int ns, hipStream_t st);
 void launch_fill_gen(const DevParams* P, const double* nd, double T, double Trot, double Tvib, const double* u3,

@@ -77,6 +77,12 @@ __device__ __forceinline__ void scan_publish(int32_t* out, int32_t n, int total,
         S->n_old = S->n;
         S->n = (int32_t)(nn > capacity ? capacity : nn);
         if (nn > capacity) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - capacity));
+    } else if (mode == SCAN_APPEND_NEXT) {
+        // the next step's inflow, appended after the sorted population while
+        // this step's sort and collide still use n and n_old (commit installs it)
+        const int64_t nn = (int64_t)S->n_sorted + total;
+        S->n_next = (int32_t)(nn > capacity ? capacity : nn);
+        if (nn > capacity) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - capacity));
     }
 }
 
@@ -223,8 +229,10 @@ __device__ __forceinline__ void new_internal(const DevParams& P, DevParticles& b
 // ---------------------------------------------------------------------------
 // freestream inflow (dsmcFreeStreamInflowPatch.C:91-364)
 // ---------------------------------------------------------------------------
+// ahead = 1: the next step's inflow, drawn before this step's commit
+// advances the step counter.
 __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restrict__ Pp, DevFaces F, int ns,
-                                                        DevScalars* __restrict__ S) {
+                                                        DevScalars* __restrict__ S, int ahead) {
     const DevParams& P = *Pp;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= F.n * ns) return;
@@ -235,7 +243,7 @@ __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restr
     int ni = (int)a;
     if (ni < 0) ni = 0;
     PhiloxStream r;
-    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 0);
+    r.init(P, RNG_INJECT, (uint32_t)e, S->step + (uint32_t)ahead, 0);
     if ((a - (double)ni) > r.u()) ni += 1;
     a -= (double)ni;
     *acc = a;
@@ -245,7 +253,8 @@ __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restr
 // Parcel j of this step belongs to the entry e (face*ns + species) with
 // off[e] <= j < off[e+1] and is its (j-off[e])-th; it is written at idx.
 // off: the entries' offsets (LDS copy or global).
-__device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t* off, int ns, DevParticles& b,
+__device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t* off, int ns, uint32_t step,
+                           DevParticles& b,
                            const DevScalars* S, int j, int64_t idx) {
     const int ne = F.n * ns;
     int lo = 0, hi = ne;
@@ -258,7 +267,7 @@ __device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t*
     const int k = j - off[e];
     const int f = e / ns, s = e - f * ns;
     PhiloxStream r;
-    r.init(P, RNG_INJECT, (uint32_t)e, S->step, 1u + (uint32_t)k);
+    r.init(P, RNG_INJECT, (uint32_t)e, step, 1u + (uint32_t)k);
     const int a = F.axis[f], dir = F.dir[f];
     const int a1 = a == 0 ? 1 : 0, a2 = a == 2 ? 1 : 2;
     const int nn[3] = { P.nx, P.ny, P.nz };
@@ -306,7 +315,7 @@ __device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t*
 // when they fit.
 #define INJ_LDS_OFF 4096
 __global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
-                                                      DevParticles b, DevScalars* __restrict__ S) {
+                                                      DevParticles b, DevScalars* __restrict__ S, int ahead) {
     __shared__ int32_t soff[INJ_LDS_OFF];
     const DevParams& P = *Pp;
     const int ne = F.n * ns;
@@ -317,23 +326,23 @@ __global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restric
     }
     const int32_t* off = lds ? soff : F.off;
     const int j = blockIdx.x * BLOCK + threadIdx.x;
-    const int64_t idx = (int64_t)S->n_old + j;
+    const int64_t idx = (int64_t)(ahead ? S->n_sorted : S->n_old) + j;
     const bool mine = j < off[ne] && idx < P.capacity;
-    if (mine) inject_one(P, F, off, ns, b, S, j, idx);
+    if (mine) inject_one(P, F, off, ns, S->step + (uint32_t)ahead, b, S, j, idx);
     // every lane reaches the wave reduction of the counter
     add_counter(S, CNT_INJ, mine ? 1ull : 0ull);
 }
 
-void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, hipStream_t st) {
+void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, int ahead, hipStream_t st) {
     const int n = F.n * ns;
     if (n == 0) return;
-    k_inject_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, S);
+    k_inject_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, S, ahead);
 }
 
-void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S,
+void launch_inject_gen(const DevParams* P, const DevFaces& F, int ns, DevParticles b, DevScalars* S, int ahead,
                        hipStream_t st) {
     if (F.n * ns == 0 || F.max_new <= 0) return;
-    k_inject_gen<<<(F.max_new + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, b, S);
+    k_inject_gen<<<(F.max_new + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, F, ns, b, S, ahead);
 }
 
 // ---------------------------------------------------------------------------
@@ -1712,15 +1721,18 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                                                         samples, vol, S, do_collide, do_sample, nc, big_list);
 }
 
-// publish the sorted population; optionally advance the step counter
-__global__ void k_commit(DevScalars* S, int advance) {
-    S->n = S->n_sorted;
+// publish the sorted population (with the next step's inflow when it was
+// injected ahead); optionally advance the step counter
+__global__ void k_commit(DevScalars* S, int advance, int with_next) {
+    S->n = with_next ? S->n_next : S->n_sorted;
     S->n_old = S->n_sorted;
     S->ord[0] = 0;
     S->ord[1] = 0;
     if (advance) S->step += 1;
 }
-void launch_commit(DevScalars* S, int advance, hipStream_t st) { k_commit<<<1, 1, 0, st>>>(S, advance); }
+void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st) {
+    k_commit<<<1, 1, 0, st>>>(S, advance, with_next);
+}
 
 // ---------------------------------------------------------------------------
 // initial fill (dsmcMeshFill.C:68-~230)
