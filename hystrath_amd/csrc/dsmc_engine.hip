@@ -74,6 +74,9 @@ struct hs_dsmc_engine {
     unsigned long long* d_face_flux = nullptr;
     DevScalars* d_S = nullptr;
     ScanWs ws;
+    unsigned long long* d_lb_status = nullptr;  // look-back status words of the cell scan
+    unsigned long long* d_lb_ticket = nullptr;  // its block tickets
+    uint32_t scan_epoch = 0;                    // cell scans launched so far
     int64_t n_sampled = 0;
     int move_grid = 1024;     // grid-stride launches over the parcels (scatter, hist-only move)
     int flight_grid[2] = { 1024, 1024 };  // the moving launch, without / with the face tracker:
@@ -507,7 +510,11 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     int64_t scan_n = (int64_t)nc * ns;
     if ((int64_t)nf * ns > scan_n) scan_n = (int64_t)nf * ns;
     eng->ws.nb_max = (int32_t)((scan_n + 2047) / 2048 + 1);
-    if ((rc = dev_alloc(eng, &eng->ws.bsum, eng->ws.nb_max))) return bail(rc);
+    if ((rc = dev_alloc(eng, &eng->ws.bsum, eng->ws.nb_max)) ||
+        (rc = dev_alloc(eng, &eng->d_lb_status, scan_cells_blocks(nc))) || (rc = dev_alloc(eng, &eng->d_lb_ticket, 1)))
+        return bail(rc);
+    launch_fill_i64(reinterpret_cast<int64_t*>(eng->d_lb_status), 0, scan_cells_blocks(nc), eng->stream);
+    launch_fill_i64(reinterpret_cast<int64_t*>(eng->d_lb_ticket), 0, 1, eng->stream);
     if (nf > 0) {
         std::vector<int32_t> fc(nf), a1(nf), a2(nf);
         std::vector<int8_t> ax(nf), dr(nf);
@@ -841,7 +848,9 @@ static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
 static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance, int inject_next) {
     {
         StageTimer tm(eng, T_SCAN);
-        launch_scan(eng->d_count, eng->d_start, eng->nc, SCAN_CELLS, eng->ws, eng->d_S, eng->cap, eng->stream);
+        launch_scan_cells(eng->d_count, eng->d_start, eng->nc, eng->d_lb_status, eng->d_lb_ticket, ++eng->scan_epoch,
+                          eng->d_S, eng->tile_parcels, eng->d_tile_begin, eng->collide_parcels, eng->d_ctile_begin,
+                          eng->d_tile_heavy, eng->stream);
     }
     inject_next = inject_next && advance && eng->faces.n > 0;
     if (inject_next) {
@@ -856,8 +865,6 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     {
         StageTimer tm(eng, T_GATHER);
-        launch_tiles(eng->d_start, eng->nc, eng->tile_parcels, eng->d_tile_begin, eng->collide_parcels,
-                     eng->d_ctile_begin, eng->d_tile_heavy, eng->stream);
         launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
                       eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
     }

@@ -70,6 +70,10 @@ void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, c
                     int32_t* perm, int grid, hipStream_t st);
 // particle-balanced tiles of pt parcels (tile_begin) and their heavy flags
 // (clear before the first use; the collide order clears the flags it reads)
+int scan_cells_blocks(int32_t n);
+void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long long* status,
+                       unsigned long long* ticket, uint32_t epoch, DevScalars* S, int pt_g, int32_t* tile_g,
+                       int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st);
 void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
                   int32_t* heavy, hipStream_t st);
 // stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel

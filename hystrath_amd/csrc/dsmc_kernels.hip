@@ -184,6 +184,118 @@ void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, Scan
 }
 
 // ---------------------------------------------------------------------------
+// the cell scan in one pass (decoupled look-back): counts -> start offsets,
+// counts cleared, n_sorted published, and the gather and collide tilings
+// marked from the same values
+// ---------------------------------------------------------------------------
+// A block's status word: epoch << 34 | flag << 32 | value, flag 1 = the
+// block's aggregate, 2 = the inclusive prefix through the block.  A word from
+// an earlier epoch (an earlier scan) reads as not ready, so nothing is cleared
+// between scans.
+#define HEAVY_CELL 128   // parcels: a cell this full marks its collide tile heavy
+#define LB_AGG 1ull
+#define LB_INCL 2ull
+#define LB_SPIN_MAX (1 << 24)
+
+__device__ __forceinline__ unsigned long long lb_word(uint32_t epoch, unsigned long long flag, int32_t v) {
+    return ((unsigned long long)epoch << 34) | (flag << 32) | (unsigned long long)(uint32_t)v;
+}
+
+// The gather and collide tiles starting between cells i and i+1 (the parcels
+// of cell i occupy [e, e + c)): tile q of size pt starts at the first cell
+// whose start offset is >= q*pt.
+__device__ __forceinline__ void tile_marks_after(int i, int32_t e, int32_t c, int pt, int32_t* __restrict__ tb) {
+    const int q0 = e / pt, q1 = (e + c) / pt;
+    for (int q = q0 + 1; q <= q1; ++q) tb[q] = i + 1;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt, int32_t* __restrict__ start, int32_t n,
+                                                      unsigned long long* __restrict__ status,
+                                                      unsigned long long* __restrict__ ticket, uint32_t epoch,
+                                                      DevScalars* __restrict__ S, int pt_g,
+                                                      int32_t* __restrict__ tile_g, int pt_c,
+                                                      int32_t* __restrict__ tile_c, int32_t* __restrict__ heavy) {
+    __shared__ int s_blk;
+    __shared__ int s_excl;
+    // logical block = order of start, so every predecessor has started
+    if (threadIdx.x == 0)
+        s_blk = (int)(atomicAdd(ticket, 1ull) - (unsigned long long)(epoch - 1) * gridDim.x);
+    __syncthreads();
+    const int b = s_blk;
+    if (b < 0 || b >= (int)gridDim.x) {  // tickets out of step with the epoch: report, touch nothing
+        if (threadIdx.x == 0) atomicAdd(&S->counters[CNT_TILE_OVF], 1ull);
+        return;
+    }
+    const int i0 = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;  // SCAN_ITEMS consecutive cells per thread
+    int v[SCAN_ITEMS];
+    int sum = 0;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        v[k] = (i0 + k < n) ? cnt[i0 + k] : 0;
+        sum += v[k];
+    }
+    int tot;
+    int ex = block_excl_scan(sum, &tot);
+    if (threadIdx.x == 0) {
+        int excl = 0;
+        if (b == 0) {
+            atomicExch(&status[0], lb_word(epoch, LB_INCL, tot));
+        } else {
+            atomicExch(&status[b], lb_word(epoch, LB_AGG, tot));
+            int j = b - 1, spins = 0;
+            while (j >= 0) {
+                const unsigned long long w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                if ((uint32_t)(w >> 34) != epoch) {
+                    if (++spins > LB_SPIN_MAX) {  // a predecessor never published: report, do not hang
+                        atomicAdd(&S->counters[CNT_TILE_OVF], 1ull);
+                        break;
+                    }
+                    continue;
+                }
+                excl += (int32_t)(uint32_t)w;
+                if (((w >> 32) & 3ull) == LB_INCL) break;
+                --j;
+            }
+            atomicExch(&status[b], lb_word(epoch, LB_INCL, excl + tot));
+        }
+        s_excl = excl;
+    }
+    __syncthreads();
+    ex += s_excl;
+#pragma unroll
+    for (int k = 0; k < SCAN_ITEMS; ++k) {
+        const int i = i0 + k;
+        if (i < n) {
+            start[i] = ex;
+            cnt[i] = 0;
+            tile_marks_after(i, ex, v[k], pt_g, tile_g);
+            tile_marks_after(i, ex, v[k], pt_c, tile_c);
+            if (v[k] > HEAVY_CELL) heavy[ex / pt_c] = 1;
+            if (i == n - 1) {
+                const int total = ex + v[k];
+                start[n] = total;
+                S->total = total;
+                S->n_sorted = total;
+                tile_g[0] = 0;
+                tile_c[0] = 0;
+                tile_g[total / pt_g + 1] = n;
+                tile_c[total / pt_c + 1] = n;
+            }
+        }
+        ex += v[k];
+    }
+}
+
+void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long long* status,
+                       unsigned long long* ticket, uint32_t epoch, DevScalars* S, int pt_g, int32_t* tile_g,
+                       int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st) {
+    k_scan_cells<<<scan_cells_blocks(n), BLOCK, 0, st>>>(cnt, start, n, status, ticket, epoch, S, pt_g, tile_g, pt_c,
+                                                          tile_c, heavy);
+}
+
+int scan_cells_blocks(int32_t n) { return n > 0 ? (n + SCAN_TILE - 1) / SCAN_TILE : 1; }
+
+// ---------------------------------------------------------------------------
 // generic fills (graph-safe replacements for hipMemsetAsync)
 // ---------------------------------------------------------------------------
 __global__ void k_fill_i32(int32_t* p, int32_t v, int64_t n) {
@@ -1626,7 +1738,6 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
 // tilings from one pass: the gather's (pt_g parcels) and the collide's (pt_c).
 // Also flags the collide tiles holding a cell of more than HEAVY_CELL parcels
 // (heavy[] cleared beforehand).
-#define HEAVY_CELL 128
 __device__ __forceinline__ void tile_marks(const int32_t* __restrict__ start, int i, int nc, int pt,
                                            int32_t* __restrict__ tile_begin, int* q0_out) {
     const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
