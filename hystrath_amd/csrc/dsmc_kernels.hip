@@ -236,29 +236,41 @@ __global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt,
     }
     int tot;
     int ex = block_excl_scan(sum, &tot);
-    if (threadIdx.x == 0) {
-        int excl = 0;
-        if (b == 0) {
-            atomicExch(&status[0], lb_word(epoch, LB_INCL, tot));
-        } else {
-            atomicExch(&status[b], lb_word(epoch, LB_AGG, tot));
-            int j = b - 1, spins = 0;
-            while (j >= 0) {
-                const unsigned long long w = __hip_atomic_load(&status[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                if ((uint32_t)(w >> 34) != epoch) {
-                    if (++spins > LB_SPIN_MAX) {  // a predecessor never published: report, do not hang
-                        atomicAdd(&S->counters[CNT_TILE_OVF], 1ull);
-                        break;
-                    }
-                    continue;
+    if (threadIdx.x < WAVE) {
+        // wave 0 looks back over 64 predecessors at a time: the nearest one
+        // with its inclusive prefix ends the walk (blocks below 0 read as an
+        // inclusive prefix of 0)
+        const int lane = threadIdx.x;
+        if (lane == 0) atomicExch(&status[b], lb_word(epoch, b == 0 ? LB_INCL : LB_AGG, tot));
+        int excl = 0, j = b - 1, spins = 0;
+        while (j >= 0) {
+            const int jj = j - lane;
+            const unsigned long long w = jj >= 0 ? __hip_atomic_load(&status[jj], __ATOMIC_RELAXED,
+                                                                     __HIP_MEMORY_SCOPE_AGENT)
+                                                 : lb_word(epoch, LB_INCL, 0);
+            const bool ready = (uint32_t)(w >> 34) == epoch;
+            const bool incl = ready && ((w >> 32) & 3ull) == LB_INCL;
+            const unsigned long long im = __ballot(incl), nr = __ballot(!ready);
+            const int fi = im ? __ffsll((long long)im) - 1 : WAVE;
+            const unsigned long long need = fi >= WAVE - 1 ? ~0ull : ((2ull << fi) - 1ull);
+            if (nr & need) {
+                if (++spins > LB_SPIN_MAX) {  // a predecessor never published: report, do not hang
+                    if (lane == 0) atomicAdd(&S->counters[CNT_TILE_OVF], 1ull);
+                    break;
                 }
-                excl += (int32_t)(uint32_t)w;
-                if (((w >> 32) & 3ull) == LB_INCL) break;
-                --j;
+                continue;
             }
-            atomicExch(&status[b], lb_word(epoch, LB_INCL, excl + tot));
+            int val = lane <= fi ? (int32_t)(uint32_t)w : 0;
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) val += __shfl_xor(val, o, WAVE);
+            excl += val;
+            if (fi < WAVE) break;
+            j -= WAVE;
         }
-        s_excl = excl;
+        if (lane == 0) {
+            if (b > 0) atomicExch(&status[b], lb_word(epoch, LB_INCL, excl + tot));
+            s_excl = excl;
+        }
     }
     __syncthreads();
     ex += s_excl;
