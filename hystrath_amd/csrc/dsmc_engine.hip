@@ -906,6 +906,7 @@ extern "C" int hs_dsmc_move(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
     enqueue_move(eng, 1, 0);
+    launch_clear_queue(eng->d_S, eng->stream);  // no commit follows a lone move
     return check_launch(eng, "move");
 }
 

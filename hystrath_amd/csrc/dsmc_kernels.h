@@ -30,8 +30,7 @@ struct DevScalars {
     int32_t n_sorted; // population after the cell scan (published by commit)
     int32_t n_next;   // n_sorted plus the next step's inflow when injected ahead (installed by commit)
     int32_t ord[2];   // heavy / light tile counters of the collide order (cleared by commit)
-    int32_t qn;       // flights queued for the wall pass of the move (cleared by that pass)
-    int32_t qdone;    // workgroups of the wall pass that finished
+    int32_t qn;       // flights queued for the wall pass of the move (cleared by commit)
     unsigned long long counters[CNT_N];
 };
 
@@ -85,6 +84,7 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                     int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st);
+void launch_clear_queue(DevScalars* S, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
                        int32_t* cnt, int nc, int ns, hipStream_t st);

@@ -805,8 +805,8 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
 }
 
 // The queued flights: the wall interaction they stopped at, the rest of the
-// flight (every further wall handled here), the cell histogram.  The last
-// workgroup to finish clears the queue for the next step.
+// flight (every further wall handled here), the cell histogram.  The queue is
+// cleared afterwards by k_commit (or k_clear_queue).
 template <bool TRACK>
 __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict__ Pp, DevParticles b,
                                                      DevScalars* __restrict__ S, int32_t* __restrict__ count,
@@ -847,16 +847,10 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
     add_counter(S, CNT_DEL, mc.del);
     add_counter(S, CNT_WALL, mc.wall);
     add_counter(S, CNT_TRACK_OVF, mc.ovf);
-    // the last workgroup clears the queue
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        __threadfence();
-        if (atomicAdd(&S->qdone, 1) == (int)gridDim.x - 1) {
-            S->qn = 0;
-            S->qdone = 0;
-        }
-    }
 }
+
+__global__ void k_clear_queue(DevScalars* S) { S->qn = 0; }
+void launch_clear_queue(DevScalars* S, hipStream_t st) { k_clear_queue<<<1, 1, 0, st>>>(S); }
 
 void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* count, int32_t* slot,
                  const WallQueue& Q, int do_move, int do_hist, int track, int grid, hipStream_t st) {
@@ -1851,6 +1845,7 @@ __global__ void k_commit(DevScalars* S, int advance, int with_next) {
     S->n_old = S->n_sorted;
     S->ord[0] = 0;
     S->ord[1] = 0;
+    S->qn = 0;  // the move's wall queue, consumed by k_move_wall
     if (advance) S->step += 1;
 }
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st) {
