@@ -1242,16 +1242,30 @@ __device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, d
 }
 
 // Rank of key k among key[ss..ee) (LDS, 16-byte aligned array, reads may run
-// up to 3 entries past ee inside the array): 4 keys per 128-bit read.
+// up to 3 entries past ee inside the array): 4 keys per 128-bit read.  Only
+// the first and last 4-key windows can hold keys outside [ss, ee); the
+// windows between are compared without bounds tests.
 __device__ __forceinline__ int rank_in(const int32_t* key, int ss, int ee, int k) {
+    const int q0 = ss & ~3, q1 = (ee - 1) & ~3;  // first and last window
     int r = 0;
+    {
+        const int4 v = *reinterpret_cast<const int4*>(key + q0);
+        r += (q0 >= ss && q0 < ee && v.x < k) ? 1 : 0;
+        r += (q0 + 1 >= ss && q0 + 1 < ee && v.y < k) ? 1 : 0;
+        r += (q0 + 2 >= ss && q0 + 2 < ee && v.z < k) ? 1 : 0;
+        r += (q0 + 3 >= ss && q0 + 3 < ee && v.w < k) ? 1 : 0;
+    }
 #pragma unroll 4
-    for (int q = ss & ~3; q < ee; q += 4) {
+    for (int q = q0 + 4; q < q1; q += 4) {
         const int4 v = *reinterpret_cast<const int4*>(key + q);
-        r += (q >= ss && v.x < k) ? 1 : 0;
-        r += (q + 1 >= ss && q + 1 < ee && v.y < k) ? 1 : 0;
-        r += (q + 2 >= ss && q + 2 < ee && v.z < k) ? 1 : 0;
-        r += (q + 3 >= ss && q + 3 < ee && v.w < k) ? 1 : 0;
+        r += (v.x < k ? 1 : 0) + (v.y < k ? 1 : 0) + (v.z < k ? 1 : 0) + (v.w < k ? 1 : 0);
+    }
+    if (q1 > q0) {
+        const int4 v = *reinterpret_cast<const int4*>(key + q1);
+        r += (v.x < k) ? 1 : 0;
+        r += (q1 + 1 < ee && v.y < k) ? 1 : 0;
+        r += (q1 + 2 < ee && v.z < k) ? 1 : 0;
+        r += (q1 + 3 < ee && v.w < k) ? 1 : 0;
     }
     return r;
 }
