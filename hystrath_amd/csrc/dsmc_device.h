@@ -35,6 +35,7 @@ enum { CNT_CAND = 0, CNT_COLL = 1, CNT_DEL = 2, CNT_INJ = 3, CNT_WALL = 4,
 // read through uniform (scalar) loads.
 struct DevParams {
     int nx, ny, nz, nc;
+    double rnx, rnxy;  // 1/nx and 1/(nx*ny), for cell_ijk
     double origin[3], delta[3];
     int empty[3];
     double centre[3];
@@ -482,6 +483,25 @@ __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3],
 // ---------------------------------------------------------------------------
 // walls
 // ---------------------------------------------------------------------------
+// q = n / d and n - q*d for 0 <= n < 2^31, 0 < d < 2^31, from the double
+// reciprocal r = 1/d: the product n*r is within one of n/d (relative error
+// below 2^-51), so one correction step makes the quotient exact.
+__device__ __forceinline__ int div_exact(int n, int d, double r, int* rem) {
+    int q = (int)((double)n * r);
+    long long m = (long long)n - (long long)q * d;
+    if (m < 0) { q -= 1; m += d; }
+    else if (m >= d) { q += 1; m -= d; }
+    *rem = (int)m;
+    return q;
+}
+
+// The block indices (i, j, k) of cell c = i + nx*(j + ny*k).
+__device__ __forceinline__ void cell_ijk(const DevParams& P, int c, int ijk[3]) {
+    int r;
+    ijk[2] = div_exact(c, P.nx * P.ny, P.rnxy, &r);
+    ijk[1] = div_exact(r, P.nx, P.rnx, &ijk[0]);
+}
+
 __device__ __forceinline__ double face_coord(const DevParams& P, int a, int idx) {
     return P.origin[a] + (double)idx * P.delta[a];
 }

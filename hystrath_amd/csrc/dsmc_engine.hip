@@ -329,6 +329,8 @@ static void derive(hs_dsmc_engine* eng) {
     memset(&P, 0, sizeof(P));
     P.nx = c.nx; P.ny = c.ny; P.nz = c.nz;
     P.nc = eng->nc;
+    P.rnx = 1.0 / (double)c.nx;
+    P.rnxy = 1.0 / ((double)c.nx * (double)c.ny);
     const int nn[3] = { c.nx, c.ny, c.nz };
     for (int a = 0; a < 3; ++a) {
         P.origin[a] = c.origin[a];

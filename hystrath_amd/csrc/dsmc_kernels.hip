@@ -688,9 +688,7 @@ __device__ __forceinline__ void load_pu(const DevParticles& b, int64_t i, Flight
 }
 
 __device__ __forceinline__ void set_cell(const DevParams& P, int32_t c, Flight& s) {
-    s.idx[0] = c % P.nx;
-    s.idx[1] = (c / P.nx) % P.ny;
-    s.idx[2] = c / (P.nx * P.ny);
+    cell_ijk(P, c, s.idx);
     s.keep = true;
     s.vch = false;
 }
@@ -1234,7 +1232,9 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
 }
 
 __device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, double y, double z) {
-    const int ci = c % P.nx, cj = (c / P.nx) % P.ny, ck = c / (P.nx * P.ny);
+    int ijk[3];
+    cell_ijk(P, c, ijk);
+    const int ci = ijk[0], cj = ijk[1], ck = ijk[2];
     const double cc0 = P.origin[0] + ((double)ci + 0.5) * P.delta[0];
     const double cc1 = P.origin[1] + ((double)cj + 0.5) * P.delta[1];
     const double cc2 = P.origin[2] + ((double)ck + 0.5) * P.delta[2];
@@ -1902,7 +1902,8 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gen(const DevParams* __restrict_
     const int ni = cnt[e];
     if (ni == 0) return;
     const int c = e / ns, s = e - c * ns;
-    const int idx[3] = { c % P.nx, (c / P.nx) % P.ny, c / (P.nx * P.ny) };
+    int idx[3];
+    cell_ijk(P, c, idx);
     PhiloxStream r;
     r.init(P, RNG_FILL_GEN, (uint32_t)c, fill_id, (uint32_t)s);
     const double sc = sqrt(HS_KB * T / P.mass[s]);

@@ -87,3 +87,20 @@ def test_quotient_below_one_iff_dividend_below_divisor():
             l = np.maximum(n_ / d_, 0.0)
             fast = (n_ < d_) if sgn > 0 else (n_ > d_)
             assert np.array_equal(l < 1.0, fast)
+
+
+def test_reciprocal_division_is_exact():
+    """The kernels split a cell index c = i + nx*(j + ny*k) with q = trunc(n*r),
+    r = fl(1/d), plus one correction step (div_exact): exact integer division
+    for 0 <= n < 2^31, 0 < d < 2^31, checked on random and boundary values
+    (multiples of d and their neighbours) for small and large divisors."""
+    rng = np.random.default_rng(3)
+    for trial in range(60):
+        d = int(rng.integers(1, 2**31)) if trial % 2 else int(rng.integers(1, 5000))
+        mult = np.arange(0, 2**31 // d, max(1, (2**31 // d) // 20000), dtype=np.int64)[:20000] * d
+        n = np.concatenate([rng.integers(0, 2**31, 20000), mult])
+        n = np.concatenate([n, np.maximum(n - 1, 0), np.minimum(n + 1, 2**31 - 1)]).astype(np.int64)
+        q = np.trunc(n.astype(np.float64) * (1.0 / d)).astype(np.int64)
+        m = n - q * d
+        q = np.where(m < 0, q - 1, np.where(m >= d, q + 1, q))
+        assert np.array_equal(q, n // d)
