@@ -192,7 +192,7 @@ void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, Scan
 // block's aggregate, 2 = the inclusive prefix through the block.  A word from
 // an earlier epoch (an earlier scan) reads as not ready, so nothing is cleared
 // between scans.
-#define HEAVY_CELL 128   // parcels: a cell this full marks its collide tile heavy
+#define HEAVY_CELL 96   // parcels: a cell this full marks its collide tile heavy
 #define LB_AGG 1ull
 #define LB_INCL 2ull
 #define LB_SPIN_MAX (1 << 24)
