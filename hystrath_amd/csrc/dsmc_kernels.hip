@@ -1664,7 +1664,7 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
             for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
             if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
         }
-#pragma unroll 4
+#pragma unroll 8
         for (int j = t; j < cnt; j += CT) {
             const int64_t gj = (int64_t)S0 + j;
             L.vx[j] = dst.ux[gj]; L.vy[j] = dst.uy[gj]; L.vz[j] = dst.uz[gj];
