@@ -863,7 +863,8 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     {
         StageTimer tm(eng, T_SCATTER);
-        launch_scatter(cur_buf(eng), eng->d_S, eng->d_start, eng->d_slot, eng->d_perm, eng->move_grid, eng->stream);
+        launch_scatter(cur_buf(eng), eng->d_S, eng->d_start, eng->d_slot, eng->d_perm, eng->d_tile_heavy,
+                       eng->collide_parcels, eng->d_tile_order, eng->move_grid, eng->stream);
     }
     {
         StageTimer tm(eng, T_GATHER);

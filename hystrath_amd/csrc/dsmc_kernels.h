@@ -65,8 +65,8 @@ void launch_move(const DevParams* P, DevParticles b, DevScalars* S, int32_t* cou
                  const WallQueue& Q, int do_move, int do_hist, int track, int grid, hipStream_t st);
 void launch_set_nold(DevScalars* S, hipStream_t st);
 int move_blocks_per_cu(int track);
-void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
-                    int32_t* perm, int grid, hipStream_t st);
+void launch_scatter(DevParticles b, DevScalars* S, const int32_t* start, const int32_t* slot, int32_t* perm,
+                    int32_t* heavy, int pt, int32_t* order, int grid, hipStream_t st);
 // particle-balanced tiles of pt parcels (tile_begin) and their heavy flags
 // (clear before the first use; the collide order clears the flags it reads)
 int scan_cells_blocks(int32_t n);

@@ -877,19 +877,33 @@ void launch_set_nold(DevScalars* S, hipStream_t st) { k_set_nold<<<1, 1, 0, st>>
 // ---------------------------------------------------------------------------
 // scatter: perm[start[cell] + slot] = old index
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ cell, const DevScalars* __restrict__ S,
+// Also builds the order in which the collide workgroups take their tiles:
+// heavy tiles first (they run longest), the others after.  Tiles are
+// independent, so the order does not change any result.  heavy[] comes from
+// the cell scan and is cleared here.
+__global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ cell, DevScalars* __restrict__ S,
                                                    const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ slot, int32_t* __restrict__ perm) {
+                                                   const int32_t* __restrict__ slot, int32_t* __restrict__ perm,
+                                                   int32_t* __restrict__ heavy, int pt, int32_t* __restrict__ order) {
     const int64_t n = S->n;  // pre-sort population (the cell scan writes n_sorted)
+    const int nt = (int)((int64_t)S->n_sorted / pt) + 1;
+    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nt; q += gridDim.x * BLOCK) {
+        if (heavy[q]) {
+            heavy[q] = 0;
+            order[atomicAdd(&S->ord[0], 1)] = q;
+        } else {
+            order[nt - 1 - atomicAdd(&S->ord[1], 1)] = q;
+        }
+    }
     for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
         const int32_t c = cell[i];
         if (c >= 0) perm[start[c] + slot[i]] = (int32_t)i;
     }
 }
 
-void launch_scatter(DevParticles b, const DevScalars* S, const int32_t* start, const int32_t* slot,
-                    int32_t* perm, int grid, hipStream_t st) {
-    k_scatter<<<grid, BLOCK, 0, st>>>(b.cell, S, start, slot, perm);
+void launch_scatter(DevParticles b, DevScalars* S, const int32_t* start, const int32_t* slot, int32_t* perm,
+                    int32_t* heavy, int pt, int32_t* order, int grid, hipStream_t st) {
+    k_scatter<<<grid, BLOCK, 0, st>>>(b.cell, S, start, slot, perm, heavy, pt, order);
 }
 
 // ---------------------------------------------------------------------------
@@ -1778,22 +1792,6 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ sta
     if (start[i] - start[i - 1] > HEAVY_CELL) heavy[qc] = 1;
 }
 
-// Order in which the collide workgroups take the tiles: heavy tiles first
-// (they run longest), the others after.  Tiles are independent, so the order
-// does not change any result.
-__global__ __launch_bounds__(BLOCK) void k_tile_order(int32_t* __restrict__ heavy, int pt, DevScalars* __restrict__ S,
-                                                      int32_t* __restrict__ order) {
-    const int q = blockIdx.x * BLOCK + threadIdx.x;
-    const int nt = (int)((int64_t)S->n_sorted / pt) + 1;
-    if (q >= nt) return;
-    if (heavy[q]) {
-        heavy[q] = 0;
-        order[atomicAdd(&S->ord[0], 1)] = q;
-    } else {
-        order[nt - 1 - atomicAdd(&S->ord[1], 1)] = q;
-    }
-}
-
 template <bool GEN, int PMAX>
 __global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
                                                    const int32_t* __restrict__ start,
@@ -1843,7 +1841,7 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st) {
     const int nt = (int)(capacity / pt) + 2;
-    k_tile_order<<<(nt + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(heavy, pt, S, order);
+    (void)heavy;  // the order was built by k_scatter
     if (general)
         k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples,
                                                       vol, S, do_collide, do_sample, nc, big_list);
