@@ -1363,7 +1363,7 @@ __device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t*
 template <bool GEN, int PMAX>
 struct TileLds {
     alignas(16) int32_t key[PMAX];  // old index of each slot; then the NTC conflict stamps
-    uint8_t loc[PMAX];       // sub-tile cell of each slot
+    alignas(16) uint8_t loc[PMAX];  // sub-tile cell of each slot (filled 4 bytes at a time)
     alignas(16) uint8_t sub[PMAX];
     uint16_t slist[PMAX];    // cells of more than 64 parcels: their parcels by sub-cell, ascending
     float vx[PMAX], vy[PMAX], vz[PMAX];
@@ -1673,7 +1673,15 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
                 pf_rem = rem[c0 + t];
                 pf_vol = vol[c0 + t];
             }
-            for (int j = L.segS[t] - S0; j < L.segS[t + 1] - S0; ++j) L.loc[j] = (uint8_t)t;
+            {
+                // the cell's slot range, four bytes per store where it covers whole words
+                int j = L.segS[t] - S0;
+                const int e = L.segS[t + 1] - S0;
+                for (; j < e && (j & 3); ++j) L.loc[j] = (uint8_t)t;
+                const uint32_t w4 = 0x01010101u * (uint32_t)t;
+                for (; j + 4 <= e; j += 4) *reinterpret_cast<uint32_t*>(&L.loc[j]) = w4;
+                for (; j < e; ++j) L.loc[j] = (uint8_t)t;
+            }
 #pragma unroll
             for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
             if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
