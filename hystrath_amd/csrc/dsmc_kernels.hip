@@ -1284,42 +1284,6 @@ __device__ __forceinline__ int rank_in(const int32_t* key, int ss, int ee, int k
     return r;
 }
 
-// 0x80 in byte i when lo <= p + i < hi
-__device__ __forceinline__ uint32_t byte_window(int p, int lo, int hi) {
-    uint32_t m = 0;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) m |= (p + i >= lo && p + i < hi) ? (0x80u << (8 * i)) : 0u;
-    return m;
-}
-
-// Sub-cell statistics of slot j (sub-cell q) in cell [ss, ee): parcels of
-// lower sub-cells, of sub-cell q, and of sub-cell q before j.  16 sub-cell
-// bytes per 128-bit read; bytes are 0..7, so per-byte adds never carry:
-// (b + 0x7F) sets bit 7 iff b != 0, (b + 0x80 - q) sets it iff b >= q.
-__device__ __forceinline__ void subcell_rank(const uint8_t* sub, int ss, int ee, int j, int q, int& before, int& tot,
-                                             int& rnk) {
-    const uint32_t qq = (uint32_t)q * 0x01010101u;
-    const uint32_t ge_add = (0x80u - (uint32_t)q) * 0x01010101u;
-    before = 0; tot = 0; rnk = 0;
-    for (int p0 = ss & ~15; p0 < ee; p0 += 16) {
-        const uint4 v = *reinterpret_cast<const uint4*>(sub + p0);
-        const uint32_t w4[4] = { v.x, v.y, v.z, v.w };
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int p = p0 + 4 * i;
-            const uint32_t w = w4[i];
-            const uint32_t in_cell = byte_window(p, ss, ee);
-            const uint32_t in_pre = byte_window(p, ss, j);
-            const uint32_t ne = ((w ^ qq) + 0x7F7F7F7Fu) & 0x80808080u;  // byte != q
-            const uint32_t ge = (w + ge_add) & 0x80808080u;               // byte >= q
-            const uint32_t eq = ~ne & 0x80808080u;
-            before += __popc(in_cell & ~ge);
-            tot += __popc(in_cell & eq);
-            rnk += __popc(in_pre & eq);
-        }
-    }
-}
-
 // Sub-cell member lists of one cell (ss, nC) by a whole wave: counts and
 // ascending fills from ballots over 64-parcel windows; tab[q] gets
 // (offset | count << 16) of sub-cell q.
