@@ -26,3 +26,48 @@ cells = np.array(cells, np.int16)
 out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "couette_positions.npz")
 np.savez_compressed(out, xyz=xyz, cell=cells)
 print(f"{len(cells)} parcels -> {out}")
+
+# The first HEAD parcels' velocity, rotational energy, species and vibrational
+# level (U, ERot, typeId, vibLevel of the same snapshot), parsed here with
+# plain regular expressions, independently of hystrath_amd.lagrangian.  U and
+# ERot are kept in single precision, as the engine holds them.
+HEAD = 8192
+d = os.path.dirname(path)
+
+
+def entries(name):
+    txt = open(os.path.join(d, name)).read()
+    body = txt[txt.index("\n(", txt.index("FoamFile")) + 2:]
+    return body.split("\n")
+
+
+U = []
+for line in entries("U"):
+    m = re.match(r"^\(([-0-9.eE+]+) ([-0-9.eE+]+) ([-0-9.eE+]+)\)\s*$", line)
+    if m:
+        U.append([float(m.group(k)) for k in (1, 2, 3)])
+    if len(U) == HEAD:
+        break
+erot = []
+for line in entries("ERot"):
+    if re.match(r"^[-0-9.eE+]+\s*$", line):
+        erot.append(float(line))
+    if len(erot) == HEAD:
+        break
+tid = []
+for line in entries("typeId"):
+    if re.match(r"^\d+\s*$", line):
+        tid.append(int(line))
+    if len(tid) == HEAD:
+        break
+vib = []
+for line in entries("vibLevel"):
+    m = re.match(r"^1\((\d+)\)\s*$", line)
+    if m:
+        vib.append(int(m.group(1)))
+    if len(vib) == HEAD:
+        break
+out2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "couette_cloud_head.npz")
+np.savez_compressed(out2, xyz=xyz[:HEAD], cell=cells[:HEAD].astype(np.int32), U=np.array(U, np.float64).astype(np.float32),
+                    erot=np.array(erot, np.float64).astype(np.float32), type=np.array(tid, np.uint8), vib=np.array(vib, np.uint16))
+print(f"{HEAD} parcels with U, ERot, typeId, vibLevel -> {out2}")

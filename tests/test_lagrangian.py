@@ -1,0 +1,115 @@
+"""dsmcFoam+'s lagrangian cloud files (hystrath_amd.lagrangian): reading the
+couette_N2-O2 tutorial's own snapshot, exact write/read round trips, and (on a
+GPU) the engine evolving that snapshot in step with the oracle."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+import helpers as H
+from hystrath_amd import cases, lagrangian as lg
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+SNAP = "/root/reference/run/hyStrath/dsmcFoam+/couette_N2-O2/backup-5"
+
+
+@pytest.mark.skipif(not os.path.isdir(SNAP), reason="hyStrath tutorials not present")
+def test_reads_reference_couette_cloud():
+    """All 47,583 parcels of the tutorial's snapshot, checked against the
+    fixtures that tests/golden/make_couette_fixture.py parsed independently
+    (positions and cells of every parcel; U, ERot, typeId and vibLevel of the
+    first 8,192), plus what the case fixes: two species in the ratio of
+    dsmcInitialiseDict's number densities, one vibrational mode, and a gas
+    temperature between the 2000 K and 3000 K walls."""
+    c = lg.read_cloud(SNAP, vib_modes=1)
+    n = len(c["px"])
+    assert n == 47583
+    g = np.load(os.path.join(GOLDEN, "couette_positions.npz"))
+    np.testing.assert_array_equal(np.stack([c["px"], c["py"], c["pz"]], 1), g["xyz"])
+    np.testing.assert_array_equal(c["cell"], g["cell"])
+    h = np.load(os.path.join(GOLDEN, "couette_cloud_head.npz"))
+    k = len(h["type"])
+    np.testing.assert_array_equal(np.stack([c["ux"], c["uy"], c["uz"]], 1)[:k], h["U"])
+    np.testing.assert_array_equal(c["erot"][:k], h["erot"])
+    np.testing.assert_array_equal(c["type"][:k], h["type"])
+    np.testing.assert_array_equal(c["vib"][0, :k], h["vib"])
+    frac_n2 = 3.2320494711701479e19 / (3.2320494711701479e19 + 8.591523910705456e18)
+    assert abs((c["type"] == 0).mean() - frac_n2) < 0.01
+    m = np.array([46.5e-27, 53.12e-27])[c["type"]]
+    u2 = c["ux"].astype(float) ** 2 + c["uy"].astype(float) ** 2 + c["uz"].astype(float) ** 2
+    T = (2.0 / 3.0) * (0.5 * m * u2).mean() / 1.380649e-23
+    assert 2000.0 < T < 3000.0
+
+
+def test_cloud_round_trip_is_exact(tmp_path):
+    rng = np.random.default_rng(4)
+    n = 500
+    c = dict(px=rng.uniform(0, 0.05, n), py=rng.uniform(0, 1, n), pz=np.full(n, 0.005),
+             cell=rng.integers(0, 500, n).astype(np.int32),
+             ux=rng.normal(0, 900, n).astype(np.float32), uy=rng.normal(0, 900, n).astype(np.float32),
+             uz=rng.normal(0, 900, n).astype(np.float32), erot=rng.exponential(3e-20, n).astype(np.float32),
+             type=rng.integers(0, 2, n).astype(np.uint8), vib=rng.integers(0, 5, (2, n)).astype(np.uint16))
+    lg.write_cloud(str(tmp_path / "0.001"), c)
+    d = lg.read_cloud(str(tmp_path / "0.001"), vib_modes=2)
+    for k in c:
+        np.testing.assert_array_equal(d[k], c[k], err_msg=k)
+    # OpenFOAM's bookkeeping fields are present; newParcel is a uniform list
+    base = tmp_path / "0.001" / "lagrangian" / "dsmc"
+    for f in ("newParcel", "classification", "origProcId", "origId"):
+        assert (base / f).exists()
+    np.testing.assert_array_equal(lg.read_scalars(str(base / "newParcel")), np.full(n, -1.0))
+    np.testing.assert_array_equal(lg.read_scalars(str(base / "origId")), np.arange(n))
+
+
+def test_uniform_and_nested_lists(tmp_path):
+    """The list forms OpenFOAM writes: uniform "N{v}" for scalars, vectors and
+    label lists, and per-parcel label lists "k(...)" of different lengths."""
+    hdr = "FoamFile\n{\n    version 2.0;\n    format ascii;\n}\n// comment\n"
+    (tmp_path / "s").write_text(hdr + "4{2.5}\n")
+    (tmp_path / "v").write_text(hdr + "3{(1 -2 3.5)}\n")
+    (tmp_path / "l").write_text(hdr + "3\n(\n2(4 5)\n1(7)\n0()\n)\n")
+    (tmp_path / "u").write_text(hdr + "2{1(3)}\n")
+    np.testing.assert_array_equal(lg.read_scalars(str(tmp_path / "s")), [2.5] * 4)
+    np.testing.assert_array_equal(lg.read_vectors(str(tmp_path / "v")), [[1, -2, 3.5]] * 3)
+    np.testing.assert_array_equal(lg.read_label_lists(str(tmp_path / "l"), 2), [[4, 5], [7, 0], [0, 0]])
+    np.testing.assert_array_equal(lg.read_label_lists(str(tmp_path / "u"), 1), [[3], [3]])
+
+
+def _snapshot_parcels():
+    """The first 8,192 parcels of the tutorial's snapshot, from the fixture."""
+    h = np.load(os.path.join(GOLDEN, "couette_cloud_head.npz"))
+    xyz = h["xyz"]
+    return dict(px=xyz[:, 0].copy(), py=xyz[:, 1].copy(), pz=xyz[:, 2].copy(), cell=h["cell"].astype(np.int32),
+                ux=h["U"][:, 0].copy(), uy=h["U"][:, 1].copy(), uz=h["U"][:, 2].copy(),
+                erot=h["erot"].copy(), type=h["type"].copy(), vib=h["vib"].reshape(1, -1).copy())
+
+
+@pytest.mark.gpu
+def test_engine_evolves_reference_snapshot_like_the_oracle(tmp_path):
+    """Parity on the reference's own state: the tutorial's parcels (LB-VHS
+    N2/O2 between diffuse walls, cyclic in x) uploaded to the engine and the
+    oracle, five steps each, bit-identical; then the engine's cloud written in
+    the reference's format and read back unchanged."""
+    from hystrath_amd.engine import Engine
+    case = cases.couette_n2o2(capacity=20000)
+    d = _snapshot_parcels()
+    with Engine(case) as eng:
+        st = ob.OracleState(case)
+        # sigmaTcRMax of the order of pi d^2 c_r at 2500 K, so that candidates are drawn
+        sigma = np.full(case.ncells, 1.0e-15)
+        rem = np.zeros(case.ncells)
+        eng.upload(d)
+        eng.set_cell_state(sigma, rem)
+        st.load(d, sigma=sigma, rem=rem)
+        for k in range(5):
+            eng.evolve(1)
+            st.evolve(1)
+            H.assert_same_particles(eng.download(), st.particles(), f"snapshot/step{k}")
+        np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+        assert eng.counters()["collisions"] > 1000
+        got = eng.download()
+        lg.write_cloud(str(tmp_path / "5.00005"), got)
+        back = lg.read_cloud(str(tmp_path / "5.00005"), vib_modes=1)
+        for k in ("px", "py", "pz", "cell", "ux", "uy", "uz", "erot", "type", "vib"):
+            np.testing.assert_array_equal(back[k], got[k], err_msg=k)
