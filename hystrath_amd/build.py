@@ -30,7 +30,7 @@ def hipcc():
 
 HOST = os.path.join(HERE, "host")
 BIN = os.path.join(HERE, "bin")
-HOST_LIB_SOURCES = ["dictionary.C", "dsmcCloud.C"]
+HOST_LIB_SOURCES = ["dictionary.C", "lagrangianIO.C", "dsmcCloud.C"]
 APPS = ["dsmcFoamPlus", "dsmcInitialisePlus"]
 
 
@@ -48,6 +48,9 @@ def build_host(verbose=False):
         outs.append((cmd, out))
     outs.append((common + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "dictCheck.C"), "-o",
                            os.path.join(BIN, "dictCheck")], os.path.join(BIN, "dictCheck")))
+    outs.append((common + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "lagrangianIO.C"),
+                           os.path.join(HOST, "cloudCheck.C"), "-o", os.path.join(BIN, "cloudCheck")],
+                 os.path.join(BIN, "cloudCheck")))
     for cmd, out in outs:
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -138,7 +138,7 @@ std::string patchType(const BlockMesh& m, const std::string& p) {
 
 void writeScalarField(const std::string& path, const std::string& time, const std::string& name,
                       const std::string& dims, const std::vector<double>& v, const BlockMesh& m, int prec,
-                      const PatchValues* bv = nullptr) {
+                      const PatchValues* bv = nullptr, const char* wallType = "calculated") {
     std::ofstream o(path);
     if (!o) throw FatalError("cannot write " + path);
     writeHeader(o, "volScalarField", time, name);
@@ -147,7 +147,8 @@ void writeScalarField(const std::string& path, const std::string& time, const st
     for (double x : v) o << x << "\n";
     o << ")\n;\n\nboundaryField\n{\n";
     for (const auto& p : m.patchNames) {
-        const std::string type = patchType(m, p);
+        std::string type = patchType(m, p);
+        if (type == "calculated") type = wallType;
         o << "    " << p << "\n    {\n        type            " << type << ";\n";
         if (type == "calculated") {
             auto it = bv ? bv->find(p) : PatchValues::const_iterator();
@@ -372,8 +373,9 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
     }
 
     // capacity: the state to restart from, or the fill of dsmcInitialiseDict
-    double expect = 0.0;
-    std::string state;
+    double expect = 0.0, cloudTime = 0.0;
+    std::string state, cloudDir;
+    LagrangianCloud cloud;
     if (initialise) {
         const dictionary ini = readDictionary(case_ + "/system/dsmcInitialiseDict");
         double vtot = 0.0;
@@ -385,16 +387,28 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
             for (const auto& k : nd.toc()) expect += nd.scalar(k) * vtot / cfg_.fn;
         }
     } else {
-        state = latestTimeWithState();
-        if (state.empty())
+        // the latest time holding an engine checkpoint or a lagrangian cloud
+        // (OpenFOAM's files, e.g. a dsmcFoam+ run's); at one time the
+        // checkpoint wins, as it also carries remainders and samples
+        double ts = 0.0;
+        state = latestTimeWith("/lagrangian/dsmc/hystrath.state", ts);
+        cloudDir = latestTimeWith("/lagrangian/dsmc/positions", cloudTime);
+        if (!cloudDir.empty() && (state.empty() || cloudTime > ts)) {
+            state.clear();
+            cloud = readLagrangianCloud(cloudDir, vibModes(), (int64_t)vol_.size(), cfg_.n_species);
+            expect = (double)cloud.size();
+        } else if (!state.empty()) {
+            cloudDir.clear();
+            std::ifstream f(state + "/lagrangian/dsmc/hystrath.state", std::ios::binary);
+            char magic[8];
+            int64_t n = 0;
+            f.read(magic, 8);
+            f.read((char*)&n, 8);
+            if (!f || std::memcmp(magic, kMagic, 8) != 0) throw FatalError("not an engine checkpoint: " + state);
+            expect = (double)n;
+        } else {
             throw FatalError("no initial parcel state under " + case_ + ": run dsmcInitialisePlus first");
-        std::ifstream f(state + "/lagrangian/dsmc/hystrath.state", std::ios::binary);
-        char magic[8];
-        int64_t n = 0;
-        f.read(magic, 8);
-        f.read((char*)&n, 8);
-        if (!f || std::memcmp(magic, kMagic, 8) != 0) throw FatalError("not an engine checkpoint: " + state);
-        expect = (double)n;
+        }
     }
     const double def = std::max(2.0 * expect + 65536.0, 100000.0);
     cfg_.capacity = (int64_t)std::min(def, 2.0e9);
@@ -402,7 +416,8 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
         throw FatalError(std::string("hs_dsmc_create: ") + hs_dsmc_create_error() + " (" + std::to_string(rc) + ")");
     }
     time_ = startTime_;
-    if (!initialise) readCheckpoint(state + "/lagrangian/dsmc/hystrath.state");
+    if (!initialise && !state.empty()) readCheckpoint(state + "/lagrangian/dsmc/hystrath.state");
+    if (!initialise && !cloudDir.empty()) readLagrangian(cloudDir, cloudTime, cloud);
 }
 
 dsmcCloud::~dsmcCloud() {
@@ -941,29 +956,84 @@ void dsmcCloud::write() {
         }
         if (fm.resetAtOutput && time_ < fm.resetAtOutputUntilTime) reset = true;
     }
+    writeLagrangian(dir, tn);
     // checkpoint before the reset so a restart continues the average
     writeCheckpoint(dir + "/lagrangian/dsmc/hystrath.state");
     if (reset) check(hs_dsmc_reset_samples(eng_), "reset samples");
 }
 
-void dsmcCloud::writeCheckpoint(const std::string& path) const {
-    int64_t n = nParticles();
-    const int ns = cfg_.n_species;
+int dsmcCloud::vibModes() const {
     int modes = 0;
-    bool rot = false;
-    for (int s = 0; s < ns; ++s) {
-        modes = std::max(modes, cfg_.species[s].n_vib_modes);
-        rot = rot || cfg_.species[s].rot_dof > 0;
-    }
-    std::vector<double> px(n), py(n), pz(n);
-    std::vector<float> ux(n), uy(n), uz(n), er(rot ? n : 0);
-    std::vector<int32_t> cell(n);
-    std::vector<uint16_t> vib((size_t)modes * n);
-    std::vector<uint8_t> type(ns > 1 ? n : 0);
-    hs_particles p{ px.data(), py.data(), pz.data(), ux.data(), uy.data(), uz.data(), cell.data(),
-                    rot ? er.data() : nullptr, modes ? vib.data() : nullptr, ns > 1 ? type.data() : nullptr };
+    for (int s = 0; s < cfg_.n_species; ++s) modes = std::max(modes, cfg_.species[s].n_vib_modes);
+    return modes;
+}
+
+bool dsmcCloud::rotational() const {
+    for (int s = 0; s < cfg_.n_species; ++s)
+        if (cfg_.species[s].rot_dof > 0) return true;
+    return false;
+}
+
+LagrangianCloud dsmcCloud::downloadCloud() const {
+    const int64_t n = nParticles();
+    const int ns = cfg_.n_species, modes = vibModes();
+    const bool rot = rotational();
+    LagrangianCloud c;
+    c.resize(n, modes);
+    hs_particles p{ c.px.data(), c.py.data(), c.pz.data(), c.ux.data(), c.uy.data(), c.uz.data(), c.cell.data(),
+                    rot ? c.erot.data() : nullptr, modes ? c.vib.data() : nullptr, ns > 1 ? c.type.data() : nullptr };
     int64_t got = 0;
     check(hs_dsmc_download_particles(eng_, &p, n, &got), "download particles");
+    return c;
+}
+
+void dsmcCloud::writeLagrangian(const std::string& timeDir, const std::string& timeName) const {
+    writeLagrangianCloud(timeDir, timeName, downloadCloud());
+    std::vector<double> sig(vol_.size()), rem(vol_.size());
+    check(hs_dsmc_get_cell_state(eng_, sig.data(), rem.data()), "cell state");
+    writeScalarField(timeDir + "/dsmcSigmaTcRMax", timeName, "dsmcSigmaTcRMax", "[0 3 -1 0 0 0 0]", sig, mesh_,
+                     precision_, nullptr, "zeroGradient");
+}
+
+void dsmcCloud::readLagrangian(const std::string& timeDir, double t, LagrangianCloud& c) {
+    const size_t nc = vol_.size();
+    const std::string sigPath = timeDir + "/dsmcSigmaTcRMax";
+    if (!isFile(sigPath)) throw FatalError("cannot find file " + sigPath + " (sigmaTcRMax is read with the cloud)");
+    std::vector<double> sig = readVolScalarInternalField(sigPath, (int64_t)nc);
+    // collision-selection remainders start uniform in [0, 1) per cell
+    // (dsmcCloud.C:105-116), here from a splitmix64 sequence of seedNumber
+    std::vector<double> rem(nc);
+    uint64_t x = cfg_.seed ^ 0x6a09e667f3bcc909ull;
+    for (size_t k = 0; k < nc; ++k) {
+        uint64_t z = (x += 0x9e3779b97f4a7c15ull);
+        z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+        z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+        z ^= z >> 31;
+        rem[k] = (double)(z >> 11) * 0x1.0p-53;
+    }
+    const int ns = cfg_.n_species;
+    hs_particles p{ c.px.data(), c.py.data(), c.pz.data(), c.ux.data(), c.uy.data(), c.uz.data(), c.cell.data(),
+                    rotational() ? c.erot.data() : nullptr, c.modes ? c.vib.data() : nullptr,
+                    ns > 1 ? c.type.data() : nullptr };
+    check(hs_dsmc_upload_particles(eng_, &p, c.size()), "upload particles");
+    check(hs_dsmc_set_cell_state(eng_, sig.data(), rem.data()), "cell state");
+    check(hs_dsmc_set_step(eng_, (uint32_t)std::llround(t / cfg_.dt)), "step");
+    time_ = startTime_ = t;
+    stepIndex_ = 0;
+    std::cout << "Read " << c.size() << " parcels at time " << t << " from " << timeDir << "/lagrangian/dsmc"
+              << std::endl;
+}
+
+void dsmcCloud::writeCheckpoint(const std::string& path) const {
+    const LagrangianCloud c = downloadCloud();
+    const int64_t got = c.size();
+    const int ns = cfg_.n_species, modes = c.modes;
+    const bool rot = rotational();
+    const std::vector<double>&px = c.px, &py = c.py, &pz = c.pz;
+    const std::vector<float>&ux = c.ux, &uy = c.uy, &uz = c.uz, &er = c.erot;
+    const std::vector<int32_t>& cell = c.cell;
+    const std::vector<uint16_t>& vib = c.vib;
+    const std::vector<uint8_t>& type = c.type;
     const size_t nc = vol_.size();
     std::vector<double> sig(nc), rem(nc);
     check(hs_dsmc_get_cell_state(eng_, sig.data(), rem.data()), "cell state");
@@ -1031,7 +1101,7 @@ void dsmcCloud::readCheckpoint(const std::string& path) {
     std::cout << "Read " << n << " parcels at time " << time_ << " from " << path << std::endl;
 }
 
-std::string dsmcCloud::latestTimeWithState() const {
+std::string dsmcCloud::latestTimeWith(const std::string& rel, double& t) const {
     DIR* d = opendir(case_.c_str());
     if (!d) throw FatalError("cannot open case directory " + case_);
     double best = -1e300;
@@ -1039,15 +1109,16 @@ std::string dsmcCloud::latestTimeWithState() const {
     while (dirent* e = readdir(d)) {
         const std::string name = e->d_name;
         char* end = nullptr;
-        const double t = std::strtod(name.c_str(), &end);
+        const double tn = std::strtod(name.c_str(), &end);
         if (name.empty() || *end != '\0') continue;
         const std::string p = case_ + "/" + name;
-        if (isFile(p + "/lagrangian/dsmc/hystrath.state") && t > best) {
-            best = t;
+        if (isFile(p + rel) && tn > best) {
+            best = tn;
             bestDir = p;
         }
     }
     closedir(d);
+    t = bestDir.empty() ? 0.0 : best;
     return bestDir;
 }
 

@@ -17,6 +17,7 @@
 
 #include "../../include/hystrath_dsmc.h"
 #include "dictionary.H"
+#include "lagrangianIO.H"
 
 namespace hs {
 
@@ -73,7 +74,14 @@ public:
 
     void writeCheckpoint(const std::string& path) const;
     void readCheckpoint(const std::string& path);
+    // the engine's parcels, and the lagrangian cloud files of <timeDir>
+    LagrangianCloud downloadCloud() const;
+    void writeLagrangian(const std::string& timeDir, const std::string& timeName) const;
+    // start from <timeDir>'s lagrangian cloud and dsmcSigmaTcRMax field
+    void readLagrangian(const std::string& timeDir, double t, LagrangianCloud& c);
     int64_t nParticles() const;
+    int vibModes() const;         // most vibrational modes of any species
+    bool rotational() const;      // any species with rotational DoF
 
     hs_dsmc_engine* engine() { return eng_; }
     const hs_config& config() const { return cfg_; }
@@ -87,7 +95,8 @@ private:
     void readBoundaries(const dictionary& d);
     void readFields(const dictionary& d);
     void check(int rc, const char* what) const;
-    std::string latestTimeWithState() const;
+    // latest time directory of the case holding <time>/<rel>, and its time
+    std::string latestTimeWith(const std::string& rel, double& t) const;
 
     std::string case_;
     hs_config cfg_{};

@@ -9,7 +9,7 @@ layout and written back from it:
     U               vectorField
     ERot            scalarField
     typeId          labelField (index into constant/dsmcProperties typeIdList)
-    vibLevel        labelListField: "k(l0 .. lk-1)" per parcel
+    vibLevel        labelFieldField: "k(l0 .. lk-1)" per parcel
     newParcel, classification, origProcId, origId   labelFields (written for
                     OpenFOAM's readers; newParcel -1: not injected this step)
 
@@ -104,7 +104,7 @@ def read_positions(path):
 
 
 def read_label_lists(path, width):
-    """A labelListField ("k(l ...)" per parcel) as an (N, width) array, each
+    """A labelFieldField ("k(l ...)" per parcel) as an (N, width) array, each
     list padded with zeros."""
     body = _body(path)
     u = _uniform(body)
@@ -180,7 +180,7 @@ def write_cloud(time_dir, parcels, time_name=None):
     vib = np.asarray(vib).reshape(-1, n)
     m = vib.shape[0]
     lines = "\n".join(f"{m}(" + " ".join(str(int(vib[q, k])) for q in range(m)) + ")" for k in range(n))
-    _write(os.path.join(d, "vibLevel"), "labelListField", loc, "vibLevel", f"{n}\n(\n{lines}\n)\n")
+    _write(os.path.join(d, "vibLevel"), "labelFieldField", loc, "vibLevel", f"{n}\n(\n{lines}\n)\n")
     for obj, val in (("newParcel", -1), ("classification", 0), ("origProcId", 0)):
         _write(os.path.join(d, obj), "labelField", loc, obj, f"{n}{{{val}}}\n")
     _write(os.path.join(d, "origId"), "labelField", loc, "origId",

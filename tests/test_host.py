@@ -143,3 +143,89 @@ def test_couette_pipeline_and_restart(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "from" in r.stdout and "0.0004" in r.stdout
     assert os.path.exists(os.path.join(case, "0.0006", "Ttra_mixture"))
+
+
+def _read_state(path):
+    """Parcels of an engine checkpoint (hystrath.state): magic, eight int64
+    (n, cells, species, vib modes, rotational, inflow faces, step, samples),
+    the time, then the parcel arrays in structure-of-arrays order."""
+    b = open(path, "rb").read()
+    assert b[:8] == b"HSDSMC01"
+    hdr = np.frombuffer(b, np.int64, 8, 8)
+    n, ns, modes, rot = int(hdr[0]), int(hdr[2]), int(hdr[3]), int(hdr[4])
+    off = 8 + 64 + 8
+    out = {}
+
+    def take(name, dt, count):
+        nonlocal off
+        out[name] = np.frombuffer(b, dt, count, off)
+        off += np.dtype(dt).itemsize * count
+
+    for k in ("px", "py", "pz"):
+        take(k, np.float64, n)
+    for k in ("ux", "uy", "uz"):
+        take(k, np.float32, n)
+    take("cell", np.int32, n)
+    if rot:
+        take("erot", np.float32, n)
+    if modes:
+        take("vib", np.uint16, modes * n)
+        out["vib"] = out["vib"].reshape(modes, n)
+    if ns > 1:
+        take("type", np.uint8, n)
+    return out
+
+
+@pytest.mark.gpu
+def test_dsmcfoam_starts_from_a_lagrangian_cloud(tmp_path):
+    """dsmcFoamPlus started from OpenFOAM cloud files (the first 8,192 parcels
+    of the couette_N2-O2 tutorial's snapshot, written by the Python writer) and
+    a dsmcSigmaTcRMax field: the walls and cyclic sides keep every parcel and
+    both species' counts, the cloud written at each write time holds exactly
+    the parcels of the engine checkpoint beside it, every parcel lies in the
+    cell it is filed under, and a cloud newer than the latest checkpoint is
+    the one a restart reads."""
+    from hystrath_amd import lagrangian as lg
+    case = casegen.couette_case(str(tmp_path / "couette"), end_time=2e-4, write_interval=1e-4, n_terminal=10)
+    h = np.load(os.path.join(ROOT, "tests", "golden", "couette_cloud_head.npz"))
+    xyz = h["xyz"]
+    parcels = dict(px=xyz[:, 0], py=xyz[:, 1], pz=xyz[:, 2], cell=h["cell"].astype(np.int32),
+                   ux=h["U"][:, 0], uy=h["U"][:, 1], uz=h["U"][:, 2], erot=h["erot"], type=h["type"],
+                   vib=h["vib"].reshape(1, -1))
+    n = len(parcels["px"])
+    lg.write_cloud(os.path.join(case, "0"), parcels, "0")
+    with open(os.path.join(case, "0", "dsmcSigmaTcRMax"), "w") as f:
+        f.write("FoamFile\n{\n    version 2.0;\n    format ascii;\n    class volScalarField;\n"
+                "    object dsmcSigmaTcRMax;\n}\ndimensions [0 3 -1 0 0 0 0];\n"
+                "internalField   uniform 1.2e-15;\nboundaryField\n{\n}\n")
+    r = run("dsmcFoamPlus", case)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert f"Read {n} parcels at time 0 from" in r.stdout and "lagrangian/dsmc" in r.stdout, r.stdout[:2000]
+    n_type = np.bincount(parcels["type"], minlength=2)
+    for t in ("0.0001", "0.0002"):
+        d = os.path.join(case, t)
+        c = lg.read_cloud(d, vib_modes=1)
+        assert len(c["px"]) == n, t
+        np.testing.assert_array_equal(np.bincount(c["type"], minlength=2), n_type)
+        s = _read_state(os.path.join(d, "lagrangian", "dsmc", "hystrath.state"))
+        for k in ("px", "py", "pz", "cell", "ux", "uy", "uz", "erot", "vib", "type"):
+            np.testing.assert_array_equal(c[k], s[k], err_msg=f"{t}/{k}")
+        i = np.floor(c["px"] / 0.01).astype(int)
+        j = np.floor(c["py"] / 0.01).astype(int)
+        np.testing.assert_array_equal(c["cell"], i + 5 * j)
+        sig = _field(os.path.join(d, "dsmcSigmaTcRMax"))
+        assert sig.shape == (500,) and (sig >= 1.2e-15 * (1 - 1e-9)).all()
+    # a cloud at a later time than the latest checkpoint is where a restart starts
+    later = os.path.join(case, "0.00025")
+    shutil.copytree(os.path.join(case, "0.0002", "lagrangian"), os.path.join(later, "lagrangian"),
+                    ignore=shutil.ignore_patterns("hystrath.state"))
+    shutil.copy(os.path.join(case, "0.0002", "dsmcSigmaTcRMax"), later)
+    cd = os.path.join(case, "system/controlDict")
+    # write times are counted from the start time: 0.00025 + 1e-4
+    ctxt = open(cd).read()
+    assert "endTime 0.0002;" in ctxt
+    open(cd, "w").write(ctxt.replace("endTime 0.0002;", "endTime 0.00035;"))
+    r = run("dsmcFoamPlus", case)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    assert f"Read {n} parcels at time 0.00025 from" in r.stdout, r.stdout[:2000]
+    assert len(lg.read_cloud(os.path.join(case, "0.00035"), 1)["px"]) == n

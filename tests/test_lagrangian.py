@@ -113,3 +113,104 @@ def test_engine_evolves_reference_snapshot_like_the_oracle(tmp_path):
         back = lg.read_cloud(str(tmp_path / "5.00005"), vib_modes=1)
         for k in ("px", "py", "pz", "cell", "ux", "uy", "uz", "erot", "type", "vib"):
             np.testing.assert_array_equal(back[k], got[k], err_msg=k)
+
+
+# --- the C++ host's reader and writer (hystrath_amd/host/lagrangianIO.C) ------
+CLOUDCHECK = os.path.join(os.path.dirname(GOLDEN), "..", "hystrath_amd", "bin", "cloudCheck")
+
+
+def _cloudcheck(*args):
+    import subprocess
+    return subprocess.run([CLOUDCHECK] + [str(a) for a in args], capture_output=True, text=True, timeout=120)
+
+
+def _random_cloud(n, modes, seed=4):
+    rng = np.random.default_rng(seed)
+    return dict(px=rng.uniform(0, 0.05, n), py=rng.uniform(0, 1, n), pz=np.full(n, 0.005),
+                cell=rng.integers(0, 500, n).astype(np.int32),
+                ux=rng.normal(0, 900, n).astype(np.float32), uy=rng.normal(0, 900, n).astype(np.float32),
+                uz=rng.normal(0, 900, n).astype(np.float32), erot=rng.exponential(3e-20, n).astype(np.float32),
+                type=rng.integers(0, 2, n).astype(np.uint8), vib=rng.integers(0, 5, (modes, n)).astype(np.uint16))
+
+
+def test_cpp_cloud_round_trip_is_exact(tmp_path):
+    """Python writes, the C++ reader reads and the C++ writer writes again,
+    Python reads: every parcel field unchanged, to the bit."""
+    c = _random_cloud(700, 2)
+    lg.write_cloud(str(tmp_path / "a"), c, "0.001")
+    r = _cloudcheck("copy", tmp_path / "a", tmp_path / "b", "0.002", 2, 500, 2)
+    assert r.returncode == 0 and "copied 700 parcels" in r.stdout, r.stdout + r.stderr
+    d = lg.read_cloud(str(tmp_path / "b"), vib_modes=2)
+    for k in c:
+        np.testing.assert_array_equal(d[k], c[k], err_msg=k)
+    base = tmp_path / "b" / "lagrangian" / "dsmc"
+    assert '"0.002/lagrangian/dsmc"' in (base / "U").read_text()
+    np.testing.assert_array_equal(lg.read_scalars(str(base / "origId")), np.arange(700))
+    assert "700{-1}" in (base / "newParcel").read_text()
+
+
+@pytest.mark.skipif(not os.path.isdir(SNAP), reason="hyStrath tutorials not present")
+def test_cpp_reads_reference_couette_cloud(tmp_path):
+    """The C++ reader on the tutorial's snapshot gives what the Python reader
+    (pinned by the fixtures above) gives; dsmcSigmaTcRMax's internalField too."""
+    r = _cloudcheck("copy", SNAP, tmp_path / "5", "5", 1, 500, 2)
+    assert r.returncode == 0 and "copied 47583 parcels" in r.stdout, r.stdout + r.stderr
+    a, b = lg.read_cloud(SNAP, 1), lg.read_cloud(str(tmp_path / "5"), 1)
+    for k in a:
+        np.testing.assert_array_equal(b[k], a[k], err_msg=k)
+    r = _cloudcheck("field", os.path.join(SNAP, "dsmcSigmaTcRMax"), 500)
+    assert r.returncode == 0, r.stdout
+    got = np.array(r.stdout.split(), float)
+    txt = open(os.path.join(SNAP, "dsmcSigmaTcRMax")).read()
+    body = txt[txt.index("internalField"):]
+    want = np.array(body[body.index("(") + 1:body.index(")")].split(), float)
+    assert len(want) == 500
+    np.testing.assert_array_equal(got, want)
+
+
+def test_cpp_reader_list_forms_and_errors(tmp_path):
+    """Uniform lists, absent optional fields (zeros), and the errors the reader
+    raises: a cell outside the mesh, a typeId outside the typeIdList, a
+    vibLevel list longer than the modes, a field of the wrong length."""
+    hdr = "FoamFile\n{\n    version 2.0;\n    format ascii;\n}\n// comment\n"
+
+    def cloud(root, positions, **files):
+        d = tmp_path / root / "lagrangian" / "dsmc"
+        d.mkdir(parents=True)
+        (d / "positions").write_text(hdr + positions)
+        for k, v in files.items():
+            (d / k).write_text(hdr + v)
+        return tmp_path / root
+
+    pos = "3\n(\n(0.001 0.002 0.005) 0\n(0.011 0.002 0.005) 1 7 9\n(0.021 0.5 0.005) 252\n)\n"
+    src = cloud("u", pos, U="3{(10 -20 30)}", typeId="3{1}", vibLevel="3{1(4)}")
+    r = _cloudcheck("copy", src, tmp_path / "u_out", "0", 1, 500, 2)
+    assert r.returncode == 0, r.stdout
+    d = lg.read_cloud(str(tmp_path / "u_out"), 1)
+    np.testing.assert_array_equal(d["cell"], [0, 1, 252])
+    np.testing.assert_array_equal(d["ux"], [10, 10, 10])
+    np.testing.assert_array_equal(d["uy"], [-20] * 3)
+    np.testing.assert_array_equal(d["type"], [1, 1, 1])
+    np.testing.assert_array_equal(d["vib"], [[4, 4, 4]])
+    np.testing.assert_array_equal(d["erot"], [0, 0, 0])          # no ERot file
+    src = cloud("short", pos, U="3((1 2 3) (4 5 6) (7 8 9))", vibLevel="3(0() 1(2) 0())")
+    r = _cloudcheck("copy", src, tmp_path / "short_out", "0", 2, 500, 1)
+    assert r.returncode == 0, r.stdout
+    np.testing.assert_array_equal(lg.read_cloud(str(tmp_path / "short_out"), 2)["vib"], [[0, 2, 0], [0, 0, 0]])
+    for root, files, modes, msg in (
+            ("badcell", dict(U="3{(0 0 0)}"), 1, None),
+            ("badtype", dict(U="3{(0 0 0)}", typeId="3(0 2 1)"), 1, "outside the typeIdList"),
+            ("badvib", dict(U="3{(0 0 0)}", vibLevel="3{2(1 1)}"), 1, "vibrational modes"),
+            ("badlen", dict(U="2{(0 0 0)}"), 1, "2 entries for 3 parcels")):
+        p = pos.replace(" 252\n", " 500\n") if root == "badcell" else pos
+        src = cloud(root, p, **files)
+        r = _cloudcheck("copy", src, tmp_path / (root + "_out"), "0", modes, 500, 2)
+        assert r.returncode == 1 and r.stdout.startswith("!! "), (root, r.stdout)
+        assert (msg or "outside the mesh") in r.stdout, (root, r.stdout)
+    (tmp_path / "sig_u").write_text(hdr + "dimensions [0 3 -1 0 0 0 0];\ninternalField   uniform 2.5e-15;\n"
+                                          "boundaryField\n{\n}\n")
+    r = _cloudcheck("field", tmp_path / "sig_u", 4)
+    assert r.returncode == 0 and r.stdout.split() == ["2.5e-15"] * 4, r.stdout
+    (tmp_path / "sig_n").write_text(hdr + "internalField nonuniform List<scalar> 3(1e-15 2e-15 3e-15);\n")
+    r = _cloudcheck("field", tmp_path / "sig_n", 4)
+    assert r.returncode == 1 and "3 values for 4 cells" in r.stdout, r.stdout
