@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -76,7 +77,13 @@ struct hs_dsmc_engine {
     ScanWs ws;
     unsigned long long* d_lb_status = nullptr;  // look-back status words of the cell scan
     unsigned long long* d_lb_ticket = nullptr;  // its block tickets
-    uint32_t scan_epoch = 0;                    // cell scans launched so far
+    // evolve's steady-state step pair (move, sort+collide with the next
+    // step's inflow ahead, twice) captured once per starting buffer: the
+    // pair returns the buffers to where it found them, so one graph replays
+    // every pair that starts on that buffer
+    hipGraphExec_t pair_graph[2] = { nullptr, nullptr };
+    int64_t pair_launches[T_NSTAGE] = { 0, 0, 0, 0, 0, 0 };
+    bool graphs = true;  // HS_DSMC_GRAPHS=0 runs every step eagerly
     int64_t n_sampled = 0;
     int move_grid = 1024;     // grid-stride launches over the parcels (scatter, hist-only move)
     int flight_grid[2] = { 1024, 1024 };  // the moving launch, without / with the face tracker:
@@ -435,11 +442,19 @@ extern "C" int32_t hs_dsmc_sample_slots(const hs_config* cfg) {
     return n;
 }
 
+static void drop_graphs(hs_dsmc_engine* eng) {
+    for (auto& g : eng->pair_graph) {
+        if (g) (void)hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+}
+
 extern "C" void hs_dsmc_destroy(hs_dsmc_engine* eng) {
     if (!eng) return;
     (void)hipSetDevice(eng->device);
     if (eng->stream) (void)hipStreamSynchronize(eng->stream);
     if (eng->side) (void)hipStreamSynchronize(eng->side);
+    drop_graphs(eng);
     if (eng->ev_scanned) (void)hipEventDestroy(eng->ev_scanned);
     if (eng->ev_injected) (void)hipEventDestroy(eng->ev_injected);
     for (auto& t : eng->timed) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
@@ -461,6 +476,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         return fail(nullptr, HS_E_HIP, "no HIP device available (the engine has no CPU path)");
     hs_dsmc_engine* eng = new hs_dsmc_engine();
     eng->cfg = *cfg;
+    if (const char* g = getenv("HS_DSMC_GRAPHS")) eng->graphs = std::strcmp(g, "0") != 0;
     (void)hipGetDevice(&eng->device);
     eng->nc = cfg->nx * cfg->ny * cfg->nz;
     eng->ns = cfg->n_species;
@@ -850,8 +866,8 @@ static void enqueue_move(hs_dsmc_engine* eng, int do_move, int do_hist) {
 static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sample, int advance, int inject_next) {
     {
         StageTimer tm(eng, T_SCAN);
-        launch_scan_cells(eng->d_count, eng->d_start, eng->nc, eng->d_lb_status, eng->d_lb_ticket, ++eng->scan_epoch,
-                          eng->d_S, eng->tile_parcels, eng->d_tile_begin, eng->collide_parcels, eng->d_ctile_begin,
+        launch_scan_cells(eng->d_count, eng->d_start, eng->nc, eng->d_lb_status, eng->d_lb_ticket, eng->d_S,
+                          eng->tile_parcels, eng->d_tile_begin, eng->collide_parcels, eng->d_ctile_begin,
                           eng->d_tile_heavy, eng->stream);
     }
     inject_next = inject_next && advance && eng->faces.n > 0;
@@ -884,14 +900,56 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     if (do_sample) eng->n_sampled += 1;
 }
 
+// Captures the steady-state step pair starting on the current buffer.  The
+// capture runs nothing; the host-side step bookkeeping it did is undone.
+static int capture_pair(hs_dsmc_engine* eng, int collide, int sample) {
+    int64_t launches[T_NSTAGE];
+    std::memcpy(launches, eng->launches, sizeof(launches));
+    const int64_t n_sampled = eng->n_sampled;
+    const int cur = eng->cur;
+    HS_TRY(hipStreamBeginCapture(eng->stream, hipStreamCaptureModeThreadLocal));
+    for (int j = 0; j < 2; ++j) {
+        enqueue_move(eng, 1, 1);
+        enqueue_sort_collide(eng, collide, sample, 1, 1);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(eng->stream, &g);
+    for (int s = 0; s < T_NSTAGE; ++s) eng->pair_launches[s] = eng->launches[s] - launches[s];
+    std::memcpy(eng->launches, launches, sizeof(launches));
+    eng->n_sampled = n_sampled;
+    eng->cur = cur;
+    if (e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(eng, HS_E_HIP, std::string("capturing the step pair: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&eng->pair_graph[cur], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        eng->pair_graph[cur] = nullptr;
+        return fail(eng, HS_E_HIP, std::string("instantiating the step pair: ") + hipGetErrorString(ei));
+    }
+    return 0;
+}
+
 extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     if (nsteps < 0) return fail(eng, HS_E_INVALID, "nsteps must be non-negative");
     HS_TRY(hipSetDevice(eng->device));
     const int sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
     const int collide = eng->cfg.collision_model != HS_COLL_NONE;
-    for (int k = 0; k < nsteps; ++k) {
-        if (k == 0) enqueue_inject(eng, 0);  // later steps' inflow is injected ahead, in the step before
+    // per-stage timing records events between the launches: eager steps only
+    const bool graphs = eng->graphs && !(eng->cfg.flags & HS_FLAG_TIMING);
+    int k = 0;
+    if (nsteps > 0) enqueue_inject(eng, 0);  // later steps' inflow is injected ahead, in the step before
+    // steps k and k+1 both inject ahead; the last step does not
+    for (; graphs && k + 2 < nsteps; k += 2) {
+        if (!eng->pair_graph[eng->cur])
+            if (int rc = capture_pair(eng, collide, sample)) return rc;
+        HS_TRY(hipGraphLaunch(eng->pair_graph[eng->cur], eng->stream));
+        for (int s = 0; s < T_NSTAGE; ++s) eng->launches[s] += eng->pair_launches[s];
+        eng->n_sampled += 2 * sample;
+    }
+    for (; k < nsteps; ++k) {
         enqueue_move(eng, 1, 1);
         enqueue_sort_collide(eng, collide, sample, 1, k + 1 < nsteps);
     }
@@ -942,6 +1000,8 @@ extern "C" int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags) {
     if ((flags & HS_FLAG_HEATFLUX) && !(eng->cfg.flags & HS_FLAG_HEATFLUX))
         return fail(eng, HS_E_INVALID, "heat-flux sums can only be enabled at create time");
     HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    drop_graphs(eng);  // captured with the old flags' kernels and stages
     eng->cfg.flags = flags;
     eng->hp.flags = flags;
     HS_TRY(hipMemcpyAsync(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice, eng->stream));

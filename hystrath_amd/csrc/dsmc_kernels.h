@@ -71,7 +71,7 @@ void launch_scatter(DevParticles b, DevScalars* S, const int32_t* start, const i
 // (clear before the first use; the collide order clears the flags it reads)
 int scan_cells_blocks(int32_t n);
 void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long long* status,
-                       unsigned long long* ticket, uint32_t epoch, DevScalars* S, int pt_g, int32_t* tile_g,
+                       unsigned long long* ticket, DevScalars* S, int pt_g, int32_t* tile_g,
                        int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st);
 void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
                   int32_t* heavy, hipStream_t st);

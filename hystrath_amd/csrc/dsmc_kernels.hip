@@ -191,7 +191,10 @@ void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, Scan
 // A block's status word: epoch << 34 | flag << 32 | value, flag 1 = the
 // block's aggregate, 2 = the inclusive prefix through the block.  A word from
 // an earlier epoch (an earlier scan) reads as not ready, so nothing is cleared
-// between scans.
+// between scans.  Every launch over the same cells takes gridDim.x tickets from
+// one counter, so ticket t is block t % gridDim.x of scan t / gridDim.x: the
+// block and the epoch come from the device alone, and a captured launch
+// replays correctly.
 #define HEAVY_CELL 96   // parcels: a cell this full marks its collide tile heavy
 #define LB_AGG 1ull
 #define LB_INCL 2ull
@@ -211,21 +214,22 @@ __device__ __forceinline__ void tile_marks_after(int i, int32_t e, int32_t c, in
 
 __global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt, int32_t* __restrict__ start, int32_t n,
                                                       unsigned long long* __restrict__ status,
-                                                      unsigned long long* __restrict__ ticket, uint32_t epoch,
+                                                      unsigned long long* __restrict__ ticket,
                                                       DevScalars* __restrict__ S, int pt_g,
                                                       int32_t* __restrict__ tile_g, int pt_c,
                                                       int32_t* __restrict__ tile_c, int32_t* __restrict__ heavy) {
     __shared__ int s_blk;
+    __shared__ uint32_t s_epoch;
     __shared__ int s_excl;
     // logical block = order of start, so every predecessor has started
-    if (threadIdx.x == 0)
-        s_blk = (int)(atomicAdd(ticket, 1ull) - (unsigned long long)(epoch - 1) * gridDim.x);
+    if (threadIdx.x == 0) {
+        const unsigned long long t = atomicAdd(ticket, 1ull);
+        s_blk = (int)(t % gridDim.x);
+        s_epoch = (uint32_t)((t / gridDim.x) % 0x3fffffffull) + 1;  // 1 .. 2^30 - 1: never a cleared word's 0
+    }
     __syncthreads();
     const int b = s_blk;
-    if (b < 0 || b >= (int)gridDim.x) {  // tickets out of step with the epoch: report, touch nothing
-        if (threadIdx.x == 0) atomicAdd(&S->counters[CNT_TILE_OVF], 1ull);
-        return;
-    }
+    const uint32_t epoch = s_epoch;
     const int i0 = b * SCAN_TILE + threadIdx.x * SCAN_ITEMS;  // SCAN_ITEMS consecutive cells per thread
     int v[SCAN_ITEMS];
     int sum = 0;
@@ -299,9 +303,9 @@ __global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt,
 }
 
 void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long long* status,
-                       unsigned long long* ticket, uint32_t epoch, DevScalars* S, int pt_g, int32_t* tile_g,
+                       unsigned long long* ticket, DevScalars* S, int pt_g, int32_t* tile_g,
                        int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st) {
-    k_scan_cells<<<scan_cells_blocks(n), BLOCK, 0, st>>>(cnt, start, n, status, ticket, epoch, S, pt_g, tile_g, pt_c,
+    k_scan_cells<<<scan_cells_blocks(n), BLOCK, 0, st>>>(cnt, start, n, status, ticket, S, pt_g, tile_g, pt_c,
                                                           tile_c, heavy);
 }
 

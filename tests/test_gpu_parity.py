@@ -201,3 +201,40 @@ def test_sparse_wide_tiles_parity():
         _check_state(eng, st, f"sparse/step{k}")
     np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
     eng.close()
+
+
+@pytest.mark.parametrize("which", ["cylinder", "couette"])
+def test_graph_replayed_steps_match_the_oracle(which):
+    """evolve(n >= 3) replays a captured HIP graph of two steps (one per
+    starting buffer) for all but its last step.  Every replay is checked
+    against the oracle: calls of three steps (one replay, one eager step, with
+    the inflow of the cylinder case injected on the side stream inside the
+    graph) alternate between the two captured graphs; longer calls replay one
+    graph several times; toggling the timing flag drops the graphs (timed
+    steps run eagerly) and the next call captures them again."""
+    case, filler = {
+        "cylinder": (H.small_cylinder(), H.fill_cylinder),
+        "couette": (cases.couette_n2o2(), H.fill_couette),
+    }[which]
+    eng, st = _pair(case, filler)
+    flags = case.cfg.flags
+    total = 0
+    for r, n in enumerate((3, 3, 3, 3, 9, 1, 7, 4)):
+        if r == 5:
+            eng.set_flags(flags | abi.HS_FLAG_TIMING)
+        if r == 6:
+            eng.set_flags(flags)
+        eng.evolve(n)
+        st.evolve(n)
+        total += n
+        _check_state(eng, st, f"{which}/call{r}")
+    s_eng, n_eng = eng.samples()
+    np.testing.assert_array_equal(s_eng, st.sample_array())
+    assert n_eng == total
+    c, oc = eng.counters(), st.counters()
+    assert (c["candidates"], c["collisions"], c["deleted"], c["injected"], c["wall_hits"]) == tuple(oc[:5])
+    assert c["tile_overflow"] == 0 and c["capacity_overflow"] == 0
+    if which == "cylinder":
+        assert c["injected"] > 0
+    assert eng.step == st.step == total
+    eng.close()
