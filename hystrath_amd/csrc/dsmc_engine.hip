@@ -64,6 +64,10 @@ struct hs_dsmc_engine {
     WallQueue wq = { nullptr, nullptr, nullptr };  // flights stopped at walls
     int tile_parcels = 1536;     // parcels per gather workgroup (particle-balanced tiles)
     int collide_parcels = 2048;  // parcels per collide workgroup
+    int tile_max[2] = { 1536, 2048 };   // gather, collide: the tile sizes of a full device
+    int tile_req[2] = { 0, 0 };         // hs_dsmc_set_tiles; 0: chosen from the population
+    int ncu = 256;
+    int64_t n_hint = 0;  // population the host last saw (fill, upload, scalar reads)
     DevFaces faces;
     int32_t* d_face_off = nullptr;
     // wall faces (boundaryMeasurements)
@@ -492,6 +496,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     eng->general = eng->has_erot || eng->has_vib || eng->has_type;
     // collide tiles sized to the LDS chunk of the kernel variant
     eng->collide_parcels = eng->general ? 1536 : 2048;
+    eng->tile_max[1] = eng->collide_parcels;
     eng->nslots = hs_dsmc_sample_slots(cfg);
     derive(eng);
     auto bail = [&](int code) {
@@ -628,6 +633,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     int ncu = 256;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, eng->device) == hipSuccess) ncu = prop.multiProcessorCount;
+    eng->ncu = ncu;
     eng->move_grid = ncu * 8;
     for (int tr = 0; tr < 2; ++tr) eng->flight_grid[tr] = ncu * move_blocks_per_cu(tr);
     if (check_launch(eng, "init fills") || hipStreamSynchronize(eng->stream) != hipSuccess)
@@ -642,6 +648,7 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
 static int read_scalars(hs_dsmc_engine* eng, DevScalars* s) {
     HS_TRY(hipMemcpyAsync(s, eng->d_S, sizeof(DevScalars), hipMemcpyDeviceToHost, eng->stream));
     HS_TRY(hipStreamSynchronize(eng->stream));
+    eng->n_hint = s->n;
     return 0;
 }
 
@@ -678,6 +685,7 @@ extern "C" int hs_dsmc_upload_particles(hs_dsmc_engine* eng, const hs_particles*
     if (int rc = read_scalars(eng, &s)) return rc;
     s.n = (int32_t)n;
     s.n_old = (int32_t)n;
+    eng->n_hint = n;
     HS_TRY(hipMemcpyAsync(eng->d_S, &s, sizeof(s), hipMemcpyHostToDevice, st));
     HS_TRY(hipStreamSynchronize(st));
     return 0;
@@ -829,6 +837,8 @@ extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, doub
         launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, st);
         rc = check_launch(eng, "fill");
         if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(eng, HS_E_HIP, "fill failed");
+        DevScalars sc;
+        if (!rc) rc = read_scalars(eng, &sc);
     }
     (void)hipFree(cnt);
     (void)hipFree(off);
@@ -900,6 +910,29 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     if (do_sample) eng->n_sampled += 1;
 }
 
+// Gather and collide tiles: whole cells grouped up to pt parcels per
+// workgroup.  A full device takes the largest tiles; a small population is
+// cut into smaller ones (halving, down to 256 parcels) until it makes about
+// four workgroups per compute unit.  The tiling changes no result: the random
+// streams belong to cells and candidates, and each cell is sampled whole.
+static int tile_for(const hs_dsmc_engine* eng, int which) {
+    if (eng->tile_req[which] > 0) return eng->tile_req[which];
+    const int64_t per = eng->n_hint / (4 * (int64_t)eng->ncu);
+    int pt = eng->tile_max[which];
+    while (pt / 2 >= 256 && pt > per) pt /= 2;
+    return pt;
+}
+
+static int choose_tiles(hs_dsmc_engine* eng) {
+    const int g = tile_for(eng, 0), c = tile_for(eng, 1);
+    if (g == eng->tile_parcels && c == eng->collide_parcels) return 0;
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    drop_graphs(eng);  // captured with the old tile sizes
+    eng->tile_parcels = g;
+    eng->collide_parcels = c;
+    return 0;
+}
+
 // Captures the steady-state step pair starting on the current buffer.  The
 // capture runs nothing; the host-side step bookkeeping it did is undone.
 static int capture_pair(hs_dsmc_engine* eng, int collide, int sample) {
@@ -935,6 +968,7 @@ extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     if (nsteps < 0) return fail(eng, HS_E_INVALID, "nsteps must be non-negative");
     HS_TRY(hipSetDevice(eng->device));
+    if (int rc = choose_tiles(eng)) return rc;
     const int sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
     const int collide = eng->cfg.collision_model != HS_COLL_NONE;
     // per-stage timing records events between the launches: eager steps only
@@ -974,6 +1008,7 @@ extern "C" int hs_dsmc_move(hs_dsmc_engine* eng) {
 extern "C" int hs_dsmc_sort(hs_dsmc_engine* eng) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
+    if (int rc = choose_tiles(eng)) return rc;
     enqueue_move(eng, 0, 1);
     enqueue_sort_collide(eng, 0, 0, 0, 0);
     return check_launch(eng, "sort");
@@ -982,6 +1017,7 @@ extern "C" int hs_dsmc_sort(hs_dsmc_engine* eng) {
 extern "C" int hs_dsmc_collide(hs_dsmc_engine* eng, int32_t sample) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     HS_TRY(hipSetDevice(eng->device));
+    if (int rc = choose_tiles(eng)) return rc;
     enqueue_move(eng, 0, 1);
     enqueue_sort_collide(eng, eng->cfg.collision_model != HS_COLL_NONE, sample ? 1 : 0, 0, 0);
     return check_launch(eng, "collide");
@@ -1006,6 +1042,25 @@ extern "C" int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags) {
     eng->hp.flags = flags;
     HS_TRY(hipMemcpyAsync(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice, eng->stream));
     HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_set_tiles(hs_dsmc_engine* eng, int32_t gather_parcels, int32_t collide_parcels) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    if (gather_parcels < 0 || collide_parcels < 0 || (gather_parcels && gather_parcels < 64) ||
+        (collide_parcels && collide_parcels < 64) || gather_parcels > eng->tile_max[0] ||
+        collide_parcels > eng->tile_max[1])
+        return fail(eng, HS_E_INVALID, "tile sizes must be 0 (automatic) or between 64 and the full-device size");
+    HS_TRY(hipSetDevice(eng->device));
+    eng->tile_req[0] = gather_parcels;
+    eng->tile_req[1] = collide_parcels;
+    return 0;
+}
+
+extern "C" int hs_dsmc_get_tiles(hs_dsmc_engine* eng, int32_t* gather_parcels, int32_t* collide_parcels) {
+    if (!eng || !gather_parcels || !collide_parcels) return fail(eng, HS_E_INVALID, "NULL argument");
+    *gather_parcels = tile_for(eng, 0);
+    *collide_parcels = tile_for(eng, 1);
     return 0;
 }
 

@@ -26,6 +26,7 @@ EXPORTS = [
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
     "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
+    "hs_dsmc_set_tiles", "hs_dsmc_get_tiles",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide", "gather"]
@@ -88,6 +89,8 @@ def load_library(path=LIB_PATH):
                                P(C.c_double)], C.c_int),
         "hs_dsmc_device_copy": ([C.c_int64, C.c_int32, P(C.c_double)], C.c_int),
         "hs_dsmc_set_flags": ([eng, C.c_int32], C.c_int),
+        "hs_dsmc_set_tiles": ([eng, C.c_int32, C.c_int32], C.c_int),
+        "hs_dsmc_get_tiles": ([eng, P(C.c_int32), P(C.c_int32)], C.c_int),
         "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_wall_faces": ([eng, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)], C.c_int),
         "hs_dsmc_read_wall_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_int64)], C.c_int),
@@ -256,6 +259,15 @@ class Engine:
     def set_flags(self, flags):
         self._ck(self.L.hs_dsmc_set_flags(self.h, flags), "set_flags")
         self.case.cfg.flags = flags
+
+    def set_tiles(self, gather_parcels=0, collide_parcels=0):
+        """Parcels per gather / collide workgroup; 0 chooses from the population."""
+        self._ck(self.L.hs_dsmc_set_tiles(self.h, gather_parcels, collide_parcels), "set_tiles")
+
+    def tiles(self):
+        g, c = C.c_int32(), C.c_int32()
+        self._ck(self.L.hs_dsmc_get_tiles(self.h, C.byref(g), C.byref(c)), "get_tiles")
+        return g.value, c.value
 
     def synchronize(self):
         self._ck(self.L.hs_dsmc_synchronize(self.h), "synchronize")

@@ -256,6 +256,16 @@ int hs_dsmc_synchronize(hs_dsmc_engine* eng);
  * (the sampling schedule of timeDataMeas, timeDataMeas.C:71-104);
  * HS_FLAG_HEATFLUX can only be kept as it was at create time. */
 int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
+/* Parcels per gather / collide workgroup (whole cells are grouped up to this
+ * many).  No counterpart in the reference, whose cells are processed one by
+ * one: a tuning knob for small populations.  0 (the default) chooses from the
+ * population last seen by the host (fill, upload, counts): the full-device
+ * tiles (1536 / 2048, 1536 with internal energy) for large populations,
+ * halved down to 256 until there are about four workgroups per compute unit.
+ * Results do not depend on the tiling.  hs_dsmc_get_tiles reports the sizes
+ * the next step uses. */
+int hs_dsmc_set_tiles(hs_dsmc_engine* eng, int32_t gather_parcels, int32_t collide_parcels);
+int hs_dsmc_get_tiles(hs_dsmc_engine* eng, int32_t* gather_parcels, int32_t* collide_parcels);
 
 /* ---- sampling ----------------------------------------------------------- */
 /* dsmcVolFields cumulative sums, plane-major [slot][cell]; nsteps_out gets

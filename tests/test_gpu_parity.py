@@ -238,3 +238,31 @@ def test_graph_replayed_steps_match_the_oracle(which):
         assert c["injected"] > 0
     assert eng.step == st.step == total
     eng.close()
+
+
+@pytest.mark.parametrize("which,tiles", [("cylinder", (0, 0)), ("cylinder", (64, 64)), ("cylinder", (1536, 2048)),
+                                         ("couette", (64, 96)), ("couette", (1536, 1536))])
+def test_tilings_give_the_same_steps(which, tiles):
+    """Parcels per gather and collide workgroup change no result: the engine
+    matches the oracle with the automatic choice (small populations: 384 and
+    256), the smallest tiles and the full-device ones."""
+    case, filler = {
+        "cylinder": (H.small_cylinder(), H.fill_cylinder),
+        "couette": (cases.couette_n2o2(), H.fill_couette),
+    }[which]
+    eng, st = _pair(case, filler)
+    eng.set_tiles(*tiles)
+    got = eng.tiles()
+    if tiles == (0, 0):
+        assert got == (384, 256), got   # ~36k parcels on 256 compute units
+    else:
+        assert got == tiles
+    for k in range(4):
+        eng.evolve(1)
+        st.evolve(1)
+        _check_state(eng, st, f"{which}/{tiles}/step{k}")
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    eng.evolve(5)   # graph replays with these tiles
+    st.evolve(5)
+    _check_state(eng, st, f"{which}/{tiles}/after5")
+    eng.close()
