@@ -266,3 +266,49 @@ def test_tilings_give_the_same_steps(which, tiles):
     st.evolve(5)
     _check_state(eng, st, f"{which}/{tiles}/after5")
     eng.close()
+
+
+def test_empty_domain_fills_from_the_inflow_like_the_oracle():
+    """No initial parcels: the freestream inflow fills the cylinder case from
+    empty.  Graph-replayed and single steps equal the oracle's, and a closed
+    box with no parcels steps without creating or sampling any."""
+    case = H.small_cylinder()
+    eng, st = Engine(case), ob.OracleState(case)
+    assert eng.count() == 0
+    for n in (7, 1, 1, 4):
+        eng.evolve(n)
+        st.evolve(n)
+        _check_state(eng, st, f"from-empty/{n}")
+    assert eng.count() > 0
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    c, oc = eng.counters(), st.counters()
+    assert (c["candidates"], c["collisions"], c["deleted"], c["injected"], c["wall_hits"]) == tuple(oc[:5])
+    eng.close()
+    box = H.small_box()
+    with Engine(box) as e:
+        e.evolve(5)
+        e.evolve(1)
+        assert e.count() == 0
+        s, nsamp = e.samples()
+        assert nsamp == 6 and not s.any()
+        assert all(v == 0 for k, v in e.counters().items() if k != "particle_steps")
+
+
+def test_inflow_beyond_capacity_is_dropped_and_counted():
+    """A capacity of 1,500 parcels under a Mach-10 inflow of ~160 parcels a
+    step: the population stops at the capacity, the parcels that did not fit
+    are counted, and every stored parcel stays a valid one."""
+    base = H.small_cylinder()
+    case = cases.cylinder_flow(nx=60, ny=60, ppc=10, dt=2.5e-7 * 400 / 60, seed=5, capacity=1500,
+                               flags=base.cfg.flags)
+    with Engine(case) as e:
+        for n in (9, 1, 6):
+            e.evolve(n)
+        c = e.counters()
+        assert e.count() <= 1500
+        assert c["capacity_overflow"] > 0
+        assert c["injected"] - c["deleted"] == e.count()
+        p = e.download()
+        assert ((p["cell"] >= 0) & (p["cell"] < case.ncells)).all()
+        assert np.isfinite(p["ux"]).all() and np.isfinite(p["uy"]).all()
+        assert ((p["px"] >= -0.35) & (p["px"] <= 0.65) & (p["py"] >= -0.5) & (p["py"] <= 0.5)).all()
