@@ -964,30 +964,82 @@ static int capture_pair(hs_dsmc_engine* eng, int collide, int sample) {
     return 0;
 }
 
+// One engine's way through an evolve of nsteps: evolve_begin enqueues the
+// first step's inflow (later steps' inflow is injected ahead, in the step
+// before); each evolve_advance enqueues the next graph-replayed pair, or a
+// single eager step at the end.
+struct EvolveRun {
+    hs_dsmc_engine* eng;
+    int nsteps, k, collide, sample;
+    bool graphs;
+};
+
+static int evolve_begin(EvolveRun& r, hs_dsmc_engine* eng, int nsteps) {
+    r.eng = eng;
+    r.nsteps = nsteps;
+    r.k = 0;
+    HS_TRY(hipSetDevice(eng->device));
+    if (int rc = choose_tiles(eng)) return rc;
+    r.sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
+    r.collide = eng->cfg.collision_model != HS_COLL_NONE;
+    // per-stage timing records events between the launches: eager steps only
+    r.graphs = eng->graphs && !(eng->cfg.flags & HS_FLAG_TIMING);
+    if (nsteps > 0) enqueue_inject(eng, 0);
+    return 0;
+}
+
+static int evolve_advance(EvolveRun& r) {
+    hs_dsmc_engine* eng = r.eng;
+    HS_TRY(hipSetDevice(eng->device));
+    // steps k and k+1 both inject ahead; the last step does not
+    if (r.graphs && r.k + 2 < r.nsteps) {
+        if (!eng->pair_graph[eng->cur])
+            if (int rc = capture_pair(eng, r.collide, r.sample)) return rc;
+        HS_TRY(hipGraphLaunch(eng->pair_graph[eng->cur], eng->stream));
+        for (int s = 0; s < T_NSTAGE; ++s) eng->launches[s] += eng->pair_launches[s];
+        eng->n_sampled += 2 * r.sample;
+        r.k += 2;
+        return 0;
+    }
+    enqueue_move(eng, 1, 1);
+    enqueue_sort_collide(eng, r.collide, r.sample, 1, r.k + 1 < r.nsteps);
+    r.k += 1;
+    return 0;
+}
+
 extern "C" int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps) {
     if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
     if (nsteps < 0) return fail(eng, HS_E_INVALID, "nsteps must be non-negative");
-    HS_TRY(hipSetDevice(eng->device));
-    if (int rc = choose_tiles(eng)) return rc;
-    const int sample = (eng->cfg.flags & HS_FLAG_SAMPLE) ? 1 : 0;
-    const int collide = eng->cfg.collision_model != HS_COLL_NONE;
-    // per-stage timing records events between the launches: eager steps only
-    const bool graphs = eng->graphs && !(eng->cfg.flags & HS_FLAG_TIMING);
-    int k = 0;
-    if (nsteps > 0) enqueue_inject(eng, 0);  // later steps' inflow is injected ahead, in the step before
-    // steps k and k+1 both inject ahead; the last step does not
-    for (; graphs && k + 2 < nsteps; k += 2) {
-        if (!eng->pair_graph[eng->cur])
-            if (int rc = capture_pair(eng, collide, sample)) return rc;
-        HS_TRY(hipGraphLaunch(eng->pair_graph[eng->cur], eng->stream));
-        for (int s = 0; s < T_NSTAGE; ++s) eng->launches[s] += eng->pair_launches[s];
-        eng->n_sampled += 2 * sample;
-    }
-    for (; k < nsteps; ++k) {
-        enqueue_move(eng, 1, 1);
-        enqueue_sort_collide(eng, collide, sample, 1, k + 1 < nsteps);
-    }
+    EvolveRun r;
+    if (int rc = evolve_begin(r, eng, nsteps)) return rc;
+    while (r.k < nsteps)
+        if (int rc = evolve_advance(r)) return rc;
     return check_launch(eng, "evolve");
+}
+
+extern "C" int hs_dsmc_evolve_batch(hs_dsmc_engine* const* engines, int32_t count, int32_t nsteps) {
+    if (!engines || count < 0) return fail(nullptr, HS_E_INVALID, "engines is NULL or count negative");
+    for (int32_t e = 0; e < count; ++e) {
+        if (!engines[e]) return fail(nullptr, HS_E_INVALID, "NULL engine in the batch");
+        for (int32_t f = 0; f < e; ++f)
+            if (engines[f] == engines[e]) return fail(engines[e], HS_E_INVALID, "an engine appears twice in the batch");
+    }
+    if (nsteps < 0) return fail(count ? engines[0] : nullptr, HS_E_INVALID, "nsteps must be non-negative");
+    std::vector<EvolveRun> runs((size_t)count);
+    for (int32_t e = 0; e < count; ++e)
+        if (int rc = evolve_begin(runs[e], engines[e], nsteps)) return rc;
+    // step-major: every engine's next pair is enqueued before any engine's
+    // following one, so the engines' streams keep the device busy together
+    for (int done = 0; done < count;) {
+        done = 0;
+        for (auto& r : runs) {
+            if (r.k >= nsteps) { ++done; continue; }
+            if (int rc = evolve_advance(r)) return rc;
+        }
+    }
+    for (int32_t e = 0; e < count; ++e)
+        if (int rc = check_launch(engines[e], "evolve_batch")) return rc;
+    return 0;
 }
 
 extern "C" int hs_dsmc_inject(hs_dsmc_engine* eng) {

@@ -26,7 +26,7 @@ EXPORTS = [
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
     "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
-    "hs_dsmc_set_tiles", "hs_dsmc_get_tiles",
+    "hs_dsmc_set_tiles", "hs_dsmc_get_tiles", "hs_dsmc_evolve_batch",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide", "gather"]
@@ -90,6 +90,7 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_device_copy": ([C.c_int64, C.c_int32, P(C.c_double)], C.c_int),
         "hs_dsmc_set_flags": ([eng, C.c_int32], C.c_int),
         "hs_dsmc_set_tiles": ([eng, C.c_int32, C.c_int32], C.c_int),
+        "hs_dsmc_evolve_batch": ([P(C.c_void_p), C.c_int32, C.c_int32], C.c_int),
         "hs_dsmc_get_tiles": ([eng, P(C.c_int32), P(C.c_int32)], C.c_int),
         "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_wall_faces": ([eng, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)], C.c_int),
@@ -119,6 +120,16 @@ def device_copy_ms(nbytes, iters=20):
     if rc != 0:
         raise EngineError(f"device copy failed ({rc}): {L.hs_dsmc_create_error().decode()}")
     return ms.value
+
+
+def evolve_batch(engines, nsteps=1):
+    """hs_dsmc_evolve_batch: nsteps of every engine, interleaved on the device."""
+    L = load_library()
+    hs = (C.c_void_p * len(engines))(*[e.h for e in engines])
+    rc = L.hs_dsmc_evolve_batch(hs, len(engines), nsteps)
+    if rc != 0:
+        msgs = [L.hs_dsmc_last_error(e.h).decode() for e in engines if e.h]
+        raise EngineError(f"evolve_batch failed ({rc}): {'; '.join(m for m in msgs if m) or L.hs_dsmc_create_error().decode()}")
 
 
 def _d(a):

@@ -244,6 +244,15 @@ int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot,
  * (dsmcCloud.C:63-74), noTimeCounter::collide (noTimeCounter.C:81-339) and
  * dsmcVolFields::calculateField sums (dsmcVolFields.C:1071-1290). */
 int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps);
+/* Evolves several independent engines nsteps each, as hs_dsmc_evolve would
+ * one after the other, with every engine's result the same as when it runs
+ * alone.  The engines' steps are enqueued interleaved on their own streams, so
+ * small cases share the device.  No counterpart in the reference, which runs
+ * one case per process: the batch form for parameter sweeps and ensembles.
+ * HS_E_INVALID for a NULL or repeated engine; on a failure the message is in
+ * the failing engine (NULL engine: hs_dsmc_create_error), and engines before
+ * it in the batch may have advanced further. */
+int hs_dsmc_evolve_batch(hs_dsmc_engine* const* engines, int32_t count, int32_t nsteps);
 /* Stage entry points (same kernels, for tests and custom loops). */
 int hs_dsmc_inject(hs_dsmc_engine* eng);                 /* controlParcelsBeforeMove */
 int hs_dsmc_move(hs_dsmc_engine* eng);                   /* Cloud<dsmcParcel>::move  */

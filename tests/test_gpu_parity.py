@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 from hystrath_amd import abi, cases
-from hystrath_amd.engine import Engine
+from hystrath_amd.engine import Engine, EngineError, evolve_batch
 import helpers as H
 import oracle_bind as ob
 
@@ -312,3 +312,30 @@ def test_inflow_beyond_capacity_is_dropped_and_counted():
         assert ((p["cell"] >= 0) & (p["cell"] < case.ncells)).all()
         assert np.isfinite(p["ux"]).all() and np.isfinite(p["uy"]).all()
         assert ((p["px"] >= -0.35) & (p["px"] <= 0.65) & (p["py"] >= -0.5) & (p["py"] <= 0.5)).all()
+
+
+def test_batched_engines_match_the_oracle():
+    """hs_dsmc_evolve_batch over three different cases at once (the cylinder
+    with its inflow, the LB-VHS couette, a closed VHS box): every engine ends
+    each call bit-identical to the oracle of its own case, over odd and even
+    step counts; a batch naming one engine twice is rejected."""
+    built = [(H.small_cylinder(), H.fill_cylinder), (cases.couette_n2o2(), H.fill_couette),
+             (H.small_box(), H.fill_box)]
+    pairs = [_pair(c, f) for c, f in built]
+    engs = [e for e, _ in pairs]
+    total = 0
+    for n in (5, 1, 4):
+        evolve_batch(engs, n)
+        total += n
+        for k, (e, st) in enumerate(pairs):
+            st.evolve(n)
+            _check_state(e, st, f"batch/engine{k}/+{n}")
+    for e, st in pairs:
+        s_eng, n_eng = e.samples()
+        np.testing.assert_array_equal(s_eng, st.sample_array())
+        assert n_eng == total and e.step == total
+    with pytest.raises(EngineError, match="twice"):
+        evolve_batch([engs[0], engs[1], engs[0]], 1)
+    evolve_batch([], 3)
+    for e in engs:
+        e.close()
