@@ -123,12 +123,73 @@ def cpu_baseline(case, eng, budget_s):
             "seconds": round(t1, 3)}
 
 
+def run_batch(args):
+    """64 independent cylinder cases (Mach 5-25, Kn 0.01-1, 50^2 to 300^2
+    cells, 10k to 2M parcels each) stepped together by hs_dsmc_evolve_batch;
+    value = particle-steps of all cases per second."""
+    from hystrath_amd import cases
+    from hystrath_amd.engine import Engine, evolve_batch
+
+    cs = cases.cylinder_batch(64)
+    engs = []
+    for c in cs:
+        e = Engine(c)
+        e.fill([c.n_inf], c.T_inf, u=(c.U_inf, 0.0, 0.0), fill_id=0)
+        engs.append(e)
+    sizes = [e.count() for e in engs]
+    log(f"[bench] batch of {len(engs)} cases, {sum(sizes)} particles ({min(sizes)} to {max(sizes)} a case); "
+        f"warm-up {args.warmup} steps")
+
+    def steps(n, what):
+        done, t0 = 0, time.time()
+        while done < n:
+            k = min(50, n - done)
+            evolve_batch(engs, k)
+            done += k
+            if time.time() - t0 > 10:
+                for e in engs:
+                    e.synchronize()
+                log(f"  {what}: {done}/{n}")
+                t0 = time.time()
+        for e in engs:
+            e.synchronize()
+
+    steps(args.warmup, "warm-up")
+    for e in engs:
+        e.reset_counters()
+        e.synchronize()
+    t0 = time.perf_counter()
+    steps(args.steps, "timed")
+    elapsed = time.perf_counter() - t0
+    cnts = [e.counters() for e in engs]
+    psteps = sum(c["particle_steps"] for c in cnts)
+    bad = {k: sum(c[k] for c in cnts) for k in ("track_overflow", "tile_overflow", "capacity_overflow")}
+    sizes = [e.count() for e in engs]
+    value = psteps / elapsed
+    log(f"[bench] {args.steps} steps of the batch in {elapsed:.3f}s: {value:.4e} particle-steps/s {bad}")
+    for e in engs:
+        e.close()
+    out = {"metric": "particle-steps/s on 1 GPU", "value": round(value, 1), "unit": "particle-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(1e3 * elapsed / max(args.steps, 1), 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (uniform freestream fills, flows developed by the warm-up steps)",
+           "config": {"workload": "cylinder_batch_64", "cases": len(engs), "particles": sum(sizes),
+                      "particles_min": min(sizes), "particles_max": max(sizes),
+                      "cases_params": [c.batch_params for c in cs],
+                      "parallelism": "single GPU, cases interleaved on their own streams"},
+           "roofline": None, "cpu_baseline": None, "overflow": bad}
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
     ap.add_argument("--warmup", type=int, default=1500)
-    ap.add_argument("--workload", choices=["cylinder", "box"], default="cylinder")
+    ap.add_argument("--workload", choices=["cylinder", "box", "batch64"], default="cylinder",
+                    help="batch64: 64 uneven cylinder cases stepped together (hs_dsmc_evolve_batch)")
     ap.add_argument("--profile-steps", type=int, default=200, help="instrumented steps for kernel durations")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
     args = ap.parse_args()
@@ -138,6 +199,9 @@ def main():
         print("bench.py: this build runs on one GPU only; multi-GPU runs are not part of it "
               f"(asked for --gpus {args.gpus}, WORLD_SIZE {world})", file=sys.stderr)
         return 2
+
+    if args.workload == "batch64":
+        return run_batch(args)
 
     from hystrath_amd import abi
     from hystrath_amd.engine import Engine, device_copy_ms

@@ -345,3 +345,29 @@ def wall_channel(nx=4, ny=20, L=0.02, n_density=1e21, T_gas=600.0, T_lo=300.0, T
                 name="wall_channel")
     case.n_density, case.T_gas = n_density, T_gas
     return case
+
+
+def cylinder_batch(count=64, seed=20251203, d_ref=4.17e-10):
+    """count independent argon cylinder cases of uneven size: Mach uniform in
+    [5, 25], Knudsen number (mean free path over the cylinder's diameter)
+    log-uniform in [0.01, 1], square meshes of 50 to 300 cells a side and 6 to
+    25 parcels per freestream cell (log-uniform), so 10k to 2M parcels a case.
+    The time step keeps the freestream at a quarter of a cell per step."""
+    rng = np.random.default_rng(seed)
+    out = []
+    ar = make_species("Ar")
+    for k in range(count):
+        n_side = int(rng.integers(50, 301))
+        mach = float(rng.uniform(5.0, 25.0))
+        kn = float(10.0 ** rng.uniform(-2.0, 0.0))
+        ppc = float(np.exp(rng.uniform(np.log(6.0), np.log(25.0))))
+        n_inf = 1.0 / (math.sqrt(2.0) * math.pi * d_ref * d_ref * kn * 0.2)
+        T_inf = 200.0
+        U = mach * math.sqrt(5.0 / 3.0 * KB * T_inf / ar.mass)
+        dt = 0.25 * (1.0 / n_side) / U
+        case = cylinder_flow(nx=n_side, ny=n_side, n_inf=n_inf, T_inf=T_inf, mach=mach, ppc=ppc, dt=dt,
+                             seed=seed + k, capacity=int(n_side * n_side * ppc * 2.5) + (1 << 16),
+                             flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
+        case.batch_params = dict(n_side=n_side, mach=round(mach, 3), kn=round(kn, 5), ppc=round(ppc, 2))
+        out.append(case)
+    return out
