@@ -223,6 +223,7 @@ LagrangianCloud readLagrangianCloud(const std::string& timeDir, int modes, int64
         FoamText t(d + "/positions");
         n = t.label();
         if (n < 0) t.fail("negative list size");
+        if (n > 2000000000) t.fail(std::to_string(n) + " parcels: more than the engine holds (2e9)");
         c.resize(n, modes);
         t.expect('(');
         for (int64_t k = 0; k < n; ++k) {
