@@ -265,6 +265,14 @@ def test_tilings_give_the_same_steps(which, tiles):
     eng.evolve(5)   # graph replays with these tiles
     st.evolve(5)
     _check_state(eng, st, f"{which}/{tiles}/after5")
+    # new tile sizes between calls: the captured graphs are dropped and taken again
+    eng.set_tiles(128, 64)
+    eng.evolve(6)
+    st.evolve(6)
+    _check_state(eng, st, f"{which}/{tiles}/retiled")
+    for bad in ((32, 0), (0, 4096), (-1, 0)):
+        with pytest.raises(EngineError, match="tile sizes"):
+            eng.set_tiles(*bad)
     eng.close()
 
 
