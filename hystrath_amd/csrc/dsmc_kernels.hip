@@ -1495,9 +1495,13 @@ __device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell
                 for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
                 const int64_t gd = (int64_t)S0 + rank;
                 const double x = src.px[k], y = src.py[k], z = src.pz[k];
-                dst.px[gd] = x; dst.py[gd] = y; dst.pz[gd] = z;
+                // positions and cells are read again only by the next step's
+                // move: streamed past the caches, which keep what the collide
+                // reads (velocities, sub-cell bytes)
+                __builtin_nontemporal_store(x, &dst.px[gd]); __builtin_nontemporal_store(y, &dst.py[gd]);
+                __builtin_nontemporal_store(z, &dst.pz[gd]);
                 dst.ux[gd] = src.ux[k]; dst.uy[gd] = src.uy[k]; dst.uz[gd] = src.uz[k];
-                dst.cell[gd] = c;
+                __builtin_nontemporal_store(c, &dst.cell[gd]);
                 if (GEN) copy_internal(P, src, dst, k, gd);
                 sub_out[gd] = (uint8_t)subcell_of(P, c, x, y, z);
             }
@@ -1540,9 +1544,10 @@ __device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell
             for (int u = 0; u < 4; ++u) {
                 if (kk[u] < 0) continue;
                 const int64_t gd = (int64_t)S0 + dd[u];
-                dst.px[gd] = x[u]; dst.py[gd] = y[u]; dst.pz[gd] = z[u];
+                __builtin_nontemporal_store(x[u], &dst.px[gd]); __builtin_nontemporal_store(y[u], &dst.py[gd]);
+                __builtin_nontemporal_store(z[u], &dst.pz[gd]);
                 dst.ux[gd] = ux[u]; dst.uy[gd] = uy[u]; dst.uz[gd] = uz[u];
-                dst.cell[gd] = cc[u];
+                __builtin_nontemporal_store(cc[u], &dst.cell[gd]);
                 if (GEN) copy_internal(P, src, dst, kk[u], gd);
                 sub_out[gd] = (uint8_t)subcell_of(P, cc[u], x[u], y[u], z[u]);
             }
