@@ -1,0 +1,126 @@
+"""OpenFOAM polyMesh read as the engine's Cartesian block (hystrath_amd.polymesh):
+the couette_N2-O2 tutorial's mesh, cross-checked against its own parcels, and
+synthetic blockMesh-ordered meshes, including the ones that are rejected."""
+import os
+
+import numpy as np
+import pytest
+
+from hystrath_amd import lagrangian as lg, polymesh as pm
+
+REF = "/root/reference/run/hyStrath/dsmcFoam+/couette_N2-O2"
+HDR = "FoamFile\n{{\n    version 2.0;\n    format ascii;\n    class {cls};\n    object {obj};\n}}\n"
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
+def test_reads_the_couette_tutorial_mesh():
+    """The tutorial's polyMesh is its blockMeshDict's block (5x100x1 cells on
+    [0,0.05]x[0,1]x[0,0.01], walls on y, cyclic halves on x, empty z), and
+    every one of the 47,583 parcels of its snapshot lies in the cell its
+    positions file files it under, by that block's numbering."""
+    b = pm.read_block(REF)
+    assert b["n"] == (5, 100, 1)
+    np.testing.assert_allclose(b["origin"], (0, 0, 0), atol=1e-15)
+    np.testing.assert_allclose(b["spacing"], (0.01, 0.01, 0.01), rtol=1e-12)
+    assert b["sides"] == {"xmin": ("periodicX_half0", "cyclic"), "xmax": ("periodicX_half1", "cyclic"),
+                          "ymin": ("lowerWall", "wall"), "ymax": ("upperWall", "wall"),
+                          "zmin": ("frontAndBack", "empty"), "zmax": ("frontAndBack", "empty")}
+    xyz, cell = lg.read_positions(os.path.join(REF, "backup-5", "lagrangian", "dsmc", "positions"))
+    ijk = np.floor((xyz - np.array(b["origin"])) / np.array(b["spacing"])).astype(int)
+    np.testing.assert_array_equal(cell, ijk[:, 0] + 5 * (ijk[:, 1] + 100 * ijk[:, 2]))
+
+
+def write_block_mesh(root, n, lo, hi, patches, xs=None, renumber=None):
+    """A polyMesh as blockMesh writes one hex block: points i-fastest,
+    cells i + nx*(j + ny*k), internal faces in owner-then-neighbour order,
+    then the boundary faces patch by patch.  patches: [(name, type, sides)].
+    xs: optional point x coordinates (non-uniform grading); renumber: a cell
+    permutation applied to owner/neighbour."""
+    nx, ny, nz = n
+    gx = np.linspace(lo[0], hi[0], nx + 1) if xs is None else np.asarray(xs)
+    gy, gz = np.linspace(lo[1], hi[1], ny + 1), np.linspace(lo[2], hi[2], nz + 1)
+    P = lambda i, j, k: i + (nx + 1) * (j + (ny + 1) * k)
+    C = lambda i, j, k: i + nx * (j + ny * k)
+    pts = [(gx[i], gy[j], gz[k]) for k in range(nz + 1) for j in range(ny + 1) for i in range(nx + 1)]
+
+    def quad(a, i, j, k):   # the face normal to axis a at lattice point (i, j, k)
+        if a == 0:
+            return [P(i, j, k), P(i, j + 1, k), P(i, j + 1, k + 1), P(i, j, k + 1)]
+        if a == 1:
+            return [P(i, j, k), P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j, k)]
+        return [P(i, j, k), P(i + 1, j, k), P(i + 1, j + 1, k), P(i, j + 1, k)]
+
+    faces, owner, neigh = [], [], []
+    for k in range(nz):
+        for j in range(ny):
+            for i in range(nx):
+                c = C(i, j, k)
+                for a, (di, dj, dk) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+                    if i + di < nx and j + dj < ny and k + dk < nz:
+                        faces.append(quad(a, i + di, j + dj, k + dk))
+                        owner.append(c)
+                        neigh.append(C(i + di, j + dj, k + dk))
+    side_faces = {}
+    for a, m in enumerate((nx, ny, nz)):
+        for s, pos in ((0, 0), (1, m)):
+            fl = []
+            for k in range(nz if a != 2 else 1):
+                for j in range(ny if a != 1 else 1):
+                    for i in range(nx if a != 0 else 1):
+                        ii, jj, kk = (pos if a == 0 else i), (pos if a == 1 else j), (pos if a == 2 else k)
+                        ci = (min(ii, nx - 1), min(jj, ny - 1), min(kk, nz - 1))
+                        fl.append((quad(a, ii, jj, kk), C(*ci)))
+            side_faces[pm.SIDES[2 * a + s]] = fl
+    bnd = []
+    for name, ptype, sides in patches:
+        start = len(faces)
+        for sd in sides:
+            for f, c in side_faces[sd]:
+                faces.append(f)
+                owner.append(c)
+        bnd.append((name, ptype, len(faces) - start, start))
+    if renumber is not None:
+        owner = [int(renumber[c]) for c in owner]
+        neigh = [int(renumber[c]) for c in neigh]
+    d = os.path.join(root, "constant", "polyMesh")
+    os.makedirs(d, exist_ok=True)
+
+    def w(obj, cls, items):
+        with open(os.path.join(d, obj), "w") as f:
+            f.write(HDR.format(cls=cls, obj=obj) + f"{len(items)}\n(\n" + "\n".join(items) + "\n)\n")
+
+    w("points", "vectorField", [f"({float(x)!r} {float(y)!r} {float(z)!r})" for x, y, z in pts])
+    w("faces", "faceList", ["4(" + " ".join(map(str, f)) + ")" for f in faces])
+    w("owner", "labelList", [str(c) for c in owner])
+    w("neighbour", "labelList", [str(c) for c in neigh])
+    w("boundary", "polyBoundaryMesh",
+      [f"{nm}\n{{\n    type {t};\n    nFaces {nf};\n    startFace {sf};\n}}" for nm, t, nf, sf in bnd])
+
+
+SIX = [("left", "patch", ["xmin"]), ("right", "patch", ["xmax"]), ("bottom", "wall", ["ymin"]),
+       ("top", "wall", ["ymax"]), ("frontAndBack", "empty", ["zmin", "zmax"])]
+
+
+def test_synthetic_block_round_trip(tmp_path):
+    write_block_mesh(str(tmp_path), (4, 3, 2), (-0.5, 0.0, 1.0), (0.5, 0.75, 1.5), SIX)
+    b = pm.read_block(str(tmp_path))
+    assert b["n"] == (4, 3, 2)
+    np.testing.assert_allclose(b["origin"], (-0.5, 0.0, 1.0))
+    np.testing.assert_allclose(b["spacing"], (0.25, 0.25, 0.25))
+    assert b["sides"]["ymax"] == ("top", "wall") and b["sides"]["zmin"] == b["sides"]["zmax"]
+
+
+def test_meshes_that_are_not_the_engine_block_are_rejected(tmp_path):
+    # graded in x
+    write_block_mesh(str(tmp_path / "g"), (4, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX, xs=[0, 0.1, 0.3, 0.6, 1.0])
+    with pytest.raises(ValueError, match="evenly spaced"):
+        pm.read_block(str(tmp_path / "g"))
+    # cells numbered otherwise than blockMesh
+    perm = np.arange(8)[::-1]
+    write_block_mesh(str(tmp_path / "r"), (4, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX, renumber=perm)
+    with pytest.raises(ValueError, match="numbering"):
+        pm.read_block(str(tmp_path / "r"))
+    # a side left without boundary faces: its cells are open
+    write_block_mesh(str(tmp_path / "m"), (2, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX[:4] + [("front", "empty", ["zmin"])])
+    with pytest.raises(ValueError, match="only hex cells"):
+        pm.read_block(str(tmp_path / "m"))
