@@ -1148,6 +1148,74 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
     (void)d;
 }
 
+/* octant sub-cells of one cell (noTimeCounter.C:114-138): parcel k is in
+ * sub-cell pos(x - xc) + 2 pos(y - yc) + 4 pos(z - zc) with OpenFOAM's
+ * pos(s) = (s >= 0); scn = members per sub-cell, sco = their offsets in
+ * sub_list, which lists each sub-cell's members in parcel order */
+static void subcell_tables(int32_t nC, const double* px, const double* py, const double* pz,
+                           const double cc[3], int32_t* which, int32_t scn[8], int32_t sco[9],
+                           int32_t* sub_list)
+{
+    for (int q = 0; q < 8; ++q) scn[q] = 0;
+    for (int32_t k = 0; k < nC; ++k) {
+        const int sc = (px[k] - cc[0] >= 0.0 ? 1 : 0) + 2 * (py[k] - cc[1] >= 0.0 ? 1 : 0)
+                     + 4 * (pz[k] - cc[2] >= 0.0 ? 1 : 0);
+        which[k] = sc;
+        scn[sc] += 1;
+    }
+    sco[0] = 0;
+    for (int q = 0; q < 8; ++q) sco[q + 1] = sco[q] + scn[q];
+    int32_t fill[8];
+    memcpy(fill, sco, sizeof(fill));
+    for (int32_t k = 0; k < nC; ++k) sub_list[fill[which[k]]++] = k;
+}
+
+/* one candidate pair (noTimeCounter.C:157-198): P uniform over the cell; Q
+ * from P's sub-cell when it has more than one member, else from the whole
+ * cell, redrawn until Q != P */
+static void draw_pair(or_rng* r, int32_t nC, const int32_t* which, const int32_t scn[8],
+                      const int32_t sco[9], const int32_t* sub_list, int* p_out, int* q_out)
+{
+    const int pp = rng_label(r, 0, nC - 1);
+    const int sc = which[pp];
+    const int nsc = scn[sc];
+    int qq = -1, guard = 0;
+    if (nsc > 1) {
+        do { qq = sub_list[sco[sc] + rng_label(r, 0, nsc - 1)]; }
+        while (pp == qq && ++guard < OR_MAX_REDRAW);
+    } else {
+        do { qq = rng_label(r, 0, nC - 1); } while (pp == qq && ++guard < OR_MAX_REDRAW);
+    }
+    *p_out = pp;
+    *q_out = qq;
+}
+
+int or_ntc_pairs(const hs_config* cfg, int64_t cell, int32_t nC, const double* px, const double* py,
+                 const double* pz, uint32_t step, int64_t ncand, int32_t* p_out, int32_t* q_out)
+{
+    if (nC < 2 || nC > 1 << 20) return -1;
+    const int64_t c = cell;
+    const int ci = (int)(c % cfg->nx), cj = (int)((c / cfg->nx) % cfg->ny), ck = (int)(c / ((int64_t)cfg->nx * cfg->ny));
+    const double cc[3] = { cfg->origin[0] + ((double)ci + 0.5) * cfg->delta[0],
+                           cfg->origin[1] + ((double)cj + 0.5) * cfg->delta[1],
+                           cfg->origin[2] + ((double)ck + 0.5) * cfg->delta[2] };
+    int32_t* which = (int32_t*)malloc(sizeof(int32_t) * nC);
+    int32_t* sub_list = (int32_t*)malloc(sizeof(int32_t) * nC);
+    int32_t scn[8], sco[9];
+    subcell_tables(nC, px, py, pz, cc, which, scn, sco, sub_list);
+    for (int64_t k = 0; k < ncand; ++k) {
+        or_rng r;
+        rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
+        int pp, qq;
+        draw_pair(&r, nC, which, scn, sco, sub_list, &pp, &qq);
+        p_out[k] = pp;
+        q_out[k] = qq;
+    }
+    free(which);
+    free(sub_list);
+    return 0;
+}
+
 int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
 {
     or_derived d;
@@ -1166,21 +1234,8 @@ int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
             const double cc[3] = { cfg->origin[0] + ((double)ci + 0.5) * cfg->delta[0],
                                    cfg->origin[1] + ((double)cj + 0.5) * cfg->delta[1],
                                    cfg->origin[2] + ((double)ck + 0.5) * cfg->delta[2] };
-            int32_t scn[8] = { 0 }, sco[9];
-            for (int32_t k = 0; k < nC; ++k) {
-                const int64_t i = b + k;
-                const int sc = (P->px[i] - cc[0] >= 0.0 ? 1 : 0) + 2 * (P->py[i] - cc[1] >= 0.0 ? 1 : 0)
-                             + 4 * (P->pz[i] - cc[2] >= 0.0 ? 1 : 0);
-                which[k] = sc;
-                scn[sc] += 1;
-            }
-            sco[0] = 0;
-            for (int q = 0; q < 8; ++q) sco[q + 1] = sco[q] + scn[q];
-            {
-                int32_t fill[8];
-                memcpy(fill, sco, sizeof(fill));
-                for (int32_t k = 0; k < nC; ++k) sub_list[fill[which[k]]++] = k;
-            }
+            int32_t scn[8], sco[9];
+            subcell_tables(nC, &P->px[b], &P->py[b], &P->pz[b], cc, which, scn, sco, sub_list);
             const double smax = s->sigma_tcr_max[c];
             const double sel = s->remainder[c]
                 + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
@@ -1192,16 +1247,8 @@ int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
                 /* every candidate draws from its own stream (cell, step, k) */
                 or_rng r;
                 rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, (uint32_t)k);
-                const int pp = rng_label(&r, 0, nC - 1);
-                const int sc = which[pp];
-                const int nsc = scn[sc];
-                int qq = -1, guard = 0;
-                if (nsc > 1) {
-                    do { qq = sub_list[sco[sc] + rng_label(&r, 0, nsc - 1)]; }
-                    while (pp == qq && ++guard < OR_MAX_REDRAW);
-                } else {
-                    do { qq = rng_label(&r, 0, nC - 1); } while (pp == qq && ++guard < OR_MAX_REDRAW);
-                }
+                int pp, qq;
+                draw_pair(&r, nC, which, scn, sco, sub_list, &pp, &qq);
                 if (pp == qq) continue;
                 const int64_t ip = b + pp, iq = b + qq;
                 const int tp = P->type ? P->type[ip] : 0, tq = P->type ? P->type[iq] : 0;

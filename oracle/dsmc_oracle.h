@@ -87,6 +87,11 @@ int64_t or_face_count(const hs_config* cfg);
 double or_arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2]);
 int or_sort(const hs_config* cfg, or_state* s);
 int or_collide(const hs_config* cfg, or_state* s, int32_t sample);
+/* the NTC candidate pairs (P, Q) that or_collide draws for candidates
+ * 0..ncand-1 of `cell` at `step`, the cell holding the nC parcels at
+ * (px, py, pz) in sorted order (noTimeCounter.C:114-198); test hook */
+int or_ntc_pairs(const hs_config* cfg, int64_t cell, int32_t nC, const double* px, const double* py,
+                 const double* pz, uint32_t step, int64_t ncand, int32_t* p_out, int32_t* q_out);
 int or_evolve(const hs_config* cfg, or_state* s, int32_t nsteps);
 
 #ifdef __cplusplus

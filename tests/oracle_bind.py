@@ -78,6 +78,8 @@ def lib():
         L.or_arc_in_rect.restype = C.c_double
         L.or_face_count.argtypes = [cfgp]
         L.or_face_count.restype = C.c_int64
+        L.or_ntc_pairs.argtypes = [cfgp, C.c_int64, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
+                                   C.c_uint32, C.c_int64, P(C.c_int32), P(C.c_int32)]
         _lib = L
     return _lib
 
@@ -142,6 +144,19 @@ def binary_collide(case, tp, tq, up, uq, uniforms, erot_p=None, erot_q=None, vib
                                  iptr(np.ascontiguousarray(tq, np.int32)), dptr(up), dptr(uq), dptr(ep), dptr(eq),
                                  iptr(vp), iptr(vq), dptr(uni), nu, iptr(used))
     return up, uq, ep, eq, vp, vq, used, rc
+
+
+def ntc_pairs(case, cell, pos, step, ncand):
+    """The (P, Q) candidate pairs the oracle's NTC draws for candidates
+    0..ncand-1 of `cell` holding parcels at pos[k] (sorted order)."""
+    pos = np.ascontiguousarray(pos, np.float64).reshape(-1, 3)
+    px, py, pz = (np.ascontiguousarray(pos[:, a]) for a in range(3))
+    p = np.zeros(ncand, np.int32)
+    q = np.zeros(ncand, np.int32)
+    rc = lib().or_ntc_pairs(C.byref(case.cfg), cell, len(pos), dptr(px), dptr(py), dptr(pz), step, ncand,
+                            iptr(p), iptr(q))
+    assert rc == 0
+    return p, q
 
 
 # ---- whole-state driver --------------------------------------------------

@@ -1,0 +1,384 @@
+"""Pins the parts of the oracle that conservation alone cannot: the NTC
+sub-cell partner choice, the vibrational collision number Zv and the
+quantised-vibration acceptance, and the derived dsmcVolFields temperatures,
+Mach number, mean free path and collision rate.
+
+Reference behaviour being restated:
+  noTimeCounter.C:114-198 (octant sub-cells, candidate P/Q draws),
+  dsmcCloud.C:1390-1531 (postCollisionVibrationalEnergyLevel: Bird 2010
+  eq. 2 for Zv, Bird 5.61 acceptance), LarsenBorgnakkeVariableHardSphere.C:
+  150-279 (serial LB), dsmcVolFields.C:1365-1790 (derived fields).
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import stats
+
+from hystrath_amd import abi, cases, fields
+import oracle_bind as ob
+
+KB = abi.HS_KB
+
+
+# ---------------------------------------------------------------------------
+# NTC partner choice (noTimeCounter.C:157-198)
+# ---------------------------------------------------------------------------
+def _exact_pair_probabilities(octant):
+    """P(P = p, Q = q) from the reference's draw rules: P uniform over the
+    cell; Q uniform over the other members of P's sub-cell when it has more
+    than one member, else uniform over the other parcels of the cell."""
+    n = len(octant)
+    prob = np.zeros((n, n))
+    for p in range(n):
+        mates = [q for q in range(n) if octant[q] == octant[p] and q != p]
+        pool = mates if mates else [q for q in range(n) if q != p]
+        for q in pool:
+            prob[p, q] = 1.0 / n / len(pool)
+    return prob
+
+
+def _octant_positions(case, octant, on_centre=()):
+    """Positions in cell 0 lying in the given octants (bit a set: coordinate
+    a at or above the cell centre).  Parcels listed in on_centre sit exactly
+    on the centre plane of their lowest set bit (pos(0) = 1 in OpenFOAM
+    v1706)."""
+    c = case.cfg
+    cc = np.array([c.origin[a] + 0.5 * c.delta[a] for a in range(3)])
+    rng = np.random.default_rng(len(octant))
+    out = []
+    for k, o in enumerate(octant):
+        p = cc.copy()
+        for a in range(3):
+            off = rng.uniform(0.05, 0.45) * c.delta[a]
+            p[a] += off if (o >> a) & 1 else -off
+        if k in on_centre:
+            a = next(a for a in range(3) if (o >> a) & 1)
+            p[a] = cc[a]
+        out.append(p)
+    return np.array(out)
+
+
+CONFIGS = [
+    ([0, 0], ()),
+    ([0, 7], ()),
+    ([0, 0, 5], ()),
+    ([1, 2, 4], ()),
+    ([3, 3, 3], ()),
+    ([1, 1, 0], (0,)),        # parcel 0 on the x centre plane belongs to the +x half
+    ([2, 0, 2, 6], (2,)),     # ... and on the y centre plane to the +y half
+    ([4, 4, 0, 1], (1,)),     # ... and on the z centre plane to the +z half
+    ([0, 0, 1, 1], ()),
+    ([0, 0, 0, 6], ()),
+    ([2, 4, 6, 7], ()),
+    ([5, 5, 5, 5], ()),
+]
+
+
+@pytest.mark.parametrize("octant,on_centre", CONFIGS)
+def test_ntc_partner_choice_matches_the_reference_draw_rules(octant, on_centre):
+    """250k oracle candidates per configuration: pairs that the rules forbid
+    never occur and the others follow the exact probabilities (chi-square).
+    The same counts reject the simpler rule 'Q from the whole cell' wherever
+    the two differ, so a dropped sub-cell step cannot pass."""
+    case = cases.periodic_box(n_side=2, L=0.1, seed=4321)
+    pos = _octant_positions(case, octant, on_centre)
+    n = len(octant)
+    ncand = 250_000
+    p, q = ob.ntc_pairs(case, 0, pos, step=17, ncand=ncand)
+    obs = np.zeros((n, n))
+    np.add.at(obs, (p, q), 1)
+    want = _exact_pair_probabilities(octant)
+    assert obs[want == 0].sum() == 0, "a pair the draw rules forbid was drawn"
+    nz = want > 0
+    if nz.sum() > 1:
+        chi = stats.chisquare(obs[nz], want[nz] * ncand)
+        assert chi.pvalue > 1e-4, chi
+    whole = np.full((n, n), 1.0 / (n * (n - 1)))
+    np.fill_diagonal(whole, 0.0)
+    if not np.allclose(whole, want):
+        bad = ((obs > 0) & (whole == 0)).any() or (whole == 0)[nz].any()
+        if not bad:
+            chi_wrong = stats.chisquare(obs[whole > 0], whole[whole > 0] * ncand)
+            assert chi_wrong.pvalue < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# vibrational collision number (dsmcCloud.C:1429-1508)
+# ---------------------------------------------------------------------------
+N2 = cases.SPECIES["N2"]
+AR = cases.SPECIES["Ar"]
+
+
+def _zv_case(form, **n2):
+    """An N2-like molecule colliding with argon (no internal modes), so only
+    P's vibrational test decides whether P's level changes."""
+    sp = [cases.make_species("N2", **n2), cases.make_species("Ar")]
+    case = cases.periodic_box(n_side=2, species=sp, collision_model=abi.HS_COLL_LB_VHS)
+    case.cfg.inv_zv_form = form
+    case.cfg.z_vib_fixed = 0.0
+    return case
+
+
+def _inv_zv_threshold(case, imax):
+    """The probability 1/Zv the oracle uses for P (level 0) in a collision
+    whose energy puts the top reachable level at imax, found by bisection on
+    the uniform of P's vibrational test: the test passes iff that uniform is
+    below 1/Zv, and when it passes the injected draws move P to level imax."""
+    sp = case.species
+    mp, mq = sp[0].mass, sp[1].mass
+    mr = mp * mq / (mp + mq)
+    theta = sp[0].theta_v[0]
+    imax = np.asarray(imax)
+    n = len(imax)
+    et = (imax + 0.5) * KB * theta
+    up = np.zeros((n, 3))
+    up[:, 0] = np.sqrt(2.0 * et / mr)
+    uq = np.zeros((n, 3))
+    tp, tq = np.zeros(n, np.int32), np.ones(n, np.int32)
+    lo, hi = np.zeros(n), np.full(n, 1.0)
+    for _ in range(64):
+        x = 0.5 * (lo + hi)
+        uni = np.full((n, 16), 0.5)
+        uni[:, 0] = 0.9999          # electronic test of P fails (1/500)
+        uni[:, 1] = x               # P's vibrational test
+        uni[:, 2] = 1.0 - 1e-9      # level drawn: imax
+        uni[:, 3] = 0.0             # Bird 5.61 acceptance passes
+        vp = np.zeros((n, 3), np.int32)
+        vq = np.zeros((n, 3), np.int32)
+        _, _, _, _, va, _, _, rc = ob.binary_collide(case, tp, tq, up, uq, uni, np.zeros(n), np.zeros(n), vp, vq)
+        assert rc == 0
+        passed = va[:, 0] == imax
+        assert ((va[:, 0] == 0) | passed).all()
+        lo = np.where(passed, x, lo)
+        hi = np.where(passed, hi, x)
+    return 0.5 * (lo + hi)
+
+
+def _t_coll(imax, theta_v, omega):
+    """The quantised collision temperature of Bird 2010 eq. 3 (denominator
+    Bird 5.42)."""
+    return imax * theta_v / (3.5 - omega)
+
+
+def test_zv_equals_zref_at_the_reference_temperature():
+    """Bird 2010 eq. 2 gives Zv = Zref when the collision temperature is
+    TrefZv, whatever thetaD and omega are; with inverseZvFormulation 2 the
+    probability is 1/(5 Zv) (Bird 2008), with 0 it is 1/Zv."""
+    om = 0.5 * (N2["omega"] + AR["omega"])
+    for imax in (3, 7, 20):
+        T = _t_coll(imax, N2["theta_v"][0], om)
+        for form, factor in ((0, 1.0), (2, 5.0)):
+            case = _zv_case(form, tref_zv=[T], zref=[1234.5])
+            got = _inv_zv_threshold(case, [imax])[0]
+            assert abs(got * factor * 1234.5 - 1.0) < 1e-12, (imax, form, got)
+
+
+def test_zv_is_one_at_the_dissociation_temperature():
+    """At a collision temperature equal to thetaD both factors of Bird 2010
+    eq. 2 are 1, so Zv = 1."""
+    om = 0.5 * (N2["omega"] + AR["omega"])
+    imax = 40
+    T = _t_coll(imax, N2["theta_v"][0], om)
+    got0 = _inv_zv_threshold(_zv_case(0, theta_d=T), [imax])[0]
+    got2 = _inv_zv_threshold(_zv_case(2, theta_d=T), [imax])[0]
+    assert got0 > 1.0 - 1e-15      # 1/Zv = 1: every uniform passes
+    assert abs(got2 - 0.2) < 1e-14
+
+
+def test_zv_follows_bird_2010_between_the_anchors():
+    """The couette species at three collision temperatures against Bird 2010
+    eq. 2, Zv = (thetaD/T)^w [Zref (thetaD/Tref)^-w]^(((thetaD/T)^(1/3) - 1) /
+    ((thetaD/Tref)^(1/3) - 1)), evaluated here in numpy; Zv falls as T rises."""
+    om = 0.5 * (N2["omega"] + AR["omega"])
+    th, td, zr, tr = N2["theta_v"][0], N2["theta_d"], N2["zref"][0], N2["tref_zv"][0]
+    imax = np.array([2, 6, 15])
+    T = _t_coll(imax.astype(float), th, om)
+    zv = (td / T) ** om * (zr * (td / tr) ** (-om)) ** (((td / T) ** (1 / 3) - 1) / ((td / tr) ** (1 / 3) - 1))
+    got = _inv_zv_threshold(_zv_case(0), imax)
+    np.testing.assert_allclose(got, 1.0 / zv, rtol=1e-12)
+    assert (np.diff(zv) < 0).all()
+
+
+def test_fixed_zv_is_used_as_given():
+    case = _zv_case(2)
+    case.cfg.z_vib_fixed = 7.0
+    got = _inv_zv_threshold(case, [4, 9])
+    np.testing.assert_allclose(got, 1.0 / 7.0, rtol=1e-14)
+
+
+# ---------------------------------------------------------------------------
+# quantised vibration in equilibrium (Bird 5.61 acceptance, serial LB)
+# ---------------------------------------------------------------------------
+def _equilibrium_run(seed=5, nsteps=700):
+    n2 = cases.make_species("N2")
+    case = cases.periodic_box(n_side=6, L=0.03, ppc=60, n_density=1e22, dt=4e-8, seed=seed,
+                              species=[n2], collision_model=abi.HS_COLL_LB_VHS)
+    case.cfg.z_vib_fixed = 3.0
+    case.cfg.z_rot = 2.0
+    st = ob.OracleState(case)
+    # hot translation and rotation, vibration frozen in the ground level
+    st.fill([1e22], 6000.0, Trot=6000.0, Tvib=0.0, exact=True)
+    p0 = st.particles()
+    st.evolve(nsteps)
+    return case, st, p0
+
+
+def _energies(case, p):
+    m = case.species[0].mass
+    th = case.species[0].theta_v[0]
+    u = np.stack([p["ux"], p["uy"], p["uz"]], 1).astype(np.float64)
+    u -= u.mean(0)
+    et = 0.5 * m * (u * u).sum()
+    er = p["erot"].astype(np.float64).sum()
+    lv = p["vib"][0].astype(np.int64)
+    return et, er, lv, th
+
+
+def test_vibration_relaxes_to_a_boltzmann_population_at_the_common_temperature():
+    """Translation and rotation at 6000 K with every molecule in the ground
+    vibrational level relax with LB (fixed Zv 3, Zrot 2) to one equilibrium:
+    the vibrational levels are the harmonic-oscillator Boltzmann population
+    at the final translational temperature (chi-square), T_rot = T_tr,
+    T_vib = thetaV / ln(1 + 1/mean level) = T_tr, and the total energy is
+    that of the start.  The acceptance (1 - E_v/E_c)^(3/2 - omega) is what
+    makes the equilibrium Boltzmann; an exponent off by one gives a colder
+    vibration than translation."""
+    case, st, p0 = _equilibrium_run()
+    et0, er0, lv0, th = _energies(case, p0)
+    assert lv0.sum() == 0
+    p = st.particles()
+    et, er, lv, _ = _energies(case, p)
+    n = len(lv)
+    e0 = et0 + er0
+    e1 = et + er + lv.sum() * KB * th
+    assert abs(e1 / e0 - 1) < 1e-5  # FP32 velocity and rotational-energy storage
+    T_tr = 2.0 * et / (3.0 * KB * n)
+    T_rot = er / (KB * n)
+    imean = lv.mean()
+    T_vib = th / math.log(1.0 + 1.0 / imean)
+    # the equilibrium temperature from the energy balance
+    from scipy.optimize import brentq
+
+    def excess(T):
+        return n * (2.5 * KB * T + KB * th / math.expm1(th / T)) - e0
+
+    T_eq = brentq(excess, 500.0, 6000.0)
+    assert abs(T_tr / T_eq - 1) < 0.01, (T_tr, T_eq)
+    assert abs(T_rot / T_eq - 1) < 0.015, (T_rot, T_eq)
+    assert abs(T_vib / T_eq - 1) < 0.02, (T_vib, T_eq)
+    # level populations against Boltzmann at T_tr
+    x = th / T_tr
+    top = 6
+    pk = (1 - math.exp(-x)) * np.exp(-x * np.arange(top))
+    pk = np.append(pk, 1.0 - pk.sum())
+    obs = np.bincount(np.minimum(lv, top), minlength=top + 1)
+    chi = stats.chisquare(obs, pk * n)
+    assert chi.pvalue > 1e-4, (obs, pk * n)
+
+
+# ---------------------------------------------------------------------------
+# derived fields (dsmcVolFields.C:1365-1790) on hand-made sums
+# ---------------------------------------------------------------------------
+def _one_cell_case(species, flags=abi.HS_FLAG_SAMPLE):
+    cyc = cases.patch(abi.HS_PATCH_CYCLIC)
+    return cases.Case(1, 1, 1, (0, 0, 0), (1e-3, 1e-3, 1e-3), species, [cyc] * 6,
+                      collision_model=abi.HS_COLL_LB_VHS, fn=1e10, dt=1e-7, capacity=8, flags=flags)
+
+
+def _sums(case, per_species, ncoll=0.0):
+    """A sample array [slot][1 cell] from per-species sums dictionaries."""
+    ns = abi.HS_S_GLOBAL_COUNT + case.cfg.n_species * abi.HS_SS_COUNT
+    s = np.zeros((ns, 1))
+    s[abi.HS_S_NCOLL, 0] = ncoll
+    for sid, d in enumerate(per_species):
+        b = abi.HS_S_GLOBAL_COUNT + sid * abi.HS_SS_COUNT
+        s[b + abi.HS_SS_N, 0] = d["N"]
+        s[b + abi.HS_SS_MUX:b + abi.HS_SS_MUZ + 1, 0] = d.get("MU", (0, 0, 0))
+        s[b + abi.HS_SS_MCC, 0] = d.get("MCC", 0.0)
+        s[b + abi.HS_SS_EROT, 0] = d.get("EROT", 0.0)
+        for mo, e in enumerate(d.get("EVIB", ())):
+            s[b + abi.HS_SS_EVIB0 + mo, 0] = e
+    return s
+
+
+def _equilibrium_sums(sp, T, N, U=(0.0, 0.0, 0.0)):
+    """Exact expectation sums of N parcels of species sp in equilibrium at T
+    moving with U: m|u|^2 = 3kT + m|U|^2 per parcel, E_rot = (dof/2) kT, and
+    for each vibrational mode the harmonic-oscillator mean level
+    1/(exp(thetaV/T) - 1)."""
+    m = sp.mass
+    U = np.asarray(U, float)
+    d = {"N": N, "MU": tuple(N * m * U), "MCC": N * (3 * KB * T + m * (U * U).sum()),
+         "EROT": N * 0.5 * sp.rot_dof * KB * T}
+    d["EVIB"] = tuple(N * KB * sp.theta_v[k] / math.expm1(sp.theta_v[k] / T) for k in range(sp.n_vib_modes))
+    return d
+
+
+@pytest.mark.parametrize("T", [800.0, 3000.0, 12000.0])
+def test_derived_temperatures_of_an_equilibrium_cell(T):
+    """Sums of a gas in equilibrium at T give back T_tr = T_rot = T_vib = T
+    and T_ov = T for a mixture of N2 and O2; the vibrational temperature
+    inverts the Bose-Einstein mean level 1/(exp(thetaV/T) - 1)."""
+    sp = [cases.make_species("N2"), cases.make_species("O2")]
+    case = _one_cell_case(sp)
+    s = _sums(case, [_equilibrium_sums(sp[0], T, 700.0, (300, -50, 10)),
+                     _equilibrium_sums(sp[1], T, 300.0, (300, -50, 10))])
+    for ids in (None, [0], [1]):
+        f = fields.vol_fields(s, 1, case, type_ids=ids)
+        for k in ("Ttra", "Trot", "Tvib", "Tov"):
+            assert abs(f[k][0] / T - 1) < 1e-11, (ids, k, f[k][0])
+        np.testing.assert_allclose(f["U"][:, 0], [300, -50, 10], rtol=1e-12)
+
+
+def test_overall_temperature_weights_modes_by_their_degrees_of_freedom():
+    """T_ov = (3 T_tr + zeta_rot T_rot + zeta_vib T_vib)/(3 + zeta_rot +
+    zeta_vib) with zeta_vib = 2 x/(exp(x) - 1), x = thetaV/T_vib, the
+    vibrational degrees of freedom of a harmonic oscillator at T_vib."""
+    n2 = cases.make_species("N2")
+    case = _one_cell_case([n2])
+    T_tr, T_rot, T_vib = 2000.0, 1500.0, 900.0
+    N = 1000.0
+    d = _equilibrium_sums(n2, T_tr, N)
+    d["EROT"] = N * KB * T_rot
+    d["EVIB"] = (N * KB * n2.theta_v[0] / math.expm1(n2.theta_v[0] / T_vib),)
+    f = fields.vol_fields(_sums(case, [d]), 1, case)
+    x = n2.theta_v[0] / T_vib
+    zv = 2 * x / math.expm1(x)
+    want = (3 * T_tr + 2 * T_rot + zv * T_vib) / (5 + zv)
+    assert abs(f["Ttra"][0] / T_tr - 1) < 1e-12
+    assert abs(f["Trot"][0] / T_rot - 1) < 1e-12
+    assert abs(f["Tvib"][0] / T_vib - 1) < 1e-11
+    assert abs(f["Tov"][0] / want - 1) < 1e-11
+    assert abs(f["zetaVib"][0] / zv - 1) < 1e-11
+
+
+def test_mach_number_uses_the_frozen_sound_speed():
+    """Ma = |U| / sqrt(gamma k T / m) with gamma = 5/3 for argon and 7/5 for
+    N2 (translation and rotation, dsmcVolFields.C:1600-1650)."""
+    for name, gamma in (("Ar", 5 / 3), ("N2", 7 / 5)):
+        sp = cases.make_species(name)
+        case = _one_cell_case([sp])
+        T, U = 450.0, np.array([900.0, 120.0, 0.0])
+        f = fields.vol_fields(_sums(case, [_equilibrium_sums(sp, T, 500.0, U)]), 1, case)
+        a = math.sqrt(gamma * KB * T / sp.mass)
+        assert abs(f["Ma"][0] / (np.linalg.norm(U) / a) - 1) < 1e-12, name
+
+
+def test_mean_free_path_and_collision_rate_of_a_simple_gas():
+    """Bird 4.76 and 4.74 for one species: lambda = (T/Tref)^(w - 1/2) /
+    (sqrt(2) pi d^2 n), nu = 4 d^2 n sqrt(pi k Tref / m) (T/Tref)^(1 - w);
+    at T = Tref the first is the hard-sphere 1/(sqrt(2) pi d^2 n)."""
+    ar = cases.make_species("Ar")
+    case = _one_cell_case([ar])
+    N = 800.0
+    V = 1e-9
+    n = N * case.cfg.fn / V
+    for T in (273.0, 1000.0):
+        f = fields.vol_fields(_sums(case, [_equilibrium_sums(ar, T, N)]), 1, case)
+        d, w = ar.diameter, ar.omega
+        lam = (T / 273.0) ** (w - 0.5) / (math.sqrt(2) * math.pi * d * d * n)
+        nu = 4 * d * d * n * math.sqrt(math.pi * KB * 273.0 / ar.mass) * (T / 273.0) ** (1 - w)
+        assert abs(f["mfp"][0] / lam - 1) < 1e-12, T
+        assert abs(f["meanCollisionRate"][0] / nu - 1) < 1e-12, T
+        assert abs(f["mct"][0] * nu - 1) < 1e-12, T
