@@ -6,13 +6,18 @@ with wall/deletion boundaries, cell sort, NTC + VHS collisions and
 dsmcVolFields sampling.  Default workload: 2D Mach-10 argon flow over a
 cylinder of radius 0.1 m, 400x400 cells, 25 simulators per freestream cell
 (about 4M particles).  Inputs are synthetic (a uniform freestream fill) and
-resident in HBM before the timed region; the warm-up steps develop the
-shock layer first.
+resident in HBM before the timed region; a fixed number of development
+steps (--develop, default 4000, about 2.6 freestream flow-throughs of the
+1 m domain) forms the bow shock and the wake in set-up, whatever --warmup
+is.
 
 Prints one JSON line: particle-steps/s, the HBM roofline of the dominant
-kernel (algorithmic bytes / HIP-event duration) and a bounded CPU baseline
-(the C restatement of the reference algorithm, single thread, on the same
-evolved state).
+kernel (algorithmic bytes / HIP-event duration), a bounded CPU baseline
+(the C restatement of the reference algorithm built -O3 -march=native with
+OpenMP, on all of the host's threads and on one, on the same developed
+state) and the plain path's time to fields within 2% of a 10k-step long
+average.  Exits non-zero, with "valid": false, if a device-side bound
+(tracking, tile or capacity overflow) tripped in the timed steps.
 """
 import argparse
 import json
@@ -50,14 +55,20 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def cylinder_case(seed=20251202):
+    from hystrath_amd import abi, cases
+
+    # dsmcVolFields sums every step and the wall's boundaryMeasurements, which
+    # the reference's diffuse wall always takes (dsmcDiffuseWallPatch.C:167)
+    return cases.cylinder_flow(nx=400, ny=400, ppc=25, dt=2.5e-7, seed=seed,
+                               flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
+
+
 def build_case(workload):
     from hystrath_amd import abi, cases
 
     if workload == "cylinder":
-        # dsmcVolFields sums every step and the wall's boundaryMeasurements, which
-        # the reference's diffuse wall always takes (dsmcDiffuseWallPatch.C:167)
-        case = cases.cylinder_flow(nx=400, ny=400, ppc=25, dt=2.5e-7, seed=20251202,
-                                   flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
+        case = cylinder_case()
         desc = {"workload": "cylinder_mach10_argon_400x400", "cells": case.ncells, "ppc_freestream": 25,
                 "n_inf": case.n_inf, "T_inf": case.T_inf, "U_inf": round(case.U_inf, 3), "T_wall": 500.0,
                 "radius": 0.1, "dt": case.cfg.dt, "collision_model": "VariableHardSphere",
@@ -92,35 +103,138 @@ def run_evolve(eng, nsteps, chunk=50, what="steps"):
             t0 = time.time()
 
 
+def _threaded_oracle():
+    """The OpenMP build of the C oracle compiled for this host
+    (-O3 -march=native) into a temporary directory; the prebuilt portable
+    one (oracle/_build/libdsmc_oracle_mt.so, -march=x86-64-v2) if gcc
+    cannot build it.  Returns (path, march)."""
+    import subprocess
+    import tempfile
+
+    src = os.path.join(ROOT, "oracle", "dsmc_oracle.c")
+    out = os.path.join(tempfile.mkdtemp(prefix="hs_cpu_"), "libdsmc_oracle_native.so")
+    cmd = ["gcc", "-O3", "-march=native", "-fopenmp", "-std=c11", "-fPIC", "-ffp-contract=off", "-fno-fast-math",
+           "-shared", "-o", out, src, "-lm"]
+    try:
+        r = subprocess.run(cmd, capture_output=True, text=True, timeout=120)
+        if r.returncode == 0:
+            return out, "native"
+        log(f"[bench] gcc -march=native failed, using the portable build: {r.stderr[-300:]}")
+    except (OSError, subprocess.SubprocessError) as e:
+        log(f"[bench] gcc unavailable ({e}); using the portable build")
+    return os.path.join(ROOT, "oracle", "_build", "libdsmc_oracle_mt.so"), "x86-64-v2"
+
+
+def _host():
+    import platform
+    model = platform.processor() or ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return model, os.cpu_count()
+
+
 def cpu_baseline(case, eng, budget_s):
-    """Time the CPU oracle on the engine's current state (bounded sample)."""
+    """Time the CPU oracle on the engine's current state (a bounded sample of
+    full steps), on all threads the host gives this job (OMP_NUM_THREADS, or
+    every logical CPU) and on one.  Parcels are shared among threads in the
+    move, cells in the sort, the collisions and the sampling; the threaded
+    build's steps equal the single-thread checker's (tests/
+    test_cpu_baseline.py)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
     import oracle_bind as ob
 
-    st = ob.OracleState(case)
-    st.load(eng.download(), *eng.cell_state())
-    if st.inflow is not None:
-        st.inflow[:] = eng.inflow()[: st.inflow.size]
-    st.step = eng.step
-    n0 = st.n
-    t = time.perf_counter()
-    st.evolve(1)
-    t1 = time.perf_counter() - t
-    psteps = n0
-    steps = 1
-    more = max(0, int(budget_s / max(t1, 1e-6)) - 1)
-    if more:
-        n1 = st.n
-        t = time.perf_counter()
-        st.evolve(more)
-        t1 += time.perf_counter() - t
-        psteps += int(0.5 * (n1 + st.n) * more)
-        steps += more
-    return {"value": psteps / t1, "unit": "particle-steps/s", "cores": 1, "kind": "port",
-            "sample": f"{steps} full steps of the same workload state ({n0} particles), C restatement "
-                      f"of the reference loops (oracle/dsmc_oracle.c, gcc -O2), single thread",
-            "seconds": round(t1, 3)}
+    path, march = _threaded_oracle()
+    L = ob.lib(path)
+    model, ncpu = _host()
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or ncpu
+    state = (eng.download(), *eng.cell_state())
+    inflow = eng.inflow()
+    results = {}
+    for nt in (threads, 1):
+        L.or_set_threads(nt)
+        st = ob.OracleState(case, lib_path=path)
+        st.load(*state)
+        if st.inflow is not None:
+            st.inflow[:] = inflow[: st.inflow.size]
+        st.step = eng.step
+        n0 = st.n
+        t1, psteps, steps, chunk = 0.0, 0, 0, 1
+        while t1 < budget_s:
+            n1 = st.n
+            t = time.perf_counter()
+            st.evolve(chunk)
+            dt = time.perf_counter() - t
+            t1 += dt
+            psteps += int(0.5 * (n1 + st.n) * chunk)
+            steps += chunk
+            chunk = max(1, min(int(0.5 * (budget_s - t1) / max(dt / chunk, 1e-6)), 50))
+        results[nt] = dict(value=psteps / t1, steps=steps, seconds=round(t1, 3), particles=n0)
+        log(f"[bench] CPU {nt} thread(s): {psteps / t1:.4e} particle-steps/s ({steps} steps, {t1:.1f} s)")
+    a, one = results[threads], results[1]
+    return {"value": a["value"], "unit": "particle-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{a['steps']} full steps of the developed workload state ({a['particles']} particles): "
+                      f"the C restatement of the reference loops (oracle/dsmc_oracle.c) built -O3 "
+                      f"-march={march} with OpenMP, {threads} threads; the reference dsmcFoam+ itself needs "
+                      f"OpenFOAM v1706 and MPI, which this host lacks",
+            "seconds": a["seconds"], "threads": threads, "nproc": ncpu, "cpu_model": model, "march": march,
+            "single_core": {"value": one["value"], "steps": one["steps"], "seconds": one["seconds"]}}
+
+
+def time_to_2pct(args, case_fn, fill, ms_per_step):
+    """The plain path's wall time from case start to running-average fields
+    within 2% of a long average.  A judge run (another seed) develops for
+    --t2-develop steps and averages --t2-judge steps; the measured run starts
+    from its own fill, develops for the same number of steps without
+    sampling (the reference's schedule: sampling from step 10k of 20k), then
+    samples and is checked every --t2-check steps.  The time is the steps
+    taken times the timed steps' ms/step; the wall time with the checks'
+    read-backs is reported beside it."""
+    from hystrath_amd import convergence as cv
+    from hystrath_amd.engine import Engine
+
+    dev, nj, chk = args.t2_develop, args.t2_judge, args.t2_check
+    case_j = case_fn(seed=20251202 + 1)
+    with Engine(case_j) as ej:
+        fill(ej)
+        ej.set_flags(case_j.cfg.flags & ~1)
+        run_evolve(ej, dev, what="judge development")
+        ej.set_flags(case_j.cfg.flags)
+        ej.reset_samples()
+        run_evolve(ej, nj, what="judge average")
+        s, n = ej.samples()
+        judge = cv.field_set(s, n, case_j)
+    case = case_fn(seed=20251202)
+    t0 = time.perf_counter()
+    with Engine(case) as e:
+        fill(e)
+        e.set_flags(case.cfg.flags & ~1)
+        run_evolve(e, dev, what="plain development")
+        e.set_flags(case.cfg.flags)
+        e.reset_samples()
+        done, err, trace = 0, None, []
+        while done < args.t2_max:
+            run_evolve(e, chk, what="plain sampling")
+            done += chk
+            s, n = e.samples()
+            err = cv.errors(cv.field_set(s, n, case), judge)
+            trace.append([done, {k: round(v, 5) for k, v in err.items()}])
+            if cv.converged(err, 0.02):
+                break
+        wall = time.perf_counter() - t0
+    ok = err is not None and cv.converged(err, 0.02)
+    steps = dev + done
+    log(f"[bench] time-to-2%: {'reached' if ok else 'NOT reached'} after {dev} + {done} steps; errors {err}")
+    return {"reached": ok, "steps": steps, "develop_steps": dev, "sampled_steps": done,
+            "seconds": round(steps * ms_per_step * 1e-3, 4), "wall_seconds_with_checks": round(wall, 3),
+            "errors": err, "judge": {"develop_steps": dev, "average_steps": nj, "seed": 20251202 + 1},
+            "check_every": chk, "trace": trace[-5:],
+            "definition": "relative L2 over cells with >= 1 parcel on average in the judge; velocity "
+                          "components scaled by the judge's speed; argon has no internal temperature"}
 
 
 def run_batch(args):
@@ -187,11 +301,19 @@ def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=1500)
+    ap.add_argument("--warmup", type=int, default=100, help="untimed steps after the development steps")
+    ap.add_argument("--develop", type=int, default=4000,
+                    help="development steps in set-up, before the warm-up (fixed, whatever --warmup is)")
     ap.add_argument("--workload", choices=["cylinder", "box", "batch64"], default="cylinder",
                     help="batch64: 64 uneven cylinder cases stepped together (hs_dsmc_evolve_batch)")
     ap.add_argument("--profile-steps", type=int, default=200, help="instrumented steps for kernel durations")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget (0 disables)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0,
+                    help="CPU-baseline budget per thread count (0 disables)")
+    ap.add_argument("--t2pct", type=int, default=1, help="measure the plain path's time to 2%% (0 disables)")
+    ap.add_argument("--t2-develop", type=int, default=10000, help="steps before sampling starts (judge and run)")
+    ap.add_argument("--t2-judge", type=int, default=10000, help="steps of the judge's long average")
+    ap.add_argument("--t2-check", type=int, default=250, help="sampled steps between error checks")
+    ap.add_argument("--t2-max", type=int, default=10000, help="give up after this many sampled steps")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -211,8 +333,9 @@ def main():
     eng = Engine(case)
     fill(eng)
     n_fill = eng.count()
-    log(f"[bench] filled {n_fill} particles; warm-up {args.warmup} steps")
-    run_evolve(eng, args.warmup, what="warm-up")
+    develop = args.develop if args.workload == "cylinder" else 0
+    log(f"[bench] filled {n_fill} particles; {develop} development steps, then warm-up {args.warmup} steps")
+    run_evolve(eng, develop + args.warmup, what="development + warm-up")
     eng.synchronize()
 
     # ---- timed region ---------------------------------------------------------
@@ -275,20 +398,30 @@ def main():
 
     cpu = None
     if args.cpu_seconds > 0:
-        log(f"[bench] CPU baseline ({args.cpu_seconds}s budget)")
+        log(f"[bench] CPU baseline ({args.cpu_seconds}s budget per thread count)")
         cpu = cpu_baseline(case, eng, args.cpu_seconds)
-        log(f"[bench] CPU: {cpu['value']:.4e} particle-steps/s")
     eng.close()
 
-    desc.update({"particles": n_now, "warmup_steps": args.warmup, "parallelism": "single GPU"})
+    t2 = None
+    if args.t2pct and args.workload == "cylinder":
+        t2 = time_to_2pct(args, cylinder_case, fill, ms_per_step)
+
+    desc.update({"particles": n_now, "develop_steps": develop, "warmup_steps": args.warmup,
+                 "precision": "FP64 arithmetic, positions and sums; FP32 velocity storage",
+                 "parallelism": "single GPU"})
+    valid = not bad
     out = {"metric": "particle-steps/s on 1 GPU", "value": round(value, 1), "unit": "particle-steps/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5),
-           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic (uniform freestream fill, flow developed by the warm-up steps)",
-           "config": desc, "roofline": roofline, "cpu_baseline": cpu,
-           "counters": cnt}
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32",
+           "data": (f"synthetic: uniform freestream fill, then {develop} development steps in set-up "
+                    f"(bow shock and wake formed) and {args.warmup} warm-up steps, all untimed"
+                    if develop else "synthetic: bimodal Maxwellian fill"),
+           "config": desc, "roofline": roofline, "cpu_baseline": cpu, "time_to_2pct": t2,
+           "counters": cnt, "valid": valid}
+    if not valid:
+        out["invalid_reason"] = f"device-side bounds tripped in the timed steps: {bad}"
     print(json.dumps(out), flush=True)
-    return 0
+    return 0 if valid else 3
 
 
 if __name__ == "__main__":

@@ -38,6 +38,16 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+/* The OpenMP build (oracle/Makefile target mt) is the benchmark's all-core CPU
+ * baseline: parcels and cells are shared out among threads, and sums that
+ * threads share are updated atomically.  The default build has no OpenMP
+ * and is the deterministic single-thread checker. */
+#define OR_ATOMIC _Pragma("omp atomic")
+#else
+#define OR_ATOMIC
+#endif
 
 #define OR_VSMALL 1.0e-300
 #define OR_SMALL 1.0e-15
@@ -730,7 +740,7 @@ static int64_t mesh_face(const hs_config* cfg, const or_wall* w, int a, int k, c
 
 static void track_face(const or_wall* w, int t, int64_t face, int64_t v)
 {
-    w->flux[(int64_t)t * w->n_faces + face] += v;
+    OR_ATOMIC w->flux[(int64_t)t * w->n_faces + face] += v;
 }
 
 int64_t or_face_count(const hs_config* cfg)
@@ -845,18 +855,20 @@ static double wall_side_terms(const hs_config* cfg, const or_wall* w, double* su
     const double ke = 0.5 * m * (U[0] * U[0] + U[1] * U[1] + U[2] * U[2]);
     const int64_t st = w->n;
     double* base = sums + (int64_t)t * HS_WS_COUNT * st + f;
-    base[HS_WS_RHON * st] += wt;
-    base[HS_WS_KE * st] += ke * wt;
-    for (int c = 0; c < 3; ++c) base[(HS_WS_MOMX + c) * st] += m * (U[c] - un * nw[c]) * wt;
+    OR_ATOMIC base[HS_WS_RHON * st] += wt;
+    OR_ATOMIC base[HS_WS_KE * st] += ke * wt;
+    for (int c = 0; c < 3; ++c) {
+        OR_ATOMIC base[(HS_WS_MOMX + c) * st] += m * (U[c] - un * nw[c]) * wt;
+    }
     double e = ke;
     if (sp->rot_dof > 0) {
         const double er = (double)*erot;
-        base[HS_WS_EROT * st] += er * wt;
+        OR_ATOMIC base[HS_WS_EROT * st] += er * wt;
         e += er;
     }
     for (int k = 0; k < sp->n_vib_modes; ++k) {
         const double ev = (double)vib[k * vstride] * HS_KB * sp->theta_v[k];
-        base[(HS_WS_EVIB0 + k) * st] += ev * wt;
+        OR_ATOMIC base[(HS_WS_EVIB0 + k) * st] += ev * wt;
         e += ev;
     }
     return e;
@@ -870,8 +882,10 @@ static void wall_exchange_terms(const hs_config* cfg, const or_wall* w, double* 
     const double cq = cfg->fn / (w->area[f] * cfg->dt);
     const int64_t st = w->n;
     double* base = sums + (int64_t)t * HS_WS_COUNT * st + f;
-    base[HS_WS_Q * st] += cq * (ea - eb);
-    for (int c = 0; c < 3; ++c) base[(HS_WS_FDX + c) * st] += cq * (m * Ua[c] - m * Ub[c]);
+    OR_ATOMIC base[HS_WS_Q * st] += cq * (ea - eb);
+    for (int c = 0; c < 3; ++c) {
+        OR_ATOMIC base[(HS_WS_FDX + c) * st] += cq * (m * Ua[c] - m * Ub[c]);
+    }
 }
 
 static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, const or_wall* w, int64_t i,
@@ -898,7 +912,10 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
         if (d->empty[a]) p[a] = d->centre[a];
     int keep = 1, iter = 0;
     while (keep && t > OR_ROOTVSMALL) {
-        if (++iter > OR_MAX_TRACK) { s->counters[5] += 1; break; }
+        if (++iter > OR_MAX_TRACK) {
+            OR_ATOMIC s->counters[5] += 1;
+            break;
+        }
         double ut[3], e[3];
         for (int a = 0; a < 3; ++a) {
             ut[a] = d->empty[a] ? 0.0 : U[a];
@@ -943,7 +960,7 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
         for (int b = 0; b < 3; ++b) p[b] = p[b] + lam * (e[b] - p[b]);
         t = t - t * lam;
         if (wall) {
-            s->counters[4] += 1;
+            OR_ATOMIC s->counters[4] += 1;
             double nx_ = cfg->cyl_center[0] - p[0], ny_ = cfg->cyl_center[1] - p[1];
             const double nm = sqrt(nx_ * nx_ + ny_ * ny_);
             const double nw[3] = { nx_ / nm, ny_ / nm, 0.0 };
@@ -992,7 +1009,7 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
             continue;
         }
         /* diffuse or specular wall (an empty plane is never crossed) */
-        s->counters[4] += 1;
+        OR_ATOMIC s->counters[4] += 1;
         double nw[3] = { 0.0, 0.0, 0.0 };
         nw[axis] = (double)dir;
         int64_t f = -1;
@@ -1016,7 +1033,10 @@ static void move_one(const hs_config* cfg, const or_derived* d, or_state* s, con
     }
     P->px[i] = p[0]; P->py[i] = p[1]; P->pz[i] = p[2];
     P->ux[i] = (float)U[0]; P->uy[i] = (float)U[1]; P->uz[i] = (float)U[2];
-    if (!keep) { P->cell[i] = -1; s->counters[2] += 1; }
+    if (!keep) {
+        P->cell[i] = -1;
+        OR_ATOMIC s->counters[2] += 1;
+    }
     else P->cell[i] = idx[0] + cfg->nx * (idx[1] + cfg->ny * idx[2]);
 }
 
@@ -1040,6 +1060,9 @@ static void move_context(const hs_config* cfg, or_state* s, or_wall* w)
 
 static void move_all(const hs_config* cfg, const or_derived* d, or_state* s, const or_wall* w, int64_t n_old)
 {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096)
+#endif
     for (int64_t i = 0; i < s->n; ++i) move_one(cfg, d, s, w, i, i >= n_old);
 }
 
@@ -1071,11 +1094,82 @@ static void copy_particle(const hs_config* cfg, const or_derived* d, const hs_pa
     (void)cfg;
 }
 
+#ifdef _OPENMP
+/* the same stable counting sort with the parcels cut into one contiguous
+ * block per thread: a block's parcels of a cell go after those of the blocks
+ * before it, so the order is the serial one */
+static void sort_threads(const hs_config* cfg, const or_derived* d, or_state* s, int64_t nc)
+{
+    const int T = omp_get_max_threads();
+    int32_t* cnt = (int32_t*)calloc((size_t)T * nc, sizeof(int32_t));
+    const int64_t n = s->n;
+#pragma omp parallel num_threads(T)
+    {
+        const int t = omp_get_thread_num();
+        const int64_t lo = n * t / T, hi = n * (t + 1) / T;
+        int32_t* mine = cnt + (int64_t)t * nc;
+        for (int64_t i = lo; i < hi; ++i)
+            if (s->p.cell[i] >= 0) mine[s->p.cell[i]] += 1;
+#pragma omp barrier
+#pragma omp for schedule(static)
+        for (int64_t c = 0; c < nc; ++c) {
+            int32_t sum = 0;
+            for (int u = 0; u < T; ++u) {
+                const int32_t k = cnt[(int64_t)u * nc + c];
+                cnt[(int64_t)u * nc + c] = sum;
+                sum += k;
+            }
+            s->count[c] = sum;
+        }
+#pragma omp single
+        {
+            s->start[0] = 0;
+            for (int64_t c = 0; c < nc; ++c) s->start[c + 1] = s->start[c] + s->count[c];
+        }
+        for (int64_t i = lo; i < hi; ++i) {
+            const int32_t c = s->p.cell[i];
+            if (c < 0) continue;
+            copy_particle(cfg, d, &s->p, i, s->capacity, &s->tmp, s->start[c] + mine[c]++, s->capacity);
+        }
+#pragma omp barrier
+        const int64_t m = s->start[nc];
+#pragma omp for schedule(static)
+        for (int64_t j = 0; j < m; ++j) copy_particle(cfg, d, &s->tmp, j, s->capacity, &s->p, j, s->capacity);
+    }
+    s->n = s->start[nc];
+    free(cnt);
+}
+#endif
+
+int or_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+void or_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
 int or_sort(const hs_config* cfg, or_state* s)
 {
     or_derived d;
     derive(cfg, &d);
     const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
+#ifdef _OPENMP
+    if (omp_get_max_threads() > 1) {
+        sort_threads(cfg, &d, s, nc);
+        return 0;
+    }
+#endif
     memset(s->count, 0, sizeof(int32_t) * nc);
     for (int64_t i = 0; i < s->n; ++i)
         if (s->p.cell[i] >= 0) s->count[s->p.cell[i]] += 1;
@@ -1216,76 +1310,96 @@ int or_ntc_pairs(const hs_config* cfg, int64_t cell, int32_t nC, const double* p
     return 0;
 }
 
+/* one cell of or_collide; cnt gets candidates, collisions and clamps */
+static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c, int32_t sample,
+                         int64_t nc, int active, int32_t* which, int32_t* sub_list, int64_t cnt[3])
+{
+    hs_particles* P = &s->p;
+    const int32_t b = s->start[c];
+    const int32_t nC = s->start[c + 1] - b;
+    double ncoll = 0.0, sep = 0.0;
+    if (active && nC > 1) {
+        const int ci = (int)(c % cfg->nx), cj = (int)((c / cfg->nx) % cfg->ny), ck = (int)(c / ((int64_t)cfg->nx * cfg->ny));
+        const double cc[3] = { cfg->origin[0] + ((double)ci + 0.5) * cfg->delta[0],
+                               cfg->origin[1] + ((double)cj + 0.5) * cfg->delta[1],
+                               cfg->origin[2] + ((double)ck + 0.5) * cfg->delta[2] };
+        int32_t scn[8], sco[9];
+        subcell_tables(nC, &P->px[b], &P->py[b], &P->pz[b], cc, which, scn, sco, sub_list);
+        const double smax = s->sigma_tcr_max[c];
+        const double sel = s->remainder[c]
+            + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
+        int64_t ncand = (int64_t)sel;
+        s->remainder[c] = sel - (double)ncand;
+        if (ncand > OR_MAX_CAND) { ncand = OR_MAX_CAND; cnt[2] += 1; }
+        cnt[0] += ncand;
+        for (int64_t k = 0; k < ncand; ++k) {
+            /* every candidate draws from its own stream (cell, step, k) */
+            or_rng r;
+            rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, (uint32_t)k);
+            int pp, qq;
+            draw_pair(&r, nC, which, scn, sco, sub_list, &pp, &qq);
+            if (pp == qq) continue;
+            const int64_t ip = b + pp, iq = b + qq;
+            const int tp = P->type ? P->type[ip] : 0, tq = P->type ? P->type[iq] : 0;
+            if (cfg->species[tp].charge == -1 && cfg->species[tq].charge == -1) continue;
+            double up[3] = { P->ux[ip], P->uy[ip], P->uz[ip] };
+            double uq[3] = { P->ux[iq], P->uy[iq], P->uz[iq] };
+            const double stcr = sigma_tcr(d, tp, tq, up, uq);
+            if (stcr > s->sigma_tcr_max[c]) s->sigma_tcr_max[c] = stcr;
+            if (stcr / smax > rng_u(&r)) {
+                double erp = P->erot ? P->erot[ip] : 0.0, erq = P->erot ? P->erot[iq] : 0.0;
+                int32_t vp[HS_MAX_VIB_MODES] = { 0 }, vq[HS_MAX_VIB_MODES] = { 0 };
+                for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) vp[m] = P->vib[m * s->capacity + ip];
+                for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) vq[m] = P->vib[m * s->capacity + iq];
+                binary_collide(cfg, d, tp, tq, up, uq, &erp, &erq, vp, vq, &r);
+                P->ux[ip] = (float)up[0]; P->uy[ip] = (float)up[1]; P->uz[ip] = (float)up[2];
+                P->ux[iq] = (float)uq[0]; P->uy[iq] = (float)uq[1]; P->uz[iq] = (float)uq[2];
+                if (P->erot) { P->erot[ip] = (float)erp; P->erot[iq] = (float)erq; }
+                for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) P->vib[m * s->capacity + ip] = (uint16_t)vp[m];
+                for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) P->vib[m * s->capacity + iq] = (uint16_t)vq[m];
+                ncoll += 1.0;
+                if (cfg->flags & HS_FLAG_COLL_SEP) {
+                    const double ddx = P->px[ip] - P->px[iq], ddy = P->py[ip] - P->py[iq], ddz = P->pz[ip] - P->pz[iq];
+                    sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                }
+                cnt[1] += 1;
+            }
+        }
+    }
+    if (sample && s->samples) {
+        sample_cell(cfg, d, s, c, b, nC, nc);
+        s->samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
+        s->samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
+    }
+}
+
 int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
 {
     or_derived d;
     derive(cfg, &d);
     const int64_t nc = (int64_t)cfg->nx * cfg->ny * cfg->nz;
-    hs_particles* P = &s->p;
-    int32_t* sub_list = (int32_t*)malloc(sizeof(int32_t) * (s->n > 0 ? s->n : 1));
-    int32_t* which = (int32_t*)malloc(sizeof(int32_t) * (s->n > 0 ? s->n : 1));
     const int active = cfg->collision_model != HS_COLL_NONE;
-    for (int64_t c = 0; c < nc; ++c) {
-        const int32_t b = s->start[c];
-        const int32_t nC = s->start[c + 1] - b;
-        double ncoll = 0.0, sep = 0.0;
-        if (active && nC > 1) {
-            const int ci = (int)(c % cfg->nx), cj = (int)((c / cfg->nx) % cfg->ny), ck = (int)(c / ((int64_t)cfg->nx * cfg->ny));
-            const double cc[3] = { cfg->origin[0] + ((double)ci + 0.5) * cfg->delta[0],
-                                   cfg->origin[1] + ((double)cj + 0.5) * cfg->delta[1],
-                                   cfg->origin[2] + ((double)ck + 0.5) * cfg->delta[2] };
-            int32_t scn[8], sco[9];
-            subcell_tables(nC, &P->px[b], &P->py[b], &P->pz[b], cc, which, scn, sco, sub_list);
-            const double smax = s->sigma_tcr_max[c];
-            const double sel = s->remainder[c]
-                + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
-            int64_t ncand = (int64_t)sel;
-            s->remainder[c] = sel - (double)ncand;
-            if (ncand > OR_MAX_CAND) { ncand = OR_MAX_CAND; s->counters[6] += 1; }
-            s->counters[0] += ncand;
-            for (int64_t k = 0; k < ncand; ++k) {
-                /* every candidate draws from its own stream (cell, step, k) */
-                or_rng r;
-                rng_init(&r, cfg->seed, OR_RNG_COLLIDE, (uint32_t)c, s->step, (uint32_t)k);
-                int pp, qq;
-                draw_pair(&r, nC, which, scn, sco, sub_list, &pp, &qq);
-                if (pp == qq) continue;
-                const int64_t ip = b + pp, iq = b + qq;
-                const int tp = P->type ? P->type[ip] : 0, tq = P->type ? P->type[iq] : 0;
-                if (cfg->species[tp].charge == -1 && cfg->species[tq].charge == -1) continue;
-                double up[3] = { P->ux[ip], P->uy[ip], P->uz[ip] };
-                double uq[3] = { P->ux[iq], P->uy[iq], P->uz[iq] };
-                const double stcr = sigma_tcr(&d, tp, tq, up, uq);
-                if (stcr > s->sigma_tcr_max[c]) s->sigma_tcr_max[c] = stcr;
-                if (stcr / smax > rng_u(&r)) {
-                    double erp = P->erot ? P->erot[ip] : 0.0, erq = P->erot ? P->erot[iq] : 0.0;
-                    int32_t vp[HS_MAX_VIB_MODES] = { 0 }, vq[HS_MAX_VIB_MODES] = { 0 };
-                    for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) vp[m] = P->vib[m * s->capacity + ip];
-                    for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) vq[m] = P->vib[m * s->capacity + iq];
-                    binary_collide(cfg, &d, tp, tq, up, uq, &erp, &erq, vp, vq, &r);
-                    P->ux[ip] = (float)up[0]; P->uy[ip] = (float)up[1]; P->uz[ip] = (float)up[2];
-                    P->ux[iq] = (float)uq[0]; P->uy[iq] = (float)uq[1]; P->uz[iq] = (float)uq[2];
-                    if (P->erot) { P->erot[ip] = (float)erp; P->erot[iq] = (float)erq; }
-                    for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) P->vib[m * s->capacity + ip] = (uint16_t)vp[m];
-                    for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) P->vib[m * s->capacity + iq] = (uint16_t)vq[m];
-                    ncoll += 1.0;
-                    if (cfg->flags & HS_FLAG_COLL_SEP) {
-                        const double ddx = P->px[ip] - P->px[iq], ddy = P->py[ip] - P->py[iq], ddz = P->pz[ip] - P->pz[iq];
-                        sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-                    }
-                    s->counters[1] += 1;
-                }
-            }
-        }
-        if (sample && s->samples) {
-            sample_cell(cfg, &d, s, c, b, nC, nc);
-            s->samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
-            s->samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
-        }
+    int32_t max_nc = 1;
+    for (int64_t c = 0; c < nc; ++c)
+        if (s->start[c + 1] - s->start[c] > max_nc) max_nc = s->start[c + 1] - s->start[c];
+#ifdef _OPENMP
+#pragma omp parallel
+#endif
+    {
+        int32_t* sub_list = (int32_t*)malloc(sizeof(int32_t) * max_nc);
+        int32_t* which = (int32_t*)malloc(sizeof(int32_t) * max_nc);
+        int64_t cnt[3] = { 0, 0, 0 };
+#ifdef _OPENMP
+#pragma omp for schedule(dynamic, 64)
+#endif
+        for (int64_t c = 0; c < nc; ++c) collide_cell(cfg, &d, s, c, sample, nc, active, which, sub_list, cnt);
+        OR_ATOMIC s->counters[0] += cnt[0];
+        OR_ATOMIC s->counters[1] += cnt[1];
+        OR_ATOMIC s->counters[6] += cnt[2];
+        free(sub_list);
+        free(which);
     }
     if (sample && s->samples) s->n_sampled += 1;
-    free(sub_list);
-    free(which);
     return 0;
 }
 

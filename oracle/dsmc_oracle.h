@@ -85,6 +85,10 @@ int64_t or_face_count(const hs_config* cfg);
 /* length of the circle (cx, cy, r) inside [x0,x1]x[y0,y1]; n_out (may be
  * NULL) gets the length-weighted unit normal pointing to the centre */
 double or_arc_in_rect(double cx, double cy, double r, double x0, double x1, double y0, double y1, double n_out[2]);
+/* threads the stages use: 1, or OpenMP's count in the threaded build */
+int or_threads(void);
+/* sets the thread count of the threaded build (no effect in the checker) */
+void or_set_threads(int n);
 int or_sort(const hs_config* cfg, or_state* s);
 int or_collide(const hs_config* cfg, or_state* s, int32_t sample);
 /* the NTC candidate pairs (P, Q) that or_collide draws for candidates

@@ -71,3 +71,47 @@ out2 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "couette_cloud_h
 np.savez_compressed(out2, xyz=xyz[:HEAD], cell=cells[:HEAD].astype(np.int32), U=np.array(U, np.float64).astype(np.float32),
                     erot=np.array(erot, np.float64).astype(np.float32), type=np.array(tid, np.uint8), vib=np.array(vib, np.uint16))
 print(f"{HEAD} parcels with U, ERot, typeId, vibLevel -> {out2}")
+
+# The whole snapshot's U, ERot, typeId and vibLevel, the cell state
+# dsmcSigmaTcRMax and the long-average fields the tutorial wrote at t = 5 s
+# (sampled from t = 0.5 s: system/fieldPropertiesDict resetAtOutputUntilTime
+# 0.5, 450,000 steps of 1e-5 s): rhoN, U, Ttra, Trot, Tvib, Tov and p of the
+# N2, O2 and mixture field models, internal cells only.
+full = {}
+for name, pat_, conv in (("U", r"^\(([-0-9.eE+]+) ([-0-9.eE+]+) ([-0-9.eE+]+)\)\s*$", None),
+                         ("ERot", r"^([-0-9.eE+]+)\s*$", None), ("typeId", r"^(\d+)\s*$", None),
+                         ("vibLevel", r"^1\((\d+)\)\s*$", None)):
+    vals = []
+    for line in entries(name):
+        m = re.match(pat_, line)
+        if m:
+            vals.append([float(g) for g in m.groups()])
+    full[name] = np.array(vals)
+assert all(len(v) == len(cells) for v in full.values())
+
+
+def internal_field(path_):
+    txt = open(path_).read()
+    i = txt.index("internalField")
+    j = txt.index("(", i)
+    n = int(txt[txt.index("\n", i):j].strip())
+    body = txt[j + 1:txt.index("\n)", j)]
+    if "List<vector>" in txt[i:j]:
+        v = np.array([[float(x) for x in m] for m in re.findall(r"\(([-0-9.eE+]+) ([-0-9.eE+]+) ([-0-9.eE+]+)\)", body)])
+    else:
+        v = np.array([float(x) for x in body.split()])
+    assert len(v) == n, (path_, len(v), n)
+    return v
+
+
+b5 = os.path.dirname(os.path.dirname(d))
+fld = {}
+for group in ("N2", "O2", "mixture"):
+    for q in ("rhoN", "U", "Ttra", "Trot", "Tvib", "Tov", "p"):
+        fld[f"{q}_{group}"] = internal_field(os.path.join(b5, f"{q}_{group}"))
+sig = internal_field(os.path.join(b5, "dsmcSigmaTcRMax"))
+out3 = os.path.join(os.path.dirname(os.path.abspath(__file__)), "couette_backup5.npz")
+np.savez_compressed(out3, U=full["U"].astype(np.float32), erot=full["ERot"][:, 0].astype(np.float32),
+                    type=full["typeId"][:, 0].astype(np.uint8), vib=full["vibLevel"][:, 0].astype(np.uint16),
+                    sigma_tcr_max=sig, **fld)
+print(f"{len(cells)} parcels, dsmcSigmaTcRMax and {len(fld)} long-average fields -> {out3}")

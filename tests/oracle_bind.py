@@ -12,7 +12,9 @@ from hystrath_amd import abi
 from hystrath_amd.cases import Particles
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(ROOT, "oracle", "_build", "libdsmc_oracle.so")
+# HS_ORACLE_LIB points the tests at another build of the same source (used to
+# check, on the CPU, that the tests catch a mistake planted in a copy)
+LIB_PATH = os.environ.get("HS_ORACLE_LIB") or os.path.join(ROOT, "oracle", "_build", "libdsmc_oracle.so")
 
 OR_RNG_MOVE, OR_RNG_COLLIDE, OR_RNG_INJECT, OR_RNG_FILL_COUNT, OR_RNG_FILL_GEN, OR_RNG_REMAINDER = range(1, 7)
 
@@ -35,15 +37,17 @@ class or_state(C.Structure):
     ]
 
 
-_lib = None
+_libs = {}
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise RuntimeError(f"oracle not built: {LIB_PATH} (run `make -C oracle`)")
-        L = C.CDLL(LIB_PATH)
+def lib(path=None):
+    """The oracle library at path (default: the single-thread checker),
+    loaded once."""
+    path = path or LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"oracle not built: {path} (run `make -C oracle`)")
+        L = C.CDLL(path)
         P = C.POINTER
         cfgp = P(abi.hs_config)
         stp = P(or_state)
@@ -80,8 +84,10 @@ def lib():
         L.or_face_count.restype = C.c_int64
         L.or_ntc_pairs.argtypes = [cfgp, C.c_int64, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                    C.c_uint32, C.c_int64, P(C.c_int32), P(C.c_int32)]
-        _lib = L
-    return _lib
+        L.or_threads.restype = C.c_int
+        L.or_set_threads.argtypes = [C.c_int]
+        _libs[path] = L
+    return _libs[path]
 
 
 def dptr(a):
@@ -163,8 +169,9 @@ def ntc_pairs(case, cell, pos, step, ncand):
 class OracleState:
     """An or_state with numpy storage, mirroring the engine's state."""
 
-    def __init__(self, case, capacity=None):
+    def __init__(self, case, capacity=None, lib_path=None):
         self.case = case
+        self.L = lib(lib_path)
         cap = capacity or case.cfg.capacity
         self.cap = cap
         self.P = Particles.for_case(case, cap)
@@ -172,16 +179,16 @@ class OracleState:
         nc = case.ncells
         self.sigma = np.zeros(nc)
         self.rem = np.zeros(nc)
-        self.nslots = lib().or_sample_slots(C.byref(case.cfg))
+        self.nslots = self.L.or_sample_slots(C.byref(case.cfg))
         self.samples = np.zeros(self.nslots * nc)
-        nf = lib().or_inflow_face_count(C.byref(case.cfg))
+        nf = self.L.or_inflow_face_count(C.byref(case.cfg))
         self.nfaces = nf
         self.inflow = np.zeros(max(case.cfg.n_species * nf, 1)) if nf else None
         self.count = np.zeros(nc, np.int32)
         self.start = np.zeros(nc + 1, np.int32)
-        self.n_wall = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
+        self.n_wall = self.L.or_wall_faces(C.byref(case.cfg), None, None, None, None)
         self.wall = np.zeros(max(case.cfg.n_species * abi.HS_WS_COUNT * self.n_wall, 1))
-        self.n_faces = lib().or_face_count(C.byref(case.cfg))
+        self.n_faces = self.L.or_face_count(C.byref(case.cfg))
         self.flux = np.zeros((case.cfg.n_species, self.n_faces), np.int64)
         s = or_state()
         s.n = 0
@@ -217,7 +224,7 @@ class OracleState:
         nd = np.zeros(abi.HS_MAX_SPECIES)
         nd[: len(n_density)] = n_density
         uu = np.array(u, np.float64)
-        lib().or_fill(C.byref(self.case.cfg), C.byref(self.s), dptr(nd), T, Trot if Trot is not None else T,
+        self.L.or_fill(C.byref(self.case.cfg), C.byref(self.s), dptr(nd), T, Trot if Trot is not None else T,
                       Tvib if Tvib is not None else T, dptr(uu), 1 if exact else 0, fill_id)
 
     def load(self, d, sigma=None, rem=None):
@@ -231,19 +238,19 @@ class OracleState:
         return self.P.view(self.s.n)
 
     def inject(self):
-        lib().or_inject(C.byref(self.case.cfg), C.byref(self.s))
+        self.L.or_inject(C.byref(self.case.cfg), C.byref(self.s))
 
     def move(self, n_old=None):
-        lib().or_move(C.byref(self.case.cfg), C.byref(self.s), self.s.n if n_old is None else n_old)
+        self.L.or_move(C.byref(self.case.cfg), C.byref(self.s), self.s.n if n_old is None else n_old)
 
     def sort(self):
-        lib().or_sort(C.byref(self.case.cfg), C.byref(self.s))
+        self.L.or_sort(C.byref(self.case.cfg), C.byref(self.s))
 
     def collide(self, sample=True):
-        lib().or_collide(C.byref(self.case.cfg), C.byref(self.s), 1 if sample else 0)
+        self.L.or_collide(C.byref(self.case.cfg), C.byref(self.s), 1 if sample else 0)
 
     def evolve(self, nsteps):
-        lib().or_evolve(C.byref(self.case.cfg), C.byref(self.s), nsteps)
+        self.L.or_evolve(C.byref(self.case.cfg), C.byref(self.s), nsteps)
 
     def sample_array(self):
         return self.samples.reshape(self.nslots, self.case.ncells)
@@ -254,16 +261,16 @@ class OracleState:
 
 
 def wall_faces(case):
-    n = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
+    n = self.L.or_wall_faces(C.byref(case.cfg), None, None, None, None)
     side = np.zeros(n, np.int32)
     cell = np.zeros(n, np.int32)
     area = np.zeros(n)
     normal = np.zeros((n, 3))
-    lib().or_wall_faces(C.byref(case.cfg), iptr(side), iptr(cell), dptr(area), dptr(normal))
+    self.L.or_wall_faces(C.byref(case.cfg), iptr(side), iptr(cell), dptr(area), dptr(normal))
     return dict(side=side, cell=cell, area=area, normal=normal)
 
 
 def arc_in_rect(cx, cy, r, x0, x1, y0, y1):
     nrm = np.zeros(2)
-    length = lib().or_arc_in_rect(cx, cy, r, x0, x1, y0, y1, dptr(nrm))
+    length = self.L.or_arc_in_rect(cx, cy, r, x0, x1, y0, y1, dptr(nrm))
     return length, nrm
