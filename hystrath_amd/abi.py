@@ -9,6 +9,7 @@ import ctypes as C
 HS_MAX_SPECIES = 8
 HS_MAX_VIB_MODES = 3
 HS_KB = 1.38064852e-23
+HS_FILL_NOALLOC_MAX_PAIRS = 500000
 
 HS_OK = 0
 HS_E_INVALID = -1

@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -30,6 +31,8 @@ struct TimedPair {
 };
 
 }  // namespace
+
+static int scratch_on_device(int dev, void** out);
 
 struct hs_dsmc_engine {
     hs_config cfg;
@@ -504,6 +507,11 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         hs_dsmc_destroy(eng);
         return code;
     };
+    {
+        // the fixed scratch of hs_dsmc_fill_noalloc on this device
+        void* sp = nullptr;
+        if (scratch_on_device(eng->device, &sp)) return bail(fail(eng, HS_E_HIP, "hipGetSymbolAddress failed"));
+    }
     if (hipStreamCreateWithFlags(&eng->stream, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&eng->side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&eng->ev_scanned, hipEventDisableTiming) != hipSuccess ||
@@ -796,55 +804,124 @@ extern "C" int hs_dsmc_get_step(hs_dsmc_engine* eng, uint32_t* step) {
 // ---------------------------------------------------------------------------
 // initial fill
 // ---------------------------------------------------------------------------
-extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib,
-                            const double u[3], int32_t exact, uint32_t fill_id) {
+// The fill's device scratch: per-(cell, species) counts and append offsets,
+// the number densities and the drift velocity.
+struct FillScratch {
+    int32_t* cnt;
+    int32_t* off;
+    double* dn;
+    double* du;
+};
+
+static int fill_checks(hs_dsmc_engine* eng, const double* n, double t, const double u[3]) {
     if (!eng || !n || !u) return fail(eng, HS_E_INVALID, "NULL argument");
     if (!(t >= 0.0)) return fail(eng, HS_E_INVALID, "temperature must be non-negative");
-    HS_TRY(hipSetDevice(eng->device));
-    const int nc = eng->nc, ns = eng->ns;
     // species-independent expected count guards the capacity
     double expect = 0.0;
-    for (int s = 0; s < ns; ++s) {
+    for (int s = 0; s < eng->ns; ++s) {
         if (n[s] < 0.0) return fail(eng, HS_E_INVALID, "number density must be non-negative");
-        for (int c = 0; c < nc; ++c) expect += n[s] * eng->vol_h[c] / eng->cfg.fn;
+        for (int c = 0; c < eng->nc; ++c) expect += n[s] * eng->vol_h[c] / eng->cfg.fn;
     }
     if (expect > (double)eng->cap) return fail(eng, HS_E_CAPACITY, "fill would exceed the particle capacity");
-    int32_t* cnt = nullptr;
-    int32_t* off = nullptr;
-    double* dn = nullptr;
-    double* du = nullptr;
+    return 0;
+}
+
+// Every scratch entry the kernels read is written first in this call: dn and
+// du by the copies, cnt by k_fill_count (all nc*ns entries), off by the scan.
+static int fill_run(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib, const double u[3],
+                    int32_t exact, uint32_t fill_id, const FillScratch& w) {
+    const int nc = eng->nc, ns = eng->ns;
     hipStream_t st = eng->stream;
-    HS_TRY(hipMalloc(&cnt, (size_t)nc * ns * 4));
-    HS_TRY(hipMalloc(&off, ((size_t)nc * ns + 1) * 4));
-    HS_TRY(hipMalloc(&dn, HS_MAX_SPECIES * 8));
-    HS_TRY(hipMalloc(&du, 3 * 8));
     double nh[HS_MAX_SPECIES] = { 0 };
     for (int s = 0; s < ns; ++s) nh[s] = n[s];
-    int rc = 0;
-    if (hipMemcpyAsync(dn, nh, sizeof(nh), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(du, u, 24, hipMemcpyHostToDevice, st) != hipSuccess) {
-        rc = fail(eng, HS_E_HIP, "hipMemcpy in fill failed");
-    } else {
-        launch_fill_count(eng->dp, dn, exact, fill_id, eng->d_vol, cnt, nc, ns, st);
-        launch_scan(cnt, off, nc * ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap, st);
-        launch_fill_gen(eng->dp, dn, t, trot, tvib, du, fill_id, cnt, off, nc, ns, cur_buf(eng), eng->d_S, st);
-        launch_set_nold(eng->d_S, st);
-        int most = 0;
-        for (int s = 1; s < ns; ++s)
-            if (n[s] > n[most]) most = s;
-        const hs_species& sm = eng->cfg.species[most];
-        const double smax = (3.14159265358979323846 * sm.diameter * sm.diameter) * sqrt(2.0 * HS_KB * t / sm.mass);
-        launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, st);
-        rc = check_launch(eng, "fill");
-        if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(eng, HS_E_HIP, "fill failed");
-        DevScalars sc;
-        if (!rc) rc = read_scalars(eng, &sc);
-    }
-    (void)hipFree(cnt);
-    (void)hipFree(off);
-    (void)hipFree(dn);
-    (void)hipFree(du);
+    if (hipMemcpyAsync(w.dn, nh, sizeof(nh), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipMemcpyAsync(w.du, u, 24, hipMemcpyHostToDevice, st) != hipSuccess)
+        return fail(eng, HS_E_HIP, "hipMemcpy in fill failed");
+    launch_fill_count(eng->dp, w.dn, exact, fill_id, eng->d_vol, w.cnt, nc, ns, st);
+    launch_scan(w.cnt, w.off, nc * ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap, st);
+    launch_fill_gen(eng->dp, w.dn, t, trot, tvib, w.du, fill_id, w.cnt, w.off, nc, ns, cur_buf(eng), eng->d_S, st);
+    launch_set_nold(eng->d_S, st);
+    int most = 0;
+    for (int s = 1; s < ns; ++s)
+        if (n[s] > n[most]) most = s;
+    const hs_species& sm = eng->cfg.species[most];
+    const double smax = (3.14159265358979323846 * sm.diameter * sm.diameter) * sqrt(2.0 * HS_KB * t / sm.mass);
+    launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, st);
+    int rc = check_launch(eng, "fill");
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(eng, HS_E_HIP, "fill failed");
+    DevScalars sc;
+    if (!rc) rc = read_scalars(eng, &sc);
     return rc;
+}
+
+extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib,
+                            const double u[3], int32_t exact, uint32_t fill_id) {
+    int rc = fill_checks(eng, n, t, u);
+    if (rc) return rc;
+    HS_TRY(hipSetDevice(eng->device));
+    const size_t m = (size_t)eng->nc * eng->ns;
+    FillScratch w = { nullptr, nullptr, nullptr, nullptr };
+    if (hipMalloc(&w.cnt, m * 4) != hipSuccess || hipMalloc(&w.off, (m + 1) * 4) != hipSuccess ||
+        hipMalloc(&w.dn, HS_MAX_SPECIES * 8) != hipSuccess || hipMalloc(&w.du, 3 * 8) != hipSuccess)
+        rc = fail(eng, HS_E_HIP, "hipMalloc in fill failed");
+    else
+        rc = fill_run(eng, n, t, trot, tvib, u, exact, fill_id, w);
+    (void)hipFree(w.cnt);
+    (void)hipFree(w.off);
+    (void)hipFree(w.dn);
+    (void)hipFree(w.du);
+    return rc;
+}
+
+// ---------------------------------------------------------------------------
+// the fill without allocation: its scratch in a fixed device array
+// ---------------------------------------------------------------------------
+// Layout: dn [0, 64), du [64, 88), cnt from 256, off after cnt rounded up to
+// 256 bytes.  HS_FILL_NOALLOC_MAX_PAIRS cell-species pairs fit.
+constexpr size_t kScratchBytes = 256 + 4 * (size_t)HS_FILL_NOALLOC_MAX_PAIRS + 256 + 4 * ((size_t)HS_FILL_NOALLOC_MAX_PAIRS + 1);
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = { nullptr };
+}  // namespace
+
+// The device address of this device's copy of `scratch`, resolved once per
+// device with that device current (the array's name is not a device address
+// in host code).  0 on success.
+static int scratch_on_device(int dev, void** out) {
+    if (dev < 0 || dev >= kMaxDevices) return -1;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return -1;
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+extern "C" int hs_dsmc_fill_noalloc(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib,
+                                    const double u[3], int32_t exact, uint32_t fill_id) {
+    int rc = fill_checks(eng, n, t, u);
+    if (rc) return rc;
+    const int64_t m = (int64_t)eng->nc * eng->ns;
+    if (m > HS_FILL_NOALLOC_MAX_PAIRS)
+        return fail(eng, HS_E_INVALID, "hs_dsmc_fill_noalloc: " + std::to_string(eng->nc) + " cells x " +
+                    std::to_string(eng->ns) + " species exceed its fixed scratch of " +
+                    std::to_string(HS_FILL_NOALLOC_MAX_PAIRS) + " cell-species pairs (use hs_dsmc_fill)");
+    HS_TRY(hipSetDevice(eng->device));
+    unsigned char* base = nullptr;
+    if (scratch_on_device(eng->device, (void**)&base))
+        return fail(eng, HS_E_HIP, "hipGetSymbolAddress of the fill scratch failed");
+    const size_t cnt_bytes = ((4 * (size_t)m) + 255) / 256 * 256;
+    FillScratch w;
+    w.dn = (double*)base;
+    w.du = (double*)(base + 64);
+    w.cnt = (int32_t*)(base + 256);
+    w.off = (int32_t*)(base + 256 + cnt_bytes);
+    return fill_run(eng, n, t, trot, tvib, u, exact, fill_id, w);
 }
 
 // ---------------------------------------------------------------------------

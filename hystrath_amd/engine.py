@@ -21,7 +21,7 @@ EXPORTS = [
     "hs_dsmc_device_count", "hs_dsmc_upload_particles", "hs_dsmc_download_particles", "hs_dsmc_particle_count",
     "hs_dsmc_set_cell_state", "hs_dsmc_get_cell_state", "hs_dsmc_inflow_face_count",
     "hs_dsmc_set_inflow_accumulators", "hs_dsmc_get_inflow_accumulators", "hs_dsmc_set_step", "hs_dsmc_get_step",
-    "hs_dsmc_fill", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
+    "hs_dsmc_fill", "hs_dsmc_fill_noalloc", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
     "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples",
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
@@ -70,6 +70,8 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_get_step": ([eng, P(C.c_uint32)], C.c_int),
         "hs_dsmc_fill": ([eng, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32,
                           C.c_uint32], C.c_int),
+        "hs_dsmc_fill_noalloc": ([eng, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double),
+                                  C.c_int32, C.c_uint32], C.c_int),
         "hs_dsmc_evolve": ([eng, C.c_int32], C.c_int),
         "hs_dsmc_inject": ([eng], C.c_int),
         "hs_dsmc_move": ([eng], C.c_int),

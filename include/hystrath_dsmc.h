@@ -237,6 +237,20 @@ int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot,
                  double tvib, const double u[3], int32_t exact_per_cell,
                  uint32_t fill_id);
 
+/* The same fill with no allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops re-seeding populations, harnesses that forbid
+ * hipMalloc).  Its scratch (per cell-species counts and offsets) lives in a
+ * fixed device array of the library, one per GPU, held until the process
+ * ends; it holds at most HS_FILL_NOALLOC_MAX_PAIRS cell-species pairs
+ * (nx*ny*nz*n_species), beyond which the call fails with HS_E_INVALID.  Every
+ * call on a GPU shares that GPU's array: calls that overlap on one GPU, from
+ * engines used on different threads, race.  Results equal hs_dsmc_fill's. */
+#define HS_FILL_NOALLOC_MAX_PAIRS 500000
+int hs_dsmc_fill_noalloc(hs_dsmc_engine* eng, const double* n, double t, double trot,
+                         double tvib, const double u[3], int32_t exact_per_cell,
+                         uint32_t fill_id);
+
 /* ---- the timestep ------------------------------------------------------- */
 /* Replaces dsmcCloud::evolve (dsmcCloud.C:819-926) nsteps times: freestream
  * injection (dsmcFreeStreamInflowPatch::controlParcelsBeforeMove), move
