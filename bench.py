@@ -201,9 +201,10 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
     case_j = case_fn(seed=20251202 + 1)
     with Engine(case_j) as ej:
         fill(ej)
-        ej.set_flags(case_j.cfg.flags & ~1)
+        flags = case_j.cfg.flags
+        ej.set_flags(flags & ~1)
         run_evolve(ej, dev, what="judge development")
-        ej.set_flags(case_j.cfg.flags)
+        ej.set_flags(flags)
         ej.reset_samples()
         run_evolve(ej, nj, what="judge average")
         s, n = ej.samples()
@@ -212,9 +213,10 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
     t0 = time.perf_counter()
     with Engine(case) as e:
         fill(e)
-        e.set_flags(case.cfg.flags & ~1)
+        flags = case.cfg.flags
+        e.set_flags(flags & ~1)
         run_evolve(e, dev, what="plain development")
-        e.set_flags(case.cfg.flags)
+        e.set_flags(flags)
         e.reset_samples()
         done, err, trace = 0, None, []
         while done < args.t2_max:

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libhystrath_dsmc.so")
-SOURCES = ["dsmc_engine.hip", "dsmc_kernels.hip"]
+SOURCES = ["dsmc_engine.hip", "dsmc_kernels.hip", "ws_kernels.hip"]
 
 # -ffp-contract=off: every a*b+c is two IEEE roundings (bit-reproducible FP64)
 # -packed-fp32-ops off: no v_pk_*_f32 generated

@@ -280,15 +280,16 @@ def periodic_box(n_side=20, L=0.1, ppc=20, n_density=1e20, T_hot=1000.0, dt=5.77
 
 def cylinder_flow(nx=400, ny=400, Lx=1.0, Ly=1.0, x_cyl=0.35, R=0.1, n_inf=4.2e20, T_inf=200.0,
                   mach=10.0, T_wall=500.0, ppc=25, dt=2.5e-7, seed=7, capacity=None,
-                  flags=abi.HS_FLAG_SAMPLE):
+                  flags=abi.HS_FLAG_SAMPLE, species=None, collision_model=abi.HS_COLL_VHS):
     """2D Mach-10 argon flow over a cylinder of radius R: a Cartesian block
     of nx*ny cells (one cell thick in z, empty), the cylinder cut out exactly,
     diffuse isothermal wall at T_wall, freestream inflow + deletion on the
-    four outer sides."""
-    ar = make_species("Ar")
+    four outer sides.  species (default argon) may be a molecule; its
+    internal temperatures at the inflow equal T_inf."""
+    ar = species or make_species("Ar")
     dx, dy = Lx / nx, Ly / ny
     dz = dx
-    gamma = 5.0 / 3.0
+    gamma = (5.0 + ar.rot_dof) / (3.0 + ar.rot_dof)
     a_inf = math.sqrt(gamma * KB * T_inf / ar.mass)
     U = mach * a_inf
     fn = n_inf * dx * dy * dz / ppc
@@ -298,7 +299,7 @@ def cylinder_flow(nx=400, ny=400, Lx=1.0, Ly=1.0, x_cyl=0.35, R=0.1, n_inf=4.2e2
     origin = (-x_cyl, -0.5 * Ly, 0.0)
     cap = capacity or int(nx * ny * ppc * 1.6) + (1 << 16)
     case = Case(nx, ny, 1, origin, (dx, dy, dz), [ar], [outer, outer, outer, outer, empty, empty],
-                collision_model=abi.HS_COLL_VHS, fn=fn, dt=dt, seed=seed, capacity=cap, flags=flags,
+                collision_model=collision_model, fn=fn, dt=dt, seed=seed, capacity=cap, flags=flags,
                 cyl=(0.0, 0.0, R), cyl_patch=wall,
                 inflow=dict(n=[n_inf], T=T_inf, U=(U, 0.0, 0.0)), name="cylinder_flow")
     case.n_inf, case.T_inf, case.U_inf = n_inf, T_inf, U
@@ -347,27 +348,34 @@ def wall_channel(nx=4, ny=20, L=0.02, n_density=1e21, T_gas=600.0, T_lo=300.0, T
     return case
 
 
-def cylinder_batch(count=64, seed=20251203, d_ref=4.17e-10):
-    """count independent argon cylinder cases of uneven size: Mach uniform in
-    [5, 25], Knudsen number (mean free path over the cylinder's diameter)
-    log-uniform in [0.01, 1], square meshes of 50 to 300 cells a side and 6 to
-    25 parcels per freestream cell (log-uniform), so 10k to 2M parcels a case.
-    The time step keeps the freestream at a quarter of a cell per step."""
+def cylinder_batch(count=64, seed=20251203, d_ref=4.17e-10, species=("Ar", "N2")):
+    """count independent cylinder cases of uneven size: argon (VHS) and N2
+    (LB-VHS, rotation and vibration) in turn, Mach uniform in [5, 25],
+    Knudsen number (mean free path over the cylinder's diameter) log-uniform
+    in [0.01, 1], square meshes of 50 to 300 cells a side (drawn with density
+    proportional to the side, so larger meshes are as common as their share
+    of cells) and 4 to 22 parcels per freestream cell: 10k to 2M parcels a
+    case, about 37M in all for 64 cases.  The time step keeps the freestream
+    at a quarter of a cell per step."""
     rng = np.random.default_rng(seed)
     out = []
-    ar = make_species("Ar")
     for k in range(count):
-        n_side = int(rng.integers(50, 301))
+        name = species[k % len(species)]
+        sp = make_species(name)
+        n_side = int(round(math.sqrt(rng.uniform(50.0 ** 2, 300.0 ** 2))))
         mach = float(rng.uniform(5.0, 25.0))
         kn = float(10.0 ** rng.uniform(-2.0, 0.0))
-        ppc = float(np.exp(rng.uniform(np.log(6.0), np.log(25.0))))
+        ppc = float(rng.uniform(4.0, 22.0))
         n_inf = 1.0 / (math.sqrt(2.0) * math.pi * d_ref * d_ref * kn * 0.2)
         T_inf = 200.0
-        U = mach * math.sqrt(5.0 / 3.0 * KB * T_inf / ar.mass)
+        gamma = (5.0 + sp.rot_dof) / (3.0 + sp.rot_dof)
+        U = mach * math.sqrt(gamma * KB * T_inf / sp.mass)
         dt = 0.25 * (1.0 / n_side) / U
         case = cylinder_flow(nx=n_side, ny=n_side, n_inf=n_inf, T_inf=T_inf, mach=mach, ppc=ppc, dt=dt,
                              seed=seed + k, capacity=int(n_side * n_side * ppc * 2.5) + (1 << 16),
-                             flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
-        case.batch_params = dict(n_side=n_side, mach=round(mach, 3), kn=round(kn, 5), ppc=round(ppc, 2))
+                             flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL, species=sp,
+                             collision_model=abi.HS_COLL_LB_VHS if sp.rot_dof > 0 else abi.HS_COLL_VHS)
+        case.batch_params = dict(species=name, n_side=n_side, mach=round(mach, 3), kn=round(kn, 5),
+                                 ppc=round(ppc, 2))
         out.append(case)
     return out

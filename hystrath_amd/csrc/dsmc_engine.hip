@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -828,8 +829,10 @@ static int fill_checks(hs_dsmc_engine* eng, const double* n, double t, const dou
 
 // Every scratch entry the kernels read is written first in this call: dn and
 // du by the copies, cnt by k_fill_count (all nc*ns entries), off by the scan.
+// fc and sig_cell (device arrays, or NULL) give per-cell inputs.
 static int fill_run(hs_dsmc_engine* eng, const double* n, double t, double trot, double tvib, const double u[3],
-                    int32_t exact, uint32_t fill_id, const FillScratch& w) {
+                    int32_t exact, uint32_t fill_id, const FillScratch& w, FillCells fc = { nullptr, nullptr, nullptr },
+                    const double* sig_cell = nullptr) {
     const int nc = eng->nc, ns = eng->ns;
     hipStream_t st = eng->stream;
     double nh[HS_MAX_SPECIES] = { 0 };
@@ -837,16 +840,17 @@ static int fill_run(hs_dsmc_engine* eng, const double* n, double t, double trot,
     if (hipMemcpyAsync(w.dn, nh, sizeof(nh), hipMemcpyHostToDevice, st) != hipSuccess ||
         hipMemcpyAsync(w.du, u, 24, hipMemcpyHostToDevice, st) != hipSuccess)
         return fail(eng, HS_E_HIP, "hipMemcpy in fill failed");
-    launch_fill_count(eng->dp, w.dn, exact, fill_id, eng->d_vol, w.cnt, nc, ns, st);
+    launch_fill_count(eng->dp, w.dn, exact, fill_id, eng->d_vol, w.cnt, nc, ns, fc, st);
     launch_scan(w.cnt, w.off, nc * ns, SCAN_APPEND, eng->ws, eng->d_S, eng->cap, st);
-    launch_fill_gen(eng->dp, w.dn, t, trot, tvib, w.du, fill_id, w.cnt, w.off, nc, ns, cur_buf(eng), eng->d_S, st);
+    launch_fill_gen(eng->dp, w.dn, t, trot, tvib, w.du, fill_id, w.cnt, w.off, nc, ns, cur_buf(eng), eng->d_S, fc,
+                    st);
     launch_set_nold(eng->d_S, st);
     int most = 0;
     for (int s = 1; s < ns; ++s)
         if (n[s] > n[most]) most = s;
     const hs_species& sm = eng->cfg.species[most];
     const double smax = (3.14159265358979323846 * sm.diameter * sm.diameter) * sqrt(2.0 * HS_KB * t / sm.mass);
-    launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, st);
+    launch_fill_cells(eng->dp, smax, fill_id, eng->d_sigma, eng->d_rem, nc, sig_cell, st);
     int rc = check_launch(eng, "fill");
     if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = fail(eng, HS_E_HIP, "fill failed");
     DevScalars sc;
@@ -870,6 +874,58 @@ extern "C" int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, doub
     (void)hipFree(w.off);
     (void)hipFree(w.dn);
     (void)hipFree(w.du);
+    return rc;
+}
+
+// The fill with each cell's own state: used to start a population from
+// fields (the warm start of a case from predicted long-average fields).
+extern "C" int hs_dsmc_fill_cells(hs_dsmc_engine* eng, const double* n_cell, const double* u_cell,
+                                  const double* t_cell, uint32_t fill_id) {
+    if (!eng || !n_cell || !u_cell || !t_cell) return fail(eng, HS_E_INVALID, "NULL argument");
+    const int nc = eng->nc, ns = eng->ns;
+    double expect = 0.0;
+    std::vector<double> sig((size_t)nc);
+    for (int c = 0; c < nc; ++c) {
+        if (!(t_cell[c] >= 0.0) || !std::isfinite(t_cell[c]))
+            return fail(eng, HS_E_INVALID, "fill_cells: temperatures must be finite and non-negative");
+        for (int a = 0; a < 3; ++a)
+            if (!std::isfinite(u_cell[(size_t)a * nc + c]))
+                return fail(eng, HS_E_INVALID, "fill_cells: velocities must be finite");
+        int most = 0;
+        for (int s = 0; s < ns; ++s) {
+            const double v = n_cell[(size_t)s * nc + c];
+            if (!(v >= 0.0) || !std::isfinite(v))
+                return fail(eng, HS_E_INVALID, "fill_cells: number densities must be finite and non-negative");
+            expect += v * eng->vol_h[c] / eng->cfg.fn;
+            if (v > n_cell[(size_t)most * nc + c]) most = s;
+        }
+        const hs_species& sm = eng->cfg.species[most];
+        sig[c] = (3.14159265358979323846 * sm.diameter * sm.diameter) * sqrt(2.0 * HS_KB * t_cell[c] / sm.mass);
+    }
+    if (expect > (double)eng->cap) return fail(eng, HS_E_CAPACITY, "fill would exceed the particle capacity");
+    HS_TRY(hipSetDevice(eng->device));
+    const size_t m = (size_t)nc * ns;
+    FillScratch w = { nullptr, nullptr, nullptr, nullptr };
+    double *dn = nullptr, *du = nullptr, *dt = nullptr, *ds = nullptr;
+    int rc = 0;
+    const double n0[HS_MAX_SPECIES] = { 0 };
+    const double u0[3] = { 0.0, 0.0, 0.0 };
+    hipStream_t st = eng->stream;
+    if (hipMalloc(&w.cnt, m * 4) != hipSuccess || hipMalloc(&w.off, (m + 1) * 4) != hipSuccess ||
+        hipMalloc(&w.dn, HS_MAX_SPECIES * 8) != hipSuccess || hipMalloc(&w.du, 3 * 8) != hipSuccess ||
+        hipMalloc(&dn, m * 8) != hipSuccess || hipMalloc(&du, (size_t)3 * nc * 8) != hipSuccess ||
+        hipMalloc(&dt, (size_t)nc * 8) != hipSuccess || hipMalloc(&ds, (size_t)nc * 8) != hipSuccess) {
+        rc = fail(eng, HS_E_HIP, "hipMalloc in fill_cells failed");
+    } else if (hipMemcpyAsync(dn, n_cell, m * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(du, u_cell, (size_t)3 * nc * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(dt, t_cell, (size_t)nc * 8, hipMemcpyHostToDevice, st) != hipSuccess ||
+               hipMemcpyAsync(ds, sig.data(), (size_t)nc * 8, hipMemcpyHostToDevice, st) != hipSuccess) {
+        rc = fail(eng, HS_E_HIP, "hipMemcpy in fill_cells failed");
+    } else {
+        rc = fill_run(eng, n0, 0.0, 0.0, 0.0, u0, 0, fill_id, w, FillCells{ dn, du, dt }, ds);
+    }
+    for (void* p : { (void*)w.cnt, (void*)w.off, (void*)w.dn, (void*)w.du, (void*)dn, (void*)du, (void*)dt, (void*)ds })
+        if (p) (void)hipFree(p);
     return rc;
 }
 
@@ -992,8 +1048,38 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
 // cut into smaller ones (halving, down to 256 parcels) until it makes about
 // four workgroups per compute unit.  The tiling changes no result: the random
 // streams belong to cells and candidates, and each cell is sampled whole.
+// Tile sizes the autotuner measured, per kernel variant (internal energies or
+// not) and population class (log2 of the parcel count): later engines of the
+// same class on any device of the process start from them.
+namespace {
+std::mutex g_tune_mu;
+struct TuneKey {
+    int general, pop_log2;
+    bool operator<(const TuneKey& o) const {
+        return general != o.general ? general < o.general : pop_log2 < o.pop_log2;
+    }
+};
+std::map<TuneKey, std::pair<int, int>> g_tuned;
+int pop_class(int64_t n) {
+    int k = 0;
+    while (n > 1 && k < 62) { n >>= 1; ++k; }
+    return k;
+}
+}  // namespace
+
+static bool tuned_tiles(const hs_dsmc_engine* eng, int* g, int* c) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tuned.find(TuneKey{ eng->general, pop_class(eng->n_hint) });
+    if (it == g_tuned.end()) return false;
+    *g = it->second.first;
+    *c = it->second.second;
+    return true;
+}
+
 static int tile_for(const hs_dsmc_engine* eng, int which) {
     if (eng->tile_req[which] > 0) return eng->tile_req[which];
+    int tg = 0, tc = 0;
+    if (tuned_tiles(eng, &tg, &tc)) return which == 0 ? tg : tc;
     const int64_t per = eng->n_hint / (4 * (int64_t)eng->ncu);
     int pt = eng->tile_max[which];
     while (pt / 2 >= 256 && pt > per) pt /= 2;
@@ -1186,6 +1272,67 @@ extern "C" int hs_dsmc_set_tiles(hs_dsmc_engine* eng, int32_t gather_parcels, in
     return 0;
 }
 
+// Times steps of the live state under each candidate tiling (gather and
+// collide parcels per workgroup, powers of two from 256 and the full-device
+// sizes, skipping tilings of fewer workgroups than compute units) and keeps
+// the fastest.  The steps taken are ordinary steps of the run: the tiling
+// changes no result.
+extern "C" int hs_dsmc_autotune(hs_dsmc_engine* eng, int32_t steps_per_trial, int32_t* gather_out,
+                                int32_t* collide_out, double* ms_out) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    if (steps_per_trial < 1 || steps_per_trial > 10000)
+        return fail(eng, HS_E_INVALID, "autotune: steps_per_trial must be in [1, 10000]");
+    HS_TRY(hipSetDevice(eng->device));
+    int64_t n = 0;
+    if (int rc0 = hs_dsmc_particle_count(eng, &n)) return rc0;
+    std::vector<int> gs, cs;
+    for (int v : { 256, 512, 1024, 1536 })
+        if (v <= eng->tile_max[0] && (v == 256 || n / v >= eng->ncu)) gs.push_back(v);
+    for (int v : { 256, 512, 1024, 1536, 2048 })
+        if (v <= eng->tile_max[1] && (v == 256 || n / v >= eng->ncu)) cs.push_back(v);
+    hipEvent_t a = nullptr, b = nullptr;
+    HS_TRY(hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(eng, HS_E_HIP, "hipEventCreate failed");
+    }
+    const int saved[2] = { eng->tile_req[0], eng->tile_req[1] };
+    double best = 1e300;
+    int bg = 0, bc = 0, rc = 0;
+    for (int g : gs) {
+        for (int c : cs) {
+            eng->tile_req[0] = g;
+            eng->tile_req[1] = c;
+            // two steps capture the graphs of this tiling, then the timed ones
+            if ((rc = hs_dsmc_evolve(eng, 2))) break;
+            if (hipEventRecord(a, eng->stream) != hipSuccess) { rc = fail(eng, HS_E_HIP, "hipEventRecord"); break; }
+            if ((rc = hs_dsmc_evolve(eng, steps_per_trial))) break;
+            float ms = 0.0f;
+            if (hipEventRecord(b, eng->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                hipEventElapsedTime(&ms, a, b) != hipSuccess) {
+                rc = fail(eng, HS_E_HIP, "timing a tiling failed");
+                break;
+            }
+            const double per = ms / steps_per_trial;
+            if (per < best) { best = per; bg = g; bc = c; }
+        }
+        if (rc) break;
+    }
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    eng->tile_req[0] = saved[0];
+    eng->tile_req[1] = saved[1];
+    if (rc) return rc;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        g_tuned[TuneKey{ eng->general, pop_class(eng->n_hint) }] = { bg, bc };
+    }
+    if (gather_out) *gather_out = bg;
+    if (collide_out) *collide_out = bc;
+    if (ms_out) *ms_out = best;
+    return 0;
+}
+
 extern "C" int hs_dsmc_get_tiles(hs_dsmc_engine* eng, int32_t* gather_parcels, int32_t* collide_parcels) {
     if (!eng || !gather_parcels || !collide_parcels) return fail(eng, HS_E_INVALID, "NULL argument");
     *gather_parcels = tile_for(eng, 0);
@@ -1268,6 +1415,29 @@ extern "C" int hs_dsmc_read_wall_samples(hs_dsmc_engine* eng, double* out, int64
     HS_TRY(hipStreamSynchronize(eng->stream));
     if (nsteps_out) *nsteps_out = eng->n_sampled;
     return 0;
+}
+
+extern "C" int hs_dsmc_set_samples(hs_dsmc_engine* eng, const double* cell_sums, int64_t n_cell,
+                                   const double* wall_sums, int64_t n_wall, int64_t nsteps) {
+    if (!eng) return fail(eng, HS_E_INVALID, "NULL engine");
+    const int64_t need_c = (int64_t)eng->nslots * eng->nc;
+    const int64_t need_w = (int64_t)eng->ns * HS_WS_COUNT * eng->hp.n_wall;
+    if (!cell_sums || n_cell != need_c)
+        return fail(eng, HS_E_INVALID, "set_samples: cell sums must hold exactly " + std::to_string(need_c) + " values");
+    if (wall_sums && n_wall != need_w)
+        return fail(eng, HS_E_INVALID, "set_samples: wall sums must hold exactly " + std::to_string(need_w) + " values");
+    if (nsteps < 0) return fail(eng, HS_E_INVALID, "set_samples: negative step count");
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipMemcpyAsync(eng->d_samples, cell_sums, need_c * 8, hipMemcpyHostToDevice, eng->stream));
+    if (need_w > 0) {
+        if (wall_sums)
+            HS_TRY(hipMemcpyAsync(eng->d_wall_sum, wall_sums, need_w * 8, hipMemcpyHostToDevice, eng->stream));
+        else
+            launch_fill_f64(eng->d_wall_sum, 0.0, need_w, eng->stream);
+    }
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    eng->n_sampled = nsteps;
+    return check_launch(eng, "set samples");
 }
 
 extern "C" int hs_dsmc_read_counters(hs_dsmc_engine* eng, int64_t* out, int32_t n) {

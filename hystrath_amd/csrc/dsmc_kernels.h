@@ -86,13 +86,20 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                     int general, int nc, uint16_t* big_list, hipStream_t st);
 void launch_clear_queue(DevScalars* S, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
+// per-cell fill inputs (NULL: the uniform values): number densities
+// [species][cell], velocities [axis][cell], temperatures [cell]
+struct FillCells {
+    const double* nd;
+    const double* u;
+    const double* t;
+};
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
-                       int32_t* cnt, int nc, int ns, hipStream_t st);
+                       int32_t* cnt, int nc, int ns, FillCells fc, hipStream_t st);
 void launch_fill_gen(const DevParams* P, const double* nd, double T, double Trot, double Tvib, const double* u3,
                      uint32_t fill_id, const int32_t* cnt, const int32_t* off, int nc, int ns, DevParticles b,
-                     DevScalars* S, hipStream_t st);
+                     DevScalars* S, FillCells fc, hipStream_t st);
 void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double* sigma, double* rem, int nc,
-                       hipStream_t st);
+                       const double* sig_cell, hipStream_t st);
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 void launch_fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t st);

@@ -1850,14 +1850,15 @@ void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st) {
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(BLOCK) void k_fill_count(const DevParams* __restrict__ Pp, const double* __restrict__ nd,
                                                       int exact, uint32_t fill_id, const double* __restrict__ vol,
-                                                      int32_t* __restrict__ cnt, int nc, int ns) {
+                                                      int32_t* __restrict__ cnt, int nc, int ns, FillCells fc) {
     const DevParams& P = *Pp;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= nc * ns) return;
     const int c = e / ns, s = e - c * ns;
     const double V = vol[c];
-    if (!(V > 0.0) || !(nd[s] > 0.0)) { cnt[e] = 0; return; }
-    const double req = nd[s] * V / P.fn;
+    const double nds = fc.nd ? fc.nd[(int64_t)s * nc + c] : nd[s];
+    if (!(V > 0.0) || !(nds > 0.0)) { cnt[e] = 0; return; }
+    const double req = nds * V / P.fn;
     int ni;
     if (exact) {
         ni = (int)floor(req + 0.5);
@@ -1874,7 +1875,7 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gen(const DevParams* __restrict_
                                                     double T, double Trot, double Tvib, const double* __restrict__ u3,
                                                     uint32_t fill_id, const int32_t* __restrict__ cnt,
                                                     const int32_t* __restrict__ off, int nc, int ns, DevParticles b,
-                                                    DevScalars* __restrict__ S) {
+                                                    DevScalars* __restrict__ S, FillCells fc) {
     const DevParams& P = *Pp;
     const int e = blockIdx.x * BLOCK + threadIdx.x;
     if (e >= nc * ns) return;
@@ -1885,7 +1886,10 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gen(const DevParams* __restrict_
     cell_ijk(P, c, idx);
     PhiloxStream r;
     r.init(P, RNG_FILL_GEN, (uint32_t)c, fill_id, (uint32_t)s);
-    const double sc = sqrt(HS_KB * T / P.mass[s]);
+    const double Tc = fc.t ? fc.t[c] : T;
+    const double uc0 = fc.u ? fc.u[c] : u3[0], uc1 = fc.u ? fc.u[nc + c] : u3[1],
+                 uc2 = fc.u ? fc.u[2 * (int64_t)nc + c] : u3[2];
+    const double sc = sqrt(HS_KB * Tc / P.mass[s]);
     const int64_t base = (int64_t)S->n_old + off[e];
     for (int k = 0; k < ni; ++k) {
         const int64_t j = base + k;
@@ -1905,43 +1909,45 @@ __global__ __launch_bounds__(BLOCK) void k_fill_gen(const DevParams* __restrict_
         }
         const double g0 = r.gauss(), g1 = r.gauss(), g2 = r.gauss();
         b.px[j] = pos[0]; b.py[j] = pos[1]; b.pz[j] = pos[2];
-        b.ux[j] = (float)(sc * g0 + u3[0]);
-        b.uy[j] = (float)(sc * g1 + u3[1]);
-        b.uz[j] = (float)(sc * g2 + u3[2]);
+        b.ux[j] = (float)(sc * g0 + uc0);
+        b.uy[j] = (float)(sc * g1 + uc1);
+        b.uz[j] = (float)(sc * g2 + uc2);
         b.cell[j] = ok ? c : -1;
         if (b.vib)
             for (int m = 0; m < P.max_modes; ++m) b.vib[m * P.capacity + j] = 0;
-        new_internal(P, b, j, s, Trot, Tvib, r);
+        new_internal(P, b, j, s, fc.t ? Tc : Trot, fc.t ? Tc : Tvib, r);
     }
 }
 
 __global__ __launch_bounds__(BLOCK) void k_fill_cells(const DevParams* __restrict__ Pp, double smax, uint32_t fill_id,
-                                                      double* __restrict__ sigma, double* __restrict__ rem, int nc) {
+                                                      double* __restrict__ sigma, double* __restrict__ rem, int nc,
+                                                      const double* __restrict__ sig_cell) {
     const DevParams& P = *Pp;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
-    sigma[c] = smax;
+    sigma[c] = sig_cell ? sig_cell[c] : smax;
     PhiloxStream r;
     r.init(P, RNG_REMAINDER, (uint32_t)c, fill_id, 0);
     rem[c] = r.u();
 }
 
 void launch_fill_count(const DevParams* P, const double* nd, int exact, uint32_t fill_id, const double* vol,
-                       int32_t* cnt, int nc, int ns, hipStream_t st) {
+                       int32_t* cnt, int nc, int ns, FillCells fc, hipStream_t st) {
     const int n = nc * ns;
-    k_fill_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, exact, fill_id, vol, cnt, nc, ns);
+    k_fill_count<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, exact, fill_id, vol, cnt, nc, ns, fc);
 }
 
 void launch_fill_gen(const DevParams* P, const double* nd, double T, double Trot, double Tvib, const double* u3,
                      uint32_t fill_id, const int32_t* cnt, const int32_t* off, int nc, int ns, DevParticles b,
-                     DevScalars* S, hipStream_t st) {
+                     DevScalars* S, FillCells fc, hipStream_t st) {
     const int n = nc * ns;
-    k_fill_gen<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, T, Trot, Tvib, u3, fill_id, cnt, off, nc, ns, b, S);
+    k_fill_gen<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, nd, T, Trot, Tvib, u3, fill_id, cnt, off, nc, ns, b, S,
+                                                          fc);
 }
 
 void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double* sigma, double* rem, int nc,
-                       hipStream_t st) {
-    k_fill_cells<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, smax, fill_id, sigma, rem, nc);
+                       const double* sig_cell, hipStream_t st) {
+    k_fill_cells<<<(nc + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, smax, fill_id, sigma, rem, nc, sig_cell);
 }
 
 // ---------------------------------------------------------------------------

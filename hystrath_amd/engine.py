@@ -21,12 +21,12 @@ EXPORTS = [
     "hs_dsmc_device_count", "hs_dsmc_upload_particles", "hs_dsmc_download_particles", "hs_dsmc_particle_count",
     "hs_dsmc_set_cell_state", "hs_dsmc_get_cell_state", "hs_dsmc_inflow_face_count",
     "hs_dsmc_set_inflow_accumulators", "hs_dsmc_get_inflow_accumulators", "hs_dsmc_set_step", "hs_dsmc_get_step",
-    "hs_dsmc_fill", "hs_dsmc_fill_noalloc", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
-    "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples",
+    "hs_dsmc_fill", "hs_dsmc_fill_cells", "hs_dsmc_fill_noalloc", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
+    "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples", "hs_dsmc_set_samples",
     "hs_dsmc_read_counters", "hs_dsmc_reset_counters", "hs_dsmc_kernel_times", "hs_dsmc_binary_collide",
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
     "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
-    "hs_dsmc_set_tiles", "hs_dsmc_get_tiles", "hs_dsmc_evolve_batch",
+    "hs_dsmc_set_tiles", "hs_dsmc_get_tiles", "hs_dsmc_autotune", "hs_dsmc_evolve_batch",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide", "gather"]
@@ -70,6 +70,7 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_get_step": ([eng, P(C.c_uint32)], C.c_int),
         "hs_dsmc_fill": ([eng, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32,
                           C.c_uint32], C.c_int),
+        "hs_dsmc_fill_cells": ([eng, P(C.c_double), P(C.c_double), P(C.c_double), C.c_uint32], C.c_int),
         "hs_dsmc_fill_noalloc": ([eng, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double),
                                   C.c_int32, C.c_uint32], C.c_int),
         "hs_dsmc_evolve": ([eng, C.c_int32], C.c_int),
@@ -81,6 +82,7 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_synchronize": ([eng], C.c_int),
         "hs_dsmc_read_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_int64)], C.c_int),
         "hs_dsmc_reset_samples": ([eng], C.c_int),
+        "hs_dsmc_set_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_double), C.c_int64, C.c_int64], C.c_int),
         "hs_dsmc_read_counters": ([eng, P(C.c_int64), C.c_int32], C.c_int),
         "hs_dsmc_reset_counters": ([eng], C.c_int),
         "hs_dsmc_kernel_times": ([eng, P(C.c_double), P(C.c_int64), C.c_int32], C.c_int),
@@ -94,6 +96,7 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_set_tiles": ([eng, C.c_int32, C.c_int32], C.c_int),
         "hs_dsmc_evolve_batch": ([P(C.c_void_p), C.c_int32, C.c_int32], C.c_int),
         "hs_dsmc_get_tiles": ([eng, P(C.c_int32), P(C.c_int32)], C.c_int),
+        "hs_dsmc_autotune": ([eng, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double)], C.c_int),
         "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_wall_faces": ([eng, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double)], C.c_int),
         "hs_dsmc_read_wall_samples": ([eng, P(C.c_double), C.c_int64, P(C.c_int64)], C.c_int),
@@ -253,6 +256,17 @@ class Engine:
         self._ck(self.L.hs_dsmc_fill(self.h, _d(nd), T, T if Trot is None else Trot, T if Tvib is None else Tvib,
                                      _d(uu), 1 if exact else 0, fill_id), "fill")
 
+    def fill_cells(self, n_cell, u_cell, t_cell, fill_id=0):
+        """Fill with per-cell number densities [species, cell], velocities
+        [3, cell] and temperatures [cell]."""
+        n = np.ascontiguousarray(n_cell, np.float64).ravel()
+        u = np.ascontiguousarray(u_cell, np.float64).ravel()
+        t = np.ascontiguousarray(t_cell, np.float64).ravel()
+        nc, ns = self.case.ncells, self.case.cfg.n_species
+        if n.size != ns * nc or u.size != 3 * nc or t.size != nc:
+            raise EngineError("fill_cells: arrays must be [species, cells], [3, cells] and [cells]")
+        self._ck(self.L.hs_dsmc_fill_cells(self.h, _d(n), _d(u), _d(t), fill_id), "fill_cells")
+
     # ---- stepping ----------------------------------------------------------
     def evolve(self, nsteps=1):
         self._ck(self.L.hs_dsmc_evolve(self.h, nsteps), "evolve")
@@ -282,6 +296,13 @@ class Engine:
         self._ck(self.L.hs_dsmc_get_tiles(self.h, C.byref(g), C.byref(c)), "get_tiles")
         return g.value, c.value
 
+    def autotune(self, steps_per_trial=8):
+        """Times the candidate tilings on the live state (the steps are steps
+        of the run); returns (gather, collide, ms per step) of the fastest."""
+        g, c, ms = C.c_int32(), C.c_int32(), C.c_double()
+        self._ck(self.L.hs_dsmc_autotune(self.h, steps_per_trial, C.byref(g), C.byref(c), C.byref(ms)), "autotune")
+        return g.value, c.value, ms.value
+
     def synchronize(self):
         self._ck(self.L.hs_dsmc_synchronize(self.h), "synchronize")
 
@@ -294,6 +315,13 @@ class Engine:
 
     def reset_samples(self):
         self._ck(self.L.hs_dsmc_reset_samples(self.h), "reset_samples")
+
+    def set_samples(self, cell_sums, nsteps, wall_sums=None):
+        """Restores cumulative cell (and wall) sums and their step count."""
+        c = np.ascontiguousarray(cell_sums, np.float64).ravel()
+        w = None if wall_sums is None else np.ascontiguousarray(wall_sums, np.float64).ravel()
+        self._ck(self.L.hs_dsmc_set_samples(self.h, _d(c), c.size, _d(w) if w is not None else None,
+                                            0 if w is None else w.size, nsteps), "set_samples")
 
     def wall_faces(self):
         """dict of side, cell, area, normal[n,3] of the wall faces"""

@@ -109,7 +109,7 @@ double diskRectArea(double x0, double x1, double y0, double y1, double R) {
     return std::min(std::max(area, 0.0), (x1 - x0) * (y1 - y0));
 }
 
-const char* kMagic = "HSDSMC01";
+const char* kMagic = "HSDSMC02";
 
 template <class T>
 void wr(std::ofstream& f, const T* p, size_t n) { f.write((const char*)p, n * sizeof(T)); }
@@ -957,9 +957,9 @@ void dsmcCloud::write() {
         if (fm.resetAtOutput && time_ < fm.resetAtOutputUntilTime) reset = true;
     }
     writeLagrangian(dir, tn);
-    // checkpoint before the reset so a restart continues the average
-    writeCheckpoint(dir + "/lagrangian/dsmc/hystrath.state");
     if (reset) check(hs_dsmc_reset_samples(eng_), "reset samples");
+    // after the reset: a restart continues with the sums this run continues with
+    writeCheckpoint(dir + "/lagrangian/dsmc/hystrath.state");
 }
 
 int dsmcCloud::vibModes() const {
@@ -1047,6 +1047,10 @@ void dsmcCloud::writeCheckpoint(const std::string& path) const {
     std::vector<double> smp((size_t)nslots * nc);
     int64_t nsampled = 0;
     check(hs_dsmc_read_samples(eng_, smp.data(), (int64_t)smp.size(), &nsampled), "samples");
+    int64_t nw = 0;
+    check(hs_dsmc_wall_face_count(eng_, &nw), "wall faces");
+    std::vector<double> wsum((size_t)ns * HS_WS_COUNT * nw);
+    if (nw) check(hs_dsmc_read_wall_samples(eng_, wsum.data(), (int64_t)wsum.size(), nullptr), "wall samples");
     std::ofstream f(path, std::ios::binary);
     if (!f) throw FatalError("cannot write " + path);
     f.write(kMagic, 8);
@@ -1062,6 +1066,8 @@ void dsmcCloud::writeCheckpoint(const std::string& path) const {
     wr(f, sig.data(), nc); wr(f, rem.data(), nc);
     if (nf) wr(f, acc.data(), (size_t)nf * ns);
     wr(f, smp.data(), smp.size());
+    wr(f, &nw, 1);
+    if (nw) wr(f, wsum.data(), wsum.size());
 }
 
 void dsmcCloud::readCheckpoint(const std::string& path) {
@@ -1096,6 +1102,19 @@ void dsmcCloud::readCheckpoint(const std::string& path) {
     check(hs_dsmc_set_cell_state(eng_, sig.data(), rem.data()), "cell state");
     if (nf) check(hs_dsmc_set_inflow_accumulators(eng_, acc.data()), "inflow accumulators");
     check(hs_dsmc_set_step(eng_, (uint32_t)hdr[6]), "step");
+    // the cumulative cell and wall sums the run had after its last write
+    const int64_t nslots = hs_dsmc_sample_slots(&cfg_);
+    std::vector<double> smp((size_t)nslots * nc);
+    rd(f, smp.data(), smp.size(), path);
+    int64_t nw = 0, nw_case = 0;
+    rd(f, &nw, 1, path);
+    check(hs_dsmc_wall_face_count(eng_, &nw_case), "wall faces");
+    if (nw != nw_case) throw FatalError(path + " does not match the case's wall faces");
+    std::vector<double> wsum((size_t)ns * HS_WS_COUNT * nw);
+    if (nw) rd(f, wsum.data(), wsum.size(), path);
+    check(hs_dsmc_set_samples(eng_, smp.data(), (int64_t)smp.size(), nw ? wsum.data() : nullptr,
+                              (int64_t)wsum.size(), hdr[7]),
+          "restore samples");
     startTime_ = time_;
     stepIndex_ = 0;
     std::cout << "Read " << n << " parcels at time " << time_ << " from " << path << std::endl;

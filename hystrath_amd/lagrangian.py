@@ -115,9 +115,18 @@ def read_label_lists(path, width):
         return out
     n, inner = _list_items(body)
     out = np.zeros((n, width), np.int64)
-    for k, m in enumerate(re.finditer(r"(\d+)\s*\(([^)]*)\)", inner)):
-        vals = [int(x) for x in m.group(2).split()]
+    # each entry is "k(l0 l1 ...)" or OpenFOAM's uniform short form "k{l}"
+    k = -1
+    for k, m in enumerate(re.finditer(r"(\d+)\s*(?:\(([^)]*)\)|\{\s*(-?\d+)\s*\})", inner)):
+        if k >= n:
+            raise ValueError(f"{path}: more than {n} entries")
+        cnt = int(m.group(1))
+        vals = [int(x) for x in m.group(2).split()] if m.group(2) is not None else [int(m.group(3))] * cnt
+        if len(vals) != cnt:
+            raise ValueError(f"{path}: entry {k} has {len(vals)} levels, not {cnt}")
         out[k, :min(width, len(vals))] = vals[:width]
+    if k + 1 != n:
+        raise ValueError(f"{path}: {k + 1} entries for a list of {n}")
     return out
 
 

@@ -66,14 +66,27 @@ def read_boundary(path):
 
 
 def _axis_lattice(x, what):
-    """(count, origin, spacing) of cell centres x on a uniform lattice."""
-    u = np.unique(np.round(x, 12))
-    if len(u) == 1:
+    """(count, origin, spacing) of cell centres x on a uniform lattice.
+    Centres are grouped into planes by a tolerance relative to the span of
+    the coordinates, and the gaps between planes must agree to a relative
+    1e-9 (OpenFOAM writes points with about 1e-10 relative precision)."""
+    u = np.sort(np.asarray(x, np.float64))
+    span = u[-1] - u[0]
+    if span <= 1e-12 * max(abs(u[0]), abs(u[-1]), 1e-300):
         return 1, None, None
-    d = np.diff(u)
-    if np.abs(d - d.mean()).max() > 1e-9 * max(abs(d.mean()), 1e-300) + 1e-14:
+    # planes: a gap wider than 1e-7 of the span starts a new plane (uniform
+    # lattices of up to ten million cells a side are resolved)
+    breaks = np.flatnonzero(np.diff(u) > 1e-7 * span)
+    groups = np.split(u, breaks + 1)
+    planes = np.array([g.mean() for g in groups])
+    for g in groups:
+        if g[-1] - g[0] > 1e-7 * span:
+            raise ValueError(f"{what}: cell centres do not lie on planes")
+    d = np.diff(planes)
+    mean = d.mean()
+    if np.abs(d - mean).max() > 1e-9 * abs(mean) + 1e-12 * span:
         raise ValueError(f"{what}: cells are not evenly spaced (simpleGrading other than 1)")
-    return len(u), u[0] - 0.5 * d.mean(), d.mean()
+    return len(planes), planes[0] - 0.5 * mean, mean
 
 
 def read_block(case_dir):

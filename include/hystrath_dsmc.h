@@ -237,6 +237,17 @@ int hs_dsmc_fill(hs_dsmc_engine* eng, const double* n, double t, double trot,
                  double tvib, const double u[3], int32_t exact_per_cell,
                  uint32_t fill_id);
 
+/* The fill with each cell's own state: n_cell[species*ncells + cell] number
+ * densities, u_cell[axis*ncells + cell] velocities and t_cell[cell]
+ * temperatures (translational, rotational and vibrational alike); counts
+ * are Poisson-rounded as in hs_dsmc_fill and sigmaTcRMax starts at each
+ * cell's sigmaT*sqrt(2kT/m) of its most abundant species.  The reference
+ * initialises uniform states (dsmcMeshFill.C:68-~230) or maps whole
+ * solutions (mapFields); this is the start of a population from predicted
+ * fields, the learned warm start's particle initialisation. */
+int hs_dsmc_fill_cells(hs_dsmc_engine* eng, const double* n_cell, const double* u_cell,
+                       const double* t_cell, uint32_t fill_id);
+
 /* The same fill with no allocation: no counterpart in the reference.  For
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops re-seeding populations, harnesses that forbid
@@ -289,6 +300,18 @@ int hs_dsmc_set_flags(hs_dsmc_engine* eng, int32_t flags);
  * the next step uses. */
 int hs_dsmc_set_tiles(hs_dsmc_engine* eng, int32_t gather_parcels, int32_t collide_parcels);
 int hs_dsmc_get_tiles(hs_dsmc_engine* eng, int32_t* gather_parcels, int32_t* collide_parcels);
+/* The autotuner of the batched mode; no counterpart in the reference.  Steps
+ * the engine (2 + steps_per_trial steps per candidate, ordinary steps of the
+ * run, since the tiling changes no result) under each candidate tiling --
+ * gather and collide parcels per workgroup of 256, 512, 1024, 1536 and 2048,
+ * up to the full-device sizes, skipping tilings with fewer workgroups than
+ * compute units -- and times them with HIP events.  The fastest pair is
+ * returned and kept for every engine of the process with the same kernel
+ * variant (internal energies or not) and population class (log2 of the
+ * parcel count) that has no tiles set by hs_dsmc_set_tiles; this engine's
+ * own setting is left as it was. */
+int hs_dsmc_autotune(hs_dsmc_engine* eng, int32_t steps_per_trial, int32_t* gather_out,
+                     int32_t* collide_out, double* ms_out);
 
 /* ---- sampling ----------------------------------------------------------- */
 /* dsmcVolFields cumulative sums, plane-major [slot][cell]; nsteps_out gets
@@ -297,6 +320,14 @@ int hs_dsmc_read_samples(hs_dsmc_engine* eng, double* out, int64_t n_doubles, in
 /* dsmcVolFields::resetField (dsmcVolFields.C:2270-2290); clears the cell
  * and wall-face sums and the face fluxes */
 int hs_dsmc_reset_samples(hs_dsmc_engine* eng);
+/* Restores cumulative sums and their step count, as a restarted run needs
+ * them: dsmcVolFields reads its accumulated data back when it continues an
+ * average (averagingAcrossManyRuns, dsmcVolFields.C:1047-1063).  cell_sums is
+ * [slot][cell] as hs_dsmc_read_samples gives it (exactly slots*cells values);
+ * wall_sums as hs_dsmc_read_wall_samples gives it, or NULL to clear the wall
+ * sums.  The face fluxes are left as they are. */
+int hs_dsmc_set_samples(hs_dsmc_engine* eng, const double* cell_sums, int64_t n_cell,
+                        const double* wall_sums, int64_t n_wall, int64_t nsteps);
 /* Wall faces (the boundary faces of boundaryMeasurements, boundaryMeasurements.H:74-90):
  * first every face of each diffuse or specular side, sides in hs_side order,
  * faces ordered by the two tangential cell indices (x before y before z, the

@@ -1515,8 +1515,13 @@ int or_inject(const hs_config* cfg, or_state* s)
 /* ------------------------------------------------------------------------ */
 /* initial fill (dsmcMeshFill.C:68-~230)                                    */
 /* ------------------------------------------------------------------------ */
-int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, double Trot,
-            double Tvib, const double u[3], int32_t exact, uint32_t fill_id)
+/* the fill of dsmcMeshFill.C:68-~230 with, where the per-cell arrays are
+ * given, each cell's own number densities nd_c[species][cell], velocity
+ * u_c[axis][cell] and temperature t_c[cell] (internal temperatures equal to
+ * it); sigmaTcRMax then follows each cell's temperature */
+static int fill_general(const hs_config* cfg, or_state* s, const double* nd, const double* nd_c, double T,
+                        const double* t_c, double Trot, double Tvib, const double u[3], const double* u_c,
+                        int32_t exact, uint32_t fill_id)
 {
     or_derived d;
     derive(cfg, &d);
@@ -1527,17 +1532,20 @@ int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, doubl
         const double V = cfg->cell_volume[c];
         if (!(V > 0.0)) continue;
         const int idx[3] = { (int)(c % cfg->nx), (int)((c / cfg->nx) % cfg->ny), (int)(c / ((int64_t)cfg->nx * cfg->ny)) };
+        const double Tc = t_c ? t_c[c] : T;
+        const double uc[3] = { u_c ? u_c[c] : u[0], u_c ? u_c[nc + c] : u[1], u_c ? u_c[2 * nc + c] : u[2] };
         for (int sp = 0; sp < ns; ++sp) {
-            if (!(nd[sp] > 0.0)) continue;
+            const double nds = nd_c ? nd_c[(int64_t)sp * nc + c] : nd[sp];
+            if (!(nds > 0.0)) continue;
             or_rng rc, rg;
             rng_init(&rc, cfg->seed, OR_RNG_FILL_COUNT, (uint32_t)c, fill_id, (uint32_t)sp);
             rng_init(&rg, cfg->seed, OR_RNG_FILL_GEN, (uint32_t)c, fill_id, (uint32_t)sp);
-            const double req = nd[sp] * V / cfg->fn;
+            const double req = nds * V / cfg->fn;
             int ni;
             if (exact) ni = (int)floor(req + 0.5);
             else { ni = (int)req; if ((req - (double)ni) > rng_u(&rc)) ni += 1; }
             const hs_species* spp = &cfg->species[sp];
-            const double sc = sqrt(HS_KB * T / spp->mass);
+            const double sc = sqrt(HS_KB * Tc / spp->mass);
             for (int k = 0; k < ni; ++k) {
                 if (n >= s->capacity) { s->counters[6] += 1; break; }
                 double pos[3] = { 0.0, 0.0, 0.0 };
@@ -1557,30 +1565,47 @@ int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, doubl
                  * (cell -1) so the population layout does not depend on it */
                 const double g0 = rng_gauss(&rg), g1 = rng_gauss(&rg), g2 = rng_gauss(&rg);
                 s->p.px[n] = pos[0]; s->p.py[n] = pos[1]; s->p.pz[n] = pos[2];
-                s->p.ux[n] = (float)(sc * g0 + u[0]);
-                s->p.uy[n] = (float)(sc * g1 + u[1]);
-                s->p.uz[n] = (float)(sc * g2 + u[2]);
+                s->p.ux[n] = (float)(sc * g0 + uc[0]);
+                s->p.uy[n] = (float)(sc * g1 + uc[1]);
+                s->p.uz[n] = (float)(sc * g2 + uc[2]);
                 s->p.cell[n] = ok ? (int32_t)c : -1;
                 if (s->p.vib)
                     for (int m = 0; m < d.max_modes; ++m) s->p.vib[m * s->capacity + n] = 0;
-                new_particle_internal(cfg, s, n, sp, Trot, Tvib, &rg);
+                new_particle_internal(cfg, s, n, sp, t_c ? Tc : Trot, t_c ? Tc : Tvib, &rg);
                 ++n;
             }
         }
     }
     s->n = n;
     /* sigmaTcRMax = sigmaT * most probable speed of the most abundant species */
-    int most = 0;
-    for (int sp = 1; sp < ns; ++sp) if (nd[sp] > nd[most]) most = sp;
-    const hs_species* sm = &cfg->species[most];
-    const double smax = (OR_PI * sm->diameter * sm->diameter) * sqrt(2.0 * HS_KB * T / sm->mass);
     for (int64_t c = 0; c < nc; ++c) {
-        s->sigma_tcr_max[c] = smax;
+        int most = 0;
+        for (int sp = 1; sp < ns; ++sp) {
+            const double a = nd_c ? nd_c[(int64_t)sp * nc + c] : nd[sp];
+            const double b = nd_c ? nd_c[(int64_t)most * nc + c] : nd[most];
+            if (a > b) most = sp;
+        }
+        const hs_species* sm = &cfg->species[most];
+        const double Tc = t_c ? t_c[c] : T;
+        s->sigma_tcr_max[c] = (OR_PI * sm->diameter * sm->diameter) * sqrt(2.0 * HS_KB * Tc / sm->mass);
         or_rng r;
         rng_init(&r, cfg->seed, OR_RNG_REMAINDER, (uint32_t)c, fill_id, 0);
         s->remainder[c] = rng_u(&r);
     }
     return 0;
+}
+
+int or_fill(const hs_config* cfg, or_state* s, const double* nd, double T, double Trot,
+            double Tvib, const double u[3], int32_t exact, uint32_t fill_id)
+{
+    return fill_general(cfg, s, nd, NULL, T, NULL, Trot, Tvib, u, NULL, exact, fill_id);
+}
+
+int or_fill_cells(const hs_config* cfg, or_state* s, const double* nd_cell, const double* u_cell,
+                  const double* t_cell, uint32_t fill_id)
+{
+    static const double zero3[3] = { 0.0, 0.0, 0.0 };
+    return fill_general(cfg, s, NULL, nd_cell, 0.0, t_cell, 0.0, 0.0, zero3, u_cell, 0, fill_id);
 }
 
 /* ------------------------------------------------------------------------ */

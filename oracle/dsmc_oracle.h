@@ -75,6 +75,10 @@ int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const 
 /* stages on an or_state */
 int or_fill(const hs_config* cfg, or_state* s, const double* n, double t, double trot,
             double tvib, const double u[3], int32_t exact_per_cell, uint32_t fill_id);
+/* the fill with per-cell number densities [species][cell], velocities
+ * [axis][cell] and temperatures [cell] (see hs_dsmc_fill_cells) */
+int or_fill_cells(const hs_config* cfg, or_state* s, const double* nd_cell, const double* u_cell,
+                  const double* t_cell, uint32_t fill_id);
 int or_inject(const hs_config* cfg, or_state* s);
 int or_move(const hs_config* cfg, or_state* s, int64_t n_old);
 /* wall faces of boundaryMeasurements in the engine's order; any output may be

@@ -71,6 +71,7 @@ def lib(path=None):
                                         P(C.c_double), P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_double),
                                         C.c_int32, P(C.c_int32)]
         L.or_fill.argtypes = [cfgp, stp, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32, C.c_uint32]
+        L.or_fill_cells.argtypes = [cfgp, stp, P(C.c_double), P(C.c_double), P(C.c_double), C.c_uint32]
         L.or_inject.argtypes = [cfgp, stp]
         L.or_move.argtypes = [cfgp, stp, C.c_int64]
         L.or_sort.argtypes = [cfgp, stp]
@@ -227,6 +228,12 @@ class OracleState:
         self.L.or_fill(C.byref(self.case.cfg), C.byref(self.s), dptr(nd), T, Trot if Trot is not None else T,
                       Tvib if Tvib is not None else T, dptr(uu), 1 if exact else 0, fill_id)
 
+    def fill_cells(self, n_cell, u_cell, t_cell, fill_id=0):
+        n = np.ascontiguousarray(n_cell, np.float64).ravel()
+        u = np.ascontiguousarray(u_cell, np.float64).ravel()
+        t = np.ascontiguousarray(t_cell, np.float64).ravel()
+        self.L.or_fill_cells(C.byref(self.case.cfg), C.byref(self.s), dptr(n), dptr(u), dptr(t), fill_id)
+
     def load(self, d, sigma=None, rem=None):
         self.s.n = self.P.load(d)
         if sigma is not None:
@@ -261,16 +268,16 @@ class OracleState:
 
 
 def wall_faces(case):
-    n = self.L.or_wall_faces(C.byref(case.cfg), None, None, None, None)
+    n = lib().or_wall_faces(C.byref(case.cfg), None, None, None, None)
     side = np.zeros(n, np.int32)
     cell = np.zeros(n, np.int32)
     area = np.zeros(n)
     normal = np.zeros((n, 3))
-    self.L.or_wall_faces(C.byref(case.cfg), iptr(side), iptr(cell), dptr(area), dptr(normal))
+    lib().or_wall_faces(C.byref(case.cfg), iptr(side), iptr(cell), dptr(area), dptr(normal))
     return dict(side=side, cell=cell, area=area, normal=normal)
 
 
 def arc_in_rect(cx, cy, r, x0, x1, y0, y1):
     nrm = np.zeros(2)
-    length = self.L.or_arc_in_rect(cx, cy, r, x0, x1, y0, y1, dptr(nrm))
+    length = lib().or_arc_in_rect(cx, cy, r, x0, x1, y0, y1, dptr(nrm))
     return length, nrm

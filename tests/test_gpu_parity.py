@@ -347,3 +347,50 @@ def test_batched_engines_match_the_oracle():
     evolve_batch([], 3)
     for e in engs:
         e.close()
+
+
+def test_autotuner_picks_a_measured_tiling_that_changes_no_result():
+    """hs_dsmc_autotune steps the engine under each candidate tiling and
+    keeps the fastest: the steps it takes are ordinary steps (the engine still
+    equals the oracle advanced by as many steps), the choice is a candidate
+    pair, and a second engine of the same population class starts from it."""
+    case = H.small_cylinder(n=60, ppc=10)
+    eng, st = _pair(case, H.fill_cylinder)
+    s0 = eng.step
+    g, c, ms = eng.autotune(steps_per_trial=3)
+    assert g in (256, 512, 1024, 1536) and c in (256, 512, 1024, 1536, 2048) and ms > 0
+    taken = eng.step - s0
+    assert taken > 0 and taken % 5 == 0
+    st.evolve(taken)
+    _check_state(eng, st, "autotune")
+    assert eng.tiles() == (g, c)
+    eng2, st2 = _pair(case, H.fill_cylinder)
+    assert eng2.tiles() == (g, c)
+    eng2.evolve(3)
+    st2.evolve(3)
+    _check_state(eng2, st2, "autotuned tiles")
+    eng.close()
+    eng2.close()
+
+
+def test_fill_cells_parity():
+    """hs_dsmc_fill_cells (a population from per-cell fields: the warm
+    start's initialisation) against the oracle's, bit-exact: counts,
+    positions, velocities, internal energies, sigmaTcRMax and remainders."""
+    for case in (H.small_cylinder(n=40, ppc=10), cases.couette_n2o2()):
+        nc, ns = case.ncells, case.cfg.n_species
+        rng = np.random.default_rng(8)
+        base = np.array(getattr(case, "init_n", [getattr(case, "n_inf", 1e20)]))
+        n_cell = (base[:, None] * rng.uniform(0.3, 3.0, (ns, nc))).ravel()
+        u_cell = rng.normal(0, 800, (3, nc))
+        t_cell = rng.uniform(100.0, 5000.0, nc)
+        eng = Engine(case)
+        st = ob.OracleState(case)
+        eng.fill_cells(n_cell, u_cell, t_cell, fill_id=5)
+        st.fill_cells(n_cell, u_cell, t_cell, fill_id=5)
+        assert st.n > 0
+        _check_state(eng, st, f"fill_cells/{case.name}")
+        eng.evolve(2)
+        st.evolve(2)
+        _check_state(eng, st, f"fill_cells/{case.name}/2 steps")
+        eng.close()

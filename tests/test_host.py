@@ -145,12 +145,44 @@ def test_couette_pipeline_and_restart(tmp_path):
     assert os.path.exists(os.path.join(case, "0.0006", "Ttra_mixture"))
 
 
+@pytest.mark.gpu
+def test_restart_continues_the_average(tmp_path):
+    """A run of 60 steps written in one go equals 40 steps, a restart from
+    the checkpoint and 20 more: the particles and the cumulative cell and
+    wall sums (resetAtOutputUntilTime 0: one average from the start) carry
+    over, so every field file of the last write is identical."""
+    one = casegen.couette_case(str(tmp_path / "one"), end_time=6e-4, write_interval=2e-4, n_terminal=10)
+    two = casegen.couette_case(str(tmp_path / "two"), end_time=4e-4, write_interval=2e-4, n_terminal=10)
+    for c in (one, two):
+        r = run("dsmcInitialisePlus", c)
+        assert r.returncode == 0, r.stderr
+        r = run("dsmcFoamPlus", c)
+        assert r.returncode == 0, r.stderr
+    cd = os.path.join(two, "system/controlDict")
+    ctxt = open(cd).read()
+    assert "endTime 0.0004;" in ctxt
+    with open(cd, "w") as f:
+        f.write(ctxt.replace("endTime 0.0004;", "endTime 0.0006;"))
+    r = run("dsmcFoamPlus", two)
+    assert r.returncode == 0, r.stderr
+    names = sorted(f for f in os.listdir(os.path.join(one, "0.0006")) if os.path.isfile(os.path.join(one, "0.0006", f)))
+    assert {"Ttra_mixture", "rhoN_N2", "U_mixture", "wallHeatFlux_mixture", "Tvib_O2"} <= set(names)
+    for f in names:
+        a = open(os.path.join(one, "0.0006", f)).read()
+        b = open(os.path.join(two, "0.0006", f)).read()
+        assert a == b, f
+    sa = _read_state(os.path.join(one, "0.0006/lagrangian/dsmc/hystrath.state"))
+    sb = _read_state(os.path.join(two, "0.0006/lagrangian/dsmc/hystrath.state"))
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
+
+
 def _read_state(path):
     """Parcels of an engine checkpoint (hystrath.state): magic, eight int64
     (n, cells, species, vib modes, rotational, inflow faces, step, samples),
     the time, then the parcel arrays in structure-of-arrays order."""
     b = open(path, "rb").read()
-    assert b[:8] == b"HSDSMC01"
+    assert b[:8] == b"HSDSMC02"
     hdr = np.frombuffer(b, np.int64, 8, 8)
     n, ns, modes, rot = int(hdr[0]), int(hdr[2]), int(hdr[3]), int(hdr[4])
     off = 8 + 64 + 8

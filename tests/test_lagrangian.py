@@ -74,6 +74,15 @@ def test_uniform_and_nested_lists(tmp_path):
     np.testing.assert_array_equal(lg.read_vectors(str(tmp_path / "v")), [[1, -2, 3.5]] * 3)
     np.testing.assert_array_equal(lg.read_label_lists(str(tmp_path / "l"), 2), [[4, 5], [7, 0], [0, 0]])
     np.testing.assert_array_equal(lg.read_label_lists(str(tmp_path / "u"), 1), [[3], [3]])
+    # a three-mode species: a parcel whose levels are equal is written "3{l}"
+    (tmp_path / "m").write_text(hdr + "3\n(\n3(1 0 2)\n3{0}\n3(0 4 0)\n)\n")
+    np.testing.assert_array_equal(lg.read_label_lists(str(tmp_path / "m"), 3), [[1, 0, 2], [0, 0, 0], [0, 4, 0]])
+    (tmp_path / "short").write_text(hdr + "3\n(\n3(1 0 2)\n3(0 4 0)\n)\n")
+    with pytest.raises(ValueError, match="2 entries for a list of 3"):
+        lg.read_label_lists(str(tmp_path / "short"), 3)
+    (tmp_path / "bad").write_text(hdr + "2\n(\n3(1 0)\n1(2)\n)\n")
+    with pytest.raises(ValueError, match="has 2 levels, not 3"):
+        lg.read_label_lists(str(tmp_path / "bad"), 3)
 
 
 def _snapshot_parcels():

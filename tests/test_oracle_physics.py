@@ -414,3 +414,19 @@ def test_cut_cell_volumes():
         X, Y = np.meshgrid(x0 + g * d[0], y0 + g * d[1])
         frac = 1 - (X * X + Y * Y < 0.01).mean()
         assert abs(frac - v[k] / full) < 5e-4
+
+
+def test_fill_cells_with_uniform_fields_is_the_uniform_fill():
+    """The per-cell fill given the same state in every cell is the uniform
+    fill (dsmcMeshFill.C:68-~230), parcel for parcel."""
+    for case, nd, T, u in ((H.small_cylinder(n=30, ppc=10), [4.2e20], 200.0, (2634.0, 0.0, 0.0)),
+                           (cases.couette_n2o2(), [3.2e19, 8.6e18], 2000.0, (0.0, 0.0, 0.0))):
+        nc, ns = case.ncells, case.cfg.n_species
+        a = ob.OracleState(case)
+        b = ob.OracleState(case)
+        a.fill(nd, T, u=u, fill_id=3)
+        b.fill_cells(np.repeat(np.array(nd)[:, None], nc, axis=1), np.repeat(np.array(u)[:, None], nc, axis=1),
+                     np.full(nc, T), fill_id=3)
+        H.assert_same_particles(b.particles(), a.particles(), case.name)
+        np.testing.assert_array_equal(b.sigma, a.sigma)
+        np.testing.assert_array_equal(b.rem, a.rem)

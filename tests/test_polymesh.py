@@ -124,3 +124,17 @@ def test_meshes_that_are_not_the_engine_block_are_rejected(tmp_path):
     write_block_mesh(str(tmp_path / "m"), (2, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX[:4] + [("front", "empty", ["zmin"])])
     with pytest.raises(ValueError, match="only hex cells"):
         pm.read_block(str(tmp_path / "m"))
+
+
+@pytest.mark.parametrize("n,lo,hi", [
+    ((300, 2, 1), (0.0, 0.0, 0.0), (0.1, 0.01, 0.001)),        # spacing 1/3000: no short decimal form
+    ((50, 40, 1), (1e-6, -2e-6, 0.0), (3.1e-6, 2e-6, 1e-7)),   # micrometre scale
+    ((7, 3, 2), (1000.0, 1000.0, 0.0), (1000.7, 1000.3, 0.2)),  # small cells far from the origin
+])
+def test_uniform_blocks_of_awkward_spacing_are_read(tmp_path, n, lo, hi):
+    write_block_mesh(str(tmp_path), n, lo, hi, SIX)
+    b = pm.read_block(str(tmp_path))
+    assert b["n"] == n
+    spacing = [(hi[a] - lo[a]) / n[a] for a in range(3)]
+    np.testing.assert_allclose(b["spacing"], spacing, rtol=1e-9)
+    np.testing.assert_allclose(b["origin"], lo, rtol=1e-9, atol=1e-12 * max(map(abs, hi)))
