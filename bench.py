@@ -25,6 +25,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -239,10 +241,60 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
                           "components scaled by the judge's speed; argon has no internal temperature"}
 
 
+def warm_start_eval(args, log_fn):
+    """The learned warm start against the plain path on held-out 256 x 256
+    cylinder cases (Mach 5-25, Kn 0.01-1, T_wall 300-1000 K), with the
+    trained network committed in hystrath_amd/data.  Per case: the judge is
+    the 10,000-step long average of one run; the plain path (fill, 3,000
+    development steps, sampling) and the warm path (the network's fields
+    fill the domain, 300 relaxation steps, one 200-step window denoised, the
+    plain average as fallback) are independent runs timed to fields within
+    2% of the judge."""
+    from hystrath_amd import warmstart as W
+    from hystrath_amd.engine import Engine
+    path = os.path.join(ROOT, "hystrath_amd", "data", "ws_cylinder256.npz")
+    if not os.path.exists(path):
+        return None
+    net = W.load_net(path)
+    cs = W.make_cases(args.ws_cases, seed=7777)
+    gen = W.generate(cs, develop=3000, average=10000, log=log_fn)
+    out = []
+    for c, (_, target) in zip(cs, gen):
+        judge = W.judge_fields(c, target)
+        # ms per step of this case, graph-replayed, on a developed state
+        with Engine(c) as e:
+            W.fill_freestream(e, c)
+            e.evolve(1000)
+            e.synchronize()
+            t = time.perf_counter()
+            e.evolve(500)
+            e.synchronize()
+            ms = 1e3 * (time.perf_counter() - t) / 500
+        plain = W.plain_time_to_tol(c, judge, develop=3000)
+        warm = W.warm_time_to_tol(net, c, judge, relax=300)
+        tp = plain["steps"] * ms * 1e-3
+        tw = warm["steps"] * ms * 1e-3 + warm["inference_s"]
+        out.append({"params": {k: round(v, 4) for k, v in c.ws_params.items()}, "ms_per_step": round(ms, 4),
+                    "plain": {"steps": plain["steps"], "seconds": round(tp, 4), "reached": plain["reached"],
+                              "errors": plain["errors"]},
+                    "warm": {"steps": warm["steps"], "seconds": round(tw, 4), "reached": warm["reached"],
+                             "used": warm["used"], "inference_s": round(warm["inference_s"], 4),
+                             "errors": warm["errors"], "errors_denoised": warm["errors_denoised"],
+                             "errors_window": warm["errors_window"], "errors_surrogate": warm["errors_surrogate"]},
+                    "speedup": round(tp / max(tw, 1e-9), 3)})
+        log_fn(f"[bench] warm start {c.ws_params}: plain {plain['steps']} steps, warm {warm['steps']} "
+               f"({warm['used']}), x{tp / max(tw, 1e-9):.2f}")
+    sp = [r["speedup"] for r in out]
+    return {"cases": out, "speedup_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3) if sp else None,
+            "network": "6-layer 3x3 conv, 8-32-64-64-64-32-5 channels, bfloat16 MFMA kernels, FP32 accumulation",
+            "tolerance": 0.02}
+
+
 def run_batch(args):
-    """64 independent cylinder cases (Mach 5-25, Kn 0.01-1, 50^2 to 300^2
-    cells, 10k to 2M parcels each) stepped together by hs_dsmc_evolve_batch;
-    value = particle-steps of all cases per second."""
+    """64 independent cylinder cases (argon and N2 in turn, Mach 5-25, Kn
+    0.01-1, 50^2 to 300^2 cells, 10k to 2M parcels each) stepped together by
+    hs_dsmc_evolve_batch, after the autotuner has measured the tilings of
+    each population class; value = particle-steps of all cases per second."""
     from hystrath_amd import cases
     from hystrath_amd.engine import Engine, evolve_batch
 
@@ -254,7 +306,7 @@ def run_batch(args):
         engs.append(e)
     sizes = [e.count() for e in engs]
     log(f"[bench] batch of {len(engs)} cases, {sum(sizes)} particles ({min(sizes)} to {max(sizes)} a case); "
-        f"warm-up {args.warmup} steps")
+        f"{args.develop} development steps, warm-up {args.warmup} steps")
 
     def steps(n, what):
         done, t0 = 0, time.time()
@@ -270,6 +322,16 @@ def run_batch(args):
         for e in engs:
             e.synchronize()
 
+    steps(args.develop, "development")
+    # the autotuner: one engine per (kernel variant, population class) times the
+    # candidate tilings; the other engines of its class take its choice
+    tuned = {}
+    for e, c in zip(engs, cs):
+        key = (c.batch_params["species"], int(np.log2(max(e.count(), 1))))
+        if key not in tuned:
+            tuned[key] = e.autotune(steps_per_trial=4)
+            log(f"[bench] autotune {key}: gather {tuned[key][0]}, collide {tuned[key][1]} "
+                f"({tuned[key][2]:.4f} ms/step)")
     steps(args.warmup, "warm-up")
     for e in engs:
         e.reset_counters()
@@ -282,21 +344,48 @@ def run_batch(args):
     bad = {k: sum(c[k] for c in cnts) for k in ("track_overflow", "tile_overflow", "capacity_overflow")}
     sizes = [e.count() for e in engs]
     value = psteps / elapsed
+    ms_step = 1e3 * elapsed / max(args.steps, 1)
     log(f"[bench] {args.steps} steps of the batch in {elapsed:.3f}s: {value:.4e} particle-steps/s {bad}")
+    # whole-step algorithmic bytes of every case (argon record; N2 adds its
+    # internal-energy bytes, not counted here, so the figure is a lower bound)
+    per_particle = sum(b["per_particle"] for b in STAGE_BYTES.values())
+    step_bytes = sum(per_particle * n + sum(b["per_cell"] for b in STAGE_BYTES.values()) * c.ncells
+                     for n, c in zip(sizes, cs))
+    gbs = step_bytes / (ms_step * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(gbs / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "whole step, all cases",
+                "step_algorithmic_bytes": int(step_bytes), "note": "argon record bytes for every case"}
+    cpu = None
+    if args.cpu_seconds > 0:
+        # three cases of the batch (the smallest, a median one, an N2 one) on the host
+        order = np.argsort(sizes)
+        picks = [int(order[0]), int(order[len(order) // 2])]
+        picks.append(next(k for k in order[::-1] if cs[k].batch_params["species"] == "N2" and k not in picks))
+        rs = [cpu_baseline(cs[k], engs[k], args.cpu_seconds / len(picks)) for k in picks]
+        tot_ps = sum(r["value"] * r["seconds"] for r in rs)
+        tot_s = sum(r["seconds"] for r in rs)
+        cpu = dict(rs[0])
+        cpu.update(value=tot_ps / tot_s, seconds=round(tot_s, 3),
+                   sample=f"cases {picks} of the batch ({[sizes[k] for k in picks]} particles), "
+                          f"each for {args.cpu_seconds / len(picks):.1f} s per thread count: " + rs[0]["sample"])
+        cpu["single_core"] = {"value": sum(r["single_core"]["value"] * r["single_core"]["seconds"] for r in rs)
+                              / sum(r["single_core"]["seconds"] for r in rs)}
     for e in engs:
         e.close()
+    valid = not any(bad.values())
     out = {"metric": "particle-steps/s on 1 GPU", "value": round(value, 1), "unit": "particle-steps/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-           "ms_per_step": round(1e3 * elapsed / max(args.steps, 1), 5), "higher_is_better": True,
-           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
-           "data": "synthetic (uniform freestream fills, flows developed by the warm-up steps)",
+           "ms_per_step": round(ms_step, 5), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64/f32",
+           "data": f"synthetic: uniform freestream fills, {args.develop} development steps in set-up",
            "config": {"workload": "cylinder_batch_64", "cases": len(engs), "particles": sum(sizes),
                       "particles_min": min(sizes), "particles_max": max(sizes),
                       "cases_params": [c.batch_params for c in cs],
+                      "autotuned": {f"{k[0]}/2^{k[1]}": v[:2] for k, v in tuned.items()},
                       "parallelism": "single GPU, cases interleaved on their own streams"},
-           "roofline": None, "cpu_baseline": None, "overflow": bad}
+           "roofline": roofline, "cpu_baseline": cpu, "overflow": bad, "valid": valid}
     print(json.dumps(out), flush=True)
-    return 0
+    return 0 if valid else 3
 
 
 def main():
@@ -316,6 +405,7 @@ def main():
     ap.add_argument("--t2-judge", type=int, default=10000, help="steps of the judge's long average")
     ap.add_argument("--t2-check", type=int, default=250, help="sampled steps between error checks")
     ap.add_argument("--t2-max", type=int, default=10000, help="give up after this many sampled steps")
+    ap.add_argument("--ws-cases", type=int, default=2, help="held-out cases of the warm-start comparison (0 disables)")
     args = ap.parse_args()
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -326,6 +416,13 @@ def main():
 
     if args.workload == "batch64":
         return run_batch(args)
+
+    ws_weights = os.path.join(ROOT, "hystrath_amd", "data", "ws_cylinder256.npz")
+    if args.ws_cases and os.path.exists(ws_weights):
+        # the warm start's network keeps its memory in torch: torch's HIP
+        # runtime starts before the engines'
+        import torch
+        torch.zeros(1, device="cuda")
 
     from hystrath_amd import abi
     from hystrath_amd.engine import Engine, device_copy_ms
@@ -407,6 +504,9 @@ def main():
     t2 = None
     if args.t2pct and args.workload == "cylinder":
         t2 = time_to_2pct(args, cylinder_case, fill, ms_per_step)
+    ws = None
+    if args.ws_cases and args.workload == "cylinder" and os.path.exists(ws_weights):
+        ws = warm_start_eval(args, log)
 
     desc.update({"particles": n_now, "develop_steps": develop, "warmup_steps": args.warmup,
                  "precision": "FP64 arithmetic, positions and sums; FP32 velocity storage",
@@ -418,7 +518,7 @@ def main():
            "data": (f"synthetic: uniform freestream fill, then {develop} development steps in set-up "
                     f"(bow shock and wake formed) and {args.warmup} warm-up steps, all untimed"
                     if develop else "synthetic: bimodal Maxwellian fill"),
-           "config": desc, "roofline": roofline, "cpu_baseline": cpu, "time_to_2pct": t2,
+           "config": desc, "roofline": roofline, "cpu_baseline": cpu, "time_to_2pct": t2, "warm_start": ws,
            "counters": cnt, "valid": valid}
     if not valid:
         out["invalid_reason"] = f"device-side bounds tripped in the timed steps: {bad}"

@@ -46,11 +46,16 @@ def build_host(verbose=False):
         cmd = common + srcs + [os.path.join(HOST, app + ".C"), "-L", HERE, "-lhystrath_dsmc",
                                "-Wl,-rpath,$ORIGIN/..", "-o", out]
         outs.append((cmd, out))
-    outs.append((common + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "dictCheck.C"), "-o",
-                           os.path.join(BIN, "dictCheck")], os.path.join(BIN, "dictCheck")))
-    outs.append((common + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "lagrangianIO.C"),
-                           os.path.join(HOST, "cloudCheck.C"), "-o", os.path.join(BIN, "cloudCheck")],
-                 os.path.join(BIN, "cloudCheck")))
+    # the tests' own tools for the host's readers (no device needed)
+    tools = os.path.join(HERE, "..", "tests", "tools")
+    tbin = os.path.join(tools, "_bin")
+    os.makedirs(tbin, exist_ok=True)
+    inc = ["-I", HOST]
+    outs.append((common + inc + [os.path.join(HOST, "dictionary.C"), os.path.join(tools, "dictCheck.C"), "-o",
+                                 os.path.join(tbin, "dictCheck")], os.path.join(tbin, "dictCheck")))
+    outs.append((common + inc + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "lagrangianIO.C"),
+                                 os.path.join(tools, "cloudCheck.C"), "-o", os.path.join(tbin, "cloudCheck")],
+                 os.path.join(tbin, "cloudCheck")))
     for cmd, out in outs:
         if verbose:
             print(" ".join(cmd), flush=True)

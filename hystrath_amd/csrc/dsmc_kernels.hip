@@ -1249,6 +1249,83 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
     }
 }
 
+// The same sums for a cell of more than 64 parcels, by one wave: lane l adds
+// the terms of parcels l, l+64, l+128, ... in order, and the 64 partial sums
+// are combined pairwise across lane distances 32, 16, 8, 4, 2, 1 (every lane
+// ends with the same value: IEEE addition is commutative).  The oracle forms
+// these sums in the same order.  nColls and the separation are not added.
+__device__ __forceinline__ double wave_tree_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = v + __shfl_xor(v, o, WAVE);
+    return v;
+}
+
+template <bool GEN>
+__device__ void sample_cell_wave(const DevParams& P, int c, int ss, int nC, const float* vx, const float* vy,
+                                 const float* vz, const float* er, const uint16_t* vb, int vstride, const uint8_t* ty,
+                                 double* samples, int nc) {
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int ns = GEN ? P.ns : 1;
+    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    for (int s = 0; s < ns; ++s) {
+        const double m = P.mass[s];
+        double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
+        double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+        double h[HS_HF_COUNT];
+        for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
+        const int nvib = GEN ? P.nvib[s] : 0;
+        for (int k = lane; k < nC; k += WAVE) {
+            const int i = ss + k;
+            if (GEN && (int)ty[i] != s) continue;
+            const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+            const double lke = m * (ux * ux + uy * uy + uz * uz);
+            aN += 1.0;
+            aX += m * ux;
+            aY += m * uy;
+            aZ += m * uz;
+            aC += lke;
+            const double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
+            if (GEN && P.has_erot) aR += erv;
+            double evs = 0.0;
+            for (int mo = 0; mo < nvib; ++mo) {
+                const double e = (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+                ev[mo] += e;
+                evs += e;
+            }
+            if (hf) {
+                const double eint = erv + evs;
+                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            }
+        }
+        aN = wave_tree_sum(aN);
+        aX = wave_tree_sum(aX);
+        aY = wave_tree_sum(aY);
+        aZ = wave_tree_sum(aZ);
+        aC = wave_tree_sum(aC);
+        if (GEN && P.has_erot) aR = wave_tree_sum(aR);
+        for (int mo = 0; mo < nvib; ++mo) ev[mo] = wave_tree_sum(ev[mo]);
+        if (hf)
+            for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = wave_tree_sum(h[q]);
+        if (lane == 0) {
+            const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+            unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
+            if (GEN && P.has_erot) unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
+            for (int mo = 0; mo < nvib; ++mo) unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
+            if (hf) {
+                const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
+                for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
+            }
+        }
+    }
+}
+
 __device__ __forceinline__ int subcell_of(const DevParams& P, int c, double x, double y, double z) {
     int ijk[3];
     cell_ijk(P, c, ijk);
@@ -1583,26 +1660,31 @@ __device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int 
                                               double* __restrict__ samples, const double* __restrict__ vol,
                                               uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
                                               uint8_t* sub, CellCtx& cx) {
+    const int64_t cap = P.capacity;
+    float* gvx = dst.ux + S0;
+    float* gvy = dst.uy + S0;
+    float* gvz = dst.uz + S0;
+    float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
+    uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
+    const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
+    double ncoll = 0.0, sep = 0.0;
     if (threadIdx.x == 0) {
-        const int64_t cap = P.capacity;
-        double ncoll = 0.0, sep = 0.0;
         if (nC > 65535) {
             cx.ovf += 1;
-        } else {
-            float* gvx = dst.ux + S0;
-            float* gvy = dst.uy + S0;
-            float* gvz = dst.uz + S0;
-            float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
-            uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
-            const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
-            if (do_collide && nC > 1) {
-                double smax;
-                const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-                ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb,
-                              (int)cap, gty, sigma, step, dst, S0, cx, ncoll, sep);
-            }
-            if (do_sample)
-                sample_cell<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc, ncoll, sep);
+        } else if (do_collide && nC > 1) {
+            double smax;
+            const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+            ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb, (int)cap, gty,
+                          sigma, step, dst, S0, cx, ncoll, sep);
+        }
+    }
+    __syncthreads();
+    // the cell's sums by the first wave (its velocities were written by thread 0)
+    if (do_sample && nC <= 65535 && threadIdx.x < WAVE) {
+        sample_cell_wave<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc);
+        if (threadIdx.x == 0) {
+            unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
+            unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
         }
     }
     __syncthreads();
@@ -1728,10 +1810,24 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
             cx.colls += L.rcoll[t];
         }
         __syncthreads();
-        if (do_sample && t >= a && t < b) {
-            const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
-            sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
-                             (double)L.rcoll[t], L.sep[t]);
+        if (do_sample) {
+            // cells of up to 64 parcels: one thread each; larger ones: one wave each
+            const int nbig = L.nbig;
+            if (t >= a && t < b) {
+                const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
+                if (nC <= 64) {
+                    sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
+                                     (double)L.rcoll[t], L.sep[t]);
+                } else {
+                    unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + t], (double)L.rcoll[t]);
+                    unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + t], L.sep[t]);
+                }
+            }
+            for (int bi = wid; bi < nbig; bi += CT / WAVE) {
+                const int l = L.big[bi];
+                sample_cell_wave<GEN>(P, c0 + l, L.segS[l] - S0, L.segS[l + 1] - L.segS[l], L.vx, L.vy, L.vz, L.er,
+                                      L.vb, PMAX, L.ty, samples, nc);
+            }
         }
         for (int j = t; j < cnt; j += CT) {
             const int64_t gd = (int64_t)S0 + j;
@@ -1742,6 +1838,8 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
                     for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
             }
         }
+        // every wave has read the big-cell list before it is cleared
+        __syncthreads();
         if (t == 0) L.nbig = 0;
         __syncthreads();
         a = b;

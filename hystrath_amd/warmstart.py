@@ -92,8 +92,9 @@ def decode(img5, case):
     """[grid, grid, 5] network variables -> [5, ncells] physical fields."""
     x = img5.reshape(-1, 5).astype(np.float64)
     m = fluid_mask(case).reshape(-1) > 0
-    rho = np.where(m, np.exp(x[:, 0]) * case.n_inf, 0.0)
-    T = np.where(m, np.exp(x[:, 3]) * case.T_inf, 0.0)
+    # predictions are bounded to the range the encoding can hold
+    rho = np.where(m, np.exp(np.clip(x[:, 0], np.log(RHO_FLOOR), np.log(1e4))) * case.n_inf, 0.0)
+    T = np.where(m, np.exp(np.clip(x[:, 3], np.log(T_FLOOR), np.log(1e4))) * case.T_inf, 0.0)
     return np.stack([rho, x[:, 1] * case.U_inf * m, x[:, 2] * case.U_inf * m, T, x[:, 4] * case.T_inf * m])
 
 
@@ -171,38 +172,37 @@ def generate(case_list, develop=3000, average=10000, window=WINDOW, log=print):
             e.close()
 
 
-def pairs_from(case_list, generated, freestream_every=5):
-    """(inputs [N, g, g, 16], targets [N, g, g, 16] residuals, masks [N, g, g])
-    from generated data: every window of every case, and the freestream
-    image of each case repeated once per freestream_every windows."""
+def pairs_from(case_list, generated, freestream_every=5, device="cuda"):
+    """Device tensors (inputs [N, g, g, 16] bf16 bits, residual targets
+    [N, g, g, 16] FP32, masks [N, g, g] FP32) from generated data: every
+    window of every case, and the freestream image of each case repeated once
+    per freestream_every windows (the surrogate task)."""
+    import torch
     X, Y, M = [], [], []
     for c, (wins, target) in zip(case_list, generated):
         mask = fluid_mask(c)
-        inputs = list(wins)
-        inputs += [freestream_image(c)] * max(1, len(wins) // freestream_every)
-        for img in inputs:
-            x = net_input(img, c)
-            y = np.zeros_like(x)
-            y[..., :5] = (target - img) * mask[..., None]
-            X.append(x)
-            Y.append(y)
-            M.append(mask)
-    return np.stack(X), np.stack(Y), np.stack(M)
+        inputs = list(wins) + [freestream_image(c)] * max(1, len(wins) // freestream_every)
+        xs = np.stack([net_input(img, c) for img in inputs])
+        ys = np.zeros_like(xs)
+        ys[..., :5] = (target[None] - np.stack(inputs)) * mask[None, ..., None]
+        X.append(torch.as_tensor(xs).to(device).to(torch.bfloat16).view(torch.int16))
+        Y.append(torch.as_tensor(ys).to(device))
+        M.append(torch.as_tensor(np.broadcast_to(mask, (len(inputs),) + mask.shape).copy()).to(device))
+    return torch.cat(X), torch.cat(Y), torch.cat(M)
 
 
 # ---------------------------------------------------------------------------
 # training and prediction
 # ---------------------------------------------------------------------------
 def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda"):
-    """Adam on the masked MSE of the residual; returns (net, per-epoch mean
+    """Adam on the masked MSE of the residual (device tensors from
+    pairs_from), cosine learning-rate decay; returns (net, per-epoch mean
     losses)."""
     import torch
     from . import ws
     net = ws.ConvNet(device, seed=seed)
     n = len(X)
-    Xd = torch.as_tensor(X).to(device).to(torch.bfloat16).view(torch.int16)
-    Yd = torch.as_tensor(Y).to(device)
-    Md = torch.as_tensor(M).to(device)
+    Xd, Yd, Md = X, Y, M
     rng = np.random.default_rng(seed)
     hist = []
     t0 = time.time()
@@ -336,3 +336,83 @@ def _reseeded(case, offset):
     c2.cfg.seed = case.cfg.seed + offset
     c2.cfg.cell_volume = abi.ptr(case.volumes, abi.C.c_double)
     return c2
+
+
+def save_net(net, path):
+    st = net.state()
+    arrays = {f"w{k}": a for k, a in enumerate(st["w"])}
+    arrays.update({f"b{k}": a for k, a in enumerate(st["b"])})
+    np.savez_compressed(path, t=np.array(st["t"]), **arrays)
+
+
+def load_net(path, device="cuda"):
+    import torch
+    from . import ws
+    net = ws.ConvNet(device)
+    d = np.load(path)
+    for k in range(len(ws.LAYERS)):
+        net.w[k].copy_(torch.as_tensor(d[f"w{k}"]))
+        net.b[k].copy_(torch.as_tensor(d[f"b{k}"]))
+    net.t = int(d["t"])
+    net.repack()
+    return net
+
+
+def main(argv=None):
+    """Generates training pairs from plain DSMC, trains the net and compares
+    the time to 2% fields of the plain and the warm path on held-out cases.
+    Prints one JSON object."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(description=main.__doc__.splitlines()[0])
+    ap.add_argument("--train-cases", type=int, default=40)
+    ap.add_argument("--test-cases", type=int, default=4)
+    ap.add_argument("--develop", type=int, default=3000)
+    ap.add_argument("--average", type=int, default=10000)
+    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--relax", type=int, default=300)
+    ap.add_argument("--save", default="")
+    ap.add_argument("--seed", type=int, default=2025)
+    a = ap.parse_args(argv)
+
+    def log(*x):
+        import sys
+        print(*x, file=sys.stderr, flush=True)
+
+    import torch
+    # the network's memory comes from torch: its runtime starts before the
+    # engines' so both see the device
+    torch.cuda.init()
+    torch.zeros(1, device="cuda")
+    t0 = time.time()
+    train_cases = make_cases(a.train_cases, a.seed)
+    test_cases = make_cases(a.test_cases, a.seed + 1)
+    gen = generate(train_cases + test_cases, develop=a.develop, average=a.average, log=log)
+    t_gen = time.time() - t0
+    X, Y, M = pairs_from(train_cases, gen[:a.train_cases])
+    log(f"[ws] {len(X)} training pairs")
+    t1 = time.time()
+    net, hist = train(X, Y, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    t_train = time.time() - t1
+    del X, Y, M
+    if a.save:
+        save_net(net, a.save)
+    results = []
+    for c, (wins, target) in zip(test_cases, gen[a.train_cases:]):
+        judge = judge_fields(c, target)
+        plain = plain_time_to_tol(c, judge, develop=a.develop)
+        warm = warm_time_to_tol(net, c, judge, relax=a.relax)
+        ratio = plain["steps"] / max(warm["steps"], 1)
+        log(f"[ws] case {c.ws_params}: plain {plain['steps']} steps, warm {warm['steps']} steps "
+            f"({warm['used']}), x{ratio:.2f}; denoised {warm['errors_denoised']}")
+        results.append(dict(params=c.ws_params, plain=plain, warm=warm, step_ratio=ratio))
+    out = dict(generate_s=round(t_gen, 1), train_s=round(t_train, 1), pairs=len(hist) and int(a.train_cases),
+               loss=[round(h, 6) for h in hist], cases=results)
+    print(json.dumps(out, default=float))
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -1198,45 +1198,66 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
 {
     const int ns = cfg->n_species;
     const hs_particles* P = &s->p;
+    /* a cell of more than 64 parcels is summed in the order of the sampling
+     * kernel's wave: lane l adds the terms of parcels l, l+64, ... in order,
+     * then the 64 partial sums are combined pairwise across lane distances
+     * 32, 16, 8, 4, 2, 1.  Smaller cells add their terms in order. */
+    const int wave = nC > 64;
+    const int lanes = wave ? 64 : 1;
+    const int hf = (cfg->flags & HS_FLAG_HEATFLUX) != 0;
+    enum { NQ = HS_SS_COUNT + HS_HF_COUNT };
+    const int nq = hf ? NQ : HS_SS_COUNT;
     for (int sp_id = 0; sp_id < ns; ++sp_id) {
         const hs_species* sp = &cfg->species[sp_id];
         const double m = sp->mass;
-        double acc[HS_SS_COUNT];
-        double h[HS_HF_COUNT];
-        memset(acc, 0, sizeof(acc));
-        memset(h, 0, sizeof(h));
+        double part[NQ][64];
+        for (int j = 0; j < nq; ++j)
+            for (int l = 0; l < lanes; ++l) part[j][l] = 0.0;
         for (int32_t k = 0; k < nC; ++k) {
             const int64_t i = b + k;
             const int t = P->type ? P->type[i] : 0;
             if (t != sp_id) continue;
+            double q[NQ];
             const double ux = P->ux[i], uy = P->uy[i], uz = P->uz[i];
             const double lke = m * (ux * ux + uy * uy + uz * uz);
             const double er = P->erot ? (double)P->erot[i] : 0.0;
             double ev = 0.0;
-            acc[HS_SS_N] += 1.0;
-            acc[HS_SS_MUX] += m * ux;
-            acc[HS_SS_MUY] += m * uy;
-            acc[HS_SS_MUZ] += m * uz;
-            acc[HS_SS_MCC] += lke;
-            acc[HS_SS_EROT] += er;
+            for (int j = HS_SS_EVIB0; j < HS_SS_COUNT; ++j) q[j] = 0.0;
+            q[HS_SS_N] = 1.0;
+            q[HS_SS_MUX] = m * ux;
+            q[HS_SS_MUY] = m * uy;
+            q[HS_SS_MUZ] = m * uz;
+            q[HS_SS_MCC] = lke;
+            q[HS_SS_EROT] = er;
             for (int mo = 0; mo < sp->n_vib_modes; ++mo) {
                 const double e = (double)P->vib[mo * s->capacity + i] * HS_KB * sp->theta_v[mo];
-                acc[HS_SS_EVIB0 + mo] += e;
+                q[HS_SS_EVIB0 + mo] = e;
                 ev += e;
             }
-            if (cfg->flags & HS_FLAG_HEATFLUX) {
+            double* h = q + HS_SS_COUNT;
+            if (hf) {
                 const double eint = er + ev;
-                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
-                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
-                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
-                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+                h[0] = m * (ux * ux); h[1] = m * ux * uy; h[2] = m * ux * uz;
+                h[3] = m * (uy * uy); h[4] = m * uy * uz; h[5] = m * (uz * uz);
+                h[6] = lke * ux; h[7] = lke * uy; h[8] = lke * uz;
+                h[9] = eint * ux; h[10] = eint * uy; h[11] = eint * uz; h[12] = eint;
             }
+            const int lane = wave ? (int)(k & 63) : 0;
+            for (int j = 0; j < nq; ++j) part[j][lane] += q[j];
+        }
+        if (wave) {
+            for (int o = 32; o > 0; o >>= 1)
+                for (int j = 0; j < nq; ++j) {
+                    double nxt[64];
+                    for (int l = 0; l < 64; ++l) nxt[l] = part[j][l] + part[j][l ^ o];
+                    memcpy(part[j], nxt, sizeof(nxt));
+                }
         }
         const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)sp_id * HS_SS_COUNT;
-        for (int k = 0; k < HS_SS_COUNT; ++k) s->samples[(base + k) * nc + c] += acc[k];
-        if (cfg->flags & HS_FLAG_HEATFLUX) {
+        for (int k = 0; k < HS_SS_COUNT; ++k) s->samples[(base + k) * nc + c] += part[k][0];
+        if (hf) {
             const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)ns * HS_SS_COUNT + (int64_t)sp_id * HS_HF_COUNT;
-            for (int k = 0; k < HS_HF_COUNT; ++k) s->samples[(hb + k) * nc + c] += h[k];
+            for (int k = 0; k < HS_HF_COUNT; ++k) s->samples[(hb + k) * nc + c] += part[HS_SS_COUNT + k][0];
         }
     }
     (void)d;

@@ -10,6 +10,22 @@ for p in (ROOT, TESTS):
         sys.path.insert(0, p)
 
 
+def pytest_sessionstart(session):
+    """The warm-start tests put torch device memory beside the engine
+    library's own HIP runtime.  torch's runtime has to start first: an engine
+    created before it leaves torch without a device.  So a session that may
+    run GPU tests starts it before any test."""
+    mark = session.config.getoption("-m") or ""
+    if "not gpu" in mark.replace("  ", " "):
+        return
+    try:
+        import torch
+        if torch.cuda.is_available():
+            torch.zeros(1, device="cuda")
+    except Exception:
+        pass
+
+
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP engine)")
     config.addinivalue_line("markers", "slow: long statistical test")

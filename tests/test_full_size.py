@@ -62,7 +62,12 @@ def test_cylinder_400x400_steps_match_the_oracle():
             eng.evolve(1)
             st.evolve(1)
             _compare(eng, st, f"cylinder400/step{k}", samples=False)
-        np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+        got, want = eng.samples()[0], st.sample_array()
+        bad = np.argwhere(got != want)
+        if len(bad):
+            cnt = np.bincount(st.particles()["cell"], minlength=case.ncells)
+            rows = [(int(k), int(c), float(got[k, c]), float(want[k, c]), int(cnt[c])) for k, c in bad[:12]]
+            raise AssertionError(f"{len(bad)} sample sums differ (slot, cell, engine, oracle, parcels now): {rows}")
         np.testing.assert_array_equal(eng.inflow()[: st.inflow.size], st.inflow)
 
 

@@ -17,6 +17,9 @@ BIN = os.path.join(ROOT, "hystrath_amd", "bin")
 REF = "/root/reference/run/hyStrath/dsmcFoam+"
 
 
+TOOLS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tools", "_bin")
+
+
 def run(app, case, timeout=300):
     return subprocess.run([os.path.join(BIN, app), "-case", case], capture_output=True, text=True, timeout=timeout)
 
@@ -36,7 +39,7 @@ def test_reads_every_reference_tutorial_dictionary():
             if os.path.exists(p):
                 files.append(p)
     assert len(files) > 25
-    r = subprocess.run([os.path.join(BIN, "dictCheck")] + files, capture_output=True, text=True)
+    r = subprocess.run([os.path.join(TOOLS, "dictCheck")] + files, capture_output=True, text=True)
     assert r.returncode == 0, r.stdout[-2000:]
     assert "BinaryCollisionModel" in r.stdout and "dsmcPatchBoundaries" in r.stdout
 
@@ -79,7 +82,7 @@ def test_generated_cases_parse(tmp_path):
     for maker in (casegen.couette_case, casegen.cylinder_case):
         case = maker(str(tmp_path / maker.__name__))
         files = glob.glob(os.path.join(case, "*/*Dict")) + glob.glob(os.path.join(case, "constant/*"))
-        r = subprocess.run([os.path.join(BIN, "dictCheck")] + files, capture_output=True, text=True)
+        r = subprocess.run([os.path.join(TOOLS, "dictCheck")] + files, capture_output=True, text=True)
         assert r.returncode == 0, r.stdout
         if no_gpu():
             r = run("dsmcInitialisePlus", case)

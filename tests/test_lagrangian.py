@@ -125,7 +125,7 @@ def test_engine_evolves_reference_snapshot_like_the_oracle(tmp_path):
 
 
 # --- the C++ host's reader and writer (hystrath_amd/host/lagrangianIO.C) ------
-CLOUDCHECK = os.path.join(os.path.dirname(GOLDEN), "..", "hystrath_amd", "bin", "cloudCheck")
+CLOUDCHECK = os.path.join(os.path.dirname(GOLDEN), "tools", "_bin", "cloudCheck")
 
 
 def _cloudcheck(*args):

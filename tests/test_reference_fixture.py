@@ -140,7 +140,10 @@ def test_engine_fields_hold_the_reference_in_their_95_percent_band():
         # (Zv of order 1e5): the lag-one autocorrelation, pooled over the cells,
         # widens the band by the AR(1) factor (1 + rho1) / (1 - rho1)
         rho1 = float(np.clip((dev[1:] * dev[:-1]).sum() / max((dev * dev).sum(), 1e-300), 0.0, 0.9))
-        se = x.std(0, ddof=1) / np.sqrt(nb) * np.sqrt((1 + rho1) / (1 - rho1))
+        sd = x.std(0, ddof=1) * np.sqrt((1 + rho1) / (1 - rho1))
+        # the reference's value is itself an average, over 450,000 steps: its
+        # standard error, that of one batch scaled to its length, adds to ours
+        se = sd * np.sqrt(1.0 / nb + m / 450000.0)
         z = (r - mean) / se
         out = np.mean(np.abs(z) > t95)
         g = x.mean(1)
