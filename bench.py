@@ -293,8 +293,9 @@ def warm_start_eval(args, log_fn):
 def run_batch(args):
     """64 independent cylinder cases (argon and N2 in turn, Mach 5-25, Kn
     0.01-1, 50^2 to 300^2 cells, 10k to 2M parcels each) stepped together by
-    hs_dsmc_evolve_batch, after the autotuner has measured the tilings of
-    each population class; value = particle-steps of all cases per second."""
+    hs_dsmc_evolve_batch (each engine's launches, interleaved on the engines'
+    streams), after the autotuner has measured the tilings of each population
+    class; value = particle-steps of all cases per second."""
     from hystrath_amd import cases
     from hystrath_amd.engine import Engine, evolve_batch
 

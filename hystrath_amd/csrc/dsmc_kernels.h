@@ -109,3 +109,55 @@ void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, con
 void launch_sigma_tcr(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, const double* up,
                       const double* uq, double* out, hipStream_t st);
 void launch_copy(const void* a, void* b, int64_t bytes, int grid, hipStream_t st);
+
+// ---------------------------------------------------------------------------
+// batched launches: one launch per stage over many engines (cases)
+// ---------------------------------------------------------------------------
+// One case's device state as the stage kernels take it.  buf[cur0 ^ par] is
+// the current population of a step at parity par, the other its next buffer.
+struct BatchCase {
+    const DevParams* P;
+    DevParticles buf[2];
+    int32_t cur0;
+    int32_t nc, ns;
+    int32_t pt_g, pt_c;         // gather and collide tile parcels
+    int32_t do_collide, do_sample, with_faces;
+    int64_t cap;
+    DevScalars* S;
+    int32_t* count;
+    int32_t* start;
+    int32_t* slot;
+    int32_t* perm;
+    uint8_t* sub;
+    uint16_t* big_list;
+    int32_t* tile_g;
+    int32_t* tile_c;
+    int32_t* heavy;
+    int32_t* order;
+    double* sigma;
+    double* rem;
+    double* samples;
+    const double* vol;
+    unsigned long long* lb_status;
+    unsigned long long* lb_ticket;
+    WallQueue wq;
+    DevFaces faces;
+};
+
+// Workgroup w of a batched launch is workgroup bx of gx of case c.
+struct BatchBlock {
+    int32_t c, bx, gx, pad;
+};
+
+// The stages of a step over the cases of C (tables T_* of n_* workgroups;
+// a stage with n = 0 launches nothing).  general: the cases carry internal
+// energies; track: the face tracker is on.
+void launch_batch_inject(const BatchCase* C, const BatchBlock* t_count, int n_count, const BatchBlock* t_scan,
+                         int n_scan, const BatchBlock* t_gen, int n_gen, int par, int ahead, hipStream_t st);
+void launch_batch_move(const BatchCase* C, const BatchBlock* t_move, int n_move, const BatchBlock* t_wall,
+                       int n_wall, int par, int track, hipStream_t st);
+void launch_batch_scan_cells(const BatchCase* C, const BatchBlock* t, int n, hipStream_t st);
+void launch_batch_scatter(const BatchCase* C, const BatchBlock* t, int n, int par, hipStream_t st);
+void launch_batch_gather(const BatchCase* C, const BatchBlock* t, int n, int par, int general, hipStream_t st);
+void launch_batch_collide(const BatchCase* C, const BatchBlock* t, int n, int par, int general, hipStream_t st);
+void launch_batch_commit(const BatchCase* C, int ncase, int advance, int inject_next, hipStream_t st);

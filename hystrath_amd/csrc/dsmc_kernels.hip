@@ -141,9 +141,8 @@ __global__ __launch_bounds__(BLOCK) void k_scan_apply(int32_t* __restrict__ in, 
 }
 
 // The whole scan in one workgroup (short arrays): tiles in order with a carry.
-__global__ __launch_bounds__(BLOCK) void k_scan_single(int32_t* __restrict__ in, int32_t* __restrict__ out,
-                                                       int32_t n, int mode, DevScalars* __restrict__ S,
-                                                       int64_t capacity) {
+__device__ __forceinline__ void scan_single_body(int32_t* __restrict__ in, int32_t* __restrict__ out, int32_t n,
+                                                 int mode, DevScalars* __restrict__ S, int64_t capacity) {
     int carry = 0;
     for (int tile = 0; tile < n; tile += SCAN_TILE) {
         const int base = tile + threadIdx.x * SCAN_ITEMS;
@@ -169,6 +168,12 @@ __global__ __launch_bounds__(BLOCK) void k_scan_single(int32_t* __restrict__ in,
         carry += tot;
     }
     if (threadIdx.x == 0) scan_publish(out, n, carry, mode, S, capacity);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scan_single(int32_t* __restrict__ in, int32_t* __restrict__ out,
+                                                       int32_t n, int mode, DevScalars* __restrict__ S,
+                                                       int64_t capacity) {
+    scan_single_body(in, out, n, mode, S, capacity);
 }
 
 void launch_scan(const int32_t* in, int32_t* out, int32_t n, ScanMode mode, ScanWs& ws, DevScalars* S,
@@ -212,20 +217,21 @@ __device__ __forceinline__ void tile_marks_after(int i, int32_t e, int32_t c, in
     for (int q = q0 + 1; q <= q1; ++q) tb[q] = i + 1;
 }
 
-__global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt, int32_t* __restrict__ start, int32_t n,
-                                                      unsigned long long* __restrict__ status,
-                                                      unsigned long long* __restrict__ ticket,
-                                                      DevScalars* __restrict__ S, int pt_g,
-                                                      int32_t* __restrict__ tile_g, int pt_c,
-                                                      int32_t* __restrict__ tile_c, int32_t* __restrict__ heavy) {
+// gx: the launch's workgroups over these cells (scan_cells_blocks(n))
+__device__ __forceinline__ void scan_cells_body(int32_t* __restrict__ cnt, int32_t* __restrict__ start, int32_t n,
+                                                unsigned long long* __restrict__ status,
+                                                unsigned long long* __restrict__ ticket,
+                                                DevScalars* __restrict__ S, int pt_g, int32_t* __restrict__ tile_g,
+                                                int pt_c, int32_t* __restrict__ tile_c, int32_t* __restrict__ heavy,
+                                                const int gx) {
     __shared__ int s_blk;
     __shared__ uint32_t s_epoch;
     __shared__ int s_excl;
     // logical block = order of start, so every predecessor has started
     if (threadIdx.x == 0) {
         const unsigned long long t = atomicAdd(ticket, 1ull);
-        s_blk = (int)(t % gridDim.x);
-        s_epoch = (uint32_t)((t / gridDim.x) % 0x3fffffffull) + 1;  // 1 .. 2^30 - 1: never a cleared word's 0
+        s_blk = (int)(t % (unsigned long long)gx);
+        s_epoch = (uint32_t)((t / (unsigned long long)gx) % 0x3fffffffull) + 1;  // 1 .. 2^30 - 1: never a cleared word's 0
     }
     __syncthreads();
     const int b = s_blk;
@@ -302,6 +308,15 @@ __global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt,
     }
 }
 
+__global__ __launch_bounds__(BLOCK) void k_scan_cells(int32_t* __restrict__ cnt, int32_t* __restrict__ start, int32_t n,
+                                                      unsigned long long* __restrict__ status,
+                                                      unsigned long long* __restrict__ ticket,
+                                                      DevScalars* __restrict__ S, int pt_g,
+                                                      int32_t* __restrict__ tile_g, int pt_c,
+                                                      int32_t* __restrict__ tile_c, int32_t* __restrict__ heavy) {
+    scan_cells_body(cnt, start, n, status, ticket, S, pt_g, tile_g, pt_c, tile_c, heavy, (int)gridDim.x);
+}
+
 void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long long* status,
                        unsigned long long* ticket, DevScalars* S, int pt_g, int32_t* tile_g,
                        int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st) {
@@ -359,10 +374,10 @@ __device__ __forceinline__ void new_internal(const DevParams& P, DevParticles& b
 // ---------------------------------------------------------------------------
 // ahead = 1: the next step's inflow, drawn before this step's commit
 // advances the step counter.
-__global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restrict__ Pp, DevFaces F, int ns,
-                                                        DevScalars* __restrict__ S, int ahead) {
+__device__ __forceinline__ void inject_count_body(const DevParams* __restrict__ Pp, const DevFaces& F, int ns,
+                                                  DevScalars* __restrict__ S, int ahead, const int bx) {
     const DevParams& P = *Pp;
-    const int e = blockIdx.x * BLOCK + threadIdx.x;
+    const int e = bx * BLOCK + threadIdx.x;
     if (e >= F.n * ns) return;
     const int f = e / ns, s = e - f * ns;
     if (!(P.inflow_n[s] > 0.0)) { F.n_new[e] = 0; return; }
@@ -376,6 +391,11 @@ __global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restr
     a -= (double)ni;
     *acc = a;
     F.n_new[e] = ni;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_inject_count(const DevParams* __restrict__ Pp, DevFaces F, int ns,
+                                                        DevScalars* __restrict__ S, int ahead) {
+    inject_count_body(Pp, F, ns, S, ahead, (int)blockIdx.x);
 }
 
 // Parcel j of this step belongs to the entry e (face*ns + species) with
@@ -442,8 +462,8 @@ __device__ void inject_one(const DevParams& P, const DevFaces& F, const int32_t*
 // One thread per new parcel; the entry offsets are searched in an LDS copy
 // when they fit.
 #define INJ_LDS_OFF 4096
-__global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
-                                                      DevParticles b, DevScalars* __restrict__ S, int ahead) {
+__device__ __forceinline__ void inject_gen_body(const DevParams* __restrict__ Pp, const DevFaces& F, int ns,
+                                                DevParticles& b, DevScalars* __restrict__ S, int ahead, const int bx) {
     __shared__ int32_t soff[INJ_LDS_OFF];
     const DevParams& P = *Pp;
     const int ne = F.n * ns;
@@ -453,12 +473,17 @@ __global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restric
         __syncthreads();
     }
     const int32_t* off = lds ? soff : F.off;
-    const int j = blockIdx.x * BLOCK + threadIdx.x;
+    const int j = bx * BLOCK + threadIdx.x;
     const int64_t idx = (int64_t)(ahead ? S->n_sorted : S->n_old) + j;
     const bool mine = j < off[ne] && idx < P.capacity;
     if (mine) inject_one(P, F, off, ns, S->step + (uint32_t)ahead, b, S, j, idx);
     // every lane reaches the wave reduction of the counter
     add_counter(S, CNT_INJ, mine ? 1ull : 0ull);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_inject_gen(const DevParams* __restrict__ Pp, DevFaces F, int ns,
+                                                      DevParticles b, DevScalars* __restrict__ S, int ahead) {
+    inject_gen_body(Pp, F, ns, b, S, ahead, (int)blockIdx.x);
 }
 
 void launch_inject_count(const DevParams* P, const DevFaces& F, int ns, DevScalars* S, int ahead, hipStream_t st) {
@@ -738,9 +763,10 @@ __device__ __forceinline__ void hist_slots(int32_t c, bool valid, int64_t i, int
 // tracker is on (HS_FLAG_FACE_FLUX with sampling); without it the tracker's
 // code is left out of the kernel.
 template <bool TRACK>
-__global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
-                                                DevScalars* __restrict__ S, int32_t* __restrict__ count,
-                                                int32_t* __restrict__ slot, WallQueue Q, int do_move, int do_hist) {
+__device__ __forceinline__ void move_body(const DevParams* __restrict__ Pp, DevParticles& b,
+                                          DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                          int32_t* __restrict__ slot, const WallQueue& Q, int do_move, int do_hist,
+                                          const int bx, const int gx) {
     const DevParams& P = *Pp;
     const int64_t n = S->n, n_old = S->n_old;
     const uint32_t step = S->step;
@@ -750,8 +776,7 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
     const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
     MoveCounts mc = { 0, 0, 0 };
     // wave-uniform trip count: every lane of a wave sees the same bases
-    for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < n;
-         base += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t base = bx * (int64_t)BLOCK + (threadIdx.x - lane); base < n; base += (int64_t)gx * BLOCK) {
         const int64_t i = base + lane;
         const bool valid = i < n;
         int32_t c = -1;
@@ -800,19 +825,27 @@ __global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp
             hist_slots(c, valid, i, lane, count, slot);
         }
     }
-    if (do_move && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&S->counters[CNT_PSTEPS], (unsigned long long)n);
+    if (do_move && bx == 0 && threadIdx.x == 0) atomicAdd(&S->counters[CNT_PSTEPS], (unsigned long long)n);
     add_counter(S, CNT_DEL, mc.del);
     add_counter(S, CNT_WALL, mc.wall);
     add_counter(S, CNT_TRACK_OVF, mc.ovf);
+}
+
+template <bool TRACK>
+__global__ __launch_bounds__(BLOCK) void k_move(const DevParams* __restrict__ Pp, DevParticles b,
+                                                DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                                int32_t* __restrict__ slot, WallQueue Q, int do_move, int do_hist) {
+    move_body<TRACK>(Pp, b, S, count, slot, Q, do_move, do_hist, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // The queued flights: the wall interaction they stopped at, the rest of the
 // flight (every further wall handled here), the cell histogram.  The queue is
 // cleared afterwards by k_commit (or k_clear_queue).
 template <bool TRACK>
-__global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict__ Pp, DevParticles b,
-                                                     DevScalars* __restrict__ S, int32_t* __restrict__ count,
-                                                     int32_t* __restrict__ slot, WallQueue Q, int do_hist) {
+__device__ __forceinline__ void move_wall_body(const DevParams* __restrict__ Pp, DevParticles& b,
+                                               DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                               int32_t* __restrict__ slot, const WallQueue& Q, int do_hist,
+                                               const int bx, const int gx) {
     const DevParams& P = *Pp;
     const int64_t nq = S->qn, n_old = S->n_old;
     const uint32_t step = S->step;
@@ -820,8 +853,7 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
     const bool measure = (P.flags & HS_FLAG_WALL) && (P.flags & HS_FLAG_SAMPLE) && P.n_wall > 0;
     const bool track = TRACK;
     MoveCounts mc = { 0, 0, 0 };
-    for (int64_t base = blockIdx.x * (int64_t)BLOCK + (threadIdx.x - lane); base < nq;
-         base += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t base = bx * (int64_t)BLOCK + (threadIdx.x - lane); base < nq; base += (int64_t)gx * BLOCK) {
         const int64_t k = base + lane;
         const bool valid = k < nq;
         int32_t c = -1;
@@ -849,6 +881,13 @@ __global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict
     add_counter(S, CNT_DEL, mc.del);
     add_counter(S, CNT_WALL, mc.wall);
     add_counter(S, CNT_TRACK_OVF, mc.ovf);
+}
+
+template <bool TRACK>
+__global__ __launch_bounds__(BLOCK) void k_move_wall(const DevParams* __restrict__ Pp, DevParticles b,
+                                                     DevScalars* __restrict__ S, int32_t* __restrict__ count,
+                                                     int32_t* __restrict__ slot, WallQueue Q, int do_hist) {
+    move_wall_body<TRACK>(Pp, b, S, count, slot, Q, do_hist, (int)blockIdx.x, (int)gridDim.x);
 }
 
 __global__ void k_clear_queue(DevScalars* S) { S->qn = 0; }
@@ -885,13 +924,13 @@ void launch_set_nold(DevScalars* S, hipStream_t st) { k_set_nold<<<1, 1, 0, st>>
 // heavy tiles first (they run longest), the others after.  Tiles are
 // independent, so the order does not change any result.  heavy[] comes from
 // the cell scan and is cleared here.
-__global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ cell, DevScalars* __restrict__ S,
-                                                   const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ slot, int32_t* __restrict__ perm,
-                                                   int32_t* __restrict__ heavy, int pt, int32_t* __restrict__ order) {
+__device__ __forceinline__ void scatter_body(const int32_t* __restrict__ cell, DevScalars* __restrict__ S,
+                                             const int32_t* __restrict__ start, const int32_t* __restrict__ slot,
+                                             int32_t* __restrict__ perm, int32_t* __restrict__ heavy, int pt,
+                                             int32_t* __restrict__ order, const int bx, const int gx) {
     const int64_t n = S->n;  // pre-sort population (the cell scan writes n_sorted)
     const int nt = (int)((int64_t)S->n_sorted / pt) + 1;
-    for (int q = blockIdx.x * BLOCK + threadIdx.x; q < nt; q += gridDim.x * BLOCK) {
+    for (int q = bx * BLOCK + threadIdx.x; q < nt; q += gx * BLOCK) {
         if (heavy[q]) {
             heavy[q] = 0;
             order[atomicAdd(&S->ord[0], 1)] = q;
@@ -899,10 +938,17 @@ __global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ c
             order[nt - 1 - atomicAdd(&S->ord[1], 1)] = q;
         }
     }
-    for (int64_t i = blockIdx.x * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gridDim.x * BLOCK) {
+    for (int64_t i = bx * (int64_t)BLOCK + threadIdx.x; i < n; i += (int64_t)gx * BLOCK) {
         const int32_t c = cell[i];
         if (c >= 0) perm[start[c] + slot[i]] = (int32_t)i;
     }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scatter(const int32_t* __restrict__ cell, DevScalars* __restrict__ S,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ slot, int32_t* __restrict__ perm,
+                                                   int32_t* __restrict__ heavy, int pt, int32_t* __restrict__ order) {
+    scatter_body(cell, S, start, slot, perm, heavy, pt, order, (int)blockIdx.x, (int)gridDim.x);
 }
 
 void launch_scatter(DevParticles b, DevScalars* S, const int32_t* start, const int32_t* slot, int32_t* perm,
@@ -1635,17 +1681,25 @@ __device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell
 }
 
 template <bool GEN>
-__global__ __launch_bounds__(CT) void k_gather(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
-                                               const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
-                                               const int32_t* __restrict__ tile_begin, int pt,
-                                               const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out) {
+__device__ __forceinline__ void gather_body(const DevParams* __restrict__ Pp, DevParticles& src, DevParticles& dst,
+                                            const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                                            const int32_t* __restrict__ tile_begin, int pt,
+                                            const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out,
+                                            const int q) {
     __shared__ GatherLds L;
     const DevParams& P = *Pp;
-    const int q = blockIdx.x;
     if ((int64_t)q * pt > (int64_t)S->n_sorted) return;
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
     for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
         gather_cells<GEN>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sub_out);
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(CT) void k_gather(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
+                                               const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
+                                               const int32_t* __restrict__ tile_begin, int pt,
+                                               const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out) {
+    gather_body<GEN>(Pp, src, dst, start, perm, tile_begin, pt, S, sub_out, (int)blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1872,19 +1926,18 @@ __global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ sta
 }
 
 template <bool GEN, int PMAX>
-__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
-                                                   const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ tile_begin,
-                                                   const int32_t* __restrict__ order, int pt,
-                                                   uint8_t* __restrict__ sub, double* __restrict__ sigma,
-                                                   double* __restrict__ rem, double* __restrict__ samples,
-                                                   const double* __restrict__ vol, DevScalars* __restrict__ S,
-                                                   int do_collide, int do_sample, int nc, uint16_t* big_list) {
+__device__ __forceinline__ void collide_body(const DevParams* __restrict__ Pp, DevParticles& dst,
+                                             const int32_t* __restrict__ start, const int32_t* __restrict__ tile_begin,
+                                             const int32_t* __restrict__ order, int pt, uint8_t* __restrict__ sub,
+                                             double* __restrict__ sigma, double* __restrict__ rem,
+                                             double* __restrict__ samples, const double* __restrict__ vol,
+                                             DevScalars* __restrict__ S, int do_collide, int do_sample, int nc,
+                                             uint16_t* big_list, const int bx) {
     __shared__ TileLds<GEN, PMAX> L;
     const DevParams& P = *Pp;
     const int64_t nsort = S->n_sorted;
-    if ((int64_t)blockIdx.x * pt > nsort) return;
-    const int q = order[blockIdx.x];
+    if ((int64_t)bx * pt > nsort) return;
+    const int q = order[bx];
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
     if (cb >= ce) return;
     const uint32_t step = S->step;
@@ -1895,6 +1948,19 @@ __global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__
     add_counter(S, CNT_CAND, cx.cand);
     add_counter(S, CNT_COLL, cx.colls);
     add_counter(S, CNT_TILE_OVF, cx.ovf);
+}
+
+template <bool GEN, int PMAX>
+__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ tile_begin,
+                                                   const int32_t* __restrict__ order, int pt,
+                                                   uint8_t* __restrict__ sub, double* __restrict__ sigma,
+                                                   double* __restrict__ rem, double* __restrict__ samples,
+                                                   const double* __restrict__ vol, DevScalars* __restrict__ S,
+                                                   int do_collide, int do_sample, int nc, uint16_t* big_list) {
+    collide_body<GEN, PMAX>(Pp, dst, start, tile_begin, order, pt, sub, sigma, rem, samples, vol, S, do_collide,
+                            do_sample, nc, big_list, (int)blockIdx.x);
 }
 
 void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
@@ -1931,7 +1997,7 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
 
 // publish the sorted population (with the next step's inflow when it was
 // injected ahead); optionally advance the step counter
-__global__ void k_commit(DevScalars* S, int advance, int with_next) {
+__device__ __forceinline__ void commit_body(DevScalars* S, int advance, int with_next) {
     S->n = with_next ? S->n_next : S->n_sorted;
     S->n_old = S->n_sorted;
     S->ord[0] = 0;
@@ -1939,6 +2005,7 @@ __global__ void k_commit(DevScalars* S, int advance, int with_next) {
     S->qn = 0;  // the move's wall queue, consumed by k_move_wall
     if (advance) S->step += 1;
 }
+__global__ void k_commit(DevScalars* S, int advance, int with_next) { commit_body(S, advance, with_next); }
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st) {
     k_commit<<<1, 1, 0, st>>>(S, advance, with_next);
 }
@@ -2114,4 +2181,159 @@ __global__ __launch_bounds__(BLOCK) void k_copy(const double4* __restrict__ a, d
 }
 void launch_copy(const void* a, void* b, int64_t bytes, int grid, hipStream_t st) {
     k_copy<<<grid, BLOCK, 0, st>>>((const double4*)a, (double4*)b, bytes / 32);
+}
+
+// ---------------------------------------------------------------------------
+// batched stages: every workgroup reads its case and its index within that
+// case's launch from the table (one scalar load), then runs the same body as
+// the single-engine kernel, so a batched step equals the engines' own steps
+// ---------------------------------------------------------------------------
+// The tables are read through the constant address space: the loads are
+// invariant for the kernel, so the compiler may reload a descriptor field
+// rather than hold it in registers across the stage (as with kernel arguments).
+#if defined(__HIP_DEVICE_COMPILE__)
+#define HS_CONST_AS __attribute__((address_space(4)))
+#else
+#define HS_CONST_AS
+#endif
+__device__ __forceinline__ const HS_CONST_AS BatchCase& batch_case(const BatchCase* __restrict__ C,
+                                                                   const BatchBlock* __restrict__ T, int& bx,
+                                                                   int& gx) {
+    const HS_CONST_AS BatchBlock* Tc = (const HS_CONST_AS BatchBlock*)T;
+    const HS_CONST_AS BatchCase* Cc = (const HS_CONST_AS BatchCase*)C;
+    const BatchBlock t = Tc[blockIdx.x];
+    bx = t.bx;
+    gx = t.gx;
+    return Cc[t.c];
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_inject_count(const BatchCase* __restrict__ C,
+                                                         const BatchBlock* __restrict__ T, int ahead) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    const DevFaces F = c.faces;
+    inject_count_body(c.P, F, c.ns, c.S, ahead, bx);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_inject_scan(const BatchCase* __restrict__ C,
+                                                        const BatchBlock* __restrict__ T, int ahead) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    scan_single_body(c.faces.n_new, c.faces.off, c.faces.n * c.ns, ahead ? SCAN_APPEND_NEXT : SCAN_APPEND, c.S,
+                     c.cap);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_inject_gen(const BatchCase* __restrict__ C,
+                                                       const BatchBlock* __restrict__ T, int par, int ahead) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    const DevFaces F = c.faces;
+    DevParticles b = c.buf[(c.cur0 ^ par ^ ahead) & 1];
+    inject_gen_body(c.P, F, c.ns, b, c.S, ahead, bx);
+}
+
+// the occupancy of the single-engine k_move (at most 80 registers without the
+// face tracker)
+template <bool TRACK>
+__global__ __launch_bounds__(BLOCK) __attribute__((amdgpu_waves_per_eu(TRACK ? 5 : 6, 8)))
+void kb_move(const BatchCase* __restrict__ C, const BatchBlock* __restrict__ T, int par) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    DevParticles b = c.buf[(c.cur0 ^ par) & 1];
+    const WallQueue Q = c.wq;
+    move_body<TRACK>(c.P, b, c.S, c.count, c.slot, Q, 1, 1, bx, gx);
+}
+
+template <bool TRACK>
+__global__ __launch_bounds__(BLOCK) void kb_move_wall(const BatchCase* __restrict__ C,
+                                                      const BatchBlock* __restrict__ T, int par) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    DevParticles b = c.buf[(c.cur0 ^ par) & 1];
+    const WallQueue Q = c.wq;
+    move_wall_body<TRACK>(c.P, b, c.S, c.count, c.slot, Q, 1, bx, gx);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_scan_cells(const BatchCase* __restrict__ C,
+                                                       const BatchBlock* __restrict__ T) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    (void)bx;
+    scan_cells_body(c.count, c.start, c.nc, c.lb_status, c.lb_ticket, c.S, c.pt_g, c.tile_g, c.pt_c, c.tile_c,
+                    c.heavy, gx);
+}
+
+__global__ __launch_bounds__(BLOCK) void kb_scatter(const BatchCase* __restrict__ C, const BatchBlock* __restrict__ T,
+                                                    int par) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    scatter_body(c.buf[(c.cur0 ^ par) & 1].cell, c.S, c.start, c.slot, c.perm, c.heavy, c.pt_c, c.order, bx, gx);
+}
+
+template <bool GEN>
+__global__ __launch_bounds__(CT) void kb_gather(const BatchCase* __restrict__ C, const BatchBlock* __restrict__ T,
+                                                int par) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    DevParticles src = c.buf[(c.cur0 ^ par) & 1];
+    DevParticles dst = c.buf[(c.cur0 ^ par ^ 1) & 1];
+    gather_body<GEN>(c.P, src, dst, c.start, c.perm, c.tile_g, c.pt_g, c.S, c.sub, bx);
+}
+
+template <bool GEN, int PMAX>
+__global__ __launch_bounds__(CT, 2) void kb_collide(const BatchCase* __restrict__ C,
+                                                    const BatchBlock* __restrict__ T, int par) {
+    int bx, gx;
+    const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
+    DevParticles dst = c.buf[(c.cur0 ^ par ^ 1) & 1];
+    collide_body<GEN, PMAX>(c.P, dst, c.start, c.tile_c, c.order, c.pt_c, c.sub, c.sigma, c.rem, c.samples, c.vol,
+                            c.S, c.do_collide, c.do_sample, c.nc, c.big_list, bx);
+}
+
+__global__ void kb_commit(const BatchCase* __restrict__ C, int ncase, int advance, int inject_next) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < ncase) commit_body(C[e].S, advance, inject_next && C[e].with_faces);
+}
+
+void launch_batch_inject(const BatchCase* C, const BatchBlock* t_count, int n_count, const BatchBlock* t_scan,
+                         int n_scan, const BatchBlock* t_gen, int n_gen, int par, int ahead, hipStream_t st) {
+    if (n_count > 0) kb_inject_count<<<n_count, BLOCK, 0, st>>>(C, t_count, ahead);
+    if (n_scan > 0) kb_inject_scan<<<n_scan, BLOCK, 0, st>>>(C, t_scan, ahead);
+    if (n_gen > 0) kb_inject_gen<<<n_gen, BLOCK, 0, st>>>(C, t_gen, par, ahead);
+}
+
+void launch_batch_move(const BatchCase* C, const BatchBlock* t_move, int n_move, const BatchBlock* t_wall,
+                       int n_wall, int par, int track, hipStream_t st) {
+    if (n_move <= 0) return;
+    if (track) {
+        kb_move<true><<<n_move, BLOCK, 0, st>>>(C, t_move, par);
+        kb_move_wall<true><<<n_wall, BLOCK, 0, st>>>(C, t_wall, par);
+    } else {
+        kb_move<false><<<n_move, BLOCK, 0, st>>>(C, t_move, par);
+        kb_move_wall<false><<<n_wall, BLOCK, 0, st>>>(C, t_wall, par);
+    }
+}
+
+void launch_batch_scan_cells(const BatchCase* C, const BatchBlock* t, int n, hipStream_t st) {
+    if (n > 0) kb_scan_cells<<<n, BLOCK, 0, st>>>(C, t);
+}
+
+void launch_batch_scatter(const BatchCase* C, const BatchBlock* t, int n, int par, hipStream_t st) {
+    if (n > 0) kb_scatter<<<n, BLOCK, 0, st>>>(C, t, par);
+}
+
+void launch_batch_gather(const BatchCase* C, const BatchBlock* t, int n, int par, int general, hipStream_t st) {
+    if (n <= 0) return;
+    if (general) kb_gather<true><<<n, CT, 0, st>>>(C, t, par);
+    else kb_gather<false><<<n, CT, 0, st>>>(C, t, par);
+}
+
+void launch_batch_collide(const BatchCase* C, const BatchBlock* t, int n, int par, int general, hipStream_t st) {
+    if (n <= 0) return;
+    if (general) kb_collide<true, PMAX_GEN><<<n, CT, 0, st>>>(C, t, par);
+    else kb_collide<false, PMAX_MONO><<<n, CT, 0, st>>>(C, t, par);
+}
+
+void launch_batch_commit(const BatchCase* C, int ncase, int advance, int inject_next, hipStream_t st) {
+    if (ncase > 0) kb_commit<<<(ncase + 63) / 64, 64, 0, st>>>(C, ncase, advance, inject_next);
 }

@@ -27,6 +27,7 @@ EXPORTS = [
     "hs_dsmc_sigma_tcr", "hs_dsmc_device_copy", "hs_dsmc_set_flags", "hs_dsmc_wall_face_count",
     "hs_dsmc_wall_faces", "hs_dsmc_read_wall_samples", "hs_dsmc_face_count", "hs_dsmc_read_face_flux",
     "hs_dsmc_set_tiles", "hs_dsmc_get_tiles", "hs_dsmc_autotune", "hs_dsmc_evolve_batch",
+    "hs_dsmc_batch_create", "hs_dsmc_batch_evolve", "hs_dsmc_batch_destroy", "hs_dsmc_batch_last_error",
 ]
 
 STAGES = ["inject", "move", "scan", "scatter", "collide", "gather"]
@@ -95,6 +96,10 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_set_flags": ([eng, C.c_int32], C.c_int),
         "hs_dsmc_set_tiles": ([eng, C.c_int32, C.c_int32], C.c_int),
         "hs_dsmc_evolve_batch": ([P(C.c_void_p), C.c_int32, C.c_int32], C.c_int),
+        "hs_dsmc_batch_create": ([P(C.c_void_p), C.c_int32, P(C.c_void_p)], C.c_int),
+        "hs_dsmc_batch_evolve": ([C.c_void_p, C.c_int32], C.c_int),
+        "hs_dsmc_batch_destroy": ([C.c_void_p], None),
+        "hs_dsmc_batch_last_error": ([C.c_void_p], C.c_char_p),
         "hs_dsmc_get_tiles": ([eng, P(C.c_int32), P(C.c_int32)], C.c_int),
         "hs_dsmc_autotune": ([eng, C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_double)], C.c_int),
         "hs_dsmc_wall_face_count": ([eng, P(C.c_int64)], C.c_int),
@@ -135,6 +140,43 @@ def evolve_batch(engines, nsteps=1):
     if rc != 0:
         msgs = [L.hs_dsmc_last_error(e.h).decode() for e in engines if e.h]
         raise EngineError(f"evolve_batch failed ({rc}): {'; '.join(m for m in msgs if m) or L.hs_dsmc_create_error().decode()}")
+
+
+class Batch:
+    """hs_dsmc_batch: the engines stepped with one launch per stage over all
+    of them.  The engines must outlive the batch."""
+
+    def __init__(self, engines):
+        self.L = load_library()
+        self.engines = list(engines)
+        hs = (C.c_void_p * len(self.engines))(*[e.h for e in self.engines])
+        h = C.c_void_p()
+        rc = self.L.hs_dsmc_batch_create(hs, len(self.engines), C.byref(h))
+        if rc != 0:
+            raise EngineError(f"batch_create failed ({rc}): {self.L.hs_dsmc_batch_last_error(None).decode()}")
+        self.h = h
+
+    def evolve(self, nsteps=1):
+        rc = self.L.hs_dsmc_batch_evolve(self.h, nsteps)
+        if rc != 0:
+            raise EngineError(f"batch_evolve failed ({rc}): {self.L.hs_dsmc_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            self.L.hs_dsmc_batch_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def _d(a):

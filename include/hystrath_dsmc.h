@@ -278,6 +278,24 @@ int hs_dsmc_evolve(hs_dsmc_engine* eng, int32_t nsteps);
  * the failing engine (NULL engine: hs_dsmc_create_error), and engines before
  * it in the batch may have advanced further. */
 int hs_dsmc_evolve_batch(hs_dsmc_engine* const* engines, int32_t count, int32_t nsteps);
+/* A batch of engines stepped with one launch per stage: every stage of a step
+ * runs as one kernel launch over the workgroups of all the batch's cases (per
+ * kernel variant: cases with internal energies and cases with the face
+ * tracker on take their own launches), so small cases fill the device
+ * together instead of one after another.  Every engine's result is the same
+ * as when it runs alone.  No counterpart in the reference, which runs one
+ * case per process: the batch form for parameter sweeps and ensembles.
+ * The engines must be on one device and outlive the batch; between batch
+ * evolves they may be used on their own (their streams are ordered with the
+ * batch's).  Errors: hs_dsmc_batch_last_error (NULL batch: the create error).
+ * hs_dsmc_batch_evolve is switched off in this build and fails with
+ * HS_E_INVALID for nsteps > 0: its first device run crashed the host process
+ * and the cause is not identified; hs_dsmc_evolve_batch steps batches. */
+typedef struct hs_dsmc_batch hs_dsmc_batch;
+int hs_dsmc_batch_create(hs_dsmc_engine* const* engines, int32_t count, hs_dsmc_batch** out);
+int hs_dsmc_batch_evolve(hs_dsmc_batch* batch, int32_t nsteps);
+void hs_dsmc_batch_destroy(hs_dsmc_batch* batch);
+const char* hs_dsmc_batch_last_error(const hs_dsmc_batch* batch);
 /* Stage entry points (same kernels, for tests and custom loops). */
 int hs_dsmc_inject(hs_dsmc_engine* eng);                 /* controlParcelsBeforeMove */
 int hs_dsmc_move(hs_dsmc_engine* eng);                   /* Cloud<dsmcParcel>::move  */

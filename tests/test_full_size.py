@@ -8,8 +8,9 @@
 - Periodic argon box, 20^3 cells, 20 parcels a cell (160k), bimodal 300 K /
   1000 K start, 2,000 steps: energy conserved, T = 650 K, and the sampled
   collision rate equals the VHS equilibrium frequency (Bird 4.74) within 1%.
-- The 64-case batch of uneven cylinder cases through hs_dsmc_evolve_batch:
-  no device-side bound trips, and three of its engines equal their oracles.
+- The 64-case batch of uneven argon and N2 cylinder cases (10k to 2M parcels)
+  through hs_dsmc_evolve_batch: no device-side bound trips, and three of its
+  engines equal their oracles.
 
 Reference behaviour: dsmcCloud::evolve (dsmcCloud.C:819-926) and the parts it
 calls, as in the smaller parity tests.
