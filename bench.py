@@ -247,15 +247,15 @@ def warm_start_eval(args, log_fn):
     trained network committed in hystrath_amd/data.  Per case: the judge is
     the 10,000-step long average of one run; the plain path (fill, 3,000
     development steps, sampling) and the warm path (the network's fields
-    fill the domain, 300 relaxation steps, one 200-step window denoised, the
-    plain average as fallback) are independent runs timed to fields within
-    2% of the judge."""
+    fill the domain, 500 relaxation steps, one window denoised, the plain
+    average as fallback) are independent runs timed to fields within 2% of
+    the judge."""
     from hystrath_amd import warmstart as W
     from hystrath_amd.engine import Engine
     path = os.path.join(ROOT, "hystrath_amd", "data", "ws_cylinder256.npz")
     if not os.path.exists(path):
         return None
-    net = W.load_net(path)
+    nets = W.load_nets(path)
     cs = W.make_cases(args.ws_cases, seed=7777)
     gen = W.generate(cs, develop=3000, average=10000, log=log_fn)
     out = []
@@ -271,7 +271,7 @@ def warm_start_eval(args, log_fn):
             e.synchronize()
             ms = 1e3 * (time.perf_counter() - t) / 500
         plain = W.plain_time_to_tol(c, judge, develop=3000)
-        warm = W.warm_time_to_tol(net, c, judge, relax=300)
+        warm = W.warm_time_to_tol(nets, c, judge, relax=500)
         tp = plain["steps"] * ms * 1e-3
         tw = warm["steps"] * ms * 1e-3 + warm["inference_s"]
         out.append({"params": {k: round(v, 4) for k, v in c.ws_params.items()}, "ms_per_step": round(ms, 4),
@@ -286,7 +286,8 @@ def warm_start_eval(args, log_fn):
                f"({warm['used']}), x{tp / max(tw, 1e-9):.2f}")
     sp = [r["speedup"] for r in out]
     return {"cases": out, "speedup_geomean": round(float(np.exp(np.mean(np.log(sp)))), 3) if sp else None,
-            "network": "6-layer 3x3 conv, 8-32-64-64-64-32-5 channels, bfloat16 MFMA kernels, FP32 accumulation",
+            "network": "surrogate and denoiser: 6-layer dilated 3x3 conv (dilations 1-2-4-8-16-1), "
+                       "11-32-64-64-64-32-5 channels, bfloat16 MFMA kernels, FP32 accumulation",
             "tolerance": 0.02}
 
 

@@ -47,15 +47,16 @@ __device__ __forceinline__ bf16x8 zero8() {
 
 // One wave: 32 consecutive pixels of one image row times NB blocks of 32
 // output channels.  The K loop runs over the 9 taps and the input channels
-// in steps of 16.  Waves are independent (no barriers): a wave past the end
-// returns.
+// in steps of 16.  Taps are dil pixels apart (a dilated 3x3 convolution,
+// zero padding dil).  Waves are independent (no barriers): a wave past the
+// end returns.
 template <int NB>
 __global__ __launch_bounds__(kThreads) void k_conv3x3(const uint16_t* __restrict__ x,
                                                      const uint16_t* __restrict__ wp,
                                                      const float* __restrict__ bias,
                                                      const uint16_t* __restrict__ mask,
                                                      uint16_t* __restrict__ y, float* __restrict__ y32, int n,
-                                                     int h, int w, int cin, int cout, int relu) {
+                                                     int h, int w, int cin, int cout, int relu, int dil) {
     const int wave = threadIdx.x >> 6, l = threadIdx.x & 63;
     const int m = l & 31, hh = l >> 5;
     const int segs = w >> 5;
@@ -74,7 +75,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(const uint16_t* __restrict
         for (int v = 0; v < 16; ++v) acc[nb][v] = 0.0f;
     for (int tap = 0; tap < 9; ++tap) {
         const int ky = tap / 3, kx = tap - 3 * ky;
-        const int yy = yy0 + ky - 1, xx = x0 + m + kx - 1;
+        const int yy = yy0 + (ky - 1) * dil, xx = x0 + m + (kx - 1) * dil;
         const bool ok = yy >= 0 && yy < h && xx >= 0 && xx < w;
         const uint16_t* px = x + ((img * h + (ok ? yy : 0)) * w + (ok ? xx : 0)) * cin + 8 * hh;
         const uint16_t* pw = wp + ((int64_t)tap * cout_pad + m) * cin + 8 * hh;
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(kThreads) void k_conv3x3(const uint16_t* __restrict
     }
 }
 
-// Weight gradient: dw[tap][ci][co] += sum_p x(p + tap) dy(p).  A workgroup
+// Weight gradient: dw[tap][ci][co] += sum_p x(p + dil * tap) dy(p).  A workgroup
 // takes whole image rows; per chunk of 16 pixels it stages the nine shifted
 // input patches and the output-gradient patch in LDS, transposed so each
 // lane's 8 consecutive pixels are one 16-byte read, and the four waves share
@@ -116,7 +117,7 @@ constexpr int kChunk = 16;
 template <int MB, int NB>
 __global__ __launch_bounds__(kThreads) void k_wgrad(const uint16_t* __restrict__ x, const uint16_t* __restrict__ dy,
                                                    float* __restrict__ dw, float* __restrict__ db, int n, int h,
-                                                   int w, int cin, int cout, int rows_per_block) {
+                                                   int w, int cin, int cout, int rows_per_block, int dil) {
     __shared__ __attribute__((aligned(16))) uint16_t xs[9][kMaxC][kChunk];   // [tap][ci][pixel]
     __shared__ __attribute__((aligned(16))) uint16_t ds[kMaxC][kChunk];      // [co][pixel]
     constexpr int T = 9 * MB * NB;
@@ -137,7 +138,7 @@ __global__ __launch_bounds__(kThreads) void k_wgrad(const uint16_t* __restrict__
             for (int e = t; e < 9 * cin * kChunk; e += kThreads) {
                 const int ci = e % cin, j = (e / cin) % kChunk, tap = e / (cin * kChunk);
                 const int ky = tap / 3, kx = tap - 3 * ky;
-                const int sy = yy + ky - 1, sx = p0 + j + kx - 1;
+                const int sy = yy + (ky - 1) * dil, sx = p0 + j + (kx - 1) * dil;
                 uint16_t v = 0;
                 if (sy >= 0 && sy < h && sx >= 0 && sx < w) v = x[((img * h + sy) * w + sx) * cin + ci];
                 xs[tap][ci][j] = v;
@@ -270,23 +271,24 @@ extern "C" int hs_ws_pack(const float* w, int32_t cin, int32_t cout, uint16_t* w
 
 extern "C" int hs_ws_conv3x3(const uint16_t* x, const uint16_t* w_packed, const float* bias, const uint16_t* mask,
                              uint16_t* y, float* y32, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
-                             int32_t relu, void* stream) {
+                             int32_t relu, int32_t dilation, void* stream) {
     if (!x || !w_packed || (!y && !y32) || n < 1 || h < 1 || w < 32 || (w & 31) || !channels_ok(cin) ||
-        cout < 1 || cout > kMaxC)
+        cout < 1 || cout > kMaxC || dilation < 1 || dilation > 64)
         return HS_E_INVALID;
     const int64_t waves = (int64_t)n * h * (w / 32);
     const unsigned grid = (unsigned)((waves + 3) / 4);
     hipStream_t st = (hipStream_t)stream;
     if (cout <= 32)
-        k_conv3x3<1><<<grid, kThreads, 0, st>>>(x, w_packed, bias, mask, y, y32, n, h, w, cin, cout, relu);
+        k_conv3x3<1><<<grid, kThreads, 0, st>>>(x, w_packed, bias, mask, y, y32, n, h, w, cin, cout, relu, dilation);
     else
-        k_conv3x3<2><<<grid, kThreads, 0, st>>>(x, w_packed, bias, mask, y, y32, n, h, w, cin, cout, relu);
+        k_conv3x3<2><<<grid, kThreads, 0, st>>>(x, w_packed, bias, mask, y, y32, n, h, w, cin, cout, relu, dilation);
     return launched();
 }
 
 extern "C" int hs_ws_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int32_t n, int32_t h,
-                                   int32_t w, int32_t cin, int32_t cout, void* stream) {
-    if (!x || !dy || !dw || n < 1 || h < 1 || w < kChunk || (w % kChunk) || !channels_ok(cin) || !channels_ok(cout))
+                                   int32_t w, int32_t cin, int32_t cout, int32_t dilation, void* stream) {
+    if (!x || !dy || !dw || n < 1 || h < 1 || w < kChunk || (w % kChunk) || !channels_ok(cin) || !channels_ok(cout) ||
+        dilation < 1 || dilation > 64)
         return HS_E_INVALID;
     const int64_t rows = (int64_t)n * h;
     const int64_t blocks = rows < 2048 ? rows : 2048;
@@ -294,10 +296,10 @@ extern "C" int hs_ws_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float*
     const unsigned grid = (unsigned)((rows + rpb - 1) / rpb);
     hipStream_t st = (hipStream_t)stream;
     const int mb = cin > 32 ? 2 : 1, nb = cout > 32 ? 2 : 1;
-    if (mb == 1 && nb == 1) k_wgrad<1, 1><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb);
-    else if (mb == 1) k_wgrad<1, 2><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb);
-    else if (nb == 1) k_wgrad<2, 1><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb);
-    else k_wgrad<2, 2><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb);
+    if (mb == 1 && nb == 1) k_wgrad<1, 1><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb, dilation);
+    else if (mb == 1) k_wgrad<1, 2><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb, dilation);
+    else if (nb == 1) k_wgrad<2, 1><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb, dilation);
+    else k_wgrad<2, 2><<<grid, kThreads, 0, st>>>(x, dy, dw, db, n, h, w, cin, cout, rpb, dilation);
     return launched();
 }
 

@@ -4,32 +4,45 @@ No counterpart in the reference.  The plain DSMC path reaches converged
 fields by developing the flow from a uniform fill and then averaging for
 thousands of steps.  This module shortens both:
 
-- a net (hystrath_amd.ws.ConvNet, bfloat16 MFMA kernels) maps an image of
-  fields to the long-average fields of the same case: from the freestream
-  state alone (a surrogate of the developed flow) or from a 200-step
-  sampling window (a denoiser);
+- two nets of one architecture (hystrath_amd.ws.ConvNet, bfloat16 MFMA
+  kernels) map an image of fields to the long-average fields of the same
+  case: the surrogate from the freestream state alone (the developed flow
+  from the case's conditions and geometry), the denoiser from a short
+  running average of a developed flow together with the surrogate's fields
+  (its prior) and the average's length;
 - the warm path fills the domain from the surrogate's fields
-  (hs_dsmc_fill_cells), relaxes briefly, samples one window and denoises it;
+  (hs_dsmc_fill_cells), relaxes briefly, then samples, and every check
+  interval denoises the running average; it stops at the first check whose
+  denoised (or plain) fields are within the tolerance;
 - the plain long-average DSMC stays the judge of every prediction's error,
-  per field, and the fallback: the warm path keeps sampling the particles
-  it started if the prediction is not within the tolerance.
+  per field.
+
+The denoiser learns from averages of 200 to 1,600 steps cut from the plain
+runs, with priors from surrogates that did not see the case (two-fold
+cross-fitting), so it learns how far to trust a prior it has not been
+fitted to.
 
 Image channels (pixel = cell (i, j) of the mesh, row j, column i):
   0 ln(rho / rho_inf), 1 Ux / U_inf, 2 Uy / U_inf, 3 ln(T_tr / T_inf),
   4 internal temperature (zero for argon), 5 Mach / 25, 6 log10(Kn) + 1,
-  7 T_wall / T_inf / 5; channels 8..15 zero.  Cells inside the cylinder are
-  zero and masked out of the loss.
+  7 T_wall / T_inf / 5, 8 distance of the cell centre from the cylinder's
+  surface in radii / 10 (clipped to [0, 1]), 9 and 10 the cell centre's x
+  and y scaled to [-1, 1] over the domain, 11..15 the prior's channels 0..4
+  (zero for the surrogate), 16 the noise level sqrt(200 / averaged steps)
+  (zero for the surrogate); channels 17..31 zero.  Cells inside the cylinder
+  are zero and masked out of the loss.
 """
 import math
 import time
 
 import numpy as np
 
-from . import abi, cases, convergence, fields
+from . import abi, cases, convergence, fields, ws
 from .engine import Engine, evolve_batch
 
 GRID = 256
 WINDOW = 200
+WINDOW_MULTS = (1, 2, 4, 8)   # training averages of 200, 400, 800 and 1,600 steps
 RHO_FLOOR = 1e-3   # of rho_inf, before the log
 T_FLOOR = 1e-2     # of T_inf
 
@@ -103,12 +116,36 @@ def conditions(case):
     return np.array([p["mach"] / 25.0, math.log10(p["kn"]) + 1.0, p["T_wall"] / case.T_inf / 5.0], np.float32)
 
 
-def net_input(img5, case):
-    """The 16-channel input image: the 5 field channels, 3 condition planes."""
+def geometry_planes(case):
+    """[ny, nx, 3]: distance from the cylinder's surface in radii / 10
+    (clipped to [0, 1]) and the cell centre scaled to [-1, 1], fluid cells
+    only."""
+    c = case.cfg
+    ny, nx = c.ny, c.nx
+    xc = c.origin[0] + (np.arange(nx) + 0.5) * c.delta[0]
+    yc = c.origin[1] + (np.arange(ny) + 0.5) * c.delta[1]
+    X, Y = np.meshgrid(xc, yc)
+    r = np.hypot(X - c.cyl_center[0], Y - c.cyl_center[1])
+    d = np.clip((r - c.cyl_radius) / c.cyl_radius / 10.0, 0.0, 1.0)
+    sx = 2.0 * (X - c.origin[0]) / (nx * c.delta[0]) - 1.0
+    sy = 2.0 * (Y - c.origin[1]) / (ny * c.delta[1]) - 1.0
+    return np.stack([d, sx, sy], axis=-1).astype(np.float32) * fluid_mask(case)[..., None]
+
+
+def net_input(img5, case, prior=None, nsteps=0):
+    """The input image (ws.C_IN channels): the 5 field channels, 3 condition
+    planes, 3 geometry planes, the prior's 5 channels and the noise level of
+    an average of nsteps steps (both zero when not given)."""
     ny, nx = case.cfg.ny, case.cfg.nx
-    x = np.zeros((ny, nx, 16), np.float32)
+    mask = fluid_mask(case)
+    x = np.zeros((ny, nx, ws.C_IN), np.float32)
     x[..., :5] = img5
-    x[..., 5:8] = conditions(case)[None, None, :] * fluid_mask(case)[..., None]
+    x[..., 5:8] = conditions(case)[None, None, :] * mask[..., None]
+    x[..., 8:11] = geometry_planes(case)
+    if prior is not None:
+        x[..., 11:16] = prior * mask[..., None]
+    if nsteps:
+        x[..., 16] = math.sqrt(WINDOW / nsteps) * mask
     return x
 
 
@@ -123,12 +160,14 @@ def freestream_image(case):
 # ---------------------------------------------------------------------------
 # training pairs from plain DSMC
 # ---------------------------------------------------------------------------
-def generate(case_list, develop=3000, average=10000, window=WINDOW, log=print):
+def generate(case_list, develop=3000, average=10000, window=WINDOW, mults=WINDOW_MULTS, log=print):
     """Steps every case together (hs_dsmc_evolve_batch): a uniform fill,
     develop steps without sampling, then average steps sampled, read every
-    window steps.  Returns per case: (list of window images, long-average
-    image).  The window images are differences of the cumulative sums, so
-    all come from the one run whose average is the target."""
+    window steps.  Returns per case: (list of (steps, image) averages,
+    long-average image).  The averages are differences of the cumulative
+    sums over m * window steps for m in mults (non-overlapping; of the
+    shortest only every other one), so all come from the one run whose
+    average is the target."""
     engs = []
     try:
         flags = []
@@ -149,15 +188,21 @@ def generate(case_list, develop=3000, average=10000, window=WINDOW, log=print):
             e.set_flags(f)
             e.reset_samples()
         log(f"[ws] {len(engs)} cases developed for {develop} steps in {time.time() - t0:.1f} s")
-        prev = [None] * len(engs)
+        ring = [{0: None} for _ in engs]   # cumulative sums at the last boundaries (None: zero)
         wins = [[] for _ in engs]
-        for w0 in range(0, average, window):
+        keep = max(mults)
+        for j in range(1, average // window + 1):
             evolve_batch(engs, window)
             for k, (e, c) in enumerate(zip(engs, case_list)):
                 s, n = e.samples()
-                ws_ = s - prev[k] if prev[k] is not None else s.copy()
-                prev[k] = s
-                wins[k].append(encode(case_fields(ws_, window, c), c))
+                ring[k][j] = s
+                for m in mults:
+                    if j % m or (m == 1 and len(mults) > 1 and j % 2):
+                        continue
+                    base = ring[k][j - m]
+                    ws_ = s - base if base is not None else s
+                    wins[k].append((m * window, encode(case_fields(ws_, m * window, c), c)))
+                ring[k].pop(j - keep, None)
         out = []
         for k, (e, c) in enumerate(zip(engs, case_list)):
             s, n = e.samples()
@@ -165,43 +210,78 @@ def generate(case_list, develop=3000, average=10000, window=WINDOW, log=print):
             c_ = e.counters()
             if c_["track_overflow"] or c_["tile_overflow"] or c_["capacity_overflow"]:
                 raise RuntimeError(f"case {k}: device-side bound tripped while generating data: {c_}")
-        log(f"[ws] {average} averaged steps, {average // window} windows a case, {time.time() - t0:.1f} s in all")
+        log(f"[ws] {average} averaged steps, {len(wins[0]) if wins else 0} averages a case, "
+            f"{time.time() - t0:.1f} s in all")
         return out
     finally:
         for e in engs:
             e.close()
 
 
-def pairs_from(case_list, generated, freestream_every=5, device="cuda"):
-    """Device tensors (inputs [N, g, g, 16] bf16 bits, residual targets
-    [N, g, g, 16] FP32, masks [N, g, g] FP32) from generated data: every
-    window of every case, and the freestream image of each case repeated once
-    per freestream_every windows (the surrogate task)."""
+def _tensors(case_list, per_case, device):
+    """per_case: for each case (list of input images, list of the field
+    images the residual is taken from, long-average image)."""
     import torch
     X, Y, M = [], [], []
-    for c, (wins, target) in zip(case_list, generated):
+    for c, (ins, bases, target) in zip(case_list, per_case):
         mask = fluid_mask(c)
-        inputs = list(wins) + [freestream_image(c)] * max(1, len(wins) // freestream_every)
-        xs = np.stack([net_input(img, c) for img in inputs])
-        ys = np.zeros_like(xs)
-        ys[..., :5] = (target[None] - np.stack(inputs)) * mask[None, ..., None]
+        xs = np.stack(ins)
+        ys = np.zeros((len(ins),) + mask.shape + (ws.C_OUT,), np.float32)
+        ys[..., :5] = (target[None] - np.stack(bases)) * mask[None, ..., None]
         X.append(torch.as_tensor(xs).to(device).to(torch.bfloat16).view(torch.int16))
         Y.append(torch.as_tensor(ys).to(device))
-        M.append(torch.as_tensor(np.broadcast_to(mask, (len(inputs),) + mask.shape).copy()).to(device))
+        M.append(torch.as_tensor(np.broadcast_to(mask, (len(ins),) + mask.shape).copy()).to(device))
     return torch.cat(X), torch.cat(Y), torch.cat(M)
+
+
+def surrogate_pairs(case_list, generated, device="cuda"):
+    """Device tensors (inputs [N, g, g, C_IN] bf16 bits, residual targets
+    [N, g, g, C_OUT] FP32, masks [N, g, g] FP32) with one input per case:
+    its freestream image."""
+    per = [([net_input(freestream_image(c), c)], [freestream_image(c)], t) for c, (_, t) in zip(case_list, generated)]
+    return _tensors(case_list, per, device)
+
+
+def denoiser_pairs(case_list, generated, priors, device="cuda"):
+    """The same tensors for every average of every case, with the case's
+    prior (surrogate fields) and the average's length as inputs."""
+    per = []
+    for c, (wins, t), pr in zip(case_list, generated, priors):
+        per.append(([net_input(img, c, prior=pr, nsteps=n) for n, img in wins], [img for _, img in wins], t))
+    return _tensors(case_list, per, device)
+
+
+def train_surrogates(case_list, generated, epochs=400, batch=16, lr=2e-3, log=print, device="cuda"):
+    """The surrogate on every case, and each case's prior from a surrogate
+    trained on the other half of the cases (two folds): returns (surrogate,
+    priors [case][g, g, 5], per-epoch losses of the full surrogate)."""
+    priors = [None] * len(case_list)
+    idx = np.arange(len(case_list))
+    for f in (0, 1):
+        tr, te = idx[idx % 2 != f], idx[idx % 2 == f]
+        X, Y, M = surrogate_pairs([case_list[i] for i in tr], [generated[i] for i in tr], device)
+        net, _ = train(X, Y, M, epochs=epochs, batch=batch, lr=lr, log=log, device=device, what=f"surrogate fold {f}",
+                       seed=10 + f)
+        del X, Y, M
+        for i in te:
+            priors[i] = predict(net, freestream_image(case_list[i]), case_list[i], device)
+    X, Y, M = surrogate_pairs(case_list, generated, device)
+    net, hist = train(X, Y, M, epochs=epochs, batch=batch, lr=lr, log=log, device=device, what="surrogate")
+    return net, priors, hist
 
 
 # ---------------------------------------------------------------------------
 # training and prediction
 # ---------------------------------------------------------------------------
-def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda"):
+def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda", what="net"):
     """Adam on the masked MSE of the residual (device tensors from
-    pairs_from), cosine learning-rate decay; returns (net, per-epoch mean
-    losses)."""
+    denoiser_pairs or surrogate_pairs), cosine learning-rate decay; returns
+    (net, per-epoch mean losses)."""
     import torch
     from . import ws
     net = ws.ConvNet(device, seed=seed)
     n = len(X)
+    batch = min(batch, n)
     Xd, Yd, Md = X, Y, M
     rng = np.random.default_rng(seed)
     hist = []
@@ -218,14 +298,16 @@ def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda
             net.adam(net.backward(acts, dy), lr=lr_ep)
             tot += loss.item()
         hist.append(tot / max(1, n // batch))
-        log(f"[ws] epoch {ep + 1}/{epochs}: loss {hist[-1]:.5f} ({time.time() - t0:.1f} s)")
+        if ep == epochs - 1 or (ep + 1) % max(1, epochs // 10) == 0:
+            log(f"[ws] {what} epoch {ep + 1}/{epochs}: loss {hist[-1]:.5f} ({time.time() - t0:.1f} s)")
     return net, hist
 
 
-def predict(net, img5, case, device="cuda"):
-    """Long-average network variables [g, g, 5] from an input image."""
+def predict(net, img5, case, device="cuda", prior=None, nsteps=0):
+    """Long-average network variables [g, g, 5] from an input image (and,
+    for the denoiser, its prior and averaged steps)."""
     import torch
-    x = torch.as_tensor(net_input(img5, case)[None]).to(device).to(torch.bfloat16).view(torch.int16)
+    x = torch.as_tensor(net_input(img5, case, prior, nsteps)[None]).to(device).to(torch.bfloat16).view(torch.int16)
     out = net.forward(x).cpu().numpy()[0]
     return (img5 + out[..., :5]) * fluid_mask(case)[..., None]
 
@@ -282,18 +364,21 @@ def plain_time_to_tol(case, judge, develop=3000, check=250, max_sampled=20000, t
                 errors=err)
 
 
-def warm_time_to_tol(net, case, judge, relax=300, window=WINDOW, check=250, max_sampled=20000, tol=0.02,
-                     seed_offset=2000, device="cuda"):
-    """Steps of the warm path: the surrogate's fields from the freestream
-    image fill the domain, relax steps unsampled, one sampled window is
-    denoised by the net; if that is not within tol of the judge the
-    particles keep being sampled (the plain average is the fallback)."""
+def warm_time_to_tol(nets, case, judge, relax=500, check=WINDOW, max_sampled=20000, tol=0.02, seed_offset=2000,
+                     device="cuda"):
+    """Steps of the warm path: the surrogate's fields (nets[0], from the
+    freestream image) fill the domain and relax steps run unsampled; then
+    every check steps the running average is denoised (nets[1], with the
+    surrogate's fields as its prior), and the path stops at the first check
+    whose denoised fields -- or, failing that, whose plain running average
+    -- are within tol of the judge."""
     import torch
+    surrogate, denoiser = nets
     c2 = _reseeded(case, seed_offset)
     t_inf = 0.0
     with Engine(c2) as e:
         t = time.perf_counter()
-        img0 = predict(net, freestream_image(c2), c2, device)
+        img0 = predict(surrogate, freestream_image(c2), c2, device)
         torch.cuda.synchronize()
         t_inf += time.perf_counter() - t
         warm_fill(e, c2, img0)
@@ -303,28 +388,32 @@ def warm_time_to_tol(net, case, judge, relax=300, window=WINDOW, check=250, max_
             e.evolve(relax)
         e.set_flags(flags)
         e.reset_samples()
-        e.evolve(window)
-        s, n = e.samples()
-        t = time.perf_counter()
-        img1 = predict(net, encode(case_fields(s, n, c2), c2), c2, device)
-        torch.cuda.synchronize()
-        t_inf += time.perf_counter() - t
-        err_net = _errors_of_image(img1, c2, judge)
         err_surr = _errors_of_image(img0, c2, judge)
-        err_raw = _errors_of_image(encode(case_fields(s, n, c2), c2), c2, judge)
-        done, err, used = window, err_net, "denoised window"
-        if not convergence.converged(err, tol):
-            used = "plain average after the warm fill"
-            while done < max_sampled:
-                e.evolve(check)
-                done += check
-                s, n = e.samples()
-                err = _errors_of_image(encode(case_fields(s, n, c2), c2), c2, judge)
-                if convergence.converged(err, tol):
-                    break
+        done, err, used = 0, None, "none within the tolerance"
+        first = None
+        while done < max_sampled:
+            e.evolve(check)
+            done += check
+            s, n = e.samples()
+            raw = encode(case_fields(s, n, c2), c2)
+            t = time.perf_counter()
+            den = predict(denoiser, raw, c2, device, prior=img0, nsteps=n)
+            torch.cuda.synchronize()
+            t_inf += time.perf_counter() - t
+            err_d = _errors_of_image(den, c2, judge)
+            err_r = _errors_of_image(raw, c2, judge)
+            if first is None:
+                first = (err_r, err_d)
+            if convergence.converged(err_d, tol):
+                err, used = err_d, "denoised running average"
+                break
+            if convergence.converged(err_r, tol):
+                err, used = err_r, "plain running average"
+                break
+            err = err_d
         e.synchronize()
     return dict(steps=relax + done, relax=relax, sampled=done, reached=convergence.converged(err, tol),
-                errors=err, errors_surrogate=err_surr, errors_window=err_raw, errors_denoised=err_net,
+                errors=err, errors_surrogate=err_surr, errors_window=first[0], errors_denoised=first[1],
                 used=used, inference_s=t_inf)
 
 
@@ -338,24 +427,36 @@ def _reseeded(case, offset):
     return c2
 
 
-def save_net(net, path):
-    st = net.state()
-    arrays = {f"w{k}": a for k, a in enumerate(st["w"])}
-    arrays.update({f"b{k}": a for k, a in enumerate(st["b"])})
-    np.savez_compressed(path, t=np.array(st["t"]), **arrays)
+def save_nets(nets, path):
+    """Both nets (surrogate, denoiser) in one .npz, with the layer table."""
+    from . import ws
+    arrays = {"layers": np.array(ws.LAYERS, np.int32)}
+    for tag, net in zip(("s", "d"), nets):
+        st = net.state()
+        arrays.update({f"{tag}_w{k}": a for k, a in enumerate(st["w"])})
+        arrays.update({f"{tag}_b{k}": a for k, a in enumerate(st["b"])})
+        arrays[f"{tag}_t"] = np.array(st["t"])
+    np.savez_compressed(path, **arrays)
 
 
-def load_net(path, device="cuda"):
+def load_nets(path, device="cuda"):
+    """(surrogate, denoiser) from save_nets; refuses a file written for
+    another layer table."""
     import torch
     from . import ws
-    net = ws.ConvNet(device)
     d = np.load(path)
-    for k in range(len(ws.LAYERS)):
-        net.w[k].copy_(torch.as_tensor(d[f"w{k}"]))
-        net.b[k].copy_(torch.as_tensor(d[f"b{k}"]))
-    net.t = int(d["t"])
-    net.repack()
-    return net
+    if "layers" not in d or d["layers"].tolist() != [list(r) for r in ws.LAYERS]:
+        raise ValueError(f"{path}: saved for another network layout")
+    nets = []
+    for tag in ("s", "d"):
+        net = ws.ConvNet(device)
+        for k in range(len(ws.LAYERS)):
+            net.w[k].copy_(torch.as_tensor(d[f"{tag}_w{k}"]))
+            net.b[k].copy_(torch.as_tensor(d[f"{tag}_b{k}"]))
+        net.t = int(d[f"{tag}_t"])
+        net.repack()
+        nets.append(net)
+    return tuple(nets)
 
 
 def main(argv=None):
@@ -369,10 +470,11 @@ def main(argv=None):
     ap.add_argument("--test-cases", type=int, default=4)
     ap.add_argument("--develop", type=int, default=3000)
     ap.add_argument("--average", type=int, default=10000)
-    ap.add_argument("--epochs", type=int, default=20)
+    ap.add_argument("--epochs", type=int, default=20, help="denoiser epochs")
+    ap.add_argument("--epochs-surrogate", type=int, default=400)
     ap.add_argument("--batch", type=int, default=16)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--relax", type=int, default=300)
+    ap.add_argument("--relax", type=int, default=500)
     ap.add_argument("--save", default="")
     ap.add_argument("--seed", type=int, default=2025)
     a = ap.parse_args(argv)
@@ -391,25 +493,30 @@ def main(argv=None):
     test_cases = make_cases(a.test_cases, a.seed + 1)
     gen = generate(train_cases + test_cases, develop=a.develop, average=a.average, log=log)
     t_gen = time.time() - t0
-    X, Y, M = pairs_from(train_cases, gen[:a.train_cases])
-    log(f"[ws] {len(X)} training pairs")
     t1 = time.time()
-    net, hist = train(X, Y, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log)
+    surrogate, priors, hist_s = train_surrogates(train_cases, gen[:a.train_cases], epochs=a.epochs_surrogate,
+                                                 batch=a.batch, lr=a.lr, log=log)
+    X, Y, M = denoiser_pairs(train_cases, gen[:a.train_cases], priors)
+    log(f"[ws] {len(X)} denoiser pairs")
+    denoiser, hist = train(X, Y, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, what="denoiser", seed=1)
     t_train = time.time() - t1
     del X, Y, M
+    nets = (surrogate, denoiser)
     if a.save:
-        save_net(net, a.save)
+        save_nets(nets, a.save)
     results = []
     for c, (wins, target) in zip(test_cases, gen[a.train_cases:]):
         judge = judge_fields(c, target)
         plain = plain_time_to_tol(c, judge, develop=a.develop)
-        warm = warm_time_to_tol(net, c, judge, relax=a.relax)
+        warm = warm_time_to_tol(nets, c, judge, relax=a.relax)
         ratio = plain["steps"] / max(warm["steps"], 1)
         log(f"[ws] case {c.ws_params}: plain {plain['steps']} steps, warm {warm['steps']} steps "
-            f"({warm['used']}), x{ratio:.2f}; denoised {warm['errors_denoised']}")
+            f"({warm['used']}), x{ratio:.2f}; surrogate {warm['errors_surrogate']}; window {warm['errors_window']}; "
+            f"denoised {warm['errors_denoised']}")
         results.append(dict(params=c.ws_params, plain=plain, warm=warm, step_ratio=ratio))
-    out = dict(generate_s=round(t_gen, 1), train_s=round(t_train, 1), pairs=len(hist) and int(a.train_cases),
-               loss=[round(h, 6) for h in hist], cases=results)
+    out = dict(generate_s=round(t_gen, 1), train_s=round(t_train, 1), train_cases=a.train_cases,
+               loss_surrogate=[round(h, 6) for h in hist_s], loss_denoiser=[round(h, 6) for h in hist],
+               cases=results)
     print(json.dumps(out, default=float))
     return 0
 

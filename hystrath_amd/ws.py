@@ -1,10 +1,11 @@
 """The learned warm-start's network on the gfx950 kernels of include/hystrath_ws.h.
 
-No counterpart in the reference.  A six-layer 3x3 convolutional encoder-
-decoder (8 -> 32 -> 64 -> 64 -> 64 -> 32 -> 5 channels, ReLU between layers,
-same resolution throughout) maps the fields of a short sampling window plus
-three condition planes to a correction of those fields towards their long
-average.  Activations are bfloat16 NHWC images, weights FP32 masters with
+No counterpart in the reference.  A six-layer 3x3 convolutional net
+(17 -> 32 -> 64 -> 64 -> 64 -> 32 -> 5 channels, ReLU between layers, same
+resolution throughout, dilations 1, 2, 4, 8, 16, 1: each output pixel sees a
+65-pixel square) maps five field channels, three condition planes, three
+geometry planes, five prior-field channels and a noise-level plane to a
+correction of the five fields.  Activations are bfloat16 NHWC images, weights FP32 masters with
 bfloat16 packings, and every convolution, input gradient and weight gradient
 is a bfloat16 MFMA kernel accumulating in FP32.  torch provides device memory
 and the stream only.
@@ -15,11 +16,12 @@ import numpy as np
 
 from . import engine
 
-# (true in, true out, stored in, stored out) channels per layer
-LAYERS = [(8, 32, 16, 32), (32, 64, 32, 64), (64, 64, 64, 64), (64, 64, 64, 64), (64, 32, 64, 32), (32, 5, 32, 16)]
+# (true in, true out, stored in, stored out) channels and the dilation of each layer
+LAYERS = [(17, 32, 32, 32, 1), (32, 64, 32, 64, 2), (64, 64, 64, 64, 4), (64, 64, 64, 64, 8), (64, 32, 64, 32, 16),
+          (32, 5, 32, 16, 1)]
 N_FIELDS = 5
-N_IN = 8
-C_IN = 16    # stored channels of the input image
+N_IN = 17
+C_IN = 32    # stored channels of the input image
 C_OUT = 16   # stored channels of the output image
 
 _sig_done = False
@@ -32,8 +34,8 @@ def lib():
         P, vp = C.POINTER, C.c_void_p
         i32 = C.c_int32
         L.hs_ws_pack.argtypes = [vp, i32, i32, vp, vp, vp]
-        L.hs_ws_conv3x3.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]
-        L.hs_ws_conv3x3_wgrad.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, i32, vp]
+        L.hs_ws_conv3x3.argtypes = [vp, vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, i32, vp]
+        L.hs_ws_conv3x3_wgrad.argtypes = [vp, vp, vp, vp, i32, i32, i32, i32, i32, i32, vp]
         L.hs_ws_mse.argtypes = [vp, vp, vp, vp, vp, i32, i32, i32, i32, i32, C.c_double, vp]
         L.hs_ws_adam.argtypes = [vp, vp, vp, vp, C.c_int64, C.c_float, C.c_float, C.c_float, C.c_float, i32, vp]
         L.hs_ws_mfma_tile.argtypes = [i32, P(C.c_uint16), P(C.c_uint16), P(C.c_float)]
@@ -68,14 +70,15 @@ def from_bf16_bits(b):
     return b.view(torch.bfloat16).float()
 
 
-def conv3x3(x_bits, w_pack, bias, n, h, w, cin, cout, relu=False, mask_bits=None, out_bf16=True, out_f32=False):
-    """One convolution on bfloat16 NHWC images (int16 bit tensors)."""
+def conv3x3(x_bits, w_pack, bias, n, h, w, cin, cout, relu=False, mask_bits=None, out_bf16=True, out_f32=False,
+            dilation=1):
+    """One (dilated) 3x3 convolution on bfloat16 NHWC images (int16 bit tensors)."""
     import torch
     dev = x_bits.device
     y = torch.empty((n, h, w, cout), dtype=torch.int16, device=dev) if out_bf16 else None
     y32 = torch.empty((n, h, w, cout), dtype=torch.float32, device=dev) if out_f32 else None
     _ck(lib().hs_ws_conv3x3(_p(x_bits), _p(w_pack), _p(bias), _p(mask_bits), _p(y), _p(y32), n, h, w, cin, cout,
-                            1 if relu else 0, _stream()), "hs_ws_conv3x3")
+                            1 if relu else 0, dilation, _stream()), "hs_ws_conv3x3")
     return y, y32
 
 
@@ -89,8 +92,8 @@ def pack(wm, cin, cout):
     return wf, wb
 
 
-def wgrad(x_bits, dy_bits, dw, db, n, h, w, cin, cout):
-    _ck(lib().hs_ws_conv3x3_wgrad(_p(x_bits), _p(dy_bits), _p(dw), _p(db), n, h, w, cin, cout, _stream()),
+def wgrad(x_bits, dy_bits, dw, db, n, h, w, cin, cout, dilation=1):
+    _ck(lib().hs_ws_conv3x3_wgrad(_p(x_bits), _p(dy_bits), _p(dw), _p(db), n, h, w, cin, cout, dilation, _stream()),
         "hs_ws_conv3x3_wgrad")
 
 
@@ -102,7 +105,7 @@ class ConvNet:
         import torch
         g = torch.Generator().manual_seed(seed)
         self.w, self.b = [], []
-        for k, (ti, to, si, so) in enumerate(LAYERS):
+        for k, (ti, to, si, so, _) in enumerate(LAYERS):
             std = (2.0 / (9 * ti)) ** 0.5 * (0.1 if k == len(LAYERS) - 1 else 1.0)
             wm = torch.zeros(9, si, so)
             wm[:, :ti, :to] = torch.randn(9, ti, to, generator=g) * std
@@ -115,7 +118,7 @@ class ConvNet:
 
     def repack(self):
         self.wf, self.wb = [], []
-        for (ti, to, si, so), wm in zip(LAYERS, self.w):
+        for (ti, to, si, so, _), wm in zip(LAYERS, self.w):
             f, b = pack(wm, si, so)
             self.wf.append(f)
             self.wb.append(b)
@@ -130,10 +133,10 @@ class ConvNet:
         acts = [x_bits]
         a = x_bits
         out = None
-        for k, (ti, to, si, so) in enumerate(LAYERS):
+        for k, (ti, to, si, so, dl) in enumerate(LAYERS):
             last = k == len(LAYERS) - 1
             y, y32 = conv3x3(a, self.wf[k], self.b[k], n, h, w, si, so, relu=not last, out_bf16=not last,
-                             out_f32=last)
+                             out_f32=last, dilation=dl)
             if last:
                 out = y32
             else:
@@ -150,11 +153,11 @@ class ConvNet:
         gb = [torch.zeros_like(t) for t in self.b]
         d = dy_bits
         for k in range(len(LAYERS) - 1, -1, -1):
-            ti, to, si, so = LAYERS[k]
-            wgrad(acts[k], d, gw[k], gb[k], n, h, w, si, so)
+            ti, to, si, so, dl = LAYERS[k]
+            wgrad(acts[k], d, gw[k], gb[k], n, h, w, si, so, dilation=dl)
             if k > 0:
                 # dL/d(input of layer k) through the ReLU of layer k-1: masked by its output
-                d, _ = conv3x3(d, self.wb[k], None, n, h, w, so, si, mask_bits=acts[k])
+                d, _ = conv3x3(d, self.wb[k], None, n, h, w, so, si, mask_bits=acts[k], dilation=dl)
         return gw + gb
 
     def adam(self, grads, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):

@@ -30,7 +30,8 @@ extern "C" {
  * (tap 8 - t, cin_pad = cin rounded up to 32); padding is zero. */
 int hs_ws_pack(const float* w, int32_t cin, int32_t cout, uint16_t* w_fwd, uint16_t* w_bwd, void* stream);
 
-/* y = act(conv3x3(x, w) + bias), zero padding 1, stride 1.  x has cin
+/* y = act(conv3x3(x, w) + bias), taps dilation pixels apart (1: the plain
+ * 3x3), zero padding dilation, stride 1.  x has cin
  * channels, y has cout channels (true cout <= cout_pad of the packing).
  * relu: apply max(0, .).  mask (may be NULL): an NHWC bf16 tensor with cout
  * channels; the result is multiplied by (mask > 0), which turns the input
@@ -38,14 +39,14 @@ int hs_ws_pack(const float* w, int32_t cin, int32_t cout, uint16_t* w_fwd, uint1
  * bias may be NULL.  y (bf16) and/or y32 (FP32, same layout) may be NULL. */
 int hs_ws_conv3x3(const uint16_t* x, const uint16_t* w_packed, const float* bias, const uint16_t* mask,
                   uint16_t* y, float* y32, int32_t n, int32_t h, int32_t w, int32_t cin, int32_t cout,
-                  int32_t relu, void* stream);
+                  int32_t relu, int32_t dilation, void* stream);
 
 /* Weight and bias gradients of y = conv3x3(x, w) + b given dy = dL/dy (the
  * gradient at the convolution's output, ReLU already applied):
- * dw[9][cin][cout] += sum over pixels of x(p + tap) dy(p), db[cout] += sum dy.
- * Accumulates into dw and db (clear them first). */
+ * dw[9][cin][cout] += sum over pixels of x(p + dilation * tap) dy(p),
+ * db[cout] += sum dy.  Accumulates into dw and db (clear them first). */
 int hs_ws_conv3x3_wgrad(const uint16_t* x, const uint16_t* dy, float* dw, float* db, int32_t n, int32_t h,
-                        int32_t w, int32_t cin, int32_t cout, void* stream);
+                        int32_t w, int32_t cin, int32_t cout, int32_t dilation, void* stream);
 
 /* Masked mean-squared error of pred (FP32 NHWC, c channels, of which the
  * first nf count) against target (same layout), over pixels whose mask (FP32
