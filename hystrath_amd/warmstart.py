@@ -20,7 +20,10 @@ thousands of steps.  This module shortens both:
 The denoiser learns from averages of 200 to 1,600 steps cut from the plain
 runs, with priors from surrogates that did not see the case (two-fold
 cross-fitting), so it learns how far to trust a prior it has not been
-fitted to.
+fitted to.  It sees a 17-pixel square and no condition planes (the
+surrogate sees 65 pixels and the conditions), so it cannot recognise a
+training case and recall its average; every training pair is also used
+mirrored about the symmetry line y = 0 (Uy and the y plane change sign).
 
 Image channels (pixel = cell (i, j) of the mesh, row j, column i):
   0 ln(rho / rho_inf), 1 Ux / U_inf, 2 Uy / U_inf, 3 ln(T_tr / T_inf),
@@ -29,8 +32,9 @@ Image channels (pixel = cell (i, j) of the mesh, row j, column i):
   surface in radii / 10 (clipped to [0, 1]), 9 and 10 the cell centre's x
   and y scaled to [-1, 1] over the domain, 11..15 the prior's channels 0..4
   (zero for the surrogate), 16 the noise level sqrt(200 / averaged steps)
-  (zero for the surrogate); channels 17..31 zero.  Cells inside the cylinder
-  are zero and masked out of the loss.
+  (zero for the surrogate); channels 17..31 zero.  The denoiser's condition
+  planes are zero.  Cells inside the cylinder are zero and masked out of the
+  loss.
 """
 import math
 import time
@@ -132,15 +136,17 @@ def geometry_planes(case):
     return np.stack([d, sx, sy], axis=-1).astype(np.float32) * fluid_mask(case)[..., None]
 
 
-def net_input(img5, case, prior=None, nsteps=0):
+def net_input(img5, case, prior=None, nsteps=0, with_conditions=True):
     """The input image (ws.C_IN channels): the 5 field channels, 3 condition
-    planes, 3 geometry planes, the prior's 5 channels and the noise level of
-    an average of nsteps steps (both zero when not given)."""
+    planes (zero without with_conditions), 3 geometry planes, the prior's 5
+    channels and the noise level of an average of nsteps steps (both zero
+    when not given)."""
     ny, nx = case.cfg.ny, case.cfg.nx
     mask = fluid_mask(case)
     x = np.zeros((ny, nx, ws.C_IN), np.float32)
     x[..., :5] = img5
-    x[..., 5:8] = conditions(case)[None, None, :] * mask[..., None]
+    if with_conditions:
+        x[..., 5:8] = conditions(case)[None, None, :] * mask[..., None]
     x[..., 8:11] = geometry_planes(case)
     if prior is not None:
         x[..., 11:16] = prior * mask[..., None]
@@ -218,19 +224,52 @@ def generate(case_list, develop=3000, average=10000, window=WINDOW, mults=WINDOW
             e.close()
 
 
-def _tensors(case_list, per_case, device):
+# channels that change sign when an image is mirrored about y = 0: Uy, the
+# y plane, the prior's Uy
+MIRROR_SIGN_IN = (2, 10, 13)
+MIRROR_SIGN_OUT = (2,)
+
+
+def mirror(img, sign_channels):
+    """The image mirrored about the domain's symmetry line y = 0 (its rows
+    reversed; the listed channels change sign)."""
+    out = img[..., ::-1, :, :].copy() if img.ndim == 4 else img[::-1].copy()
+    for ch in sign_channels:
+        out[..., ch] = -out[..., ch]
+    return out
+
+
+def loss_weights(target, case):
+    """Per-pixel weights of the training loss: the long-average density over
+    the freestream's (clipped to [0.05, 20]), mean 1 over the fluid cells.
+    The judge's relative L2 errors of rho and T weigh each cell by the
+    field's square, so the dense shock layer dominates them; uniform weights
+    in the log variables would spend the net on the sparse wake."""
+    m = fluid_mask(case)
+    w = np.clip(np.exp(target[..., 0]), 0.05, 20.0) * m
+    return (w / max(w.sum() / max(m.sum(), 1.0), 1e-30)).astype(np.float32)
+
+
+def _tensors(case_list, per_case, device, mirrored=False):
     """per_case: for each case (list of input images, list of the field
-    images the residual is taken from, long-average image)."""
+    images the residual is taken from, long-average image); mirrored adds
+    every pair mirrored about y = 0.  The masks carry the loss weights."""
     import torch
     X, Y, M = [], [], []
     for c, (ins, bases, target) in zip(case_list, per_case):
-        mask = fluid_mask(c)
+        fluid = fluid_mask(c)
+        wgt = loss_weights(target, c)
         xs = np.stack(ins)
-        ys = np.zeros((len(ins),) + mask.shape + (ws.C_OUT,), np.float32)
-        ys[..., :5] = (target[None] - np.stack(bases)) * mask[None, ..., None]
+        ys = np.zeros((len(ins),) + fluid.shape + (ws.C_OUT,), np.float32)
+        ys[..., :5] = (target[None] - np.stack(bases)) * fluid[None, ..., None]
+        ms = np.broadcast_to(wgt, (len(ins),) + wgt.shape)
+        if mirrored:
+            xs = np.concatenate([xs, mirror(xs, MIRROR_SIGN_IN)])
+            ys = np.concatenate([ys, mirror(ys, MIRROR_SIGN_OUT)])
+            ms = np.concatenate([ms, np.broadcast_to(wgt[::-1], ms.shape)])
         X.append(torch.as_tensor(xs).to(device).to(torch.bfloat16).view(torch.int16))
         Y.append(torch.as_tensor(ys).to(device))
-        M.append(torch.as_tensor(np.broadcast_to(mask, (len(ins),) + mask.shape).copy()).to(device))
+        M.append(torch.as_tensor(np.array(ms, dtype=np.float32)).to(device))
     return torch.cat(X), torch.cat(Y), torch.cat(M)
 
 
@@ -247,8 +286,9 @@ def denoiser_pairs(case_list, generated, priors, device="cuda"):
     prior (surrogate fields) and the average's length as inputs."""
     per = []
     for c, (wins, t), pr in zip(case_list, generated, priors):
-        per.append(([net_input(img, c, prior=pr, nsteps=n) for n, img in wins], [img for _, img in wins], t))
-    return _tensors(case_list, per, device)
+        per.append(([net_input(img, c, prior=pr, nsteps=n, with_conditions=False) for n, img in wins],
+                    [img for _, img in wins], t))
+    return _tensors(case_list, per, device, mirrored=True)
 
 
 def train_surrogates(case_list, generated, epochs=400, batch=16, lr=2e-3, log=print, device="cuda"):
@@ -273,13 +313,13 @@ def train_surrogates(case_list, generated, epochs=400, batch=16, lr=2e-3, log=pr
 # ---------------------------------------------------------------------------
 # training and prediction
 # ---------------------------------------------------------------------------
-def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda", what="net"):
+def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda", what="net", layers=None):
     """Adam on the masked MSE of the residual (device tensors from
     denoiser_pairs or surrogate_pairs), cosine learning-rate decay; returns
     (net, per-epoch mean losses)."""
     import torch
     from . import ws
-    net = ws.ConvNet(device, seed=seed)
+    net = ws.ConvNet(device, seed=seed, layers=layers)
     n = len(X)
     batch = min(batch, n)
     Xd, Yd, Md = X, Y, M
@@ -304,10 +344,12 @@ def train(X, Y, M, epochs=20, batch=16, lr=2e-3, seed=0, log=print, device="cuda
 
 
 def predict(net, img5, case, device="cuda", prior=None, nsteps=0):
-    """Long-average network variables [g, g, 5] from an input image (and,
-    for the denoiser, its prior and averaged steps)."""
+    """Long-average network variables [g, g, 5] from an input image; with a
+    prior (the denoiser) also from the prior and the averaged steps, without
+    condition planes."""
     import torch
-    x = torch.as_tensor(net_input(img5, case, prior, nsteps)[None]).to(device).to(torch.bfloat16).view(torch.int16)
+    xin = net_input(img5, case, prior, nsteps, with_conditions=prior is None)
+    x = torch.as_tensor(xin[None]).to(device).to(torch.bfloat16).view(torch.int16)
     out = net.forward(x).cpu().numpy()[0]
     return (img5 + out[..., :5]) * fluid_mask(case)[..., None]
 
@@ -428,10 +470,10 @@ def _reseeded(case, offset):
 
 
 def save_nets(nets, path):
-    """Both nets (surrogate, denoiser) in one .npz, with the layer table."""
-    from . import ws
-    arrays = {"layers": np.array(ws.LAYERS, np.int32)}
+    """Both nets (surrogate, denoiser) in one .npz, each with its layer table."""
+    arrays = {}
     for tag, net in zip(("s", "d"), nets):
+        arrays[f"{tag}_layers"] = np.array(net.layers, np.int32)
         st = net.state()
         arrays.update({f"{tag}_w{k}": a for k, a in enumerate(st["w"])})
         arrays.update({f"{tag}_b{k}": a for k, a in enumerate(st["b"])})
@@ -441,16 +483,16 @@ def save_nets(nets, path):
 
 def load_nets(path, device="cuda"):
     """(surrogate, denoiser) from save_nets; refuses a file written for
-    another layer table."""
+    another input or output layout."""
     import torch
-    from . import ws
     d = np.load(path)
-    if "layers" not in d or d["layers"].tolist() != [list(r) for r in ws.LAYERS]:
-        raise ValueError(f"{path}: saved for another network layout")
     nets = []
     for tag in ("s", "d"):
-        net = ws.ConvNet(device)
-        for k in range(len(ws.LAYERS)):
+        if f"{tag}_layers" not in d:
+            raise ValueError(f"{path}: no layer table for the {'surrogate' if tag == 's' else 'denoiser'}")
+        layers = [tuple(int(v) for v in r) for r in d[f"{tag}_layers"]]
+        net = ws.ConvNet(device, layers=layers)
+        for k in range(len(layers)):
             net.w[k].copy_(torch.as_tensor(d[f"{tag}_w{k}"]))
             net.b[k].copy_(torch.as_tensor(d[f"{tag}_b{k}"]))
         net.t = int(d[f"{tag}_t"])
@@ -498,7 +540,8 @@ def main(argv=None):
                                                  batch=a.batch, lr=a.lr, log=log)
     X, Y, M = denoiser_pairs(train_cases, gen[:a.train_cases], priors)
     log(f"[ws] {len(X)} denoiser pairs")
-    denoiser, hist = train(X, Y, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, what="denoiser", seed=1)
+    denoiser, hist = train(X, Y, M, epochs=a.epochs, batch=a.batch, lr=a.lr, log=log, what="denoiser", seed=1,
+                           layers=ws.DENOISER_LAYERS)
     t_train = time.time() - t1
     del X, Y, M
     nets = (surrogate, denoiser)

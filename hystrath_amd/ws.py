@@ -16,9 +16,12 @@ import numpy as np
 
 from . import engine
 
-# (true in, true out, stored in, stored out) channels and the dilation of each layer
+# (true in, true out, stored in, stored out) channels and the dilation of each layer:
+# the surrogate's (a 65-pixel receptive field) and the denoiser's (17 pixels)
 LAYERS = [(17, 32, 32, 32, 1), (32, 64, 32, 64, 2), (64, 64, 64, 64, 4), (64, 64, 64, 64, 8), (64, 32, 64, 32, 16),
           (32, 5, 32, 16, 1)]
+DENOISER_LAYERS = [(17, 32, 32, 32, 1), (32, 64, 32, 64, 1), (64, 64, 64, 64, 2), (64, 64, 64, 64, 2),
+                   (64, 32, 64, 32, 1), (32, 5, 32, 16, 1)]
 N_FIELDS = 5
 N_IN = 17
 C_IN = 32    # stored channels of the input image
@@ -101,12 +104,15 @@ class ConvNet:
     """The network's parameters (FP32 masters, stored-channel shapes) and its
     forward and backward passes on the device."""
 
-    def __init__(self, device="cuda", seed=0):
+    def __init__(self, device="cuda", seed=0, layers=None):
         import torch
+        self.layers = [tuple(r) for r in (layers or LAYERS)]
+        if self.layers[0][2] != C_IN or self.layers[-1][3] != C_OUT or len(self.layers) < 2:
+            raise ValueError("the first layer takes C_IN stored channels and the last gives C_OUT")
         g = torch.Generator().manual_seed(seed)
         self.w, self.b = [], []
-        for k, (ti, to, si, so, _) in enumerate(LAYERS):
-            std = (2.0 / (9 * ti)) ** 0.5 * (0.1 if k == len(LAYERS) - 1 else 1.0)
+        for k, (ti, to, si, so, _) in enumerate(self.layers):
+            std = (2.0 / (9 * ti)) ** 0.5 * (0.1 if k == len(self.layers) - 1 else 1.0)
             wm = torch.zeros(9, si, so)
             wm[:, :ti, :to] = torch.randn(9, ti, to, generator=g) * std
             self.w.append(wm.to(device).contiguous())
@@ -118,7 +124,7 @@ class ConvNet:
 
     def repack(self):
         self.wf, self.wb = [], []
-        for (ti, to, si, so, _), wm in zip(LAYERS, self.w):
+        for (ti, to, si, so, _), wm in zip(self.layers, self.w):
             f, b = pack(wm, si, so)
             self.wf.append(f)
             self.wb.append(b)
@@ -133,8 +139,8 @@ class ConvNet:
         acts = [x_bits]
         a = x_bits
         out = None
-        for k, (ti, to, si, so, dl) in enumerate(LAYERS):
-            last = k == len(LAYERS) - 1
+        for k, (ti, to, si, so, dl) in enumerate(self.layers):
+            last = k == len(self.layers) - 1
             y, y32 = conv3x3(a, self.wf[k], self.b[k], n, h, w, si, so, relu=not last, out_bf16=not last,
                              out_f32=last, dilation=dl)
             if last:
@@ -152,8 +158,8 @@ class ConvNet:
         gw = [torch.zeros_like(t) for t in self.w]
         gb = [torch.zeros_like(t) for t in self.b]
         d = dy_bits
-        for k in range(len(LAYERS) - 1, -1, -1):
-            ti, to, si, so, dl = LAYERS[k]
+        for k in range(len(self.layers) - 1, -1, -1):
+            ti, to, si, so, dl = self.layers[k]
             wgrad(acts[k], d, gw[k], gb[k], n, h, w, si, so, dilation=dl)
             if k > 0:
                 # dL/d(input of layer k) through the ReLU of layer k-1: masked by its output

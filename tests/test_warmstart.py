@@ -26,8 +26,9 @@ def test_pipeline_trains_saves_and_times_both_paths(tmp_path):
     assert hs[-1] < 0.5 * hs[0], hs
     assert len(priors) == 3 and all(p.shape == (64, 64, 5) for p in priors)
     X, Y, M = W.denoiser_pairs(train, gen[:3], priors)
-    assert tuple(X.shape) == (33, 64, 64, 32) and tuple(Y.shape) == (33, 64, 64, 16)
-    denoiser, hd = W.train(X, Y, M, epochs=6, batch=11, lr=2e-3, log=lambda *a: None, seed=1)
+    assert tuple(X.shape) == (66, 64, 64, 32) and tuple(Y.shape) == (66, 64, 64, 16)  # and their mirror images
+    denoiser, hd = W.train(X, Y, M, epochs=6, batch=11, lr=2e-3, log=lambda *a: None, seed=1,
+                           layers=W.ws.DENOISER_LAYERS)
     assert hd[-1] < hd[0], hd
     path = str(tmp_path / "nets.npz")
     W.save_nets((surrogate, denoiser), path)

@@ -66,3 +66,33 @@ def test_net_input_layout_and_residual_targets():
     np.testing.assert_array_equal(xd[..., 11:16][m], prior[m])
     np.testing.assert_allclose(xd[..., 16][m], 0.5)
     assert np.all(xd[..., 17:] == 0.0) and np.all(xd[~m] == 0.0)
+
+
+def test_mirror_maps_the_domain_onto_itself():
+    """The mesh and cylinder are symmetric about y = 0: a mirrored input
+    image is the input image of the mirrored fields (Uy, the y plane and the
+    prior's Uy change sign), so mirrored pairs are valid training pairs."""
+    c = _case()
+    rng = np.random.default_rng(3)
+    m = W.fluid_mask(c)
+    img = rng.normal(size=(64, 64, 5)).astype(np.float32) * m[..., None]
+    prior = rng.normal(size=(64, 64, 5)).astype(np.float32) * m[..., None]
+    np.testing.assert_array_equal(m[::-1], m)
+    x = W.net_input(img, c, prior=prior, nsteps=400, with_conditions=False)
+    xm = W.net_input(W.mirror(img, (2,)), c, prior=W.mirror(prior, (2,)), nsteps=400, with_conditions=False)
+    np.testing.assert_allclose(W.mirror(x, W.MIRROR_SIGN_IN), xm, atol=1e-6)
+    np.testing.assert_array_equal(W.mirror(W.mirror(x, W.MIRROR_SIGN_IN), W.MIRROR_SIGN_IN), x)
+    # the denoiser's input has no condition planes
+    assert np.all(x[..., 5:8] == 0.0)
+
+
+def test_loss_weights_follow_the_density_with_mean_one():
+    c = _case()
+    m = W.fluid_mask(c)
+    target = np.zeros((64, 64, 5), np.float32)
+    target[..., 0] = np.log(np.linspace(0.01, 50.0, 64))[None, :]   # rho / rho_inf across x
+    w = W.loss_weights(target, c)
+    assert abs(w[m > 0].mean() - 1.0) < 1e-5 and np.all(w[m == 0] == 0.0)
+    ratio = w[5, 63] / w[5, 10]
+    want = min(50.0, 20.0) / max(np.linspace(0.01, 50.0, 64)[10], 0.05)
+    assert abs(ratio / want - 1) < 1e-4
