@@ -1688,8 +1688,11 @@ __device__ __forceinline__ void gather_body(const DevParams* __restrict__ Pp, De
                                             const int q) {
     __shared__ GatherLds L;
     const DevParams& P = *Pp;
-    if ((int64_t)q * pt > (int64_t)S->n_sorted) return;
+    // the tile bounds are read beside the population, not after it (q + 1 is
+    // within the tile array, which has capacity / 64 + 4 entries, for every
+    // workgroup of the launch); a tile past the population is not used
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
+    if ((int64_t)q * pt > (int64_t)S->n_sorted) return;
     for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
         gather_cells<GEN>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sub_out);
 }
@@ -1935,9 +1938,13 @@ __device__ __forceinline__ void collide_body(const DevParams* __restrict__ Pp, D
                                              uint16_t* big_list, const int bx) {
     __shared__ TileLds<GEN, PMAX> L;
     const DevParams& P = *Pp;
+    // the tile order is read beside the population (every workgroup of the
+    // launch is within the order array); its entry is used only by a
+    // workgroup inside the population, whose entry the scatter wrote
+    const int q_any = order[bx];
     const int64_t nsort = S->n_sorted;
     if ((int64_t)bx * pt > nsort) return;
-    const int q = order[bx];
+    const int q = q_any;
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
     if (cb >= ce) return;
     const uint32_t step = S->step;
