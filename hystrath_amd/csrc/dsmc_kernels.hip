@@ -1811,37 +1811,14 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         }
         if (t == 0) L.ndense = 0;
         __syncthreads();
-        // sub-cell tables: in a cell of at most 64 parcels each parcel's bit
-        // (its place in the cell) is set in its sub-cell's mask.  A wave takes
-        // 64 consecutive slots: one ballot per sub-cell, then the first lane
-        // of each run of one cell's slots ORs the run's bits into the cell's
-        // eight masks (one LDS atomic per run and sub-cell, not per parcel).
-        // Stamps cleared.
-        for (int w0 = wid * WAVE; w0 < cnt; w0 += CT) {
-            const int j = w0 + lane;
-            const bool in = j < cnt;
-            const int l = in ? (int)L.loc[j] : -1;
-            const int q = in ? (int)L.sub[j] : 8;
-            if (in) L.key[j] = 0x7FFFFFFF;
-            const int lp = __shfl_up(l, 1, WAVE);
-            const bool head = in && (lane == 0 || lp != l);
-            const unsigned long long heads = __ballot(head);
-            unsigned long long mq[8];
-#pragma unroll
-            for (int qq = 0; qq < 8; ++qq) mq[qq] = __ballot(q == qq);
-            if (head && L.segS[l + 1] - L.segS[l] <= 64) {
-                // the run is lanes [lane, nxt): lanes past cnt hold no bits
-                const unsigned long long above = lane == WAVE - 1 ? 0ull : heads & ~((2ull << lane) - 1ull);
-                const int nxt = above ? __ffsll((long long)above) - 1 : WAVE;
-                const int len = nxt - lane;
-                const unsigned long long run = len >= WAVE ? ~0ull : ((1ull << len) - 1ull);
-                const int sh = j - (L.segS[l] - S0);  // the run's first place in its cell
-#pragma unroll
-                for (int qq = 0; qq < 8; ++qq) {
-                    const unsigned long long bits = (mq[qq] >> lane) & run;
-                    if (bits) atomicOr(&L.smask[l][qq], bits << sh);
-                }
-            }
+        // sub-cell tables in parallel: a parcel of a cell of at most 64 sets
+        // its bit in its sub-cell's mask; in a larger cell it finds its place
+        // among the parcels of its sub-cell (ascending index).  Stamps cleared.
+        for (int j = t; j < cnt; j += CT) {
+            const int l = L.loc[j];
+            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
+            if (ee - ss <= 64) atomicOr(&L.smask[l][L.sub[j]], 1ull << (j - ss));
+            L.key[j] = 0x7FFFFFFF;
         }
         // cells of more than 64 parcels: one wave each builds the member
         // lists with ballots over 64-parcel windows
