@@ -1234,9 +1234,12 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
 }
 
 // dsmcVolFields::calculateField per-cell sums for one cell: per species, the
-// parcels of the cell in order (dsmcVolFields.C:1115-1239).  Each cell has
-// one writer; the cumulative sums take non-returning atomic adds (the same
-// IEEE addition as a read-add-write, without waiting for the read).
+// parcels of the cell in order (dsmcVolFields.C:1115-1239), by one thread.
+// The kernel variant with internal energies samples its cells of at most 64
+// parcels this way (four lanes a cell would cost it more register spills
+// than they save).  Each cell has one writer; the cumulative sums take
+// non-returning atomic adds (the same IEEE addition as a read-add-write,
+// without waiting for the read).
 template <bool GEN>
 __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, int nC, const float* vx,
                                             const float* vy, const float* vz, const float* er, const uint16_t* vb,
@@ -1291,6 +1294,91 @@ __device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, i
             }
             const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
             for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
+        }
+    }
+}
+
+// The same sums (the kernel variant without internal energies) for a cell of
+// at most 64 parcels by a group of four consecutive lanes (r = lane mod 4):
+// lane r adds the terms of parcels
+// r, r+4, r+8, ... in order, and the partial sums are combined as
+// (p0 + p2) + (p1 + p3) (every lane of the group ends with that value).  The
+// oracle forms these sums in the same order.  Lane r = 0 adds them, with the
+// cell's collision count and separation, to the cumulative sums; each cell
+// has one writer, and the adds are non-returning atomics (the same IEEE
+// addition as a read-add-write, without waiting for the read).
+__device__ __forceinline__ double quad_sum(double v) {
+    v = v + __shfl_xor(v, 2, WAVE);
+    return v + __shfl_xor(v, 1, WAVE);
+}
+
+template <bool GEN>
+__device__ __forceinline__ void sample_cell_quad(const DevParams& P, int c, int ss, int nC, int r, const float* vx,
+                                                 const float* vy, const float* vz, const float* er,
+                                                 const uint16_t* vb, int vstride, const uint8_t* ty, double* samples,
+                                                 int nc, double ncoll, double sep) {
+    if (r == 0) {
+        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
+        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
+    }
+    const int ns = GEN ? P.ns : 1;
+    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    for (int s = 0; s < ns; ++s) {
+        const double m = P.mass[s];
+        double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
+        double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+        const int nvib = GEN ? P.nvib[s] : 0;
+        for (int k = r; k < nC; k += 4) {
+            const int i = ss + k;
+            if (GEN && (int)ty[i] != s) continue;
+            const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+            aN += 1.0;
+            aX += m * ux;
+            aY += m * uy;
+            aZ += m * uz;
+            aC += m * (ux * ux + uy * uy + uz * uz);
+            if (GEN && P.has_erot) aR += (double)er[i];
+            for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+        }
+        aN = quad_sum(aN);
+        aX = quad_sum(aX);
+        aY = quad_sum(aY);
+        aZ = quad_sum(aZ);
+        aC = quad_sum(aC);
+        if (GEN && P.has_erot) aR = quad_sum(aR);
+        for (int mo = 0; mo < nvib; ++mo) ev[mo] = quad_sum(ev[mo]);
+        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+        if (r == 0) {
+            unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
+            unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
+            if (GEN && P.has_erot) unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
+            for (int mo = 0; mo < nvib; ++mo) unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
+        }
+        if (hf) {
+            double h[HS_HF_COUNT];
+            for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
+            for (int k = r; k < nC; k += 4) {
+                const int i = ss + k;
+                if (GEN && (int)ty[i] != s) continue;
+                const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+                const double lke = m * (ux * ux + uy * uy + uz * uz);
+                const double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
+                double evs = 0.0;
+                for (int mo = 0; mo < nvib; ++mo) evs += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+                const double eint = erv + evs;
+                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            }
+            for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = quad_sum(h[q]);
+            if (r == 0) {
+                const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
+                for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
+            }
         }
     }
 }
@@ -1868,16 +1956,31 @@ __device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0,
         }
         __syncthreads();
         if (do_sample) {
-            // cells of up to 64 parcels: one thread each; larger ones: one wave each
+            // cells of up to 64 parcels: four lanes each (one thread each in the
+            // variant with internal energies); larger ones: one wave each
             const int nbig = L.nbig;
-            if (t >= a && t < b) {
-                const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
-                if (nC <= 64) {
-                    sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
-                                     (double)L.rcoll[t], L.sep[t]);
-                } else {
-                    unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + t], (double)L.rcoll[t]);
-                    unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + t], L.sep[t]);
+            if (GEN) {
+                if (t >= a && t < b) {
+                    const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
+                    if (nC <= 64) {
+                        sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
+                                         (double)L.rcoll[t], L.sep[t]);
+                    } else {
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + t], (double)L.rcoll[t]);
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + t], L.sep[t]);
+                    }
+                }
+            } else {
+                const int r = t & 3;
+                for (int l = a + (t >> 2); l < b; l += CT / 4) {
+                    const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
+                    if (nC <= 64) {
+                        sample_cell_quad<GEN>(P, c0 + l, ss, nC, r, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty,
+                                              samples, nc, (double)L.rcoll[l], L.sep[l]);
+                    } else if (r == 0) {
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
+                    }
                 }
             }
             for (int bi = wid; bi < nbig; bi += CT / WAVE) {

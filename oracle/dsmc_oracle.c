@@ -1193,17 +1193,29 @@ int or_sort(const hs_config* cfg, or_state* s)
 /* ------------------------------------------------------------------------ */
 /* dsmcVolFields::calculateField (dsmcVolFields.C:1115-1239): per species,
  * the parcels of the cell in order */
+/* the sampling kernel's variant: with internal energies or several species
+ * (the engine's "general" kernels) or without */
+static int sums_general(const hs_config* cfg)
+{
+    if (cfg->n_species > 1) return 1;
+    for (int i = 0; i < cfg->n_species; ++i)
+        if (cfg->species[i].rot_dof > 0 || cfg->species[i].n_vib_modes > 0) return 1;
+    return 0;
+}
+
 static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c,
                         int32_t b, int32_t nC, int64_t nc)
 {
     const int ns = cfg->n_species;
     const hs_particles* P = &s->p;
-    /* a cell of more than 64 parcels is summed in the order of the sampling
-     * kernel's wave: lane l adds the terms of parcels l, l+64, ... in order,
-     * then the 64 partial sums are combined pairwise across lane distances
-     * 32, 16, 8, 4, 2, 1.  Smaller cells add their terms in order. */
-    const int wave = nC > 64;
-    const int lanes = wave ? 64 : 1;
+    /* the sums are formed in the order of the sampling kernel: a cell of more
+     * than 64 parcels by a wave, where lane l adds the terms of parcels l,
+     * l+64, ... in order and the 64 partial sums are combined pairwise across
+     * lane distances 32, 16, 8, 4, 2, 1; a smaller cell by four lanes, where
+     * lane r adds parcels r, r+4, ... and the partial sums are combined as
+     * (p0 + p2) + (p1 + p3), except in the kernel variant with internal
+     * energies, where one lane adds its parcels in order. */
+    const int lanes = nC > 64 ? 64 : (sums_general(cfg) ? 1 : 4);
     const int hf = (cfg->flags & HS_FLAG_HEATFLUX) != 0;
     enum { NQ = HS_SS_COUNT + HS_HF_COUNT };
     const int nq = hf ? NQ : HS_SS_COUNT;
@@ -1242,17 +1254,15 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
                 h[6] = lke * ux; h[7] = lke * uy; h[8] = lke * uz;
                 h[9] = eint * ux; h[10] = eint * uy; h[11] = eint * uz; h[12] = eint;
             }
-            const int lane = wave ? (int)(k & 63) : 0;
+            const int lane = (int)(k & (lanes - 1));
             for (int j = 0; j < nq; ++j) part[j][lane] += q[j];
         }
-        if (wave) {
-            for (int o = 32; o > 0; o >>= 1)
-                for (int j = 0; j < nq; ++j) {
-                    double nxt[64];
-                    for (int l = 0; l < 64; ++l) nxt[l] = part[j][l] + part[j][l ^ o];
-                    memcpy(part[j], nxt, sizeof(nxt));
-                }
-        }
+        for (int o = lanes / 2; o > 0; o >>= 1)
+            for (int j = 0; j < nq; ++j) {
+                double nxt[64];
+                for (int l = 0; l < lanes; ++l) nxt[l] = part[j][l] + part[j][l ^ o];
+                memcpy(part[j], nxt, sizeof(double) * (size_t)lanes);
+            }
         const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)sp_id * HS_SS_COUNT;
         for (int k = 0; k < HS_SS_COUNT; ++k) s->samples[(base + k) * nc + c] += part[k][0];
         if (hf) {
