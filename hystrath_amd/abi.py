@@ -89,6 +89,32 @@ class hs_patch(C.Structure):
     ]
 
 
+HS_REACT_DISSOCIATION_QK = 1
+HS_REACT_EXCHANGE_QK = 2
+HS_REACT_DISSOCIATION_EXCHANGE_QK = 3
+HS_MAX_REACTIONS = 32
+
+# reactionModel names of chemReactDict (dsmcReaction.C:106-135)
+REACTION_MODELS = {
+    "dissociationQK": HS_REACT_DISSOCIATION_QK,
+    "exchangeQK": HS_REACT_EXCHANGE_QK,
+    "dissociationExchangeQK": HS_REACT_DISSOCIATION_EXCHANGE_QK,
+}
+
+
+class hs_reaction(C.Structure):
+    _fields_ = [
+        ("model", C.c_int32),
+        ("reactants", C.c_int32 * 2),
+        ("diss_products", (C.c_int32 * 2) * 2),
+        ("exch_products", C.c_int32 * 2),
+        ("heat_exchange", C.c_double),
+        ("a_coeff", C.c_double),
+        ("b_coeff", C.c_double),
+        ("allow_splitting", C.c_int32),
+    ]
+
+
 class hs_config(C.Structure):
     _fields_ = [
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
@@ -118,6 +144,8 @@ class hs_config(C.Structure):
         ("seed", C.c_uint64),
         ("capacity", C.c_int64),
         ("flags", C.c_int32),
+        ("reactions", C.POINTER(hs_reaction)),
+        ("n_reactions", C.c_int32),
     ]
 
 

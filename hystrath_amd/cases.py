@@ -23,7 +23,31 @@ SPECIES = {
     "O2": dict(mass=53.12e-27, diameter=4.07e-10, omega=0.77, alpha=1.4, rot_dof=2.0,
                n_vib_modes=1, theta_v=[2256.0], zref=[17900.0], tref_zv=[2256.0],
                theta_d=59500.0),
+    # heatBath-5species/constant/dsmcProperties
+    "NO": dict(mass=49.81e-27, diameter=4.2e-10, omega=0.79, alpha=1.0, rot_dof=2.0,
+               n_vib_modes=1, theta_v=[2719.0], zref=[1400.0], tref_zv=[2719.0],
+               theta_d=75500.0),
+    "N": dict(mass=23.25e-27, diameter=3.00e-10, omega=0.8, alpha=1.0),
+    "O": dict(mass=26.56e-27, diameter=3.00e-10, omega=0.8, alpha=1.0),
 }
+
+# heatBath-5species/system/chemReactDict, species (N2 O2 NO N O): reaction
+# model, reactants, dissociation products per reactant, and for the exchange
+# models (products, heatOfReactionExchange K, aCoeff, bCoeff)
+HEATBATH_REACTIONS = [
+    ("dissociationQK", ("O2", "N2"), (("O", "O"), ("N", "N")), None),
+    ("dissociationExchangeQK", ("O2", "N"), (("O", "O"), ()), (("NO", "O"), 19693.6, 0.1, 0.1)),
+    ("dissociationQK", ("O2", "O2"), (("O", "O"), ("O", "O")), None),
+    ("dissociationQK", ("O2", "O"), (("O", "O"), ()), None),
+    ("dissociationQK", ("O2", "NO"), (("O", "O"), ("N", "O")), None),
+    ("dissociationQK", ("N2", "N2"), (("N", "N"), ("N", "N")), None),
+    ("dissociationQK", ("N2", "N"), (("N", "N"), ()), None),
+    ("dissociationExchangeQK", ("N2", "O"), (("N", "N"), ()), (("NO", "N"), -37482.0, 0.15, 0.15)),
+    ("dissociationQK", ("N2", "NO"), (("N", "N"), ("N", "O")), None),
+    ("dissociationExchangeQK", ("NO", "N"), (("N", "O"), ()), (("N2", "O"), 37482.0, 0.033, 0.8)),
+    ("dissociationExchangeQK", ("NO", "O"), (("N", "O"), ()), (("O2", "N"), -19693.6, 0.085, 0.65)),
+    ("dissociationQK", ("NO", "NO"), (("N", "O"), ("N", "O")), None),
+]
 
 
 def make_species(name=None, **kw):
@@ -160,6 +184,33 @@ class Case:
         c.flags = flags
         self.cfg = c
         self.species = species
+
+    def set_reactions(self, table, names, allow_splitting=True):
+        """chemReactDict's reactions: entries (reactionModel, reactants,
+        dissociationProducts, exchange) as in HEATBATH_REACTIONS, with
+        exchange = (exchangeProducts, heatOfReactionExchange, aCoeff, bCoeff)
+        or None; names are the typeIdList."""
+        arr = (abi.hs_reaction * max(len(table), 1))()
+        for k, (model, reac, diss, exch) in enumerate(table):
+            r = arr[k]
+            r.model = abi.REACTION_MODELS[model]
+            for i, nm in enumerate(reac):
+                r.reactants[i] = names.index(nm)
+            for i in range(2):
+                prods = diss[i] if i < len(diss) else ()
+                for j in range(2):
+                    r.diss_products[i][j] = names.index(prods[j]) if prods else -1
+            if exch is not None:
+                prods, heat, a, b = exch
+                r.exch_products[0] = names.index(prods[0])
+                r.exch_products[1] = names.index(prods[1])
+                r.heat_exchange, r.a_coeff, r.b_coeff = heat, a, b
+            else:
+                r.exch_products[0] = r.exch_products[1] = -1
+            r.allow_splitting = 1 if allow_splitting else 0
+        self._reactions = arr
+        self.cfg.reactions = arr if table else None
+        self.cfg.n_reactions = len(table)
 
     @property
     def ncells(self):
@@ -324,6 +375,33 @@ def couette_n2o2(seed=11, capacity=200000, flags=abi.HS_FLAG_SAMPLE):
     # system/dsmcInitialiseDict
     case.init_n = [3.2320494711701479e19, 8.591523910705456e18]
     case.init_T = 2000.0
+    return case
+
+
+def heat_bath_5species(fn=100.0, seed=31, capacity=None, flags=abi.HS_FLAG_SAMPLE, n_side=1, allow_splitting=True):
+    """The heatBath-5species tutorial: one (1e-5 m)^3 cell with specular
+    walls (system/blockMeshDict, boundariesDict), LB-VHS with every
+    relaxation collision number 1, nEquivalentParticles 100, typeIdList
+    (N2 O2 NO N O) (constant/dsmcProperties), the twelve QK reactions of
+    system/chemReactDict, deltaT 1e-9 (system/controlDict) and a fill of
+    N2 1.21753030168e22 and O2 3.23647295384e21 m^-3 at 30,000 K in every
+    mode (system/dsmcInitialiseDict).  fn scales the parcel count (the
+    tutorial's is 100, about 154,000 parcels); n_side > 1 cuts the same box
+    into n_side^3 cells."""
+    names = ["N2", "O2", "NO", "N", "O"]
+    sp = [make_species(n) for n in names]
+    wall = patch(abi.HS_PATCH_SPECULAR)
+    L = 1e-5
+    d = L / n_side
+    n_tot = 1.21753030168e22 + 3.23647295384e21
+    cap = capacity or int(1.9 * n_tot * L ** 3 / fn) + 4096
+    case = Case(n_side, n_side, n_side, (0.0, 0.0, 0.0), (d, d, d), sp, [wall] * 6,
+                collision_model=abi.HS_COLL_LB_VHS, fn=fn, dt=1e-9, seed=seed, capacity=cap, flags=flags,
+                z_rot=1.0, z_vib_fixed=1.0, z_el=1.0, name="heatBath-5species")
+    case.set_reactions(HEATBATH_REACTIONS, names, allow_splitting=allow_splitting)
+    case.names = names
+    case.init_n = [1.21753030168e22, 3.23647295384e21, 0.0, 0.0, 0.0]
+    case.init_T = 30000.0
     return case
 
 

@@ -95,6 +95,32 @@ typedef struct hs_patch {
     double velocity[3];                /* dsmcDiffuseWallPatchProperties/velocity    */
 } hs_patch;
 
+/* ---- chemical reactions (dsmcReactions, system/chemReactDict) ----------- */
+/* dsmcReaction::New selects by reactionModel (dsmcReaction.C:106-135); the
+ * quantum-kinetic models of src/lagrangian/dsmc/reactions/derived: */
+enum hs_reaction_model {
+    HS_REACT_DISSOCIATION_QK = 1,          /* dissociationQK (dissociationQK.C:199-600)              */
+    HS_REACT_EXCHANGE_QK = 2,              /* exchangeQK (exchangeQK.C:180-517)                      */
+    HS_REACT_DISSOCIATION_EXCHANGE_QK = 3  /* dissociationExchangeQK (dissociationExchangeQK.C:98-270) */
+};
+#define HS_MAX_REACTIONS 32
+
+/* One entry of chemReactDict's reactions list.  A species' kind follows
+ * dsmcParcelI.H:153-196: charge -1 electron, else atom (no vibrational
+ * mode), diatomic (one) or polyatomic molecule.  At most one reaction per
+ * unordered pair of typeIds (dsmcReactions.C:139-160).  Requires the
+ * VariableHardSphere or LarsenBorgnakkeVariableHardSphere collision model. */
+typedef struct hs_reaction {
+    int32_t model;                 /* hs_reaction_model                              */
+    int32_t reactants[2];          /* reactants (typeIds)                             */
+    int32_t diss_products[2][2];   /* dissociationProducts: two typeIds per reactant,
+                                      -1 -1 for the empty list of an atom            */
+    int32_t exch_products[2];      /* exchangeProducts: a molecule and an atom, either order */
+    double heat_exchange;          /* heatOfReactionExchange (K; positive: exothermic) */
+    double a_coeff, b_coeff;       /* aCoeff, bCoeff of the exchange                  */
+    int32_t allow_splitting;       /* allowSplitting (the reference's default is yes) */
+} hs_reaction;
+
 typedef struct hs_config {
     /* ---- mesh ---- */
     int32_t nx, ny, nz;
@@ -132,6 +158,10 @@ typedef struct hs_config {
     uint64_t seed;                     /* seedNumber                            */
     int64_t capacity;                  /* max simulator particles               */
     int32_t flags;                     /* HS_FLAG_*                             */
+
+    /* ---- chemReactDict (host, borrowed at create) ---- */
+    const hs_reaction* reactions;      /* n_reactions entries, or NULL          */
+    int32_t n_reactions;               /* 0: no chemistry                       */
 } hs_config;
 
 /* Per-particle state, structure of arrays.  erot is used when any species has

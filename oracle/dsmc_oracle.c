@@ -288,8 +288,23 @@ static int rng_label(or_rng* r, int a, int b)
 /* ------------------------------------------------------------------------ */
 /* derived constants                                                        */
 /* ------------------------------------------------------------------------ */
+/* one reaction of chemReactDict with its constants (see derive) */
+typedef struct {
+    int model, r[2], dprod[2][2];
+    int xmol, xatom;       /* exchange products: the molecule, the atom          */
+    int pos_mol;           /* exchangeQK::posMolReactant_ (exchangeQK.C:52-60)   */
+    int split;             /* allowSplitting                                      */
+    double heat_diss[2];   /* k thetaD of each reactant (dissociationQK.C:57-60)  */
+    double heat_x;         /* heatOfReactionExchange k (exchangeQK.C:387-391)     */
+    double a_dash, b;      /* Bird's QK exchange constants (exchangeQK.C:195-200) */
+    double om_x;           /* omega of the exchange's redistribution (exchangeQK.C:303-308) */
+} or_rx;
+
 typedef struct {
     int ns;
+    int n_react;
+    int react_of[HS_MAX_SPECIES][HS_MAX_SPECIES];  /* pairAddressing_, -1 none */
+    or_rx rx[HS_MAX_REACTIONS];
     double sig_coef[HS_MAX_SPECIES][HS_MAX_SPECIES];
     double ref_term[HS_MAX_SPECIES][HS_MAX_SPECIES];
     double omega_pq[HS_MAX_SPECIES][HS_MAX_SPECIES];
@@ -302,6 +317,67 @@ typedef struct {
     int any_rot, any_vib, multi;
     int max_modes;
 } or_derived;
+
+/* a species' kind (dsmcParcelI.H:153-196): 0 electron, 10/11 atom,
+ * 20/21 diatomic, 30/31 polyatomic molecule (neutral / ion) */
+static int species_kind(const hs_species* sp)
+{
+    if (sp->charge == -1) return 0;
+    const int base = sp->n_vib_modes == 0 ? 10 : (sp->n_vib_modes == 1 ? 20 : 30);
+    return base + (sp->charge == 1 ? 1 : 0);
+}
+
+/* the reactions' constants and the pair table of dsmcReactions::
+ * initialConfiguration (dsmcReactions.C:139-160) with each model's
+ * tryReactMolecules (dissociationQK.C:430-455, exchangeQK.C:420-443) */
+static void derive_reactions(const hs_config* cfg, or_derived* d)
+{
+    for (int p = 0; p < HS_MAX_SPECIES; ++p)
+        for (int q = 0; q < HS_MAX_SPECIES; ++q) d->react_of[p][q] = -1;
+    d->n_react = (cfg->reactions && cfg->n_reactions > 0) ? cfg->n_reactions : 0;
+    if (d->n_react > HS_MAX_REACTIONS) d->n_react = HS_MAX_REACTIONS;
+    for (int i = 0; i < d->n_react; ++i) {
+        const hs_reaction* R = &cfg->reactions[i];
+        or_rx* x = &d->rx[i];
+        memset(x, 0, sizeof(*x));
+        x->model = R->model;
+        x->split = R->allow_splitting != 0;
+        for (int k = 0; k < 2; ++k) {
+            x->r[k] = R->reactants[k];
+            x->dprod[k][0] = R->diss_products[k][0];
+            x->dprod[k][1] = R->diss_products[k][1];
+            x->heat_diss[k] = HS_KB * cfg->species[R->reactants[k]].theta_d;
+        }
+        x->pos_mol = -1;
+        x->xmol = x->xatom = -1;
+        if (R->model != HS_REACT_DISSOCIATION_QK) {
+            for (int k = 0; k < 2; ++k)
+                if (species_kind(&cfg->species[R->reactants[k]]) >= 20) x->pos_mol = k;
+            for (int k = 0; k < 2; ++k) {
+                const int pid = R->exch_products[k];
+                const int kind = species_kind(&cfg->species[pid]);
+                if (kind >= 20) x->xmol = pid;
+                else if (kind == 10 || kind == 11) x->xatom = pid;
+            }
+            x->heat_x = R->heat_exchange * HS_KB;
+            x->b = R->b_coeff;
+            const double om = 0.5 * (cfg->species[R->reactants[0]].omega + cfg->species[R->reactants[1]].omega);
+            const double chib = 2.5 - om;
+            x->a_dash = R->a_coeff * (pow(chib, R->b_coeff) * exp(lgamma(chib)) / exp(lgamma(chib + R->b_coeff)));
+            if (x->xatom >= 0) x->om_x = 0.5 * (cfg->species[x->xatom].omega + cfg->species[x->xatom].omega);
+        }
+        for (int p = 0; p < cfg->n_species; ++p)
+            for (int q = p; q < cfg->n_species; ++q) {
+                const int pid = p == x->r[0] ? 0 : (p == x->r[1] ? 1 : -1);
+                const int qid = q == x->r[0] ? 0 : (q == x->r[1] ? 1 : -1);
+                if (pid < 0 || qid < 0) continue;
+                int ok;
+                if (R->model == HS_REACT_EXCHANGE_QK) ok = pid != qid;
+                else ok = (pid == qid && x->r[0] == x->r[1]) || (pid != qid && x->r[0] != x->r[1]);
+                if (ok) { d->react_of[p][q] = i; d->react_of[q][p] = i; }
+            }
+    }
+}
 
 static void derive(const hs_config* cfg, or_derived* d)
 {
@@ -328,6 +404,7 @@ static void derive(const hs_config* cfg, or_derived* d)
         }
     }
     d->multi = d->ns > 1;
+    derive_reactions(cfg, d);
     const int nn[3] = { cfg->nx, cfg->ny, cfg->nz };
     for (int a = 0; a < 3; ++a) {
         d->empty[a] = cfg->patch[2 * a].model == HS_PATCH_EMPTY;
@@ -468,14 +545,24 @@ static void post_coll_vel(const hs_config* cfg, const or_derived* d, int vss, in
     }
 }
 
-/* dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531),
- * postReaction = false */
+/* dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531);
+ * post = postReaction: the level is always redrawn, with no Zv test */
 static int vib_level(const hs_config* cfg, const or_derived* d, int t, int partner, int mode,
-                     int level, int imax, double ec, double om, or_rng* r)
+                     int level, int imax, double ec, double om, int post, or_rng* r)
 {
     const hs_species* sp = &cfg->species[t];
     const double theta_v = sp->theta_v[mode];
     double inv_zv = 1.0;
+    if (post) {
+        int idash, guard = 0;
+        double func;
+        do {
+            idash = rng_label(r, 0, imax);
+            const double evib = (double)idash * HS_KB * theta_v;
+            func = or_pow(1.0 - evib / ec, 1.5 - om);
+        } while (func < rng_u(r) && ++guard < OR_MAX_REDRAW);
+        return idash;
+    }
     if (cfg->z_vib_fixed == 0.0) {
         const double T = (double)imax * theta_v / (3.5 - om);
         const double pow1 = or_pow(sp->theta_d / T, 1.0 / 3.0) - 1.0;
@@ -517,9 +604,9 @@ static double rot_ratio(double dof, double chib, or_rng* r)
 }
 
 /* LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
- * species with a single electronic level */
+ * species with a single electronic level; post = postReaction */
 static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, int partner,
-                            double* et, double* erot, int32_t* vib, double om, or_rng* r)
+                            double* et, double* erot, int32_t* vib, double om, int post, or_rng* r)
 {
     const hs_species* sp = &cfg->species[t];
     if (sp->charge == -1) return;
@@ -532,7 +619,7 @@ static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, in
             const double ec = *et + pre[m];
             const int imax = (int)(ec / (HS_KB * sp->theta_v[m]));
             if (imax > 0) {
-                vib[m] = vib_level(cfg, d, t, partner, m, vib[m], imax, ec, om, r);
+                vib[m] = vib_level(cfg, d, t, partner, m, vib[m], imax, ec, om, post, r);
                 *et = ec - (double)vib[m] * HS_KB * sp->theta_v[m];
             }
         }
@@ -565,10 +652,270 @@ static void binary_collide(const hs_config* cfg, const or_derived* d, int tp, in
     const double cr2 = dx * dx + dy * dy + dz * dz;
     double et = 0.5 * mr * cr2;
     const double om = d->omega_pq[tp][tq];
-    lb_redistribute(cfg, d, tp, tq, &et, erp, vp, om, r);
-    lb_redistribute(cfg, d, tq, tp, &et, erq, vq, om, r);
+    lb_redistribute(cfg, d, tp, tq, &et, erp, vp, om, 0, r);
+    lb_redistribute(cfg, d, tq, tp, &et, erq, vq, om, 0, r);
     const double cr = sqrt(2.0 * et / mr);
     post_coll_vel(cfg, d, model == HS_COLL_LB_VSS, 1, tp, tq, up, uq, cr, r);
+}
+
+/* ------------------------------------------------------------------------ */
+/* quantum-kinetic chemistry (src/lagrangian/dsmc/reactions)                 */
+/* ------------------------------------------------------------------------ */
+/* one parcel's state during a reaction */
+typedef struct {
+    int t;
+    double u[3];
+    double er;
+    int32_t v[HS_MAX_VIB_MODES];
+} or_pstate;
+
+static int is_lb(const hs_config* cfg)
+{
+    return cfg->collision_model == HS_COLL_LB_VHS || cfg->collision_model == HS_COLL_LB_VSS;
+}
+
+/* eVib_tot (dsmcParcelI.H:389-398) */
+static double evib_total(const hs_config* cfg, const or_pstate* a)
+{
+    const hs_species* sp = &cfg->species[a->t];
+    double e = 0.0;
+    for (int m = 0; m < sp->n_vib_modes; ++m) e += (double)a->v[m] * HS_KB * sp->theta_v[m];
+    return e;
+}
+
+static void become(or_pstate* a, int t)
+{
+    a->t = t;
+    a->er = 0.0;
+    for (int m = 0; m < HS_MAX_VIB_MODES; ++m) a->v[m] = 0;
+}
+
+/* dissociationQK::testDissociation (dissociationQK.C:199-238) */
+static void test_dissociation(const hs_config* cfg, const or_pstate* a, double et, int* mode, double* ec,
+                              double* total, double* prob)
+{
+    const hs_species* sp = &cfg->species[a->t];
+    const int kind = species_kind(sp);
+    if (kind != 20 && kind != 30) return;
+    for (int m = 0; m < sp->n_vib_modes; ++m) {
+        const double theta_v = sp->theta_v[m];
+        const double ev = (double)a->v[m] * HS_KB * theta_v;
+        const int id = (int)(sp->theta_d / theta_v);
+        *ec = et + ev;
+        const int imax = (int)(*ec / (HS_KB * theta_v));
+        if (imax > id) {
+            *prob = 1.0;
+            *total += *prob;
+            *mode = m;
+            break;
+        }
+    }
+}
+
+/* exchangeQK::testExchange (exchangeQK.C:180-267); a is the molecule */
+static void test_exchange(const hs_config* cfg, const or_rx* x, const or_pstate* a, double et, double om,
+                          double* ec, double* total, double* prob)
+{
+    const hs_species* sp = &cfg->species[a->t];
+    const double chib = 2.5 - om;
+    const double tcoll = et / (HS_KB * chib);
+    double ea = x->a_dash * or_pow(tcoll / 273.0, x->b) * fabs(x->heat_x);
+    double sum = 1.0;
+    if (x->heat_x < 0.0) ea -= x->heat_x;
+    int m = 0;
+    do {
+        const double kth = HS_KB * sp->theta_v[m];
+        const double ev = (double)a->v[m] * HS_KB * sp->theta_v[m];
+        *ec = et + ev;
+        if (*ec > ea) {
+            if (ea > kth) {
+                sum = 0.0;
+                const int ia = (int)(*ec / kth);
+                for (int i = 0; i <= ia; ++i)
+                    sum += or_pow(1.0 - (double)i * HS_KB * sp->theta_v[m] / *ec, 1.5 - om);
+            }
+            *prob = or_pow(1.0 - ea / *ec, 1.5 - om) / sum;
+            m = sp->n_vib_modes;
+        }
+        m += 1;
+    } while (m < sp->n_vib_modes);
+    *total += *prob;
+}
+
+/* VariableHardSphere::postReactionVelocities (VariableHardSphere.C:231-262):
+ * ua enters as the pair's centre-of-mass velocity */
+static void post_reaction_vel(const hs_config* cfg, int ta, int tb, double ua[3], double ub[3], double cr,
+                              or_rng* r)
+{
+    const double ma = cfg->species[ta].mass, mb = cfg->species[tb].mass;
+    const double cos_t = 2.0 * rng_u(r) - 1.0;
+    const double sin_t = sqrt(1.0 - cos_t * cos_t);
+    double sp, cp;
+    or_sincos2pi(rng_u(r), &sp, &cp);
+    const double rel[3] = { cr * cos_t, cr * (sin_t * cp), cr * (sin_t * sp) };
+    for (int c = 0; c < 3; ++c) {
+        ub[c] = ua[c] - rel[c] * ma / (ma + mb);
+        ua[c] = ua[c] + rel[c] * mb / (ma + mb);
+    }
+}
+
+/* dissociationQK::dissociateParticleByPartner (dissociationQK.C:240-378): a
+ * (reactant nr) dissociates in vibrational mode `mode`, b is the partner;
+ * a becomes the first product and *prod the second.  Returns 1 when the
+ * split happens (allowSplitting) */
+static int dissociate(const hs_config* cfg, const or_derived* d, const or_rx* x, or_pstate* a, or_pstate* b,
+                      int nr, int mode, double ec, or_rng* r, or_pstate* prod)
+{
+    if (!x->split) return 0;
+    ec -= x->heat_diss[nr];
+    const double om = d->omega_pq[a->t][b->t];
+    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &ec, &b->er, b->v, om, 1, r);
+    const double ma = cfg->species[a->t].mass, mb = cfg->species[b->t].mass;
+    const double mr = ma * mb / (ma + mb);
+    const double cr = sqrt(2.0 * ec / mr);
+    post_coll_vel(cfg, d, 0, 0, a->t, b->t, a->u, b->u, cr, r);
+    const int t1 = x->dprod[nr][0], t2 = x->dprod[nr][1];
+    const double m1 = cfg->species[t1].mass, m2 = cfg->species[t2].mass;
+    const double mrp = m1 * m2 / (m1 + m2);
+    const double evt = evib_total(cfg, a);
+    const double evm = (double)a->v[mode] * HS_KB * cfg->species[a->t].theta_v[mode];
+    const double evn = evt - evm;
+    const double left = a->er + evn + 0.0;  /* + the single electronic level's energy, 0 */
+    const double crp = sqrt(2.0 * left / mrp);
+    become(prod, t2);
+    post_reaction_vel(cfg, t1, t2, a->u, prod->u, crp, r);
+    become(a, t1);
+    return 1;
+}
+
+/* exchangeQK::exchange (exchangeQK.C:269-370): a is the molecule, b the atom */
+static int exchange(const hs_config* cfg, const or_derived* d, const or_rx* x, or_pstate* a, or_pstate* b,
+                    or_rng* r)
+{
+    if (!x->split) return 0;
+    const double ma = cfg->species[a->t].mass, mb = cfg->species[b->t].mass;
+    const double mr = ma * mb / (ma + mb);
+    const double dx = a->u[0] - b->u[0], dy = a->u[1] - b->u[1], dz = a->u[2] - b->u[2];
+    double et = 0.5 * mr * (dx * dx + dy * dy + dz * dz);
+    double ucm[3];
+    for (int c = 0; c < 3; ++c) ucm[c] = (ma * a->u[c] + mb * b->u[c]) / (ma + mb);
+    const double mpx = cfg->species[x->xatom].mass, mqx = cfg->species[x->xmol].mass;
+    const double mrx = mpx * mqx / (mpx + mqx);
+    const double evt = evib_total(cfg, a);
+    et += a->er + evt + 0.0 + 0.0 + x->heat_x;  /* + both electronic energies, 0 */
+    become(a, x->xatom);
+    become(b, x->xmol);
+    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &et, &b->er, b->v, x->om_x, 1, r);
+    const double crx = sqrt(2.0 * et / mrx);
+    for (int c = 0; c < 3; ++c) a->u[c] = ucm[c];
+    post_reaction_vel(cfg, x->xatom, x->xmol, a->u, b->u, crx, r);
+    return 1;
+}
+
+/* dsmcReaction::decreasing_sort_indices (dsmcReaction.C:176-205): n draws,
+ * then the indices by probability, equal probabilities by their draw, both
+ * decreasing (insertion sort: the order is unique unless two draws tie) */
+static void decreasing_order(const double* v, int n, or_rng* r, int* idx)
+{
+    double rnd[3];
+    for (int i = 0; i < n; ++i) {
+        rnd[i] = rng_u(r);
+        idx[i] = i;
+    }
+    for (int i = 1; i < n; ++i) {
+        const int y = idx[i];
+        int j = i - 1;
+        while (j >= 0 && (v[y] > v[idx[j]] || (v[y] == v[idx[j]] && rnd[y] > rnd[idx[j]]))) {
+            idx[j + 1] = idx[j];
+            --j;
+        }
+        idx[j + 1] = y;
+    }
+}
+
+/* kinetic energy of the relative motion, 1/2 mr |ua - ub|^2 */
+static double rel_energy(const hs_config* cfg, const or_pstate* a, const or_pstate* b)
+{
+    const double ma = cfg->species[a->t].mass, mb = cfg->species[b->t].mass;
+    const double mr = ma * mb / (ma + mb);
+    const double dx = a->u[0] - b->u[0], dy = a->u[1] - b->u[1], dz = a->u[2] - b->u[2];
+    return 0.5 * mr * (dx * dx + dy * dy + dz * dz);
+}
+
+/* The reaction of an accepted pair with a reaction model (noTimeCounter.C:
+ * 250-300): returns 1 if it reacted (relax_ is off and no binary collision
+ * follows).  A dissociation sets *prod to the new parcel and *from_q to
+ * whether Q (1) or P (0) dissociated.  p is the candidate P, q the
+ * candidate Q; the models swap them when P is not their first reactant. */
+static int react_pair(const hs_config* cfg, const or_derived* d, int ri, or_pstate* P, or_pstate* Q, or_rng* r,
+                      or_pstate* prod, int* from_q)
+{
+    const or_rx* x = &d->rx[ri];
+    or_pstate *p = P, *q = Q;
+    prod->t = -1;
+    if (x->model == HS_REACT_EXCHANGE_QK) {
+        /* exchangeQK::reaction (exchangeQK.C:457-517) */
+        const int kp = species_kind(&cfg->species[p->t]);
+        if (kp == 10 || kp == 11) { p = Q; q = P; }
+        const double om = d->omega_pq[p->t][q->t];
+        const double et = rel_energy(cfg, p, q);
+        double total = 0.0, prob = 0.0, ec = 0.0;
+        test_exchange(cfg, x, p, et, om, &ec, &total, &prob);
+        if (total > rng_u(r)) return exchange(cfg, d, x, p, q, r);
+        return 0;
+    }
+    /* dissociationQK::reaction (dissociationQK.C:469-600) and
+     * dissociationExchangeQK::reaction (dissociationExchangeQK.C:98-270) */
+    if (p->t != x->r[0]) { p = Q; q = P; }
+    const int q_is_Q = q == Q;
+    const int three = x->model == HS_REACT_DISSOCIATION_EXCHANGE_QK;
+    const double om = d->omega_pq[p->t][q->t];
+    const double et = rel_energy(cfg, p, q);
+    double total = 0.0, prob[3] = { 0.0, 0.0, 0.0 }, ec[3] = { 0.0, 0.0, 0.0 };
+    int mode_p = -1, mode_q = -1;
+    test_dissociation(cfg, p, et, &mode_p, &ec[0], &total, &prob[0]);
+    if (three || p->t != q->t) test_dissociation(cfg, q, et, &mode_q, &ec[1], &total, &prob[1]);
+    if (three) {
+        if (x->pos_mol == 0) test_exchange(cfg, x, p, et, om, &ec[2], &total, &prob[2]);
+        else test_exchange(cfg, x, q, et, om, &ec[2], &total, &prob[2]);
+    }
+    if (!(total > rng_u(r))) return 0;
+    const int n = three ? 3 : 2;
+    double np[3];
+    int idx[3];
+    for (int i = 0; i < n; ++i) np[i] = prob[i] / total;
+    decreasing_order(np, n, r, idx);
+    double cum = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int i = idx[k];
+        if (!(np[i] > OR_SMALL)) break;
+        cum += np[i];
+        if (cum > rng_u(r)) {
+            if (i == 0) {
+                *from_q = !q_is_Q;
+                return dissociate(cfg, d, x, p, q, 0, mode_p, ec[0], r, prod);
+            }
+            if (i == 1) {
+                *from_q = q_is_Q;
+                return dissociate(cfg, d, x, q, p, 1, mode_q, ec[1], r, prod);
+            }
+            return x->pos_mol == 0 ? exchange(cfg, d, x, p, q, r) : exchange(cfg, d, x, q, p, r);
+        }
+    }
+    return 0;
+}
+
+/* The accepted collision of one pair (noTimeCounter.C:250-300): its reaction
+ * when the pair has a model, else (or when it did not react) the binary
+ * collision.  Returns 1 if it reacted. */
+static int collide_or_react(const hs_config* cfg, const or_derived* d, or_pstate* P, or_pstate* Q, or_rng* r,
+                            or_pstate* prod, int* from_q)
+{
+    prod->t = -1;
+    const int ri = d->n_react ? d->react_of[P->t][Q->t] : -1;
+    if (ri >= 0 && react_pair(cfg, d, ri, P, Q, r, prod, from_q)) return 1;
+    binary_collide(cfg, d, P->t, Q->t, P->u, Q->u, &P->er, &Q->er, P->v, Q->v, r);
+    return 0;
 }
 
 int or_sigma_tcr(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
@@ -602,6 +949,50 @@ int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const 
         if (erot_p) erot_p[k] = erp;
         if (erot_q) erot_q[k] = erq;
         if (used_out) used_out[k] = r.inj_used;
+        if (r.inj_used > n_uniform) rc = HS_E_OVERFLOW;
+    }
+    return rc;
+}
+
+int or_react_pairs(const hs_config* cfg, int64_t n, int32_t* tp, int32_t* tq, double* up, double* uq,
+                   double* erot_p, double* erot_q, int32_t* vib_p, int32_t* vib_q, const double* uniforms,
+                   int32_t n_uniform, int32_t* used_out, int32_t* reacted_out, int32_t* prod_type,
+                   double* prod_u, int32_t* prod_from_q)
+{
+    or_derived d;
+    derive(cfg, &d);
+    int rc = 0;
+    for (int64_t k = 0; k < n; ++k) {
+        or_rng r;
+        rng_init(&r, 0, 0, 0, 0, 0);
+        r.inj = uniforms + k * n_uniform;
+        r.inj_n = n_uniform;
+        or_pstate a = { tp[k], { up[3 * k], up[3 * k + 1], up[3 * k + 2] }, erot_p[k], { 0 } };
+        or_pstate b = { tq[k], { uq[3 * k], uq[3 * k + 1], uq[3 * k + 2] }, erot_q[k], { 0 } };
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) {
+            a.v[m] = vib_p[k * HS_MAX_VIB_MODES + m];
+            b.v[m] = vib_q[k * HS_MAX_VIB_MODES + m];
+        }
+        or_pstate prod;
+        int from_q = 0;
+        const int reacted = collide_or_react(cfg, &d, &a, &b, &r, &prod, &from_q);
+        tp[k] = a.t;
+        tq[k] = b.t;
+        for (int c = 0; c < 3; ++c) {
+            up[3 * k + c] = a.u[c];
+            uq[3 * k + c] = b.u[c];
+            prod_u[3 * k + c] = prod.t >= 0 ? prod.u[c] : 0.0;
+        }
+        erot_p[k] = a.er;
+        erot_q[k] = b.er;
+        for (int m = 0; m < HS_MAX_VIB_MODES; ++m) {
+            vib_p[k * HS_MAX_VIB_MODES + m] = a.v[m];
+            vib_q[k * HS_MAX_VIB_MODES + m] = b.v[m];
+        }
+        reacted_out[k] = reacted;
+        prod_type[k] = prod.t;
+        prod_from_q[k] = prod.t >= 0 ? from_q : -1;
+        used_out[k] = r.inj_used;
         if (r.inj_used > n_uniform) rc = HS_E_OVERFLOW;
     }
     return rc;
@@ -1204,7 +1595,7 @@ static int sums_general(const hs_config* cfg)
 }
 
 static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c,
-                        int32_t b, int32_t nC, int64_t nc)
+                        int32_t b, int32_t nC, int64_t b2, int32_t n2, int64_t nc)
 {
     const int ns = cfg->n_species;
     const hs_particles* P = &s->p;
@@ -1214,8 +1605,10 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
      * lane distances 32, 16, 8, 4, 2, 1; a smaller cell by four lanes, where
      * lane r adds parcels r, r+4, ... and the partial sums are combined as
      * (p0 + p2) + (p1 + p3), except in the kernel variant with internal
-     * energies, where one lane adds its parcels in order. */
-    const int lanes = nC > 64 ? 64 : (sums_general(cfg) ? 1 : 4);
+     * energies, where one lane adds its parcels in order.  With reactions one
+     * lane adds the cell's parcels and then the products it made this step
+     * (b2 .. b2+n2-1), in order. */
+    const int lanes = d->n_react ? 1 : (nC > 64 ? 64 : (sums_general(cfg) ? 1 : 4));
     const int hf = (cfg->flags & HS_FLAG_HEATFLUX) != 0;
     enum { NQ = HS_SS_COUNT + HS_HF_COUNT };
     const int nq = hf ? NQ : HS_SS_COUNT;
@@ -1225,8 +1618,8 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
         double part[NQ][64];
         for (int j = 0; j < nq; ++j)
             for (int l = 0; l < lanes; ++l) part[j][l] = 0.0;
-        for (int32_t k = 0; k < nC; ++k) {
-            const int64_t i = b + k;
+        for (int32_t k = 0; k < nC + n2; ++k) {
+            const int64_t i = k < nC ? b + k : b2 + (k - nC);
             const int t = P->type ? P->type[i] : 0;
             if (t != sp_id) continue;
             double q[NQ];
@@ -1270,7 +1663,6 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
             for (int k = 0; k < HS_HF_COUNT; ++k) s->samples[(hb + k) * nc + c] += part[HS_SS_COUNT + k][0];
         }
     }
-    (void)d;
 }
 
 /* octant sub-cells of one cell (noTimeCounter.C:114-138): parcel k is in
@@ -1341,6 +1733,43 @@ int or_ntc_pairs(const hs_config* cfg, int64_t cell, int32_t nC, const double* p
     return 0;
 }
 
+/* writes a parcel's state back after its collision; after a reaction its
+ * type and every vibrational plane are rewritten too */
+static void store_state(const hs_config* cfg, const or_derived* d, or_state* s, int64_t i, const or_pstate* a,
+                        int reacted)
+{
+    hs_particles* P = &s->p;
+    P->ux[i] = (float)a->u[0]; P->uy[i] = (float)a->u[1]; P->uz[i] = (float)a->u[2];
+    if (P->erot) P->erot[i] = (float)a->er;
+    const int nm = reacted ? d->max_modes : cfg->species[a->t].n_vib_modes;
+    for (int m = 0; m < nm; ++m) P->vib[m * s->capacity + i] = (uint16_t)a->v[m];
+    if (reacted && P->type) P->type[i] = (uint8_t)a->t;
+}
+
+/* dsmcCloud::addNewParcel (dsmcCloud.C:396-432) for a dissociation product:
+ * at the dissociating parcel's position and cell, with no internal energy.
+ * Products go after the sorted population in the order they are made (cell
+ * by cell, candidate by candidate); one beyond the capacity is dropped and
+ * counted in counters[7] */
+static void append_product(const hs_config* cfg, const or_derived* d, or_state* s, int64_t parent, int32_t c,
+                           const or_pstate* prod, int64_t cnt[3])
+{
+    (void)cfg;
+    (void)cnt;
+    if (s->n >= s->capacity) {
+        s->counters[7] += 1;
+        return;
+    }
+    hs_particles* P = &s->p;
+    const int64_t j = s->n++;
+    P->px[j] = P->px[parent]; P->py[j] = P->py[parent]; P->pz[j] = P->pz[parent];
+    P->ux[j] = (float)prod->u[0]; P->uy[j] = (float)prod->u[1]; P->uz[j] = (float)prod->u[2];
+    P->cell[j] = c;
+    if (P->erot) P->erot[j] = 0.0f;
+    for (int m = 0; m < d->max_modes; ++m) P->vib[m * s->capacity + j] = 0;
+    if (P->type) P->type[j] = (uint8_t)prod->t;
+}
+
 /* one cell of or_collide; cnt gets candidates, collisions and clamps */
 static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s, int64_t c, int32_t sample,
                          int64_t nc, int active, int32_t* which, int32_t* sub_list, int64_t cnt[3])
@@ -1348,6 +1777,7 @@ static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s,
     hs_particles* P = &s->p;
     const int32_t b = s->start[c];
     const int32_t nC = s->start[c + 1] - b;
+    const int64_t n_before = s->n;  /* this cell's products start here */
     double ncoll = 0.0, sep = 0.0;
     if (active && nC > 1) {
         const int ci = (int)(c % cfg->nx), cj = (int)((c / cfg->nx) % cfg->ny), ck = (int)(c / ((int64_t)cfg->nx * cfg->ny));
@@ -1378,16 +1808,16 @@ static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s,
             const double stcr = sigma_tcr(d, tp, tq, up, uq);
             if (stcr > s->sigma_tcr_max[c]) s->sigma_tcr_max[c] = stcr;
             if (stcr / smax > rng_u(&r)) {
-                double erp = P->erot ? P->erot[ip] : 0.0, erq = P->erot ? P->erot[iq] : 0.0;
-                int32_t vp[HS_MAX_VIB_MODES] = { 0 }, vq[HS_MAX_VIB_MODES] = { 0 };
-                for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) vp[m] = P->vib[m * s->capacity + ip];
-                for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) vq[m] = P->vib[m * s->capacity + iq];
-                binary_collide(cfg, d, tp, tq, up, uq, &erp, &erq, vp, vq, &r);
-                P->ux[ip] = (float)up[0]; P->uy[ip] = (float)up[1]; P->uz[ip] = (float)up[2];
-                P->ux[iq] = (float)uq[0]; P->uy[iq] = (float)uq[1]; P->uz[iq] = (float)uq[2];
-                if (P->erot) { P->erot[ip] = (float)erp; P->erot[iq] = (float)erq; }
-                for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) P->vib[m * s->capacity + ip] = (uint16_t)vp[m];
-                for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) P->vib[m * s->capacity + iq] = (uint16_t)vq[m];
+                or_pstate a = { tp, { up[0], up[1], up[2] }, P->erot ? P->erot[ip] : 0.0, { 0 } };
+                or_pstate b = { tq, { uq[0], uq[1], uq[2] }, P->erot ? P->erot[iq] : 0.0, { 0 } };
+                for (int m = 0; m < cfg->species[tp].n_vib_modes; ++m) a.v[m] = P->vib[m * s->capacity + ip];
+                for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) b.v[m] = P->vib[m * s->capacity + iq];
+                or_pstate prod;
+                int from_q = 0;
+                const int reacted = collide_or_react(cfg, d, &a, &b, &r, &prod, &from_q);
+                store_state(cfg, d, s, ip, &a, reacted);
+                store_state(cfg, d, s, iq, &b, reacted);
+                if (prod.t >= 0) append_product(cfg, d, s, from_q ? iq : ip, (int32_t)c, &prod, cnt);
                 ncoll += 1.0;
                 if (cfg->flags & HS_FLAG_COLL_SEP) {
                     const double ddx = P->px[ip] - P->px[iq], ddy = P->py[ip] - P->py[iq], ddz = P->pz[ip] - P->pz[iq];
@@ -1398,7 +1828,7 @@ static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s,
         }
     }
     if (sample && s->samples) {
-        sample_cell(cfg, d, s, c, b, nC, nc);
+        sample_cell(cfg, d, s, c, b, nC, n_before, (int32_t)(s->n - n_before), nc);
         s->samples[(int64_t)HS_S_NCOLL * nc + c] += ncoll;
         s->samples[(int64_t)HS_S_COLLSEP * nc + c] += sep;
     }
@@ -1413,8 +1843,10 @@ int or_collide(const hs_config* cfg, or_state* s, int32_t sample)
     int32_t max_nc = 1;
     for (int64_t c = 0; c < nc; ++c)
         if (s->start[c + 1] - s->start[c] > max_nc) max_nc = s->start[c + 1] - s->start[c];
+    /* with reactions the cells run in order: their products are appended
+     * cell by cell */
 #ifdef _OPENMP
-#pragma omp parallel
+#pragma omp parallel if (d.n_react == 0)
 #endif
     {
         int32_t* sub_list = (int32_t*)malloc(sizeof(int32_t) * max_nc);

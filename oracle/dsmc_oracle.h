@@ -72,6 +72,20 @@ int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const 
                       int32_t* vib_p, int32_t* vib_q, const double* uniforms,
                       int32_t n_uniform, int32_t* used_out);
 
+/* The accepted collision of n independent pairs with an injected uniform
+ * stream (pair k reads uniforms[k*n_uniform ..]): the pair's QK reaction
+ * when chemReactDict gives it a model (dissociationQK.C:469-600,
+ * exchangeQK.C:457-517, dissociationExchangeQK.C:98-270), else or when it
+ * does not react the binary collision (noTimeCounter.C:250-300).  Types,
+ * velocities [3k+c], rotational energies and vibrational levels
+ * [k*HS_MAX_VIB_MODES+m] are in/out; reacted_out[k] = 1 if it reacted;
+ * prod_type[k] = the dissociation product's typeId or -1, prod_u its
+ * velocity, prod_from_q[k] = 1 if Q dissociated (0: P, -1: no product). */
+int or_react_pairs(const hs_config* cfg, int64_t n, int32_t* tp, int32_t* tq, double* up, double* uq,
+                   double* erot_p, double* erot_q, int32_t* vib_p, int32_t* vib_q, const double* uniforms,
+                   int32_t n_uniform, int32_t* used_out, int32_t* reacted_out, int32_t* prod_type,
+                   double* prod_u, int32_t* prod_from_q);
+
 /* stages on an or_state */
 int or_fill(const hs_config* cfg, or_state* s, const double* n, double t, double trot,
             double tvib, const double u[3], int32_t exact_per_cell, uint32_t fill_id);

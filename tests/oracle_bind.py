@@ -85,6 +85,10 @@ def lib(path=None):
         L.or_face_count.restype = C.c_int64
         L.or_ntc_pairs.argtypes = [cfgp, C.c_int64, C.c_int32, P(C.c_double), P(C.c_double), P(C.c_double),
                                    C.c_uint32, C.c_int64, P(C.c_int32), P(C.c_int32)]
+        L.or_react_pairs.argtypes = [cfgp, C.c_int64, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double),
+                                     P(C.c_double), P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_double),
+                                     C.c_int32, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_double),
+                                     P(C.c_int32)]
         L.or_threads.restype = C.c_int
         L.or_set_threads.argtypes = [C.c_int]
         _libs[path] = L
@@ -151,6 +155,33 @@ def binary_collide(case, tp, tq, up, uq, uniforms, erot_p=None, erot_q=None, vib
                                  iptr(np.ascontiguousarray(tq, np.int32)), dptr(up), dptr(uq), dptr(ep), dptr(eq),
                                  iptr(vp), iptr(vq), dptr(uni), nu, iptr(used))
     return up, uq, ep, eq, vp, vq, used, rc
+
+
+def react_pairs(case, tp, tq, up, uq, erot_p, erot_q, vib_p, vib_q, uniforms):
+    """The accepted collision of each pair, its QK reaction first (the
+    oracle's or_react_pairs).  Returns a dict of the pairs' new states and
+    the products."""
+    n = len(tp)
+    m = abi.HS_MAX_VIB_MODES
+    tp = np.array(tp, np.int32).copy()
+    tq = np.array(tq, np.int32).copy()
+    up = np.array(up, np.float64).reshape(n, 3).copy()
+    uq = np.array(uq, np.float64).reshape(n, 3).copy()
+    ep = np.array(erot_p, np.float64).copy()
+    eq = np.array(erot_q, np.float64).copy()
+    vp = np.array(vib_p, np.int32).reshape(n, m).copy()
+    vq = np.array(vib_q, np.int32).reshape(n, m).copy()
+    uni = np.ascontiguousarray(uniforms, np.float64).reshape(n, -1)
+    used = np.zeros(n, np.int32)
+    reacted = np.zeros(n, np.int32)
+    ptype = np.zeros(n, np.int32)
+    pu = np.zeros((n, 3), np.float64)
+    pfrom = np.zeros(n, np.int32)
+    rc = lib().or_react_pairs(C.byref(case.cfg), n, iptr(tp), iptr(tq), dptr(up), dptr(uq), dptr(ep), dptr(eq),
+                              iptr(vp), iptr(vq), dptr(uni), uni.shape[1], iptr(used), iptr(reacted), iptr(ptype),
+                              dptr(pu), iptr(pfrom))
+    return dict(tp=tp, tq=tq, up=up, uq=uq, erot_p=ep, erot_q=eq, vib_p=vp, vib_q=vq, used=used, reacted=reacted,
+                prod_type=ptype, prod_u=pu, prod_from_q=pfrom, rc=rc)
 
 
 def ntc_pairs(case, cell, pos, step, ncand):

@@ -36,6 +36,7 @@ def test_struct_layout_matches_c():
         "hs_species": [f for f, _ in abi.hs_species._fields_],
         "hs_patch": [f for f, _ in abi.hs_patch._fields_],
         "hs_particles": [f for f, _ in abi.hs_particles._fields_],
+        "hs_reaction": [f for f, _ in abi.hs_reaction._fields_],
     }
     lines = ['#include <stdio.h>', '#include <stddef.h>', f'#include "{HEADER}"', "int main(void){"]
     for st, fs in fields.items():

@@ -1,0 +1,240 @@
+"""Quantum-kinetic chemistry of the oracle (test infrastructure) pinned
+against the reference.
+
+* The heatBath-5species tutorial run: the oracle steps the tutorial's own
+  case (one cell, about 154,000 parcels at nEquivalentParticles 100, the
+  twelve reactions of system/chemReactDict, 30,000 K) and its species number
+  densities and mixture temperatures follow the reference's probe series
+  (tests/golden/heatbath_probes.npz, written by make_heatbath_fixture.py from
+  gnuplot/solution/*) within their statistical scatter.  The reference ran
+  the same case once, so the bands are counting noise of both runs.  A
+  dropped heat of reaction, a threshold one quantum level off or a wrong
+  exchange sign moves these series far outside the bands.
+* Single reactions with injected uniforms (or_react_pairs): the dissociation
+  threshold of dissociationQK.C:219-235 (the molecule splits when the
+  collision energy holds more quanta than thetaD/thetaV), energy and
+  momentum balances of every outcome (a dissociation removes k thetaD, an
+  exchange adds k heatOfReactionExchange, a non-reacting collision conserves
+  both), and the order in which competing reactions are tried
+  (dsmcReaction.C:176-205: decreasing probability, ties by their draw).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_bind as ob
+from hystrath_amd import abi, cases, fields
+
+KB = abi.HS_KB
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "heatbath_probes.npz")
+NAMES = ["N2", "O2", "NO", "N", "O"]
+
+
+def _fixture():
+    return np.load(GOLDEN)
+
+
+def test_fixture_holds_the_tutorial_start():
+    fx = _fixture()
+    assert fx["step"][0] == 1 and fx["step"][999] == 1000 and fx["step"][-1] == 10000
+    # the fill (dsmcInitialiseDict) and the first step's first dissociations
+    assert fx["rhoN_N2"][0] == pytest.approx(1.21753030168e22, rel=0.01)
+    assert fx["rhoN_O2"][0] == pytest.approx(3.23647295384e21, rel=0.02)
+    assert fx["rhoN_NO"][0] < 1e19
+
+
+def test_heat_bath_follows_the_reference_probe_series():
+    fx = _fixture()
+    c = cases.heat_bath_5species(fn=100.0)
+    st = ob.OracleState(c)
+    st.fill(c.init_n, c.init_T)
+    V = 1e-15
+    checks = (5, 25, 50, 100, 150, 200)
+    prev = st.sample_array().copy()
+    for k in range(1, checks[-1] + 1):
+        st.evolve(1)
+        cur = st.sample_array().copy()
+        if k in checks:
+            # this step's own sums: resetAtOutput with a write every step
+            f = fields.vol_fields(cur - prev, 1, c)
+            row = int(np.searchsorted(fx["step"], k))
+            assert fx["step"][row] == k
+            p = st.particles()
+            ty = p["type"][: st.n]
+            counts = np.bincount(ty, minlength=5)
+            for i, nm in enumerate(NAMES):
+                mine = counts[i] * c.cfg.fn / V
+                ref = fx["rhoN_" + nm][row]
+                n_ref = ref * V / 100.0
+                if n_ref < 50:
+                    continue
+                # counting noise of both runs (4 sigma) plus 1%
+                tol = 4.0 * np.sqrt(1.0 / max(counts[i], 1) + 1.0 / n_ref) + 0.01
+                assert abs(mine / ref - 1.0) < tol, (k, nm, mine, ref, tol)
+            for key, rel in (("Ttra", 0.02), ("Trot", 0.03), ("Tvib", 0.05), ("Tov", 0.02)):
+                got = float(f[key][0])
+                ref = fx[key + "_mixture"][row]
+                assert abs(got / ref - 1.0) < rel, (k, key, got, ref)
+        prev = cur
+    # every dissociation made a parcel; the population only grows
+    assert st.n > int(sum(c.init_n) * V / c.cfg.fn) * 1.15
+
+
+def _state(t, u, er=0.0, v=0):
+    return t, np.array(u, np.float64), er, [v, 0, 0]
+
+
+def _react(c, pairs, uniforms):
+    tp, tq, up, uq, ep, eq, vp, vq = [], [], [], [], [], [], [], []
+    for (a, b) in pairs:
+        tp.append(a[0]); up.append(a[1]); ep.append(a[2]); vp.append(a[3])
+        tq.append(b[0]); uq.append(b[1]); eq.append(b[2]); vq.append(b[3])
+    return ob.react_pairs(c, tp, tq, up, uq, ep, eq, vp, vq, uniforms)
+
+
+def _rel_speed_for(c, ta, tb, energy):
+    ma, mb = c.species[ta].mass, c.species[tb].mass
+    mr = ma * mb / (ma + mb)
+    return np.sqrt(2.0 * energy / mr)
+
+
+def test_dissociation_threshold_counts_quanta_above_thetaD_over_thetaV():
+    """N2 + N2 (dissociationQK, same species: only P is tested): with no
+    vibrational energy, P splits iff floor(E_t / k thetaV) > floor(thetaD /
+    thetaV) = 33, i.e. from 34 quanta (114,614 K) on, not from thetaD
+    (113,500 K) itself."""
+    c = cases.heat_bath_5species(fn=1e4)
+    iN2, iN = 0, 3
+    th = c.species[iN2].theta_v[0]
+    assert int(c.species[iN2].theta_d / th) == 33
+    rng = np.random.default_rng(5)
+    pairs, uni = [], []
+    for f in (34.0 * (1 + 1e-9), 34.0 * (1 - 1e-9), 33.5, 113500.0 / th * (1 + 1e-9), 40.0):
+        cr = _rel_speed_for(c, iN2, iN2, f * KB * th)
+        pairs.append((_state(iN2, [cr, 0, 0]), _state(iN2, [0, 0, 0])))
+        uni.append(rng.random(400))
+    out = _react(c, pairs, np.array(uni))
+    assert out["rc"] == 0
+    assert list(out["reacted"]) == [1, 0, 0, 0, 1]
+    assert list(out["prod_type"]) == [iN, -1, -1, -1, iN]
+    assert list(out["tp"]) == [iN, iN2, iN2, iN2, iN]
+    assert list(out["tq"]) == [iN2] * 5
+
+
+def _energy(c, t, u, er, v):
+    sp = c.species[t]
+    e = 0.5 * sp.mass * float(np.dot(u, u)) + er
+    for m in range(sp.n_vib_modes):
+        e += v[m] * KB * sp.theta_v[m]
+    return e
+
+
+def test_every_outcome_balances_energy_and_momentum():
+    """Random hot pairs of every reacting species pair: after the collision
+    (reaction or not) the pair plus its product carry the momentum they came
+    with, and the energy less k thetaD for a dissociation, plus k
+    heatOfReactionExchange for an exchange."""
+    c = cases.heat_bath_5species(fn=1e4)
+    rng = np.random.default_rng(11)
+    sp = c.species
+    pairs, kinds = [], []
+    for a in range(5):
+        for b in range(5):
+            for _ in range(40):
+                T = rng.uniform(2e4, 6e4)
+                ua = rng.normal(0, np.sqrt(KB * T / sp[a].mass), 3)
+                ub = rng.normal(0, np.sqrt(KB * T / sp[b].mass), 3)
+                era = rng.exponential(KB * T) if sp[a].rot_dof > 0 else 0.0
+                erb = rng.exponential(KB * T) if sp[b].rot_dof > 0 else 0.0
+                va = int(rng.integers(0, 12)) if sp[a].n_vib_modes else 0
+                vb = int(rng.integers(0, 12)) if sp[b].n_vib_modes else 0
+                pairs.append((_state(a, ua, era, va), _state(b, ub, erb, vb)))
+    uni = rng.random((len(pairs), 600))
+    out = _react(c, pairs, uni)
+    assert out["rc"] == 0
+    n_diss = n_exch = 0
+    for k, (a, b) in enumerate(pairs):
+        e0 = _energy(c, *a) + _energy(c, *b)
+        p0 = sp[a[0]].mass * a[1] + sp[b[0]].mass * b[1]
+        ta, tb = out["tp"][k], out["tq"][k]
+        e1 = (_energy(c, ta, out["up"][k], out["erot_p"][k], out["vib_p"][k])
+              + _energy(c, tb, out["uq"][k], out["erot_q"][k], out["vib_q"][k]))
+        p1 = sp[ta].mass * out["up"][k] + sp[tb].mass * out["uq"][k]
+        pt = out["prod_type"][k]
+        if pt >= 0:
+            e1 += 0.5 * sp[pt].mass * float(np.dot(out["prod_u"][k], out["prod_u"][k]))
+            p1 = p1 + sp[pt].mass * out["prod_u"][k]
+            parent = b[0] if out["prod_from_q"][k] else a[0]
+            e1 += KB * sp[parent].theta_d
+            n_diss += 1
+        elif out["reacted"][k]:
+            # an exchange: the heat of the reaction of this pair
+            heat = None
+            for (model, reac, _, exch) in cases.HEATBATH_REACTIONS:
+                if exch and {NAMES.index(reac[0]), NAMES.index(reac[1])} == {a[0], b[0]}:
+                    heat = exch[1]
+            e1 -= KB * heat
+            n_exch += 1
+            assert {ta, tb} != {a[0], b[0]}
+        assert abs(e1 - e0) < 1e-11 * e0, (k, a[0], b[0], e0, e1)
+        assert np.all(np.abs(p1 - p0) < 1e-11 * (np.abs(p0).max() + sp[a[0]].mass * 1e3)), (k, p0, p1)
+    assert n_diss > 50 and n_exch > 5
+
+
+def test_competing_dissociations_are_tried_by_probability_then_draw():
+    """O2 + N2 both above their thresholds: probabilities 1/2 each, so the
+    order comes from the two draws of decreasing_sort_indices (the larger
+    draw first) and the cumulative test of the first (0.5 > u) picks it,
+    else the second (1.0 > u).  The same pair given as (N2, O2) is swapped
+    to (O2, N2) first, so the candidate that splits changes side."""
+    c = cases.heat_bath_5species(fn=1e4)
+    iN2, iO2, iN, iO = 0, 1, 3, 4
+    cr = _rel_speed_for(c, iO2, iN2, 150000.0 * KB)
+    rng = np.random.default_rng(3)
+    rest = rng.random(300)
+    cases_ = [
+        # (P, Q, first uniforms, expected splitting species, from_q)
+        (iO2, iN2, [0.1, 0.2, 0.9, 0.3], iN2, 1),   # N2's draw larger: N2 first, 0.5 > 0.3
+        (iO2, iN2, [0.1, 0.9, 0.2, 0.3], iO2, 0),   # O2's draw larger
+        (iO2, iN2, [0.1, 0.2, 0.9, 0.7], iO2, 0),   # N2 first but 0.5 < 0.7: then O2
+        (iN2, iO2, [0.1, 0.2, 0.9, 0.3], iN2, 0),   # swapped: N2 is the candidate P
+    ]
+    pairs, uni = [], []
+    for tp, tq, head, _, _ in cases_:
+        pairs.append((_state(tp, [cr, 0, 0]), _state(tq, [0, 0, 0])))
+        uni.append(np.concatenate([head, rest]))
+    out = _react(c, pairs, np.array(uni))
+    for k, (tp, tq, _, split, from_q) in enumerate(cases_):
+        assert out["reacted"][k] == 1
+        assert out["prod_from_q"][k] == from_q, k
+        assert out["prod_type"][k] == (iN if split == iN2 else iO), k
+        after = out["tq"][k] if from_q else out["tp"][k]
+        assert after == out["prod_type"][k]
+
+
+def test_no_reaction_model_or_no_splitting_is_a_plain_collision():
+    """A pair with no reaction model (N + O), and a reacting pair with
+    allowSplitting off, collide as the binary model does: same draws, same
+    result as or_binary_collide, no species change."""
+    c = cases.heat_bath_5species(fn=1e4)
+    off = cases.heat_bath_5species(fn=1e4, allow_splitting=False)
+    rng = np.random.default_rng(7)
+    iN2, iN, iO = 0, 3, 4
+    cr = _rel_speed_for(c, iN2, iN2, 200000.0 * KB)
+    uni = rng.random((2, 300))
+    a = _react(c, [(_state(iN, [3e4, 0, 0]), _state(iO, [0, 0, 0]))], uni[:1])
+    b = ob.binary_collide(c, [iN], [iO], [[3e4, 0, 0]], [[0, 0, 0]], uni[:1], [0.0], [0.0],
+                          [[0, 0, 0]], [[0, 0, 0]])
+    assert a["reacted"][0] == 0 and a["prod_type"][0] == -1
+    np.testing.assert_array_equal(a["up"], b[0])
+    np.testing.assert_array_equal(a["uq"], b[1])
+    # allowSplitting off: the reaction's draws are taken, then the collision
+    r = _react(off, [(_state(iN2, [cr, 0, 0]), _state(iN2, [0, 0, 0]))], uni[1:])
+    assert r["reacted"][0] == 0 and r["prod_type"][0] == -1
+    assert r["tp"][0] == iN2 and r["tq"][0] == iN2
+    mr = 0.5 * c.species[iN2].mass
+    e0 = 0.5 * mr * cr * cr
+    e1 = (_energy(c, iN2, r["up"][0] - 0.5 * np.array([cr, 0, 0]), r["erot_p"][0], r["vib_p"][0])
+          + _energy(c, iN2, r["uq"][0] - 0.5 * np.array([cr, 0, 0]), r["erot_q"][0], r["vib_q"][0]))
+    assert abs(e1 - e0) < 1e-11 * e0
