@@ -357,6 +357,37 @@ def cylinder_flow(nx=400, ny=400, Lx=1.0, Ly=1.0, x_cyl=0.35, R=0.1, n_inf=4.2e2
     return case
 
 
+def air_cylinder(nx=1000, ny=1000, Lx=0.1, Ly=0.1, x_cyl=0.035, R=0.01, n_inf=1e21, T_inf=220.0, U=7000.0,
+                 T_wall=1000.0, ppc=30, dt=None, seed=41, capacity=None, flags=abi.HS_FLAG_SAMPLE):
+    """Reacting five-species air (N2 O2 NO N O, the species and the twelve QK
+    reactions of the heatBath-5species tutorial) at U past a diffuse
+    isothermal cylinder of radius R: a Cartesian block of nx*ny cells, one
+    cell thick in z (empty), the cylinder cut out exactly, freestream inflow
+    of 79% N2 and 21% O2 at T_inf and deletion on the four outer sides;
+    LB-VHS with Zrot 5 and Bird's Zv.  The defaults are 7 km/s, 220 K,
+    1e21 m^-3 air on 10^6 cells with 30 parcels per freestream cell."""
+    names = ["N2", "O2", "NO", "N", "O"]
+    sp = [make_species(n) for n in names]
+    dx, dy = Lx / nx, Ly / ny
+    dz = dx
+    dt = dt or 0.25 * dx / U
+    fn = n_inf * dx * dy * dz / ppc
+    outer = patch(abi.HS_PATCH_DELETION, inflow=1)
+    empty = patch(abi.HS_PATCH_EMPTY)
+    wall = patch(abi.HS_PATCH_DIFFUSE, temperature=T_wall)
+    origin = (-x_cyl, -0.5 * Ly, 0.0)
+    cap = capacity or int(nx * ny * ppc * 2.0) + (1 << 16)
+    n_in = [0.79 * n_inf, 0.21 * n_inf, 0.0, 0.0, 0.0]
+    case = Case(nx, ny, 1, origin, (dx, dy, dz), sp, [outer, outer, outer, outer, empty, empty],
+                collision_model=abi.HS_COLL_LB_VHS, fn=fn, dt=dt, seed=seed, capacity=cap, flags=flags,
+                cyl=(0.0, 0.0, R), cyl_patch=wall, inflow=dict(n=n_in, T=T_inf, U=(U, 0.0, 0.0)),
+                name="air_cylinder")
+    case.set_reactions(HEATBATH_REACTIONS, names)
+    case.names = names
+    case.n_in, case.T_inf, case.U_inf = n_in, T_inf, U
+    return case
+
+
 def couette_n2o2(seed=11, capacity=200000, flags=abi.HS_FLAG_SAMPLE):
     """The couette_N2-O2 tutorial: system/blockMeshDict (5x100x1 cells on
     [0,0.05]x[0,1]x[0,0.01], cyclic x, walls y, empty z), system/boundariesDict

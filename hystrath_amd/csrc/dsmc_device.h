@@ -31,6 +31,21 @@ enum { RNG_MOVE = 1, RNG_COLLIDE = 2, RNG_INJECT = 3, RNG_FILL_COUNT = 4,
 enum { CNT_CAND = 0, CNT_COLL = 1, CNT_DEL = 2, CNT_INJ = 3, CNT_WALL = 4,
        CNT_TRACK_OVF = 5, CNT_TILE_OVF = 6, CNT_CAP_OVF = 7, CNT_PSTEPS = 8, CNT_N = 9 };
 
+// One reaction of chemReactDict with its constants (the QK models of
+// src/lagrangian/dsmc/reactions/derived).
+struct DevReaction {
+    int model;             // hs_reaction_model
+    int r[2];              // reactant typeIds
+    int dprod[2][2];       // dissociation products of each reactant, -1: none
+    int xmol, xatom;       // exchange products: the molecule, the atom
+    int pos_mol;           // exchangeQK::posMolReactant_ (exchangeQK.C:52-60)
+    int split;             // allowSplitting
+    double heat_diss[2];   // k thetaD of each reactant (dissociationQK.C:57-60)
+    double heat_x;         // heatOfReactionExchange k (exchangeQK.C:387-391)
+    double a_dash, b;      // Bird's QK exchange constants (exchangeQK.C:195-200)
+    double om_x;           // omega of the exchange's redistribution (exchangeQK.C:303-308)
+};
+
 // Everything a kernel needs about the case, resident in device memory and
 // read through uniform (scalar) loads.
 struct DevParams {
@@ -82,6 +97,11 @@ struct DevParams {
     unsigned long long* face_flux;  // [ns][n_faces] two's-complement int64 counts
     int64_t n_faces;
     int64_t face_off[4];        // first face of the x, y, z planes; cylinder faces
+    // chemistry (dsmcReactions)
+    int n_react;
+    int kind[HS_MAX_SPECIES];   // dsmcParcelI.H:153-196: 0 electron, 10/11 atom, 20/21, 30/31 molecule
+    int react_of[HS_MAX_SPECIES][HS_MAX_SPECIES];  // pairAddressing_: reaction of a pair, -1 none
+    DevReaction rx[HS_MAX_REACTIONS];
 };
 
 // face-tracker face of plane k on axis a at the tangential indices of idx
@@ -378,11 +398,22 @@ __device__ __forceinline__ void post_coll_vel(const DevParams& P, bool vss, bool
     }
 }
 
-// dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531)
-template <class R>
+// dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531);
+// POST = postReaction: the level is always redrawn, with no Zv test
+template <bool POST = false, class R>
 __device__ int vib_level(const DevParams& P, int t, int partner, int mode, int level, int imax,
                          double ec, double om, R& r) {
     const double theta_v = P.theta_v[t][mode];
+    if (POST) {
+        int idash, guard = 0;
+        double func;
+        do {
+            idash = random_label(r, 0, imax);
+            const double evib = (double)idash * HS_KB * theta_v;
+            func = dm_pow(1.0 - evib / ec, 1.5 - om);
+        } while (func < r.u() && ++guard < HS_MAX_REDRAW);
+        return idash;
+    }
     double inv_zv = 1.0;
     if (P.z_vib_fixed == 0.0) {
         const double T = (double)imax * theta_v / (3.5 - om);
@@ -427,8 +458,8 @@ __device__ double rot_ratio(double dof, double chib, R& r) {
 // LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
 // species with one electronic level
 // INTERNAL false: every species has neither rotational nor vibrational
-// modes, and only the electronic test draw remains.
-template <bool INTERNAL, class R>
+// modes, and only the electronic test draw remains.  POST = postReaction.
+template <bool INTERNAL, bool POST = false, class R>
 __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* et, double* erot,
                                 int* vib, double om, R& r) {
     if (P.charge[t] == -1) return;
@@ -442,7 +473,7 @@ __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* 
             const double ec = *et + pre[m];
             const int imax = (int)(ec / (HS_KB * P.theta_v[t][m]));
             if (imax > 0) {
-                vib[m] = vib_level(P, t, partner, m, vib[m], imax, ec, om, r);
+                vib[m] = vib_level<POST>(P, t, partner, m, vib[m], imax, ec, om, r);
                 *et = ec - (double)vib[m] * HS_KB * P.theta_v[t][m];
             }
         }
@@ -478,6 +509,256 @@ __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3],
     lb_redistribute<INTERNAL>(P, tq, tp, &et, erq, vq, om, r);
     const double cr = sqrt(2.0 * et / mr);
     post_coll_vel(P, model == HS_COLL_LB_VSS, true, tp, tq, up, uq, cr, r);
+}
+
+// ---------------------------------------------------------------------------
+// quantum-kinetic chemistry (src/lagrangian/dsmc/reactions)
+// ---------------------------------------------------------------------------
+// one parcel's state during a reaction
+struct PState {
+    int t;
+    double u[3];
+    double er;
+    int v[HS_MAX_VIB_MODES];
+};
+
+__device__ __forceinline__ bool is_lb(const DevParams& P) {
+    return P.coll_model == HS_COLL_LB_VHS || P.coll_model == HS_COLL_LB_VSS;
+}
+
+// eVib_tot (dsmcParcelI.H:389-398)
+__device__ __forceinline__ double evib_total(const DevParams& P, const PState& a) {
+    double e = 0.0;
+    for (int m = 0; m < P.nvib[a.t]; ++m) e += (double)a.v[m] * HS_KB * P.theta_v[a.t][m];
+    return e;
+}
+
+__device__ __forceinline__ void become(PState& a, int t) {
+    a.t = t;
+    a.er = 0.0;
+#pragma unroll
+    for (int m = 0; m < HS_MAX_VIB_MODES; ++m) a.v[m] = 0;
+}
+
+// 1/2 mr |ua - ub|^2
+__device__ __forceinline__ double rel_energy(const DevParams& P, const PState& a, const PState& b) {
+    const double ma = P.mass[a.t], mb = P.mass[b.t];
+    const double mr = ma * mb / (ma + mb);
+    const double dx = a.u[0] - b.u[0], dy = a.u[1] - b.u[1], dz = a.u[2] - b.u[2];
+    return 0.5 * mr * (dx * dx + dy * dy + dz * dz);
+}
+
+// dissociationQK::testDissociation (dissociationQK.C:199-238)
+__device__ __forceinline__ void test_dissociation(const DevParams& P, const PState& a, double et, int* mode,
+                                                  double* ec, double* total, double* prob) {
+    const int kind = P.kind[a.t];
+    if (kind != 20 && kind != 30) return;
+    for (int m = 0; m < P.nvib[a.t]; ++m) {
+        const double theta_v = P.theta_v[a.t][m];
+        const double ev = (double)a.v[m] * HS_KB * theta_v;
+        const int id = (int)(P.theta_d[a.t] / theta_v);
+        *ec = et + ev;
+        const int imax = (int)(*ec / (HS_KB * theta_v));
+        if (imax > id) {
+            *prob = 1.0;
+            *total += *prob;
+            *mode = m;
+            break;
+        }
+    }
+}
+
+// exchangeQK::testExchange (exchangeQK.C:180-267); a is the molecule
+__device__ __forceinline__ void test_exchange(const DevParams& P, const DevReaction& x, const PState& a, double et,
+                                              double om, double* ec, double* total, double* prob) {
+    const double chib = 2.5 - om;
+    const double tcoll = et / (HS_KB * chib);
+    double ea = x.a_dash * dm_pow(tcoll / 273.0, x.b) * fabs(x.heat_x);
+    double sum = 1.0;
+    if (x.heat_x < 0.0) ea -= x.heat_x;
+    const int nm = P.nvib[a.t];
+    int m = 0;
+    do {
+        const double th = P.theta_v[a.t][m];
+        const double kth = HS_KB * th;
+        const double ev = (double)a.v[m] * HS_KB * th;
+        *ec = et + ev;
+        if (*ec > ea) {
+            if (ea > kth) {
+                sum = 0.0;
+                const int ia = (int)(*ec / kth);
+                for (int i = 0; i <= ia; ++i) sum += dm_pow(1.0 - (double)i * HS_KB * th / *ec, 1.5 - om);
+            }
+            *prob = dm_pow(1.0 - ea / *ec, 1.5 - om) / sum;
+            m = nm;
+        }
+        m += 1;
+    } while (m < nm);
+    *total += *prob;
+}
+
+// VariableHardSphere::postReactionVelocities (VariableHardSphere.C:231-262):
+// ua enters as the pair's centre-of-mass velocity
+template <class R>
+__device__ __forceinline__ void post_reaction_vel(const DevParams& P, int ta, int tb, double ua[3], double ub[3],
+                                                  double cr, R& r) {
+    const double ma = P.mass[ta], mb = P.mass[tb];
+    const double cos_t = 2.0 * r.u() - 1.0;
+    const double sin_t = sqrt(1.0 - cos_t * cos_t);
+    double sp, cp;
+    dm_sincos2pi(r.u(), &sp, &cp);
+    const double rel[3] = { cr * cos_t, cr * (sin_t * cp), cr * (sin_t * sp) };
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        ub[c] = ua[c] - rel[c] * ma / (ma + mb);
+        ua[c] = ua[c] + rel[c] * mb / (ma + mb);
+    }
+}
+
+// dissociationQK::dissociateParticleByPartner (dissociationQK.C:240-378): a
+// (reactant nr) dissociates in vibrational mode `mode`, b is the partner; a
+// becomes the first product and prod the second.  True when the split
+// happens (allowSplitting).
+template <class R>
+__device__ bool dissociate(const DevParams& P, const DevReaction& x, PState& a, PState& b, int nr, int mode,
+                           double ec, R& r, PState& prod) {
+    if (!x.split) return false;
+    ec -= x.heat_diss[nr];
+    const double om = P.omega_pq[a.t][b.t];
+    if (is_lb(P)) lb_redistribute<true, true>(P, b.t, a.t, &ec, &b.er, b.v, om, r);
+    const double ma = P.mass[a.t], mb = P.mass[b.t];
+    const double mr = ma * mb / (ma + mb);
+    const double cr = sqrt(2.0 * ec / mr);
+    post_coll_vel(P, false, false, a.t, b.t, a.u, b.u, cr, r);
+    const int t1 = x.dprod[nr][0], t2 = x.dprod[nr][1];
+    const double m1 = P.mass[t1], m2 = P.mass[t2];
+    const double mrp = m1 * m2 / (m1 + m2);
+    const double evt = evib_total(P, a);
+    const double evm = (double)a.v[mode] * HS_KB * P.theta_v[a.t][mode];
+    const double evn = evt - evm;
+    const double left = a.er + evn + 0.0;  // + the single electronic level's energy, 0
+    const double crp = sqrt(2.0 * left / mrp);
+    become(prod, t2);
+    post_reaction_vel(P, t1, t2, a.u, prod.u, crp, r);
+    become(a, t1);
+    return true;
+}
+
+// exchangeQK::exchange (exchangeQK.C:269-370): a is the molecule, b the atom
+template <class R>
+__device__ bool exchange(const DevParams& P, const DevReaction& x, PState& a, PState& b, R& r) {
+    if (!x.split) return false;
+    const double ma = P.mass[a.t], mb = P.mass[b.t];
+    double et = rel_energy(P, a, b);
+    double ucm[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) ucm[c] = (ma * a.u[c] + mb * b.u[c]) / (ma + mb);
+    const double mpx = P.mass[x.xatom], mqx = P.mass[x.xmol];
+    const double mrx = mpx * mqx / (mpx + mqx);
+    const double evt = evib_total(P, a);
+    et += a.er + evt + 0.0 + 0.0 + x.heat_x;  // + both electronic energies, 0
+    become(a, x.xatom);
+    become(b, x.xmol);
+    if (is_lb(P)) lb_redistribute<true, true>(P, b.t, a.t, &et, &b.er, b.v, x.om_x, r);
+    const double crx = sqrt(2.0 * et / mrx);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) a.u[c] = ucm[c];
+    post_reaction_vel(P, x.xatom, x.xmol, a.u, b.u, crx, r);
+    return true;
+}
+
+// dsmcReaction::decreasing_sort_indices (dsmcReaction.C:176-205): n draws,
+// then the indices by probability, equal probabilities by their draw, both
+// decreasing (insertion sort: the order is unique unless two draws tie)
+template <class R>
+__device__ __forceinline__ void decreasing_order(const double* v, int n, R& r, int* idx) {
+    double rnd[3];
+    for (int i = 0; i < n; ++i) {
+        rnd[i] = r.u();
+        idx[i] = i;
+    }
+    for (int i = 1; i < n; ++i) {
+        const int y = idx[i];
+        int j = i - 1;
+        while (j >= 0 && (v[y] > v[idx[j]] || (v[y] == v[idx[j]] && rnd[y] > rnd[idx[j]]))) {
+            idx[j + 1] = idx[j];
+            --j;
+        }
+        idx[j + 1] = y;
+    }
+}
+
+// The reaction of an accepted pair with a reaction model (noTimeCounter.C:
+// 250-300): true if it reacted (relax_ is off and no binary collision
+// follows).  A dissociation sets prod (prod.t >= 0) and from_q (Q split).
+// The models swap P and Q when P is not their first reactant.
+template <class R>
+__device__ bool react_pair(const DevParams& P, int ri, PState& Pp, PState& Qp, R& r, PState& prod, int* from_q) {
+    const DevReaction& x = P.rx[ri];
+    PState* p = &Pp;
+    PState* q = &Qp;
+    if (x.model == HS_REACT_EXCHANGE_QK) {
+        // exchangeQK::reaction (exchangeQK.C:457-517)
+        const int kp = P.kind[p->t];
+        if (kp == 10 || kp == 11) { p = &Qp; q = &Pp; }
+        const double om = P.omega_pq[p->t][q->t];
+        const double et = rel_energy(P, *p, *q);
+        double total = 0.0, prob = 0.0, ec = 0.0;
+        test_exchange(P, x, *p, et, om, &ec, &total, &prob);
+        if (total > r.u()) return exchange(P, x, *p, *q, r);
+        return false;
+    }
+    // dissociationQK::reaction (dissociationQK.C:469-600) and
+    // dissociationExchangeQK::reaction (dissociationExchangeQK.C:98-270)
+    if (p->t != x.r[0]) { p = &Qp; q = &Pp; }
+    const bool q_is_Q = q == &Qp;
+    const bool three = x.model == HS_REACT_DISSOCIATION_EXCHANGE_QK;
+    const double om = P.omega_pq[p->t][q->t];
+    const double et = rel_energy(P, *p, *q);
+    double total = 0.0, prob[3] = { 0.0, 0.0, 0.0 }, ec[3] = { 0.0, 0.0, 0.0 };
+    int mode_p = -1, mode_q = -1;
+    test_dissociation(P, *p, et, &mode_p, &ec[0], &total, &prob[0]);
+    if (three || p->t != q->t) test_dissociation(P, *q, et, &mode_q, &ec[1], &total, &prob[1]);
+    if (three) {
+        if (x.pos_mol == 0) test_exchange(P, x, *p, et, om, &ec[2], &total, &prob[2]);
+        else test_exchange(P, x, *q, et, om, &ec[2], &total, &prob[2]);
+    }
+    if (!(total > r.u())) return false;
+    const int n = three ? 3 : 2;
+    double np[3];
+    int idx[3];
+    for (int i = 0; i < n; ++i) np[i] = prob[i] / total;
+    decreasing_order(np, n, r, idx);
+    double cum = 0.0;
+    for (int k = 0; k < n; ++k) {
+        const int i = idx[k];
+        if (!(np[i] > HS_SMALL)) break;
+        cum += np[i];
+        if (cum > r.u()) {
+            if (i == 0) {
+                *from_q = q_is_Q ? 0 : 1;
+                return dissociate(P, x, *p, *q, 0, mode_p, ec[0], r, prod);
+            }
+            if (i == 1) {
+                *from_q = q_is_Q ? 1 : 0;
+                return dissociate(P, x, *q, *p, 1, mode_q, ec[1], r, prod);
+            }
+            return x.pos_mol == 0 ? exchange(P, x, *p, *q, r) : exchange(P, x, *q, *p, r);
+        }
+    }
+    return false;
+}
+
+// The accepted collision of one pair with chemistry on (noTimeCounter.C:
+// 250-300): its reaction when the pair has a model, else (or when it did not
+// react) the binary collision.  True if it reacted.
+template <class R>
+__device__ bool collide_or_react(const DevParams& P, PState& a, PState& b, R& r, PState& prod, int* from_q) {
+    prod.t = -1;
+    const int ri = P.react_of[a.t][b.t];
+    if (ri >= 0 && react_pair(P, ri, a, b, r, prod, from_q)) return true;
+    binary_collide<true>(P, a.t, b.t, a.u, b.u, &a.er, &b.er, a.v, b.v, r);
+    return false;
 }
 
 // ---------------------------------------------------------------------------

@@ -38,6 +38,8 @@ static int scratch_on_device(int dev, void** out);
 struct hs_dsmc_engine {
     hs_config cfg;
     std::vector<double> vol_h;
+    std::vector<hs_reaction> reactions;  // chemReactDict, copied at create
+    DevProducts prod = {};               // dissociation products (chemistry only)
     int device = 0;
     int nc = 0, ns = 0, general = 0, nslots = 0;
     int has_erot = 0, has_vib = 0, has_type = 0, max_modes = 0;
@@ -287,6 +289,101 @@ static void list_wall_faces(hs_dsmc_engine* eng, int wall_off[6]) {
             }
 }
 
+// a species' kind (dsmcParcelI.H:153-196): 0 electron, 10/11 atom, 20/21
+// diatomic, 30/31 polyatomic molecule (neutral / ion)
+static int species_kind(const hs_species& sp) {
+    if (sp.charge == -1) return 0;
+    const int base = sp.n_vib_modes == 0 ? 10 : (sp.n_vib_modes == 1 ? 20 : 30);
+    return base + (sp.charge == 1 ? 1 : 0);
+}
+
+// whether reaction x applies to the pair (p, q): tryReactMolecules of
+// dissociationQK.C:430-455 (dissociation models) and exchangeQK.C:420-443
+static bool reaction_applies(const hs_reaction& x, int p, int q) {
+    const int pid = p == x.reactants[0] ? 0 : (p == x.reactants[1] ? 1 : -1);
+    const int qid = q == x.reactants[0] ? 0 : (q == x.reactants[1] ? 1 : -1);
+    if (pid < 0 || qid < 0) return false;
+    if (x.model == HS_REACT_EXCHANGE_QK) return pid != qid;
+    const bool same = x.reactants[0] == x.reactants[1];
+    return (pid == qid && same) || (pid != qid && !same);
+}
+
+// the checks of dsmcReaction / dissociationQK / exchangeQK::setProperties
+// (dissociationQK.C:33-197, exchangeQK.C:44-178) and the one-model-per-pair
+// rule of dsmcReactions.C:139-160
+static int validate_reactions(const hs_config* c) {
+    if (c->n_reactions == 0) return 0;
+    if (c->n_reactions < 0 || c->n_reactions > HS_MAX_REACTIONS)
+        return fail(nullptr, HS_E_INVALID, "n_reactions out of range");
+    if (!c->reactions) return fail(nullptr, HS_E_INVALID, "reactions is NULL");
+    if (c->collision_model != HS_COLL_VHS && c->collision_model != HS_COLL_LB_VHS)
+        return fail(nullptr, HS_E_UNSUPPORTED,
+                    "chemical reactions need VariableHardSphere or LarsenBorgnakkeVariableHardSphere scattering");
+    const int ns = c->n_species;
+    auto kind = [&](int t) { return species_kind(c->species[t]); };
+    auto is_mol = [&](int t) { return kind(t) == 20 || kind(t) == 30; };
+    for (int i = 0; i < c->n_reactions; ++i) {
+        const hs_reaction& x = c->reactions[i];
+        if (x.model < HS_REACT_DISSOCIATION_QK || x.model > HS_REACT_DISSOCIATION_EXCHANGE_QK)
+            return fail(nullptr, HS_E_INVALID, "unknown reactionModel");
+        for (int r = 0; r < 2; ++r)
+            if (x.reactants[r] < 0 || x.reactants[r] >= ns)
+                return fail(nullptr, HS_E_INVALID, "Cannot find type id of a reactant");
+        if (x.model != HS_REACT_EXCHANGE_QK) {
+            if (!is_mol(x.reactants[0]) && !is_mol(x.reactants[1]))
+                return fail(nullptr, HS_E_INVALID, "None of the reactants is a molecule");
+            for (int r = 0; r < 2; ++r) {
+                const int p0 = x.diss_products[r][0], p1 = x.diss_products[r][1];
+                if (p0 < 0 && p1 < 0) {
+                    if (is_mol(x.reactants[r]))
+                        return fail(nullptr, HS_E_INVALID, "a molecule reactant should have dissociation products");
+                    continue;
+                }
+                if (!is_mol(x.reactants[r]))
+                    return fail(nullptr, HS_E_INVALID,
+                                "a reactant that is not a molecule should have no dissociation products");
+                if (p0 < 0 || p1 < 0 || p0 >= ns || p1 >= ns)
+                    return fail(nullptr, HS_E_INVALID, "There should be 2 dissociation products of known type ids");
+                if (kind(x.reactants[r]) == 30)
+                    return fail(nullptr, HS_E_INVALID,
+                                "Dissociation product of a polyatomic molecule must be a diatomic/polyatomic molecule");
+                if (kind(p0) != 10 || kind(p1) != 10)
+                    return fail(nullptr, HS_E_INVALID, "Dissociation product of a diatomic molecule must be an atom");
+            }
+        }
+        if (x.model != HS_REACT_DISSOCIATION_QK) {
+            bool mol = false, atom = false;
+            for (int r = 0; r < 2; ++r) {
+                const int k = kind(x.reactants[r]);
+                if (k >= 20) mol = true;
+                else if (k == 10 || k == 11) atom = true;
+                else return fail(nullptr, HS_E_INVALID, "an exchange reactant must be an atom or a molecule");
+            }
+            if (!mol) return fail(nullptr, HS_E_INVALID, "None of the exchange reactants is a molecule");
+            if (!atom) return fail(nullptr, HS_E_INVALID, "None of the exchange reactants is an atom");
+            mol = atom = false;
+            for (int r = 0; r < 2; ++r) {
+                const int t = x.exch_products[r];
+                if (t < 0 || t >= ns) return fail(nullptr, HS_E_INVALID, "Cannot find type id of an exchange product");
+                const int k = kind(t);
+                if (k >= 20) mol = true;
+                else if (k == 10 || k == 11) atom = true;
+                else return fail(nullptr, HS_E_INVALID, "an exchange product must be an atom or a molecule");
+            }
+            if (!mol || !atom)
+                return fail(nullptr, HS_E_INVALID, "the exchange products must be one molecule and one atom");
+        }
+    }
+    for (int p = 0; p < ns; ++p)
+        for (int q = p; q < ns; ++q) {
+            int found = 0;
+            for (int i = 0; i < c->n_reactions; ++i) found += reaction_applies(c->reactions[i], p, q) ? 1 : 0;
+            if (found > 1)
+                return fail(nullptr, HS_E_INVALID, "There is more than one reaction model specified for a typeId pair");
+        }
+    return 0;
+}
+
 static int validate(const hs_config* c) {
     if (!c) return fail(nullptr, HS_E_INVALID, "config is NULL");
     if (c->nx <= 0 || c->ny <= 0 || c->nz <= 0) return fail(nullptr, HS_E_INVALID, "mesh dimensions must be positive");
@@ -335,7 +432,7 @@ static int validate(const hs_config* c) {
     }
     if (c->capacity <= 0 || c->capacity >= ((int64_t)1 << 31) - 1) return fail(nullptr, HS_E_INVALID, "capacity out of range");
     if (!(c->fn > 0.0) || !(c->dt > 0.0)) return fail(nullptr, HS_E_INVALID, "nEquivalentParticles and deltaT must be positive");
-    return 0;
+    return validate_reactions(c);
 }
 
 static void derive(hs_dsmc_engine* eng) {
@@ -410,6 +507,46 @@ static void derive(hs_dsmc_engine* eng) {
     P.inflow_tvib = c.inflow_tvib;
     for (int a = 0; a < 3; ++a) P.inflow_u[a] = c.inflow_u[a];
     for (int s = 0; s < HS_MAX_SPECIES; ++s) P.inflow_n[s] = c.inflow_n[s];
+    // chemistry: the reactions' constants (dissociationQK.C:57-60,
+    // exchangeQK.C:195-200,303-308,387-391) and the pair table
+    P.n_react = (int)eng->reactions.size();
+    for (int p = 0; p < HS_MAX_SPECIES; ++p) {
+        P.kind[p] = p < P.ns ? species_kind(c.species[p]) : 0;
+        for (int q = 0; q < HS_MAX_SPECIES; ++q) P.react_of[p][q] = -1;
+    }
+    for (int i = 0; i < P.n_react; ++i) {
+        const hs_reaction& R = eng->reactions[i];
+        DevReaction& x = P.rx[i];
+        x.model = R.model;
+        x.split = R.allow_splitting != 0;
+        for (int k = 0; k < 2; ++k) {
+            x.r[k] = R.reactants[k];
+            x.dprod[k][0] = R.diss_products[k][0];
+            x.dprod[k][1] = R.diss_products[k][1];
+            x.heat_diss[k] = HS_KB * c.species[R.reactants[k]].theta_d;
+        }
+        x.pos_mol = -1;
+        x.xmol = x.xatom = -1;
+        if (R.model != HS_REACT_DISSOCIATION_QK) {
+            for (int k = 0; k < 2; ++k)
+                if (species_kind(c.species[R.reactants[k]]) >= 20) x.pos_mol = k;
+            for (int k = 0; k < 2; ++k) {
+                const int t = R.exch_products[k];
+                const int kd = species_kind(c.species[t]);
+                if (kd >= 20) x.xmol = t;
+                else if (kd == 10 || kd == 11) x.xatom = t;
+            }
+            x.heat_x = R.heat_exchange * HS_KB;
+            x.b = R.b_coeff;
+            const double om = 0.5 * (c.species[R.reactants[0]].omega + c.species[R.reactants[1]].omega);
+            const double chib = 2.5 - om;
+            x.a_dash = R.a_coeff * (pow(chib, R.b_coeff) * exp(lgamma(chib)) / exp(lgamma(chib + R.b_coeff)));
+            x.om_x = 0.5 * (c.species[x.xatom].omega + c.species[x.xatom].omega);
+        }
+        for (int p = 0; p < P.ns; ++p)
+            for (int q = p; q < P.ns; ++q)
+                if (reaction_applies(R, p, q)) { P.react_of[p][q] = i; P.react_of[q][p] = i; }
+    }
 }
 
 static int alloc_particles(hs_dsmc_engine* eng, DevParticles& b) {
@@ -491,6 +628,8 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
     eng->cap = cfg->capacity;
     eng->vol_h.assign(cfg->cell_volume, cfg->cell_volume + eng->nc);
     eng->cfg.cell_volume = eng->vol_h.data();
+    if (cfg->n_reactions > 0) eng->reactions.assign(cfg->reactions, cfg->reactions + cfg->n_reactions);
+    eng->cfg.reactions = eng->reactions.empty() ? nullptr : eng->reactions.data();
     for (int s = 0; s < eng->ns; ++s) {
         if (cfg->species[s].rot_dof > 0) eng->has_erot = 1;
         if (cfg->species[s].n_vib_modes > 0) eng->has_vib = 1;
@@ -534,6 +673,18 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         (rc = dev_alloc(eng, &eng->wq.code_iter, eng->cap)) ||
         (rc = dev_alloc(eng, &eng->d_S, 1)))
         return bail(rc);
+    if (!eng->reactions.empty()) {
+        DevProducts& pr = eng->prod;
+        const int64_t cap = eng->cap;
+        if ((rc = dev_alloc(eng, &pr.cell, cap)) || (rc = dev_alloc(eng, &pr.rank, cap)) ||
+            (rc = dev_alloc(eng, &pr.parent, cap)) || (rc = dev_alloc(eng, &pr.type, cap)) ||
+            (rc = dev_alloc(eng, &pr.ux, cap)) || (rc = dev_alloc(eng, &pr.uy, cap)) ||
+            (rc = dev_alloc(eng, &pr.uz, cap)) || (rc = dev_alloc(eng, &pr.count, nc)) ||
+            (rc = dev_alloc(eng, &pr.off, (int64_t)nc + 1)) || (rc = dev_alloc(eng, &pr.list, cap)))
+            return bail(rc);
+        launch_fill_i32(pr.count, 0, nc, eng->stream);
+        launch_fill_i32(pr.off, 0, (int64_t)nc + 1, eng->stream);
+    }
     // inflow faces
     std::vector<HostFace> hf = list_faces(eng->cfg, eng->vol_h);
     eng->faces.n = (int32_t)hf.size();
@@ -1014,6 +1165,11 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
                           eng->d_tile_heavy, eng->stream);
     }
     inject_next = inject_next && advance && eng->faces.n > 0;
+    // with chemistry the products go after the sorted population, so the next
+    // step's inflow is appended after them, at the end of this step
+    const bool react = !eng->reactions.empty();
+    const bool inject_after = react && inject_next;
+    if (react) inject_next = 0;
     if (inject_next) {
         (void)hipEventRecord(eng->ev_scanned, eng->stream);
         (void)hipStreamWaitEvent(eng->side, eng->ev_scanned, 0);
@@ -1030,7 +1186,12 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
         launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
                       eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
     }
-    if (do_collide || do_sample) {
+    if ((do_collide || do_sample) && react) {
+        StageTimer tm(eng, T_COLLIDE);
+        launch_collide_react(eng->dp, nxt_buf(eng), eng->d_start, eng->d_big_sub, eng->d_sigma, eng->d_rem,
+                             eng->d_samples, eng->d_vol, eng->d_S, eng->prod, do_collide, do_sample, eng->nc, eng->ws,
+                             eng->cap, eng->stream);
+    } else if (do_collide || do_sample) {
         StageTimer tm(eng, T_COLLIDE);
         launch_collide(eng->dp, nxt_buf(eng), eng->d_start, eng->d_ctile_begin, eng->d_tile_heavy, eng->d_tile_order,
                        eng->collide_parcels, eng->cap, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
@@ -1040,6 +1201,7 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     if (inject_next) (void)hipStreamWaitEvent(eng->stream, eng->ev_injected, 0);
     launch_commit(eng->d_S, advance, inject_next, eng->stream);
     eng->cur = 1 - eng->cur;
+    if (inject_after) enqueue_inject(eng, 0);
     if (do_sample) eng->n_sampled += 1;
 }
 
@@ -1504,6 +1666,8 @@ extern "C" int hs_dsmc_batch_create(hs_dsmc_engine* const* engines, int32_t coun
             return bfail(nullptr, HS_E_INVALID, "the engines of a batch must be on one device");
         for (int32_t f = 0; f < e; ++f)
             if (engines[f] == engines[e]) return bfail(nullptr, HS_E_INVALID, "an engine appears twice in the batch");
+        if (!engines[e]->reactions.empty())
+            return bfail(nullptr, HS_E_UNSUPPORTED, "engines with chemical reactions cannot join a one-launch batch");
     }
     hs_dsmc_batch* b = new hs_dsmc_batch();
     b->engines.assign(engines, engines + count);

@@ -31,6 +31,7 @@ struct DevScalars {
     int32_t n_next;   // n_sorted plus the next step's inflow when injected ahead (installed by commit)
     int32_t ord[2];   // heavy / light tile counters of the collide order (cleared by commit)
     int32_t qn;       // flights queued for the wall pass of the move (cleared by commit)
+    int32_t nprod;    // dissociation products staged by the reacting collide (cleared after placement)
     unsigned long long counters[CNT_N];
 };
 
@@ -84,6 +85,29 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                     int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
                     double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
                     int general, int nc, uint16_t* big_list, hipStream_t st);
+// Parcels made by dissociations in one collide (dsmcCloud::addNewParcel):
+// staged in any order with their cell, their rank among the cell's
+// products and the dissociating parcel, then placed after the sorted
+// population in (cell, rank) order.  cap entries each.
+struct DevProducts {
+    int32_t* cell;
+    int32_t* rank;
+    int32_t* parent;  // sorted index of the parcel that dissociated
+    uint8_t* type;
+    float* ux;
+    float* uy;
+    float* uz;
+    int32_t* count;   // [nc] products per cell
+    int32_t* off;     // [nc+1] exclusive prefix of count
+    int32_t* list;    // [cap] sub-cell member lists of the reacting collide
+};
+// NTC with QK chemistry on the sorted population in dst, one thread per
+// cell; then the products' prefix, placement, the cells' sums (parcels, then
+// the cell's products) and the population update.
+void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
+                          double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
+                          DevProducts pr, int do_collide, int do_sample, int nc, ScanWs& ws, int64_t capacity,
+                          hipStream_t st);
 void launch_clear_queue(DevScalars* S, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
 // per-cell fill inputs (NULL: the uniform values): number densities

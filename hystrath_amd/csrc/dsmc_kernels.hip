@@ -2105,6 +2105,238 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                                                         samples, vol, S, do_collide, do_sample, nc, big_list);
 }
 
+// ---------------------------------------------------------------------------
+// collide with QK chemistry (dsmcReactions): one thread per cell, candidate
+// after candidate (noTimeCounter.C:81-339 with the reaction branch of
+// :250-300), on the sorted population in global memory
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void load_state(const DevParams& P, const DevParticles& d, int64_t i, int64_t cap,
+                                           PState& a) {
+    a.t = d.type ? (int)d.type[i] : 0;
+    a.u[0] = (double)d.ux[i]; a.u[1] = (double)d.uy[i]; a.u[2] = (double)d.uz[i];
+    a.er = d.erot ? (double)d.erot[i] : 0.0;
+#pragma unroll
+    for (int m = 0; m < HS_MAX_VIB_MODES; ++m) a.v[m] = 0;
+    for (int m = 0; m < P.nvib[a.t]; ++m) a.v[m] = d.vib[m * cap + i];
+}
+
+// after a reaction the type and every vibrational plane are rewritten too
+__device__ __forceinline__ void store_state(const DevParams& P, DevParticles& d, int64_t i, int64_t cap,
+                                            const PState& a, bool reacted) {
+    d.ux[i] = (float)a.u[0]; d.uy[i] = (float)a.u[1]; d.uz[i] = (float)a.u[2];
+    if (d.erot) d.erot[i] = (float)a.er;
+    const int nm = reacted ? P.max_modes : P.nvib[a.t];
+    for (int m = 0; m < nm; ++m) d.vib[m * cap + i] = (uint16_t)a.v[m];
+    if (reacted && d.type) d.type[i] = (uint8_t)a.t;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_collide_react(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                         const int32_t* __restrict__ start,
+                                                         const uint8_t* __restrict__ sub,
+                                                         double* __restrict__ sigma, double* __restrict__ rem,
+                                                         double* __restrict__ samples, const double* __restrict__ vol,
+                                                         DevScalars* __restrict__ S, DevProducts pr, int do_collide,
+                                                         int do_sample, int nc) {
+    const DevParams& P = *Pp;
+    const int64_t cap = P.capacity;
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    CellCtx cx = { 0, 0, 0 };
+    if (c < nc) {
+        const int32_t b = start[c], nC = start[c + 1] - b;
+        int nprod = 0;
+        double ncoll = 0.0, sep = 0.0;
+        if (do_collide && nC > 1) {
+            // sub-cell member lists in ascending index order (noTimeCounter.C:114-138)
+            int scn[8], sco[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) scn[q] = 0;
+            for (int k = 0; k < nC; ++k) scn[sub[b + k]] += 1;
+            int acc = 0;
+#pragma unroll
+            for (int q = 0; q < 8; ++q) { sco[q] = acc; acc += scn[q]; }
+            {
+                int fill[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) fill[q] = sco[q];
+                for (int k = 0; k < nC; ++k) pr.list[b + fill[sub[b + k]]++] = k;
+            }
+            double smax;
+            const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+            double smax_cur = smax;
+            cx.cand += (unsigned long long)ncand;
+            const uint32_t step = S->step;
+            for (long long k = 0; k < ncand; ++k) {
+                PhiloxStream r;
+                r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
+                const int pp = random_label(r, 0, nC - 1);
+                const int q = sub[b + pp];
+                const int nsc = scn[q];
+                int qq = -1, guard = 0;
+                if (nsc > 1) {
+                    do { qq = pr.list[b + sco[q] + random_label(r, 0, nsc - 1)]; }
+                    while (pp == qq && ++guard < HS_MAX_REDRAW);
+                } else {
+                    do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
+                }
+                if (pp == qq) continue;
+                const int64_t ip = (int64_t)b + pp, iq = (int64_t)b + qq;
+                PState a, bb;
+                load_state(P, dst, ip, cap, a);
+                load_state(P, dst, iq, cap, bb);
+                if (P.charge[a.t] == -1 && P.charge[bb.t] == -1) continue;
+                const double stcr = sigma_tcr(P, a.t, bb.t, a.u, bb.u);
+                if (stcr > smax_cur) smax_cur = stcr;
+                if (stcr / smax > r.u()) {
+                    PState prod;
+                    int from_q = 0;
+                    const bool reacted = collide_or_react(P, a, bb, r, prod, &from_q);
+                    store_state(P, dst, ip, cap, a, reacted);
+                    store_state(P, dst, iq, cap, bb, reacted);
+                    if (prod.t >= 0) {
+                        const int slot = atomicAdd(&S->nprod, 1);
+                        if (slot < cap) {
+                            pr.cell[slot] = c;
+                            pr.rank[slot] = nprod;
+                            pr.parent[slot] = (int32_t)(from_q ? iq : ip);
+                            pr.type[slot] = (uint8_t)prod.t;
+                            pr.ux[slot] = (float)prod.u[0];
+                            pr.uy[slot] = (float)prod.u[1];
+                            pr.uz[slot] = (float)prod.u[2];
+                        }
+                        nprod += 1;
+                    }
+                    ncoll += 1.0;
+                    if (P.flags & HS_FLAG_COLL_SEP) {
+                        const double ddx = dst.px[ip] - dst.px[iq], ddy = dst.py[ip] - dst.py[iq],
+                                     ddz = dst.pz[ip] - dst.pz[iq];
+                        sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+                    }
+                    cx.colls += 1;
+                }
+            }
+            sigma[c] = smax_cur;
+        }
+        pr.count[c] = nprod;
+        if (do_sample) {
+            unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
+            unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
+        }
+    }
+    add_counter(S, CNT_CAND, cx.cand);
+    add_counter(S, CNT_COLL, cx.colls);
+    add_counter(S, CNT_TILE_OVF, cx.ovf);
+}
+
+// the staged products after the sorted population, in (cell, rank) order
+// (dsmcCloud::addNewParcel, dsmcCloud.C:396-432): at the dissociating
+// parcel's position, with no internal energy; beyond the capacity dropped
+__global__ __launch_bounds__(BLOCK) void k_place_products(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                          const DevScalars* __restrict__ S, DevProducts pr) {
+    const DevParams& P = *Pp;
+    const int64_t cap = P.capacity;
+    const int64_t base = S->n_sorted;
+    const int64_t np = S->nprod < cap ? S->nprod : cap;
+    for (int64_t e = (int64_t)blockIdx.x * BLOCK + threadIdx.x; e < np; e += (int64_t)gridDim.x * BLOCK) {
+        const int c = pr.cell[e];
+        const int64_t j = base + pr.off[c] + pr.rank[e];
+        if (j >= cap) continue;
+        const int64_t par = pr.parent[e];
+        dst.px[j] = dst.px[par]; dst.py[j] = dst.py[par]; dst.pz[j] = dst.pz[par];
+        dst.ux[j] = pr.ux[e]; dst.uy[j] = pr.uy[e]; dst.uz[j] = pr.uz[e];
+        dst.cell[j] = c;
+        if (dst.erot) dst.erot[j] = 0.0f;
+        for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + j] = 0;
+        if (dst.type) dst.type[j] = pr.type[e];
+    }
+}
+
+// The cell's sums (dsmcVolFields.C:1115-1239) after its reactions: per
+// species, its parcels and then the products it made this step, one thread
+// adding them in order (the oracle's order with reactions).
+__global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                        const int32_t* __restrict__ start,
+                                                        const DevScalars* __restrict__ S, DevProducts pr,
+                                                        double* __restrict__ samples, int nc) {
+    const DevParams& P = *Pp;
+    const int c = blockIdx.x * BLOCK + threadIdx.x;
+    if (c >= nc) return;
+    const int64_t cap = P.capacity;
+    const int64_t b = start[c];
+    const int nC = start[c + 1] - start[c];
+    const int64_t b2 = (int64_t)S->n_sorted + pr.off[c];
+    int n2 = pr.count[c];
+    if (b2 + n2 > cap) n2 = b2 < cap ? (int)(cap - b2) : 0;
+    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    for (int s = 0; s < P.ns; ++s) {
+        const double m = P.mass[s];
+        const int nvib = P.nvib[s];
+        double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
+        double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+        double h[HS_HF_COUNT];
+        for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
+        for (int k = 0; k < nC + n2; ++k) {
+            const int64_t i = k < nC ? b + k : b2 + (k - nC);
+            if ((int)dst.type[i] != s) continue;
+            const double ux = (double)dst.ux[i], uy = (double)dst.uy[i], uz = (double)dst.uz[i];
+            const double lke = m * (ux * ux + uy * uy + uz * uz);
+            const double er = dst.erot ? (double)dst.erot[i] : 0.0;
+            aN += 1.0;
+            aX += m * ux;
+            aY += m * uy;
+            aZ += m * uz;
+            aC += lke;
+            aR += er;
+            double evs = 0.0;
+            for (int mo = 0; mo < nvib; ++mo) {
+                const double e = (double)dst.vib[mo * cap + i] * HS_KB * P.theta_v[s][mo];
+                ev[mo] += e;
+                evs += e;
+            }
+            if (hf) {
+                const double eint = er + evs;
+                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
+            }
+        }
+        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+        unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
+        unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
+        unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
+        for (int mo = 0; mo < HS_MAX_VIB_MODES; ++mo)
+            unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
+        if (hf) {
+            const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
+            for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
+        }
+    }
+}
+
+// the population after this step's products; the staging counter cleared
+__global__ void k_products_commit(DevScalars* S, const int32_t* __restrict__ off, int nc, int64_t cap) {
+    const int64_t nn = (int64_t)S->n_sorted + off[nc];
+    S->n_sorted = (int32_t)(nn > cap ? cap : nn);
+    if (nn > cap) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - cap));
+    S->nprod = 0;
+}
+
+void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
+                          double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
+                          DevProducts pr, int do_collide, int do_sample, int nc, ScanWs& ws, int64_t capacity,
+                          hipStream_t st) {
+    const int g = (nc + BLOCK - 1) / BLOCK;
+    k_collide_react<<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr, do_collide, do_sample,
+                                         nc);
+    launch_scan(pr.count, pr.off, nc, SCAN_PLAIN, ws, S, capacity, st);
+    k_place_products<<<grid_for(capacity, 1024), BLOCK, 0, st>>>(P, dst, S, pr);
+    if (do_sample) k_sample_react<<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
+    k_products_commit<<<1, 1, 0, st>>>(S, pr.off, nc, capacity);
+}
+
 // publish the sorted population (with the next step's inflow when it was
 // injected ahead); optionally advance the step counter
 __device__ __forceinline__ void commit_body(DevScalars* S, int advance, int with_next) {

@@ -238,3 +238,31 @@ def test_no_reaction_model_or_no_splitting_is_a_plain_collision():
     e1 = (_energy(c, iN2, r["up"][0] - 0.5 * np.array([cr, 0, 0]), r["erot_p"][0], r["vib_p"][0])
           + _energy(c, iN2, r["uq"][0] - 0.5 * np.array([cr, 0, 0]), r["erot_q"][0], r["vib_q"][0]))
     assert abs(e1 - e0) < 1e-11 * e0
+
+
+def _create_error(case):
+    import ctypes as C
+    from hystrath_amd import engine
+    L = engine.load_library()
+    h = C.c_void_p()
+    rc = L.hs_dsmc_create(C.byref(case.cfg), C.byref(h))
+    return rc, L.hs_dsmc_create_error().decode()
+
+
+@pytest.mark.parametrize("edit,code,msg", [
+    # VariableSoftSphere scattering has its own post-collision velocities
+    (lambda c: setattr(c.cfg, "collision_model", abi.HS_COLL_LB_VSS), abi.HS_E_UNSUPPORTED, "VariableHardSphere"),
+    # an atom given dissociation products (dissociationQK.C:88-100)
+    (lambda c: c._reactions[3].diss_products[1].__setitem__(0, 4) or c._reactions[3].diss_products[1].__setitem__(1, 4),
+     abi.HS_E_INVALID, "not a molecule"),
+    # a diatomic split into a molecule (dissociationQK.C:141-152)
+    (lambda c: c._reactions[0].diss_products[0].__setitem__(0, 2), abi.HS_E_INVALID, "must be an atom"),
+    # two models for one pair (dsmcReactions.C:152-158)
+    (lambda c: setattr(c._reactions[2], "reactants", (abi.C.c_int32 * 2)(1, 0)), abi.HS_E_INVALID, "more than one"),
+    (lambda c: setattr(c._reactions[1], "model", 7), abi.HS_E_INVALID, "reactionModel"),
+])
+def test_engine_rejects_reaction_tables_the_reference_rejects(edit, code, msg):
+    c = cases.heat_bath_5species(fn=1e5)
+    edit(c)
+    rc, err = _create_error(c)
+    assert rc == code and msg in err, (rc, err)

@@ -1,0 +1,104 @@
+"""QK chemistry on the GPU against the oracle, bit for bit.
+
+The reacting collide (one thread per cell, the reaction branch of
+noTimeCounter.C:250-300 with dissociationQK, exchangeQK and
+dissociationExchangeQK) must leave every parcel, the dissociation products
+(placed after the sorted population in cell order, candidate order within
+the cell, dsmcCloud::addNewParcel at dsmcCloud.C:396-432), sigmaTcRMax, the
+remainders, the cell sums (which include the step's products,
+dsmcCloud.C:904-907) and the counters equal to the oracle's
+(tests/test_chemistry.py pins the oracle to the reference's heatBath-5species
+run).  Cases: the tutorial's one-cell heat bath at fewer parcels, the same
+box cut into 27 cells, and reacting air past the cylinder with freestream
+inflow (whose next-step inflow is appended after the products), each
+through eager steps and graph-replayed pairs; and the stage entry points.
+"""
+import numpy as np
+import pytest
+
+import helpers as H
+import oracle_bind as ob
+from hystrath_amd import abi, cases
+from hystrath_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+
+def _fill_bath(obj, c):
+    obj.fill(c.init_n, c.init_T)
+
+
+def _fill_air(obj, c):
+    # a hot start: the domain at 30,000 K, so reactions begin at once
+    obj.fill(c.n_in, 30000.0, u=(c.U_inf, 0.0, 0.0))
+
+
+def _pair(c, fill):
+    eng = Engine(c)
+    st = ob.OracleState(c)
+    fill(eng, c)
+    fill(st, c)
+    return eng, st
+
+
+def _check(eng, st, label):
+    H.assert_same_particles(eng.download(), st.particles(), label)
+    sig, rem = eng.cell_state()
+    np.testing.assert_array_equal(sig, st.sigma, err_msg=f"{label}: sigmaTcRMax")
+    np.testing.assert_array_equal(rem, st.rem, err_msg=f"{label}: remainder")
+
+
+def _check_end(eng, st, steps, label):
+    s, n = eng.samples()
+    np.testing.assert_array_equal(s, st.sample_array(), err_msg=f"{label}: cell sums")
+    assert n == steps
+    c, oc = eng.counters(), st.counters()
+    assert (c["candidates"], c["collisions"], c["deleted"], c["injected"], c["wall_hits"]) == tuple(oc[:5]), label
+    assert c["capacity_overflow"] == 0 and c["tile_overflow"] == 0
+
+
+@pytest.mark.parametrize("which", ["bath1", "bath27", "air"])
+def test_reacting_steps_match_the_oracle(which):
+    hf = abi.HS_FLAG_SAMPLE | abi.HS_FLAG_HEATFLUX | abi.HS_FLAG_COLL_SEP
+    c, fill, calls = {
+        "bath1": (cases.heat_bath_5species(fn=2000.0, flags=hf), _fill_bath, (1, 1, 1, 4, 3)),
+        "bath27": (cases.heat_bath_5species(fn=400.0, n_side=3), _fill_bath, (1, 2, 5)),
+        "air": (cases.air_cylinder(nx=24, ny=24, ppc=12, seed=3, flags=hf), _fill_air, (1, 1, 3, 4, 2)),
+    }[which]
+    eng, st = _pair(c, fill)
+    n0 = st.n
+    total = 0
+    print(f"[chem] {which}: {n0} parcels, calls {calls}", flush=True)
+    for r, n in enumerate(calls):
+        eng.evolve(n)
+        st.evolve(n)
+        total += n
+        _check(eng, st, f"{which}/call{r}")
+    _check_end(eng, st, total, which)
+    ty = st.particles()["type"]
+    # dissociations happened (N or O parcels), and in the air case the inflow
+    if which == "air":
+        assert st.counters()[3] > 0
+        np.testing.assert_array_equal(eng.inflow()[: st.inflow.size], st.inflow)
+    else:
+        assert st.n > n0
+    assert np.count_nonzero(ty >= 3) > 10
+    eng.close()
+
+
+def test_stage_entry_points_with_reactions():
+    """hs_dsmc_sort + hs_dsmc_collide (buildCellOccupancy, then
+    noTimeCounter::collide with its reactions) equal the oracle's sort and
+    collide, products included; the step counter advances separately."""
+    c = cases.heat_bath_5species(fn=1000.0, n_side=2)
+    eng, st = _pair(c, _fill_bath)
+    for k in range(3):
+        eng.collide(sample=True)
+        st.sort()
+        st.collide(sample=True)
+        _check(eng, st, f"stage{k}")
+        eng.step = eng.step + 1
+        st.step = st.step + 1
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    assert st.n > int(sum(c.init_n) * 1e-15 / c.cfg.fn)
+    eng.close()
