@@ -66,10 +66,44 @@ def cylinder_case(seed=20251202):
                                flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
 
 
+# Reacting air (workload "air"): particles carry rotational energy (4 B),
+# one vibrational level (2 B) and a species byte; the collide stage is the
+# reacting NTC (its sub-cell lists, 8 B read+write), the products' placement
+# and the cells' sums of five species read in a second pass.
+STAGE_BYTES_AIR = {
+    "move": dict(per_particle=72.0, per_cell=0.0),
+    "scan": dict(per_particle=0.0, per_cell=12.0),
+    "scatter": dict(per_particle=12.0, per_cell=0.0),
+    # read perm + pos + vel + erot + vib + type 47, write pos + vel + cell + sub + erot + vib + type 48
+    "gather": dict(per_particle=95.0, per_cell=4.0),
+    # read vel + erot + vib + type + sub 20, write vel + erot + vib + type 19, list 8, sums read 19;
+    # per cell: start, sigmaTcRMax and remainder read+write, volume, product count and
+    # offset, the 2 + 5*10 sums read+write
+    "collide": dict(per_particle=66.0, per_cell=4 + 16 + 16 + 8 + 16 + 52 * 16),
+    "inject": dict(per_particle=0.0, per_cell=0.0),
+}
+
+
+def air_case(seed=20251203):
+    from hystrath_amd import abi, cases
+
+    return cases.air_cylinder(nx=1000, ny=1000, ppc=30, seed=seed, flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_WALL)
+
+
 def build_case(workload):
     from hystrath_amd import abi, cases
 
-    if workload == "cylinder":
+    if workload == "air":
+        case = air_case()
+        desc = {"workload": "air5_qk_cylinder_7kms_1000x1000", "cells": case.ncells, "ppc_freestream": 30,
+                "n_inf": 1e21, "T_inf": case.T_inf, "U_inf": case.U_inf, "T_wall": 1000.0, "radius": 0.01,
+                "dt": case.cfg.dt, "species": "N2 O2 NO N O", "reactions": "heatBath-5species chemReactDict (12 QK)",
+                "collision_model": "LarsenBorgnakkeVariableHardSphere", "partner_selection": "noTimeCounter",
+                "sampling": "every step, cells and wall faces"}
+
+        def fill(e):
+            e.fill(case.n_in, case.T_inf, u=(case.U_inf, 0.0, 0.0), fill_id=0)
+    elif workload == "cylinder":
         case = cylinder_case()
         desc = {"workload": "cylinder_mach10_argon_400x400", "cells": case.ncells, "ppc_freestream": 25,
                 "n_inf": case.n_inf, "T_inf": case.T_inf, "U_inf": round(case.U_inf, 3), "T_wall": 500.0,
@@ -397,7 +431,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=100, help="untimed steps after the development steps")
     ap.add_argument("--develop", type=int, default=4000,
                     help="development steps in set-up, before the warm-up (fixed, whatever --warmup is)")
-    ap.add_argument("--workload", choices=["cylinder", "box", "batch64"], default="cylinder",
+    ap.add_argument("--workload", choices=["cylinder", "box", "batch64", "air"], default="cylinder",
                     help="batch64: 64 uneven cylinder cases stepped together (hs_dsmc_evolve_batch)")
     ap.add_argument("--profile-steps", type=int, default=200, help="instrumented steps for kernel durations")
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
@@ -434,7 +468,7 @@ def main():
     eng = Engine(case)
     fill(eng)
     n_fill = eng.count()
-    develop = args.develop if args.workload == "cylinder" else 0
+    develop = {"cylinder": args.develop, "air": min(args.develop, 2000)}.get(args.workload, 0)
     log(f"[bench] filled {n_fill} particles; {develop} development steps, then warm-up {args.warmup} steps")
     run_evolve(eng, develop + args.warmup, what="development + warm-up")
     eng.synchronize()
@@ -468,7 +502,7 @@ def main():
         if launches == 0:
             continue
         avg_ms = ms / launches
-        b = STAGE_BYTES[name]
+        b = (STAGE_BYTES_AIR if args.workload == "air" else STAGE_BYTES)[name]
         nbytes = b["per_particle"] * n_step + b["per_cell"] * case.ncells
         stages[name] = {"avg_ms": round(avg_ms, 5), "launches": launches,
                         "algorithmic_bytes": int(nbytes),

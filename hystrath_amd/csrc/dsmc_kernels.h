@@ -106,8 +106,8 @@ struct DevProducts {
 // the cell's products) and the population update.
 void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
                           double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
-                          DevProducts pr, int do_collide, int do_sample, int nc, ScanWs& ws, int64_t capacity,
-                          hipStream_t st);
+                          DevProducts pr, int do_collide, int do_sample, int nc, int ns, ScanWs& ws,
+                          int64_t capacity, hipStream_t st);
 void launch_clear_queue(DevScalars* S, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
 // per-cell fill inputs (NULL: the uniform values): number densities

@@ -2130,6 +2130,51 @@ __device__ __forceinline__ void store_state(const DevParams& P, DevParticles& d,
     if (reacted && d.type) d.type[i] = (uint8_t)a.t;
 }
 
+// The cell's sums without the heat-flux terms in one pass: parcels
+// b .. b+nC-1, then b2 .. b2+n2-1, each added to its species' accumulators
+// (every species' sums keep the parcel order: the same values as a pass per
+// species).  MAXS bounds the species count.
+template <int MAXS>
+__device__ __forceinline__ void sample_cell_react(const DevParams& P, const DevParticles& d, int64_t b, int nC,
+                                                  int64_t b2, int n2, double* __restrict__ samples, int nc, int c) {
+    const int64_t cap = P.capacity;
+    double acc[MAXS][HS_SS_COUNT];
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s)
+#pragma unroll
+        for (int q = 0; q < HS_SS_COUNT; ++q) acc[s][q] = 0.0;
+    for (int k = 0; k < nC + n2; ++k) {
+        const int64_t i = k < nC ? b + k : b2 + (k - nC);
+        const int t = (int)d.type[i];
+        const double ux = (double)d.ux[i], uy = (double)d.uy[i], uz = (double)d.uz[i];
+        const double er = d.erot ? (double)d.erot[i] : 0.0;
+        int lv[HS_MAX_VIB_MODES] = { 0, 0, 0 };
+        for (int mo = 0; mo < P.nvib[t]; ++mo) lv[mo] = d.vib[mo * cap + i];
+#pragma unroll
+        for (int s = 0; s < MAXS; ++s) {
+            if (t != s) continue;
+            const double m = P.mass[s];
+            acc[s][HS_SS_N] += 1.0;
+            acc[s][HS_SS_MUX] += m * ux;
+            acc[s][HS_SS_MUY] += m * uy;
+            acc[s][HS_SS_MUZ] += m * uz;
+            acc[s][HS_SS_MCC] += m * (ux * ux + uy * uy + uz * uz);
+            acc[s][HS_SS_EROT] += er;
+#pragma unroll
+            for (int mo = 0; mo < HS_MAX_VIB_MODES; ++mo)
+                if (mo < P.nvib[s]) acc[s][HS_SS_EVIB0 + mo] += (double)lv[mo] * HS_KB * P.theta_v[s][mo];
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < MAXS; ++s) {
+        if (s >= P.ns) break;
+        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+#pragma unroll
+        for (int q = 0; q < HS_SS_COUNT; ++q) unsafeAtomicAdd(&samples[(base + q) * nc + c], acc[s][q]);
+    }
+}
+
+template <int MAXS>
 __global__ __launch_bounds__(BLOCK) void k_collide_react(const DevParams* __restrict__ Pp, DevParticles dst,
                                                          const int32_t* __restrict__ start,
                                                          const uint8_t* __restrict__ sub,
@@ -2220,6 +2265,10 @@ __global__ __launch_bounds__(BLOCK) void k_collide_react(const DevParams* __rest
         if (do_sample) {
             unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
             unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
+            // a cell that made no products is summed here, while its parcels
+            // are in cache; the others after their products are placed
+            if (nprod == 0 && !(P.flags & HS_FLAG_HEATFLUX))
+                sample_cell_react<MAXS>(P, dst, b, nC, 0, 0, samples, nc, c);
         }
     }
     add_counter(S, CNT_CAND, cx.cand);
@@ -2252,7 +2301,10 @@ __global__ __launch_bounds__(BLOCK) void k_place_products(const DevParams* __res
 
 // The cell's sums (dsmcVolFields.C:1115-1239) after its reactions: per
 // species, its parcels and then the products it made this step, one thread
-// adding them in order (the oracle's order with reactions).
+// adding them in order (the oracle's order with reactions).  Without the
+// heat-flux terms the reacting collide has summed the cells that made no
+// products.
+template <int MAXS>
 __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restrict__ Pp, DevParticles dst,
                                                         const int32_t* __restrict__ start,
                                                         const DevScalars* __restrict__ S, DevProducts pr,
@@ -2260,13 +2312,18 @@ __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restr
     const DevParams& P = *Pp;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
+    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    if (!hf && pr.count[c] == 0) return;
     const int64_t cap = P.capacity;
     const int64_t b = start[c];
     const int nC = start[c + 1] - start[c];
     const int64_t b2 = (int64_t)S->n_sorted + pr.off[c];
     int n2 = pr.count[c];
     if (b2 + n2 > cap) n2 = b2 < cap ? (int)(cap - b2) : 0;
-    const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
+    if (!hf) {
+        sample_cell_react<MAXS>(P, dst, b, nC, b2, n2, samples, nc, c);
+        return;
+    }
     for (int s = 0; s < P.ns; ++s) {
         const double m = P.mass[s];
         const int nvib = P.nvib[s];
@@ -2326,14 +2383,21 @@ __global__ void k_products_commit(DevScalars* S, const int32_t* __restrict__ off
 
 void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
                           double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
-                          DevProducts pr, int do_collide, int do_sample, int nc, ScanWs& ws, int64_t capacity,
-                          hipStream_t st) {
+                          DevProducts pr, int do_collide, int do_sample, int nc, int ns, ScanWs& ws,
+                          int64_t capacity, hipStream_t st) {
     const int g = (nc + BLOCK - 1) / BLOCK;
-    k_collide_react<<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr, do_collide, do_sample,
-                                         nc);
+    if (ns <= 5)
+        k_collide_react<5><<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr, do_collide,
+                                                 do_sample, nc);
+    else
+        k_collide_react<HS_MAX_SPECIES><<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr,
+                                                              do_collide, do_sample, nc);
     launch_scan(pr.count, pr.off, nc, SCAN_PLAIN, ws, S, capacity, st);
     k_place_products<<<grid_for(capacity, 1024), BLOCK, 0, st>>>(P, dst, S, pr);
-    if (do_sample) k_sample_react<<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
+    if (do_sample) {
+        if (ns <= 5) k_sample_react<5><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
+        else k_sample_react<HS_MAX_SPECIES><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
+    }
     k_products_commit<<<1, 1, 0, st>>>(S, pr.off, nc, capacity);
 }
 
