@@ -38,11 +38,32 @@ def _species_block(name, p):
     return "\n".join(lines)
 
 
+def _reactions_text(reactions):
+    """chemReactDict entries from (reactionModel, reactants,
+    dissociationProducts, exchange) tuples (see cases.HEATBATH_REACTIONS)."""
+    out = ""
+    for k, (model, reac, diss, exch) in enumerate(reactions):
+        body = f"        reactionModel {model};\n        reactants ({' '.join(reac)});\n        allowSplitting yes;\n"
+        if model != "exchangeQK":
+            lists = " ".join("(" + " ".join(d) + ")" for d in diss) if len(diss) == 2 else \
+                "(" + " ".join(diss[0]) + ") ()"
+            body += f"        dissociationQKProperties\n        {{\n            dissociationProducts ({lists});\n        }}\n"
+        if exch is not None:
+            prods, heat, a, b = exch
+            body += (f"        exchangeQKProperties\n        {{\n            exchangeProducts ({' '.join(prods)});\n"
+                     f"            heatOfReactionExchange {heat!r};\n            aCoeff {a!r};\n"
+                     f"            bCoeff {b!r};\n        }}\n")
+        out += f"    reaction{k}\n    {{\n{body}    }}\n"
+    return out
+
+
 def write_case(root, *, species, n, lo, hi, patches, fn, dt, end_time, write_interval, model="VariableHardSphere",
-               coeffs=None, seed=1, fields=None, init=None, inflow=None, cylinder=None, n_terminal=50):
+               coeffs=None, seed=1, fields=None, init=None, inflow=None, cylinder=None, n_terminal=50,
+               reactions=None):
     """patches: {side: (name, type, model, props)} for side in xmin..zmax;
     type is the polyPatch type (wall, patch, cyclic, empty), model the
-    dsmcPatchBoundaries model name or None."""
+    dsmcPatchBoundaries model name or None; reactions: chemReactDict entries
+    as in cases.HEATBATH_REACTIONS."""
     sp_list = " ".join(species)
     mol = "\n".join(_species_block(k, v) for k, v in species.items())
     coeff_txt = ""
@@ -112,7 +133,8 @@ def write_case(root, *, species, n, lo, hi, patches, fn, dt, end_time, write_int
                  f"        vibrationalTemperature {c.get('Tvib', c['T'])!r};\n        electronicTemperature 0;\n"
                  f"        velocity ({' '.join(repr(x) for x in c.get('U', (0, 0, 0)))});\n    }}\n")
     _w(os.path.join(root, "system/dsmcInitialiseDict"), "dsmcInitialiseDict", f"configurations\n(\n{cfgs});\n")
-    _w(os.path.join(root, "system/chemReactDict"), "chemReactDict", "reactions\n(\n);\n")
+    _w(os.path.join(root, "system/chemReactDict"), "chemReactDict",
+       f"reactions\n(\n{_reactions_text(reactions or [])});\n")
     if cylinder:
         props = "".join(f"    {k} {v};\n" for k, v in cylinder.get("props", {}).items())
         _w(os.path.join(root, "system/embeddedCylinderDict"), "embeddedCylinderDict",
@@ -128,6 +150,34 @@ O2 = dict(mass=53.12e-27, diameter=4.07e-10, omega=0.77, alpha=1.4, rotationalDe
           nVibrationalModes=1, characteristicVibrationalTemperature=[2256], dissociationTemperature=59500,
           Zref=[17900], referenceTempForZref=[2256], charge=0)
 AR = dict(mass=6.63e-26, diameter=4.17e-10, omega=0.81, alpha=1.0)
+# heatBath-5species/constant/dsmcProperties
+NO = dict(mass=49.81e-27, diameter=4.2e-10, omega=0.79, alpha=1.0, rotationalDegreesOfFreedom=2,
+          nVibrationalModes=1, characteristicVibrationalTemperature=[2719], dissociationTemperature=75500,
+          Zref=[1400], referenceTempForZref=[2719], charge=0)
+N = dict(mass=23.25e-27, diameter=3.00e-10, omega=0.8, alpha=1.0, charge=0)
+O = dict(mass=26.56e-27, diameter=3.00e-10, omega=0.8, alpha=1.0, charge=0)
+
+
+def heat_bath_case(root, fn=100.0, end_time=1e-5, write_interval=1e-9, seed=31, n_terminal=100):
+    """The heatBath-5species tutorial: one (1e-5 m)^3 cell with specular
+    walls, LB-VHS with relaxation collision numbers 1, five-species air and
+    the twelve QK reactions of its chemReactDict, filled with N2 and O2 at
+    30,000 K; fn scales the parcel count (the tutorial's is 100)."""
+    from .cases import HEATBATH_REACTIONS
+
+    spec = {"dsmcSpecularWallPatch": None}
+    wall = ("fixedWalls", "wall", "dsmcSpecularWallPatch", spec.get("dsmcSpecularWallPatch"))
+    L = 1e-5
+    return write_case(
+        root, species={"N2": N2, "O2": O2, "NO": NO, "N": N, "O": O}, n=(1, 1, 1), lo=(0, 0, 0), hi=(L, L, L),
+        patches={s: wall for s in ("xmin", "xmax", "ymin", "ymax", "zmin", "zmax")}, fn=fn, dt=1e-9,
+        end_time=end_time, write_interval=write_interval, model="LarsenBorgnakkeVariableHardSphere",
+        coeffs={"rotationalRelaxationCollisionNumber": 1.0, "vibrationalRelaxationCollisionNumber": 1.0,
+                "electronicRelaxationCollisionNumber": 1.0}, seed=seed,
+        fields={"N2": ["N2"], "O2": ["O2"], "NO": ["NO"], "N": ["N"], "O": ["O"],
+                "mixture": ["N2", "O2", "NO", "N", "O"]},
+        init=[{"n": {"N2": 1.21753030168e22, "O2": 3.23647295384e21, "NO": 0.0, "N": 0.0, "O": 0.0},
+               "T": 30000.0}], n_terminal=n_terminal, reactions=HEATBATH_REACTIONS)
 
 
 def couette_case(root, end_time=2e-4, write_interval=1e-4, seed=11, n_terminal=50):

@@ -308,6 +308,67 @@ std::vector<double> cellVolumes(const BlockMesh& m, bool cylinder, double cx, do
 }
 
 // ---------------------------------------------------------------------------
+// chemReactDict (dsmcReactions.C:76-116): "reactions ( name { ... } ... );",
+// each entry selected by reactionModel (dsmcReaction.C:106-135) with its
+// <model>Properties sub-dictionaries
+// ---------------------------------------------------------------------------
+void dsmcCloud::readReactions(const dictionary& cr) {
+    if (!cr.found("reactions")) return;
+    const auto& L = cr.list("reactions");
+    auto typeId = [&](const std::string& w, const std::string& where) {
+        for (size_t k = 0; k < typeIdList_.size(); ++k)
+            if (typeIdList_[k] == w) return (int)k;
+        throw FatalError(where + ": Cannot find type id: " + w);
+    };
+    for (size_t i = 0; i + 1 < L.size(); i += 2) {
+        const std::string name = L[i]->text;
+        if (L[i + 1]->kind != Node::Dict) throw FatalError(cr.name() + ": reaction " + name + " needs a dictionary");
+        const dictionary rd(L[i + 1], cr.name() + "/reactions/" + name);
+        const std::string model = rd.word("reactionModel");
+        hs_reaction x{};
+        if (model == "dissociationQK") x.model = HS_REACT_DISSOCIATION_QK;
+        else if (model == "exchangeQK") x.model = HS_REACT_EXCHANGE_QK;
+        else if (model == "dissociationExchangeQK") x.model = HS_REACT_DISSOCIATION_EXCHANGE_QK;
+        else
+            throw FatalError(rd.name() + ": unknown dsmc reaction model type " + model +
+                             "; valid types are: dissociationQK exchangeQK dissociationExchangeQK");
+        const auto reac = rd.wordList("reactants");
+        if (reac.size() != 2) throw FatalError(rd.name() + ": There should be two reactants");
+        for (int r = 0; r < 2; ++r) x.reactants[r] = typeId(reac[r], rd.name());
+        x.allow_splitting = rd.switchOrDefault("allowSplitting", true) ? 1 : 0;
+        for (int r = 0; r < 2; ++r) x.diss_products[r][0] = x.diss_products[r][1] = -1;
+        x.exch_products[0] = x.exch_products[1] = -1;
+        if (x.model != HS_REACT_EXCHANGE_QK) {
+            const dictionary dp = rd.subDict("dissociationQKProperties");
+            const auto& tok = dp.lookup("dissociationProducts");
+            if (tok.size() != 1 || tok[0]->kind != Node::List || tok[0]->items.size() != 2)
+                throw FatalError(dp.name() + ": There should be two lists of dissociation products "
+                                             "(NB: a list can be left empty)");
+            for (int r = 0; r < 2; ++r) {
+                const NodePtr& l = tok[0]->items[r];
+                if (l->kind != Node::List) throw FatalError(dp.name() + ": dissociationProducts needs lists");
+                if (l->items.empty()) continue;
+                if (l->items.size() != 2)
+                    throw FatalError(dp.name() + ": There should be 2 dissociation products for reactant " + reac[r]);
+                for (int k = 0; k < 2; ++k) x.diss_products[r][k] = typeId(l->items[k]->text, dp.name());
+            }
+        }
+        if (x.model != HS_REACT_DISSOCIATION_QK) {
+            const dictionary ep = rd.subDict("exchangeQKProperties");
+            const auto prods = ep.wordList("exchangeProducts");
+            if (prods.size() != 2) throw FatalError(ep.name() + ": There should be two exchange products");
+            for (int k = 0; k < 2; ++k) x.exch_products[k] = typeId(prods[k], ep.name());
+            x.heat_exchange = ep.scalar("heatOfReactionExchange");
+            x.a_coeff = ep.scalar("aCoeff");
+            x.b_coeff = ep.scalar("bCoeff");
+        }
+        reactions_.push_back(x);
+    }
+    cfg_.reactions = reactions_.empty() ? nullptr : reactions_.data();
+    cfg_.n_reactions = (int32_t)reactions_.size();
+}
+
+// ---------------------------------------------------------------------------
 // construction
 // ---------------------------------------------------------------------------
 dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDir) {
@@ -365,12 +426,7 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
     for (int sd = 0; sd < 6; ++sd)
         walls = walls || cfg_.patch[sd].model == HS_PATCH_DIFFUSE || cfg_.patch[sd].model == HS_PATCH_SPECULAR;
     if (walls) cfg_.flags |= HS_FLAG_WALL;
-    if (isFile(case_ + "/system/chemReactDict")) {
-        const dictionary cr = readDictionary(case_ + "/system/chemReactDict");
-        if (cr.found("reactions") && !cr.list("reactions").empty())
-            throw FatalError(cr.name() + ": QK chemistry (dsmcReactions) is not available in this engine; "
-                                         "run without reactions or use the reference solver");
-    }
+    if (isFile(case_ + "/system/chemReactDict")) readReactions(readDictionary(case_ + "/system/chemReactDict"));
 
     // capacity: the state to restart from, or the fill of dsmcInitialiseDict
     double expect = 0.0, cloudTime = 0.0;

@@ -94,6 +94,7 @@ private:
     void readProperties(const dictionary& d);
     void readBoundaries(const dictionary& d);
     void readFields(const dictionary& d);
+    void readReactions(const dictionary& d);
     void check(int rc, const char* what) const;
     // latest time directory of the case holding <time>/<rel>, and its time
     std::string latestTimeWith(const std::string& rel, double& t) const;
@@ -103,6 +104,7 @@ private:
     BlockMesh mesh_;
     std::vector<double> vol_;
     std::vector<std::string> typeIdList_;
+    std::vector<hs_reaction> reactions_;  // system/chemReactDict
     std::vector<FieldModel> fields_;
     hs_dsmc_engine* eng_ = nullptr;
     int nTerminalOutputs_ = 1;

@@ -46,8 +46,9 @@ def test_reads_every_reference_tutorial_dictionary():
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
 def test_reference_case_outcomes(tmp_path):
-    """couette_N2-O2 is accepted up to the device; the others stop with the
-    reason they are outside this engine (multi-block mesh, QK chemistry)."""
+    """couette_N2-O2 and heatBath-5species (its twelve QK reactions read
+from system/chemReactDict) are accepted up to the device; the multi-block
+tutorials stop with the reason they are outside this engine."""
     outcomes = {}
     for c in ("couette_N2-O2", "heatBath-5species", "hypersonicCorner", "supersonicFlatPlate"):
         d = tmp_path / c
@@ -55,10 +56,63 @@ def test_reference_case_outcomes(tmp_path):
         shutil.copytree(os.path.join(REF, c, "constant"), d / "constant", ignore=shutil.ignore_patterns("polyMesh"))
         r = run("dsmcInitialisePlus", str(d))
         outcomes[c] = r.stdout + r.stderr
-    assert "FOAM FATAL ERROR" in outcomes["heatBath-5species"] and "QK chemistry" in outcomes["heatBath-5species"]
     assert "one block" in outcomes["hypersonicCorner"] and "one block" in outcomes["supersonicFlatPlate"]
     if no_gpu():
         assert "no HIP device" in outcomes["couette_N2-O2"]
+        assert "no HIP device" in outcomes["heatBath-5species"]
+
+
+def test_reaction_dictionary_errors(tmp_path):
+    """chemReactDict entries are selected by reactionModel and checked as the
+    reference checks them."""
+    case = casegen.heat_bath_case(str(tmp_path / "hb"), fn=1e5)
+    p = os.path.join(case, "system/chemReactDict")
+    txt = open(p).read()
+    open(p, "w").write(txt.replace("reactionModel dissociationQK;", "reactionModel ionisationQK;", 1))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1 and "unknown dsmc reaction model type ionisationQK" in r.stderr
+    open(p, "w").write(txt.replace("reactants (O2 N2);", "reactants (O2 Ar);", 1))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1 and "Cannot find type id: Ar" in r.stderr
+    open(p, "w").write(txt.replace("dissociationProducts ((O O) (N N));", "dissociationProducts ((O O));", 1))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1 and "two lists of dissociation products" in r.stderr
+    open(p, "w").write(txt.replace("dissociationProducts ((O O) (N N));", "dissociationProducts ((O O) (N NO));", 1))
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 1 and "must be an atom" in r.stderr
+
+
+@pytest.mark.gpu
+def test_heat_bath_pipeline_follows_the_reference_run(tmp_path):
+    """dsmcInitialisePlus and dsmcFoamPlus on the heatBath-5species set-up
+    (nEquivalentParticles 2000 instead of 100: about 7,700 parcels) for 100
+    steps: the written averages of the species number densities and the
+    mixture temperature over those steps follow the reference's probe
+    series averaged over the same steps (tests/golden/heatbath_probes.npz)."""
+    case = casegen.heat_bath_case(str(tmp_path / "hb"), fn=2000.0, end_time=1e-7, write_interval=1e-7)
+    r = run("dsmcInitialisePlus", case)
+    assert r.returncode == 0, r.stderr
+    r = run("dsmcFoamPlus", case)
+    assert r.returncode == 0, r.stderr
+    last = sorted((d for d in os.listdir(case) if re.match(r"^[0-9.e+-]+$", d) and float(d) > 0), key=float)[-1]
+    assert abs(float(last) - 1e-7) < 1e-12
+    fx = np.load(os.path.join(ROOT, "tests", "golden", "heatbath_probes.npz"))
+    first = fx["step"] <= 100
+    for sp, tol in (("N2", 0.05), ("O2", 0.08), ("N", 0.08), ("O", 0.08)):
+        got = float(_field_or_uniform(os.path.join(case, last, "rhoN_" + sp)))
+        want = float(fx["rhoN_" + sp][first].mean())
+        assert abs(got / want - 1.0) < tol, (sp, got, want)
+    T = float(_field_or_uniform(os.path.join(case, last, "Ttra_mixture")))
+    want = float(fx["Ttra_mixture"][first].mean())
+    assert abs(T / want - 1.0) < 0.03, (T, want)
+
+
+def _field_or_uniform(path):
+    txt = open(path).read()
+    m = re.search(r"internalField\s+uniform\s+([-0-9.eE+]+)\s*;", txt)
+    if m:
+        return float(m.group(1))
+    return _field(path)[0]
 
 
 def test_runtime_selection_errors(tmp_path):
