@@ -753,7 +753,8 @@ __device__ bool react_pair(const DevParams& P, int ri, PState& Pp, PState& Qp, R
 // 250-300): its reaction when the pair has a model, else (or when it did not
 // react) the binary collision.  True if it reacted.
 template <class R>
-__device__ bool collide_or_react(const DevParams& P, PState& a, PState& b, R& r, PState& prod, int* from_q) {
+__device__ __attribute__((noinline)) bool collide_or_react(const DevParams& P, PState& a, PState& b, R& r,
+                                                           PState& prod, int* from_q) {
     prod.t = -1;
     const int ri = P.react_of[a.t][b.t];
     if (ri >= 0 && react_pair(P, ri, a, b, r, prod, from_q)) return true;

@@ -1188,8 +1188,9 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     }
     if ((do_collide || do_sample) && react) {
         StageTimer tm(eng, T_COLLIDE);
-        launch_collide_react(eng->dp, nxt_buf(eng), eng->d_start, eng->d_big_sub, eng->d_sigma, eng->d_rem,
-                             eng->d_samples, eng->d_vol, eng->d_S, eng->prod, do_collide, do_sample, eng->nc, eng->ns,
+        launch_collide_react(eng->dp, nxt_buf(eng), eng->d_start, eng->d_ctile_begin, eng->d_tile_order,
+                             eng->collide_parcels, eng->d_big_sub, eng->d_sigma, eng->d_rem, eng->d_samples,
+                             eng->d_vol, eng->d_S, eng->prod, do_collide, do_sample, eng->nc, eng->ns, eng->d_big_list,
                              eng->ws, eng->cap, eng->stream);
     } else if (do_collide || do_sample) {
         StageTimer tm(eng, T_COLLIDE);

@@ -101,13 +101,14 @@ struct DevProducts {
     int32_t* off;     // [nc+1] exclusive prefix of count
     int32_t* list;    // [cap] sub-cell member lists of the reacting collide
 };
-// NTC with QK chemistry on the sorted population in dst, one thread per
-// cell; then the products' prefix, placement, the cells' sums (parcels, then
-// the cell's products) and the population update.
-void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
-                          double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
-                          DevProducts pr, int do_collide, int do_sample, int nc, int ns, ScanWs& ws,
-                          int64_t capacity, hipStream_t st);
+// NTC with QK chemistry on the sorted population in dst (the collide tiles
+// of launch_collide, the variant with internal energies); then the products'
+// prefix, placement, the sums of the cells that made products (parcels, then
+// products) and the population update.
+void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
+                          int32_t* order, int pt, uint8_t* sub, double* sigma, double* rem, double* samples,
+                          const double* vol, DevScalars* S, DevProducts pr, int do_collide, int do_sample, int nc,
+                          int ns, uint16_t* big_list, ScanWs& ws, int64_t capacity, hipStream_t st);
 void launch_clear_queue(DevScalars* S, hipStream_t st);
 void launch_commit(DevScalars* S, int advance, int with_next, hipStream_t st);
 // per-cell fill inputs (NULL: the uniform values): number densities

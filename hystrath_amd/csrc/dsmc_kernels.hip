@@ -1091,6 +1091,56 @@ __device__ __forceinline__ void do_collision(const DevParams& P, PhiloxStream& r
     }
 }
 
+// Where a reacting collide stages the products it makes (see DevProducts).
+struct ProdOut {
+    DevScalars* S;
+    DevProducts pr;
+};
+
+// Stages a dissociation product: cell c, its rank among the cell's products
+// of this step (candidate order), the sorted index of the parcel that split.
+__device__ __forceinline__ void stage_product(const DevParams& P, const ProdOut& po, int c, int rank,
+                                              int64_t parent, const PState& prod) {
+    const int slot = atomicAdd(&po.S->nprod, 1);
+    if (slot >= P.capacity) return;  // more than the capacity: dropped, counted at the commit
+    po.pr.cell[slot] = c;
+    po.pr.rank[slot] = rank;
+    po.pr.parent[slot] = (int32_t)parent;
+    po.pr.type[slot] = (uint8_t)prod.t;
+    po.pr.ux[slot] = (float)prod.u[0];
+    po.pr.uy[slot] = (float)prod.u[1];
+    po.pr.uz[slot] = (float)prod.u[2];
+}
+
+// The accepted collision of candidate (ip, iq) with chemistry: its QK
+// reaction when the pair has a model, else the binary collision (noTimeCounter.C:
+// 250-300).  After a reaction the types and every vibrational plane of both
+// parcels are rewritten.  prod.t >= 0: a product was made; from_q: Q split.
+__device__ __forceinline__ void react_collision(const DevParams& P, PhiloxStream& r, int ip, int iq, double up[3],
+                                                double uq[3], float* vx, float* vy, float* vz, float* er,
+                                                uint16_t* vb, int vstride, uint8_t* ty, PState& prod, int* from_q) {
+    PState a, b;
+    a.t = ty[ip];
+    b.t = ty[iq];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) { a.u[c] = up[c]; b.u[c] = uq[c]; }
+    a.er = er ? (double)er[ip] : 0.0;
+    b.er = er ? (double)er[iq] : 0.0;
+#pragma unroll
+    for (int m = 0; m < HS_MAX_VIB_MODES; ++m) { a.v[m] = 0; b.v[m] = 0; }
+    for (int m = 0; m < P.nvib[a.t]; ++m) a.v[m] = vb[m * vstride + ip];
+    for (int m = 0; m < P.nvib[b.t]; ++m) b.v[m] = vb[m * vstride + iq];
+    const bool reacted = collide_or_react(P, a, b, r, prod, from_q);
+    vx[ip] = (float)a.u[0]; vy[ip] = (float)a.u[1]; vz[ip] = (float)a.u[2];
+    vx[iq] = (float)b.u[0]; vy[iq] = (float)b.u[1]; vz[iq] = (float)b.u[2];
+    if (er) { er[ip] = (float)a.er; er[iq] = (float)b.er; }
+    const int np = reacted ? P.max_modes : P.nvib[a.t];
+    const int nq = reacted ? P.max_modes : P.nvib[b.t];
+    for (int m = 0; m < np; ++m) vb[m * vstride + ip] = (uint16_t)a.v[m];
+    for (int m = 0; m < nq; ++m) vb[m * vstride + iq] = (uint16_t)b.v[m];
+    if (reacted) { ty[ip] = (uint8_t)a.t; ty[iq] = (uint8_t)b.t; }
+}
+
 // Sub-cell lists of one cell built by one lane (ascending index per sub-cell).
 __device__ __forceinline__ void subcell_lists_lane(const uint8_t* sub, uint16_t* slist, int ss, int nC, Pack8& cnt,
                                                    Pack8& off) {
@@ -1107,12 +1157,15 @@ __device__ __forceinline__ void subcell_lists_lane(const uint8_t* sub, uint16_t*
     }
 }
 
-// NTC of one cell by one lane, candidate after candidate.
-template <bool GEN>
+// NTC of one cell by one lane, candidate after candidate.  REACT: with
+// chemistry; the cell's products are staged in candidate order and counted
+// in *nprod.
+template <bool GEN, bool REACT = false>
 __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
                          float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
                          const uint8_t* ty, double* sigma, uint32_t step, const DevParticles& pos, int64_t gs,
-                         CellCtx& cx, double& ncoll, double& sep) {
+                         CellCtx& cx, double& ncoll, double& sep, const ProdOut* po = nullptr,
+                         int* nprod = nullptr) {
     PackTab tab;
     subcell_lists_lane(sub, slist, ss, nC, tab.c, tab.o);
     tab.slist = slist;
@@ -1131,7 +1184,18 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
         const double stcr = sigma_tcr(P, tp, tq, up, uq);
         if (stcr > smax_cur) smax_cur = stcr;
         if (stcr / smax > r.u()) {
-            do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+            if (REACT) {
+                PState prod;
+                int fq = 0;
+                react_collision(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, const_cast<uint8_t*>(ty), prod,
+                                &fq);
+                if (prod.t >= 0) {
+                    stage_product(P, *po, c, *nprod, gs + (fq ? qq : pp), prod);
+                    *nprod += 1;
+                }
+            } else {
+                do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+            }
             ncoll += 1.0;
             if (P.flags & HS_FLAG_COLL_SEP) {
                 const int64_t gp = gs + pp, gq = gs + qq;
@@ -1156,11 +1220,17 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
 // per batch are the depth of its parcel-sharing chains.  The sub-cell lists
 // and tables are built and the stamps cleared beforehand; they are clear
 // again on return.
-template <bool GEN>
+// REACT: with chemistry; each batch stages its products with their ranks in
+// candidate order (a prefix over the batch's lanes), counted in *nprod_out.
+// FENCE: the parcel arrays are in global memory; a workgroup-scope fence after
+// every round makes one round's writes visible to the next round's lanes.
+template <bool GEN, bool REACT = false, bool FENCE = false>
 __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
                          float* vz, const uint8_t* sub, const CellTab& tab, float* er,
                          uint16_t* vb, int vstride, const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step,
-                         const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out) {
+                         const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out,
+                         const ProdOut* po = nullptr, int* nprod_out = nullptr) {
+    int nprod = 0;
     const int lane = threadIdx.x & (WAVE - 1);
     const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
     double smax_cur = smax;
@@ -1177,6 +1247,9 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
         }
         bool doit = false;
         double sm = 0.0, sep_l = 0.0;
+        PState prod;
+        prod.t = -1;
+        int fq = 0;
         while (__ballot(pend)) {
             if (pend) {
                 atomicMin(&stamp[ss + pp], lane);
@@ -1197,7 +1270,11 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
                 const double stcr = sigma_tcr(P, tp, tq, up, uq);
                 sm = stcr;
                 if (stcr / smax > r.u()) {
-                    do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+                    if (REACT)
+                        react_collision(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, const_cast<uint8_t*>(ty),
+                                        prod, &fq);
+                    else
+                        do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
                     doit = true;
                     if (sep_on) {
                         const int64_t gp = gs + pp, gq = gs + qq;
@@ -1208,7 +1285,14 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
                 }
                 pend = false;
             }
+            if (FENCE) __threadfence_block();
             __builtin_amdgcn_wave_barrier();
+        }
+        if (REACT) {
+            const unsigned long long pm = __ballot(prod.t >= 0);
+            if (prod.t >= 0)
+                stage_product(P, *po, c, nprod + __popcll(pm & ((1ull << lane) - 1ull)), gs + (fq ? qq : pp), prod);
+            nprod += __popcll(pm);
         }
         // running maximum of sigmaTcR over the batch
 #pragma unroll
@@ -1231,70 +1315,66 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
     }
     ncoll_out = (uint32_t)ncoll_i;
     sep_out = sep;
+    if (REACT) *nprod_out = nprod;
 }
 
-// dsmcVolFields::calculateField per-cell sums for one cell: per species, the
-// parcels of the cell in order (dsmcVolFields.C:1115-1239), by one thread.
-// The kernel variant with internal energies samples its cells of at most 64
-// parcels this way (four lanes a cell would cost it more register spills
-// than they save).  Each cell has one writer; the cumulative sums take
-// non-returning atomic adds (the same IEEE addition as a read-add-write,
-// without waiting for the read).
+// dsmcVolFields::calculateField per-cell sums for one cell and one species
+// s: the species' parcels of the cell in order (dsmcVolFields.C:1115-1239),
+// by one thread.  The kernel variant with internal energies samples its
+// cells of at most 64 parcels this way, one thread per (cell, species), so a
+// workgroup's threads share the cells' species.  Each sum has one writer;
+// the cumulative sums take non-returning atomic adds (the same IEEE addition
+// as a read-add-write, without waiting for the read).
 template <bool GEN>
-__device__ __forceinline__ void sample_cell(const DevParams& P, int c, int ss, int nC, const float* vx,
-                                            const float* vy, const float* vz, const float* er, const uint16_t* vb,
-                                            int vstride, const uint8_t* ty, double* samples, int nc, double ncoll,
-                                            double sep) {
-    unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
-    unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
-    const int ns = GEN ? P.ns : 1;
+__device__ __forceinline__ void sample_cell_species(const DevParams& P, int c, int ss, int nC, int s, const float* vx,
+                                                    const float* vy, const float* vz, const float* er,
+                                                    const uint16_t* vb, int vstride, const uint8_t* ty,
+                                                    double* samples, int nc) {
     const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
-    for (int s = 0; s < ns; ++s) {
-        const double m = P.mass[s];
-        double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
-        double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
-        const int nvib = GEN ? P.nvib[s] : 0;
+    const double m = P.mass[s];
+    double aN = 0.0, aX = 0.0, aY = 0.0, aZ = 0.0, aC = 0.0, aR = 0.0;
+    double ev[HS_MAX_VIB_MODES] = { 0.0, 0.0, 0.0 };
+    const int nvib = GEN ? P.nvib[s] : 0;
 #pragma unroll 4
+    for (int k = 0; k < nC; ++k) {
+        const int i = ss + k;
+        if (GEN && (int)ty[i] != s) continue;
+        const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
+        aN += 1.0;
+        aX += m * ux;
+        aY += m * uy;
+        aZ += m * uz;
+        aC += m * (ux * ux + uy * uy + uz * uz);
+        if (GEN && P.has_erot) aR += (double)er[i];
+        for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+    }
+    const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
+    unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
+    unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
+    unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
+    unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
+    unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
+    if (GEN && P.has_erot) unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
+    for (int mo = 0; mo < nvib; ++mo) unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
+    if (hf) {
+        double h[HS_HF_COUNT];
+        for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
         for (int k = 0; k < nC; ++k) {
             const int i = ss + k;
             if (GEN && (int)ty[i] != s) continue;
             const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
-            aN += 1.0;
-            aX += m * ux;
-            aY += m * uy;
-            aZ += m * uz;
-            aC += m * (ux * ux + uy * uy + uz * uz);
-            if (GEN && P.has_erot) aR += (double)er[i];
-            for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+            const double lke = m * (ux * ux + uy * uy + uz * uz);
+            const double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
+            double evs = 0.0;
+            for (int mo = 0; mo < nvib; ++mo) evs += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
+            const double eint = erv + evs;
+            h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
+            h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
+            h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
+            h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
         }
-        const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
-        unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
-        unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
-        unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
-        unsafeAtomicAdd(&samples[(base + HS_SS_MUZ) * nc + c], aZ);
-        unsafeAtomicAdd(&samples[(base + HS_SS_MCC) * nc + c], aC);
-        if (GEN && P.has_erot) unsafeAtomicAdd(&samples[(base + HS_SS_EROT) * nc + c], aR);
-        for (int mo = 0; mo < nvib; ++mo) unsafeAtomicAdd(&samples[(base + HS_SS_EVIB0 + mo) * nc + c], ev[mo]);
-        if (hf) {
-            double h[HS_HF_COUNT];
-            for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
-            for (int k = 0; k < nC; ++k) {
-                const int i = ss + k;
-                if (GEN && (int)ty[i] != s) continue;
-                const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
-                const double lke = m * (ux * ux + uy * uy + uz * uz);
-                const double erv = (GEN && P.has_erot) ? (double)er[i] : 0.0;
-                double evs = 0.0;
-                for (int mo = 0; mo < nvib; ++mo) evs += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
-                const double eint = erv + evs;
-                h[0] += m * (ux * ux); h[1] += m * ux * uy; h[2] += m * ux * uz;
-                h[3] += m * (uy * uy); h[4] += m * uy * uz; h[5] += m * (uz * uz);
-                h[6] += lke * ux; h[7] += lke * uy; h[8] += lke * uz;
-                h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
-            }
-            const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
-            for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
-        }
+        const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
+        for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
     }
 }
 
@@ -1539,7 +1619,7 @@ __device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t*
     }
 }
 
-template <bool GEN, int PMAX>
+template <bool GEN, int PMAX, bool REACT = false>
 struct TileLds {
     alignas(16) int32_t key[PMAX];  // old index of each slot; then the NTC conflict stamps
     alignas(16) uint8_t loc[PMAX];  // sub-tile cell of each slot (filled 4 bytes at a time)
@@ -1565,6 +1645,9 @@ struct TileLds {
     uint32_t rcoll[CT_CELLS];
     double gsep[CT];         // collision separation of each slot of a pass
     uint8_t gacc[CT];        // the slot's candidate collided
+    // chemistry: products made per cell, and each slot's count of earlier products in a pass
+    int32_t pcount[REACT ? CT_CELLS : 1];
+    int32_t gpre[REACT ? CT : 1];
 };
 
 // NTC of the sparse cells of a chunk (at most DENSE_CAND candidates each),
@@ -1577,9 +1660,12 @@ struct TileLds {
 // against exactly the velocities the sequential loop of noTimeCounter.C:157-302
 // gives them.  A pass ends when all its slots are done; the next pass (the
 // later candidates) starts after it.
-template <bool GEN, int PMAX>
-__device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int a, int b, int S0, uint32_t step,
-                           const DevParticles& pos, CellCtx& cx) {
+// REACT: with chemistry; at the end of each pass every product is staged
+// with its rank in its cell's candidate order (the cell's products of earlier
+// passes plus a prefix over the pass's slots), and L.pcount counts them.
+template <bool GEN, int PMAX, bool REACT = false>
+__device__ void ntc_rounds(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, int c0, int a, int b, int S0,
+                           uint32_t step, const DevParticles& pos, CellCtx& cx, const ProdOut* po = nullptr) {
     const int t = threadIdx.x;
     const bool own = t >= a && t < b;
     const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
@@ -1593,6 +1679,9 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
         bool pend = false, doit = false;
         double sep_l = 0.0;
         PhiloxStream r;
+        PState prod;
+        prod.t = -1;
+        int fq = 0;
         if (g < total) {
             // the last cell whose first slot is <= g
             int lo = a, hi = b - 1;
@@ -1629,7 +1718,10 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
                 const double stcr = sigma_tcr(P, tp, tq, up, uq);
                 atomicMax(&L.rsmax[l], (unsigned long long)__double_as_longlong(stcr));
                 if (stcr / L.smax[l] > r.u()) {
-                    do_collision<GEN>(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
+                    if (REACT)
+                        react_collision(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, prod, &fq);
+                    else
+                        do_collision<GEN>(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
                     atomicAdd(&L.rcoll[l], 1u);
                     doit = true;
                     if (sep_on) {
@@ -1642,6 +1734,22 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, in
                 pend = false;
             }
             if (!__syncthreads_or(pend)) break;
+        }
+        if (REACT) {
+            // the slots of one cell are consecutive and in candidate order
+            const int made = prod.t >= 0 ? 1 : 0;
+            int ptot;
+            const int ex = block_excl_scan(made, &ptot);
+            L.gpre[t] = ex;
+            __syncthreads();
+            if (made) {
+                const int first = max(L.rpfx[l], g0) - g0;
+                stage_product(P, *po, c0 + l, L.pcount[l] + ex - L.gpre[first], (int64_t)S0 + ss + (fq ? qq : pp),
+                              prod);
+            }
+            __syncthreads();
+            if (made) atomicAdd(&L.pcount[l], 1);
+            __syncthreads();
         }
         if (sep_on) {
             // each cell adds its slots' separations in candidate order
@@ -1794,321 +1902,9 @@ __global__ __launch_bounds__(CT) void k_gather(const DevParams* __restrict__ Pp,
 }
 
 // ---------------------------------------------------------------------------
-// k_collide: NTC, collisions and sampling on the sorted population
-// ---------------------------------------------------------------------------
-// One cell with more parcels than an LDS chunk holds: NTC on one lane and the
-// sampling in global memory.  Inlined: a call would cost the kernel more
-// registers (the caller keeps its live state across it) than the path does.
-template <bool GEN>
-__device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int S0, int nC, DevParticles& dst,
-                                              double* __restrict__ sigma, double* __restrict__ rem,
-                                              double* __restrict__ samples, const double* __restrict__ vol,
-                                              uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
-                                              uint8_t* sub, CellCtx& cx) {
-    const int64_t cap = P.capacity;
-    float* gvx = dst.ux + S0;
-    float* gvy = dst.uy + S0;
-    float* gvz = dst.uz + S0;
-    float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
-    uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
-    const uint8_t* gty = GEN ? dst.type + S0 : nullptr;
-    double ncoll = 0.0, sep = 0.0;
-    if (threadIdx.x == 0) {
-        if (nC > 65535) {
-            cx.ovf += 1;
-        } else if (do_collide && nC > 1) {
-            double smax;
-            const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-            ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb, (int)cap, gty,
-                          sigma, step, dst, S0, cx, ncoll, sep);
-        }
-    }
-    __syncthreads();
-    // the cell's sums by the first wave (its velocities were written by thread 0)
-    if (do_sample && nC <= 65535 && threadIdx.x < WAVE) {
-        sample_cell_wave<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc);
-        if (threadIdx.x == 0) {
-            unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
-            unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
-        }
-    }
-    __syncthreads();
-}
-
-// Collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS) in LDS
-// chunks of at most PMAX parcels.
-template <bool GEN, int PMAX>
-__device__ void collide_cells(TileLds<GEN, PMAX>& L, const DevParams& P, int c0, int ncell, DevParticles& dst,
-                              const int32_t* __restrict__ start, uint8_t* __restrict__ sub,
-                              double* __restrict__ sigma, double* __restrict__ rem, double* __restrict__ samples,
-                              const double* __restrict__ vol, uint32_t step, int do_collide, int do_sample, int nc,
-                              uint16_t* big_list, CellCtx& cx) {
-    const int t = threadIdx.x;
-    const int wid = t / WAVE, lane = t & (WAVE - 1);
-    const int64_t cap = P.capacity;
-    if (t < ncell) L.segS[t] = start[c0 + t];
-    if (t == 0) {
-        L.segS[ncell] = start[c0 + ncell];
-        L.nbig = 0;
-    }
-    __syncthreads();
-    int a = 0;
-    while (a < ncell) {
-        const int S0 = L.segS[a];
-        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX;
-        const int b = a + __syncthreads_count(ok);
-        if (b == a) {
-            collide_oversize<GEN>(P, c0 + a, S0, L.segS[a + 1] - S0, dst, sigma, rem, samples, vol, step, do_collide,
-                                  do_sample, nc, big_list, sub, cx);
-            a += 1;
-            continue;
-        }
-        const int cnt = L.segS[b] - S0;
-        // slot -> cell map; each cell's sigmaTcRMax, remainder and volume; the
-        // sorted velocities and sub-cells of the chunk (contiguous)
-        double pf_sig = 0.0, pf_rem = 0.0, pf_vol = 1.0;
-        if (t >= a && t < b) {
-            if (do_collide) {
-                pf_sig = sigma[c0 + t];
-                pf_rem = rem[c0 + t];
-                pf_vol = vol[c0 + t];
-            }
-            {
-                // the cell's slot range, four bytes per store where it covers whole words
-                int j = L.segS[t] - S0;
-                const int e = L.segS[t + 1] - S0;
-                for (; j < e && (j & 3); ++j) L.loc[j] = (uint8_t)t;
-                const uint32_t w4 = 0x01010101u * (uint32_t)t;
-                for (; j + 4 <= e; j += 4) *reinterpret_cast<uint32_t*>(&L.loc[j]) = w4;
-                for (; j < e; ++j) L.loc[j] = (uint8_t)t;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
-            if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
-        }
-#pragma unroll 8
-        for (int j = t; j < cnt; j += CT) {
-            const int64_t gj = (int64_t)S0 + j;
-            L.vx[j] = dst.ux[gj]; L.vy[j] = dst.uy[gj]; L.vz[j] = dst.uz[gj];
-            L.sub[j] = sub[gj];
-            if (GEN) {
-                L.er[j] = dst.erot ? dst.erot[gj] : 0.0f;
-                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + j] = dst.vib[m * cap + gj];
-                L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
-            }
-        }
-        if (t == 0) L.ndense = 0;
-        __syncthreads();
-        // sub-cell tables in parallel: a parcel of a cell of at most 64 sets
-        // its bit in its sub-cell's mask; in a larger cell it finds its place
-        // among the parcels of its sub-cell (ascending index).  Stamps cleared.
-        for (int j = t; j < cnt; j += CT) {
-            const int l = L.loc[j];
-            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
-            if (ee - ss <= 64) atomicOr(&L.smask[l][L.sub[j]], 1ull << (j - ss));
-            L.key[j] = 0x7FFFFFFF;
-        }
-        // cells of more than 64 parcels: one wave each builds the member
-        // lists with ballots over 64-parcel windows
-        for (int bi = wid; bi < L.nbig; bi += CT / WAVE)
-            subcell_lists_wave(L.sub, L.slist, L.segS[L.big[bi]] - S0,
-                               L.segS[L.big[bi] + 1] - L.segS[L.big[bi]], L.smask[L.big[bi]]);
-        // candidates per cell (noTimeCounter.C:142-153)
-        if (t >= a && t < b) {
-            const int c = c0 + t;
-            const int nC = L.segS[t + 1] - L.segS[t];
-            int nsparse = 0;
-            L.rcoll[t] = 0u;
-            L.sep[t] = 0.0;
-            if (do_collide && nC > 1) {
-                const double smax = pf_sig;
-                const long long ncand = ntc_candidates_of(P, c, nC, smax, pf_rem, pf_vol, rem, cx);
-                L.smax[t] = smax;
-                L.rsmax[t] = (unsigned long long)__double_as_longlong(smax);
-                L.ncand[t] = (int32_t)ncand;
-                if (ncand > DENSE_CAND) {
-                    L.dense[atomicAdd(&L.ndense, 1)] = t;
-                } else {
-                    nsparse = (int)ncand;
-                    cx.cand += (unsigned long long)ncand;
-                }
-            }
-            L.rn[t] = nsparse;
-        }
-        __syncthreads();
-        // dense cells: one wave each
-        const int nd = L.ndense;
-        for (int di = wid; di < nd; di += CT / WAVE) {
-            const int l = L.dense[di];
-            const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
-            const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
-            uint32_t ncoll;
-            double sep;
-            ntc_wave<GEN>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, tab, L.er, L.vb, PMAX,
-                          L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep);
-            if (lane == 0) { L.rcoll[l] = ncoll; L.sep[l] = sep; }
-        }
-        // sparse cells: candidate-parallel rounds over the workgroup
-        ntc_rounds<GEN, PMAX>(L, P, c0, a, b, S0, step, dst, cx);
-        if (t >= a && t < b && L.rn[t] > 0) {
-            sigma[c0 + t] = __longlong_as_double((long long)L.rsmax[t]);
-            cx.colls += L.rcoll[t];
-        }
-        __syncthreads();
-        if (do_sample) {
-            // cells of up to 64 parcels: four lanes each (one thread each in the
-            // variant with internal energies); larger ones: one wave each
-            const int nbig = L.nbig;
-            if (GEN) {
-                if (t >= a && t < b) {
-                    const int ss = L.segS[t] - S0, nC = L.segS[t + 1] - L.segS[t];
-                    if (nC <= 64) {
-                        sample_cell<GEN>(P, c0 + t, ss, nC, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, samples, nc,
-                                         (double)L.rcoll[t], L.sep[t]);
-                    } else {
-                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + t], (double)L.rcoll[t]);
-                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + t], L.sep[t]);
-                    }
-                }
-            } else {
-                const int r = t & 3;
-                for (int l = a + (t >> 2); l < b; l += CT / 4) {
-                    const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
-                    if (nC <= 64) {
-                        sample_cell_quad<GEN>(P, c0 + l, ss, nC, r, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty,
-                                              samples, nc, (double)L.rcoll[l], L.sep[l]);
-                    } else if (r == 0) {
-                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
-                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
-                    }
-                }
-            }
-            for (int bi = wid; bi < nbig; bi += CT / WAVE) {
-                const int l = L.big[bi];
-                sample_cell_wave<GEN>(P, c0 + l, L.segS[l] - S0, L.segS[l + 1] - L.segS[l], L.vx, L.vy, L.vz, L.er,
-                                      L.vb, PMAX, L.ty, samples, nc);
-            }
-        }
-        for (int j = t; j < cnt; j += CT) {
-            const int64_t gd = (int64_t)S0 + j;
-            dst.ux[gd] = L.vx[j]; dst.uy[gd] = L.vy[j]; dst.uz[gd] = L.vz[j];
-            if (GEN) {
-                if (dst.erot) dst.erot[gd] = L.er[j];
-                if (dst.vib)
-                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
-            }
-        }
-        // every wave has read the big-cell list before it is cleared
-        __syncthreads();
-        if (t == 0) L.nbig = 0;
-        __syncthreads();
-        a = b;
-    }
-}
-
-// Tile q of the particle-balanced decomposition owns the cells whose start
-// offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.  Two
-// tilings from one pass: the gather's (pt_g parcels) and the collide's (pt_c).
-// Also flags the collide tiles holding a cell of more than HEAVY_CELL parcels
-// (heavy[] cleared beforehand).
-__device__ __forceinline__ void tile_marks(const int32_t* __restrict__ start, int i, int nc, int pt,
-                                           int32_t* __restrict__ tile_begin, int* q0_out) {
-    const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
-    for (int q = q0 + 1; q <= q1; ++q) tile_begin[q] = i;
-    if (i == nc) tile_begin[q1 + 1] = nc;
-    *q0_out = q0;
-}
-
-__global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt_g,
-                                                 int32_t* __restrict__ tile_begin_g, int pt_c,
-                                                 int32_t* __restrict__ tile_begin_c, int32_t* __restrict__ heavy) {
-    const int i = blockIdx.x * BLOCK + threadIdx.x;
-    if (i > nc) return;
-    if (i == 0) { tile_begin_g[0] = 0; tile_begin_c[0] = 0; return; }
-    int qg, qc;
-    tile_marks(start, i, nc, pt_g, tile_begin_g, &qg);
-    tile_marks(start, i, nc, pt_c, tile_begin_c, &qc);
-    if (start[i] - start[i - 1] > HEAVY_CELL) heavy[qc] = 1;
-}
-
-template <bool GEN, int PMAX>
-__device__ __forceinline__ void collide_body(const DevParams* __restrict__ Pp, DevParticles& dst,
-                                             const int32_t* __restrict__ start, const int32_t* __restrict__ tile_begin,
-                                             const int32_t* __restrict__ order, int pt, uint8_t* __restrict__ sub,
-                                             double* __restrict__ sigma, double* __restrict__ rem,
-                                             double* __restrict__ samples, const double* __restrict__ vol,
-                                             DevScalars* __restrict__ S, int do_collide, int do_sample, int nc,
-                                             uint16_t* big_list, const int bx) {
-    __shared__ TileLds<GEN, PMAX> L;
-    const DevParams& P = *Pp;
-    // the tile order is read beside the population (every workgroup of the
-    // launch is within the order array); its entry is used only by a
-    // workgroup inside the population, whose entry the scatter wrote
-    const int q_any = order[bx];
-    const int64_t nsort = S->n_sorted;
-    if ((int64_t)bx * pt > nsort) return;
-    const int q = q_any;
-    const int cb = tile_begin[q], ce = tile_begin[q + 1];
-    if (cb >= ce) return;
-    const uint32_t step = S->step;
-    CellCtx cx = { 0, 0, 0 };
-    for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
-        collide_cells<GEN, PMAX>(L, P, c0, min(CT_CELLS, ce - c0), dst, start, sub, sigma, rem, samples, vol, step,
-                                 do_collide, do_sample, nc, big_list, cx);
-    add_counter(S, CNT_CAND, cx.cand);
-    add_counter(S, CNT_COLL, cx.colls);
-    add_counter(S, CNT_TILE_OVF, cx.ovf);
-}
-
-template <bool GEN, int PMAX>
-__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
-                                                   const int32_t* __restrict__ start,
-                                                   const int32_t* __restrict__ tile_begin,
-                                                   const int32_t* __restrict__ order, int pt,
-                                                   uint8_t* __restrict__ sub, double* __restrict__ sigma,
-                                                   double* __restrict__ rem, double* __restrict__ samples,
-                                                   const double* __restrict__ vol, DevScalars* __restrict__ S,
-                                                   int do_collide, int do_sample, int nc, uint16_t* big_list) {
-    collide_body<GEN, PMAX>(Pp, dst, start, tile_begin, order, pt, sub, sigma, rem, samples, vol, S, do_collide,
-                            do_sample, nc, big_list, (int)blockIdx.x);
-}
-
-void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
-                  int32_t* heavy, hipStream_t st) {
-    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt_g, tile_begin_g, pt_c, tile_begin_c, heavy);
-}
-
-#define PMAX_MONO 2304
-#define PMAX_GEN 1520
-
-void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
-                   const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
-                   int general, hipStream_t st) {
-    const int nt = (int)(capacity / pt) + 2;
-    if (general)
-        k_gather<true><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
-    else
-        k_gather<false><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
-}
-
-void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
-                    int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
-                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
-                    int general, int nc, uint16_t* big_list, hipStream_t st) {
-    const int nt = (int)(capacity / pt) + 2;
-    (void)heavy;  // the order was built by k_scatter
-    if (general)
-        k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples,
-                                                      vol, S, do_collide, do_sample, nc, big_list);
-    else
-        k_collide<false, PMAX_MONO><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem,
-                                                        samples, vol, S, do_collide, do_sample, nc, big_list);
-}
-
-// ---------------------------------------------------------------------------
-// collide with QK chemistry (dsmcReactions): one thread per cell, candidate
-// after candidate (noTimeCounter.C:81-339 with the reaction branch of
-// :250-300), on the sorted population in global memory
+// collide with QK chemistry (dsmcReactions), global-memory parts: parcel
+// states, the one-pass cell sums and the NTC of a cell larger than the LDS
+// tiles (the reaction branch of noTimeCounter.C:250-300)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void load_state(const DevParams& P, const DevParticles& d, int64_t i, int64_t cap,
                                            PState& a) {
@@ -2174,106 +1970,462 @@ __device__ __forceinline__ void sample_cell_react(const DevParams& P, const DevP
     }
 }
 
-template <int MAXS>
-__global__ __launch_bounds__(BLOCK) void k_collide_react(const DevParams* __restrict__ Pp, DevParticles dst,
-                                                         const int32_t* __restrict__ start,
-                                                         const uint8_t* __restrict__ sub,
-                                                         double* __restrict__ sigma, double* __restrict__ rem,
-                                                         double* __restrict__ samples, const double* __restrict__ vol,
-                                                         DevScalars* __restrict__ S, DevProducts pr, int do_collide,
-                                                         int do_sample, int nc) {
-    const DevParams& P = *Pp;
+// NTC with chemistry of one cell of any size by one thread, on the sorted
+// population in global memory (sub-cell member lists in pr.list, 32-bit);
+// the cell's products are staged in candidate order and counted in nprod.
+__device__ void ntc_cell_react(const DevParams& P, DevParticles& dst, int c, int32_t b, int32_t nC,
+                               const uint8_t* __restrict__ sub, double* __restrict__ sigma,
+                               double* __restrict__ rem, const double* __restrict__ vol, const ProdOut& po,
+                               uint32_t step, CellCtx& cx, double& ncoll, double& sep, int& nprod) {
     const int64_t cap = P.capacity;
-    const int c = blockIdx.x * BLOCK + threadIdx.x;
-    CellCtx cx = { 0, 0, 0 };
-    if (c < nc) {
-        const int32_t b = start[c], nC = start[c + 1] - b;
-        int nprod = 0;
-        double ncoll = 0.0, sep = 0.0;
-        if (do_collide && nC > 1) {
-            // sub-cell member lists in ascending index order (noTimeCounter.C:114-138)
-            int scn[8], sco[8];
+    // sub-cell member lists in ascending index order (noTimeCounter.C:114-138)
+    int scn[8], sco[8];
 #pragma unroll
-            for (int q = 0; q < 8; ++q) scn[q] = 0;
-            for (int k = 0; k < nC; ++k) scn[sub[b + k]] += 1;
-            int acc = 0;
+    for (int q = 0; q < 8; ++q) scn[q] = 0;
+    for (int k = 0; k < nC; ++k) scn[sub[b + k]] += 1;
+    int acc = 0;
 #pragma unroll
-            for (int q = 0; q < 8; ++q) { sco[q] = acc; acc += scn[q]; }
-            {
-                int fill[8];
+    for (int q = 0; q < 8; ++q) { sco[q] = acc; acc += scn[q]; }
+    {
+        int fill[8];
 #pragma unroll
-                for (int q = 0; q < 8; ++q) fill[q] = sco[q];
-                for (int k = 0; k < nC; ++k) pr.list[b + fill[sub[b + k]]++] = k;
+        for (int q = 0; q < 8; ++q) fill[q] = sco[q];
+        for (int k = 0; k < nC; ++k) po.pr.list[b + fill[sub[b + k]]++] = k;
+    }
+    double smax;
+    const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+    double smax_cur = smax;
+    cx.cand += (unsigned long long)ncand;
+    for (long long k = 0; k < ncand; ++k) {
+        PhiloxStream r;
+        r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
+        const int pp = random_label(r, 0, nC - 1);
+        const int q = sub[b + pp];
+        const int nsc = scn[q];
+        int qq = -1, guard = 0;
+        if (nsc > 1) {
+            do { qq = po.pr.list[b + sco[q] + random_label(r, 0, nsc - 1)]; }
+            while (pp == qq && ++guard < HS_MAX_REDRAW);
+        } else {
+            do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
+        }
+        if (pp == qq) continue;
+        const int64_t ip = (int64_t)b + pp, iq = (int64_t)b + qq;
+        PState a, bb;
+        load_state(P, dst, ip, cap, a);
+        load_state(P, dst, iq, cap, bb);
+        if (P.charge[a.t] == -1 && P.charge[bb.t] == -1) continue;
+        const double stcr = sigma_tcr(P, a.t, bb.t, a.u, bb.u);
+        if (stcr > smax_cur) smax_cur = stcr;
+        if (stcr / smax > r.u()) {
+            PState prod;
+            int from_q = 0;
+            const bool reacted = collide_or_react(P, a, bb, r, prod, &from_q);
+            store_state(P, dst, ip, cap, a, reacted);
+            store_state(P, dst, iq, cap, bb, reacted);
+            if (prod.t >= 0) {
+                stage_product(P, po, c, nprod, from_q ? iq : ip, prod);
+                nprod += 1;
             }
+            ncoll += 1.0;
+            if (P.flags & HS_FLAG_COLL_SEP) {
+                const double ddx = dst.px[ip] - dst.px[iq], ddy = dst.py[ip] - dst.py[iq], ddz = dst.pz[ip] - dst.pz[iq];
+                sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
+            }
+            cx.colls += 1;
+        }
+    }
+    sigma[c] = smax_cur;
+}
+
+// ---------------------------------------------------------------------------
+// k_collide: NTC, collisions and sampling on the sorted population
+// ---------------------------------------------------------------------------
+// One cell with more parcels than an LDS chunk holds, in global memory.  Up
+// to 65,535 parcels and as many 32-bit stamps as the chunk's LDS arrays hold
+// (lds_ints), the first wave runs the wave-cooperative NTC (ntc_wave) on the
+// cell's parcels in place, with its conflict stamps in those arrays (unused
+// while this cell is handled) and the sub-cell lists built by the wave.  Larger
+// cells run on one lane: with chemistry any size (32-bit lists), without it
+// up to 65,535 parcels (beyond: counted as overflow, not collided).  Then the
+// first wave sums the cell; with chemistry a cell that made products is
+// summed after they are placed (k_sample_react).
+template <bool GEN, bool REACT = false>
+__device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int S0, int nC, DevParticles& dst,
+                                              double* __restrict__ sigma, double* __restrict__ rem,
+                                              double* __restrict__ samples, const double* __restrict__ vol,
+                                              uint32_t step, int do_collide, int do_sample, int nc, uint16_t* big_list,
+                                              uint8_t* sub, CellCtx& cx, const ProdOut* po, int32_t* lds,
+                                              int lds_ints) {
+    const int64_t cap = P.capacity;
+    const int t = threadIdx.x;
+    float* gvx = dst.ux + S0;
+    float* gvy = dst.uy + S0;
+    float* gvz = dst.uz + S0;
+    float* ger = GEN && dst.erot ? dst.erot + S0 : nullptr;
+    uint16_t* gvb = GEN && dst.vib ? dst.vib + S0 : nullptr;
+    uint8_t* gty = GEN ? dst.type + S0 : nullptr;
+    __shared__ unsigned long long s_tab[8];
+    __shared__ double s_smax;
+    __shared__ long long s_ncand;
+    __shared__ int s_nprod;
+    double ncoll = 0.0, sep = 0.0;
+    const bool wave_path = nC <= 65535 && nC <= lds_ints;
+    if (t == 0) s_nprod = 0;
+    if (do_collide && nC > 1 && wave_path) {
+        if (t < WAVE) subcell_lists_wave(sub + S0, big_list + S0, 0, nC, s_tab);
+        for (int j = t; j < nC; j += CT) lds[j] = 0x7FFFFFFF;
+        if (t == 0) {
+            double smax;
+            s_ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
+            s_smax = smax;
+        }
+        __syncthreads();
+        if (t < WAVE) {
+            const CellTab tab = { s_tab, big_list + S0, 0, false };
+            uint32_t nco = 0;
+            double sp = 0.0;
+            int np = 0;
+            ntc_wave<GEN, REACT, true>(P, c, 0, nC, s_ncand, s_smax, gvx, gvy, gvz, sub + S0, tab, ger, gvb, (int)cap,
+                                       gty, lds, sigma, step, dst, S0, cx, nco, sp, po, &np);
+            ncoll = (double)nco;
+            sep = sp;
+            if (REACT && t == 0) s_nprod = np;
+        }
+    } else if (t == 0) {
+        if (REACT) {
+            int np = 0;
+            if (do_collide && nC > 1)
+                ntc_cell_react(P, dst, c, S0, nC, sub, sigma, rem, vol, *po, step, cx, ncoll, sep, np);
+            s_nprod = np;
+        } else if (nC > 65535) {
+            cx.ovf += 1;
+        } else if (do_collide && nC > 1) {
             double smax;
             const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-            double smax_cur = smax;
-            cx.cand += (unsigned long long)ncand;
-            const uint32_t step = S->step;
-            for (long long k = 0; k < ncand; ++k) {
-                PhiloxStream r;
-                r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-                const int pp = random_label(r, 0, nC - 1);
-                const int q = sub[b + pp];
-                const int nsc = scn[q];
-                int qq = -1, guard = 0;
-                if (nsc > 1) {
-                    do { qq = pr.list[b + sco[q] + random_label(r, 0, nsc - 1)]; }
-                    while (pp == qq && ++guard < HS_MAX_REDRAW);
-                } else {
-                    do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
-                }
-                if (pp == qq) continue;
-                const int64_t ip = (int64_t)b + pp, iq = (int64_t)b + qq;
-                PState a, bb;
-                load_state(P, dst, ip, cap, a);
-                load_state(P, dst, iq, cap, bb);
-                if (P.charge[a.t] == -1 && P.charge[bb.t] == -1) continue;
-                const double stcr = sigma_tcr(P, a.t, bb.t, a.u, bb.u);
-                if (stcr > smax_cur) smax_cur = stcr;
-                if (stcr / smax > r.u()) {
-                    PState prod;
-                    int from_q = 0;
-                    const bool reacted = collide_or_react(P, a, bb, r, prod, &from_q);
-                    store_state(P, dst, ip, cap, a, reacted);
-                    store_state(P, dst, iq, cap, bb, reacted);
-                    if (prod.t >= 0) {
-                        const int slot = atomicAdd(&S->nprod, 1);
-                        if (slot < cap) {
-                            pr.cell[slot] = c;
-                            pr.rank[slot] = nprod;
-                            pr.parent[slot] = (int32_t)(from_q ? iq : ip);
-                            pr.type[slot] = (uint8_t)prod.t;
-                            pr.ux[slot] = (float)prod.u[0];
-                            pr.uy[slot] = (float)prod.u[1];
-                            pr.uz[slot] = (float)prod.u[2];
-                        }
-                        nprod += 1;
-                    }
-                    ncoll += 1.0;
-                    if (P.flags & HS_FLAG_COLL_SEP) {
-                        const double ddx = dst.px[ip] - dst.px[iq], ddy = dst.py[ip] - dst.py[iq],
-                                     ddz = dst.pz[ip] - dst.pz[iq];
-                        sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-                    }
-                    cx.colls += 1;
-                }
-            }
-            sigma[c] = smax_cur;
+            ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb, (int)cap, gty,
+                          sigma, step, dst, S0, cx, ncoll, sep);
         }
-        pr.count[c] = nprod;
+    }
+    if (t == 0) {
+        if (REACT) po->pr.count[c] = s_nprod;
         if (do_sample) {
             unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
             unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
-            // a cell that made no products is summed here, while its parcels
-            // are in cache; the others after their products are placed
-            if (nprod == 0 && !(P.flags & HS_FLAG_HEATFLUX))
-                sample_cell_react<MAXS>(P, dst, b, nC, 0, 0, samples, nc, c);
         }
     }
+    __threadfence_block();
+    __syncthreads();
+    // the cell's sums by the first wave
+    const bool summed_later = REACT && s_nprod > 0;
+    if (do_sample && !summed_later && (REACT || nC <= 65535) && t < WAVE)
+        sample_cell_wave<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc);
+    __syncthreads();
+}
+
+// Collide and sample the cells c0 .. c0+ncell-1 (ncell <= CT_CELLS) in LDS
+// chunks of at most PMAX parcels.
+// REACT: with chemistry (ProdOut po): product counts per cell, types
+// written back, and a cell that made products left to k_sample_react.
+template <bool GEN, int PMAX, bool REACT = false>
+__device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, int c0, int ncell, DevParticles& dst,
+                              const int32_t* __restrict__ start, uint8_t* __restrict__ sub,
+                              double* __restrict__ sigma, double* __restrict__ rem, double* __restrict__ samples,
+                              const double* __restrict__ vol, uint32_t step, int do_collide, int do_sample, int nc,
+                              uint16_t* big_list, CellCtx& cx, const ProdOut* po = nullptr) {
+    const int t = threadIdx.x;
+    const int wid = t / WAVE, lane = t & (WAVE - 1);
+    const int64_t cap = P.capacity;
+    if (t < ncell) L.segS[t] = start[c0 + t];
+    if (t == 0) {
+        L.segS[ncell] = start[c0 + ncell];
+        L.nbig = 0;
+    }
+    __syncthreads();
+    int a = 0;
+    while (a < ncell) {
+        const int S0 = L.segS[a];
+        const bool ok = t >= a && t < ncell && L.segS[t + 1] - S0 <= PMAX;
+        const int b = a + __syncthreads_count(ok);
+        if (b == a) {
+            collide_oversize<GEN, REACT>(P, c0 + a, S0, L.segS[a + 1] - S0, dst, sigma, rem, samples, vol, step,
+                                         do_collide, do_sample, nc, big_list, sub, cx, po, L.key,
+                                         (int)((reinterpret_cast<char*>(&L.segS[0]) - reinterpret_cast<char*>(&L.key[0]))
+                                               / sizeof(int32_t)));
+            a += 1;
+            continue;
+        }
+        const int cnt = L.segS[b] - S0;
+        // slot -> cell map; each cell's sigmaTcRMax, remainder and volume; the
+        // sorted velocities and sub-cells of the chunk (contiguous)
+        double pf_sig = 0.0, pf_rem = 0.0, pf_vol = 1.0;
+        if (t >= a && t < b) {
+            if (do_collide) {
+                pf_sig = sigma[c0 + t];
+                pf_rem = rem[c0 + t];
+                pf_vol = vol[c0 + t];
+            }
+            {
+                // the cell's slot range, four bytes per store where it covers whole words
+                int j = L.segS[t] - S0;
+                const int e = L.segS[t + 1] - S0;
+                for (; j < e && (j & 3); ++j) L.loc[j] = (uint8_t)t;
+                const uint32_t w4 = 0x01010101u * (uint32_t)t;
+                for (; j + 4 <= e; j += 4) *reinterpret_cast<uint32_t*>(&L.loc[j]) = w4;
+                for (; j < e; ++j) L.loc[j] = (uint8_t)t;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
+            if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
+        }
+#pragma unroll 8
+        for (int j = t; j < cnt; j += CT) {
+            const int64_t gj = (int64_t)S0 + j;
+            L.vx[j] = dst.ux[gj]; L.vy[j] = dst.uy[gj]; L.vz[j] = dst.uz[gj];
+            L.sub[j] = sub[gj];
+            if (GEN) {
+                L.er[j] = dst.erot ? dst.erot[gj] : 0.0f;
+                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + j] = dst.vib[m * cap + gj];
+                L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
+            }
+        }
+        if (t == 0) L.ndense = 0;
+        __syncthreads();
+        // sub-cell tables in parallel: a parcel of a cell of at most 64 sets
+        // its bit in its sub-cell's mask; in a larger cell it finds its place
+        // among the parcels of its sub-cell (ascending index).  Stamps cleared.
+        for (int j = t; j < cnt; j += CT) {
+            const int l = L.loc[j];
+            const int ss = L.segS[l] - S0, ee = L.segS[l + 1] - S0;
+            if (ee - ss <= 64) atomicOr(&L.smask[l][L.sub[j]], 1ull << (j - ss));
+            L.key[j] = 0x7FFFFFFF;
+        }
+        // cells of more than 64 parcels: one wave each builds the member
+        // lists with ballots over 64-parcel windows
+        for (int bi = wid; bi < L.nbig; bi += CT / WAVE)
+            subcell_lists_wave(L.sub, L.slist, L.segS[L.big[bi]] - S0,
+                               L.segS[L.big[bi] + 1] - L.segS[L.big[bi]], L.smask[L.big[bi]]);
+        // candidates per cell (noTimeCounter.C:142-153)
+        if (t >= a && t < b) {
+            const int c = c0 + t;
+            const int nC = L.segS[t + 1] - L.segS[t];
+            int nsparse = 0;
+            L.rcoll[t] = 0u;
+            L.sep[t] = 0.0;
+            if (REACT) L.pcount[t] = 0;
+            if (do_collide && nC > 1) {
+                const double smax = pf_sig;
+                const long long ncand = ntc_candidates_of(P, c, nC, smax, pf_rem, pf_vol, rem, cx);
+                L.smax[t] = smax;
+                L.rsmax[t] = (unsigned long long)__double_as_longlong(smax);
+                L.ncand[t] = (int32_t)ncand;
+                if (ncand > DENSE_CAND) {
+                    L.dense[atomicAdd(&L.ndense, 1)] = t;
+                } else {
+                    nsparse = (int)ncand;
+                    cx.cand += (unsigned long long)ncand;
+                }
+            }
+            L.rn[t] = nsparse;
+        }
+        __syncthreads();
+        // dense cells: one wave each
+        const int nd = L.ndense;
+        for (int di = wid; di < nd; di += CT / WAVE) {
+            const int l = L.dense[di];
+            const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
+            const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
+            uint32_t ncoll;
+            double sep;
+            int np = 0;
+            ntc_wave<GEN, REACT>(P, c0 + l, ss, nC, L.ncand[l], L.smax[l], L.vx, L.vy, L.vz, L.sub, tab, L.er, L.vb,
+                                 PMAX, L.ty, L.key, sigma, step, dst, (int64_t)S0 + ss, cx, ncoll, sep, po, &np);
+            if (lane == 0) {
+                L.rcoll[l] = ncoll;
+                L.sep[l] = sep;
+                if (REACT) L.pcount[l] = np;
+            }
+        }
+        // sparse cells: candidate-parallel rounds over the workgroup
+        ntc_rounds<GEN, PMAX, REACT>(L, P, c0, a, b, S0, step, dst, cx, po);
+        if (t >= a && t < b && L.rn[t] > 0) {
+            sigma[c0 + t] = __longlong_as_double((long long)L.rsmax[t]);
+            cx.colls += L.rcoll[t];
+        }
+        __syncthreads();
+        if (REACT && t >= a && t < b) po->pr.count[c0 + t] = L.pcount[t];
+        if (do_sample) {
+            // cells of up to 64 parcels: four lanes each (one thread per species
+            // in the variant with internal energies); larger ones: one wave each
+            const int nbig = L.nbig;
+            if (GEN) {
+                // one thread per (cell, species); a cell that made products is
+                // summed with them once they are placed
+                const int ns = P.ns;
+                const int nw = (b - a) * ns;
+                for (int w = t; w < nw; w += CT) {
+                    const int l = a + w / ns, sp = w - (w / ns) * ns;
+                    const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
+                    if (sp == 0) {
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
+                    }
+                    if (nC <= 64 && !(REACT && L.pcount[l] > 0))
+                        sample_cell_species<GEN>(P, c0 + l, ss, nC, sp, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty,
+                                                 samples, nc);
+                }
+            } else {
+                const int r = t & 3;
+                for (int l = a + (t >> 2); l < b; l += CT / 4) {
+                    const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
+                    if (nC <= 64) {
+                        sample_cell_quad<GEN>(P, c0 + l, ss, nC, r, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty,
+                                              samples, nc, (double)L.rcoll[l], L.sep[l]);
+                    } else if (r == 0) {
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
+                        unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
+                    }
+                }
+            }
+            for (int bi = wid; bi < nbig; bi += CT / WAVE) {
+                const int l = L.big[bi];
+                if (REACT && L.pcount[l] > 0) continue;
+                sample_cell_wave<GEN>(P, c0 + l, L.segS[l] - S0, L.segS[l + 1] - L.segS[l], L.vx, L.vy, L.vz, L.er,
+                                      L.vb, PMAX, L.ty, samples, nc);
+            }
+        }
+        for (int j = t; j < cnt; j += CT) {
+            const int64_t gd = (int64_t)S0 + j;
+            dst.ux[gd] = L.vx[j]; dst.uy[gd] = L.vy[j]; dst.uz[gd] = L.vz[j];
+            if (GEN) {
+                if (dst.erot) dst.erot[gd] = L.er[j];
+                if (dst.vib)
+                    for (int m = 0; m < P.max_modes; ++m) dst.vib[m * cap + gd] = L.vb[m * PMAX + j];
+                if (REACT) dst.type[gd] = L.ty[j];
+            }
+        }
+        // every wave has read the big-cell list before it is cleared
+        __syncthreads();
+        if (t == 0) L.nbig = 0;
+        __syncthreads();
+        a = b;
+    }
+}
+
+// Tile q of the particle-balanced decomposition owns the cells whose start
+// offset lies in [q*PT, (q+1)*PT): tile_begin[q] is its first cell.  Two
+// tilings from one pass: the gather's (pt_g parcels) and the collide's (pt_c).
+// Also flags the collide tiles holding a cell of more than HEAVY_CELL parcels
+// (heavy[] cleared beforehand).
+__device__ __forceinline__ void tile_marks(const int32_t* __restrict__ start, int i, int nc, int pt,
+                                           int32_t* __restrict__ tile_begin, int* q0_out) {
+    const int q0 = start[i - 1] / pt, q1 = start[i] / pt;
+    for (int q = q0 + 1; q <= q1; ++q) tile_begin[q] = i;
+    if (i == nc) tile_begin[q1 + 1] = nc;
+    *q0_out = q0;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_tiles(const int32_t* __restrict__ start, int nc, int pt_g,
+                                                 int32_t* __restrict__ tile_begin_g, int pt_c,
+                                                 int32_t* __restrict__ tile_begin_c, int32_t* __restrict__ heavy) {
+    const int i = blockIdx.x * BLOCK + threadIdx.x;
+    if (i > nc) return;
+    if (i == 0) { tile_begin_g[0] = 0; tile_begin_c[0] = 0; return; }
+    int qg, qc;
+    tile_marks(start, i, nc, pt_g, tile_begin_g, &qg);
+    tile_marks(start, i, nc, pt_c, tile_begin_c, &qc);
+    if (start[i] - start[i - 1] > HEAVY_CELL) heavy[qc] = 1;
+}
+
+template <bool GEN, int PMAX, bool REACT = false>
+__device__ __forceinline__ void collide_body(const DevParams* __restrict__ Pp, DevParticles& dst,
+                                             const int32_t* __restrict__ start, const int32_t* __restrict__ tile_begin,
+                                             const int32_t* __restrict__ order, int pt, uint8_t* __restrict__ sub,
+                                             double* __restrict__ sigma, double* __restrict__ rem,
+                                             double* __restrict__ samples, const double* __restrict__ vol,
+                                             DevScalars* __restrict__ S, int do_collide, int do_sample, int nc,
+                                             uint16_t* big_list, const int bx, const ProdOut* po = nullptr) {
+    __shared__ TileLds<GEN, PMAX, REACT> L;
+    const DevParams& P = *Pp;
+    // the tile order is read beside the population (every workgroup of the
+    // launch is within the order array); its entry is used only by a
+    // workgroup inside the population, whose entry the scatter wrote
+    const int q_any = order[bx];
+    const int64_t nsort = S->n_sorted;
+    if ((int64_t)bx * pt > nsort) return;
+    const int q = q_any;
+    const int cb = tile_begin[q], ce = tile_begin[q + 1];
+    if (cb >= ce) return;
+    const uint32_t step = S->step;
+    CellCtx cx = { 0, 0, 0 };
+    for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
+        collide_cells<GEN, PMAX, REACT>(L, P, c0, min(CT_CELLS, ce - c0), dst, start, sub, sigma, rem, samples, vol,
+                                        step, do_collide, do_sample, nc, big_list, cx, po);
     add_counter(S, CNT_CAND, cx.cand);
     add_counter(S, CNT_COLL, cx.colls);
     add_counter(S, CNT_TILE_OVF, cx.ovf);
+}
+
+template <bool GEN, int PMAX>
+__global__ __launch_bounds__(CT, 2) void k_collide(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                   const int32_t* __restrict__ start,
+                                                   const int32_t* __restrict__ tile_begin,
+                                                   const int32_t* __restrict__ order, int pt,
+                                                   uint8_t* __restrict__ sub, double* __restrict__ sigma,
+                                                   double* __restrict__ rem, double* __restrict__ samples,
+                                                   const double* __restrict__ vol, DevScalars* __restrict__ S,
+                                                   int do_collide, int do_sample, int nc, uint16_t* big_list) {
+    collide_body<GEN, PMAX>(Pp, dst, start, tile_begin, order, pt, sub, sigma, rem, samples, vol, S, do_collide,
+                            do_sample, nc, big_list, (int)blockIdx.x);
+}
+
+// the collide with QK chemistry: the variant with internal energies, whose
+// accepted pairs react (or collide), staging their products
+template <int PMAX>
+__global__ __launch_bounds__(CT, 2) void k_collide_chem(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                        const int32_t* __restrict__ start,
+                                                        const int32_t* __restrict__ tile_begin,
+                                                        const int32_t* __restrict__ order, int pt,
+                                                        uint8_t* __restrict__ sub, double* __restrict__ sigma,
+                                                        double* __restrict__ rem, double* __restrict__ samples,
+                                                        const double* __restrict__ vol, DevScalars* __restrict__ S,
+                                                        int do_collide, int do_sample, int nc, uint16_t* big_list,
+                                                        DevProducts pr) {
+    const ProdOut po = { S, pr };
+    collide_body<true, PMAX, true>(Pp, dst, start, tile_begin, order, pt, sub, sigma, rem, samples, vol, S, do_collide,
+                                   do_sample, nc, big_list, (int)blockIdx.x, &po);
+}
+
+void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
+                  int32_t* heavy, hipStream_t st) {
+    k_tiles<<<(nc + 1 + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(start, nc, pt_g, tile_begin_g, pt_c, tile_begin_c, heavy);
+}
+
+#define PMAX_MONO 2304
+#define PMAX_GEN 1520
+
+void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
+                   const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
+                   int general, hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
+    if (general)
+        k_gather<true><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
+    else
+        k_gather<false><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
+}
+
+void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
+                    int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
+                    double* rem, double* samples, const double* vol, DevScalars* S, int do_collide, int do_sample,
+                    int general, int nc, uint16_t* big_list, hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
+    (void)heavy;  // the order was built by k_scatter
+    if (general)
+        k_collide<true, PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples,
+                                                      vol, S, do_collide, do_sample, nc, big_list);
+    else
+        k_collide<false, PMAX_MONO><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem,
+                                                        samples, vol, S, do_collide, do_sample, nc, big_list);
 }
 
 // the staged products after the sorted population, in (cell, rank) order
@@ -2301,9 +2453,8 @@ __global__ __launch_bounds__(BLOCK) void k_place_products(const DevParams* __res
 
 // The cell's sums (dsmcVolFields.C:1115-1239) after its reactions: per
 // species, its parcels and then the products it made this step, one thread
-// adding them in order (the oracle's order with reactions).  Without the
-// heat-flux terms the reacting collide has summed the cells that made no
-// products.
+// adding them in order (the oracle's order for such a cell).  The collide
+// has summed the cells that made no products.
 template <int MAXS>
 __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restrict__ Pp, DevParticles dst,
                                                         const int32_t* __restrict__ start,
@@ -2313,7 +2464,7 @@ __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restr
     const int c = blockIdx.x * BLOCK + threadIdx.x;
     if (c >= nc) return;
     const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
-    if (!hf && pr.count[c] == 0) return;
+    if (pr.count[c] == 0) return;
     const int64_t cap = P.capacity;
     const int64_t b = start[c];
     const int nC = start[c + 1] - start[c];
@@ -2381,19 +2532,16 @@ __global__ void k_products_commit(DevScalars* S, const int32_t* __restrict__ off
     S->nprod = 0;
 }
 
-void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const uint8_t* sub,
-                          double* sigma, double* rem, double* samples, const double* vol, DevScalars* S,
-                          DevProducts pr, int do_collide, int do_sample, int nc, int ns, ScanWs& ws,
-                          int64_t capacity, hipStream_t st) {
-    const int g = (nc + BLOCK - 1) / BLOCK;
-    if (ns <= 5)
-        k_collide_react<5><<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr, do_collide,
-                                                 do_sample, nc);
-    else
-        k_collide_react<HS_MAX_SPECIES><<<g, BLOCK, 0, st>>>(P, dst, start, sub, sigma, rem, samples, vol, S, pr,
-                                                              do_collide, do_sample, nc);
+void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
+                          int32_t* order, int pt, uint8_t* sub, double* sigma, double* rem, double* samples,
+                          const double* vol, DevScalars* S, DevProducts pr, int do_collide, int do_sample, int nc,
+                          int ns, uint16_t* big_list, ScanWs& ws, int64_t capacity, hipStream_t st) {
+    const int nt = (int)(capacity / pt) + 2;
+    k_collide_chem<PMAX_GEN><<<nt, CT, 0, st>>>(P, dst, start, tile_begin, order, pt, sub, sigma, rem, samples, vol, S,
+                                                 do_collide, do_sample, nc, big_list, pr);
     launch_scan(pr.count, pr.off, nc, SCAN_PLAIN, ws, S, capacity, st);
     k_place_products<<<grid_for(capacity, 1024), BLOCK, 0, st>>>(P, dst, S, pr);
+    const int g = (nc + BLOCK - 1) / BLOCK;
     if (do_sample) {
         if (ns <= 5) k_sample_react<5><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
         else k_sample_react<HS_MAX_SPECIES><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
