@@ -680,7 +680,8 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
             (rc = dev_alloc(eng, &pr.parent, cap)) || (rc = dev_alloc(eng, &pr.type, cap)) ||
             (rc = dev_alloc(eng, &pr.ux, cap)) || (rc = dev_alloc(eng, &pr.uy, cap)) ||
             (rc = dev_alloc(eng, &pr.uz, cap)) || (rc = dev_alloc(eng, &pr.count, nc)) ||
-            (rc = dev_alloc(eng, &pr.off, (int64_t)nc + 1)) || (rc = dev_alloc(eng, &pr.list, cap)))
+            (rc = dev_alloc(eng, &pr.off, (int64_t)nc + 1)) || (rc = dev_alloc(eng, &pr.list, cap)) ||
+            (rc = dev_alloc(eng, &pr.bigcells, nc)))
             return bail(rc);
         launch_fill_i32(pr.count, 0, nc, eng->stream);
         launch_fill_i32(pr.off, 0, (int64_t)nc + 1, eng->stream);
@@ -1184,7 +1185,7 @@ static void enqueue_sort_collide(hs_dsmc_engine* eng, int do_collide, int do_sam
     {
         StageTimer tm(eng, T_GATHER);
         launch_gather(eng->dp, cur_buf(eng), nxt_buf(eng), eng->d_start, eng->d_perm, eng->d_tile_begin,
-                      eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->stream);
+                      eng->tile_parcels, eng->cap, eng->d_S, eng->d_big_sub, eng->general, eng->d_slot, eng->stream);
     }
     if ((do_collide || do_sample) && react) {
         StageTimer tm(eng, T_COLLIDE);

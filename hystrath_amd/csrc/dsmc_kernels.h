@@ -32,6 +32,7 @@ struct DevScalars {
     int32_t ord[2];   // heavy / light tile counters of the collide order (cleared by commit)
     int32_t qn;       // flights queued for the wall pass of the move (cleared by commit)
     int32_t nprod;    // dissociation products staged by the reacting collide (cleared after placement)
+    int32_t nbigprod; // cells of more than 64 items with products, listed for their wave sums (cleared likewise)
     unsigned long long counters[CNT_N];
 };
 
@@ -76,10 +77,11 @@ void launch_scan_cells(int32_t* cnt, int32_t* start, int32_t n, unsigned long lo
                        int pt_c, int32_t* tile_c, int32_t* heavy, hipStream_t st);
 void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g, int pt_c, int32_t* tile_begin_c,
                   int32_t* heavy, hipStream_t st);
-// stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel
+// stable cell order: rank + gather of src into dst in cell order, sub-cell byte per parcel;
+// scratch: capacity 32-bit words free during the gather (the histogram slots)
 void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
                    const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
-                   int general, hipStream_t st);
+                   int general, int32_t* scratch, hipStream_t st);
 // NTC, collisions and sampling on the sorted population in dst
 void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
                     int32_t* heavy, int32_t* order, int pt, int64_t capacity, uint8_t* sub, double* sigma,
@@ -99,7 +101,8 @@ struct DevProducts {
     float* uz;
     int32_t* count;   // [nc] products per cell
     int32_t* off;     // [nc+1] exclusive prefix of count
-    int32_t* list;    // [cap] sub-cell member lists of the reacting collide
+    int32_t* list;    // [cap] 32-bit sub-cell member lists of cells of more than 65,535 parcels
+    int32_t* bigcells;  // [nc] cells of more than 64 items with products (their sums by a wave)
 };
 // NTC with QK chemistry on the sorted population in dst (the collide tiles
 // of launch_collide, the variant with internal energies); then the products'

@@ -964,23 +964,6 @@ void launch_scatter(DevParticles b, DevScalars* S, const int32_t* start, const i
 #define CT_CELLS 256     // cells per sub-tile: one thread per cell
 #define DENSE_CAND 32    // cells with more candidates use the wave-cooperative NTC
 
-// 8 sub-cell counters packed as 16-bit fields in two 64-bit words
-struct Pack8 {
-    unsigned long long lo, hi;
-    __device__ __forceinline__ void zero() { lo = 0; hi = 0; }
-    __device__ __forceinline__ int get(int s) const {
-        return s < 4 ? (int)((lo >> (16 * s)) & 0xFFFF) : (int)((hi >> (16 * (s - 4))) & 0xFFFF);
-    }
-    __device__ __forceinline__ void inc(int s) {
-        if (s < 4) lo += 1ull << (16 * s);
-        else hi += 1ull << (16 * (s - 4));
-    }
-    __device__ __forceinline__ void set(int s, int v) {
-        if (s < 4) lo = (lo & ~(0xFFFFull << (16 * s))) | ((unsigned long long)v << (16 * s));
-        else hi = (hi & ~(0xFFFFull << (16 * (s - 4)))) | ((unsigned long long)v << (16 * (s - 4)));
-    }
-};
-
 struct CellCtx {
     unsigned long long cand, colls, ovf;
 };
@@ -1006,14 +989,6 @@ __device__ __forceinline__ int select_bit(unsigned long long m, int k) {
 
 // The members of each sub-cell of one cell, in ascending index order: cnt(q)
 // and member(q, k), the k-th member's index in the cell.
-// PackTab: one lane's packed sizes/offsets into a member list.
-struct PackTab {
-    Pack8 c, o;
-    const uint16_t* slist;
-    int ss;
-    __device__ __forceinline__ int cnt(int q) const { return c.get(q); }
-    __device__ __forceinline__ int member(int q, int k) const { return slist[ss + o.get(q) + k]; }
-};
 // CellTab: the LDS table of a cell -- for at most 64 parcels, a membership
 // bit mask per sub-cell; for more, (offset | count << 16) into the member list.
 struct CellTab {
@@ -1027,6 +1002,15 @@ struct CellTab {
     __device__ __forceinline__ int member(int q, int k) const {
         return small ? select_bit(m[q], k) : (int)slist[ss + (int)(m[q] & 0xFFFFull) + k];
     }
+};
+
+// CellTab32: the table of a cell of any size in global memory: per sub-cell
+// (offset | count << 32) into a 32-bit member list.
+struct CellTab32 {
+    const unsigned long long* m;
+    const int32_t* slist;
+    __device__ __forceinline__ int cnt(int q) const { return (int)(m[q] >> 32); }
+    __device__ __forceinline__ int member(int q, int k) const { return slist[(int)(m[q] & 0xFFFFFFFFull) + k]; }
 };
 
 // Candidate count of noTimeCounter.C:142-153 from the cell's sigmaTcRMax,
@@ -1141,73 +1125,6 @@ __device__ __forceinline__ void react_collision(const DevParams& P, PhiloxStream
     if (reacted) { ty[ip] = (uint8_t)a.t; ty[iq] = (uint8_t)b.t; }
 }
 
-// Sub-cell lists of one cell built by one lane (ascending index per sub-cell).
-__device__ __forceinline__ void subcell_lists_lane(const uint8_t* sub, uint16_t* slist, int ss, int nC, Pack8& cnt,
-                                                   Pack8& off) {
-    cnt.zero();
-    for (int k = 0; k < nC; ++k) cnt.inc(sub[ss + k]);
-    off.zero();
-    int acc = 0;
-    for (int q = 0; q < 8; ++q) { off.set(q, acc); acc += cnt.get(q); }
-    Pack8 fill = off;
-    for (int k = 0; k < nC; ++k) {
-        const int q = sub[ss + k];
-        slist[ss + fill.get(q)] = (uint16_t)k;
-        fill.inc(q);
-    }
-}
-
-// NTC of one cell by one lane, candidate after candidate.  REACT: with
-// chemistry; the cell's products are staged in candidate order and counted
-// in *nprod.
-template <bool GEN, bool REACT = false>
-__device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
-                         float* vz, const uint8_t* sub, uint16_t* slist, float* er, uint16_t* vb, int vstride,
-                         const uint8_t* ty, double* sigma, uint32_t step, const DevParticles& pos, int64_t gs,
-                         CellCtx& cx, double& ncoll, double& sep, const ProdOut* po = nullptr,
-                         int* nprod = nullptr) {
-    PackTab tab;
-    subcell_lists_lane(sub, slist, ss, nC, tab.c, tab.o);
-    tab.slist = slist;
-    tab.ss = ss;
-    double smax_cur = smax;
-    cx.cand += (unsigned long long)ncand;
-    for (long long k = 0; k < ncand; ++k) {
-        PhiloxStream r;
-        r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-        int pp, qq;
-        if (!draw_pair<GEN>(P, r, nC, sub, ss, tab, ty, &pp, &qq)) continue;
-        const int ip = ss + pp, iq = ss + qq;
-        const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
-        double up[3] = { (double)vx[ip], (double)vy[ip], (double)vz[ip] };
-        double uq[3] = { (double)vx[iq], (double)vy[iq], (double)vz[iq] };
-        const double stcr = sigma_tcr(P, tp, tq, up, uq);
-        if (stcr > smax_cur) smax_cur = stcr;
-        if (stcr / smax > r.u()) {
-            if (REACT) {
-                PState prod;
-                int fq = 0;
-                react_collision(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, const_cast<uint8_t*>(ty), prod,
-                                &fq);
-                if (prod.t >= 0) {
-                    stage_product(P, *po, c, *nprod, gs + (fq ? qq : pp), prod);
-                    *nprod += 1;
-                }
-            } else {
-                do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
-            }
-            ncoll += 1.0;
-            if (P.flags & HS_FLAG_COLL_SEP) {
-                const int64_t gp = gs + pp, gq = gs + qq;
-                const double ddx = pos.px[gp] - pos.px[gq], ddy = pos.py[gp] - pos.py[gq], ddz = pos.pz[gp] - pos.pz[gq];
-                sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            }
-            cx.colls += 1;
-        }
-    }
-    sigma[c] = smax_cur;
-}
-
 // NTC of one dense cell by a whole wave, 64 candidates per batch.  A
 // candidate's pair comes from its own stream alone, so each lane draws its
 // pair once; only the acceptance test and the collision need the velocities.
@@ -1224,12 +1141,16 @@ __device__ void ntc_lane(const DevParams& P, int c, int ss, int nC, long long nc
 // candidate order (a prefix over the batch's lanes), counted in *nprod_out.
 // FENCE: the parcel arrays are in global memory; a workgroup-scope fence after
 // every round makes one round's writes visible to the next round's lanes.
-template <bool GEN, bool REACT = false, bool FENCE = false>
+// The stamp of parcel x is stamp[(ss + x) & smask]: with a mask smaller than
+// the cell, parcels share stamps, which only holds a candidate back a round
+// more (the earliest pending candidate is always ready, and none runs before
+// an earlier one that shares a parcel with it).
+template <bool GEN, bool REACT = false, bool FENCE = false, class Tab = CellTab>
 __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long ncand, double smax, float* vx, float* vy,
-                         float* vz, const uint8_t* sub, const CellTab& tab, float* er,
+                         float* vz, const uint8_t* sub, const Tab& tab, float* er,
                          uint16_t* vb, int vstride, const uint8_t* ty, int32_t* stamp, double* sigma, uint32_t step,
                          const DevParticles& pos, int64_t gs, CellCtx& cx, uint32_t& ncoll_out, double& sep_out,
-                         const ProdOut* po = nullptr, int* nprod_out = nullptr) {
+                         const ProdOut* po = nullptr, int* nprod_out = nullptr, int smask = 0x7FFFFFFF) {
     int nprod = 0;
     const int lane = threadIdx.x & (WAVE - 1);
     const bool sep_on = (P.flags & HS_FLAG_COLL_SEP) != 0;
@@ -1252,15 +1173,15 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
         int fq = 0;
         while (__ballot(pend)) {
             if (pend) {
-                atomicMin(&stamp[ss + pp], lane);
-                atomicMin(&stamp[ss + qq], lane);
+                atomicMin(&stamp[(ss + pp) & smask], lane);
+                atomicMin(&stamp[(ss + qq) & smask], lane);
             }
             __builtin_amdgcn_wave_barrier();
-            const bool ready = pend && stamp[ss + pp] == lane && stamp[ss + qq] == lane;
+            const bool ready = pend && stamp[(ss + pp) & smask] == lane && stamp[(ss + qq) & smask] == lane;
             __builtin_amdgcn_wave_barrier();
             if (pend) {
-                stamp[ss + pp] = 0x7FFFFFFF;
-                stamp[ss + qq] = 0x7FFFFFFF;
+                stamp[(ss + pp) & smask] = 0x7FFFFFFF;
+                stamp[(ss + qq) & smask] = 0x7FFFFFFF;
             }
             if (ready) {
                 const int ip = ss + pp, iq = ss + qq;
@@ -1474,10 +1395,12 @@ __device__ __forceinline__ double wave_tree_sum(double v) {
     return v;
 }
 
+// b2, n2: a second range of the cell's items after its nC parcels (the
+// products it made this step), taken in the same lane order.
 template <bool GEN>
 __device__ void sample_cell_wave(const DevParams& P, int c, int ss, int nC, const float* vx, const float* vy,
                                  const float* vz, const float* er, const uint16_t* vb, int vstride, const uint8_t* ty,
-                                 double* samples, int nc) {
+                                 double* samples, int nc, int b2 = 0, int n2 = 0) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int ns = GEN ? P.ns : 1;
     const bool hf = (P.flags & HS_FLAG_HEATFLUX) != 0;
@@ -1488,8 +1411,8 @@ __device__ void sample_cell_wave(const DevParams& P, int c, int ss, int nC, cons
         double h[HS_HF_COUNT];
         for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = 0.0;
         const int nvib = GEN ? P.nvib[s] : 0;
-        for (int k = lane; k < nC; k += WAVE) {
-            const int i = ss + k;
+        for (int k = lane; k < nC + n2; k += WAVE) {
+            const int i = k < nC ? ss + k : b2 + (k - nC);
             if (GEN && (int)ty[i] != s) continue;
             const double ux = (double)vx[i], uy = (double)vy[i], uz = (double)vz[i];
             const double lke = m * (ux * ux + uy * uy + uz * uz);
@@ -1582,8 +1505,11 @@ __device__ __forceinline__ int rank_in(const int32_t* key, int ss, int ee, int k
 // Sub-cell member lists of one cell (ss, nC) by a whole wave: counts and
 // ascending fills from ballots over 64-parcel windows; tab[q] gets
 // (offset | count << 16) of sub-cell q.
-__device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t* slist, int ss, int nC,
+template <class IDX = uint16_t>
+__device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, IDX* slist, int ss, int nC,
                                                    unsigned long long* tab) {
+    // 16-bit lists: (offset | count << 16); 32-bit: (offset | count << 32)
+    const int shift = sizeof(IDX) == 2 ? 16 : 32;
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
     int myc = 0;
@@ -1603,7 +1529,7 @@ __device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t*
         for (int qq = 0; qq < 8; ++qq) {
             const int cq = __shfl(myc, qq, WAVE);
             run[qq] = acc;
-            if (lane == qq) tab[qq] = (unsigned long long)acc | ((unsigned long long)cq << 16);
+            if (lane == qq) tab[qq] = (unsigned long long)acc | ((unsigned long long)cq << shift);
             acc += cq;
         }
     }
@@ -1613,7 +1539,7 @@ __device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, uint16_t*
 #pragma unroll
         for (int qq = 0; qq < 8; ++qq) {
             const unsigned long long m = __ballot(q == qq);
-            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (uint16_t)j;
+            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (IDX)j;
             run[qq] += __popcll(m);
         }
     }
@@ -1791,10 +1717,74 @@ __device__ __forceinline__ void copy_internal(const DevParams& P, const DevParti
     if (dst.type) dst.type[gd] = src.type[k];
 }
 
+// Ranks in a cell larger than a chunk (keys = its parcels' old indices, all
+// distinct).  When the keys span at most 16 times the parcel count, a bitmap
+// over the key range and the prefix of its words' bit counts go to the
+// cell's own stretch of `scratch` (the histogram slots, free once the scatter
+// has read them), and a key's rank is the prefix of its word plus the bits
+// below it in the word.  Returns false (nothing built) otherwise.
+struct BigRanks {
+    const uint32_t* bm;
+    const uint32_t* pre;
+    int kmin;
+    __device__ __forceinline__ int rank(int k) const {
+        const int d = k - kmin;
+        return (int)pre[d >> 5] + __popc(bm[d >> 5] & ((1u << (d & 31)) - 1u));
+    }
+};
+
+__device__ __forceinline__ bool big_cell_ranks(const int32_t* __restrict__ perm, int32_t* __restrict__ scratch, int S0,
+                                               int nC, BigRanks* out) {
+    __shared__ int s_min, s_max;
+    const int t = threadIdx.x;
+    if (t == 0) { s_min = 0x7FFFFFFF; s_max = -1; }
+    __syncthreads();
+    int lo = 0x7FFFFFFF, hi = -1;
+    for (int j = t; j < nC; j += CT) {
+        const int k = perm[S0 + j];
+        lo = min(lo, k);
+        hi = max(hi, k);
+    }
+    atomicMin(&s_min, lo);
+    atomicMax(&s_max, hi);
+    __syncthreads();
+    const int kmin = s_min;
+    const int64_t range = (int64_t)s_max - kmin + 1;
+    const int nw = (int)((range + 31) / 32);
+    __syncthreads();  // every thread has read s_min, s_max
+    if (2 * (int64_t)nw > nC) return false;
+    uint32_t* bm = reinterpret_cast<uint32_t*>(scratch + S0);
+    uint32_t* pre = bm + nw;
+    for (int w = t; w < nw; w += CT) bm[w] = 0u;
+    __threadfence_block();
+    __syncthreads();
+    for (int j = t; j < nC; j += CT) {
+        const int d = perm[S0 + j] - kmin;
+        atomicOr(&bm[d >> 5], 1u << (d & 31));
+    }
+    __threadfence_block();
+    __syncthreads();
+    int running = 0;
+    for (int w0 = 0; w0 < nw; w0 += CT) {
+        const int w = w0 + t;
+        const int v = w < nw ? __popc(bm[w]) : 0;
+        int tot;
+        const int ex = block_excl_scan(v, &tot);
+        if (w < nw) pre[w] = (uint32_t)(running + ex);
+        running += tot;
+    }
+    __threadfence_block();
+    __syncthreads();
+    out->bm = bm;
+    out->pre = pre;
+    out->kmin = kmin;
+    return true;
+}
+
 template <bool GEN>
 __device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell, const DevParticles& src,
                              DevParticles& dst, const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
-                             uint8_t* __restrict__ sub_out) {
+                             uint8_t* __restrict__ sub_out, int32_t* __restrict__ scratch) {
     const int t = threadIdx.x;
     if (t < ncell) L.segS[t] = start[c0 + t];
     if (t == 0) L.segS[ncell] = start[c0 + ncell];
@@ -1808,10 +1798,16 @@ __device__ void gather_cells(GatherLds& L, const DevParams& P, int c0, int ncell
             // one cell larger than a chunk: ranks from global memory
             const int nC = L.segS[a + 1] - S0;
             const int c = c0 + a;
+            BigRanks br;
+            const bool bitmap = big_cell_ranks(perm, scratch, S0, nC, &br);
             for (int j = t; j < nC; j += CT) {
                 const int k = perm[S0 + j];
                 int rank = 0;
-                for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+                if (bitmap) {
+                    rank = br.rank(k);
+                } else {
+                    for (int q = 0; q < nC; ++q) rank += (perm[S0 + q] < k) ? 1 : 0;
+                }
                 const int64_t gd = (int64_t)S0 + rank;
                 const double x = src.px[k], y = src.py[k], z = src.pz[k];
                 // positions and cells are read again only by the next step's
@@ -1881,7 +1877,7 @@ __device__ __forceinline__ void gather_body(const DevParams* __restrict__ Pp, De
                                             const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
                                             const int32_t* __restrict__ tile_begin, int pt,
                                             const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out,
-                                            const int q) {
+                                            const int q, int32_t* __restrict__ scratch) {
     __shared__ GatherLds L;
     const DevParams& P = *Pp;
     // the tile bounds are read beside the population, not after it (q + 1 is
@@ -1890,42 +1886,22 @@ __device__ __forceinline__ void gather_body(const DevParams* __restrict__ Pp, De
     const int cb = tile_begin[q], ce = tile_begin[q + 1];
     if ((int64_t)q * pt > (int64_t)S->n_sorted) return;
     for (int c0 = cb; c0 < ce; c0 += CT_CELLS)
-        gather_cells<GEN>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sub_out);
+        gather_cells<GEN>(L, P, c0, min(CT_CELLS, ce - c0), src, dst, start, perm, sub_out, scratch);
 }
 
 template <bool GEN>
 __global__ __launch_bounds__(CT) void k_gather(const DevParams* __restrict__ Pp, DevParticles src, DevParticles dst,
                                                const int32_t* __restrict__ start, const int32_t* __restrict__ perm,
                                                const int32_t* __restrict__ tile_begin, int pt,
-                                               const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out) {
-    gather_body<GEN>(Pp, src, dst, start, perm, tile_begin, pt, S, sub_out, (int)blockIdx.x);
+                                               const DevScalars* __restrict__ S, uint8_t* __restrict__ sub_out,
+                                               int32_t* __restrict__ scratch) {
+    gather_body<GEN>(Pp, src, dst, start, perm, tile_begin, pt, S, sub_out, (int)blockIdx.x, scratch);
 }
 
 // ---------------------------------------------------------------------------
-// collide with QK chemistry (dsmcReactions), global-memory parts: parcel
-// states, the one-pass cell sums and the NTC of a cell larger than the LDS
-// tiles (the reaction branch of noTimeCounter.C:250-300)
+// collide with QK chemistry (dsmcReactions): the one-pass sums of a cell and
+// its products in global memory
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void load_state(const DevParams& P, const DevParticles& d, int64_t i, int64_t cap,
-                                           PState& a) {
-    a.t = d.type ? (int)d.type[i] : 0;
-    a.u[0] = (double)d.ux[i]; a.u[1] = (double)d.uy[i]; a.u[2] = (double)d.uz[i];
-    a.er = d.erot ? (double)d.erot[i] : 0.0;
-#pragma unroll
-    for (int m = 0; m < HS_MAX_VIB_MODES; ++m) a.v[m] = 0;
-    for (int m = 0; m < P.nvib[a.t]; ++m) a.v[m] = d.vib[m * cap + i];
-}
-
-// after a reaction the type and every vibrational plane are rewritten too
-__device__ __forceinline__ void store_state(const DevParams& P, DevParticles& d, int64_t i, int64_t cap,
-                                            const PState& a, bool reacted) {
-    d.ux[i] = (float)a.u[0]; d.uy[i] = (float)a.u[1]; d.uz[i] = (float)a.u[2];
-    if (d.erot) d.erot[i] = (float)a.er;
-    const int nm = reacted ? P.max_modes : P.nvib[a.t];
-    for (int m = 0; m < nm; ++m) d.vib[m * cap + i] = (uint16_t)a.v[m];
-    if (reacted && d.type) d.type[i] = (uint8_t)a.t;
-}
-
 // The cell's sums without the heat-flux terms in one pass: parcels
 // b .. b+nC-1, then b2 .. b2+n2-1, each added to its species' accumulators
 // (every species' sums keep the parcel order: the same values as a pass per
@@ -1970,84 +1946,17 @@ __device__ __forceinline__ void sample_cell_react(const DevParams& P, const DevP
     }
 }
 
-// NTC with chemistry of one cell of any size by one thread, on the sorted
-// population in global memory (sub-cell member lists in pr.list, 32-bit);
-// the cell's products are staged in candidate order and counted in nprod.
-__device__ void ntc_cell_react(const DevParams& P, DevParticles& dst, int c, int32_t b, int32_t nC,
-                               const uint8_t* __restrict__ sub, double* __restrict__ sigma,
-                               double* __restrict__ rem, const double* __restrict__ vol, const ProdOut& po,
-                               uint32_t step, CellCtx& cx, double& ncoll, double& sep, int& nprod) {
-    const int64_t cap = P.capacity;
-    // sub-cell member lists in ascending index order (noTimeCounter.C:114-138)
-    int scn[8], sco[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) scn[q] = 0;
-    for (int k = 0; k < nC; ++k) scn[sub[b + k]] += 1;
-    int acc = 0;
-#pragma unroll
-    for (int q = 0; q < 8; ++q) { sco[q] = acc; acc += scn[q]; }
-    {
-        int fill[8];
-#pragma unroll
-        for (int q = 0; q < 8; ++q) fill[q] = sco[q];
-        for (int k = 0; k < nC; ++k) po.pr.list[b + fill[sub[b + k]]++] = k;
-    }
-    double smax;
-    const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-    double smax_cur = smax;
-    cx.cand += (unsigned long long)ncand;
-    for (long long k = 0; k < ncand; ++k) {
-        PhiloxStream r;
-        r.init(P, RNG_COLLIDE, (uint32_t)c, step, (uint32_t)k);
-        const int pp = random_label(r, 0, nC - 1);
-        const int q = sub[b + pp];
-        const int nsc = scn[q];
-        int qq = -1, guard = 0;
-        if (nsc > 1) {
-            do { qq = po.pr.list[b + sco[q] + random_label(r, 0, nsc - 1)]; }
-            while (pp == qq && ++guard < HS_MAX_REDRAW);
-        } else {
-            do { qq = random_label(r, 0, nC - 1); } while (pp == qq && ++guard < HS_MAX_REDRAW);
-        }
-        if (pp == qq) continue;
-        const int64_t ip = (int64_t)b + pp, iq = (int64_t)b + qq;
-        PState a, bb;
-        load_state(P, dst, ip, cap, a);
-        load_state(P, dst, iq, cap, bb);
-        if (P.charge[a.t] == -1 && P.charge[bb.t] == -1) continue;
-        const double stcr = sigma_tcr(P, a.t, bb.t, a.u, bb.u);
-        if (stcr > smax_cur) smax_cur = stcr;
-        if (stcr / smax > r.u()) {
-            PState prod;
-            int from_q = 0;
-            const bool reacted = collide_or_react(P, a, bb, r, prod, &from_q);
-            store_state(P, dst, ip, cap, a, reacted);
-            store_state(P, dst, iq, cap, bb, reacted);
-            if (prod.t >= 0) {
-                stage_product(P, po, c, nprod, from_q ? iq : ip, prod);
-                nprod += 1;
-            }
-            ncoll += 1.0;
-            if (P.flags & HS_FLAG_COLL_SEP) {
-                const double ddx = dst.px[ip] - dst.px[iq], ddy = dst.py[ip] - dst.py[iq], ddz = dst.pz[ip] - dst.pz[iq];
-                sep += sqrt(ddx * ddx + ddy * ddy + ddz * ddz);
-            }
-            cx.colls += 1;
-        }
-    }
-    sigma[c] = smax_cur;
-}
-
 // ---------------------------------------------------------------------------
 // k_collide: NTC, collisions and sampling on the sorted population
 // ---------------------------------------------------------------------------
-// One cell with more parcels than an LDS chunk holds, in global memory.  Up
-// to 65,535 parcels and as many 32-bit stamps as the chunk's LDS arrays hold
-// (lds_ints), the first wave runs the wave-cooperative NTC (ntc_wave) on the
-// cell's parcels in place, with its conflict stamps in those arrays (unused
-// while this cell is handled) and the sub-cell lists built by the wave.  Larger
-// cells run on one lane: with chemistry any size (32-bit lists), without it
-// up to 65,535 parcels (beyond: counted as overflow, not collided).  Then the
+// One cell with more parcels than an LDS chunk holds, in global memory: the
+// first wave runs the wave-cooperative NTC (ntc_wave) on the cell's parcels
+// in place.  Its conflict stamps are the chunk's LDS arrays (lds_ints 32-bit
+// words, unused while this cell is handled): one per parcel when the cell
+// fits, else shared by parcels whose index agrees modulo a power of two
+// (which only delays candidates).  The wave builds the sub-cell lists: 16-bit
+// up to 65,535 parcels; beyond, with chemistry, 32-bit (any size), and
+// without it the cell is counted as overflow and not collided.  Then the
 // first wave sums the cell; with chemistry a cell that made products is
 // summed after they are placed (k_sample_react).
 template <bool GEN, bool REACT = false>
@@ -2070,11 +1979,18 @@ __device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int 
     __shared__ long long s_ncand;
     __shared__ int s_nprod;
     double ncoll = 0.0, sep = 0.0;
-    const bool wave_path = nC <= 65535 && nC <= lds_ints;
+    const bool lists16 = nC <= 65535;
     if (t == 0) s_nprod = 0;
-    if (do_collide && nC > 1 && wave_path) {
-        if (t < WAVE) subcell_lists_wave(sub + S0, big_list + S0, 0, nC, s_tab);
-        for (int j = t; j < nC; j += CT) lds[j] = 0x7FFFFFFF;
+    if (do_collide && nC > 1 && (lists16 || REACT)) {
+        int hash = 1;
+        while (hash * 2 <= lds_ints) hash *= 2;
+        const int nstamp = nC <= lds_ints ? nC : hash;
+        const int smask = nC <= lds_ints ? 0x7FFFFFFF : hash - 1;
+        if (t < WAVE) {
+            if (lists16) subcell_lists_wave<uint16_t>(sub + S0, big_list + S0, 0, nC, s_tab);
+            else subcell_lists_wave<int32_t>(sub + S0, po->pr.list + S0, 0, nC, s_tab);
+        }
+        for (int j = t; j < nstamp; j += CT) lds[j] = 0x7FFFFFFF;
         if (t == 0) {
             double smax;
             s_ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
@@ -2082,30 +1998,25 @@ __device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int 
         }
         __syncthreads();
         if (t < WAVE) {
-            const CellTab tab = { s_tab, big_list + S0, 0, false };
             uint32_t nco = 0;
             double sp = 0.0;
             int np = 0;
-            ntc_wave<GEN, REACT, true>(P, c, 0, nC, s_ncand, s_smax, gvx, gvy, gvz, sub + S0, tab, ger, gvb, (int)cap,
-                                       gty, lds, sigma, step, dst, S0, cx, nco, sp, po, &np);
+            if (lists16) {
+                const CellTab tab = { s_tab, big_list + S0, 0, false };
+                ntc_wave<GEN, REACT, true>(P, c, 0, nC, s_ncand, s_smax, gvx, gvy, gvz, sub + S0, tab, ger, gvb,
+                                           (int)cap, gty, lds, sigma, step, dst, S0, cx, nco, sp, po, &np, smask);
+            } else {
+                const CellTab32 tab = { s_tab, po->pr.list + S0 };
+                ntc_wave<GEN, REACT, true, CellTab32>(P, c, 0, nC, s_ncand, s_smax, gvx, gvy, gvz, sub + S0, tab, ger,
+                                                      gvb, (int)cap, gty, lds, sigma, step, dst, S0, cx, nco, sp, po,
+                                                      &np, smask);
+            }
             ncoll = (double)nco;
             sep = sp;
             if (REACT && t == 0) s_nprod = np;
         }
-    } else if (t == 0) {
-        if (REACT) {
-            int np = 0;
-            if (do_collide && nC > 1)
-                ntc_cell_react(P, dst, c, S0, nC, sub, sigma, rem, vol, *po, step, cx, ncoll, sep, np);
-            s_nprod = np;
-        } else if (nC > 65535) {
-            cx.ovf += 1;
-        } else if (do_collide && nC > 1) {
-            double smax;
-            const long long ncand = ntc_candidates(P, c, nC, sigma, rem, vol, &smax, cx);
-            ntc_lane<GEN>(P, c, 0, nC, ncand, smax, gvx, gvy, gvz, sub + S0, big_list + S0, ger, gvb, (int)cap, gty,
-                          sigma, step, dst, S0, cx, ncoll, sep);
-        }
+    } else if (t == 0 && !lists16 && !REACT) {
+        cx.ovf += 1;
     }
     if (t == 0) {
         if (REACT) po->pr.count[c] = s_nprod;
@@ -2118,7 +2029,7 @@ __device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int 
     __syncthreads();
     // the cell's sums by the first wave
     const bool summed_later = REACT && s_nprod > 0;
-    if (do_sample && !summed_later && (REACT || nC <= 65535) && t < WAVE)
+    if (do_sample && !summed_later && (REACT || lists16) && t < WAVE)
         sample_cell_wave<GEN>(P, c, 0, nC, gvx, gvy, gvz, ger, gvb, (int)cap, gty, samples, nc);
     __syncthreads();
 }
@@ -2406,12 +2317,12 @@ void launch_tiles(const int32_t* start, int nc, int pt_g, int32_t* tile_begin_g,
 
 void launch_gather(const DevParams* P, DevParticles src, DevParticles dst, const int32_t* start, const int32_t* perm,
                    const int32_t* tile_begin, int pt, int64_t capacity, const DevScalars* S, uint8_t* sub,
-                   int general, hipStream_t st) {
+                   int general, int32_t* scratch, hipStream_t st) {
     const int nt = (int)(capacity / pt) + 2;
     if (general)
-        k_gather<true><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
+        k_gather<true><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub, scratch);
     else
-        k_gather<false><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub);
+        k_gather<false><<<nt, CT, 0, st>>>(P, src, dst, start, perm, tile_begin, pt, S, sub, scratch);
 }
 
 void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
@@ -2458,7 +2369,7 @@ __global__ __launch_bounds__(BLOCK) void k_place_products(const DevParams* __res
 template <int MAXS>
 __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restrict__ Pp, DevParticles dst,
                                                         const int32_t* __restrict__ start,
-                                                        const DevScalars* __restrict__ S, DevProducts pr,
+                                                        DevScalars* __restrict__ S, DevProducts pr,
                                                         double* __restrict__ samples, int nc) {
     const DevParams& P = *Pp;
     const int c = blockIdx.x * BLOCK + threadIdx.x;
@@ -2471,6 +2382,11 @@ __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restr
     const int64_t b2 = (int64_t)S->n_sorted + pr.off[c];
     int n2 = pr.count[c];
     if (b2 + n2 > cap) n2 = b2 < cap ? (int)(cap - b2) : 0;
+    if (nC + n2 > 64) {
+        // summed by a wave (k_sample_react_big)
+        pr.bigcells[atomicAdd(&S->nbigprod, 1)] = c;
+        return;
+    }
     if (!hf) {
         sample_cell_react<MAXS>(P, dst, b, nC, b2, n2, samples, nc, c);
         return;
@@ -2524,12 +2440,36 @@ __global__ __launch_bounds__(BLOCK) void k_sample_react(const DevParams* __restr
     }
 }
 
-// the population after this step's products; the staging counter cleared
+// The sums of a cell of more than 64 items (parcels and products) that made
+// products, by one wave in the lane order of sample_cell_wave; the cells
+// listed by k_sample_react, one wave each over a fixed grid.
+__global__ __launch_bounds__(BLOCK) void k_sample_react_big(const DevParams* __restrict__ Pp, DevParticles dst,
+                                                            const int32_t* __restrict__ start,
+                                                            const DevScalars* __restrict__ S, DevProducts pr,
+                                                            double* __restrict__ samples, int nc) {
+    const DevParams& P = *Pp;
+    const int64_t cap = P.capacity;
+    const int nw = gridDim.x * (BLOCK / WAVE);
+    const int nlist = S->nbigprod;
+    for (int w = blockIdx.x * (BLOCK / WAVE) + threadIdx.x / WAVE; w < nlist; w += nw) {
+        const int c = pr.bigcells[w];
+        const int b = start[c];
+        const int nC = start[c + 1] - b;
+        const int64_t b2 = (int64_t)S->n_sorted + pr.off[c];
+        int n2 = pr.count[c];
+        if (b2 + n2 > cap) n2 = b2 < cap ? (int)(cap - b2) : 0;
+        sample_cell_wave<true>(P, c, b, nC, dst.ux, dst.uy, dst.uz, dst.erot, dst.vib, (int)cap, dst.type, samples, nc,
+                               (int)b2, n2);
+    }
+}
+
+// the population after this step's products; the staging counters cleared
 __global__ void k_products_commit(DevScalars* S, const int32_t* __restrict__ off, int nc, int64_t cap) {
     const int64_t nn = (int64_t)S->n_sorted + off[nc];
     S->n_sorted = (int32_t)(nn > cap ? cap : nn);
     if (nn > cap) atomicAdd(&S->counters[CNT_CAP_OVF], (unsigned long long)(nn - cap));
     S->nprod = 0;
+    S->nbigprod = 0;
 }
 
 void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* start, const int32_t* tile_begin,
@@ -2545,6 +2485,7 @@ void launch_collide_react(const DevParams* P, DevParticles dst, const int32_t* s
     if (do_sample) {
         if (ns <= 5) k_sample_react<5><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
         else k_sample_react<HS_MAX_SPECIES><<<g, BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
+        k_sample_react_big<<<grid_for(nc, 1024), BLOCK, 0, st>>>(P, dst, start, S, pr, samples, nc);
     }
     k_products_commit<<<1, 1, 0, st>>>(S, pr.off, nc, capacity);
 }
@@ -2831,7 +2772,7 @@ __global__ __launch_bounds__(CT) void kb_gather(const BatchCase* __restrict__ C,
     const HS_CONST_AS BatchCase& c = batch_case(C, T, bx, gx);
     DevParticles src = c.buf[(c.cur0 ^ par) & 1];
     DevParticles dst = c.buf[(c.cur0 ^ par ^ 1) & 1];
-    gather_body<GEN>(c.P, src, dst, c.start, c.perm, c.tile_g, c.pt_g, c.S, c.sub, bx);
+    gather_body<GEN>(c.P, src, dst, c.start, c.perm, c.tile_g, c.pt_g, c.S, c.sub, bx, c.slot);
 }
 
 template <bool GEN, int PMAX>

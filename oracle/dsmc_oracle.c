@@ -1606,10 +1606,11 @@ static void sample_cell(const hs_config* cfg, const or_derived* d, or_state* s, 
      * lane r adds parcels r, r+4, ... and the partial sums are combined as
      * (p0 + p2) + (p1 + p3), except in the kernel variant with internal
      * energies, where one lane adds its parcels in order.  With reactions a
-     * cell that made products this step is summed by one lane, its parcels
-     * and then its products (b2 .. b2+n2-1) in order; the other cells as
-     * above. */
-    const int lanes = (d->n_react && n2 > 0) ? 1 : (nC > 64 ? 64 : (sums_general(cfg) ? 1 : 4));
+     * cell that made products this step is summed over its parcels and then
+     * its products (b2 .. b2+n2-1): by a wave as above when there are more
+     * than 64 of them, else by one lane in order; the other cells as above. */
+    const int lanes = (d->n_react && n2 > 0) ? (nC + n2 > 64 ? 64 : 1)
+                                            : (nC > 64 ? 64 : (sums_general(cfg) ? 1 : 4));
     const int hf = (cfg->flags & HS_FLAG_HEATFLUX) != 0;
     enum { NQ = HS_SS_COUNT + HS_HF_COUNT };
     const int nq = hf ? NQ : HS_SS_COUNT;

@@ -86,6 +86,24 @@ def test_reacting_steps_match_the_oracle(which):
     eng.close()
 
 
+@pytest.mark.parametrize("fn", [700.0, 210.0])
+def test_one_large_reacting_cell_matches_the_oracle(fn):
+    """The tutorial's single cell at about 22,000 parcels (Fn 700) and 73,000
+    (Fn 210): more parcels than the LDS tiles and the per-parcel conflict
+    stamps hold, so the wave-cooperative NTC shares stamps between parcels,
+    and at 73,000 builds 32-bit sub-cell lists; the cell's sums with its
+    products are taken by a wave."""
+    c = cases.heat_bath_5species(fn=fn)
+    eng, st = _pair(c, _fill_bath)
+    print(f"[chem] one cell, {st.n} parcels", flush=True)
+    for r, n in enumerate((1, 2)):
+        eng.evolve(n)
+        st.evolve(n)
+        _check(eng, st, f"fn{fn}/call{r}")
+    _check_end(eng, st, 3, f"fn{fn}")
+    eng.close()
+
+
 def test_stage_entry_points_with_reactions():
     """hs_dsmc_sort + hs_dsmc_collide (buildCellOccupancy, then
     noTimeCounter::collide with its reactions) equal the oracle's sort and
@@ -101,4 +119,41 @@ def test_stage_entry_points_with_reactions():
         st.step = st.step + 1
     np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
     assert st.n > int(sum(c.init_n) * 1e-15 / c.cfg.fn)
+    eng.close()
+
+
+def test_heat_bath_tutorial_at_full_size_follows_the_reference_run():
+    """The heatBath-5species tutorial as the reference ran it (one cell,
+    nEquivalentParticles 100, about 154,000 parcels) for 200 steps on the
+    engine: species number densities within the counting noise of both runs
+    (4 sigma plus 1%) of the reference's probe series
+    (tests/golden/heatbath_probes.npz), and the mixture's translational and
+    rotational temperatures, from this step's cell sums, within 2-3%."""
+    import os
+    from hystrath_amd import fields
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "heatbath_probes.npz"))
+    c = cases.heat_bath_5species(fn=100.0)
+    eng = Engine(c)
+    eng.fill(c.init_n, c.init_T)
+    V = 1e-15
+    done = 0
+    for k in (50, 100, 150, 200):
+        eng.evolve(k - 1 - done)
+        before = eng.samples()[0].copy()
+        eng.evolve(1)
+        done = k
+        f = fields.vol_fields(eng.samples()[0] - before, 1, c)
+        row = int(np.searchsorted(fx["step"], k))
+        counts = np.bincount(eng.download()["type"], minlength=5)
+        for i, nm in enumerate(["N2", "O2", "NO", "N", "O"]):
+            ref = fx["rhoN_" + nm][row]
+            n_ref = ref * V / 100.0
+            if n_ref < 50:
+                continue
+            tol = 4.0 * np.sqrt(1.0 / max(counts[i], 1) + 1.0 / n_ref) + 0.01
+            assert abs(counts[i] * c.cfg.fn / V / ref - 1.0) < tol, (k, nm)
+        for key, rel in (("Ttra", 0.02), ("Trot", 0.03)):
+            assert abs(float(f[key][0]) / fx[key + "_mixture"][row] - 1.0) < rel, (k, key)
+        print(f"[chem] heat bath step {k}: {eng.count()} parcels", flush=True)
+    assert eng.counters()["capacity_overflow"] == 0
     eng.close()

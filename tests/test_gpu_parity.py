@@ -173,11 +173,13 @@ def test_evolve_parity(which, steps):
     eng.close()
 
 
-@pytest.mark.parametrize("ppc,steps", [(600, 4), (5000, 2)])
+@pytest.mark.parametrize("ppc,steps", [(600, 4), (5000, 2), (15000, 2)])
 def test_dense_and_oversize_cell_parity(ppc, steps):
     """A single dense cell: 600 parcels run the wave-cooperative NTC (many
-    candidates per step), 5000 exceed an LDS tile and take the global-memory
-    path.  Both must equal the candidate-by-candidate oracle."""
+    candidates per step) in LDS; 5000 exceed an LDS tile and run it on the
+    parcels in global memory with a stamp per parcel; 15000 exceed the stamps
+    too, and parcels share them.  All must equal the candidate-by-candidate
+    oracle."""
     case = cases.periodic_box(n_side=1, L=0.01, ppc=ppc, seed=17)
     eng, st = _pair(case, H.fill_box)
     assert st.n == ppc
