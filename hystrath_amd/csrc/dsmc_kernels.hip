@@ -1269,6 +1269,9 @@ __device__ __forceinline__ void sample_cell_species(const DevParams& P, int c, i
         if (GEN && P.has_erot) aR += (double)er[i];
         for (int mo = 0; mo < nvib; ++mo) ev[mo] += (double)vb[mo * vstride + i] * HS_KB * P.theta_v[s][mo];
     }
+    // no parcel of the species: its sums are +0, and adding +0 changes no
+    // cumulative sum (they start at +0 and never become -0)
+    if (aN == 0.0) return;
     const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
     unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
     unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
@@ -1318,7 +1321,7 @@ __device__ __forceinline__ void sample_cell_quad(const DevParams& P, int c, int 
                                                  const float* vy, const float* vz, const float* er,
                                                  const uint16_t* vb, int vstride, const uint8_t* ty, double* samples,
                                                  int nc, double ncoll, double sep) {
-    if (r == 0) {
+    if (r == 0 && ncoll != 0.0) {
         unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
         unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
     }
@@ -1349,7 +1352,7 @@ __device__ __forceinline__ void sample_cell_quad(const DevParams& P, int c, int 
         if (GEN && P.has_erot) aR = quad_sum(aR);
         for (int mo = 0; mo < nvib; ++mo) ev[mo] = quad_sum(ev[mo]);
         const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
-        if (r == 0) {
+        if (r == 0 && aN != 0.0) {
             unsafeAtomicAdd(&samples[(base + HS_SS_N) * nc + c], aN);
             unsafeAtomicAdd(&samples[(base + HS_SS_MUX) * nc + c], aX);
             unsafeAtomicAdd(&samples[(base + HS_SS_MUY) * nc + c], aY);
@@ -1376,7 +1379,7 @@ __device__ __forceinline__ void sample_cell_quad(const DevParams& P, int c, int 
                 h[9] += eint * ux; h[10] += eint * uy; h[11] += eint * uz; h[12] += eint;
             }
             for (int q = 0; q < HS_HF_COUNT; ++q) h[q] = quad_sum(h[q]);
-            if (r == 0) {
+            if (r == 0 && aN != 0.0) {
                 const int64_t hb = HS_S_GLOBAL_COUNT + (int64_t)P.ns * HS_SS_COUNT + (int64_t)s * HS_HF_COUNT;
                 for (int q = 0; q < HS_HF_COUNT; ++q) unsafeAtomicAdd(&samples[(hb + q) * nc + c], h[q]);
             }
@@ -1438,6 +1441,7 @@ __device__ void sample_cell_wave(const DevParams& P, int c, int ss, int nC, cons
             }
         }
         aN = wave_tree_sum(aN);
+        if (aN == 0.0) continue;  // no parcel of the species: adding zeros changes no sum
         aX = wave_tree_sum(aX);
         aY = wave_tree_sum(aY);
         aZ = wave_tree_sum(aZ);
@@ -1940,6 +1944,7 @@ __device__ __forceinline__ void sample_cell_react(const DevParams& P, const DevP
 #pragma unroll
     for (int s = 0; s < MAXS; ++s) {
         if (s >= P.ns) break;
+        if (acc[s][HS_SS_N] == 0.0) continue;  // no parcel of the species: adding zeros changes no sum
         const int64_t base = HS_S_GLOBAL_COUNT + (int64_t)s * HS_SS_COUNT;
 #pragma unroll
         for (int q = 0; q < HS_SS_COUNT; ++q) unsafeAtomicAdd(&samples[(base + q) * nc + c], acc[s][q]);
@@ -2020,7 +2025,7 @@ __device__ __forceinline__ void collide_oversize(const DevParams& P, int c, int 
     }
     if (t == 0) {
         if (REACT) po->pr.count[c] = s_nprod;
-        if (do_sample) {
+        if (do_sample && ncoll != 0.0) {
             unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c], ncoll);
             unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c], sep);
         }
@@ -2177,7 +2182,7 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
                 for (int w = t; w < nw; w += CT) {
                     const int l = a + w / ns, sp = w - (w / ns) * ns;
                     const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
-                    if (sp == 0) {
+                    if (sp == 0 && L.rcoll[l] != 0u) {
                         unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
                         unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
                     }
@@ -2192,7 +2197,7 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
                     if (nC <= 64) {
                         sample_cell_quad<GEN>(P, c0 + l, ss, nC, r, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty,
                                               samples, nc, (double)L.rcoll[l], L.sep[l]);
-                    } else if (r == 0) {
+                    } else if (r == 0 && L.rcoll[l] != 0u) {
                         unsafeAtomicAdd(&samples[(int64_t)HS_S_NCOLL * nc + c0 + l], (double)L.rcoll[l]);
                         unsafeAtomicAdd(&samples[(int64_t)HS_S_COLLSEP * nc + c0 + l], L.sep[l]);
                     }
