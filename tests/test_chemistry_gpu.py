@@ -157,3 +157,20 @@ def test_heat_bath_tutorial_at_full_size_follows_the_reference_run():
         print(f"[chem] heat bath step {k}: {eng.count()} parcels", flush=True)
     assert eng.counters()["capacity_overflow"] == 0
     eng.close()
+
+
+def test_reacting_engines_stepped_together_match_their_oracles():
+    """hs_dsmc_evolve_batch with two reacting cases (the heat bath in one cell
+    and cut into 8, other seeds): each ends equal to its own oracle."""
+    from hystrath_amd.engine import evolve_batch
+    cs = [cases.heat_bath_5species(fn=3000.0, seed=5), cases.heat_bath_5species(fn=1500.0, n_side=2, seed=6)]
+    pairs = [_pair(c, _fill_bath) for c in cs]
+    engs = [e for e, _ in pairs]
+    for n in (3, 4):
+        evolve_batch(engs, n)
+        for k, (e, st) in enumerate(pairs):
+            st.evolve(n)
+            _check(e, st, f"batch/case{k}/+{n}")
+    for k, (e, st) in enumerate(pairs):
+        _check_end(e, st, 7, f"batch/case{k}")
+        e.close()
