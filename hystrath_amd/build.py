@@ -30,7 +30,7 @@ def hipcc():
 
 HOST = os.path.join(HERE, "host")
 BIN = os.path.join(HERE, "bin")
-HOST_LIB_SOURCES = ["dictionary.C", "lagrangianIO.C", "dsmcCloud.C"]
+HOST_LIB_SOURCES = ["dictionary.C", "lagrangianIO.C", "polyMesh.C", "dsmcCloud.C"]
 APPS = ["dsmcFoamPlus", "dsmcInitialisePlus"]
 
 
@@ -56,6 +56,9 @@ def build_host(verbose=False):
     outs.append((common + inc + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "lagrangianIO.C"),
                                  os.path.join(tools, "cloudCheck.C"), "-o", os.path.join(tbin, "cloudCheck")],
                  os.path.join(tbin, "cloudCheck")))
+    outs.append((common + inc + [os.path.join(HOST, "dictionary.C"), os.path.join(HOST, "polyMesh.C"),
+                                 os.path.join(tools, "meshCheck.C"), "-o", os.path.join(tbin, "meshCheck")],
+                 os.path.join(tbin, "meshCheck")))
     for cmd, out in outs:
         if verbose:
             print(" ".join(cmd), flush=True)

@@ -381,7 +381,7 @@ dsmcCloud::dsmcCloud(const std::string& caseDir, bool initialise) : case_(caseDi
     nTerminalOutputs_ = control.lookupOrDefault("nTerminalOutputs", 1);
     precision_ = control.lookupOrDefault("writePrecision", 10);
 
-    mesh_ = readBlockMesh(readDictionary(case_ + "/system/blockMeshDict"));
+    mesh_ = hasPolyMesh(case_) ? readPolyMesh(case_) : readBlockMesh(readDictionary(case_ + "/system/blockMeshDict"));
     cfg_.nx = mesh_.n[0];
     cfg_.ny = mesh_.n[1];
     cfg_.nz = mesh_.n[2];

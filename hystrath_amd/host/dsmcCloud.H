@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/hystrath_dsmc.h"
+#include "blockMesh.H"
 #include "dictionary.H"
 #include "lagrangianIO.H"
 
@@ -30,17 +31,6 @@ struct FieldModel {
     bool resetAtOutput = true;
     double resetAtOutputUntilTime = 1e300;
 };
-
-// The mesh: one axis-aligned hex block of system/blockMeshDict
-struct BlockMesh {
-    int n[3] = { 1, 1, 1 };
-    double lo[3] = { 0, 0, 0 }, hi[3] = { 1, 1, 1 };
-    std::string sidePatch[6];      // patch name of each side (hs_side order)
-    std::string sideType[6];       // its polyPatch type (wall, patch, cyclic, empty, symmetryPlane)
-    std::vector<std::string> patchNames;
-};
-
-BlockMesh readBlockMesh(const dictionary& blockMeshDict);
 
 // Exact fluid volume of every cell with an embedded cylinder (axis z) removed.
 std::vector<double> cellVolumes(const BlockMesh& m, bool cylinder, double cx, double cy, double r);

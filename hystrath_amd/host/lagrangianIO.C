@@ -1,5 +1,6 @@
 // OpenFOAM ASCII reader and writer for dsmcFoam+'s lagrangian cloud files.
 #include "lagrangianIO.H"
+#include "foamText.H"
 
 #include <sys/stat.h>
 
@@ -26,137 +27,6 @@ void mkdirs(const std::string& p) {
         cur += p[i];
         if (p[i] == '/' || i + 1 == p.size()) mkdir(cur.c_str(), 0755);
     }
-}
-
-// The text of an OpenFOAM ASCII file after its FoamFile header, with
-// comments blanked, and a cursor over it.
-class FoamText {
-public:
-    explicit FoamText(const std::string& path) : path_(path) {
-        std::ifstream f(path, std::ios::binary);
-        if (!f) throw FatalError("cannot open " + path);
-        std::ostringstream ss;
-        ss << f.rdbuf();
-        s_ = ss.str();
-        blankComments();
-        const size_t h = s_.find("FoamFile");
-        if (h != std::string::npos) {
-            const size_t close = s_.find('}', h);
-            if (close == std::string::npos) fail("unterminated FoamFile header");
-            i_ = close + 1;
-        }
-    }
-
-    void ws() {
-        while (i_ < s_.size() && std::isspace((unsigned char)s_[i_])) ++i_;
-    }
-    bool peek(char c) {
-        ws();
-        return i_ < s_.size() && s_[i_] == c;
-    }
-    void expect(char c) {
-        if (!peek(c)) fail(std::string("expected '") + c + "'");
-        ++i_;
-    }
-    double real() {
-        ws();
-        const char* b = s_.c_str() + i_;
-        char* e = nullptr;
-        const double v = std::strtod(b, &e);
-        if (e == b) fail("expected a number");
-        i_ += (size_t)(e - b);
-        return v;
-    }
-    long long label() {
-        ws();
-        const char* b = s_.c_str() + i_;
-        char* e = nullptr;
-        const long long v = std::strtoll(b, &e, 10);
-        if (e == b) fail("expected a label");
-        i_ += (size_t)(e - b);
-        return v;
-    }
-    bool atLabel() {
-        ws();
-        return i_ < s_.size() && (std::isdigit((unsigned char)s_[i_]) || s_[i_] == '-');
-    }
-    // moves past the next occurrence of `word`
-    void seek(const std::string& word) {
-        const size_t k = s_.find(word, i_);
-        if (k == std::string::npos) fail("no " + word);
-        i_ = k + word.size();
-    }
-    bool next(const std::string& word) {
-        ws();
-        if (s_.compare(i_, word.size(), word) != 0) return false;
-        i_ += word.size();
-        return true;
-    }
-    [[noreturn]] void fail(const std::string& what) const {
-        size_t line = 1;
-        for (size_t k = 0; k < i_ && k < s_.size(); ++k) line += s_[k] == '\n';
-        throw FatalError(path_ + " line " + std::to_string(line) + ": " + what);
-    }
-    const std::string& path() const { return path_; }
-
-private:
-    void blankComments() {
-        for (size_t k = 0; k + 1 < s_.size(); ++k) {
-            if (s_[k] == '/' && s_[k + 1] == '/') {
-                while (k < s_.size() && s_[k] != '\n') s_[k++] = ' ';
-            } else if (s_[k] == '/' && s_[k + 1] == '*') {
-                const size_t e = s_.find("*/", k + 2);
-                const size_t end = e == std::string::npos ? s_.size() : e + 2;
-                for (; k < end; ++k)
-                    if (s_[k] != '\n') s_[k] = ' ';
-                --k;
-            }
-        }
-    }
-
-    std::string path_, s_;
-    size_t i_ = 0;
-};
-
-// An OpenFOAM list of n items, full "N ( item ... )" or uniform "N{item}":
-// one(t) parses an item into a value, put(k, value) stores item k.
-template <class T, class One, class Put>
-int64_t readList(FoamText& t, One one, Put put) {
-    const long long n = t.label();
-    if (n < 0) t.fail("negative list size");
-    if (t.peek('{')) {
-        t.expect('{');
-        const T v = one(t);
-        t.expect('}');
-        for (long long k = 0; k < n; ++k) put(k, v);
-        return n;
-    }
-    t.expect('(');
-    for (long long k = 0; k < n; ++k) put(k, one(t));
-    t.expect(')');
-    return n;
-}
-
-struct Vec3 {
-    double x, y, z;
-};
-
-Vec3 vec3(FoamText& t) {
-    t.expect('(');
-    Vec3 v{ t.real(), t.real(), t.real() };
-    t.expect(')');
-    return v;
-}
-
-// labelList "k(l0 ... lk-1)" or uniform "k{l}"
-std::vector<long long> labelList(FoamText& t) {
-    std::vector<long long> v;
-    readList<long long>(t, [](FoamText& u) { return u.label(); },
-                        [&](long long k, long long x) {
-                            if ((size_t)k >= v.size()) v.resize(k + 1);
-                            v[k] = x;
-                        });
-    return v;
 }
 
 std::string cloudDir(const std::string& timeDir) { return timeDir + "/lagrangian/dsmc"; }

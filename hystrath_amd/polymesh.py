@@ -129,7 +129,10 @@ def read_block(case_dir):
     for name, ptype, cnt_p, start in patches:
         if cnt_p == 0:
             continue
+        if start < len(neigh) or start + cnt_p > nf:
+            raise ValueError(f"{d}: patch {name} holds faces outside the boundary faces")
         c = fc[start:start + cnt_p]
+        on_sides = 0
         for a in range(3):
             for s, plane in ((0, lo[a]), (1, hi[a])):
                 on = np.abs(c[:, a] - plane) <= 1e-9 * max(1.0, abs(plane)) + 1e-12 * spacing[a]
@@ -142,9 +145,11 @@ def read_block(case_dir):
                 need = nc // n[a]
                 if on.sum() != need:
                     raise ValueError(f"{d}: patch {name} covers {on.sum()} of the {need} faces of side {side}")
-        covered = sum(1 for v in sides.values() if v[0] == name)
-        if covered == 0:
+                on_sides += int(on.sum())
+        if on_sides == 0:
             raise ValueError(f"{d}: patch {name} lies on no side of the block (an internal body?)")
+        if on_sides != cnt_p:
+            raise ValueError(f"{d}: patch {name} has {cnt_p - on_sides} faces inside the block")
     missing = [s for s in SIDES if s not in sides]
     if missing:
         raise ValueError(f"{d}: sides {missing} have no patch")

@@ -46,14 +46,15 @@ def test_reads_every_reference_tutorial_dictionary():
 
 @pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
 def test_reference_case_outcomes(tmp_path):
-    """couette_N2-O2 and heatBath-5species (its twelve QK reactions read
-from system/chemReactDict) are accepted up to the device; the multi-block
-tutorials stop with the reason they are outside this engine."""
+    """couette_N2-O2 (its mesh read from its constant/polyMesh) and
+    heatBath-5species (its twelve QK reactions read from system/chemReactDict)
+    are accepted up to the device; the multi-block tutorials stop with the
+    reason they are outside this engine."""
     outcomes = {}
     for c in ("couette_N2-O2", "heatBath-5species", "hypersonicCorner", "supersonicFlatPlate"):
         d = tmp_path / c
         shutil.copytree(os.path.join(REF, c, "system"), d / "system")
-        shutil.copytree(os.path.join(REF, c, "constant"), d / "constant", ignore=shutil.ignore_patterns("polyMesh"))
+        shutil.copytree(os.path.join(REF, c, "constant"), d / "constant")
         r = run("dsmcInitialisePlus", str(d))
         outcomes[c] = r.stdout + r.stderr
     assert "one block" in outcomes["hypersonicCorner"] and "one block" in outcomes["supersonicFlatPlate"]
@@ -200,6 +201,59 @@ def test_couette_pipeline_and_restart(tmp_path):
     assert r.returncode == 0, r.stderr
     assert "from" in r.stdout and "0.0004" in r.stdout
     assert os.path.exists(os.path.join(case, "0.0006", "Ttra_mixture"))
+
+
+COUETTE_PATCHES = [("periodicX_half0", "cyclic", ["xmin"]), ("periodicX_half1", "cyclic", ["xmax"]),
+                   ("lowerWall", "wall", ["ymin"]), ("upperWall", "wall", ["ymax"]),
+                   ("frontAndBack", "empty", ["zmin", "zmax"])]
+
+
+def _couette_from_polymesh(root, **kw):
+    """The generated couette case with its mesh given as constant/polyMesh
+    only (blockMeshDict removed), patches in the blockMeshDict's order."""
+    from test_polymesh import write_block_mesh
+    case = casegen.couette_case(root, **kw)
+    write_block_mesh(case, (5, 100, 1), (0.0, 0.0, 0.0), (0.05, 1.0, 0.01), COUETTE_PATCHES)
+    os.remove(os.path.join(case, "system", "blockMeshDict"))
+    return case
+
+
+def test_mesh_is_read_from_constant_polymesh(tmp_path):
+    """With constant/polyMesh present the host reads the mesh from it (as
+    fvMesh does); a polyMesh that is not one uniform block stops the run with
+    the reason."""
+    case = _couette_from_polymesh(str(tmp_path / "c"))
+    r = run("dsmcInitialisePlus", case)
+    if no_gpu():
+        assert r.returncode == 1 and "no HIP device" in r.stderr, r.stdout + r.stderr
+    else:
+        assert r.returncode == 0, r.stdout + r.stderr
+    from test_polymesh import write_block_mesh
+    bad = casegen.couette_case(str(tmp_path / "g"))
+    xs = np.concatenate([[0.0], np.cumsum([0.005, 0.01, 0.01, 0.01, 0.015])])
+    write_block_mesh(bad, (5, 100, 1), (0.0, 0.0, 0.0), (0.05, 1.0, 0.01), COUETTE_PATCHES, xs=xs)
+    r = run("dsmcInitialisePlus", bad)
+    assert r.returncode == 1 and "constant/polyMesh (x): cells are not evenly spaced" in r.stderr, r.stderr
+
+
+@pytest.mark.gpu
+def test_polymesh_and_blockmeshdict_runs_are_identical(tmp_path):
+    """The couette case run from its blockMeshDict and from the equivalent
+    polyMesh writes identical fields and parcels."""
+    a = casegen.couette_case(str(tmp_path / "a"), end_time=2e-4, write_interval=1e-4, n_terminal=10)
+    b = _couette_from_polymesh(str(tmp_path / "b"), end_time=2e-4, write_interval=1e-4, n_terminal=10)
+    for c in (a, b):
+        for app in ("dsmcInitialisePlus", "dsmcFoamPlus"):
+            r = run(app, c)
+            assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+    names = sorted(f for f in os.listdir(os.path.join(a, "0.0002")) if os.path.isfile(os.path.join(a, "0.0002", f)))
+    assert {"Ttra_mixture", "rhoN_N2", "U_mixture", "wallHeatFlux_mixture"} <= set(names)
+    for f in names:
+        assert open(os.path.join(a, "0.0002", f)).read() == open(os.path.join(b, "0.0002", f)).read(), f
+    sa = _read_state(os.path.join(a, "0.0002/lagrangian/dsmc/hystrath.state"))
+    sb = _read_state(os.path.join(b, "0.0002/lagrangian/dsmc/hystrath.state"))
+    for k in sa:
+        np.testing.assert_array_equal(sa[k], sb[k], err_msg=k)
 
 
 @pytest.mark.gpu

@@ -138,3 +138,103 @@ def test_uniform_blocks_of_awkward_spacing_are_read(tmp_path, n, lo, hi):
     spacing = [(hi[a] - lo[a]) / n[a] for a in range(3)]
     np.testing.assert_allclose(b["spacing"], spacing, rtol=1e-9)
     np.testing.assert_allclose(b["origin"], lo, rtol=1e-9, atol=1e-12 * max(map(abs, hi)))
+
+
+# --- the C++ host's reader (hystrath_amd/host/polyMesh.C) ---------------------
+MESHCHECK = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tools", "_bin", "meshCheck")
+
+
+def _meshcheck(case):
+    import subprocess
+    r = subprocess.run([MESHCHECK, str(case)], capture_output=True, text=True, timeout=120)
+    if r.returncode != 0:
+        return r.stdout.strip()
+    out = {}
+    for line in r.stdout.splitlines():
+        k, *v = line.split()
+        if k == "side":
+            out.setdefault("sides", {})[v[0]] = (v[1], v[2])
+        elif k == "n":
+            out["n"] = tuple(int(x) for x in v)
+        elif k == "patches":
+            out["patches"] = v
+        else:
+            out[k] = tuple(float(x) for x in v)
+    return out
+
+
+def _agree(case):
+    """The C++ reader's block equals the Python reader's: counts, sides and
+    patch order exactly, the bounds to the last bit of the points."""
+    b, c = pm.read_block(str(case)), _meshcheck(case)
+    assert isinstance(c, dict), c
+    assert c["n"] == b["n"] and c["sides"] == b["sides"]
+    assert c["patches"] == [p[0] for p in b["patches"]]
+    assert c["lo"] == tuple(b["origin"])
+    hi = tuple(o + k * h for o, k, h in zip(b["origin"], b["n"], b["spacing"]))
+    np.testing.assert_allclose(c["hi"], hi, rtol=1e-14, atol=1e-300)
+    return c
+
+
+@pytest.mark.skipif(not os.path.isdir(REF), reason="hyStrath tutorials not present")
+def test_cpp_reader_reads_the_couette_tutorial_mesh():
+    c = _agree(REF)
+    assert c["n"] == (5, 100, 1) and c["lo"] == (0.0, 0.0, 0.0) and c["hi"] == (0.05, 1.0, 0.01)
+
+
+@pytest.mark.parametrize("n,lo,hi", [
+    ((4, 3, 2), (-0.5, 0.0, 1.0), (0.5, 0.75, 1.5)),
+    ((300, 2, 1), (0.0, 0.0, 0.0), (0.1, 0.01, 0.001)),
+    ((50, 40, 1), (1e-6, -2e-6, 0.0), (3.1e-6, 2e-6, 1e-7)),
+    ((7, 3, 2), (1000.0, 1000.0, 0.0), (1000.7, 1000.3, 0.2)),
+])
+def test_cpp_and_python_readers_agree(tmp_path, n, lo, hi):
+    write_block_mesh(str(tmp_path), n, lo, hi, SIX)
+    c = _agree(tmp_path)
+    assert c["n"] == n
+
+
+def test_cpp_reader_rejects_what_python_rejects(tmp_path):
+    """The same meshes fail in both readers, for the same reason."""
+    cases = {}
+    write_block_mesh(str(tmp_path / "g"), (4, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX, xs=[0, 0.1, 0.3, 0.6, 1.0])
+    cases["g"] = "evenly spaced"
+    write_block_mesh(str(tmp_path / "r"), (4, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX, renumber=np.arange(8)[::-1])
+    cases["r"] = "numbering"
+    write_block_mesh(str(tmp_path / "m"), (2, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX[:4] + [("front", "empty", ["zmin"])])
+    cases["m"] = "only hex cells"
+    # the side x- split between two patches of one face each
+    write_block_mesh(str(tmp_path / "t"), (2, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX)
+    bf = tmp_path / "t" / "constant" / "polyMesh" / "boundary"
+    txt = bf.read_text().replace("5\n(\n", "6\n(\n", 1)
+    bf.write_text(txt.replace("left\n{\n    type patch;\n    nFaces 2;\n    startFace 4;\n}",
+                              "left\n{\n    type patch;\n    nFaces 1;\n    startFace 4;\n}\n"
+                              "left2\n{\n    type patch;\n    nFaces 1;\n    startFace 5;\n}", 1))
+    cases["t"] = "covers 1 of the 2 faces of side xmin"
+    for k, why in cases.items():
+        with pytest.raises(ValueError, match=why):
+            pm.read_block(str(tmp_path / k))
+        msg = _meshcheck(tmp_path / k)
+        assert isinstance(msg, str) and msg.startswith("!! ") and why in msg, (k, msg)
+
+
+def test_cpp_reader_file_errors(tmp_path):
+    """A triangle among the faces, a short owner list, a patch outside the
+    faces and a compressed file are refused with the file and the reason."""
+    base = tmp_path / "b"
+    write_block_mesh(str(base), (2, 2, 1), (0, 0, 0), (1, 1, 0.1), SIX)
+    d = base / "constant" / "polyMesh"
+    good = {f: (d / f).read_text() for f in ("faces", "owner", "boundary")}
+    (d / "faces").write_text(good["faces"].replace("4(", "3(", 1))
+    assert "a face of 3 points" in _meshcheck(base)
+    (d / "faces").write_text(good["faces"])
+    (d / "owner").write_text(good["owner"].replace("20\n(\n", "19\n(\n", 1))
+    assert "owner" in _meshcheck(base)
+    (d / "owner").write_text(good["owner"])
+    (d / "boundary").write_text(good["boundary"].replace("startFace 12;", "startFace 14;", 1))
+    msg = _meshcheck(base)
+    assert "patch frontAndBack holds faces outside the boundary faces" in msg, msg
+    (d / "boundary").write_text(good["boundary"])
+    assert _meshcheck(base)["n"] == (2, 2, 1)
+    os.rename(d / "faces", d / "faces.gz")
+    assert "compressed files are not read" in _meshcheck(base)
