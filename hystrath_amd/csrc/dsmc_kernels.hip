@@ -2094,12 +2094,35 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
             for (int q = 0; q < 8; ++q) L.smask[t][q] = 0ull;
             if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
         }
-#pragma unroll 8
-        for (int j = t; j < cnt; j += CT) {
-            const int64_t gj = (int64_t)S0 + j;
-            L.vx[j] = dst.ux[gj]; L.vy[j] = dst.uy[gj]; L.vz[j] = dst.uz[gj];
-            L.sub[j] = sub[gj];
-            if (GEN) {
+        // the chunk's velocities and sub-cells: a batch of loads is issued
+        // before any of its LDS stores, so their latencies overlap (one at a
+        // time in the variant with internal energies, where the batch's
+        // registers cost spills in the NTC)
+        constexpr int LB = GEN ? (REACT ? 4 : 1) : 8;
+        for (int j0 = t; j0 < cnt; j0 += LB * CT) {
+            float bx[LB], by[LB], bz[LB];
+            uint8_t bs[LB];
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int j = j0 + u * CT;
+                if (j < cnt) {
+                    const int64_t gj = (int64_t)S0 + j;
+                    bx[u] = dst.ux[gj]; by[u] = dst.uy[gj]; bz[u] = dst.uz[gj];
+                    bs[u] = sub[gj];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < LB; ++u) {
+                const int j = j0 + u * CT;
+                if (j < cnt) {
+                    L.vx[j] = bx[u]; L.vy[j] = by[u]; L.vz[j] = bz[u];
+                    L.sub[j] = bs[u];
+                }
+            }
+        }
+        if (GEN) {
+            for (int j = t; j < cnt; j += CT) {
+                const int64_t gj = (int64_t)S0 + j;
                 L.er[j] = dst.erot ? dst.erot[gj] : 0.0f;
                 for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + j] = dst.vib[m * cap + gj];
                 L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
