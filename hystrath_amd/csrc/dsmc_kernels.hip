@@ -775,16 +775,39 @@ __device__ __forceinline__ void move_body(const DevParams* __restrict__ Pp, DevP
     const bool track = TRACK;
     const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
     MoveCounts mc = { 0, 0, 0 };
+    // the next parcel's cell, position and velocity are loaded while this
+    // one flies
+    const int64_t stride = (int64_t)gx * BLOCK;
+    int32_t nx_c = -1;
+    double nx_p[3] = { 0.0, 0.0, 0.0 };
+    float nx_u[3] = { 0.0f, 0.0f, 0.0f };
+    auto fetch = [&](int64_t j) {
+        if (j < n) {
+            nx_c = b.cell[j];
+            if (do_move) {
+                nx_p[0] = b.px[j]; nx_p[1] = b.py[j]; nx_p[2] = b.pz[j];
+                nx_u[0] = b.ux[j]; nx_u[1] = b.uy[j]; nx_u[2] = b.uz[j];
+            }
+        }
+    };
+    fetch(bx * (int64_t)BLOCK + threadIdx.x);
     // wave-uniform trip count: every lane of a wave sees the same bases
-    for (int64_t base = bx * (int64_t)BLOCK + (threadIdx.x - lane); base < n; base += (int64_t)gx * BLOCK) {
+    for (int64_t base = bx * (int64_t)BLOCK + (threadIdx.x - lane); base < n; base += stride) {
         const int64_t i = base + lane;
         const bool valid = i < n;
         int32_t c = -1;
         int code = -1;
         Flight s;
         if (valid) {
-            c = b.cell[i];
-            if (do_move) load_pu(b, i, s);  // in flight together with the cell
+            c = nx_c;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                s.p[a] = nx_p[a];
+                s.U[a] = (double)nx_u[a];
+            }
+        }
+        fetch(i + stride);
+        if (valid) {
             if (do_move && c >= 0) {
                 PhiloxStream r;
                 move_stream(P, r, i, step, i >= n_old, &s.t);
