@@ -1589,6 +1589,7 @@ struct TileLds {
     int32_t ncand[CT_CELLS];
     int32_t dense[CT_CELLS];
     int32_t ndense;
+    int32_t dnext;           // next dense cell a wave takes
     int32_t big[CT_CELLS];   // cells of more than 64 parcels
     int32_t nbig;
     // candidate rounds of the sparse cells
@@ -2151,7 +2152,10 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
                 L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
             }
         }
-        if (t == 0) L.ndense = 0;
+        if (t == 0) {
+            L.ndense = 0;
+            L.dnext = 0;
+        }
         __syncthreads();
         // sub-cell tables in parallel: a parcel of a cell of at most 64 sets
         // its bit in its sub-cell's mask; in a larger cell it finds its place
@@ -2191,9 +2195,30 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
             L.rn[t] = nsparse;
         }
         __syncthreads();
-        // dense cells: one wave each
+        // dense cells: one wave each.  In the variant with internal energies
+        // (without chemistry) the cells with most candidates go first, each
+        // wave taking the next cell when it finishes one; otherwise wave w
+        // takes cells w, w+4, ... (measured faster there)
+        constexpr bool LPT = GEN && !REACT;
         const int nd = L.ndense;
-        for (int di = wid; di < nd; di += CT / WAVE) {
+        if (LPT && nd > CT / WAVE && nd <= 64) {
+            if (t == 0) {
+                for (int x = 1; x < nd; ++x) {
+                    const int v = L.dense[x], k = L.ncand[v];
+                    int y = x - 1;
+                    for (; y >= 0 && L.ncand[L.dense[y]] < k; --y) L.dense[y + 1] = L.dense[y];
+                    L.dense[y + 1] = v;
+                }
+            }
+            __syncthreads();
+        }
+        for (int dw = wid;; dw += CT / WAVE) {
+            int di = dw;
+            if (LPT) {
+                if (lane == 0) di = atomicAdd(&L.dnext, 1);
+                di = __shfl(di, 0, WAVE);
+            }
+            if (di >= nd) break;
             const int l = L.dense[di];
             const int ss = L.segS[l] - S0, nC = L.segS[l + 1] - L.segS[l];
             const CellTab tab = { L.smask[l], L.slist, ss, nC <= 64 };
