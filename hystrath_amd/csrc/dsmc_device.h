@@ -76,6 +76,7 @@ struct DevParams {
     int coll_model;
     double z_rot, z_el, z_vib_fixed;
     int inv_zv_form;
+    const double* t_overall;    // [nc] cells' overall temperature (inverseZvFormulation 2008 only)
     double fn, dt;
     uint32_t seed_lo, seed_hi;
     int flags, nslots;
@@ -399,10 +400,14 @@ __device__ __forceinline__ void post_coll_vel(const DevParams& P, bool vss, bool
 }
 
 // dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531);
-// POST = postReaction: the level is always redrawn, with no Zv test
+// POST = postReaction: the level is always redrawn, with no Zv test.
+// cell: the collision's cell, whose overall temperature inverseZvFormulation
+// 2008 (inv_zv_form 1) uses in Zv when it exceeds SMALL (dsmcCloud.C:
+// 1440-1455); otherwise (or with cell -1) the quantised collision
+// temperature is used.
 template <bool POST = false, class R>
 __device__ int vib_level(const DevParams& P, int t, int partner, int mode, int level, int imax,
-                         double ec, double om, R& r) {
+                         double ec, double om, R& r, int cell = -1) {
     const double theta_v = P.theta_v[t][mode];
     if (POST) {
         int idash, guard = 0;
@@ -416,7 +421,11 @@ __device__ int vib_level(const DevParams& P, int t, int partner, int mode, int l
     }
     double inv_zv = 1.0;
     if (P.z_vib_fixed == 0.0) {
-        const double T = (double)imax * theta_v / (3.5 - om);
+        double T = (double)imax * theta_v / (3.5 - om);
+        if (P.inv_zv_form == 1 && cell >= 0) {
+            const double tm = P.t_overall[cell];
+            if (tm > HS_SMALL) T = tm;
+        }
         const double pow1 = dm_pow(P.theta_d[t] / T, 1.0 / 3.0) - 1.0;
         const double zvp1 = dm_pow(P.theta_d[t] / T, om);
         const double zvp2 = dm_pow(P.zv_b[t][partner][mode], pow1 / P.zv_pow2[t][mode]);
@@ -461,7 +470,7 @@ __device__ double rot_ratio(double dof, double chib, R& r) {
 // modes, and only the electronic test draw remains.  POST = postReaction.
 template <bool INTERNAL, bool POST = false, class R>
 __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* et, double* erot,
-                                int* vib, double om, R& r) {
+                                int* vib, double om, R& r, int cell = -1) {
     if (P.charge[t] == -1) return;
     if (1.0 / P.z_el > r.u()) (void)r.u();
     if (!INTERNAL) return;
@@ -473,7 +482,7 @@ __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* 
             const double ec = *et + pre[m];
             const int imax = (int)(ec / (HS_KB * P.theta_v[t][m]));
             if (imax > 0) {
-                vib[m] = vib_level<POST>(P, t, partner, m, vib[m], imax, ec, om, r);
+                vib[m] = vib_level<POST>(P, t, partner, m, vib[m], imax, ec, om, r, cell);
                 *et = ec - (double)vib[m] * HS_KB * P.theta_v[t][m];
             }
         }
@@ -493,7 +502,7 @@ __device__ void lb_redistribute(const DevParams& P, int t, int partner, double* 
 // lb_redistribute)
 template <bool INTERNAL = true, class R>
 __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3], double uq[3],
-                               double* erp, double* erq, int* vp, int* vq, R& r) {
+                               double* erp, double* erq, int* vp, int* vq, R& r, int cell = -1) {
     const int model = P.coll_model;
     if (model == HS_COLL_VHS || model == HS_COLL_VSS) {
         post_coll_vel(P, model == HS_COLL_VSS, false, tp, tq, up, uq, -1.0, r);
@@ -505,8 +514,8 @@ __device__ void binary_collide(const DevParams& P, int tp, int tq, double up[3],
     const double cr2 = dx * dx + dy * dy + dz * dz;
     double et = 0.5 * mr * cr2;
     const double om = P.omega_pq[tp][tq];
-    lb_redistribute<INTERNAL>(P, tp, tq, &et, erp, vp, om, r);
-    lb_redistribute<INTERNAL>(P, tq, tp, &et, erq, vq, om, r);
+    lb_redistribute<INTERNAL>(P, tp, tq, &et, erp, vp, om, r, cell);
+    lb_redistribute<INTERNAL>(P, tq, tp, &et, erq, vq, om, r, cell);
     const double cr = sqrt(2.0 * et / mr);
     post_coll_vel(P, model == HS_COLL_LB_VSS, true, tp, tq, up, uq, cr, r);
 }
@@ -754,11 +763,11 @@ __device__ bool react_pair(const DevParams& P, int ri, PState& Pp, PState& Qp, R
 // react) the binary collision.  True if it reacted.
 template <class R>
 __device__ __attribute__((noinline)) bool collide_or_react(const DevParams& P, PState& a, PState& b, R& r,
-                                                           PState& prod, int* from_q) {
+                                                           PState& prod, int* from_q, int cell) {
     prod.t = -1;
     const int ri = P.react_of[a.t][b.t];
     if (ri >= 0 && react_pair(P, ri, a, b, r, prod, from_q)) return true;
-    binary_collide<true>(P, a.t, b.t, a.u, b.u, &a.er, &b.er, a.v, b.v, r);
+    binary_collide<true>(P, a.t, b.t, a.u, b.u, &a.er, &b.er, a.v, b.v, r, cell);
     return false;
 }
 

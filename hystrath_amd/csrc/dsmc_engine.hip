@@ -54,6 +54,7 @@ struct hs_dsmc_engine {
     int cur = 0;
     std::vector<void*> allocs;
     double* d_vol = nullptr;
+    double* d_t_overall = nullptr;   // inverseZvFormulation 2008: cells' overall temperature
     double* d_sigma = nullptr;
     double* d_rem = nullptr;
     double* d_samples = nullptr;
@@ -408,9 +409,7 @@ static int validate(const hs_config* c) {
     if (c->collision_model >= HS_COLL_LB_VHS) {
         if (!(c->z_rot > 0.0) || !(c->z_el > 0.0))
             return fail(nullptr, HS_E_INVALID, "relaxation collision numbers must be positive");
-        if (c->inv_zv_form == 1)
-            return fail(nullptr, HS_E_UNSUPPORTED, "inverseZvFormulation 2008 (macroscopic temperature) is not supported");
-        if (c->inv_zv_form != 0 && c->inv_zv_form != 2) return fail(nullptr, HS_E_INVALID, "inverseZvFormulation out of range");
+        if (c->inv_zv_form < 0 || c->inv_zv_form > 2) return fail(nullptr, HS_E_INVALID, "inverseZvFormulation out of range");
     }
     const int nn[3] = { c->nx, c->ny, c->nz };
     for (int a = 0; a < 3; ++a) {
@@ -779,6 +778,14 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         launch_fill_i64((int64_t*)eng->d_face_flux, 0, (int64_t)ns * eng->hp.n_faces, eng->stream);
         eng->hp.face_flux = eng->d_face_flux;
     }
+    // inverseZvFormulation 2008: the cells' overall temperature, zero (the
+    // collision temperature is used) until hs_dsmc_set_cell_temperature
+    eng->hp.t_overall = nullptr;
+    if (cfg->collision_model >= HS_COLL_LB_VHS && cfg->inv_zv_form == 1) {
+        if ((rc = dev_alloc(eng, &eng->d_t_overall, nc))) return bail(rc);
+        launch_fill_f64(eng->d_t_overall, 0.0, nc, eng->stream);
+        eng->hp.t_overall = eng->d_t_overall;
+    }
     if (hipMemcpy(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(eng->d_vol, eng->vol_h.data(), (size_t)nc * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "hipMemcpy of parameters failed"));
@@ -894,6 +901,17 @@ extern "C" int hs_dsmc_set_cell_state(hs_dsmc_engine* eng, const double* sigma, 
     HS_TRY(hipSetDevice(eng->device));
     if (sigma) HS_TRY(hipMemcpyAsync(eng->d_sigma, sigma, (size_t)eng->nc * 8, hipMemcpyHostToDevice, eng->stream));
     if (rem) HS_TRY(hipMemcpyAsync(eng->d_rem, rem, (size_t)eng->nc * 8, hipMemcpyHostToDevice, eng->stream));
+    HS_TRY(hipStreamSynchronize(eng->stream));
+    return 0;
+}
+
+extern "C" int hs_dsmc_set_cell_temperature(hs_dsmc_engine* eng, const double* t_overall) {
+    if (!eng || !t_overall) return fail(eng, HS_E_INVALID, "NULL argument");
+    if (!eng->d_t_overall)
+        return fail(eng, HS_E_INVALID,
+                    "cell temperatures are read only by inverseZvFormulation 2008 with a Larsen-Borgnakke model");
+    HS_TRY(hipSetDevice(eng->device));
+    HS_TRY(hipMemcpyAsync(eng->d_t_overall, t_overall, (size_t)eng->nc * 8, hipMemcpyHostToDevice, eng->stream));
     HS_TRY(hipStreamSynchronize(eng->stream));
     return 0;
 }

@@ -1075,11 +1075,12 @@ __device__ __forceinline__ bool draw_pair(const DevParams& P, PhiloxStream& r, i
     return !(P.charge[tp] == -1 && P.charge[tq] == -1);
 }
 
-// The accepted collision of candidate (pp, qq): BinaryCollisionModel::collide.
+// The accepted collision of candidate (pp, qq) in cell c:
+// BinaryCollisionModel::collide.
 template <bool GEN>
 __device__ __forceinline__ void do_collision(const DevParams& P, PhiloxStream& r, int ip, int iq, double up[3],
                                              double uq[3], float* vx, float* vy, float* vz, float* er, uint16_t* vb,
-                                             int vstride, const uint8_t* ty) {
+                                             int vstride, const uint8_t* ty, int c) {
     const int tp = GEN ? (int)ty[ip] : 0, tq = GEN ? (int)ty[iq] : 0;
     double erp = 0.0, erq = 0.0;
     int vp[HS_MAX_VIB_MODES] = { 0, 0, 0 }, vq[HS_MAX_VIB_MODES] = { 0, 0, 0 };
@@ -1088,7 +1089,7 @@ __device__ __forceinline__ void do_collision(const DevParams& P, PhiloxStream& r
         for (int m = 0; m < P.nvib[tp]; ++m) vp[m] = vb[m * vstride + ip];
         for (int m = 0; m < P.nvib[tq]; ++m) vq[m] = vb[m * vstride + iq];
     }
-    binary_collide<GEN>(P, tp, tq, up, uq, &erp, &erq, vp, vq, r);
+    binary_collide<GEN>(P, tp, tq, up, uq, &erp, &erq, vp, vq, r, c);
     vx[ip] = (float)up[0]; vy[ip] = (float)up[1]; vz[ip] = (float)up[2];
     vx[iq] = (float)uq[0]; vy[iq] = (float)uq[1]; vz[iq] = (float)uq[2];
     if (GEN) {
@@ -1125,7 +1126,8 @@ __device__ __forceinline__ void stage_product(const DevParams& P, const ProdOut&
 // parcels are rewritten.  prod.t >= 0: a product was made; from_q: Q split.
 __device__ __forceinline__ void react_collision(const DevParams& P, PhiloxStream& r, int ip, int iq, double up[3],
                                                 double uq[3], float* vx, float* vy, float* vz, float* er,
-                                                uint16_t* vb, int vstride, uint8_t* ty, PState& prod, int* from_q) {
+                                                uint16_t* vb, int vstride, uint8_t* ty, PState& prod, int* from_q,
+                                                int c) {
     PState a, b;
     a.t = ty[ip];
     b.t = ty[iq];
@@ -1137,7 +1139,7 @@ __device__ __forceinline__ void react_collision(const DevParams& P, PhiloxStream
     for (int m = 0; m < HS_MAX_VIB_MODES; ++m) { a.v[m] = 0; b.v[m] = 0; }
     for (int m = 0; m < P.nvib[a.t]; ++m) a.v[m] = vb[m * vstride + ip];
     for (int m = 0; m < P.nvib[b.t]; ++m) b.v[m] = vb[m * vstride + iq];
-    const bool reacted = collide_or_react(P, a, b, r, prod, from_q);
+    const bool reacted = collide_or_react(P, a, b, r, prod, from_q, c);
     vx[ip] = (float)a.u[0]; vy[ip] = (float)a.u[1]; vz[ip] = (float)a.u[2];
     vx[iq] = (float)b.u[0]; vy[iq] = (float)b.u[1]; vz[iq] = (float)b.u[2];
     if (er) { er[ip] = (float)a.er; er[iq] = (float)b.er; }
@@ -1216,9 +1218,9 @@ __device__ void ntc_wave(const DevParams& P, int c, int ss, int nC, long long nc
                 if (stcr / smax > r.u()) {
                     if (REACT)
                         react_collision(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, const_cast<uint8_t*>(ty),
-                                        prod, &fq);
+                                        prod, &fq, c);
                     else
-                        do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty);
+                        do_collision<GEN>(P, r, ip, iq, up, uq, vx, vy, vz, er, vb, vstride, ty, c);
                     doit = true;
                     if (sep_on) {
                         const int64_t gp = gs + pp, gq = gs + qq;
@@ -1673,9 +1675,10 @@ __device__ void ntc_rounds(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, int
                 atomicMax(&L.rsmax[l], (unsigned long long)__double_as_longlong(stcr));
                 if (stcr / L.smax[l] > r.u()) {
                     if (REACT)
-                        react_collision(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, prod, &fq);
+                        react_collision(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, prod, &fq,
+                                        c0 + l);
                     else
-                        do_collision<GEN>(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty);
+                        do_collision<GEN>(P, r, ip, iq, up, uq, L.vx, L.vy, L.vz, L.er, L.vb, PMAX, L.ty, c0 + l);
                     atomicAdd(&L.rcoll[l], 1u);
                     doit = true;
                     if (sep_on) {

@@ -19,7 +19,7 @@ LIB_PATH = os.path.join(HERE, "libhystrath_dsmc.so")
 EXPORTS = [
     "hs_dsmc_sample_slots", "hs_dsmc_create", "hs_dsmc_destroy", "hs_dsmc_last_error", "hs_dsmc_create_error",
     "hs_dsmc_device_count", "hs_dsmc_upload_particles", "hs_dsmc_download_particles", "hs_dsmc_particle_count",
-    "hs_dsmc_set_cell_state", "hs_dsmc_get_cell_state", "hs_dsmc_inflow_face_count",
+    "hs_dsmc_set_cell_state", "hs_dsmc_get_cell_state", "hs_dsmc_set_cell_temperature", "hs_dsmc_inflow_face_count",
     "hs_dsmc_set_inflow_accumulators", "hs_dsmc_get_inflow_accumulators", "hs_dsmc_set_step", "hs_dsmc_get_step",
     "hs_dsmc_fill", "hs_dsmc_fill_cells", "hs_dsmc_fill_noalloc", "hs_dsmc_evolve", "hs_dsmc_inject", "hs_dsmc_move", "hs_dsmc_sort", "hs_dsmc_collide",
     "hs_dsmc_advance_step", "hs_dsmc_synchronize", "hs_dsmc_read_samples", "hs_dsmc_reset_samples", "hs_dsmc_set_samples",
@@ -63,6 +63,7 @@ def load_library(path=LIB_PATH):
         "hs_dsmc_download_particles": ([eng, P(abi.hs_particles), C.c_int64, P(C.c_int64)], C.c_int),
         "hs_dsmc_particle_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_set_cell_state": ([eng, P(C.c_double), P(C.c_double)], C.c_int),
+        "hs_dsmc_set_cell_temperature": ([eng, P(C.c_double)], C.c_int),
         "hs_dsmc_get_cell_state": ([eng, P(C.c_double), P(C.c_double)], C.c_int),
         "hs_dsmc_inflow_face_count": ([eng, P(C.c_int64)], C.c_int),
         "hs_dsmc_set_inflow_accumulators": ([eng, P(C.c_double)], C.c_int),
@@ -265,6 +266,13 @@ class Engine:
         s = None if sigma is None else np.ascontiguousarray(sigma, np.float64)
         r = None if rem is None else np.ascontiguousarray(rem, np.float64)
         self._ck(self.L.hs_dsmc_set_cell_state(self.h, _d(s), _d(r)), "set_cell_state")
+
+    def set_cell_temperature(self, t_overall):
+        """Each cell's overall temperature for inverseZvFormulation 2008."""
+        t = np.ascontiguousarray(t_overall, np.float64)
+        if t.shape != (self.case.ncells,):
+            raise EngineError(f"set_cell_temperature: {t.shape} values for {self.case.ncells} cells")
+        self._ck(self.L.hs_dsmc_set_cell_temperature(self.h, _d(t)), "set_cell_temperature")
 
     def cell_state(self):
         nc = self.case.ncells

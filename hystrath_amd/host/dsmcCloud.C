@@ -974,6 +974,10 @@ void dsmcCloud::write() {
     bool reset = false;
     for (const auto& fm : fields_) {
         const VolFields f = computeVolFields(*this, s, nsampled, fm);
+        // inverseZvFormulation 2008 reads the first field's overall
+        // temperature until the next write (fields().overallT, dsmcCloud.C:1440-1455)
+        if (&fm == &fields_.front() && cfg_.inv_zv_form == 1 && cfg_.collision_model >= HS_COLL_LB_VHS)
+            check(hs_dsmc_set_cell_temperature(eng_, f.Tov.data()), "cell temperature");
         const WallFields w = computeWallFields(*this, wsums, nw, wnormal, nsampled, fm);
         const std::string n = "_" + fm.fieldName;
         const std::vector<double> zero(vol_.size(), 0.0);

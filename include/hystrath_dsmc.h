@@ -143,7 +143,8 @@ typedef struct hs_config {
     double z_rot;                      /* rotationalRelaxationCollisionNumber (5) */
     double z_vib_fixed;                /* vibrationalRelaxationCollisionNumber (0 = Bird 2010 Zv) */
     double z_el;                       /* electronicRelaxationCollisionNumber (500) */
-    int32_t inv_zv_form;               /* inverseZvFormulation: 0 pre-2008, 2 default (1 unsupported) */
+    int32_t inv_zv_form;               /* inverseZvFormulation: 0 pre-2008, 1 2008 (the cells' overall
+                                          temperature, hs_dsmc_set_cell_temperature), 2 default */
 
     /* ---- dsmcFreeStreamInflowPatch properties (shared by inflow patches) ---- */
     double inflow_n[HS_MAX_SPECIES];   /* numberDensities per species           */
@@ -245,6 +246,15 @@ int hs_dsmc_particle_count(hs_dsmc_engine* eng, int64_t* n_out);
  * either pointer may be NULL. */
 int hs_dsmc_set_cell_state(hs_dsmc_engine* eng, const double* sigma_tcr_max, const double* remainder);
 int hs_dsmc_get_cell_state(hs_dsmc_engine* eng, double* sigma_tcr_max, double* remainder);
+/* Each cell's overall temperature (nc values), which inverseZvFormulation
+ * 2008 puts in the vibrational collision number in place of the collision
+ * temperature wherever it exceeds SMALL: replaces fields().overallT(cellI)
+ * (dsmcCloud.C:1440-1455), the Tov of the first dsmcVolFields
+ * (dsmcFieldProperties.C:235-244, dsmcVolFields.H:244-247), which that field
+ * recomputes at each write (dsmcVolFields.C:1365, 1500-1508).  Zero at
+ * creation; HS_E_INVALID unless the engine has inv_zv_form 1 and a
+ * Larsen-Borgnakke model. */
+int hs_dsmc_set_cell_temperature(hs_dsmc_engine* eng, const double* t_overall);
 /* freestream inflow accumulators accumulatedParcelsToInsert_
  * (dsmcFreeStreamInflowPatch.C:132-140), [species][inflow face]; values
  * lie in [-1, 1) between steps and the setter rejects others */

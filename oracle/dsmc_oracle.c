@@ -546,9 +546,13 @@ static void post_coll_vel(const hs_config* cfg, const or_derived* d, int vss, in
 }
 
 /* dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531);
- * post = postReaction: the level is always redrawn, with no Zv test */
+ * post = postReaction: the level is always redrawn, with no Zv test.
+ * t_macro: the cell's overall temperature, which inverseZvFormulation 2008
+ * (inv_zv_form 1) uses in Zv when it exceeds SMALL (dsmcCloud.C:1440-1455,
+ * fields().overallT); otherwise, and in the other formulations, the
+ * quantised collision temperature */
 static int vib_level(const hs_config* cfg, const or_derived* d, int t, int partner, int mode,
-                     int level, int imax, double ec, double om, int post, or_rng* r)
+                     int level, int imax, double ec, double om, int post, or_rng* r, double t_macro)
 {
     const hs_species* sp = &cfg->species[t];
     const double theta_v = sp->theta_v[mode];
@@ -564,7 +568,8 @@ static int vib_level(const hs_config* cfg, const or_derived* d, int t, int partn
         return idash;
     }
     if (cfg->z_vib_fixed == 0.0) {
-        const double T = (double)imax * theta_v / (3.5 - om);
+        const double T = (cfg->inv_zv_form == 1 && t_macro > OR_SMALL) ? t_macro
+                                                                       : (double)imax * theta_v / (3.5 - om);
         const double pow1 = or_pow(sp->theta_d / T, 1.0 / 3.0) - 1.0;
         const double zvp1 = or_pow(sp->theta_d / T, om);
         const double zvp2 = or_pow(d->zv_b[t][partner][mode], pow1 / d->zv_pow2[t][mode]);
@@ -606,7 +611,8 @@ static double rot_ratio(double dof, double chib, or_rng* r)
 /* LarsenBorgnakkeVariableHardSphere::redistribute (LB-VHS.C:150-279) for a
  * species with a single electronic level; post = postReaction */
 static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, int partner,
-                            double* et, double* erot, int32_t* vib, double om, int post, or_rng* r)
+                            double* et, double* erot, int32_t* vib, double om, int post, or_rng* r,
+                            double t_macro)
 {
     const hs_species* sp = &cfg->species[t];
     if (sp->charge == -1) return;
@@ -619,7 +625,7 @@ static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, in
             const double ec = *et + pre[m];
             const int imax = (int)(ec / (HS_KB * sp->theta_v[m]));
             if (imax > 0) {
-                vib[m] = vib_level(cfg, d, t, partner, m, vib[m], imax, ec, om, post, r);
+                vib[m] = vib_level(cfg, d, t, partner, m, vib[m], imax, ec, om, post, r, t_macro);
                 *et = ec - (double)vib[m] * HS_KB * sp->theta_v[m];
             }
         }
@@ -638,7 +644,7 @@ static void lb_redistribute(const hs_config* cfg, const or_derived* d, int t, in
 /* BinaryCollisionModel::collide for the configured model */
 static void binary_collide(const hs_config* cfg, const or_derived* d, int tp, int tq,
                            double up[3], double uq[3], double* erp, double* erq,
-                           int32_t* vp, int32_t* vq, or_rng* r)
+                           int32_t* vp, int32_t* vq, or_rng* r, double t_macro)
 {
     const int model = cfg->collision_model;
     if (model == HS_COLL_VHS || model == HS_COLL_VSS) {
@@ -652,8 +658,8 @@ static void binary_collide(const hs_config* cfg, const or_derived* d, int tp, in
     const double cr2 = dx * dx + dy * dy + dz * dz;
     double et = 0.5 * mr * cr2;
     const double om = d->omega_pq[tp][tq];
-    lb_redistribute(cfg, d, tp, tq, &et, erp, vp, om, 0, r);
-    lb_redistribute(cfg, d, tq, tp, &et, erq, vq, om, 0, r);
+    lb_redistribute(cfg, d, tp, tq, &et, erp, vp, om, 0, r, t_macro);
+    lb_redistribute(cfg, d, tq, tp, &et, erq, vq, om, 0, r, t_macro);
     const double cr = sqrt(2.0 * et / mr);
     post_coll_vel(cfg, d, model == HS_COLL_LB_VSS, 1, tp, tq, up, uq, cr, r);
 }
@@ -769,7 +775,7 @@ static int dissociate(const hs_config* cfg, const or_derived* d, const or_rx* x,
     if (!x->split) return 0;
     ec -= x->heat_diss[nr];
     const double om = d->omega_pq[a->t][b->t];
-    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &ec, &b->er, b->v, om, 1, r);
+    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &ec, &b->er, b->v, om, 1, r, 0.0);
     const double ma = cfg->species[a->t].mass, mb = cfg->species[b->t].mass;
     const double mr = ma * mb / (ma + mb);
     const double cr = sqrt(2.0 * ec / mr);
@@ -805,7 +811,7 @@ static int exchange(const hs_config* cfg, const or_derived* d, const or_rx* x, o
     et += a->er + evt + 0.0 + 0.0 + x->heat_x;  /* + both electronic energies, 0 */
     become(a, x->xatom);
     become(b, x->xmol);
-    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &et, &b->er, b->v, x->om_x, 1, r);
+    if (is_lb(cfg)) lb_redistribute(cfg, d, b->t, a->t, &et, &b->er, b->v, x->om_x, 1, r, 0.0);
     const double crx = sqrt(2.0 * et / mrx);
     for (int c = 0; c < 3; ++c) a->u[c] = ucm[c];
     post_reaction_vel(cfg, x->xatom, x->xmol, a->u, b->u, crx, r);
@@ -909,12 +915,12 @@ static int react_pair(const hs_config* cfg, const or_derived* d, int ri, or_psta
  * when the pair has a model, else (or when it did not react) the binary
  * collision.  Returns 1 if it reacted. */
 static int collide_or_react(const hs_config* cfg, const or_derived* d, or_pstate* P, or_pstate* Q, or_rng* r,
-                            or_pstate* prod, int* from_q)
+                            or_pstate* prod, int* from_q, double t_macro)
 {
     prod->t = -1;
     const int ri = d->n_react ? d->react_of[P->t][Q->t] : -1;
     if (ri >= 0 && react_pair(cfg, d, ri, P, Q, r, prod, from_q)) return 1;
-    binary_collide(cfg, d, P->t, Q->t, P->u, Q->u, &P->er, &Q->er, P->v, Q->v, r);
+    binary_collide(cfg, d, P->t, Q->t, P->u, Q->u, &P->er, &Q->er, P->v, Q->v, r, t_macro);
     return 0;
 }
 
@@ -930,7 +936,7 @@ int or_sigma_tcr(const hs_config* cfg, int64_t n, const int32_t* tp, const int32
 int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
                       double* up, double* uq, double* erot_p, double* erot_q,
                       int32_t* vib_p, int32_t* vib_q, const double* uniforms,
-                      int32_t n_uniform, int32_t* used_out)
+                      int32_t n_uniform, int32_t* used_out, const double* t_macro)
 {
     or_derived d;
     derive(cfg, &d);
@@ -945,7 +951,7 @@ int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const 
         int32_t* vq = vib_q ? vib_q + k * HS_MAX_VIB_MODES : NULL;
         int32_t zp[HS_MAX_VIB_MODES] = { 0 }, zq[HS_MAX_VIB_MODES] = { 0 };
         binary_collide(cfg, &d, tp[k], tq[k], up + 3 * k, uq + 3 * k, &erp, &erq,
-                       vp ? vp : zp, vq ? vq : zq, &r);
+                       vp ? vp : zp, vq ? vq : zq, &r, t_macro ? t_macro[k] : 0.0);
         if (erot_p) erot_p[k] = erp;
         if (erot_q) erot_q[k] = erq;
         if (used_out) used_out[k] = r.inj_used;
@@ -975,7 +981,7 @@ int or_react_pairs(const hs_config* cfg, int64_t n, int32_t* tp, int32_t* tq, do
         }
         or_pstate prod;
         int from_q = 0;
-        const int reacted = collide_or_react(cfg, &d, &a, &b, &r, &prod, &from_q);
+        const int reacted = collide_or_react(cfg, &d, &a, &b, &r, &prod, &from_q, 0.0);
         tp[k] = a.t;
         tq[k] = b.t;
         for (int c = 0; c < 3; ++c) {
@@ -1789,6 +1795,7 @@ static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s,
         int32_t scn[8], sco[9];
         subcell_tables(nC, &P->px[b], &P->py[b], &P->pz[b], cc, which, scn, sco, sub_list);
         const double smax = s->sigma_tcr_max[c];
+        const double t_macro = s->t_overall ? s->t_overall[c] : 0.0;
         const double sel = s->remainder[c]
             + 0.5 * nC * (nC - 1) * cfg->fn * smax * cfg->dt / cfg->cell_volume[c];
         int64_t ncand = (int64_t)sel;
@@ -1816,7 +1823,7 @@ static void collide_cell(const hs_config* cfg, const or_derived* d, or_state* s,
                 for (int m = 0; m < cfg->species[tq].n_vib_modes; ++m) b.v[m] = P->vib[m * s->capacity + iq];
                 or_pstate prod;
                 int from_q = 0;
-                const int reacted = collide_or_react(cfg, d, &a, &b, &r, &prod, &from_q);
+                const int reacted = collide_or_react(cfg, d, &a, &b, &r, &prod, &from_q, t_macro);
                 store_state(cfg, d, s, ip, &a, reacted);
                 store_state(cfg, d, s, iq, &b, reacted);
                 if (prod.t >= 0) append_product(cfg, d, s, from_q ? iq : ip, (int32_t)c, &prod, cnt);

@@ -44,6 +44,8 @@ typedef struct or_state {
     int64_t* face_flux;      /* [n_species][or_face_count] or NULL; counted
                                 by or_move with HS_FLAG_FACE_FLUX and
                                 HS_FLAG_SAMPLE                            */
+    double* t_overall;       /* [nc] or NULL: each cell's overall temperature
+                                for inverseZvFormulation 2008 (0 until set) */
 } or_state;
 
 /* primitives (exported for pinning tests) */
@@ -70,7 +72,7 @@ int or_sigma_tcr(const hs_config* cfg, int64_t n, const int32_t* tp, const int32
 int or_binary_collide(const hs_config* cfg, int64_t n, const int32_t* tp, const int32_t* tq,
                       double* up, double* uq, double* erot_p, double* erot_q,
                       int32_t* vib_p, int32_t* vib_q, const double* uniforms,
-                      int32_t n_uniform, int32_t* used_out);
+                      int32_t n_uniform, int32_t* used_out, const double* t_macro);
 
 /* The accepted collision of n independent pairs with an injected uniform
  * stream (pair k reads uniforms[k*n_uniform ..]): the pair's QK reaction

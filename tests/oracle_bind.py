@@ -34,6 +34,7 @@ class or_state(C.Structure):
         ("counters", C.c_int64 * 8),
         ("wall", C.POINTER(C.c_double)),
         ("face_flux", C.POINTER(C.c_int64)),
+        ("t_overall", C.POINTER(C.c_double)),
     ]
 
 
@@ -69,7 +70,7 @@ def lib(path=None):
         L.or_sigma_tcr.argtypes = [cfgp, C.c_int64, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double), P(C.c_double)]
         L.or_binary_collide.argtypes = [cfgp, C.c_int64, P(C.c_int32), P(C.c_int32), P(C.c_double), P(C.c_double),
                                         P(C.c_double), P(C.c_double), P(C.c_int32), P(C.c_int32), P(C.c_double),
-                                        C.c_int32, P(C.c_int32)]
+                                        C.c_int32, P(C.c_int32), P(C.c_double)]
         L.or_fill.argtypes = [cfgp, stp, P(C.c_double), C.c_double, C.c_double, C.c_double, P(C.c_double), C.c_int32, C.c_uint32]
         L.or_fill_cells.argtypes = [cfgp, stp, P(C.c_double), P(C.c_double), P(C.c_double), C.c_uint32]
         L.or_inject.argtypes = [cfgp, stp]
@@ -139,8 +140,9 @@ def sigma_tcr(case, tp, tq, up, uq):
     return out
 
 
-def binary_collide(case, tp, tq, up, uq, uniforms, erot_p=None, erot_q=None, vib_p=None, vib_q=None):
-    """Returns (up', uq', erot_p', erot_q', vib_p', vib_q', used) in FP64."""
+def binary_collide(case, tp, tq, up, uq, uniforms, erot_p=None, erot_q=None, vib_p=None, vib_q=None, t_macro=None):
+    """Returns (up', uq', erot_p', erot_q', vib_p', vib_q', used) in FP64.
+    t_macro: each pair's cell overall temperature (inverseZvFormulation 2008)."""
     n = len(tp)
     up = np.array(up, np.float64).reshape(n, 3).copy()
     uq = np.array(uq, np.float64).reshape(n, 3).copy()
@@ -153,7 +155,8 @@ def binary_collide(case, tp, tq, up, uq, uniforms, erot_p=None, erot_q=None, vib
     used = np.zeros(n, np.int32)
     rc = lib().or_binary_collide(C.byref(case.cfg), n, iptr(np.ascontiguousarray(tp, np.int32)),
                                  iptr(np.ascontiguousarray(tq, np.int32)), dptr(up), dptr(uq), dptr(ep), dptr(eq),
-                                 iptr(vp), iptr(vq), dptr(uni), nu, iptr(used))
+                                 iptr(vp), iptr(vq), dptr(uni), nu, iptr(used),
+                                 None if t_macro is None else dptr(np.ascontiguousarray(t_macro, np.float64)))
     return up, uq, ep, eq, vp, vq, used, rc
 
 
@@ -235,7 +238,12 @@ class OracleState:
         s.start = iptr(self.start)
         s.wall = dptr(self.wall)
         s.face_flux = self.flux.ctypes.data_as(C.POINTER(C.c_int64))
+        self.t_overall = np.zeros(nc)
+        s.t_overall = dptr(self.t_overall)
         self.s = s
+
+    def set_cell_temperature(self, t_overall):
+        self.t_overall[:] = np.asarray(t_overall, np.float64)
 
     @property
     def n(self):

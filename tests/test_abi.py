@@ -85,10 +85,17 @@ def test_invalid_configuration_is_rejected(mutate, code, msg):
 
 
 def test_unsupported_option_is_reported():
-    case = cases.couette_n2o2()
-    case.cfg.inv_zv_form = 1
+    case = cases.heat_bath_5species(fn=1e5)
+    case.cfg.collision_model = abi.HS_COLL_LB_VSS
     rc, err, _ = _create(case)
-    assert rc == abi.HS_E_UNSUPPORTED and "2008" in err
+    assert rc == abi.HS_E_UNSUPPORTED and "chemical reactions need" in err
+
+
+def test_zv_formulation_range():
+    case = cases.couette_n2o2()
+    case.cfg.inv_zv_form = 3
+    rc, err, _ = _create(case)
+    assert rc == abi.HS_E_INVALID and "inverseZvFormulation" in err
 
 
 def test_no_cpu_fallback_without_device():

@@ -173,6 +173,34 @@ def test_evolve_parity(which, steps):
     eng.close()
 
 
+def test_zv_2008_with_cell_temperatures_matches_the_oracle():
+    """inverseZvFormulation 2008 on the couette case: the cells' overall
+    temperatures (zero in some cells, so those use the collision temperature)
+    set on the engine and the oracle, changed between calls as a write does;
+    every step bit-identical.  An engine of another formulation refuses them."""
+    case = cases.couette_n2o2(flags=abi.HS_FLAG_SAMPLE | abi.HS_FLAG_HEATFLUX)
+    case.cfg.inv_zv_form = 1
+    eng, st = _pair(case, H.fill_couette)
+    rng = np.random.default_rng(8)
+    for call in range(3):
+        t = rng.uniform(800.0, 9000.0, case.ncells)
+        t[rng.random(case.ncells) < 0.3] = 0.0
+        eng.set_cell_temperature(t)
+        st.set_cell_temperature(t)
+        for k in range(2):
+            eng.evolve(1)
+            st.evolve(1)
+            _check_state(eng, st, f"2008/call{call}/step{k}")
+    np.testing.assert_array_equal(eng.samples()[0], st.sample_array())
+    c, oc = eng.counters(), st.counters()
+    assert (c["candidates"], c["collisions"]) == tuple(oc[:2])
+    eng.close()
+    other = Engine(cases.couette_n2o2())
+    with pytest.raises(EngineError, match="inverseZvFormulation 2008"):
+        other.set_cell_temperature(np.zeros(other.case.ncells))
+    other.close()
+
+
 @pytest.mark.parametrize("ppc,steps", [(600, 4), (5000, 2), (15000, 2)])
 def test_dense_and_oversize_cell_parity(ppc, steps):
     """A single dense cell: 600 parcels run the wave-cooperative NTC (many

@@ -257,6 +257,28 @@ def test_polymesh_and_blockmeshdict_runs_are_identical(tmp_path):
 
 
 @pytest.mark.gpu
+def test_zv_2008_case_runs_with_the_first_fields_temperature(tmp_path):
+    """A couette case with inverseZvFormulation "2008": the host passes the
+    first field's overall temperature to the engine at each write, and the run
+    differs from the pre-2008 one only after the first write."""
+    runs = {}
+    for form in ("pre-2008", "2008"):
+        case = casegen.couette_case(str(tmp_path / form), end_time=4e-4, write_interval=2e-4, n_terminal=10)
+        p = os.path.join(case, "constant", "dsmcProperties")
+        txt = open(p).read()
+        assert 'inverseZvFormulation "pre-2008";' in txt
+        open(p, "w").write(txt.replace('"pre-2008"', f'"{form}"'))
+        for app in ("dsmcInitialisePlus", "dsmcFoamPlus"):
+            r = run(app, case)
+            assert r.returncode == 0, r.stdout[-2000:] + r.stderr
+        runs[form] = case
+    first = [open(os.path.join(runs[f], "0.0002", "Tvib_N2")).read() for f in runs]
+    second = [open(os.path.join(runs[f], "0.0004", "Tvib_N2")).read() for f in runs]
+    assert first[0] == first[1]
+    assert second[0] != second[1]
+
+
+@pytest.mark.gpu
 def test_restart_continues_the_average(tmp_path):
     """A run of 60 steps written in one go equals 40 steps, a restart from
     the checkpoint and 20 more: the particles and the cumulative cell and

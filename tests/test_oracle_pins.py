@@ -120,7 +120,7 @@ def _zv_case(form, **n2):
     return case
 
 
-def _inv_zv_threshold(case, imax):
+def _inv_zv_threshold(case, imax, t_macro=None):
     """The probability 1/Zv the oracle uses for P (level 0) in a collision
     whose energy puts the top reachable level at imax, found by bisection on
     the uniform of P's vibrational test: the test passes iff that uniform is
@@ -146,7 +146,8 @@ def _inv_zv_threshold(case, imax):
         uni[:, 3] = 0.0             # Bird 5.61 acceptance passes
         vp = np.zeros((n, 3), np.int32)
         vq = np.zeros((n, 3), np.int32)
-        _, _, _, _, va, _, _, rc = ob.binary_collide(case, tp, tq, up, uq, uni, np.zeros(n), np.zeros(n), vp, vq)
+        _, _, _, _, va, _, _, rc = ob.binary_collide(case, tp, tq, up, uq, uni, np.zeros(n), np.zeros(n), vp, vq,
+                                                     t_macro=t_macro)
         assert rc == 0
         passed = va[:, 0] == imax
         assert ((va[:, 0] == 0) | passed).all()
@@ -198,6 +199,27 @@ def test_zv_follows_bird_2010_between_the_anchors():
     got = _inv_zv_threshold(_zv_case(0), imax)
     np.testing.assert_allclose(got, 1.0 / zv, rtol=1e-12)
     assert (np.diff(zv) < 0).all()
+
+
+def test_zv_2008_takes_the_cells_overall_temperature():
+    """inverseZvFormulation 2008 puts the cell's overall temperature in Bird
+    2010 eq. 2 (evaluated here in numpy) in place of the collision
+    temperature, with the probability 1/Zv; where that temperature is not
+    above SMALL (a cell never written yet) the collision temperature is used,
+    as in the pre-2008 formulation."""
+    om = 0.5 * (N2["omega"] + AR["omega"])
+    th, td, zr, tr = N2["theta_v"][0], N2["theta_d"], N2["zref"][0], N2["tref_zv"][0]
+    imax = np.array([2, 6, 15, 6])
+    t_cell = np.array([1500.0, 4000.0, 12000.0, 900.0])
+    zv = (td / t_cell) ** om * (zr * (td / tr) ** (-om)) ** (((td / t_cell) ** (1 / 3) - 1) / ((td / tr) ** (1 / 3) - 1))
+    got = _inv_zv_threshold(_zv_case(1), imax, t_macro=t_cell)
+    np.testing.assert_allclose(got, 1.0 / zv, rtol=1e-12)
+    # the same pairs without a cell temperature: the pre-2008 values
+    np.testing.assert_array_equal(_inv_zv_threshold(_zv_case(1), imax, t_macro=np.zeros(4)),
+                                  _inv_zv_threshold(_zv_case(0), imax))
+    # the other formulations ignore it
+    np.testing.assert_array_equal(_inv_zv_threshold(_zv_case(2), imax, t_macro=t_cell),
+                                  _inv_zv_threshold(_zv_case(2), imax))
 
 
 def test_fixed_zv_is_used_as_given():
