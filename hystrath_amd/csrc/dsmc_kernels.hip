@@ -1541,36 +1541,43 @@ __device__ __forceinline__ void subcell_lists_wave(const uint8_t* sub, IDX* slis
     const int shift = sizeof(IDX) == 2 ? 16 : 32;
     const int lane = threadIdx.x & (WAVE - 1);
     const unsigned long long lt = lane == 0 ? 0ull : ((1ull << lane) - 1ull);
-    int myc = 0;
+    // a window's lanes of sub-cell qq, from three ballots of the sub-cell's bits
+    auto members = [](int qq, unsigned long long v, unsigned long long b0, unsigned long long b1,
+                      unsigned long long b2) {
+        return v & ((qq & 1) ? b0 : ~b0) & ((qq & 2) ? b1 : ~b1) & ((qq & 4) ? b2 : ~b2);
+    };
+    int cnt[8] = { 0, 0, 0, 0, 0, 0, 0, 0 };
     for (int base = 0; base < nC; base += WAVE) {
         const int j = base + lane;
-        const int q = j < nC ? (int)sub[ss + j] : 8;
+        const int q = j < nC ? (int)sub[ss + j] : 0;
+        const unsigned long long v = __ballot(j < nC);
+        const unsigned long long b0 = __ballot(q & 1), b1 = __ballot(q & 2), b2 = __ballot(q & 4);
 #pragma unroll
-        for (int qq = 0; qq < 8; ++qq) {
-            const int pc = __popcll(__ballot(q == qq));
-            if (lane == qq) myc += pc;
-        }
+        for (int qq = 0; qq < 8; ++qq) cnt[qq] += __popcll(members(qq, v, b0, b1, b2));
     }
     int run[8];
     {
         int acc = 0;
 #pragma unroll
         for (int qq = 0; qq < 8; ++qq) {
-            const int cq = __shfl(myc, qq, WAVE);
             run[qq] = acc;
-            if (lane == qq) tab[qq] = (unsigned long long)acc | ((unsigned long long)cq << shift);
-            acc += cq;
+            if (lane == qq) tab[qq] = (unsigned long long)acc | ((unsigned long long)cnt[qq] << shift);
+            acc += cnt[qq];
         }
     }
     for (int base = 0; base < nC; base += WAVE) {
         const int j = base + lane;
-        const int q = j < nC ? (int)sub[ss + j] : 8;
+        const int q = j < nC ? (int)sub[ss + j] : 0;
+        const unsigned long long v = __ballot(j < nC);
+        const unsigned long long b0 = __ballot(q & 1), b1 = __ballot(q & 2), b2 = __ballot(q & 4);
+        int at = 0;
 #pragma unroll
         for (int qq = 0; qq < 8; ++qq) {
-            const unsigned long long m = __ballot(q == qq);
-            if (q == qq) slist[ss + run[qq] + __popcll(m & lt)] = (IDX)j;
+            const unsigned long long m = members(qq, v, b0, b1, b2);
+            if (q == qq) at = run[qq] + __popcll(m & lt);
             run[qq] += __popcll(m);
         }
+        if (j < nC) slist[ss + at] = (IDX)j;
     }
 }
 
