@@ -77,6 +77,8 @@ struct DevParams {
     double z_rot, z_el, z_vib_fixed;
     int inv_zv_form;
     const double* t_overall;    // [nc] cells' overall temperature (inverseZvFormulation 2008 only)
+    const double* zv_tab;       // 1/Zv by (species, partner, mode, imax < HS_ZV_TAB) from the collision
+                                // temperature, or null (see inv_zv_collision)
     double fn, dt;
     uint32_t seed_lo, seed_hi;
     int flags, nslots;
@@ -399,6 +401,21 @@ __device__ __forceinline__ void post_coll_vel(const DevParams& P, bool vss, bool
     }
 }
 
+// 1/Zv (dsmcCloud.C:1427-1508) from the quantised collision temperature of
+// level imax, for inverseZvFormulation pre-2008 and the default (1/(5 Zv));
+// the engine tabulates it for imax < HS_ZV_TAB with this same code, so a
+// table entry and the value computed here are the same double.
+#define HS_ZV_TAB 256
+__device__ __forceinline__ double inv_zv_collision(const DevParams& P, int t, int partner, int mode, int imax,
+                                                   double om) {
+    const double T = (double)imax * P.theta_v[t][mode] / (3.5 - om);
+    const double pow1 = dm_pow(P.theta_d[t] / T, 1.0 / 3.0) - 1.0;
+    const double zvp1 = dm_pow(P.theta_d[t] / T, om);
+    const double zvp2 = dm_pow(P.zv_b[t][partner][mode], pow1 / P.zv_pow2[t][mode]);
+    const double zv = zvp1 * zvp2;
+    return P.inv_zv_form == 2 ? 1.0 / (5.0 * zv) : 1.0 / zv;
+}
+
 // dsmcCloud::postCollisionVibrationalEnergyLevel (dsmcCloud.C:1390-1531);
 // POST = postReaction: the level is always redrawn, with no Zv test.
 // cell: the collision's cell, whose overall temperature inverseZvFormulation
@@ -421,16 +438,19 @@ __device__ int vib_level(const DevParams& P, int t, int partner, int mode, int l
     }
     double inv_zv = 1.0;
     if (P.z_vib_fixed == 0.0) {
-        double T = (double)imax * theta_v / (3.5 - om);
-        if (P.inv_zv_form == 1 && cell >= 0) {
-            const double tm = P.t_overall[cell];
-            if (tm > HS_SMALL) T = tm;
+        double tm = 0.0;
+        if (P.inv_zv_form == 1 && cell >= 0) tm = P.t_overall[cell];
+        if (tm > HS_SMALL) {
+            // inverseZvFormulation 2008 with the cell's overall temperature
+            const double pow1 = dm_pow(P.theta_d[t] / tm, 1.0 / 3.0) - 1.0;
+            const double zvp1 = dm_pow(P.theta_d[t] / tm, om);
+            const double zvp2 = dm_pow(P.zv_b[t][partner][mode], pow1 / P.zv_pow2[t][mode]);
+            inv_zv = 1.0 / (zvp1 * zvp2);
+        } else if (P.zv_tab && imax < HS_ZV_TAB) {
+            inv_zv = P.zv_tab[((t * P.ns + partner) * HS_MAX_VIB_MODES + mode) * HS_ZV_TAB + imax];
+        } else {
+            inv_zv = inv_zv_collision(P, t, partner, mode, imax, om);
         }
-        const double pow1 = dm_pow(P.theta_d[t] / T, 1.0 / 3.0) - 1.0;
-        const double zvp1 = dm_pow(P.theta_d[t] / T, om);
-        const double zvp2 = dm_pow(P.zv_b[t][partner][mode], pow1 / P.zv_pow2[t][mode]);
-        const double zv = zvp1 * zvp2;
-        inv_zv = P.inv_zv_form == 2 ? 1.0 / (5.0 * zv) : 1.0 / zv;
     } else {
         inv_zv = 1.0 / P.z_vib_fixed;
     }

@@ -55,6 +55,7 @@ struct hs_dsmc_engine {
     std::vector<void*> allocs;
     double* d_vol = nullptr;
     double* d_t_overall = nullptr;   // inverseZvFormulation 2008: cells' overall temperature
+    double* d_zv_tab = nullptr;      // 1/Zv by species, partner, mode and top level (LB)
     double* d_sigma = nullptr;
     double* d_rem = nullptr;
     double* d_samples = nullptr;
@@ -786,9 +787,17 @@ extern "C" int hs_dsmc_create(const hs_config* cfg, hs_dsmc_engine** out) {
         launch_fill_f64(eng->d_t_overall, 0.0, nc, eng->stream);
         eng->hp.t_overall = eng->d_t_overall;
     }
+    // Larsen-Borgnakke with a computed Zv: its values at the collision
+    // temperatures of the levels below HS_ZV_TAB, filled once below
+    eng->hp.zv_tab = nullptr;
+    if (cfg->collision_model >= HS_COLL_LB_VHS && cfg->z_vib_fixed == 0.0 && eng->hp.max_modes > 0) {
+        if ((rc = dev_alloc(eng, &eng->d_zv_tab, (int64_t)ns * ns * HS_MAX_VIB_MODES * HS_ZV_TAB))) return bail(rc);
+        eng->hp.zv_tab = eng->d_zv_tab;
+    }
     if (hipMemcpy(eng->dp, &eng->hp, sizeof(DevParams), hipMemcpyHostToDevice) != hipSuccess ||
         hipMemcpy(eng->d_vol, eng->vol_h.data(), (size_t)nc * 8, hipMemcpyHostToDevice) != hipSuccess)
         return bail(fail(eng, HS_E_HIP, "hipMemcpy of parameters failed"));
+    if (eng->d_zv_tab) launch_zv_table(eng->dp, eng->d_zv_tab, ns, eng->stream);
     DevScalars s0;
     memset(&s0, 0, sizeof(s0));
     if (hipMemcpy(eng->d_S, &s0, sizeof(s0), hipMemcpyHostToDevice) != hipSuccess)

@@ -131,6 +131,8 @@ void launch_fill_cells(const DevParams* P, double smax, uint32_t fill_id, double
 void launch_fill_i32(int32_t* p, int32_t v, int64_t n, hipStream_t st);
 void launch_fill_f64(double* p, double v, int64_t n, hipStream_t st);
 void launch_fill_i64(int64_t* p, int64_t v, int64_t n, hipStream_t st);
+// 1/Zv by (species, partner, mode, imax < HS_ZV_TAB) into tab (ns*ns*HS_MAX_VIB_MODES*HS_ZV_TAB)
+void launch_zv_table(const DevParams* P, double* tab, int ns, hipStream_t st);
 void launch_binary_collide(const DevParams* P, int64_t n, const int32_t* tp, const int32_t* tq, double* up,
                            double* uq, double* erp, double* erq, int32_t* vp, int32_t* vq, const double* uni,
                            int32_t nu, int32_t* used, hipStream_t st);

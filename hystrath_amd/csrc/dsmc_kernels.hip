@@ -2425,6 +2425,27 @@ void launch_collide(const DevParams* P, DevParticles dst, const int32_t* start, 
                                                         samples, vol, S, do_collide, do_sample, nc, big_list);
 }
 
+// The table of 1/Zv from the collision temperature (inv_zv_collision) for
+// every species, partner, vibrational mode and top level 1 <= imax < HS_ZV_TAB.
+__global__ __launch_bounds__(BLOCK) void k_zv_table(const DevParams* __restrict__ Pp, double* __restrict__ tab) {
+    const DevParams& P = *Pp;
+    const int n = P.ns * P.ns * HS_MAX_VIB_MODES * HS_ZV_TAB;
+    for (int e = blockIdx.x * BLOCK + threadIdx.x; e < n; e += gridDim.x * BLOCK) {
+        const int imax = e % HS_ZV_TAB;
+        int r = e / HS_ZV_TAB;
+        const int mode = r % HS_MAX_VIB_MODES;
+        r /= HS_MAX_VIB_MODES;
+        const int partner = r % P.ns, t = r / P.ns;
+        tab[e] = (imax > 0 && mode < P.nvib[t]) ? inv_zv_collision(P, t, partner, mode, imax, P.omega_pq[t][partner])
+                                                : 0.0;
+    }
+}
+
+void launch_zv_table(const DevParams* P, double* tab, int ns, hipStream_t st) {
+    const int n = ns * ns * HS_MAX_VIB_MODES * HS_ZV_TAB;
+    k_zv_table<<<(n + BLOCK - 1) / BLOCK, BLOCK, 0, st>>>(P, tab);
+}
+
 // the staged products after the sorted population, in (cell, rank) order
 // (dsmcCloud::addNewParcel, dsmcCloud.C:396-432): at the dissociating
 // parcel's position, with no internal energy; beyond the capacity dropped
