@@ -2129,37 +2129,66 @@ __device__ void collide_cells(TileLds<GEN, PMAX, REACT>& L, const DevParams& P, 
             if (L.segS[t + 1] - L.segS[t] > 64) L.big[atomicAdd(&L.nbig, 1)] = t;
         }
         // the chunk's velocities and sub-cells: a batch of loads is issued
-        // before any of its LDS stores, so their latencies overlap (one at a
-        // time in the variant with internal energies, where the batch's
-        // registers cost spills in the NTC)
-        constexpr int LB = GEN ? (REACT ? 4 : 1) : 8;
-        for (int j0 = t; j0 < cnt; j0 += LB * CT) {
-            float bx[LB], by[LB], bz[LB];
-            uint8_t bs[LB];
-#pragma unroll
-            for (int u = 0; u < LB; ++u) {
-                const int j = j0 + u * CT;
-                if (j < cnt) {
-                    const int64_t gj = (int64_t)S0 + j;
-                    bx[u] = dst.ux[gj]; by[u] = dst.uy[gj]; bz[u] = dst.uz[gj];
-                    bs[u] = sub[gj];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < LB; ++u) {
-                const int j = j0 + u * CT;
-                if (j < cnt) {
-                    L.vx[j] = bx[u]; L.vy[j] = by[u]; L.vz[j] = bz[u];
-                    L.sub[j] = bs[u];
-                }
-            }
-        }
+        // before any of its LDS stores, so their latencies overlap.  With
+        // internal energies a batch is one parcel's velocity, sub-cell,
+        // rotational energy, vibrational levels and species (measured: batches
+        // of 2 parcels were no faster, of 4 slower, from the NTC's spills)
         if (GEN) {
-            for (int j = t; j < cnt; j += CT) {
-                const int64_t gj = (int64_t)S0 + j;
-                L.er[j] = dst.erot ? dst.erot[gj] : 0.0f;
-                for (int m = 0; m < P.max_modes; ++m) L.vb[m * PMAX + j] = dst.vib[m * cap + gj];
-                L.ty[j] = dst.type ? dst.type[gj] : (uint8_t)0;
+            constexpr int LG = 1;
+            for (int j0 = t; j0 < cnt; j0 += LG * CT) {
+                float bx[LG], by[LG], bz[LG], be[LG];
+                uint8_t bs[LG], bt[LG];
+                uint16_t bv[LG][HS_MAX_VIB_MODES];
+#pragma unroll
+                for (int u = 0; u < LG; ++u) {
+                    const int j = j0 + u * CT;
+                    if (j < cnt) {
+                        const int64_t gj = (int64_t)S0 + j;
+                        bx[u] = dst.ux[gj]; by[u] = dst.uy[gj]; bz[u] = dst.uz[gj];
+                        bs[u] = sub[gj];
+                        be[u] = dst.erot ? dst.erot[gj] : 0.0f;
+#pragma unroll
+                        for (int m = 0; m < HS_MAX_VIB_MODES; ++m)
+                            if (m < P.max_modes) bv[u][m] = dst.vib[m * cap + gj];
+                        bt[u] = dst.type ? dst.type[gj] : (uint8_t)0;
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < LG; ++u) {
+                    const int j = j0 + u * CT;
+                    if (j < cnt) {
+                        L.vx[j] = bx[u]; L.vy[j] = by[u]; L.vz[j] = bz[u];
+                        L.sub[j] = bs[u];
+                        L.er[j] = be[u];
+#pragma unroll
+                        for (int m = 0; m < HS_MAX_VIB_MODES; ++m)
+                            if (m < P.max_modes) L.vb[m * PMAX + j] = bv[u][m];
+                        L.ty[j] = bt[u];
+                    }
+                }
+            }
+        } else {
+            constexpr int LB = 8;
+            for (int j0 = t; j0 < cnt; j0 += LB * CT) {
+                float bx[LB], by[LB], bz[LB];
+                uint8_t bs[LB];
+#pragma unroll
+                for (int u = 0; u < LB; ++u) {
+                    const int j = j0 + u * CT;
+                    if (j < cnt) {
+                        const int64_t gj = (int64_t)S0 + j;
+                        bx[u] = dst.ux[gj]; by[u] = dst.uy[gj]; bz[u] = dst.uz[gj];
+                        bs[u] = sub[gj];
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < LB; ++u) {
+                    const int j = j0 + u * CT;
+                    if (j < cnt) {
+                        L.vx[j] = bx[u]; L.vy[j] = by[u]; L.vz[j] = bz[u];
+                        L.sub[j] = bs[u];
+                    }
+                }
             }
         }
         if (t == 0) {
