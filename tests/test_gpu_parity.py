@@ -76,8 +76,12 @@ def test_sigma_and_binary_collide_argon(model):
     eng.close()
 
 
-@pytest.mark.parametrize("model", [abi.HS_COLL_LB_VHS, abi.HS_COLL_LB_VSS])
-def test_binary_collide_lb(model):
+@pytest.mark.parametrize("model,speed", [(abi.HS_COLL_LB_VHS, 2500.0), (abi.HS_COLL_LB_VSS, 2500.0),
+                                         (abi.HS_COLL_LB_VHS, 15000.0)])
+def test_binary_collide_lb(model, speed):
+    """Larsen-Borgnakke pairs against the oracle.  At 15 km/s per component
+    most pairs reach vibrational levels of 256 and more, past the engine's
+    table of 1/Zv, where it is computed in the collision."""
     case = cases.couette_n2o2()
     case.cfg.collision_model = model
     eng = Engine(case)
@@ -85,15 +89,21 @@ def test_binary_collide_lb(model):
     n = 4096
     tp = rng.integers(0, 2, n).astype(np.int32)
     tq = rng.integers(0, 2, n).astype(np.int32)
-    up = rng.normal(0, 2500, (n, 3))
-    uq = rng.normal(0, 2500, (n, 3))
+    up = rng.normal(0, speed, (n, 3))
+    uq = rng.normal(0, speed, (n, 3))
     ep = rng.exponential(5e-20, n)
     eq = rng.exponential(5e-20, n)
     vp = np.zeros((n, 3), np.int32)
     vq = np.zeros((n, 3), np.int32)
     vp[:, 0] = rng.integers(0, 4, n)
     vq[:, 0] = rng.integers(0, 4, n)
-    uni = rng.uniform(0, 1, (n, 64))
+    uni = rng.uniform(0, 1, (n, 256))
+    if speed > 10000.0:
+        th = np.array([case.species[0].theta_v[0], case.species[1].theta_v[0]])
+        mr = np.array([s.mass for s in case.species[:2]])
+        mr = mr[tp] * mr[tq] / (mr[tp] + mr[tq])
+        imax = 0.5 * mr * ((up - uq) ** 2).sum(1) / (abi.HS_KB * th[tp])
+        assert (imax >= 256).mean() > 0.3
     g = eng.binary_collide(tp, tq, up, uq, uni, ep, eq, vp, vq)
     c = ob.binary_collide(case, tp, tq, up, uq, uni, ep, eq, vp, vq)
     assert g[7] == 0 and c[7] == 0
