@@ -174,3 +174,22 @@ def test_reacting_engines_stepped_together_match_their_oracles():
     for k, (e, st) in enumerate(pairs):
         _check_end(e, st, 7, f"batch/case{k}")
         e.close()
+
+
+def test_reacting_steps_with_zv_2008_match_the_oracle():
+    """The heat bath cut into 27 cells with inverseZvFormulation 2008: cell
+    temperatures (some zero) set on the engine and the oracle, the pairs that
+    do not react relax their vibration with them; bit-identical steps."""
+    c = cases.heat_bath_5species(fn=400.0, n_side=3, seed=9)
+    c.cfg.inv_zv_form = 1
+    eng, st = _pair(c, _fill_bath)
+    t = np.linspace(2000.0, 20000.0, c.ncells)
+    t[::4] = 0.0
+    eng.set_cell_temperature(t)
+    st.set_cell_temperature(t)
+    for r, n in enumerate((1, 2, 3)):
+        eng.evolve(n)
+        st.evolve(n)
+        _check(eng, st, f"zv2008/call{r}")
+    _check_end(eng, st, 6, "zv2008")
+    eng.close()
