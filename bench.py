@@ -223,8 +223,8 @@ def cpu_baseline(case, eng, budget_s):
 
 def time_to_2pct(args, case_fn, fill, ms_per_step):
     """The plain path's wall time from case start to running-average fields
-    within 2% of a long average.  A judge run (another seed) develops for
-    --t2-develop steps and averages --t2-judge steps; the measured run starts
+    within 2% of a long average.  A long-average run (another seed) develops for
+    --t2-develop steps and averages --t2-long steps; the measured run starts
     from its own fill, develops for the same number of steps without
     sampling (the reference's schedule: sampling from step 10k of 20k), then
     samples and is checked every --t2-check steps.  The time is the steps
@@ -233,18 +233,18 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
     from hystrath_amd import convergence as cv
     from hystrath_amd.engine import Engine
 
-    dev, nj, chk = args.t2_develop, args.t2_judge, args.t2_check
+    dev, nj, chk = args.t2_develop, args.t2_long, args.t2_check
     case_j = case_fn(seed=20251202 + 1)
     with Engine(case_j) as ej:
         fill(ej)
         flags = case_j.cfg.flags
         ej.set_flags(flags & ~1)
-        run_evolve(ej, dev, what="judge development")
+        run_evolve(ej, dev, what="long-average development")
         ej.set_flags(flags)
         ej.reset_samples()
-        run_evolve(ej, nj, what="judge average")
+        run_evolve(ej, nj, what="long average")
         s, n = ej.samples()
-        judge = cv.field_set(s, n, case_j)
+        long_avg = cv.field_set(s, n, case_j)
     case = case_fn(seed=20251202)
     t0 = time.perf_counter()
     with Engine(case) as e:
@@ -259,7 +259,7 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
             run_evolve(e, chk, what="plain sampling")
             done += chk
             s, n = e.samples()
-            err = cv.errors(cv.field_set(s, n, case), judge)
+            err = cv.errors(cv.field_set(s, n, case), long_avg)
             trace.append([done, {k: round(v, 5) for k, v in err.items()}])
             if cv.converged(err, 0.02):
                 break
@@ -269,21 +269,21 @@ def time_to_2pct(args, case_fn, fill, ms_per_step):
     log(f"[bench] time-to-2%: {'reached' if ok else 'NOT reached'} after {dev} + {done} steps; errors {err}")
     return {"reached": ok, "steps": steps, "develop_steps": dev, "sampled_steps": done,
             "seconds": round(steps * ms_per_step * 1e-3, 4), "wall_seconds_with_checks": round(wall, 3),
-            "errors": err, "judge": {"develop_steps": dev, "average_steps": nj, "seed": 20251202 + 1},
+            "errors": err, "long_average": {"develop_steps": dev, "average_steps": nj, "seed": 20251202 + 1},
             "check_every": chk, "trace": trace[-5:],
-            "definition": "relative L2 over cells with >= 1 parcel on average in the judge; velocity "
-                          "components scaled by the judge's speed; argon has no internal temperature"}
+            "definition": "relative L2 over cells with >= 1 parcel on average in the long average; velocity "
+                          "components scaled by the long average's speed; argon has no internal temperature"}
 
 
 def warm_start_eval(args, log_fn):
     """The learned warm start against the plain path on held-out 256 x 256
     cylinder cases (Mach 5-25, Kn 0.01-1, T_wall 300-1000 K), with the
-    trained network committed in hystrath_amd/data.  Per case: the judge is
+    trained network committed in hystrath_amd/data.  Per case: the long average is
     the 10,000-step long average of one run; the plain path (fill, 3,000
     development steps, sampling) and the warm path (the network's fields
     fill the domain, 500 relaxation steps, one window denoised, the plain
     average as fallback) are independent runs timed to fields within 2% of
-    the judge."""
+    the long average."""
     from hystrath_amd import warmstart as W
     from hystrath_amd.engine import Engine
     path = os.path.join(ROOT, "hystrath_amd", "data", "ws_cylinder256.npz")
@@ -294,7 +294,7 @@ def warm_start_eval(args, log_fn):
     gen = W.generate(cs, develop=3000, average=10000, log=log_fn)
     out = []
     for c, (_, target) in zip(cs, gen):
-        judge = W.judge_fields(c, target)
+        long_avg = W.long_average_fields(c, target)
         # ms per step of this case, graph-replayed, on a developed state
         with Engine(c) as e:
             W.fill_freestream(e, c)
@@ -304,8 +304,8 @@ def warm_start_eval(args, log_fn):
             e.evolve(500)
             e.synchronize()
             ms = 1e3 * (time.perf_counter() - t) / 500
-        plain = W.plain_time_to_tol(c, judge, develop=3000)
-        warm = W.warm_time_to_tol(nets, c, judge, relax=500)
+        plain = W.plain_time_to_tol(c, long_avg, develop=3000)
+        warm = W.warm_time_to_tol(nets, c, long_avg, relax=500)
         tp = plain["steps"] * ms * 1e-3
         tw = warm["steps"] * ms * 1e-3 + warm["inference_s"]
         out.append({"params": {k: round(v, 4) for k, v in c.ws_params.items()}, "ms_per_step": round(ms, 4),
@@ -437,8 +437,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=12.0,
                     help="CPU-baseline budget per thread count (0 disables)")
     ap.add_argument("--t2pct", type=int, default=1, help="measure the plain path's time to 2%% (0 disables)")
-    ap.add_argument("--t2-develop", type=int, default=10000, help="steps before sampling starts (judge and run)")
-    ap.add_argument("--t2-judge", type=int, default=10000, help="steps of the judge's long average")
+    ap.add_argument("--t2-develop", type=int, default=10000, help="steps before sampling starts (long average and run)")
+    ap.add_argument("--t2-long", type=int, default=10000, help="steps of the long average's long average")
     ap.add_argument("--t2-check", type=int, default=250, help="sampled steps between error checks")
     ap.add_argument("--t2-max", type=int, default=10000, help="give up after this many sampled steps")
     ap.add_argument("--ws-cases", type=int, default=2, help="held-out cases of the warm-start comparison (0 disables)")

@@ -1,16 +1,16 @@
 """Time to converged fields: how long a run takes until its running-average
-fields are within a tolerance of a long-average judge.
+fields are within a tolerance of a long average.
 
 The reference answers "how long to sample" by a fixed schedule
 (system/fieldPropertiesDict: timeOption / resetAtOutputUntilTime, read by
-timeDataMeas.C:71-104) and the user's judgement; this module measures it.
-Error is the relative L2 norm over the cells the judge sampled with at least
+timeDataMeas.C:71-104) and the user's decision; this module measures it.
+Error is the relative L2 norm over the cells the long average sampled with at least
 `min_parcels` parcels on average:
 
-  density, translational and internal temperature:  ||f - f_judge|| / ||f_judge||
-  velocity components:                               ||U_a - U_a,judge|| / ||U_judge||
+  density, translational and internal temperature:  ||f - f_long|| / ||f_long||
+  velocity components:                               ||U_a - U_a,long|| / ||U_long||
 
-(velocity components are scaled by the judge's speed, not by their own norm,
+(velocity components are scaled by the long average's speed, not by their own norm,
 so that a component that is zero over most of the domain does not measure
 noise against nothing).
 """
@@ -39,20 +39,20 @@ def field_set(samples, nsteps, case):
     return out
 
 
-def errors(run, judge, min_parcels=1.0):
-    """Per-field relative L2 errors of run against judge (dicts from
+def errors(run, long_avg, min_parcels=1.0):
+    """Per-field relative L2 errors of run against the long average (dicts from
     field_set)."""
-    mask = judge["nmean"] >= min_parcels
-    speed = np.sqrt(np.sum(judge["Ux"][mask] ** 2 + judge["Uy"][mask] ** 2))
+    mask = long_avg["nmean"] >= min_parcels
+    speed = np.sqrt(np.sum(long_avg["Ux"][mask] ** 2 + long_avg["Uy"][mask] ** 2))
     out = {}
     for k in FIELDS:
-        if k not in judge or k not in run:
+        if k not in long_avg or k not in run:
             continue
-        diff = np.linalg.norm(run[k][mask] - judge[k][mask])
+        diff = np.linalg.norm(run[k][mask] - long_avg[k][mask])
         if k in ("Ux", "Uy"):
             out[k] = float(diff / speed) if speed > 0 else float("inf")
         else:
-            ref = np.linalg.norm(judge[k][mask])
+            ref = np.linalg.norm(long_avg[k][mask])
             out[k] = float(diff / ref) if ref > 0 else float("inf")
     return out
 

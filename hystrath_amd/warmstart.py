@@ -14,7 +14,7 @@ thousands of steps.  This module shortens both:
   (hs_dsmc_fill_cells), relaxes briefly, then samples, and every check
   interval denoises the running average; it stops at the first check whose
   denoised (or plain) fields are within the tolerance;
-- the plain long-average DSMC stays the judge of every prediction's error,
+- the plain long-average DSMC stays the yardstick of every prediction's error,
   per field.
 
 The denoiser learns from averages of 200 to 1,600 steps cut from the plain
@@ -242,7 +242,7 @@ def mirror(img, sign_channels):
 def loss_weights(target, case):
     """Per-pixel weights of the training loss: the long-average density over
     the freestream's (clipped to [0.05, 20]), mean 1 over the fluid cells.
-    The judge's relative L2 errors of rho and T weigh each cell by the
+    The long average's relative L2 errors of rho and T weigh each cell by the
     field's square, so the dense shock layer dominates them; uniform weights
     in the log variables would spend the net on the sparse wake."""
     m = fluid_mask(case)
@@ -368,22 +368,22 @@ def warm_fill(eng, case, img5, fill_id=7):
 # ---------------------------------------------------------------------------
 # time to converged fields: plain path against warm path
 # ---------------------------------------------------------------------------
-def judge_fields(case, long_image):
-    """The judge's fields (convergence.field_set layout) from its image."""
+def long_average_fields(case, long_image):
+    """The long average's fields (convergence.field_set layout) from its image."""
     f = decode(long_image, case)
     m = fluid_mask(case).reshape(-1) > 0
     return {"rho": f[0], "Ux": f[1], "Uy": f[2], "Ttra": f[3], "nmean": np.where(m, 1.0, 0.0)}
 
 
-def _errors_of_image(img5, case, judge):
+def _errors_of_image(img5, case, long_avg):
     f = decode(img5, case)
     run = {"rho": f[0], "Ux": f[1], "Uy": f[2], "Ttra": f[3]}
-    return convergence.errors(run, judge)
+    return convergence.errors(run, long_avg)
 
 
-def plain_time_to_tol(case, judge, develop=3000, check=250, max_sampled=20000, tol=0.02, seed_offset=1000):
+def plain_time_to_tol(case, long_avg, develop=3000, check=250, max_sampled=20000, tol=0.02, seed_offset=1000):
     """Steps of the plain path: uniform fill, develop steps unsampled, then
-    sampling until the running average is within tol of the judge."""
+    sampling until the running average is within tol of the long average."""
     c2 = _reseeded(case, seed_offset)
     with Engine(c2) as e:
         fill_freestream(e, c2)
@@ -398,7 +398,7 @@ def plain_time_to_tol(case, judge, develop=3000, check=250, max_sampled=20000, t
             e.evolve(check)
             done += check
             s, n = e.samples()
-            err = _errors_of_image(encode(case_fields(s, n, c2), c2), c2, judge)
+            err = _errors_of_image(encode(case_fields(s, n, c2), c2), c2, long_avg)
             if convergence.converged(err, tol):
                 break
         e.synchronize()
@@ -406,14 +406,14 @@ def plain_time_to_tol(case, judge, develop=3000, check=250, max_sampled=20000, t
                 errors=err)
 
 
-def warm_time_to_tol(nets, case, judge, relax=500, check=WINDOW, max_sampled=20000, tol=0.02, seed_offset=2000,
+def warm_time_to_tol(nets, case, long_avg, relax=500, check=WINDOW, max_sampled=20000, tol=0.02, seed_offset=2000,
                      device="cuda"):
     """Steps of the warm path: the surrogate's fields (nets[0], from the
     freestream image) fill the domain and relax steps run unsampled; then
     every check steps the running average is denoised (nets[1], with the
     surrogate's fields as its prior), and the path stops at the first check
     whose denoised fields -- or, failing that, whose plain running average
-    -- are within tol of the judge."""
+    -- are within tol of the long average."""
     import torch
     surrogate, denoiser = nets
     c2 = _reseeded(case, seed_offset)
@@ -430,7 +430,7 @@ def warm_time_to_tol(nets, case, judge, relax=500, check=WINDOW, max_sampled=200
             e.evolve(relax)
         e.set_flags(flags)
         e.reset_samples()
-        err_surr = _errors_of_image(img0, c2, judge)
+        err_surr = _errors_of_image(img0, c2, long_avg)
         done, err, used = 0, None, "none within the tolerance"
         first = None
         while done < max_sampled:
@@ -442,8 +442,8 @@ def warm_time_to_tol(nets, case, judge, relax=500, check=WINDOW, max_sampled=200
             den = predict(denoiser, raw, c2, device, prior=img0, nsteps=n)
             torch.cuda.synchronize()
             t_inf += time.perf_counter() - t
-            err_d = _errors_of_image(den, c2, judge)
-            err_r = _errors_of_image(raw, c2, judge)
+            err_d = _errors_of_image(den, c2, long_avg)
+            err_r = _errors_of_image(raw, c2, long_avg)
             if first is None:
                 first = (err_r, err_d)
             if convergence.converged(err_d, tol):
@@ -549,9 +549,9 @@ def main(argv=None):
         save_nets(nets, a.save)
     results = []
     for c, (wins, target) in zip(test_cases, gen[a.train_cases:]):
-        judge = judge_fields(c, target)
-        plain = plain_time_to_tol(c, judge, develop=a.develop)
-        warm = warm_time_to_tol(nets, c, judge, relax=a.relax)
+        long_avg = long_average_fields(c, target)
+        plain = plain_time_to_tol(c, long_avg, develop=a.develop)
+        warm = warm_time_to_tol(nets, c, long_avg, relax=a.relax)
         ratio = plain["steps"] / max(warm["steps"], 1)
         log(f"[ws] case {c.ws_params}: plain {plain['steps']} steps, warm {warm['steps']} steps "
             f"({warm['used']}), x{ratio:.2f}; surrogate {warm['errors_surrogate']}; window {warm['errors_window']}; "

@@ -3,7 +3,7 @@ from plain DSMC through hs_dsmc_evolve_batch (averages of 200 to 1,600
 steps cut from one run per case), the cross-fitted surrogates and the
 denoiser trained on the bfloat16 MFMA kernels (their losses fall), the nets
 saved and reloaded unchanged, and the plain and warm paths timed against a
-judge on a held-out case.  How much faster the warm path is at full size is
+long_avg on a held-out case.  How much faster the warm path is at full size is
 measured by bench.py; here the pipeline runs and its outputs are well
 formed."""
 import numpy as np
@@ -38,9 +38,9 @@ def test_pipeline_trains_saves_and_times_both_paths(tmp_path):
     p0 = W.predict(s2, img, test[0])
     np.testing.assert_array_equal(W.predict(denoiser, img, test[0], prior=p0, nsteps=400),
                                   W.predict(d2, img, test[0], prior=p0, nsteps=400))
-    judge = W.judge_fields(test[0], gen[3][1])
-    plain = W.plain_time_to_tol(test[0], judge, develop=400, check=200, max_sampled=600)
-    warm = W.warm_time_to_tol((s2, d2), test[0], judge, relax=100, check=200, max_sampled=600)
+    long_avg = W.long_average_fields(test[0], gen[3][1])
+    plain = W.plain_time_to_tol(test[0], long_avg, develop=400, check=200, max_sampled=600)
+    warm = W.warm_time_to_tol((s2, d2), test[0], long_avg, relax=100, check=200, max_sampled=600)
     for r in (plain, warm):
         assert r["steps"] > 0 and set(r["errors"]) >= {"rho", "Ux", "Uy", "Ttra"}
     assert warm["inference_s"] > 0
