@@ -438,7 +438,7 @@ def main():
                     help="CPU-baseline budget per thread count (0 disables)")
     ap.add_argument("--t2pct", type=int, default=1, help="measure the plain path's time to 2%% (0 disables)")
     ap.add_argument("--t2-develop", type=int, default=10000, help="steps before sampling starts (long average and run)")
-    ap.add_argument("--t2-long", type=int, default=10000, help="steps of the long average's long average")
+    ap.add_argument("--t2-long", type=int, default=10000, help="steps of the long average")
     ap.add_argument("--t2-check", type=int, default=250, help="sampled steps between error checks")
     ap.add_argument("--t2-max", type=int, default=10000, help="give up after this many sampled steps")
     ap.add_argument("--ws-cases", type=int, default=2, help="held-out cases of the warm-start comparison (0 disables)")
