@@ -156,10 +156,19 @@ def test_move_sort_collide_stages(which):
     eng.close()
 
 
-@pytest.mark.parametrize("which,steps", [("box", 6), ("cylinder", 8), ("couette", 5), ("couette_hf", 4)])
+def _n2_box(flags):
+    return cases.periodic_box(n_side=6, L=0.03, ppc=30, seed=12, species=[cases.make_species("N2")],
+                              collision_model=abi.HS_COLL_LB_VHS, flags=flags)
+
+
+@pytest.mark.parametrize("which,steps", [("box", 6), ("cylinder", 8), ("couette", 5), ("couette_hf", 4),
+                                         ("n2_box_hf", 5)])
 def test_evolve_parity(which, steps):
+    """n2_box_hf: one molecular species (rotation and vibration, LB-VHS) with
+    the heat-flux sums."""
     hf = abi.HS_FLAG_SAMPLE | abi.HS_FLAG_HEATFLUX | abi.HS_FLAG_COLL_SEP
     case, filler = {
+        "n2_box_hf": (_n2_box(hf), H.fill_box),
         "box": (H.small_box(), H.fill_box),
         "cylinder": (H.small_cylinder(), H.fill_cylinder),
         "couette": (cases.couette_n2o2(), H.fill_couette),
